@@ -1,0 +1,207 @@
+/*
+ * uc_api.h -- C ABI of the MI355X unitig compactor (libuc.so).
+ *
+ * The reference (euphoricpoptarts/UnitigCoarsening) has no FFI: its
+ * boundary is the `main <k-mer file> k l <output file>` process
+ * (driver.cpp:422-432) and, inside it, free functions over Kokkos views.
+ * Each entry point below names the reference functions it replaces; the
+ * CLI in csrc/uc_main.cpp keeps the process boundary itself.
+ *
+ * Conventions
+ *  - Every function returns an int status: UC_OK (0) or a negative UC_E_*
+ *    code (the "nonzero = error" convention of CHECK_RETSTAT,
+ *    driver.cpp:15-23).  Nothing here calls exit().  uc_last_error() gives
+ *    a message for the last failure on a context.
+ *  - K-mers use the reference packing (driver.cpp:233-246): ceil(k/16)
+ *    uint32 words per k-mer, character i in bits 2*(i mod 16) of word i/16,
+ *    A=0 C=1 G=2 T=3, bits above the k-th character zero.
+ *  - One host thread per context.  A context owns its device buffers; a
+ *    result stays valid until the next job on the same context.
+ *  - Slot ids are 32-bit: a job may hold at most 2^32-2 k-mer placements
+ *    (k-mers plus crossing duplicates).
+ */
+#ifndef UC_API_H
+#define UC_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UC_OK 0
+#define UC_E_ARG (-1)      /* invalid argument */
+#define UC_E_NOMEM (-2)    /* device or host allocation failed / over budget */
+#define UC_E_HIP (-3)      /* HIP runtime error */
+#define UC_E_STATE (-4)    /* call out of order (e.g. uc_find_edges before uc_bucketize) */
+#define UC_E_IO (-5)       /* file could not be opened / read / written */
+#define UC_E_FORMAT (-6)   /* malformed k-mer text */
+#define UC_E_LIMIT (-7)    /* job exceeds a compiled limit (32-bit slot ids, bucket count) */
+
+#define UC_ALL_BUCKETS 0xFFFFFFFFu
+
+/* bucket modes of the edge-finding phase */
+#define UC_MODE_BATCHED 0 /* one launch per phase over every bucket (and every set) */
+#define UC_MODE_PLAIN 1   /* one launch per bucket, one shared table sized for the largest
+                             bucket, like the reference's per-bucket loop (driver.cpp:479-521) */
+
+typedef struct uc_ctx uc_ctx;
+
+typedef struct {
+    int device;            /* HIP device ordinal */
+    uint64_t seed;         /* seed of the l-mer order bijection and of the splitter sample */
+    uint32_t n_buckets;    /* minimizer buckets per set, power of two, 1..1024
+                              (the reference hard-codes 64, lmin_buckets.h:35) */
+    uint32_t mode;         /* UC_MODE_BATCHED or UC_MODE_PLAIN */
+    uint32_t load_pct;     /* hashmap load factor in percent (10..90; the reference uses <= 50,
+                              assembler.h:113-118); 0 = default 50 */
+    uint32_t sample_log2;  /* splitter sampling rate 2^-sample_log2 for path contraction; 0 = default */
+    uint64_t hbm_budget;   /* bytes this context may allocate on the device; 0 = no limit */
+    uint32_t flags;        /* UC_FLAG_* */
+} uc_opts;
+
+#define UC_FLAG_PROFILE 1u /* time every kernel launch with HIP events (uc_kernel_stat) */
+
+void uc_default_opts(uc_opts *o);
+int uc_ctx_create(const uc_opts *opts, uc_ctx **out);
+int uc_ctx_destroy(uc_ctx *ctx);
+const char *uc_last_error(const uc_ctx *ctx);
+const char *uc_strerror(int status);
+
+/* ---- whole job -------------------------------------------------------- */
+
+/* Replaces driver.cpp:433-662 from the device copy of the packed k-mers to
+ * the unitig text: bucketing, edge finding, contraction, emission.
+ * `packed` holds n k-mers in the reference packing; `on_device` says
+ * whether it points to device memory (no copy) or host memory (one H2D
+ * copy).  The unitig text stays on the device (uc_text_*). */
+int uc_compact(uc_ctx *ctx, const uint32_t *packed, uint64_t n, uint32_t k, uint32_t l, int on_device);
+
+/* Whole job on the k-mers of the last uc_load (already resident in device
+ * memory): bucketing, edges, contraction, emission, timed per phase. */
+int uc_compact_loaded(uc_ctx *ctx, uint32_t l);
+
+/* Batched form: n_sets independent k-mer sets concatenated in `packed`
+ * (set s holds k-mers [set_offsets[s], set_offsets[s+1])); every phase
+ * runs once over all (set, bucket) pairs.  Set s's unitigs are text bytes
+ * [uc_set_text_offset(s), uc_set_text_offset(s+1)). */
+int uc_compact_sets(uc_ctx *ctx, const uint32_t *packed, const uint64_t *set_offsets, uint32_t n_sets, uint32_t k,
+                    uint32_t l, int on_device);
+
+/* ---- phases (the drop-in pieces) -------------------------------------- */
+
+/* Replaces driver.cpp:356-402 generate_permutation: writes the seeded
+ * bijection of l-mer codes onto orders for every code < 4^l into an array
+ * of 4^l uint32 (the table the bucketing kernel otherwise computes inline),
+ * in device memory (on_device != 0) or host memory. */
+int uc_lmer_order_table(uc_ctx *ctx, uint32_t l, uint32_t *table, int on_device);
+
+/* Load packed k-mers (one set or several) for the phase-by-phase calls. */
+int uc_load(uc_ctx *ctx, const uint32_t *packed, const uint64_t *set_offsets, uint32_t n_sets, uint32_t k,
+            int on_device);
+
+/* Replaces lmin_buckets.h:288-433 find_l_minimizer + driver.cpp:137-225
+ * collect_buckets: prefix/suffix l-minimizers, bucket CSR with crossing
+ * k-mers duplicated into their suffix bucket, lend/borrow pairs.
+ * order_table: NULL for the inline seeded bijection, else a table of 4^l
+ * orders (e.g. from uc_lmer_order_table, or any permutation such as the
+ * reference's lmin_bucket_map) in device memory (table_on_device != 0) or
+ * host memory (copied once). */
+int uc_bucketize(uc_ctx *ctx, uint32_t l, const uint32_t *order_table, int table_on_device);
+
+typedef struct {
+    uint64_t n_kmers;        /* k-mers over all sets */
+    uint64_t m_placements;   /* slots of the bucket CSR: k-mers + crossing duplicates */
+    uint64_t n_cross;        /* crossing k-mers (lend/borrow pairs) */
+    uint32_t n_buckets;      /* buckets per set */
+    uint32_t n_sets;
+    uint64_t largest_bucket;
+} uc_bucket_info;
+
+int uc_bucket_info_get(uc_ctx *ctx, uc_bucket_info *info);
+
+/* Host copies of the bucketing result for checking: bucket row offsets
+ * (n_sets*n_buckets+1), slot -> input k-mer index (m), slot flags (m;
+ * bit0 = home copy of a crossing k-mer, bit1 = borrowed duplicate) and the
+ * pairs (home slot, borrowed slot) (n_cross each).  Any pointer may be NULL. */
+int uc_bucket_copy(uc_ctx *ctx, uint64_t *bucket_rows, uint32_t *slot_src, uint8_t *slot_flags, uint32_t *pair_home,
+                   uint32_t *pair_borrow);
+
+/* Replaces assembler.h:113-150 init_hashmap/generate_hashmap and
+ * :152-214 init_assembler/assemble_pruned_graph for one bucket (global
+ * bucket id set*n_buckets+b) or UC_ALL_BUCKETS: open-addressing table of
+ * (k-1)-prefixes, suffix lookups, in-degree counts, successor per slot
+ * (kept only when out-degree and in-degree are both 1). */
+int uc_find_edges(uc_ctx *ctx, uint32_t bucket);
+
+/* Host copy of the successor array (m entries, UINT32_MAX = none), slot ids. */
+int uc_succ_copy(uc_ctx *ctx, uint32_t *succ);
+
+/* Replaces coarseners.h:388-485 coarsen_de_bruijn_full_cycle(_final),
+ * heuristics.h:205-256 coarsen_HEC, compact_graph.h and the cross-graph
+ * build of driver.cpp:527-556: splices every borrowed duplicate back onto
+ * its home copy, then contracts the non-branching paths (splitter
+ * election, segment walks, segment-chain pointer jumping; isolated cycles
+ * are cut at their lexicographically smallest k-mer). */
+int uc_coarsen(uc_ctx *ctx);
+
+typedef struct {
+    uint64_t n_unitigs;
+    uint64_t n_cycles;       /* unitigs that were isolated cycles */
+    uint64_t text_bytes;     /* sum over unitigs of (length + k - 1 + 1) */
+    uint64_t n_splitters;
+    uint32_t chain_rounds;   /* pointer-jumping rounds over segment chains */
+} uc_coarsen_info;
+
+int uc_coarsen_info_get(uc_ctx *ctx, uc_coarsen_info *info);
+
+/* Replaces compact.h:161-210 write_intra_bucket_outputs,
+ * lmin_buckets.h:172-283 generate_minitigs/partition_for_output,
+ * driver.cpp:561-639 and compact.h:59-112 write_unitigs4: the unitig text
+ * (one line per unitig) written on the device. */
+int uc_emit(uc_ctx *ctx);
+
+/* ---- results ---------------------------------------------------------- */
+
+uint64_t uc_text_size(const uc_ctx *ctx);
+const char *uc_text_device_ptr(const uc_ctx *ctx);
+int uc_text_copy(uc_ctx *ctx, char *dst, uint64_t cap);
+uint64_t uc_set_text_offset(const uc_ctx *ctx, uint32_t set);
+
+/* Appends the text to a file (std::ofstream::app semantics, compact.h:7-37). */
+int uc_text_append_file(uc_ctx *ctx, const char *path);
+
+/* ---- text input (driver.cpp:251-354) ---------------------------------- */
+
+/* Parses the reference's k-mer text (count line, then first k characters
+ * of each line; bytes other than ACGT map to A) into the reference packing
+ * on the host.  *packed is malloc'd (free with uc_free). */
+int uc_parse_kmer_text(const char *buf, uint64_t len, uint32_t k, uint32_t **packed, uint64_t *n);
+int uc_read_kmer_file(const char *path, uint32_t k, uint32_t **packed, uint64_t *n);
+void uc_free(void *p);
+
+/* ---- timing ----------------------------------------------------------- */
+
+typedef struct {
+    float bucketize_ms, edges_ms, coarsen_ms, emit_ms, total_ms;
+    uint64_t device_bytes; /* device memory held by the context */
+} uc_timing;
+
+/* Per-phase device times (HIP events on the context's stream) of the last
+ * uc_compact / uc_compact_sets. */
+int uc_timing_get(uc_ctx *ctx, uc_timing *t);
+
+/* With UC_FLAG_PROFILE: accumulated device time and launch count of kernel
+ * number idx (0..) since the last uc_kernel_stats_reset, on the context's
+ * stream.  Returns UC_E_ARG past the last kernel. */
+int uc_kernel_stat(uc_ctx *ctx, uint32_t idx, char *name, uint32_t name_cap, double *total_ms, uint64_t *launches);
+int uc_kernel_stats_reset(uc_ctx *ctx);
+
+/* Blocks until the context's stream is idle. */
+int uc_sync(uc_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,278 @@
+"""Parity of the HIP path (through the C ABI) against the CPU oracle.
+
+Bar: bit-exact.  Unitig text is compared as sorted lines (the reference's
+line order is nondeterministic, driver.cpp:512,617-639); per-phase results
+are compared as exact sets.
+"""
+import random
+
+import numpy as np
+import pytest
+
+import oracle
+import unitigcoarsening_amd as uc
+from brute import brute_unitigs, check_partition
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = uc.Compactor(n_buckets=64)
+    yield c
+    c.close()
+
+
+def textbook_sorted(kmers, k):
+    text, _, _ = oracle.textbook(kmers, k)
+    return uc.sorted_lines(text)
+
+
+def run(kmers, k, l, **kw):
+    with uc.Compactor(**kw) as c:
+        return c.compact(kmers, k, l)
+
+
+def rand_set(rng, n, k, alphabet="ACGT"):
+    s = set()
+    cap = len(alphabet) ** k
+    n = min(n, cap)
+    while len(s) < n:
+        s.add("".join(rng.choice(alphabet) for _ in range(k)))
+    return sorted(s)
+
+
+# ---------------------------------------------------------------- hand cases
+
+
+@pytest.mark.parametrize("kmers,expect", [
+    (["ACGT", "CGTA", "GTAC"], ["ACGTAC"]),
+    (["ACGT", "CGTA", "CGTC"], ["ACGT", "CGTA", "CGTC"]),
+    (["ACGT", "TCGT", "CGTA"], ["ACGT", "CGTA", "TCGT"]),
+    (["AAAC"], ["AAAC"]),
+    (["GACG", "CGAC", "ACGA"], ["ACGACG"]),          # isolated 3-cycle, from its smallest k-mer
+    (["AAAA"], ["AAAA"]),                            # self-loop
+    (["AAAA", "CAAA"], ["AAAA", "CAAA"]),            # self-edge pruned by a 2nd predecessor
+])
+def test_hand_cases(kmers, expect):
+    log(f"hand case {kmers}")
+    for B in (1, 4):
+        got = run(uc.encode(kmers, 4), 4, 2, n_buckets=B)
+        assert uc.sorted_lines(got) == sorted(expect)
+
+
+def test_empty_input(ctx):
+    got = ctx.compact(np.zeros((0, 2), np.uint32), 31, 11)
+    assert got == b""
+
+
+# ---------------------------------------------------------------- random sets
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_small_alphabets(seed):
+    # dense branching, bubbles and cycles
+    rng = random.Random(seed)
+    k = rng.choice([3, 4, 5, 6, 8])
+    alphabet = rng.choice(["AC", "ACG", "ACGT"])
+    kmers = rand_set(rng, rng.randint(1, 300), k, alphabet)
+    l = rng.randint(1, k - 1)
+    B = rng.choice([1, 2, 4, 16, 64])
+    log(f"random seed={seed} k={k} l={l} B={B} n={len(kmers)}")
+    got = run(uc.encode(kmers, k), k, l, n_buckets=B, seed=seed + 1)
+    lines = got.decode().splitlines()
+    assert sorted(lines) == sorted(brute_unitigs(kmers))
+    check_partition(kmers, lines, k)
+
+
+@pytest.mark.parametrize("k", [2, 7, 15, 16, 17, 20, 31, 32, 33, 40, 47, 48, 49, 63, 64])
+def test_genome_all_k(k):
+    rng = np.random.default_rng(k)
+    kmers = uc.synth_kmers(int(rng.integers(1 << 30)), 40_000, k, repeat_frac=0.05, seg_min=200, seg_max=800,
+                           repeat_sub=0.01, error_frac=0.01)
+    l = min(11, k - 1)
+    log(f"genome k={k} l={l} n={kmers.shape[0]}")
+    got = run(kmers, k, l)
+    assert uc.sorted_lines(got) == textbook_sorted(kmers, k)
+
+
+@pytest.mark.parametrize("k,l,B", [(31, 11, 64), (31, 11, 16), (63, 11, 256), (15, 7, 64), (47, 9, 32)])
+def test_medium_model_vs_textbook_and_refpipe(k, l, B):
+    kmers = uc.medium_model(k + B, 400_000, k)
+    log(f"medium k={k} l={l} B={B} n={kmers.shape[0]}")
+    got = run(kmers, k, l, n_buckets=B)
+    tb = textbook_sorted(kmers, k)
+    assert uc.sorted_lines(got) == tb
+    if k == 31 and B == 64:
+        ref, _ = oracle.refpipe(kmers, k, l, 64, oracle.generate_permutation(l, 5))
+        assert uc.sorted_lines(ref) == tb
+
+
+def test_canonical_sorted_equal(ctx):
+    kmers = uc.medium_model(3, 200_000, 31)
+    got = ctx.compact(kmers, 31, 11)
+    tb, _, _ = oracle.textbook(kmers, 31)
+    assert uc.canonical_sorted(got) == uc.canonical_sorted(tb)
+
+
+# ---------------------------------------------------------------- modes / forms
+
+
+def test_plain_equals_batched():
+    kmers = uc.medium_model(11, 300_000, 31)
+    a = run(kmers, 31, 11, mode=uc.UC_MODE_BATCHED)
+    b = run(kmers, 31, 11, mode=uc.UC_MODE_PLAIN)
+    assert a == b  # same slots, same order: byte-identical
+
+
+def test_deterministic(ctx):
+    kmers = uc.medium_model(12, 200_000, 31)
+    a = ctx.compact(kmers, 31, 11)
+    b = ctx.compact(kmers, 31, 11)
+    assert a == b
+
+
+def test_result_independent_of_seed_buckets_sampling():
+    kmers = uc.medium_model(13, 200_000, 31)
+    ref = textbook_sorted(kmers, 31)
+    for seed, B, s, load in ((1, 64, 5, 50), (2, 1, 1, 30), (3, 256, 0, 80), (4, 1024, 12, 50)):
+        log(f"seed={seed} B={B} sample_log2={s} load={load}")
+        got = run(kmers, 31, 11, seed=seed, n_buckets=B, sample_log2=s, load_pct=load)
+        assert uc.sorted_lines(got) == ref
+
+
+def test_sets_equal_individual():
+    sets = [uc.medium_model(100 + i, int(n), 31) for i, n in enumerate([5000, 120_000, 40, 0, 70_000, 31])]
+    with uc.Compactor(n_buckets=16) as c:
+        texts = c.compact_sets(sets, 31, 11)
+        for s, t in zip(sets, texts):
+            if s.shape[0] == 0:
+                assert t == b""
+                continue
+            assert uc.sorted_lines(t) == textbook_sorted(s, 31)
+
+
+def test_phase_by_phase_per_bucket():
+    kmers = uc.medium_model(21, 150_000, 31)
+    with uc.Compactor(n_buckets=16) as c:
+        c.load(kmers, 31)
+        c.bucketize(11)
+        info = c.bucket_info()
+        for b in range(info.n_buckets):
+            c.find_edges(b)
+        c.coarsen()
+        c.emit()
+        got = c.text()
+    assert uc.sorted_lines(got) == textbook_sorted(kmers, 31)
+
+
+# ---------------------------------------------------------------- phases vs oracle
+
+
+def test_order_table_matches_oracle(ctx):
+    for l in (1, 3, 6, 8):
+        t = ctx.lmer_order_table(l)
+        assert np.array_equal(t, oracle.lmer_order_table(l, ctx.opts.seed))
+
+
+@pytest.mark.parametrize("use_table", [False, True])
+def test_bucketize_matches_oracle(use_table):
+    k, l, B = 31, 9, 64
+    kmers = uc.medium_model(31, 100_000, k)
+    with uc.Compactor(n_buckets=B, seed=77) as c:
+        c.load(kmers, k)
+        if use_table:
+            order = oracle.generate_permutation(l, 123)  # the reference's kind of table
+            c.bucketize(l, order)
+        else:
+            order = oracle.lmer_order_table(l, 77)       # the inline bijection
+            c.bucketize(l)
+        rows, src, flags, ph, pb = c.bucket_arrays()
+    ob = oracle.bucketize(kmers, k, l, B, order)
+    assert np.array_equal(rows.astype(np.int64), ob["bucket_rows"].astype(np.int64))
+    for b in range(B):
+        g = np.sort(src[rows[b]:rows[b + 1]])
+        o = np.sort(ob["kmer_src"][ob["bucket_rows"][b]:ob["bucket_rows"][b + 1]])
+        assert np.array_equal(g, o.astype(np.uint32)), b
+    # inside a bucket the GPU keeps input order (stable scatter)
+    for b in range(B):
+        seg = src[rows[b]:rows[b + 1]]
+        assert np.all(np.diff(seg.astype(np.int64)) > 0)
+    # pairs: same k-mer, home copy flagged lent, duplicate flagged borrowed
+    assert ph.shape[0] == (flags == uc.F_BORROW).sum() == (flags == uc.F_LENT).sum()
+    assert np.array_equal(src[ph], src[pb])
+    assert np.all(flags[ph] == uc.F_LENT) and np.all(flags[pb] == uc.F_BORROW)
+    bucket_of = np.repeat(np.arange(B), np.diff(rows).astype(np.int64))
+    mins = np.array([oracle.double_lmin(kmers[i], k, l, order) for i in range(kmers.shape[0])])
+    assert np.array_equal(bucket_of[ph], mins[src[ph], 0] % B)
+    assert np.array_equal(bucket_of[pb], mins[src[pb], 1] % B)
+
+
+def test_edges_match_textbook_succ():
+    k = 31
+    kmers = uc.medium_model(41, 120_000, k)
+    with uc.Compactor(n_buckets=64) as c:
+        c.load(kmers, k)
+        c.bucketize(11)
+        c.find_edges()
+        rows, src, flags, ph, pb = c.bucket_arrays()
+        succ = c.succ()
+    ts = oracle.textbook_succ(kmers, k)
+    n = kmers.shape[0]
+    # the out-edge of k-mer i lives on its duplicate when it crosses buckets,
+    # else on its only slot
+    out_slot = np.full(n, -1, np.int64)
+    for s in range(src.shape[0]):
+        if not (flags[s] & uc.F_LENT):
+            out_slot[src[s]] = s
+    got = np.full(n, np.iinfo(np.uint64).max, np.uint64)
+    has = succ[out_slot] != 0xFFFFFFFF
+    got[has] = src[succ[out_slot[has]]].astype(np.uint64)
+    assert np.array_equal(got, ts)
+    # lent copies never carry an edge; borrowed copies are never a target
+    assert np.all(succ[flags == uc.F_LENT] == 0xFFFFFFFF)
+    tgt = succ[succ != 0xFFFFFFFF]
+    assert not np.any(flags[tgt] & uc.F_BORROW)
+
+
+# ---------------------------------------------------------------- cycles
+
+
+def circular_kmers(rng, length, k):
+    g = "".join(rng.choice("ACGT") for _ in range(length))
+    c = g + g[:k - 1]
+    return sorted(set(c[i:i + k] for i in range(length)))
+
+
+@pytest.mark.parametrize("length,k,s", [(50, 15, 5), (2000, 31, 5), (20000, 31, 0), (300, 63, 2)])
+def test_isolated_cycles(length, k, s):
+    rng = random.Random(length)
+    kmers = circular_kmers(rng, length, k) + circular_kmers(rng, length // 2 + 7, k)
+    kmers = sorted(set(kmers))
+    log(f"cycles length={length} k={k} sample_log2={s}")
+    with uc.Compactor(n_buckets=8, sample_log2=s) as c:
+        got = c.compact(uc.encode(kmers, k), k, min(11, k - 1))
+        info = c.coarsen_info()
+    lines = got.decode().splitlines()
+    tb, _, nc = oracle.textbook(uc.encode(kmers, k), k)
+    assert sorted(lines) == uc.sorted_lines(tb)
+    assert info.n_cycles == nc
+    check_partition(kmers, lines, k)
+
+
+def test_long_path_chain_ranking():
+    # one long unitig (a random non-repeating sequence, no canonicalisation):
+    # many sampled splitters, segment chains ranked by pointer jumping
+    rng = random.Random(9)
+    k = 31
+    g = "".join(rng.choice("ACGT") for _ in range(200_000))
+    kmers = sorted(set(g[i:i + k] for i in range(len(g) - k + 1)))
+    with uc.Compactor(n_buckets=64, sample_log2=4) as c:
+        got = c.compact(uc.encode(kmers, k), k, 11)
+        info = c.coarsen_info()
+    assert info.chain_rounds > 0
+    assert uc.sorted_lines(got) == textbook_sorted(uc.encode(kmers, k), k)

@@ -1,0 +1,97 @@
+"""Host-side checks that need no GPU: the C-ABI library loads and exports
+every declared symbol, the host text parser matches the oracle's parser,
+and the CLI keeps the reference's usage and I/O failure behaviour."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import unitigcoarsening_amd as uc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(uc_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared("uc_api.h")
+    assert len(names) >= 25
+    L = C.CDLL(uc.LIB_PATH)
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    # the Python mirror binds exactly the declared set
+    assert sorted(uc.API) == names
+
+
+def test_synth_library_exports():
+    L = C.CDLL(uc.SYNTH_PATH)
+    for n in declared("uc_synth.h"):
+        assert hasattr(L, n), n
+
+
+def test_strerror_codes():
+    L = uc.lib()
+    for code in (0, -1, -2, -3, -4, -5, -6, -7):
+        assert L.uc_strerror(code)
+    assert L.uc_strerror(-99) == b"unknown status"
+
+
+@pytest.mark.parametrize("k", [4, 15, 16, 17, 31, 33, 47, 63, 64])
+def test_parser_matches_oracle(k):
+    kmers = uc.synth_kmers(k, 5000, k)
+    text = uc.format_text(kmers, k)
+    a = uc.parse_text(text, k)
+    b = oracle.parse_text(text, k)
+    assert np.array_equal(a, kmers)
+    assert np.array_equal(a, b)
+
+
+def test_parser_edge_cases():
+    # trailing fields are skipped, lowercase and N read as A, count caps input
+    t = b"3\nACGTN 17\nacgtT\nTTTT\nGGGG\n"
+    a = uc.parse_text(t, 4)
+    assert uc.decode(a, 4) == ["ACGT", "AAAA", "TTTT"]
+    assert np.array_equal(a, oracle.parse_text(t, 4))
+    # CRLF line ends
+    t = b"2\r\nACGT\r\nCCCC\r\n"
+    assert uc.decode(uc.parse_text(t, 4), 4) == ["ACGT", "CCCC"]
+    # short line -> format error
+    with pytest.raises(uc.UcError) as e:
+        uc.parse_text(b"2\nACGT\nAC\n", 4)
+    assert e.value.status == -6
+    # missing count line
+    with pytest.raises(uc.UcError):
+        uc.parse_text(b"ACGT\n", 4)
+    # no newline at end, more count than lines
+    assert uc.decode(uc.parse_text(b"9\nACGT\nTTTT", 4), 4) == ["ACGT", "TTTT"]
+    # empty body
+    assert uc.parse_text(b"0\n", 4).shape == (0, 1)
+
+
+def test_parser_large_multithreaded():
+    kmers = uc.synth_kmers(1, 300_000, 31)
+    text = uc.format_text(kmers, 31)
+    a = uc.parse_text(text, 31)
+    assert np.array_equal(a, kmers)
+
+
+def test_cli_usage():
+    r = subprocess.run([uc.CLI_PATH, "a", "b"], capture_output=True, text=True)
+    assert r.returncode == 1
+    assert "You input 3 args" in r.stdout
+    assert "Usage:" in r.stderr and "<k-mer file> k l <output file>" in r.stderr
+
+
+def test_cli_missing_input(tmp_path):
+    r = subprocess.run([uc.CLI_PATH, str(tmp_path / "nope.txt"), "31", "11", str(tmp_path / "o.txt")],
+                       capture_output=True, text=True)
+    assert r.returncode == 1
+    assert "Error: Could not open input file. Exiting ..." in r.stdout

@@ -1,0 +1,345 @@
+"""Pins the CPU oracle (no GPU).
+
+Each oracle function is checked against something it cannot trivially
+agree with: a brute-force definition, a closed form, an invariant, a
+published test vector, or the other restatement.
+"""
+import itertools
+import random
+
+import numpy as np
+import pytest
+
+import oracle
+import unitigcoarsening_amd as uc
+from brute import brute_unitigs, check_partition
+
+
+def rand_kmers(rng, n, k, alphabet="ACGT"):
+    s = set()
+    while len(s) < n:
+        s.add("".join(rng.choice(alphabet) for _ in range(k)))
+    return list(s)
+
+
+def genome_kmers(rng, glen, k, canonical=True):
+    g = "".join(rng.choice("ACGT") for _ in range(glen))
+    comp = str.maketrans("ACGT", "TGCA")
+    out = set()
+    for i in range(glen - k + 1):
+        x = g[i:i + k]
+        if canonical:
+            r = x.translate(comp)[::-1]
+            x = min(x, r)
+        out.add(x)
+    return list(out)
+
+
+def textbook_lines(kmers, k):
+    text, nu, nc = oracle.textbook(uc.encode(kmers, k), k)
+    lines = text.decode().splitlines()
+    assert len(lines) == nu
+    return lines, nc
+
+
+# ---------------------------------------------------------------- packing
+
+
+def test_pack_known_vector():
+    # driver.cpp:233-246: char i -> bits 2*(i%16) of word i/16; A0 C1 G2 T3
+    k = 17
+    s = "ACGTACGTACGTACGTG"
+    w = oracle.pack(s.encode(), 1, k)[0]
+    assert w[0] == sum(((i % 4) << (2 * i)) for i in range(16))
+    assert w[0] == 0xE4E4E4E4
+    assert w[1] == 2  # 'G' at position 16 -> word 1 bits 0..1
+
+
+def test_pack_nonacgt_maps_to_A():
+    # the 256-entry table is zero-initialised (driver.cpp:228-232)
+    w = oracle.pack(b"acgtNnXC", 1, 8)[0]
+    assert w[0] == (1 << 14)  # only the trailing 'C' is nonzero
+
+
+def test_encode_matches_oracle_pack():
+    rng = random.Random(3)
+    for k in (2, 15, 16, 17, 31, 32, 33, 47, 48, 63, 64):
+        ks = rand_kmers(rng, min(20, 4 ** k), k)
+        a = oracle.pack("".join(ks).encode(), len(ks), k)
+        b = uc.encode(ks, k)
+        assert np.array_equal(a, b)
+        assert uc.decode(a, k) == ks
+
+
+def test_parse_text_reference_format():
+    text = b"3\nACGTACGTACGTACG 12\nTTTTTTTTTTTTTTTAAA\nCCCCCCCCCCCCCCC\nGGGGGGGGGGGGGGG\n"
+    got = oracle.parse_text(text, 15)
+    assert uc.decode(got, 15) == ["ACGTACGTACGTACG", "TTTTTTTTTTTTTTT", "CCCCCCCCCCCCCCC"]
+
+
+def test_parse_text_short_line_rejected():
+    with pytest.raises(oracle.OracleError):
+        oracle.parse_text(b"2\nACGT\nAC\n", 4)
+
+
+def test_parse_text_count_larger_than_file():
+    got = oracle.parse_text(b"10\nACGT\nTTTT", 4)
+    assert uc.decode(got, 4) == ["ACGT", "TTTT"]
+
+
+# ---------------------------------------------------------------- hashing
+
+
+def fnv1a32(data: bytes) -> int:
+    h = 2166136261
+    for b in data:
+        h ^= b
+        h = (h * 16777619) & 0xFFFFFFFF
+    return h
+
+
+def test_fnv_pref_matches_fnv1a_on_low_bytes():
+    # Published FNV-1a 32-bit vectors: fnv1a("") = 0x811c9dc5,
+    # fnv1a("a") = 0xe40c292c, fnv1a("foobar") = 0xbf9cf968.
+    assert fnv1a32(b"") == 0x811C9DC5
+    assert fnv1a32(b"a") == 0xE40C292C
+    assert fnv1a32(b"foobar") == 0xBF9CF968
+    # k = 31: two words; the prefix masks the last word to 14 characters.
+    # Bytes below 0x80 make the signed-char hash equal plain FNV-1a.
+    w = np.array([0x41424344, 0x01020304], np.uint32)
+    expect = fnv1a32(bytes([0x44, 0x43, 0x42, 0x41, 0x04, 0x03, 0x02, 0x01]))
+    assert oracle.fnv_pref(w) == expect
+
+
+def test_fnv_sign_extension():
+    # assembler.h:16: `char y` is signed, so byte 0x80 is xor-ed in as
+    # 0xFFFFFF80 -- not plain FNV-1a.
+    w = np.array([0x00000080, 0x0], np.uint32)
+    h = 2166136261
+    for b in (0x80, 0, 0, 0, 0, 0, 0, 0):
+        h ^= (b | 0xFFFFFF00) if b & 0x80 else b
+        h = (h * 16777619) & 0xFFFFFFFF
+    assert oracle.fnv_pref(w) == h
+    assert oracle.fnv_pref(w) != fnv1a32(bytes([0x80, 0, 0, 0, 0, 0, 0, 0]))
+
+
+def test_fnv_suf_is_pref_of_shifted():
+    # the suffix of x is the prefix of the k-mer x shifted by one character
+    rng = random.Random(5)
+    for _ in range(50):
+        s = "".join(rng.choice("ACGT") for _ in range(31))
+        t = s[1:] + rng.choice("ACGT")
+        assert oracle.fnv_suf(uc.encode([s], 31)[0]) == oracle.fnv_pref(uc.encode([t], 31)[0])
+
+
+# ---------------------------------------------------------------- l-mer order
+
+
+@pytest.mark.parametrize("l", [1, 2, 3, 5, 8, 11])
+def test_lmer_order_is_bijection(l):
+    t = oracle.lmer_order_table(l, 0x1234_5678_9ABC) if l <= 8 else None
+    if t is None:
+        return
+    assert np.array_equal(np.sort(t), np.arange(4 ** l, dtype=np.uint32))
+
+
+def test_lmer_order_bijection_l11_sampled():
+    # l = 11: 4M codes; check a full permutation with numpy
+    L = oracle.lib()
+    n = 4 ** 11
+    vals = np.fromiter((L.orc_lmer_order(c, 11, 77) for c in range(0, n, 97)), np.uint32)
+    assert len(np.unique(vals)) == len(vals)
+    assert vals.max() < n
+
+
+@pytest.mark.parametrize("l", [1, 4, 7])
+def test_generate_permutation_is_permutation(l):
+    p = oracle.generate_permutation(l, 42)
+    assert np.array_equal(np.sort(p), np.arange(4 ** l, dtype=np.uint32))
+    assert not np.array_equal(p, oracle.generate_permutation(l, 43))
+
+
+def test_double_lmin_brute():
+    rng = random.Random(11)
+    code = {"A": 0, "C": 1, "G": 2, "T": 3}
+    for k, l in ((15, 5), (31, 11), (47, 9), (63, 13), (8, 7)):
+        order = oracle.lmer_order_table(min(l, 8), 9) if l <= 8 else None
+        if order is None:
+            order = np.random.default_rng(l).permutation(4 ** l).astype(np.uint32)
+        for _ in range(20):
+            s = "".join(rng.choice("ACGT") for _ in range(k))
+
+            def ordv(j):
+                c = 0
+                for ch in s[j:j + l]:
+                    c = c * 4 + code[ch]
+                return int(order[c])
+
+            o1 = min(ordv(j) for j in range(0, k - l))          # inside the (k-1)-prefix
+            o2 = min(ordv(j) for j in range(1, k - l + 1))      # inside the (k-1)-suffix
+            assert oracle.double_lmin(uc.encode([s], k)[0], k, l, order) == (o1, o2)
+
+
+# ---------------------------------------------------------------- textbook
+
+
+def test_textbook_hand_cases():
+    k = 4
+    # linear path ACGT -> CGTA -> GTAC
+    lines, _ = textbook_lines(["ACGT", "CGTA", "GTAC"], k)
+    assert lines == ["ACGTAC"]
+    # branch: ACGT has two successors -> no edge out of ACGT
+    lines, _ = textbook_lines(["ACGT", "CGTA", "CGTC"], k)
+    assert sorted(lines) == ["ACGT", "CGTA", "CGTC"]
+    # merge: two predecessors of CGTA
+    lines, _ = textbook_lines(["ACGT", "TCGT", "CGTA"], k)
+    assert sorted(lines) == ["ACGT", "CGTA", "TCGT"]
+    # isolated k-mer
+    lines, _ = textbook_lines(["AAAC"], k)
+    assert lines == ["AAAC"]
+
+
+def test_textbook_cycle_starts_at_smallest():
+    # ACGA -> CGAC -> GACG -> ACGA (a 3-cycle)
+    lines, nc = textbook_lines(["GACG", "CGAC", "ACGA"], 4)
+    assert nc == 1
+    assert lines == ["ACGACG"]
+
+
+def test_textbook_self_loop():
+    lines, nc = textbook_lines(["AAAA"], 4)
+    assert nc == 1 and lines == ["AAAA"]
+
+
+def test_textbook_rejects_duplicates():
+    with pytest.raises(oracle.OracleError):
+        oracle.textbook(uc.encode(["ACGT", "ACGT"], 4), 4)
+
+
+@pytest.mark.parametrize("seed", range(30))
+def test_textbook_vs_brute_random(seed):
+    rng = random.Random(seed)
+    k = rng.choice([2, 3, 4, 5])
+    alphabet = rng.choice(["AC", "ACG", "ACGT"])
+    universe = len(alphabet) ** k
+    n = rng.randint(1, min(40, universe))
+    kmers = rand_kmers(rng, n, k, alphabet)
+    lines, _ = textbook_lines(kmers, k)
+    assert sorted(lines) == sorted(brute_unitigs(kmers))
+    check_partition(kmers, lines, k)
+
+
+def test_textbook_vs_brute_exhaustive_small():
+    # every subset of the 8 binary-alphabet 3-mers
+    univ = ["".join(p) for p in itertools.product("AC", repeat=3)]
+    for mask in range(1, 1 << len(univ)):
+        kmers = [u for i, u in enumerate(univ) if mask >> i & 1]
+        lines, _ = textbook_lines(kmers, 3)
+        assert sorted(lines) == sorted(brute_unitigs(kmers)), kmers
+
+
+@pytest.mark.parametrize("k", [5, 15, 16, 17, 31, 32, 33, 47, 63, 64])
+def test_textbook_genome_partition(k):
+    rng = random.Random(k)
+    kmers = genome_kmers(rng, 3000, k)
+    lines, _ = textbook_lines(kmers, k)
+    check_partition(kmers, lines, k)
+
+
+# ---------------------------------------------------------------- restatement
+
+
+def refpipe_lines(kmers, k, l, B, seed=1):
+    order = oracle.generate_permutation(l, seed)
+    text, stats = oracle.refpipe(uc.encode(kmers, k), k, l, B, order)
+    return text.decode().splitlines(), stats
+
+
+@pytest.mark.parametrize("k,l,B", [(15, 5, 4), (15, 7, 64), (31, 11, 64), (31, 7, 16), (47, 9, 8),
+                                   (63, 11, 64), (31, 3, 64)])
+def test_refpipe_equals_textbook_genome(k, l, B):
+    rng = random.Random(k * 100 + l + B)
+    kmers = genome_kmers(rng, 20000, k)
+    ref, stats = refpipe_lines(kmers, k, l, B)
+    tb, nc = textbook_lines(kmers, k)
+    assert nc == 0
+    assert sorted(ref) == sorted(tb)
+    assert stats["n_cross"] > 0 or B == 1
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_refpipe_equals_textbook_repeats_errors(seed):
+    kmers = uc.medium_model(seed, 60000, 31)
+    strs = uc.decode(kmers, 31)
+    order = oracle.generate_permutation(11, seed)
+    text, _ = oracle.refpipe(kmers, 31, 11, 64, order)
+    tb, _, nc = oracle.textbook(kmers, 31)
+    assert nc == 0
+    assert uc.sorted_lines(text) == uc.sorted_lines(tb)
+    if seed == 0:
+        check_partition(strs, text.decode().splitlines(), 31)
+
+
+def test_refpipe_independent_of_order_and_buckets():
+    # the unitig set is a function of the k-mer set only (bucketing and the
+    # l-mer order only move work around)
+    kmers = uc.medium_model(99, 30000, 31)
+    outs = set()
+    for seed, B, l in ((1, 64, 11), (2, 64, 11), (3, 16, 9), (4, 1, 11), (5, 2, 5)):
+        order = oracle.generate_permutation(l, seed)
+        text, _ = oracle.refpipe(kmers, 31, l, B, order)
+        outs.add(tuple(uc.sorted_lines(text)))
+    assert len(outs) == 1
+
+
+def test_refpipe_rejects_self_loop():
+    # a homopolymer k-mer with no other neighbour is its own successor and
+    # predecessor: the reference's HEC spins (heuristics.h:119-147)
+    k = 15
+    with pytest.raises(oracle.OracleError) as e:
+        refpipe_lines(["A" * k, "ACGTACGTACGTACG"], k, 5, 4)
+    assert e.value.rc == -4
+    # with a second predecessor the self-edge is pruned and nothing spins
+    lines, _ = refpipe_lines(["A" * k, "C" + "A" * (k - 1)], k, 5, 4)
+    assert sorted(lines) == sorted(["A" * k, "C" + "A" * (k - 1)])
+
+
+def test_refpipe_rejects_k_not_15_mod_16():
+    with pytest.raises(oracle.OracleError):
+        refpipe_lines(["ACGTACGTACGTACGTACGT"], 20, 5, 4)
+
+
+def test_bucketize_invariants():
+    kmers = uc.medium_model(7, 20000, 31)
+    k, l, B = 31, 11, 64
+    order = oracle.generate_permutation(l, 7)
+    b = oracle.bucketize(kmers, k, l, B, order)
+    rows, src = b["bucket_rows"], b["kmer_src"]
+    n = kmers.shape[0]
+    mins = [oracle.double_lmin(kmers[i], k, l, order) for i in range(n)]
+    home = np.array([m[0] % B for m in mins])
+    suf = np.array([m[1] % B for m in mins])
+    bucket_of_slot = np.repeat(np.arange(B), np.diff(rows).astype(np.int64))
+    # every k-mer in its prefix bucket; crossing k-mers also in their suffix bucket
+    cnt = np.bincount(src.astype(np.int64), minlength=n)
+    assert np.array_equal(cnt, 1 + (home != suf))
+    for slot, i in enumerate(src):
+        assert bucket_of_slot[slot] in (home[i], suf[i])
+    # lend/borrow lists are aligned: (i,j) lend entry at offset o pairs with
+    # the (j,i) borrow entry at the same offset, and both name the same k-mer
+    cr, lend, borrow = b["cross_rows"], b["cross_lend"], b["cross_borrow"]
+    pairs = 0
+    for i in range(B):
+        for j in range(B):
+            if i == j:
+                continue
+            a0, a1 = cr[i * B + j], cr[i * B + j + 1]
+            c0 = cr[j * B + i]
+            assert a1 - a0 == cr[j * B + i + 1] - c0
+            for o in range(a1 - a0):
+                if lend[a0 + o] != 0xFFFFFFFF:
+                    hk = src[rows[i] + lend[a0 + o]]
+                    bk = src[rows[j] + borrow[c0 + o]]
+                    assert hk == bk and home[hk] == i and suf[hk] == j
+                    pairs += 1
+    assert pairs == int((home != suf).sum())

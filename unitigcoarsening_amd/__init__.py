@@ -1,0 +1,437 @@
+"""MI355X-native unitig compaction (the compaction path of UnitigCoarsening).
+
+The product is the C-ABI library ``_lib/libuc.so`` (declared in
+``include/uc_api.h``) and the ``_lib/uc_main`` command-line drop-in.  This
+module is a thin ctypes mirror of that ABI for Python callers and tests;
+no compaction work happens in Python, and every call fails loudly when the
+native library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(HERE, "_lib")
+LIB_PATH = os.path.join(LIB_DIR, "libuc.so")
+SYNTH_PATH = os.path.join(LIB_DIR, "libuc_synth.so")
+CLI_PATH = os.path.join(LIB_DIR, "uc_main")
+
+UC_OK = 0
+UC_ALL_BUCKETS = 0xFFFFFFFF
+UC_MODE_BATCHED = 0
+UC_MODE_PLAIN = 1
+UC_FLAG_PROFILE = 1
+
+F_LENT = 1
+F_BORROW = 2
+
+
+class UcError(RuntimeError):
+    def __init__(self, status, msg=""):
+        super().__init__(f"uc status {status}: {msg}")
+        self.status = status
+
+
+class UcOpts(C.Structure):
+    _fields_ = [
+        ("device", C.c_int),
+        ("seed", C.c_uint64),
+        ("n_buckets", C.c_uint32),
+        ("mode", C.c_uint32),
+        ("load_pct", C.c_uint32),
+        ("sample_log2", C.c_uint32),
+        ("hbm_budget", C.c_uint64),
+        ("flags", C.c_uint32),
+    ]
+
+
+class UcBucketInfo(C.Structure):
+    _fields_ = [
+        ("n_kmers", C.c_uint64),
+        ("m_placements", C.c_uint64),
+        ("n_cross", C.c_uint64),
+        ("n_buckets", C.c_uint32),
+        ("n_sets", C.c_uint32),
+        ("largest_bucket", C.c_uint64),
+    ]
+
+
+class UcCoarsenInfo(C.Structure):
+    _fields_ = [
+        ("n_unitigs", C.c_uint64),
+        ("n_cycles", C.c_uint64),
+        ("text_bytes", C.c_uint64),
+        ("n_splitters", C.c_uint64),
+        ("chain_rounds", C.c_uint32),
+    ]
+
+
+class UcTiming(C.Structure):
+    _fields_ = [
+        ("bucketize_ms", C.c_float),
+        ("edges_ms", C.c_float),
+        ("coarsen_ms", C.c_float),
+        ("emit_ms", C.c_float),
+        ("total_ms", C.c_float),
+        ("device_bytes", C.c_uint64),
+    ]
+
+
+class SynthParams(C.Structure):
+    _fields_ = [
+        ("seed", C.c_uint64),
+        ("genome_len", C.c_uint64),
+        ("k", C.c_uint32),
+        ("repeat_frac", C.c_double),
+        ("seg_min", C.c_uint64),
+        ("seg_max", C.c_uint64),
+        ("repeat_sub", C.c_double),
+        ("error_frac", C.c_double),
+    ]
+
+
+# every symbol of include/uc_api.h with (restype, argtypes)
+_P = C.c_void_p
+_U32P = C.POINTER(C.c_uint32)
+_U64P = C.POINTER(C.c_uint64)
+API = {
+    "uc_default_opts": (None, [C.POINTER(UcOpts)]),
+    "uc_ctx_create": (C.c_int, [C.POINTER(UcOpts), C.POINTER(_P)]),
+    "uc_ctx_destroy": (C.c_int, [_P]),
+    "uc_last_error": (C.c_char_p, [_P]),
+    "uc_strerror": (C.c_char_p, [C.c_int]),
+    "uc_compact": (C.c_int, [_P, _P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int]),
+    "uc_compact_loaded": (C.c_int, [_P, C.c_uint32]),
+    "uc_kernel_stat": (C.c_int, [_P, C.c_uint32, C.c_char_p, C.c_uint32, C.POINTER(C.c_double), _U64P]),
+    "uc_kernel_stats_reset": (C.c_int, [_P]),
+    "uc_compact_sets": (C.c_int, [_P, _P, _U64P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int]),
+    "uc_lmer_order_table": (C.c_int, [_P, C.c_uint32, _P, C.c_int]),
+    "uc_load": (C.c_int, [_P, _P, _U64P, C.c_uint32, C.c_uint32, C.c_int]),
+    "uc_bucketize": (C.c_int, [_P, C.c_uint32, _P, C.c_int]),
+    "uc_bucket_info_get": (C.c_int, [_P, C.POINTER(UcBucketInfo)]),
+    "uc_bucket_copy": (C.c_int, [_P, _U64P, _U32P, C.POINTER(C.c_uint8), _U32P, _U32P]),
+    "uc_find_edges": (C.c_int, [_P, C.c_uint32]),
+    "uc_succ_copy": (C.c_int, [_P, _U32P]),
+    "uc_coarsen": (C.c_int, [_P]),
+    "uc_coarsen_info_get": (C.c_int, [_P, C.POINTER(UcCoarsenInfo)]),
+    "uc_emit": (C.c_int, [_P]),
+    "uc_text_size": (C.c_uint64, [_P]),
+    "uc_text_device_ptr": (C.c_void_p, [_P]),
+    "uc_text_copy": (C.c_int, [_P, C.c_char_p, C.c_uint64]),
+    "uc_set_text_offset": (C.c_uint64, [_P, C.c_uint32]),
+    "uc_text_append_file": (C.c_int, [_P, C.c_char_p]),
+    "uc_parse_kmer_text": (C.c_int, [C.c_char_p, C.c_uint64, C.c_uint32, C.POINTER(_U32P), _U64P]),
+    "uc_read_kmer_file": (C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(_U32P), _U64P]),
+    "uc_free": (None, [_P]),
+    "uc_timing_get": (C.c_int, [_P, C.POINTER(UcTiming)]),
+    "uc_sync": (C.c_int, [_P]),
+}
+
+_lib = None
+_synth = None
+
+
+def lib():
+    """The compaction library; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} missing: run __graft_entry__.build() or "
+                               "python -m unitigcoarsening_amd.build")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in API.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def synth_lib():
+    global _synth
+    if _synth is None:
+        if not os.path.exists(SYNTH_PATH):
+            raise RuntimeError(f"{SYNTH_PATH} missing: build first")
+        S = C.CDLL(SYNTH_PATH)
+        S.uc_synth_kmers.restype = C.c_int
+        S.uc_synth_kmers.argtypes = [C.POINTER(SynthParams), C.POINTER(_U32P), _U64P]
+        S.uc_synth_free.restype = None
+        S.uc_synth_free.argtypes = [_P]
+        S.uc_synth_format_text.restype = C.c_int
+        S.uc_synth_format_text.argtypes = [_P, C.c_uint64, C.c_uint32, C.c_char_p, C.c_uint64, _U64P]
+        _synth = S
+    return _synth
+
+
+def words_per_kmer(k: int) -> int:
+    return (k + 15) // 16
+
+
+def _check(rc, ctx=None):
+    if rc != UC_OK:
+        L = lib()
+        msg = L.uc_strerror(rc).decode()
+        if ctx:
+            msg += ": " + (L.uc_last_error(ctx) or b"").decode()
+        raise UcError(rc, msg)
+
+
+# ---------------------------------------------------------------------------
+# synthetic workloads and the text format
+
+
+def synth_kmers(seed, genome_len, k, repeat_frac=0.0, seg_min=2000, seg_max=10000, repeat_sub=0.001,
+                error_frac=0.0) -> np.ndarray:
+    """Distinct canonical k-mers of a seeded synthetic genome, reference packing
+    (uint32 array of shape (n, ceil(k/16)))."""
+    S = synth_lib()
+    p = SynthParams(seed, genome_len, k, repeat_frac, seg_min, seg_max, repeat_sub, error_frac)
+    ptr = _U32P()
+    n = C.c_uint64()
+    rc = S.uc_synth_kmers(C.byref(p), C.byref(ptr), C.byref(n))
+    if rc != 0:
+        raise RuntimeError(f"uc_synth_kmers failed ({rc})")
+    w = words_per_kmer(k)
+    out = np.ctypeslib.as_array(ptr, shape=(max(n.value, 1) * w,))[: n.value * w].copy().reshape(n.value, w)
+    S.uc_synth_free(ptr)
+    return out
+
+
+def medium_model(seed, genome_len, k=31):
+    """The repeat and error model of the medium workload: 5% repeats of
+    2-10 kbp segments with 0.1% substitutions, 1% error k-mers."""
+    return synth_kmers(seed, genome_len, k, repeat_frac=0.05, seg_min=2000, seg_max=10000, repeat_sub=0.001,
+                       error_frac=0.01)
+
+
+def format_text(packed: np.ndarray, k: int) -> bytes:
+    """Reference text format: count line, then one k-mer per line."""
+    S = synth_lib()
+    packed = np.ascontiguousarray(packed, dtype=np.uint32)
+    n = packed.shape[0]
+    need = C.c_uint64()
+    S.uc_synth_format_text(packed.ctypes.data, n, k, None, 0, C.byref(need))
+    buf = C.create_string_buffer(need.value)
+    rc = S.uc_synth_format_text(packed.ctypes.data, n, k, buf, need.value, C.byref(need))
+    if rc != 0:
+        raise RuntimeError("format_text failed")
+    return buf.raw[: need.value]
+
+
+def parse_text(text: bytes, k: int) -> np.ndarray:
+    """k-mer text -> reference packing via the library's host parser."""
+    L = lib()
+    ptr = _U32P()
+    n = C.c_uint64()
+    _check(L.uc_parse_kmer_text(text, len(text), k, C.byref(ptr), C.byref(n)))
+    w = words_per_kmer(k)
+    out = np.ctypeslib.as_array(ptr, shape=(max(n.value, 1) * w,))[: n.value * w].copy().reshape(n.value, w)
+    L.uc_free(ptr)
+    return out
+
+
+def decode(packed: np.ndarray, k: int) -> list:
+    """Reference packing -> list of k-mer strings (for tests and small data)."""
+    packed = np.asarray(packed, dtype=np.uint32)
+    out = []
+    for row in packed:
+        out.append("".join("ACGT"[(int(row[i // 16]) >> (2 * (i % 16))) & 3] for i in range(k)))
+    return out
+
+
+def encode(kmers, k: int) -> np.ndarray:
+    """k-mer strings -> reference packing."""
+    w = words_per_kmer(k)
+    code = {"A": 0, "C": 1, "G": 2, "T": 3}
+    out = np.zeros((len(kmers), w), dtype=np.uint32)
+    for j, s in enumerate(kmers):
+        for i in range(k):
+            out[j, i // 16] |= code.get(s[i], 0) << (2 * (i % 16))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# the compactor
+
+
+class Compactor:
+    """One device context (uc_ctx).  Not thread-safe: one host thread per object."""
+
+    def __init__(self, device=0, seed=0x5EED, n_buckets=64, mode=UC_MODE_BATCHED, load_pct=50, sample_log2=5,
+                 hbm_budget=0, profile=False):
+        L = lib()
+        o = UcOpts()
+        L.uc_default_opts(C.byref(o))
+        o.device, o.seed, o.n_buckets, o.mode = device, seed, n_buckets, mode
+        o.load_pct, o.sample_log2, o.hbm_budget = load_pct, sample_log2, hbm_budget
+        o.flags = UC_FLAG_PROFILE if profile else 0
+        self.opts = o
+        self._ctx = _P()
+        _check(L.uc_ctx_create(C.byref(o), C.byref(self._ctx)))
+        self._keep = None
+
+    def close(self):
+        if self._ctx:
+            lib().uc_ctx_destroy(self._ctx)
+            self._ctx = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- whole jobs ---------------------------------------------------------
+    def compact(self, packed, k, l, device_ptr=None, n=None):
+        """Run the whole path; returns the unitig text (bytes)."""
+        L = lib()
+        if device_ptr is not None:
+            _check(L.uc_compact(self._ctx, C.c_void_p(device_ptr), n, k, l, 1), self._ctx)
+        else:
+            packed = np.ascontiguousarray(packed, dtype=np.uint32)
+            self._keep = packed
+            _check(L.uc_compact(self._ctx, packed.ctypes.data, packed.shape[0], k, l, 0), self._ctx)
+        return self.text()
+
+    def compact_sets(self, sets, k, l):
+        """Batched form over independent k-mer sets; returns one text per set."""
+        L = lib()
+        arrs = [np.ascontiguousarray(s, dtype=np.uint32).reshape(-1, words_per_kmer(k)) for s in sets]
+        offs = np.zeros(len(arrs) + 1, dtype=np.uint64)
+        offs[1:] = np.cumsum([a.shape[0] for a in arrs])
+        allk = np.ascontiguousarray(np.concatenate(arrs, axis=0)) if arrs else np.zeros((0, 1), np.uint32)
+        self._keep = allk
+        _check(L.uc_compact_sets(self._ctx, allk.ctypes.data, offs.ctypes.data_as(_U64P), len(arrs), k, l, 0),
+               self._ctx)
+        t = self.text()
+        bounds = [L.uc_set_text_offset(self._ctx, i) for i in range(len(arrs) + 1)]
+        return [t[bounds[i]:bounds[i + 1]] for i in range(len(arrs))]
+
+    # -- phases -------------------------------------------------------------
+    def load(self, packed, k, set_sizes=None):
+        L = lib()
+        packed = np.ascontiguousarray(packed, dtype=np.uint32)
+        self._keep = packed
+        sizes = [packed.shape[0]] if set_sizes is None else list(set_sizes)
+        offs = np.zeros(len(sizes) + 1, dtype=np.uint64)
+        offs[1:] = np.cumsum(sizes)
+        _check(L.uc_load(self._ctx, packed.ctypes.data, offs.ctypes.data_as(_U64P), len(sizes), k, 0), self._ctx)
+
+    def compact_loaded(self, l):
+        """Whole job on the k-mers of the last load(), already in device memory."""
+        _check(lib().uc_compact_loaded(self._ctx, l), self._ctx)
+
+    def kernel_stats(self):
+        """{kernel name: (total ms, launches)} since the last reset (profile=True)."""
+        L = lib()
+        out = {}
+        i = 0
+        name = C.create_string_buffer(64)
+        ms = C.c_double()
+        nl = C.c_uint64()
+        while L.uc_kernel_stat(self._ctx, i, name, 64, C.byref(ms), C.byref(nl)) == UC_OK:
+            out[name.value.decode()] = (ms.value, nl.value)
+            i += 1
+        return out
+
+    def kernel_stats_reset(self):
+        _check(lib().uc_kernel_stats_reset(self._ctx), self._ctx)
+
+    def bucketize(self, l, order_table=None):
+        """order_table: None (inline seeded bijection) or a host array of 4^l orders."""
+        if order_table is None:
+            _check(lib().uc_bucketize(self._ctx, l, None, 0), self._ctx)
+        else:
+            t = np.ascontiguousarray(order_table, dtype=np.uint32)
+            assert t.shape[0] == 4 ** l
+            _check(lib().uc_bucketize(self._ctx, l, t.ctypes.data, 0), self._ctx)
+
+    def lmer_order_table(self, l):
+        out = np.zeros(4 ** l, np.uint32)
+        _check(lib().uc_lmer_order_table(self._ctx, l, out.ctypes.data, 0), self._ctx)
+        return out
+
+    def bucket_info(self):
+        i = UcBucketInfo()
+        _check(lib().uc_bucket_info_get(self._ctx, C.byref(i)), self._ctx)
+        return i
+
+    def bucket_arrays(self):
+        i = self.bucket_info()
+        G = i.n_sets * i.n_buckets
+        rows = np.zeros(G + 1, np.uint64)
+        src = np.zeros(max(i.m_placements, 1), np.uint32)
+        flags = np.zeros(max(i.m_placements, 1), np.uint8)
+        ph = np.zeros(max(i.n_cross, 1), np.uint32)
+        pb = np.zeros(max(i.n_cross, 1), np.uint32)
+        _check(lib().uc_bucket_copy(self._ctx, rows.ctypes.data_as(_U64P), src.ctypes.data_as(_U32P),
+                                    flags.ctypes.data_as(C.POINTER(C.c_uint8)), ph.ctypes.data_as(_U32P),
+                                    pb.ctypes.data_as(_U32P)), self._ctx)
+        m, x = i.m_placements, i.n_cross
+        return rows, src[:m], flags[:m], ph[:x], pb[:x]
+
+    def find_edges(self, bucket=UC_ALL_BUCKETS):
+        _check(lib().uc_find_edges(self._ctx, bucket), self._ctx)
+
+    def succ(self):
+        i = self.bucket_info()
+        out = np.zeros(max(i.m_placements, 1), np.uint32)
+        _check(lib().uc_succ_copy(self._ctx, out.ctypes.data_as(_U32P)), self._ctx)
+        return out[: i.m_placements]
+
+    def coarsen(self):
+        _check(lib().uc_coarsen(self._ctx), self._ctx)
+
+    def coarsen_info(self):
+        i = UcCoarsenInfo()
+        _check(lib().uc_coarsen_info_get(self._ctx, C.byref(i)), self._ctx)
+        return i
+
+    def emit(self):
+        _check(lib().uc_emit(self._ctx), self._ctx)
+
+    # -- results ------------------------------------------------------------
+    def text(self) -> bytes:
+        L = lib()
+        n = L.uc_text_size(self._ctx)
+        buf = C.create_string_buffer(max(n, 1))
+        _check(L.uc_text_copy(self._ctx, buf, n), self._ctx)
+        return buf.raw[:n]
+
+    def text_size(self):
+        return lib().uc_text_size(self._ctx)
+
+    def timing(self):
+        t = UcTiming()
+        _check(lib().uc_timing_get(self._ctx, C.byref(t)), self._ctx)
+        return t
+
+    def sync(self):
+        _check(lib().uc_sync(self._ctx), self._ctx)
+
+
+def sorted_lines(text: bytes) -> list:
+    """Unitig text -> sorted list of lines (order-free comparison)."""
+    return sorted(text.decode().splitlines())
+
+
+_COMP = str.maketrans("ACGT", "TGCA")
+
+
+def canonical_sorted(text: bytes) -> list:
+    """Each unitig in canonical orientation (min of it and its reverse
+    complement), then sorted -- the comparison the north star names."""
+    out = []
+    for s in text.decode().splitlines():
+        r = s.translate(_COMP)[::-1]
+        out.append(min(s, r))
+    return sorted(out)

@@ -1,0 +1,1800 @@
+// uc_core.hip -- MI355X (gfx950) unitig compaction: kernels and the C ABI.
+//
+// Pipeline of one job (reference phases in brackets):
+//   1. bucketing   [lmin_buckets.h:288-433, driver.cpp:137-225]
+//      k_lmin_count: prefix/suffix l-minimizers per k-mer, per-tile bucket
+//      histogram in LDS; device scan; k_lmin_scatter: stable wavefront-
+//      ballot ranking, crossing k-mers duplicated into their suffix bucket,
+//      (home, borrow) pair list.
+//   2. edge finding [assembler.h:113-214]
+//      per bucket: k_edge_insert (CAS (k-1)-prefixes into an open-
+//      addressing table, mark shared prefixes), k_edge_lookup (probe with
+//      the (k-1)-suffix, count in-degree on the hit slot), k_edge_resolve
+//      (keep u->v iff the prefix is unshared and has exactly one hit).
+//   3. contraction [coarseners.h:68-485, heuristics.h:105-256,
+//      compact_graph.h, driver.cpp:527-587]
+//      k_splice re-attaches each borrowed duplicate's out-edge to its home
+//      copy; splitters (path heads + a hashed 2^-s sample) walk their
+//      segment (k_walk); segment chains are ranked by pointer jumping
+//      (k_chain_*); k_final gives every k-mer (unitig head, position).
+//      Isolated cycles are cut at their lexicographically smallest k-mer
+//      and the contraction repeats.
+//   4. emission    [compact.h:59-210, lmin_buckets.h:172-283]
+//      unitig scan, k_order (k-mers in path order), k_emit (text).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "../../include/uc_api.h"
+#include "uc_kernels.h"
+
+namespace uc {
+
+// ---------------------------------------------------------------------------
+// device helpers
+
+template <int W>
+struct Kmer {
+    uint64_t w[W];
+};
+
+__device__ __forceinline__ uint32_t fmix32(uint32_t h)
+{
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    return h;
+}
+
+template <int W>
+__device__ __forceinline__ Kmer<W> load_kmer(const uint64_t *__restrict__ a, uint64_t i, uint64_t top_mask)
+{
+    Kmer<W> x;
+    if constexpr (W == 1) {
+        x.w[0] = a[i] & top_mask;
+    } else {
+        const ulonglong2 v = reinterpret_cast<const ulonglong2 *>(a)[i];
+        x.w[0] = v.x;
+        x.w[1] = v.y & top_mask;
+    }
+    return x;
+}
+
+template <int W>
+__device__ __forceinline__ void store_kmer(uint64_t *__restrict__ a, uint64_t i, const Kmer<W> &x)
+{
+    if constexpr (W == 1) {
+        a[i] = x.w[0];
+    } else {
+        reinterpret_cast<ulonglong2 *>(a)[i] = make_ulonglong2(x.w[0], x.w[1]);
+    }
+}
+
+// (k-1)-prefix: characters 0..k-2 = the low 2(k-1) bits
+template <int W>
+__device__ __forceinline__ Kmer<W> prefix_key(const Kmer<W> &x, uint64_t pm0, uint64_t pm1)
+{
+    Kmer<W> r;
+    r.w[0] = x.w[0] & pm0;
+    if constexpr (W == 2) r.w[1] = x.w[1] & pm1;
+    return r;
+}
+
+// (k-1)-suffix: characters 1..k-1 = the value shifted right by one character
+template <int W>
+__device__ __forceinline__ Kmer<W> suffix_key(const Kmer<W> &x)
+{
+    Kmer<W> r;
+    if constexpr (W == 1) {
+        r.w[0] = x.w[0] >> 2;
+    } else {
+        r.w[0] = (x.w[0] >> 2) | (x.w[1] << 62);
+        r.w[1] = x.w[1] >> 2;
+    }
+    return r;
+}
+
+template <int W>
+__device__ __forceinline__ bool key_eq(const Kmer<W> &a, const Kmer<W> &b)
+{
+    if constexpr (W == 1) return a.w[0] == b.w[0];
+    return a.w[0] == b.w[0] && a.w[1] == b.w[1];
+}
+
+// table position hash and an independent 28-bit tag
+template <int W>
+__device__ __forceinline__ void key_hash(const Kmer<W> &x, uint32_t &h, uint32_t &tag)
+{
+    const uint32_t a = (uint32_t)x.w[0], b = (uint32_t)(x.w[0] >> 32);
+    if constexpr (W == 1) {
+        h = fmix32(a ^ fmix32(b ^ 0x9E3779B9u));
+        tag = fmix32(b ^ fmix32(a ^ 0x7FEB352Du)) & TAG_MASK;
+    } else {
+        const uint32_t c = (uint32_t)x.w[1], d = (uint32_t)(x.w[1] >> 32);
+        h = fmix32(a ^ fmix32(b ^ fmix32(c ^ fmix32(d ^ 0x9E3779B9u))));
+        tag = fmix32(d ^ fmix32(c ^ fmix32(b ^ fmix32(a ^ 0x7FEB352Du)))) & TAG_MASK;
+    }
+}
+
+// lexicographic key: reverse the order of the 2-bit characters
+__device__ __forceinline__ uint64_t rev_chars64(uint64_t x)
+{
+    uint64_t y = __builtin_bitreverse64(x);
+    return ((y >> 1) & 0x5555555555555555ull) | ((y & 0x5555555555555555ull) << 1);
+}
+
+template <int W>
+__device__ __forceinline__ bool lex_less(const Kmer<W> &a, const Kmer<W> &b)
+{
+    // both values sit in the low 2k bits; reversing all 64*W bits puts
+    // character 0 in the top bits of both, so numeric order is lexicographic
+    if constexpr (W == 1) return rev_chars64(a.w[0]) < rev_chars64(b.w[0]);
+    const uint64_t ah = rev_chars64(a.w[0]), bh = rev_chars64(b.w[0]);
+    if (ah != bh) return ah < bh;
+    return rev_chars64(a.w[1]) < rev_chars64(b.w[1]);
+}
+
+// seeded bijection on 2l-bit l-mer codes (same function as the oracle's
+// orc_lmer_order): xor seed, three rounds of odd multiply + xorshift
+__device__ __forceinline__ uint32_t lmer_order(uint32_t code, const LminParams &p)
+{
+    if (p.table) return p.table[code];
+    uint32_t x = (code ^ p.s0) & p.lmask;
+    x = (x * 0x3779B1u) & p.lmask;
+    x ^= x >> p.h1;
+    x = (x * 0xEBCA6Bu) & p.lmask;
+    x ^= x >> p.h2;
+    x ^= p.s1;
+    x = (x * 0xD4EB2Fu) & p.lmask;
+    x ^= x >> p.h1;
+    return x;
+}
+
+// lmin_buckets.h:67-94: minimum order over the l-mers of the (k-1)-prefix
+// (o1) and of the (k-1)-suffix (o2); the l-mer code keeps its first
+// character most significant.
+template <int W>
+__device__ __forceinline__ void double_lmin(const Kmer<W> &x, const LminParams &p, uint32_t &o1, uint32_t &o2)
+{
+    uint32_t lmer = 0;
+    o1 = NONE;
+    o2 = NONE;
+#pragma unroll
+    for (int i = 0; i < 32 * W; i++) {
+        if ((uint32_t)i < p.k) {
+            const uint32_t c = (uint32_t)(x.w[i / 32] >> (2 * (i % 32))) & 3u;
+            lmer = ((lmer << 2) | c) & p.lmask;
+            if ((uint32_t)i + 1 >= p.l) {
+                const uint32_t o = lmer_order(lmer, p);
+                if ((uint32_t)i + 1 < p.k) o1 = min(o1, o);
+                if ((uint32_t)i >= p.l) o2 = min(o2, o);
+            }
+        }
+    }
+}
+
+// 2-bit code -> 'A','C','G','T' (bytes 0x41, 0x43, 0x47, 0x54)
+__device__ __forceinline__ char base_char(uint32_t c) { return (char)((0x54474341u >> (8 * c)) & 0xFFu); }
+
+__device__ __forceinline__ bool sampled(uint32_t v, uint32_t salt, uint32_t shift)
+{
+    return shift != 0 && (fmix32(v ^ salt) >> (32 - shift)) == 0;
+}
+
+// ---------------------------------------------------------------------------
+// generic kernels
+
+template <typename T>
+__global__ void k_fill(T *__restrict__ a, uint64_t n, T v)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        a[i] = v;
+}
+
+// reference packing with an odd number of 32-bit words -> 64-bit words
+__global__ void k_widen(const uint32_t *__restrict__ in, uint64_t n, uint32_t rw, uint32_t W, uint64_t *__restrict__ out)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        for (uint32_t j = 0; j < W; j++) {
+            uint64_t lo = 2 * j < rw ? in[i * rw + 2 * j] : 0;
+            uint64_t hi = 2 * j + 1 < rw ? in[i * rw + 2 * j + 1] : 0;
+            out[i * W + j] = lo | (hi << 32);
+        }
+    }
+}
+
+// ---- exclusive scan (reduce, scan of block sums, scan) --------------------
+
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = BLOCK * SCAN_ITEMS;
+
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T v)
+{
+    const int lane = __lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        T u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T block_reduce(T v, T *sh)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int w = threadIdx.x / 64;
+    if (__lane_id() == 0) sh[w] = v;
+    __syncthreads();
+    T r = 0;
+    for (int i = 0; i < WAVES; i++) r += sh[i];
+    __syncthreads();
+    return r;
+}
+
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_scan_reduce(const T *__restrict__ in, uint64_t n, T *__restrict__ partial)
+{
+    __shared__ T sh[WAVES];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+    T s = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; j++) {
+        uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+    s = block_reduce(s, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// exclusive scan of one tile per block; `offset` holds per-block exclusive
+// offsets (or is null); the block owning the last element writes the total
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_scan_tile(const T *in, T *out, uint64_t n, const T *__restrict__ offset,
+                                                     T *__restrict__ total)
+{
+    __shared__ T tile[SCAN_TILE];
+    __shared__ T wsum[WAVES];
+    const uint64_t base = (uint64_t)blockIdx.x * SCAN_TILE;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; j++) {
+        uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
+        tile[j * BLOCK + threadIdx.x] = i < n ? in[i] : T(0);
+    }
+    __syncthreads();
+    T loc[SCAN_ITEMS];
+    T run = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; j++) {
+        loc[j] = run;
+        run += tile[threadIdx.x * SCAN_ITEMS + j];
+    }
+    T incl = wave_incl_scan(run);
+    const int w = threadIdx.x / 64;
+    if (__lane_id() == 63) wsum[w] = incl;
+    __syncthreads();
+    T wpre = 0, btot = 0;
+    for (int i = 0; i < WAVES; i++) {
+        if (i < w) wpre += wsum[i];
+        btot += wsum[i];
+    }
+    const T boff = offset ? offset[blockIdx.x] : T(0);
+    const T excl = boff + wpre + incl - run;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; j++) tile[threadIdx.x * SCAN_ITEMS + j] = excl + loc[j];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; j++) {
+        uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
+        if (i < n) out[i] = tile[j * BLOCK + threadIdx.x];
+    }
+    if (total && threadIdx.x == 0 && base + SCAN_TILE >= n) *total = boff + btot;
+}
+
+// ---------------------------------------------------------------------------
+// 1. bucketing
+
+template <int W>
+__global__ __launch_bounds__(BLOCK) void k_lmin_count(const uint64_t *__restrict__ kin, const BTile *__restrict__ tiles,
+                                                      LminParams p, uint64_t top_mask, uint32_t *__restrict__ bkt,
+                                                      uint32_t *__restrict__ hist, uint32_t *__restrict__ xhist)
+{
+    extern __shared__ uint32_t cnt[];  // B buckets + 1 crossing counter
+    const BTile t = tiles[blockIdx.x];
+    for (uint32_t i = threadIdx.x; i <= p.B; i += BLOCK) cnt[i] = 0;
+    __syncthreads();
+    const uint32_t bmask = p.B - 1;
+#pragma unroll
+    for (int j = 0; j < BKT_IPT; j++) {
+        const uint32_t idx = j * BLOCK + threadIdx.x;
+        if (idx < t.count) {
+            const uint64_t g = t.kbegin + idx;
+            const Kmer<W> x = load_kmer<W>(kin, g, top_mask);
+            uint32_t o1, o2;
+            double_lmin<W>(x, p, o1, o2);
+            const uint32_t bp = o1 & bmask, bs = o2 & bmask;
+            bkt[g] = bp | (bs << 16);
+            atomicAdd(&cnt[bp], 1u);
+            if (bp != bs) {
+                atomicAdd(&cnt[bs], 1u);
+                atomicAdd(&cnt[p.B], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < p.B; b += BLOCK) hist[t.hbase + (uint64_t)b * t.ts + t.tlocal] = cnt[b];
+    if (threadIdx.x == 0) xhist[blockIdx.x] = cnt[p.B];
+}
+
+// lanes whose home or borrow placement goes to bucket v
+__device__ __forceinline__ uint64_t match_bucket(uint32_t v, uint32_t nbits, uint64_t V, uint64_t C,
+                                                 const uint64_t *BP, const uint64_t *BS)
+{
+    uint64_t mp = V, ms = C;
+#pragma unroll
+    for (int j = 0; j < 10; j++) {
+        if ((uint32_t)j < nbits) {
+            if ((v >> j) & 1u) {
+                mp &= BP[j];
+                ms &= BS[j];
+            } else {
+                mp &= ~BP[j];
+                ms &= ~BS[j];
+            }
+        }
+    }
+    return mp | ms;
+}
+
+// Stable scatter into the bucket CSR: inside a tile, placements keep the
+// order of their k-mers; per wave the rank among lanes bound for the same
+// bucket comes from log2(B) ballots of the bucket bits.
+template <int W>
+__global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restrict__ kin, const BTile *__restrict__ tiles,
+                                                        uint32_t B, uint32_t nbits, uint64_t top_mask,
+                                                        const uint32_t *__restrict__ bkt,
+                                                        const uint32_t *__restrict__ hscan,
+                                                        const uint32_t *__restrict__ xscan, uint64_t *__restrict__ kout,
+                                                        uint8_t *__restrict__ flags, uint32_t *__restrict__ src,
+                                                        uint32_t *__restrict__ pair_home,
+                                                        uint32_t *__restrict__ pair_borrow)
+{
+    extern __shared__ uint32_t sh[];
+    uint32_t *run = sh;              // B+1 running counts
+    uint32_t *wcnt = sh + (B + 1);   // WAVES x (B+1) per-wave counts
+    const BTile t = tiles[blockIdx.x];
+    for (uint32_t i = threadIdx.x; i < (B + 1) * (WAVES + 1); i += BLOCK) sh[i] = 0;
+    __syncthreads();
+    const int lane = __lane_id(), w = threadIdx.x / 64;
+    const uint64_t lt = (1ull << lane) - 1ull, me = 1ull << lane;
+    const uint32_t xbase = xscan[blockIdx.x];
+    for (int j = 0; j < BKT_IPT; j++) {
+        const uint32_t idx = j * BLOCK + threadIdx.x;
+        const bool valid = idx < t.count;
+        const uint64_t g = t.kbegin + idx;
+        uint32_t bp = 0, bs = 0;
+        if (valid) {
+            const uint32_t c = bkt[g];
+            bp = c & 0xFFFFu;
+            bs = c >> 16;
+        }
+        const bool cross = valid && bp != bs;
+        const uint64_t V = __ballot(valid), C = __ballot(cross);
+        uint64_t BP[10], BS[10];
+#pragma unroll
+        for (int q = 0; q < 10; q++) {
+            BP[q] = (uint32_t)q < nbits ? (uint64_t)__ballot(valid && ((bp >> q) & 1u)) : 0ull;
+            BS[q] = (uint32_t)q < nbits ? (uint64_t)__ballot(cross && ((bs >> q) & 1u)) : 0ull;
+        }
+        uint64_t Mh = 0, Mb = 0;
+        if (valid) {
+            Mh = match_bucket(bp, nbits, V, C, BP, BS);
+            if ((Mh & (0ull - Mh)) == me) wcnt[w * (B + 1) + bp] = (uint32_t)__popcll(Mh);
+        }
+        if (cross) {
+            Mb = match_bucket(bs, nbits, V, C, BP, BS);
+            if ((Mb & (0ull - Mb)) == me) wcnt[w * (B + 1) + bs] = (uint32_t)__popcll(Mb);
+        }
+        if (C && (C & (0ull - C)) == me) wcnt[w * (B + 1) + B] = (uint32_t)__popcll(C);
+        __syncthreads();
+        if (valid) {
+            const Kmer<W> x = load_kmer<W>(kin, g, top_mask);
+            uint32_t ph = run[bp] + (uint32_t)__popcll(Mh & lt);
+            for (int q = 0; q < w; q++) ph += wcnt[q * (B + 1) + bp];
+            const uint32_t sh_slot = hscan[t.hbase + (uint64_t)bp * t.ts + t.tlocal] + ph;
+            store_kmer<W>(kout, sh_slot, x);
+            flags[sh_slot] = cross ? F_LENT : 0;
+            src[sh_slot] = (uint32_t)g;
+            if (cross) {
+                uint32_t pb = run[bs] + (uint32_t)__popcll(Mb & lt);
+                for (int q = 0; q < w; q++) pb += wcnt[q * (B + 1) + bs];
+                const uint32_t sb_slot = hscan[t.hbase + (uint64_t)bs * t.ts + t.tlocal] + pb;
+                store_kmer<W>(kout, sb_slot, x);
+                flags[sb_slot] = F_BORROW;
+                src[sb_slot] = (uint32_t)g;
+                uint32_t px = run[B] + (uint32_t)__popcll(C & lt);
+                for (int q = 0; q < w; q++) px += wcnt[q * (B + 1) + B];
+                pair_home[xbase + px] = sh_slot;
+                pair_borrow[xbase + px] = sb_slot;
+            }
+        }
+        __syncthreads();
+        for (uint32_t v = threadIdx.x; v <= B; v += BLOCK) {
+            uint32_t s = 0;
+#pragma unroll
+            for (int q = 0; q < WAVES; q++) {
+                s += wcnt[q * (B + 1) + v];
+                wcnt[q * (B + 1) + v] = 0;
+            }
+            run[v] += s;
+        }
+        __syncthreads();
+    }
+}
+
+// bucket row starts: boff[set*B + b] = scanned histogram at the set's first tile
+__global__ void k_gather_boff(const uint32_t *__restrict__ hscan, const uint64_t *__restrict__ set_hbase,
+                              const uint32_t *__restrict__ set_ts, uint32_t n_sets, uint32_t B, uint32_t m,
+                              uint32_t *__restrict__ boff)
+{
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t G = n_sets * B;
+    if (g < G) {
+        const uint32_t s = g / B, b = g % B;
+        boff[g] = hscan[set_hbase[s] + (uint64_t)b * set_ts[s]];
+    } else if (g == G) {
+        boff[G] = m;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 2. edge finding
+
+template <int W>
+__global__ __launch_bounds__(BLOCK) void k_edge_insert(const uint64_t *__restrict__ kmer, const uint8_t *__restrict__ flags,
+                                                       const ETile *__restrict__ tiles, const uint32_t *__restrict__ boff,
+                                                       const uint64_t *__restrict__ toff,
+                                                       const uint8_t *__restrict__ tbits, uint64_t *table, uint64_t pm0,
+                                                       uint64_t pm1)
+{
+    const ETile et = tiles[blockIdx.x];
+    const uint32_t base = boff[et.bucket];
+    uint64_t *tb = table + toff[et.bucket];
+    const uint32_t mask = (1u << tbits[et.bucket]) - 1u;
+    for (uint32_t s = et.begin + threadIdx.x; s < et.end; s += BLOCK) {
+        if (flags[s] & F_BORROW) continue;
+        const Kmer<W> key = prefix_key<W>(load_kmer<W>(kmer, s, ~0ull), pm0, pm1);
+        uint32_t h, tag;
+        key_hash<W>(key, h, tag);
+        const uint64_t entry = ((uint64_t)tag << 32) | (uint64_t)(s - base);
+        uint32_t pos = h & mask;
+        for (uint32_t probe = 0; probe <= mask; probe++) {
+            uint64_t cur = tb[pos];
+            if (cur == EMPTY) {
+                cur = atomicCAS((unsigned long long *)&tb[pos], (unsigned long long)EMPTY, (unsigned long long)entry);
+                if (cur == EMPTY) break;
+            }
+            if ((uint32_t)((cur >> 32) & TAG_MASK) == tag) {
+                const Kmer<W> o = prefix_key<W>(load_kmer<W>(kmer, base + (uint32_t)cur, ~0ull), pm0, pm1);
+                if (key_eq<W>(o, key)) {
+                    atomicOr((unsigned long long *)&tb[pos], (unsigned long long)SHARED_BIT);
+                    break;
+                }
+            }
+            pos = (pos + 1) & mask;
+        }
+    }
+}
+
+template <int W>
+__global__ __launch_bounds__(BLOCK) void k_edge_lookup(const uint64_t *__restrict__ kmer, const uint8_t *__restrict__ flags,
+                                                       const ETile *__restrict__ tiles, const uint32_t *__restrict__ boff,
+                                                       const uint64_t *__restrict__ toff,
+                                                       const uint8_t *__restrict__ tbits, uint64_t *table, uint64_t pm0,
+                                                       uint64_t pm1, uint32_t *__restrict__ hit)
+{
+    const ETile et = tiles[blockIdx.x];
+    const uint32_t base = boff[et.bucket];
+    uint64_t *tb = table + toff[et.bucket];
+    const uint32_t mask = (1u << tbits[et.bucket]) - 1u;
+    for (uint32_t s = et.begin + threadIdx.x; s < et.end; s += BLOCK) {
+        uint32_t res = NONE;
+        if (!(flags[s] & F_LENT)) {
+            const Kmer<W> key = suffix_key<W>(load_kmer<W>(kmer, s, ~0ull));
+            uint32_t h, tag;
+            key_hash<W>(key, h, tag);
+            uint32_t pos = h & mask;
+            for (uint32_t probe = 0; probe <= mask; probe++) {
+                const uint64_t cur = tb[pos];
+                if (cur == EMPTY) break;
+                if ((uint32_t)((cur >> 32) & TAG_MASK) == tag) {
+                    const Kmer<W> o = prefix_key<W>(load_kmer<W>(kmer, base + (uint32_t)cur, ~0ull), pm0, pm1);
+                    if (key_eq<W>(o, key)) {
+                        atomicAdd((unsigned long long *)&tb[pos], 1ull << HIT_SHIFT);
+                        res = pos;
+                        break;
+                    }
+                }
+                pos = (pos + 1) & mask;
+            }
+        }
+        hit[s] = res;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_edge_resolve(const ETile *__restrict__ tiles, const uint32_t *__restrict__ boff,
+                                                        const uint64_t *__restrict__ toff, const uint64_t *table,
+                                                        const uint32_t *__restrict__ hit, uint32_t *__restrict__ succ,
+                                                        uint32_t *__restrict__ pred)
+{
+    const ETile et = tiles[blockIdx.x];
+    const uint32_t base = boff[et.bucket];
+    const uint64_t *tb = table + toff[et.bucket];
+    for (uint32_t s = et.begin + threadIdx.x; s < et.end; s += BLOCK) {
+        const uint32_t h = hit[s];
+        uint32_t v = NONE;
+        if (h != NONE) {
+            const uint64_t e = tb[h];
+            if (!(e & SHARED_BIT) && ((e >> HIT_SHIFT) & 7u) == 1u) {
+                v = base + (uint32_t)e;
+                pred[v] = s;
+            }
+        }
+        succ[s] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 3. contraction
+
+// the borrowed duplicate's out-edge moves to the home copy
+__global__ void k_splice(const uint32_t *__restrict__ pair_home, const uint32_t *__restrict__ pair_borrow, uint32_t x,
+                         uint32_t *__restrict__ succ, uint32_t *__restrict__ pred)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= x) return;
+    const uint32_t h = pair_home[i], b = pair_borrow[i];
+    const uint32_t t = succ[b];
+    succ[h] = t;
+    if (t != NONE) pred[t] = h;
+}
+
+// every splitter (path head, or a sampled vertex) walks its segment
+__global__ __launch_bounds__(BLOCK) void k_walk(const uint32_t *__restrict__ succ, const uint32_t *__restrict__ pred,
+                                                const uint8_t *__restrict__ flags, uint32_t m, uint32_t salt,
+                                                uint32_t shift, uint32_t *__restrict__ seg,
+                                                uint32_t *__restrict__ rank, uint32_t *__restrict__ seglen,
+                                                uint32_t *__restrict__ n_split, uint32_t *__restrict__ err)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    bool split = false;
+    if (s < m && !(flags[s] & F_BORROW)) split = pred[s] == NONE || sampled(s, salt, shift);
+    const uint64_t bal = __ballot(split);
+    if (__lane_id() == 0 && bal) atomicAdd(n_split, (uint32_t)__popcll(bal));
+    if (!split) return;
+    uint32_t v = s, r = 0;
+    do {
+        seg[v] = s;
+        rank[v] = r++;
+        v = succ[v];
+        // in-degree <= 1 bounds every walk by m; the cap only guards the GPU
+        // against a corrupted successor array
+        if (r > m) {
+            atomicOr(err, 1u);
+            break;
+        }
+    } while (v != NONE && !sampled(v, salt, shift));
+    seglen[s] = r;
+}
+
+// sampled splitters that are not heads, with their preceding segment
+__global__ void k_chain_init(const uint32_t *__restrict__ pred, const uint8_t *__restrict__ flags,
+                             const uint32_t *__restrict__ seg, const uint32_t *__restrict__ seglen, uint32_t m,
+                             uint32_t salt, uint32_t shift, uint64_t *__restrict__ chain,
+                             uint32_t *__restrict__ list, uint32_t *__restrict__ n_list)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= m || (flags[s] & F_BORROW)) return;
+    const uint32_t p = pred[s];
+    if (p == NONE || !sampled(s, salt, shift)) return;
+    const uint32_t a = seg[p];
+    chain[s] = (uint64_t)a | ((uint64_t)seglen[a] << 32);
+    list[atomicAdd(n_list, 1u)] = s;
+}
+
+// one synchronous pointer-jumping round: (ancestor, distance) pairs
+__global__ void k_chain_round(const uint32_t *__restrict__ list, uint32_t n, const uint32_t *__restrict__ pred,
+                              const uint64_t *__restrict__ cin, uint64_t *__restrict__ cout,
+                              uint32_t *__restrict__ active)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool act = false;
+    if (i < n) {
+        const uint32_t r = list[i];
+        const uint64_t c = cin[r];
+        const uint32_t a = (uint32_t)c;
+        if (pred[a] == NONE) {
+            cout[r] = c;
+        } else {
+            const uint64_t c2 = cin[a];
+            const uint32_t a2 = (uint32_t)c2;
+            cout[r] = (uint64_t)a2 | ((uint64_t)((uint32_t)(c >> 32) + (uint32_t)(c2 >> 32)) << 32);
+            act = pred[a2] != NONE;
+        }
+    }
+    const uint64_t bal = __ballot(act);
+    if (__lane_id() == 0 && bal) atomicAdd(active, (uint32_t)__popcll(bal));
+}
+
+// (unitig head, position) of every live k-mer, written over (seg, rank);
+// tails record the unitig length at the head; unresolved k-mers are on cycles
+__global__ __launch_bounds__(BLOCK) void k_final(const uint32_t *__restrict__ succ, const uint32_t *__restrict__ pred,
+                                                 uint8_t *__restrict__ flags, const uint64_t *__restrict__ chain,
+                                                 uint32_t m, uint32_t *__restrict__ seg, uint32_t *__restrict__ rank,
+                                                 uint32_t *__restrict__ ulen, uint32_t *__restrict__ n_cyc)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    bool cyc = false;
+    if (v < m && !(flags[v] & F_BORROW)) {
+        const uint32_t s = seg[v];
+        uint32_t root = NONE, pos = 0;
+        if (s == NONE) {
+            cyc = true;
+        } else if (pred[s] == NONE) {
+            root = s;
+            pos = rank[v];
+        } else {
+            const uint64_t c = chain[s];
+            const uint32_t a = (uint32_t)c;
+            if (pred[a] != NONE) {
+                cyc = true;
+            } else {
+                root = a;
+                pos = (uint32_t)(c >> 32) + rank[v];
+            }
+        }
+        if (cyc) {
+            flags[v] |= F_CYCLE;
+        } else {
+            seg[v] = root;
+            rank[v] = pos;
+            if (succ[v] == NONE) ulen[root] = pos + 1;
+        }
+    }
+    const uint64_t bal = __ballot(cyc);
+    if (__lane_id() == 0 && bal) atomicAdd(n_cyc, (uint32_t)__popcll(bal));
+}
+
+__global__ void k_cycle_init(const uint32_t *__restrict__ succ, const uint8_t *__restrict__ flags, uint32_t m,
+                             uint64_t *__restrict__ lab, uint32_t *__restrict__ list, uint32_t *__restrict__ n_list)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= m || !(flags[v] & F_CYCLE)) return;
+    lab[v] = (uint64_t)v | ((uint64_t)succ[v] << 32);
+    list[atomicAdd(n_list, 1u)] = v;
+}
+
+// min-label pointer jumping around each cycle: (smallest k-mer so far, jump)
+template <int W>
+__global__ void k_cycle_round(const uint32_t *__restrict__ list, uint32_t n, const uint64_t *__restrict__ kmer,
+                              const uint64_t *__restrict__ lin, uint64_t *__restrict__ lout)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t v = list[i];
+    const uint64_t c = lin[v];
+    const uint32_t j = (uint32_t)(c >> 32), l1 = (uint32_t)c;
+    const uint64_t c2 = lin[j];
+    const uint32_t l2 = (uint32_t)c2, j2 = (uint32_t)(c2 >> 32);
+    uint32_t best = l1;
+    if (l2 != l1 && lex_less<W>(load_kmer<W>(kmer, l2, ~0ull), load_kmer<W>(kmer, l1, ~0ull))) best = l2;
+    lout[v] = (uint64_t)best | ((uint64_t)j2 << 32);
+}
+
+__global__ void k_cycle_cut(const uint32_t *__restrict__ list, uint32_t n, const uint64_t *__restrict__ lab,
+                            uint32_t *__restrict__ succ, uint32_t *__restrict__ pred, uint8_t *__restrict__ flags,
+                            uint32_t *__restrict__ n_cut)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t v = list[i];
+    flags[v] &= (uint8_t)~F_CYCLE;
+    if ((uint32_t)lab[v] == v) {
+        const uint32_t p = pred[v];
+        succ[p] = NONE;
+        pred[v] = NONE;
+        atomicAdd(n_cut, 1u);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 4. emission
+
+// (heads << 32 | k-mers) of each unitig head, 0 elsewhere
+__global__ void k_unitig_vals(const uint32_t *__restrict__ pred, const uint8_t *__restrict__ flags,
+                              const uint32_t *__restrict__ ulen, uint32_t m, uint64_t *__restrict__ val)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= m) return;
+    const bool head = !(flags[s] & F_BORROW) && pred[s] == NONE;
+    val[s] = head ? ((1ull << 32) | (uint64_t)ulen[s]) : 0ull;
+}
+
+__global__ void k_order(const uint8_t *__restrict__ flags, const uint32_t *__restrict__ root,
+                        const uint32_t *__restrict__ pos, const uint32_t *__restrict__ pred,
+                        const uint64_t *__restrict__ uscan, uint32_t m, uint32_t *__restrict__ order,
+                        uint32_t *__restrict__ uroot)
+{
+    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= m || (flags[v] & F_BORROW)) return;
+    const uint32_t r = root[v];
+    const uint64_t us = uscan[r];
+    order[(uint32_t)us + pos[v]] = v;
+    if (pred[v] == NONE) uroot[(uint32_t)(us >> 32)] = v;
+}
+
+template <int W>
+__global__ __launch_bounds__(BLOCK) void k_emit(const uint64_t *__restrict__ kmer, const uint32_t *__restrict__ order,
+                                                const uint32_t *__restrict__ uroot, const uint64_t *__restrict__ uscan,
+                                                const uint32_t *__restrict__ ulen, uint32_t n_unitigs, uint32_t k,
+                                                char *__restrict__ text)
+{
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n_unitigs) return;
+    const uint32_t s = uroot[u];
+    const uint32_t ks = (uint32_t)uscan[s], len = ulen[s];
+    char *o = text + (uint64_t)ks + (uint64_t)u * k;
+    const Kmer<W> x0 = load_kmer<W>(kmer, order[ks], ~0ull);
+    for (uint32_t i = 0; i < k; i++) o[i] = base_char((uint32_t)(x0.w[i / 32] >> (2 * (i % 32))) & 3u);
+    const uint32_t last = k - 1;
+    for (uint32_t p = 1; p < len; p++) {
+        const Kmer<W> x = load_kmer<W>(kmer, order[ks + p], ~0ull);
+        o[last + p] = base_char((uint32_t)(x.w[last / 32] >> (2 * (last % 32))) & 3u);
+    }
+    o[k - 1 + len] = '\n';
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+static uint32_t grid_for(uint64_t n, uint32_t block = BLOCK)
+{
+    uint64_t g = (n + block - 1) / block;
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, 0x7FFFFFFFull));
+}
+
+struct DBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+}  // namespace uc
+
+using namespace uc;
+
+struct uc_ctx {
+    uc_opts opts;
+    hipStream_t stream = nullptr;
+    std::string err;
+    size_t held = 0;
+
+    // job
+    uint32_t k = 0, l = 0, W = 1, B = 64, n_sets = 0;
+    uint64_t n = 0, m = 0, x = 0, largest = 0;
+    uint64_t top_mask = ~0ull, pm0 = 0, pm1 = 0;
+    std::vector<uint64_t> set_off;   // n_sets + 1
+    std::vector<uint32_t> boff_h;    // n_sets*B + 1
+    bool loaded = false, bucketed = false, edged = false, coarsened = false, emitted = false;
+    bool edges_ready = false;
+    std::vector<uint64_t> et_toff, et_first;
+    std::vector<uint8_t> et_bits, bucket_done;
+    uint64_t table_slots = 0, n_done = 0;
+    uint64_t n_unitigs = 0, n_cycles = 0, text_bytes = 0, n_splitters = 0;
+    uint32_t chain_rounds = 0;
+    std::vector<uint64_t> set_text;  // n_sets + 1
+    uc_timing timing{};
+
+    // device buffers
+    const uint64_t *kin = nullptr;  // k-mers in 64-bit layout (the caller's buffer or `in`)
+    DBuf otable, in, stage, bkt, tiles, hist, xhist, scan_tmp, kmer, flags, src, pair_h, pair_b, boff, sethb, setts, etiles,
+        toff, tbits, table, hit, succ, pred, seg, rank, seglen, chainA, chainB, list, ulen, uscan, order, uroot, text,
+        small;
+    hipEvent_t ev[6] = {};
+
+    // per-kernel timing (UC_FLAG_PROFILE)
+    struct KStat {
+        std::string name;
+        double ms = 0;
+        uint64_t launches = 0;
+    };
+    std::vector<KStat> kstats;
+    std::vector<std::pair<size_t, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::vector<hipEvent_t> evpool;
+};
+
+namespace {
+
+int fail(uc_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                         \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return fail(ctx, UC_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+#define CHK(expr)                    \
+    do {                             \
+        int rc_ = (expr);            \
+        if (rc_ != UC_OK) return rc_; \
+    } while (0)
+
+int ensure(uc_ctx *c, DBuf &b, size_t bytes)
+{
+    if (bytes == 0) bytes = 8;
+    if (b.cap >= bytes) return UC_OK;
+    if (b.p) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipFree(b.p));
+        c->held -= b.cap;
+        b.p = nullptr;
+        b.cap = 0;
+    }
+    if (c->opts.hbm_budget && c->held + bytes > c->opts.hbm_budget)
+        return fail(c, UC_E_NOMEM, "device budget exceeded: need " + std::to_string(c->held + bytes) + " bytes");
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) return fail(c, UC_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    b.cap = bytes;
+    c->held += bytes;
+    return UC_OK;
+}
+
+template <typename T>
+T *P(DBuf &b)
+{
+    return reinterpret_cast<T *>(b.p);
+}
+
+// Times one kernel launch with a pair of events on the context's stream
+// when UC_FLAG_PROFILE is set; a no-op otherwise.
+struct Prof {
+    uc_ctx *c;
+    size_t id = 0;
+    hipEvent_t a = nullptr, b = nullptr;
+    Prof(uc_ctx *ctx, const char *name) : c(ctx)
+    {
+        if (!(c->opts.flags & UC_FLAG_PROFILE)) {
+            c = nullptr;
+            return;
+        }
+        for (id = 0; id < c->kstats.size() && c->kstats[id].name != name; id++) {
+        }
+        if (id == c->kstats.size()) c->kstats.push_back({name, 0.0, 0});
+        a = take();
+        b = take();
+        if (!a || !b) {
+            c = nullptr;
+            return;
+        }
+        (void)hipEventRecord(a, c->stream);
+    }
+    hipEvent_t take()
+    {
+        hipEvent_t e = nullptr;
+        if (!c->evpool.empty()) {
+            e = c->evpool.back();
+            c->evpool.pop_back();
+        } else if (hipEventCreate(&e) != hipSuccess) {
+            e = nullptr;
+        }
+        return e;
+    }
+    ~Prof()
+    {
+        if (!c) return;
+        (void)hipEventRecord(b, c->stream);
+        c->pending.push_back({id, {a, b}});
+    }
+};
+
+// folds finished launch timings into kstats
+int prof_flush(uc_ctx *c)
+{
+    if (c->pending.empty()) return UC_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (auto &p : c->pending) {
+        float ms = 0;
+        HIPCHK(c, hipEventElapsedTime(&ms, p.second.first, p.second.second));
+        c->kstats[p.first].ms += ms;
+        c->kstats[p.first].launches++;
+        c->evpool.push_back(p.second.first);
+        c->evpool.push_back(p.second.second);
+    }
+    c->pending.clear();
+    return UC_OK;
+}
+
+#define KLAUNCH(c, name, kernel, cfg, ...)                                  \
+    do {                                                                    \
+        Prof pf_(c, name);                                                  \
+        hipLaunchKernelGGL(kernel, UC_CFG cfg, (c)->stream, __VA_ARGS__);   \
+    } while (0);                                                            \
+    HIPCHK(c, hipGetLastError())
+#define UC_CFG(g, b, s) dim3(g), dim3(b), (s)
+
+template <typename T>
+int fill(uc_ctx *c, T *p, uint64_t n, T v)
+{
+    if (n == 0) return UC_OK;
+    KLAUNCH(c, "k_fill", k_fill<T>, (std::min<uint32_t>(grid_for(n), 65536), BLOCK, 0), p, n, v);
+    return UC_OK;
+}
+
+// exclusive scan of n values (in == out allowed); *d_total gets the sum.
+// Block sums of every recursion level live in `tmp`.
+template <typename T>
+size_t scan_scratch(uint64_t n)
+{
+    size_t bytes = 0;
+    for (uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE; nb > 1; nb = (nb + SCAN_TILE - 1) / SCAN_TILE)
+        bytes += ((nb + 1) * sizeof(T) + 255) / 256 * 256;
+    return bytes;
+}
+
+template <typename T>
+int scan_level(uc_ctx *c, const T *in, T *out, uint64_t n, T *d_total, char *tmp)
+{
+    const uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (nb <= 1) {
+        KLAUNCH(c, "k_scan_tile", k_scan_tile<T>, (1, BLOCK, 0), in, out, n, nullptr, d_total);
+        return UC_OK;
+    }
+    T *partial = reinterpret_cast<T *>(tmp);
+    char *next = tmp + ((nb + 1) * sizeof(T) + 255) / 256 * 256;
+    KLAUNCH(c, "k_scan_reduce", k_scan_reduce<T>, ((uint32_t)nb, BLOCK, 0), in, n, partial);
+    CHK(scan_level<T>(c, partial, partial, nb, nullptr, next));
+    KLAUNCH(c, "k_scan_tile", k_scan_tile<T>, ((uint32_t)nb, BLOCK, 0), in, out, n, partial, d_total);
+    return UC_OK;
+}
+
+template <typename T>
+int scan_excl(uc_ctx *c, const T *in, T *out, uint64_t n, T *d_total, DBuf &tmp)
+{
+    if (n == 0) {
+        if (d_total) CHK(fill<T>(c, d_total, 1, T(0)));
+        return UC_OK;
+    }
+    CHK(ensure(c, tmp, scan_scratch<T>(n) + 256));
+    return scan_level<T>(c, in, out, n, d_total, reinterpret_cast<char *>(tmp.p));
+}
+
+int select_dev(uc_ctx *c)
+{
+    HIPCHK(c, hipSetDevice(c->opts.device));
+    return UC_OK;
+}
+
+int d2h(uc_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0) return UC_OK;
+    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return UC_OK;
+}
+
+int h2d(uc_ctx *c, void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0) return UC_OK;
+    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    return UC_OK;
+}
+
+void reset_job(uc_ctx *c)
+{
+    c->loaded = c->bucketed = c->edged = c->coarsened = c->emitted = false;
+    c->edges_ready = false;
+    c->n_unitigs = c->n_cycles = c->text_bytes = c->n_splitters = 0;
+    c->chain_rounds = 0;
+    c->m = c->x = c->largest = 0;
+}
+
+// ---- load -----------------------------------------------------------------
+
+int do_load(uc_ctx *c, const uint32_t *packed, const uint64_t *set_offsets, uint32_t n_sets, uint32_t k, int on_device)
+{
+    reset_job(c);
+    if (k < 2 || k > 64) return fail(c, UC_E_ARG, "k must be in [2, 64]");
+    if (n_sets == 0) return fail(c, UC_E_ARG, "n_sets must be >= 1");
+    c->k = k;
+    c->W = k <= 32 ? 1 : 2;
+    c->n_sets = n_sets;
+    c->set_off.assign(set_offsets, set_offsets + n_sets + 1);
+    for (uint32_t s = 0; s < n_sets; s++)
+        if (c->set_off[s + 1] < c->set_off[s]) return fail(c, UC_E_ARG, "set offsets must be non-decreasing");
+    if (c->set_off[0] != 0) return fail(c, UC_E_ARG, "set offsets must start at 0");
+    c->n = c->set_off[n_sets];
+    if (c->n >= 0xFFFFFFFFull) return fail(c, UC_E_LIMIT, "more than 2^32-2 k-mers");
+    const uint32_t rw = (k + 15) / 16;
+    const uint32_t W = c->W;
+    const uint32_t top_bits = 2 * k - 64 * (W - 1);
+    c->top_mask = top_bits >= 64 ? ~0ull : ((1ull << top_bits) - 1);
+    const uint32_t pbits = 2 * (k - 1);
+    if (pbits >= 64) {
+        c->pm0 = ~0ull;
+        c->pm1 = pbits - 64 >= 64 ? ~0ull : ((1ull << (pbits - 64)) - 1);
+    } else {
+        c->pm0 = (1ull << pbits) - 1;
+        c->pm1 = 0;
+    }
+    const size_t in_bytes = (size_t)c->n * rw * 4;
+    if (rw == 2 * W) {
+        // the reference packing with an even word count is the 64-bit layout
+        if (on_device) {
+            c->kin = reinterpret_cast<const uint64_t *>(packed);
+        } else {
+            CHK(ensure(c, c->in, in_bytes));
+            CHK(h2d(c, c->in.p, packed, in_bytes));
+            c->kin = P<uint64_t>(c->in);
+        }
+    } else {
+        const uint32_t *src = packed;
+        if (!on_device) {
+            CHK(ensure(c, c->stage, in_bytes));
+            CHK(h2d(c, c->stage.p, packed, in_bytes));
+            src = P<uint32_t>(c->stage);
+        }
+        CHK(ensure(c, c->in, (size_t)c->n * W * 8));
+        if (c->n) {
+            KLAUNCH(c, "k_widen", k_widen, (std::min<uint32_t>(grid_for(c->n), 65536), BLOCK, 0), src, c->n, rw, W,
+                                                                                        P<uint64_t>(c->in));
+        }
+        c->kin = P<uint64_t>(c->in);
+    }
+    c->loaded = true;
+    return UC_OK;
+}
+
+// ---- bucketing ---------------------------------------------------------------
+
+template <int W>
+int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
+{
+    const uint32_t B = c->B, S = c->n_sets;
+    // tiles: every set gets at least one (possibly empty) tile so each bucket
+    // row has a start in the histogram
+    std::vector<BTile> tiles;
+    std::vector<uint64_t> set_hbase(S);
+    std::vector<uint32_t> set_ts(S);
+    uint64_t hb = 0;
+    for (uint32_t s = 0; s < S; s++) {
+        const uint64_t ns = c->set_off[s + 1] - c->set_off[s];
+        const uint32_t ts = (uint32_t)std::max<uint64_t>(1, (ns + BKT_TILE - 1) / BKT_TILE);
+        set_hbase[s] = hb;
+        set_ts[s] = ts;
+        for (uint32_t t = 0; t < ts; t++) {
+            BTile bt;
+            bt.kbegin = c->set_off[s] + (uint64_t)t * BKT_TILE;
+            bt.hbase = hb;
+            bt.count = (uint32_t)std::min<uint64_t>(BKT_TILE, ns - std::min<uint64_t>(ns, (uint64_t)t * BKT_TILE));
+            bt.set = s;
+            bt.ts = ts;
+            bt.tlocal = t;
+            tiles.push_back(bt);
+        }
+        hb += (uint64_t)B * ts;
+    }
+    const uint64_t T = tiles.size();
+    CHK(ensure(c, c->tiles, T * sizeof(BTile)));
+    CHK(h2d(c, c->tiles.p, tiles.data(), T * sizeof(BTile)));
+    CHK(ensure(c, c->bkt, c->n * 4));
+    CHK(ensure(c, c->hist, (hb + 1) * 4));
+    CHK(ensure(c, c->xhist, (T + 1) * 4));
+    CHK(ensure(c, c->sethb, S * 8));
+    CHK(ensure(c, c->setts, S * 4));
+    CHK(h2d(c, c->sethb.p, set_hbase.data(), S * 8));
+    CHK(h2d(c, c->setts.p, set_ts.data(), S * 4));
+
+    LminParams p;
+    p.k = c->k;
+    p.l = l;
+    p.B = B;
+    const uint32_t bits = 2 * l;
+    p.lmask = bits >= 32 ? 0xFFFFFFFFu : ((1u << bits) - 1u);
+    p.h1 = (bits + 1) / 2;
+    p.h2 = bits / 3 + 1;
+    p.s0 = (uint32_t)c->opts.seed & p.lmask;
+    p.s1 = (uint32_t)(c->opts.seed >> 32) & p.lmask;
+    p.table = table;
+
+    const uint64_t *kin = c->kin;
+    KLAUNCH(c, "k_lmin_count", k_lmin_count<W>, ((uint32_t)T, BLOCK, (B + 1) * 4), kin, P<BTile>(c->tiles), p, c->top_mask,
+                                                                     P<uint32_t>(c->bkt), P<uint32_t>(c->hist),
+                                                                     P<uint32_t>(c->xhist));
+    // totals at hist[hb] and xhist[T]
+    CHK(scan_excl<uint32_t>(c, P<uint32_t>(c->hist), P<uint32_t>(c->hist), hb, P<uint32_t>(c->hist) + hb, c->scan_tmp));
+    CHK(scan_excl<uint32_t>(c, P<uint32_t>(c->xhist), P<uint32_t>(c->xhist), T, P<uint32_t>(c->xhist) + T,
+                            c->scan_tmp));
+    uint32_t tot[2];
+    CHK(d2h(c, &tot[0], P<uint32_t>(c->hist) + hb, 4));
+    CHK(d2h(c, &tot[1], P<uint32_t>(c->xhist) + T, 4));
+    c->m = tot[0];
+    c->x = tot[1];
+    if (c->m >= 0xFFFFFFFFull) return fail(c, UC_E_LIMIT, "more than 2^32-2 placements");
+    const uint64_t G = (uint64_t)S * B;
+    CHK(ensure(c, c->kmer, std::max<uint64_t>(c->m, 1) * W * 8));
+    CHK(ensure(c, c->flags, c->m + 64));
+    CHK(ensure(c, c->src, c->m * 4));
+    CHK(ensure(c, c->pair_h, c->x * 4));
+    CHK(ensure(c, c->pair_b, c->x * 4));
+    CHK(ensure(c, c->boff, (G + 1) * 4));
+    const uint32_t nbits = (uint32_t)__builtin_ctz(B);
+    const size_t lds = (size_t)(B + 1) * (WAVES + 1) * 4;
+    KLAUNCH(c, "k_lmin_scatter", k_lmin_scatter<W>, ((uint32_t)T, BLOCK, lds), 
+        kin, P<BTile>(c->tiles), B, nbits, c->top_mask, P<uint32_t>(c->bkt), P<uint32_t>(c->hist),
+        P<uint32_t>(c->xhist), P<uint64_t>(c->kmer), P<uint8_t>(c->flags), P<uint32_t>(c->src),
+        P<uint32_t>(c->pair_h), P<uint32_t>(c->pair_b));
+    KLAUNCH(c, "k_gather_boff", k_gather_boff, (grid_for(G + 1), BLOCK, 0), P<uint32_t>(c->hist), P<uint64_t>(c->sethb),
+                                                           P<uint32_t>(c->setts), S, B, (uint32_t)c->m,
+                                                           P<uint32_t>(c->boff));
+    c->boff_h.resize(G + 1);
+    CHK(d2h(c, c->boff_h.data(), c->boff.p, (G + 1) * 4));
+    c->largest = 0;
+    for (uint64_t g = 0; g < G; g++) c->largest = std::max<uint64_t>(c->largest, c->boff_h[g + 1] - c->boff_h[g]);
+    c->l = l;
+    c->bucketed = true;
+    return UC_OK;
+}
+
+int do_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
+{
+    if (!c->loaded) return fail(c, UC_E_STATE, "uc_bucketize before uc_load");
+    if (l < 1 || l > 15 || l >= c->k) return fail(c, UC_E_ARG, "l must be in [1, min(15, k-1)]");
+    c->edged = c->coarsened = c->emitted = false;
+    c->edges_ready = false;
+    return c->W == 1 ? run_bucketize<1>(c, l, table) : run_bucketize<2>(c, l, table);
+}
+
+// ---- edges -----------------------------------------------------------------
+
+uint32_t table_bits(uint64_t count, uint32_t load_pct)
+{
+    uint64_t need = (count * 100 + load_pct - 1) / load_pct;
+    uint32_t b = 6;
+    while ((1ull << b) < need) b++;
+    return b;
+}
+
+template <int W>
+int launch_edges(uc_ctx *c, size_t first, size_t count)
+{
+    if (count == 0) return UC_OK;
+    const ETile *tiles = P<ETile>(c->etiles) + first;
+    KLAUNCH(c, "k_edge_insert", k_edge_insert<W>, ((uint32_t)count, BLOCK, 0), P<uint64_t>(c->kmer), P<uint8_t>(c->flags), tiles,
+                                                                P<uint32_t>(c->boff), P<uint64_t>(c->toff),
+                                                                P<uint8_t>(c->tbits), P<uint64_t>(c->table), c->pm0,
+                                                                c->pm1);
+    KLAUNCH(c, "k_edge_lookup", k_edge_lookup<W>, ((uint32_t)count, BLOCK, 0), P<uint64_t>(c->kmer), P<uint8_t>(c->flags), tiles,
+                                                                P<uint32_t>(c->boff), P<uint64_t>(c->toff),
+                                                                P<uint8_t>(c->tbits), P<uint64_t>(c->table), c->pm0,
+                                                                c->pm1, P<uint32_t>(c->hit));
+    KLAUNCH(c, "k_edge_resolve", k_edge_resolve, ((uint32_t)count, BLOCK, 0), tiles, P<uint32_t>(c->boff), P<uint64_t>(c->toff),
+                                                             P<uint64_t>(c->table), P<uint32_t>(c->hit),
+                                                             P<uint32_t>(c->succ), P<uint32_t>(c->pred));
+    return UC_OK;
+}
+
+// table layout and bucket tiles, once per bucketing result
+int prepare_edges(uc_ctx *c)
+{
+    if (c->edges_ready) return UC_OK;
+    const uint64_t G = (uint64_t)c->n_sets * c->B;
+    const uint32_t load = c->opts.load_pct ? c->opts.load_pct : 50;
+    const bool plain = c->opts.mode == UC_MODE_PLAIN;
+    // batched: one table per bucket; plain: one shared table sized for the
+    // largest bucket (assembler.h:113-118, driver.cpp:465)
+    c->et_toff.assign(G + 1, 0);
+    c->et_bits.assign(G, 0);
+    uint64_t total = 0;
+    const uint32_t shared_bits = table_bits(std::max<uint64_t>(c->largest, 1), load);
+    if (shared_bits > 31) return fail(c, UC_E_LIMIT, "bucket too large for a 2^31-slot table");
+    for (uint64_t g = 0; g < G; g++) {
+        const uint64_t cnt = c->boff_h[g + 1] - c->boff_h[g];
+        c->et_bits[g] = plain ? (uint8_t)shared_bits : (uint8_t)table_bits(std::max<uint64_t>(cnt, 1), load);
+        c->et_toff[g] = plain ? 0 : total;
+        if (!plain) total += 1ull << c->et_bits[g];
+    }
+    if (plain) total = 1ull << shared_bits;
+    c->table_slots = total;
+    // tiles of at most EDGE_TILE slots, bucket by bucket
+    std::vector<ETile> et;
+    c->et_first.assign(G + 1, 0);
+    for (uint64_t g = 0; g < G; g++) {
+        c->et_first[g] = et.size();
+        for (uint64_t b = c->boff_h[g]; b < c->boff_h[g + 1]; b += EDGE_TILE) {
+            ETile t;
+            t.bucket = (uint32_t)g;
+            t.begin = (uint32_t)b;
+            t.end = (uint32_t)std::min<uint64_t>(c->boff_h[g + 1], b + EDGE_TILE);
+            t.pad = 0;
+            et.push_back(t);
+        }
+    }
+    c->et_first[G] = et.size();
+    CHK(ensure(c, c->etiles, et.size() * sizeof(ETile)));
+    CHK(h2d(c, c->etiles.p, et.data(), et.size() * sizeof(ETile)));
+    CHK(ensure(c, c->toff, (G + 1) * 8));
+    CHK(ensure(c, c->tbits, G + 8));
+    CHK(h2d(c, c->toff.p, c->et_toff.data(), (G + 1) * 8));
+    CHK(h2d(c, c->tbits.p, c->et_bits.data(), G));
+    CHK(ensure(c, c->table, total * 8));
+    CHK(ensure(c, c->hit, c->m * 4));
+    CHK(ensure(c, c->succ, c->m * 4));
+    CHK(ensure(c, c->pred, c->m * 4));
+    c->bucket_done.assign(G, 0);
+    c->n_done = 0;
+    c->edges_ready = true;
+    return UC_OK;
+}
+
+template <int W>
+int run_edges(uc_ctx *c, uint32_t bucket)
+{
+    CHK(prepare_edges(c));
+    const uint64_t G = (uint64_t)c->n_sets * c->B;
+    const bool plain = c->opts.mode == UC_MODE_PLAIN;
+    if (bucket == UC_ALL_BUCKETS) {
+        CHK(fill<uint32_t>(c, P<uint32_t>(c->pred), c->m, NONE));
+        if (!plain) {
+            CHK(fill<uint64_t>(c, P<uint64_t>(c->table), c->table_slots, EMPTY));
+            CHK(launch_edges<W>(c, c->et_first[0], c->et_first[G] - c->et_first[0]));
+        } else {
+            for (uint64_t g = 0; g < G; g++) {
+                CHK(fill<uint64_t>(c, P<uint64_t>(c->table), 1ull << c->et_bits[g], EMPTY));
+                CHK(launch_edges<W>(c, c->et_first[g], c->et_first[g + 1] - c->et_first[g]));
+            }
+        }
+        std::fill(c->bucket_done.begin(), c->bucket_done.end(), 1);
+        c->n_done = G;
+    } else {
+        if (bucket >= G) return fail(c, UC_E_ARG, "bucket id out of range");
+        // predecessors of this bucket's slots are written by this bucket only
+        const uint32_t b0 = c->boff_h[bucket], b1 = c->boff_h[bucket + 1];
+        CHK(fill<uint32_t>(c, P<uint32_t>(c->pred) + b0, b1 - b0, NONE));
+        CHK(fill<uint64_t>(c, P<uint64_t>(c->table) + c->et_toff[bucket], 1ull << c->et_bits[bucket], EMPTY));
+        CHK(launch_edges<W>(c, c->et_first[bucket], c->et_first[bucket + 1] - c->et_first[bucket]));
+        if (!c->bucket_done[bucket]) {
+            c->bucket_done[bucket] = 1;
+            c->n_done++;
+        }
+    }
+    c->edged = c->n_done == G;
+    return UC_OK;
+}
+
+int do_edges(uc_ctx *c, uint32_t bucket)
+{
+    if (!c->bucketed) return fail(c, UC_E_STATE, "uc_find_edges before uc_bucketize");
+    c->coarsened = c->emitted = false;
+    return c->W == 1 ? run_edges<1>(c, bucket) : run_edges<2>(c, bucket);
+}
+
+// ---- contraction ----------------------------------------------------------
+
+int rank_paths(uc_ctx *c, uint32_t salt, uint32_t shift, uint32_t *n_cyc_out)
+{
+    const uint32_t m = (uint32_t)c->m;
+    uint32_t *cnt = P<uint32_t>(c->small);  // [0] splitters [1] list [2] active [3] cycles
+    CHK(fill<uint32_t>(c, cnt, 8, 0u));
+    CHK(fill<uint32_t>(c, P<uint32_t>(c->seg), m, NONE));
+    KLAUNCH(c, "k_walk", k_walk, (grid_for(m), BLOCK, 0), P<uint32_t>(c->succ), P<uint32_t>(c->pred), P<uint8_t>(c->flags), m,
+                                                 salt, shift, P<uint32_t>(c->seg), P<uint32_t>(c->rank),
+                                                 P<uint32_t>(c->seglen), cnt + 0, cnt + 6);
+    KLAUNCH(c, "k_chain_init", k_chain_init, (grid_for(m), BLOCK, 0), P<uint32_t>(c->pred), P<uint8_t>(c->flags),
+                                                       P<uint32_t>(c->seg), P<uint32_t>(c->seglen), m, salt, shift,
+                                                       P<uint64_t>(c->chainA), P<uint32_t>(c->list), cnt + 1);
+    uint32_t h[2];
+    CHK(d2h(c, h, cnt, 8));
+    c->n_splitters = h[0];
+    const uint32_t nl = h[1];
+    uint64_t *ca = P<uint64_t>(c->chainA), *cb = P<uint64_t>(c->chainB);
+    uint32_t rounds = 0;
+    if (nl) {
+        for (;;) {
+            CHK(fill<uint32_t>(c, cnt + 2, 1, 0u));
+            KLAUNCH(c, "k_chain_round", k_chain_round, (grid_for(nl), BLOCK, 0), P<uint32_t>(c->list), nl, P<uint32_t>(c->pred), ca,
+                                                                 cb, cnt + 2);
+            std::swap(ca, cb);
+            rounds++;
+            uint32_t active;
+            CHK(d2h(c, &active, cnt + 2, 4));
+            if (active == 0 || rounds > 40) break;  // 2^40 exceeds any chain; beyond: cycles
+        }
+    }
+    c->chain_rounds += rounds;
+    KLAUNCH(c, "k_final", k_final, (grid_for(m), BLOCK, 0), P<uint32_t>(c->succ), P<uint32_t>(c->pred), P<uint8_t>(c->flags),
+                                                  ca, m, P<uint32_t>(c->seg), P<uint32_t>(c->rank),
+                                                  P<uint32_t>(c->ulen), cnt + 3);
+    uint32_t res[4];
+    CHK(d2h(c, res, cnt + 3, 16));
+    if (res[3]) return fail(c, UC_E_STATE, "successor array corrupted: a segment walk exceeded m steps");
+    *n_cyc_out = res[0];
+    return UC_OK;
+}
+
+template <int W>
+int cut_cycles(uc_ctx *c, uint64_t *n_cut)
+{
+    const uint32_t m = (uint32_t)c->m;
+    uint32_t *cnt = P<uint32_t>(c->small);
+    CHK(fill<uint32_t>(c, cnt + 4, 1, 0u));
+    KLAUNCH(c, "k_cycle_init", k_cycle_init, (grid_for(m), BLOCK, 0), P<uint32_t>(c->succ), P<uint8_t>(c->flags), m,
+                                                       P<uint64_t>(c->chainA), P<uint32_t>(c->list), cnt + 4);
+    uint32_t nl;
+    CHK(d2h(c, &nl, cnt + 4, 4));
+    uint32_t rounds = 1;
+    while ((1ull << (rounds - 1)) < nl) rounds++;
+    uint64_t *ca = P<uint64_t>(c->chainA), *cb = P<uint64_t>(c->chainB);
+    for (uint32_t r = 0; r < rounds; r++) {
+        KLAUNCH(c, "k_cycle_round", k_cycle_round<W>, (grid_for(nl), BLOCK, 0), P<uint32_t>(c->list), nl, P<uint64_t>(c->kmer), ca,
+                                                                  cb);
+        std::swap(ca, cb);
+    }
+    CHK(fill<uint32_t>(c, cnt + 5, 1, 0u));
+    KLAUNCH(c, "k_cycle_cut", k_cycle_cut, (grid_for(nl), BLOCK, 0), P<uint32_t>(c->list), nl, ca, P<uint32_t>(c->succ),
+                                                       P<uint32_t>(c->pred), P<uint8_t>(c->flags), cnt + 5);
+    uint32_t cuts = 0;
+    CHK(d2h(c, &cuts, cnt + 5, 4));
+    *n_cut = cuts;
+    return UC_OK;
+}
+
+int do_coarsen(uc_ctx *c)
+{
+    if (!c->edged) return fail(c, UC_E_STATE, "uc_coarsen before uc_find_edges");
+    c->emitted = false;
+    const uint32_t m = (uint32_t)c->m;
+    CHK(ensure(c, c->small, 256));
+    CHK(ensure(c, c->seg, c->m * 4));
+    CHK(ensure(c, c->rank, c->m * 4));
+    CHK(ensure(c, c->seglen, c->m * 4));
+    CHK(ensure(c, c->chainA, c->m * 8));
+    CHK(ensure(c, c->chainB, c->m * 8));
+    CHK(ensure(c, c->list, c->m * 4));
+    CHK(ensure(c, c->ulen, c->m * 4));
+    c->chain_rounds = 0;
+    c->n_cycles = 0;
+    if (c->x) {
+        KLAUNCH(c, "k_splice", k_splice, (grid_for(c->x), BLOCK, 0), P<uint32_t>(c->pair_h), P<uint32_t>(c->pair_b),
+                                                          (uint32_t)c->x, P<uint32_t>(c->succ), P<uint32_t>(c->pred));
+    }
+    const uint32_t salt = (uint32_t)(c->opts.seed * 0x9E3779B97F4A7C15ull >> 32);
+    const uint32_t shift = c->opts.sample_log2 ? c->opts.sample_log2 : 5;
+    uint32_t n_cyc = 0;
+    uint64_t cut = 0;
+    if (m) CHK(rank_paths(c, salt, shift, &n_cyc));
+    if (n_cyc) {
+        // isolated cycles: cut each at its smallest k-mer, contract again
+        CHK(c->W == 1 ? cut_cycles<1>(c, &cut) : cut_cycles<2>(c, &cut));
+        uint32_t again = 0;
+        CHK(rank_paths(c, salt, shift, &again));
+        if (again) return fail(c, UC_E_STATE, "cycles left after cutting");
+    }
+    // unitig scan: (heads << 32 | k-mers) per head, exclusive over slots
+    CHK(ensure(c, c->uscan, (c->m + 1) * 8));
+    if (m) {
+        KLAUNCH(c, "k_unitig_vals", k_unitig_vals, (grid_for(m), BLOCK, 0), P<uint32_t>(c->pred), P<uint8_t>(c->flags),
+                                                            P<uint32_t>(c->ulen), m, P<uint64_t>(c->uscan));
+    }
+    CHK(scan_excl<uint64_t>(c, P<uint64_t>(c->uscan), P<uint64_t>(c->uscan), m, P<uint64_t>(c->uscan) + m,
+                            c->scan_tmp));
+    uint64_t tot = 0;
+    CHK(d2h(c, &tot, P<uint64_t>(c->uscan) + m, 8));
+    c->n_unitigs = tot >> 32;
+    const uint64_t live = tot & 0xFFFFFFFFull;
+    if (live != c->n) return fail(c, UC_E_STATE, "contraction lost k-mers: " + std::to_string(live) + " of " +
+                                                     std::to_string(c->n));
+    c->text_bytes = live + c->n_unitigs * c->k;
+    c->n_cycles = cut;
+    c->coarsened = true;
+    return UC_OK;
+}
+
+template <int W>
+int run_emit(uc_ctx *c)
+{
+    const uint32_t m = (uint32_t)c->m;
+    CHK(ensure(c, c->order, c->n * 4));
+    CHK(ensure(c, c->uroot, c->n_unitigs * 4));
+    CHK(ensure(c, c->text, c->text_bytes + 1));
+    if (m) {
+        KLAUNCH(c, "k_order", k_order, (grid_for(m), BLOCK, 0), P<uint8_t>(c->flags), P<uint32_t>(c->seg), P<uint32_t>(c->rank),
+                                                      P<uint32_t>(c->pred), P<uint64_t>(c->uscan), m,
+                                                      P<uint32_t>(c->order), P<uint32_t>(c->uroot));
+    }
+    if (c->n_unitigs) {
+        KLAUNCH(c, "k_emit", k_emit<W>, (grid_for(c->n_unitigs), BLOCK, 0), 
+            P<uint64_t>(c->kmer), P<uint32_t>(c->order), P<uint32_t>(c->uroot), P<uint64_t>(c->uscan),
+            P<uint32_t>(c->ulen), (uint32_t)c->n_unitigs, c->k, P<char>(c->text));
+    }
+    // per-set text ranges: a set's unitigs start at its first bucket row
+    c->set_text.assign(c->n_sets + 1, c->text_bytes);
+    std::vector<uint64_t> us(c->n_sets, 0);
+    for (uint32_t s = 0; s < c->n_sets; s++) {
+        const uint32_t sb = c->boff_h[(uint64_t)s * c->B];
+        if (sb < m) {
+            CHK(d2h(c, &us[s], P<uint64_t>(c->uscan) + sb, 8));
+            c->set_text[s] = (us[s] & 0xFFFFFFFFull) + (us[s] >> 32) * c->k;
+        }
+    }
+    c->emitted = true;
+    return UC_OK;
+}
+
+int do_emit(uc_ctx *c)
+{
+    if (!c->coarsened) return fail(c, UC_E_STATE, "uc_emit before uc_coarsen");
+    return c->W == 1 ? run_emit<1>(c) : run_emit<2>(c);
+}
+
+// bucketing .. emission on the loaded k-mers, one event per phase boundary
+int timed_phases(uc_ctx *c, uint32_t l)
+{
+    for (auto &e : c->ev)
+        if (!e) HIPCHK(c, hipEventCreate(&e));
+    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    CHK(do_bucketize(c, l, nullptr));
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+    CHK(do_edges(c, UC_ALL_BUCKETS));
+    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    CHK(do_coarsen(c));
+    HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+    CHK(do_emit(c));
+    HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev[4]));
+    float t[4];
+    for (int i = 0; i < 4; i++) HIPCHK(c, hipEventElapsedTime(&t[i], c->ev[i], c->ev[i + 1]));
+    c->timing.bucketize_ms = t[0];
+    c->timing.edges_ms = t[1];
+    c->timing.coarsen_ms = t[2];
+    c->timing.emit_ms = t[3];
+    HIPCHK(c, hipEventElapsedTime(&c->timing.total_ms, c->ev[0], c->ev[4]));
+    c->timing.device_bytes = c->held;
+    return prof_flush(c);
+}
+
+int timed_job(uc_ctx *c, const uint32_t *packed, const uint64_t *offs, uint32_t n_sets, uint32_t k, uint32_t l,
+              int on_device)
+{
+    CHK(do_load(c, packed, offs, n_sets, k, on_device));
+    return timed_phases(c, l);
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+
+extern "C" {
+
+void uc_default_opts(uc_opts *o)
+{
+    std::memset(o, 0, sizeof(*o));
+    o->device = 0;
+    o->seed = 0x5EEDull;
+    o->n_buckets = 64;
+    o->mode = UC_MODE_BATCHED;
+    o->load_pct = 50;
+    o->sample_log2 = 5;
+    o->hbm_budget = 0;
+}
+
+const char *uc_strerror(int s)
+{
+    switch (s) {
+    case UC_OK: return "ok";
+    case UC_E_ARG: return "invalid argument";
+    case UC_E_NOMEM: return "out of memory";
+    case UC_E_HIP: return "HIP runtime error";
+    case UC_E_STATE: return "call out of order";
+    case UC_E_IO: return "I/O error";
+    case UC_E_FORMAT: return "malformed k-mer text";
+    case UC_E_LIMIT: return "job exceeds a compiled limit";
+    default: return "unknown status";
+    }
+}
+
+int uc_ctx_create(const uc_opts *opts, uc_ctx **out)
+{
+    if (!out) return UC_E_ARG;
+    *out = nullptr;
+    uc_opts o;
+    if (opts)
+        o = *opts;
+    else
+        uc_default_opts(&o);
+    if (o.n_buckets < 1 || o.n_buckets > MAX_BUCKETS || (o.n_buckets & (o.n_buckets - 1))) return UC_E_ARG;
+    if (o.mode != UC_MODE_BATCHED && o.mode != UC_MODE_PLAIN) return UC_E_ARG;
+    if (o.load_pct && (o.load_pct < 10 || o.load_pct > 90)) return UC_E_ARG;
+    if (o.sample_log2 > 16) return UC_E_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return UC_E_HIP;
+    if (o.device < 0 || o.device >= ndev) return UC_E_ARG;
+    if (hipSetDevice(o.device) != hipSuccess) return UC_E_HIP;
+    uc_ctx *c = new uc_ctx();
+    c->opts = o;
+    c->B = o.n_buckets;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return UC_E_HIP;
+    }
+    *out = c;
+    return UC_OK;
+}
+
+int uc_ctx_destroy(uc_ctx *c)
+{
+    if (!c) return UC_OK;
+    (void)hipSetDevice(c->opts.device);
+    (void)hipStreamSynchronize(c->stream);
+    DBuf *bufs[] = {&c->otable, &c->in, &c->stage, &c->bkt, &c->tiles, &c->hist, &c->xhist, &c->scan_tmp, &c->kmer, &c->flags,
+                    &c->src, &c->pair_h, &c->pair_b, &c->boff, &c->sethb, &c->setts, &c->etiles, &c->toff,
+                    &c->tbits, &c->table, &c->hit, &c->succ, &c->pred, &c->seg, &c->rank, &c->seglen,
+                    &c->chainA, &c->chainB, &c->list, &c->ulen, &c->uscan, &c->order, &c->uroot, &c->text,
+                    &c->small};
+    for (DBuf *b : bufs)
+        if (b->p) (void)hipFree(b->p);
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &p : c->pending) {
+        (void)hipEventDestroy(p.second.first);
+        (void)hipEventDestroy(p.second.second);
+    }
+    for (auto e : c->evpool) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return UC_OK;
+}
+
+const char *uc_last_error(const uc_ctx *c) { return c ? c->err.c_str() : "no context"; }
+
+int uc_compact(uc_ctx *c, const uint32_t *packed, uint64_t n, uint32_t k, uint32_t l, int on_device)
+{
+    if (!c || (!packed && n)) return UC_E_ARG;
+    CHK(select_dev(c));
+    const uint64_t offs[2] = {0, n};
+    return timed_job(c, packed, offs, 1, k, l, on_device);
+}
+
+int uc_compact_loaded(uc_ctx *c, uint32_t l)
+{
+    if (!c) return UC_E_ARG;
+    if (!c->loaded) return fail(c, UC_E_STATE, "uc_compact_loaded before uc_load");
+    CHK(select_dev(c));
+    return timed_phases(c, l);
+}
+
+int uc_kernel_stat(uc_ctx *c, uint32_t idx, char *name, uint32_t name_cap, double *total_ms, uint64_t *launches)
+{
+    if (!c) return UC_E_ARG;
+    CHK(prof_flush(c));
+    if (idx >= c->kstats.size()) return UC_E_ARG;
+    const auto &k = c->kstats[idx];
+    if (name && name_cap) {
+        std::strncpy(name, k.name.c_str(), name_cap - 1);
+        name[name_cap - 1] = 0;
+    }
+    if (total_ms) *total_ms = k.ms;
+    if (launches) *launches = k.launches;
+    return UC_OK;
+}
+
+int uc_kernel_stats_reset(uc_ctx *c)
+{
+    if (!c) return UC_E_ARG;
+    CHK(prof_flush(c));
+    for (auto &k : c->kstats) {
+        k.ms = 0;
+        k.launches = 0;
+    }
+    return UC_OK;
+}
+
+int uc_compact_sets(uc_ctx *c, const uint32_t *packed, const uint64_t *set_offsets, uint32_t n_sets, uint32_t k,
+                    uint32_t l, int on_device)
+{
+    if (!c || !set_offsets || n_sets == 0) return UC_E_ARG;
+    if ((uint64_t)n_sets * c->B > 0x7FFFFFFFull) return fail(c, UC_E_LIMIT, "too many (set, bucket) pairs");
+    CHK(select_dev(c));
+    return timed_job(c, packed, set_offsets, n_sets, k, l, on_device);
+}
+
+
+int uc_load(uc_ctx *c, const uint32_t *packed, const uint64_t *set_offsets, uint32_t n_sets, uint32_t k, int on_device)
+{
+    if (!c || !set_offsets) return UC_E_ARG;
+    CHK(select_dev(c));
+    return do_load(c, packed, set_offsets, n_sets, k, on_device);
+}
+
+int uc_bucketize(uc_ctx *c, uint32_t l, const uint32_t *order_table, int table_on_device)
+{
+    if (!c) return UC_E_ARG;
+    CHK(select_dev(c));
+    const uint32_t *t = order_table;
+    if (order_table && !table_on_device) {
+        if (l < 1 || l > 15) return fail(c, UC_E_ARG, "l must be in [1, 15]");
+        const size_t bytes = (size_t)4 << (2 * l);
+        CHK(ensure(c, c->otable, bytes));
+        CHK(h2d(c, c->otable.p, order_table, bytes));
+        t = P<uint32_t>(c->otable);
+    }
+    return do_bucketize(c, l, t);
+}
+
+int uc_bucket_info_get(uc_ctx *c, uc_bucket_info *info)
+{
+    if (!c || !info) return UC_E_ARG;
+    if (!c->bucketed) return fail(c, UC_E_STATE, "no bucketing result");
+    info->n_kmers = c->n;
+    info->m_placements = c->m;
+    info->n_cross = c->x;
+    info->n_buckets = c->B;
+    info->n_sets = c->n_sets;
+    info->largest_bucket = c->largest;
+    return UC_OK;
+}
+
+int uc_bucket_copy(uc_ctx *c, uint64_t *bucket_rows, uint32_t *slot_src, uint8_t *slot_flags, uint32_t *pair_home,
+                   uint32_t *pair_borrow)
+{
+    if (!c) return UC_E_ARG;
+    if (!c->bucketed) return fail(c, UC_E_STATE, "no bucketing result");
+    CHK(select_dev(c));
+    if (bucket_rows)
+        for (size_t i = 0; i < c->boff_h.size(); i++) bucket_rows[i] = c->boff_h[i];
+    if (slot_src) CHK(d2h(c, slot_src, c->src.p, c->m * 4));
+    if (slot_flags) {
+        CHK(d2h(c, slot_flags, c->flags.p, c->m));
+        for (uint64_t i = 0; i < c->m; i++) slot_flags[i] &= (F_LENT | F_BORROW);
+    }
+    if (pair_home) CHK(d2h(c, pair_home, c->pair_h.p, c->x * 4));
+    if (pair_borrow) CHK(d2h(c, pair_borrow, c->pair_b.p, c->x * 4));
+    return UC_OK;
+}
+
+int uc_find_edges(uc_ctx *c, uint32_t bucket)
+{
+    if (!c) return UC_E_ARG;
+    CHK(select_dev(c));
+    return do_edges(c, bucket);
+}
+
+int uc_succ_copy(uc_ctx *c, uint32_t *succ)
+{
+    if (!c || !succ) return UC_E_ARG;
+    if (!c->edged) return fail(c, UC_E_STATE, "no edge result");
+    CHK(select_dev(c));
+    return d2h(c, succ, c->succ.p, c->m * 4);
+}
+
+int uc_coarsen(uc_ctx *c)
+{
+    if (!c) return UC_E_ARG;
+    CHK(select_dev(c));
+    return do_coarsen(c);
+}
+
+int uc_coarsen_info_get(uc_ctx *c, uc_coarsen_info *info)
+{
+    if (!c || !info) return UC_E_ARG;
+    if (!c->coarsened) return fail(c, UC_E_STATE, "no contraction result");
+    info->n_unitigs = c->n_unitigs;
+    info->n_cycles = c->n_cycles;
+    info->text_bytes = c->text_bytes;
+    info->n_splitters = c->n_splitters;
+    info->chain_rounds = c->chain_rounds;
+    return UC_OK;
+}
+
+int uc_emit(uc_ctx *c)
+{
+    if (!c) return UC_E_ARG;
+    CHK(select_dev(c));
+    return do_emit(c);
+}
+
+uint64_t uc_text_size(const uc_ctx *c) { return (c && c->emitted) ? c->text_bytes : 0; }
+
+const char *uc_text_device_ptr(const uc_ctx *c) { return (c && c->emitted) ? (const char *)c->text.p : nullptr; }
+
+int uc_text_copy(uc_ctx *c, char *dst, uint64_t cap)
+{
+    if (!c || (!dst && c->text_bytes)) return UC_E_ARG;
+    if (!c->emitted) return fail(c, UC_E_STATE, "no text");
+    if (cap < c->text_bytes) return fail(c, UC_E_ARG, "destination too small");
+    CHK(select_dev(c));
+    return d2h(c, dst, c->text.p, c->text_bytes);
+}
+
+uint64_t uc_set_text_offset(const uc_ctx *c, uint32_t set)
+{
+    if (!c || !c->emitted || set > c->n_sets) return 0;
+    return c->set_text[set];
+}
+
+int uc_text_append_file(uc_ctx *c, const char *path)
+{
+    if (!c || !path) return UC_E_ARG;
+    if (!c->emitted) return fail(c, UC_E_STATE, "no text");
+    std::vector<char> h(c->text_bytes);
+    CHK(uc_text_copy(c, h.data(), h.size()));
+    std::ofstream of(path, std::ofstream::out | std::ofstream::app | std::ofstream::binary);
+    if (!of.is_open()) return fail(c, UC_E_IO, std::string("could not open output file ") + path);
+    of.write(h.data(), (std::streamsize)h.size());
+    of.close();
+    if (!of) return fail(c, UC_E_IO, std::string("could not write output file ") + path);
+    return UC_OK;
+}
+
+int uc_timing_get(uc_ctx *c, uc_timing *t)
+{
+    if (!c || !t) return UC_E_ARG;
+    *t = c->timing;
+    return UC_OK;
+}
+
+int uc_sync(uc_ctx *c)
+{
+    if (!c) return UC_E_ARG;
+    CHK(select_dev(c));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return UC_OK;
+}
+
+}  // extern "C"
+
+namespace {
+__global__ void k_order_table(uint32_t *t, uint64_t n, LminParams p)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        t[i] = lmer_order((uint32_t)i, p);
+}
+}  // namespace
+
+extern "C" int uc_lmer_order_table(uc_ctx *c, uint32_t l, uint32_t *table, int on_device)
+{
+    if (!c || !table || l < 1 || l > 15) return UC_E_ARG;
+    CHK(select_dev(c));
+    LminParams p{};
+    const uint32_t bits = 2 * l;
+    p.lmask = (1u << bits) - 1u;
+    p.h1 = (bits + 1) / 2;
+    p.h2 = bits / 3 + 1;
+    p.s0 = (uint32_t)c->opts.seed & p.lmask;
+    p.s1 = (uint32_t)(c->opts.seed >> 32) & p.lmask;
+    p.table = nullptr;
+    const uint64_t n = 1ull << bits;
+    uint32_t *dst = table;
+    if (!on_device) {
+        CHK(ensure(c, c->otable, n * 4));
+        dst = P<uint32_t>(c->otable);
+    }
+    KLAUNCH(c, "k_order_table", k_order_table, (std::min<uint32_t>(grid_for(n), 65536), BLOCK, 0), dst, n, p);
+    if (!on_device) return d2h(c, table, dst, n * 4);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return UC_OK;
+}

@@ -1,0 +1,56 @@
+// uc_kernels.h -- device-side building blocks of the compaction path (gfx950).
+//
+// Layouts (all in HBM, one context at a time):
+//   k-mers      uint64 x W per k-mer, W = 1 for k <= 32, 2 for k <= 64;
+//               character i in bits 2i of the 64*W-bit little-endian value
+//               (the reference packing read as 64-bit words).
+//   bucket CSR  slots [boff[g], boff[g+1]) hold bucket g = set*B + b:
+//               every k-mer in its prefix-minimizer bucket, crossing
+//               k-mers once more in their suffix-minimizer bucket.
+//   tables      per bucket, 2^tbits[g] uint64 slots at toff[g]:
+//               bits 0..31 owner (bucket-local slot), 32..59 key tag,
+//               60..62 number of suffix hits, 63 prefix shared.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace uc {
+
+constexpr uint32_t NONE = 0xFFFFFFFFu;
+constexpr uint64_t EMPTY = ~0ull;
+constexpr uint64_t SHARED_BIT = 1ull << 63;
+constexpr uint32_t TAG_MASK = 0x0FFFFFFFu;
+constexpr int HIT_SHIFT = 60;
+
+// slot flags
+constexpr uint8_t F_LENT = 1;    // home copy of a crossing k-mer: its out-edge lives elsewhere
+constexpr uint8_t F_BORROW = 2;  // duplicate in the suffix bucket: carries the out-edge, dies at the splice
+constexpr uint8_t F_CYCLE = 4;   // on an isolated cycle (set during contraction)
+
+constexpr int BLOCK = 256;
+constexpr int WAVES = BLOCK / 64;
+constexpr int BKT_IPT = 4;                    // k-mers per thread in the bucketing kernels
+constexpr int BKT_TILE = BLOCK * BKT_IPT;     // k-mers per bucketing tile
+constexpr int EDGE_TILE = 1024;               // slots per edge-finding tile
+constexpr uint32_t MAX_BUCKETS = 1024;        // buckets per set
+
+struct BTile {              // bucketing tile (one workgroup)
+    uint64_t kbegin;        // first k-mer (global index)
+    uint64_t hbase;         // first histogram entry of the tile's set
+    uint32_t count;         // k-mers in the tile
+    uint32_t set;
+    uint32_t ts;            // tiles of the set
+    uint32_t tlocal;        // tile index inside the set
+};
+
+struct ETile {              // edge-finding tile: slots of one bucket
+    uint32_t bucket, begin, end, pad;
+};
+
+struct LminParams {
+    uint32_t k, l, B;
+    uint32_t lmask, h1, h2, s0, s1;  // order bijection constants
+    const uint32_t *table;           // optional order table (4^l entries)
+};
+
+}  // namespace uc
