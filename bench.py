@@ -60,13 +60,14 @@ def kernel_bytes(name, n, m, x, W, n_unitigs, text_bytes, k):
         "k_edge_lookup": m * (1 + kb + 4) + look * (8 + kb + 8),
         # read hit, read the table slot, write succ and pred
         "k_edge_resolve": m * (4 + 4) + look * 8 + n * 4,
-        # read pred + flag (+ succ per visited vertex), write seg + rank
-        "k_walk": m * (4 + 1) + n * (4 + 4 + 4),
-        "k_final": m * (1 + 4) + n * (4 + 4 + 4 + 4),
-        "k_unitig_vals": m * (4 + 1 + 8),
-        "k_order": m * (1 + 4 + 4 + 4) + n * (8 + 4),
-        # read order + k-mer per position, write the text
-        "k_emit": n * (4 + kb) + n_unitigs * (4 + 8 + 4) + text_bytes,
+        # read flag + pred per slot, succ per k-mer; write length and next
+        # splitter per head, unitig length per head, one sum per tile
+        "k_seg_count": m * (1 + 4) + n * 4 + n_unitigs * (4 + 4 + 4),
+        # read flag + pred per slot, length per head; succ + k-mer per
+        # k-mer; write the text
+        "k_head_emit": m * (1 + 4) + n_unitigs * (4 + 4 + 4) + n * (4 + kb) + text_bytes,
+        # move the borrowed out-edge onto the home copy
+        "k_splice": x * (4 + 4 + 4 + 4 + 4),
     }
     return table.get(name)
 
@@ -124,6 +125,9 @@ def main():
     ap.add_argument("--l", type=int, default=11)
     ap.add_argument("--buckets", type=int, default=64)
     ap.add_argument("--mode", choices=["batched", "plain"], default="batched")
+    ap.add_argument("--load-pct", type=int, default=50, help="hashmap load factor (percent)")
+    ap.add_argument("--sample-log2", type=int, default=5, help="splitter sampling rate 2^-s")
+    ap.add_argument("--xcd-map", action="store_true", help="XCD-contiguous tile order")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
     ap.add_argument("--cpu-sample-bp", type=int, default=4_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -145,7 +149,8 @@ def main():
     log(f"workload: {n} canonical {k}-mers from {args.genome_bp} bp in {time.time() - t:.1f}s")
 
     mode = uc.UC_MODE_PLAIN if args.mode == "plain" else uc.UC_MODE_BATCHED
-    c = uc.Compactor(n_buckets=B, mode=mode, profile=True)
+    tun = {"load_pct": args.load_pct, "sample_log2": args.sample_log2}
+    c = uc.Compactor(n_buckets=B, mode=mode, profile=True, xcd_map=args.xcd_map, **tun)
     c.load(kmers, k)  # one H2D copy; every step starts from HBM
     for i in range(args.warmup):
         c.compact_loaded(l)
@@ -191,7 +196,7 @@ def main():
 
     plain = None
     if not args.no_plain and mode == uc.UC_MODE_BATCHED:
-        with uc.Compactor(n_buckets=B, mode=uc.UC_MODE_PLAIN) as cp:
+        with uc.Compactor(n_buckets=B, mode=uc.UC_MODE_PLAIN, **tun) as cp:
             cp.load(kmers, k)
             cp.compact_loaded(l)
             cp.sync()
@@ -226,7 +231,7 @@ def main():
         "config": {
             "workload": f"medium: {args.genome_bp / 1e6:.0f} Mbp seeded genome (seed {args.seed}), 5% repeats of "
                         f"2-10 kbp copies with 0.1% substitutions, 1% error k-mers; distinct canonical {k}-mers",
-            "n_kmers": n, "k": k, "l": l, "buckets": B, "mode": args.mode,
+            "n_kmers": n, "k": k, "l": l, "buckets": B, "mode": args.mode, **tun,
             "m_placements": b.m_placements, "n_cross": b.n_cross, "n_unitigs": ci.n_unitigs,
             "text_bytes": text_bytes,
         },

@@ -62,6 +62,7 @@ typedef struct {
 } uc_opts;
 
 #define UC_FLAG_PROFILE 1u /* time every kernel launch with HIP events (uc_kernel_stat) */
+#define UC_FLAG_XCD_MAP 2u /* give each XCD a contiguous range of bucket tiles */
 
 void uc_default_opts(uc_opts *o);
 int uc_ctx_create(const uc_opts *opts, uc_ctx **out);

@@ -4,6 +4,7 @@ Outputs under unitigcoarsening_amd/_lib/:
   libuc.so        the compaction library (HIP kernels + C ABI, include/uc_api.h)
   uc_main         the command-line drop-in for the reference's `main`
   libuc_synth.so  the seeded synthetic k-mer generator (host only)
+  uc_copy_probe   a plain device copy of known size (traffic-counter calibration)
 
 Run ``python -m unitigcoarsening_amd.build`` or call :func:`build_all`.
 """
@@ -77,6 +78,15 @@ def build_cli(verbose=False):
     return out
 
 
+def build_probe(verbose=False):
+    """Plain copy kernel of known size (counter calibration)."""
+    out = os.path.join(LIB, "uc_copy_probe")
+    tmp = out + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, os.path.join(CSRC, "uc_probe.hip"), "-o", tmp], verbose)
+    _atomic_move(tmp, out)
+    return out
+
+
 def build_synth(verbose=False):
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libuc_synth.so")
@@ -96,7 +106,8 @@ def build_oracle(verbose=False):
 def build_all(verbose=False):
     if not shutil.which(HIPCC) and not os.path.exists(HIPCC):
         raise RuntimeError("hipcc not found")
-    paths = [build_lib(verbose), build_cli(verbose), build_synth(verbose), build_oracle(verbose)]
+    paths = [build_lib(verbose), build_cli(verbose), build_synth(verbose), build_probe(verbose),
+             build_oracle(verbose)]
     return paths
 
 

@@ -148,12 +148,23 @@ __device__ __forceinline__ uint32_t lmer_order(uint32_t code, const LminParams &
 {
     if (p.table) return p.table[code];
     uint32_t x = (code ^ p.s0) & p.lmask;
-    x = (x * 0x3779B1u) & p.lmask;
-    x ^= x >> p.h1;
-    x = (x * 0xEBCA6Bu) & p.lmask;
-    x ^= x >> p.h2;
-    x ^= p.s1;
-    x = (x * 0xD4EB2Fu) & p.lmask;
+    if (p.lmask < (1u << 24)) {
+        // codes and constants below 2^24: the full-rate 24-bit multiply
+        // gives the same residue modulo 2^(2l)
+        x = __umul24(x, 0x3779B1u) & p.lmask;
+        x ^= x >> p.h1;
+        x = __umul24(x, 0xEBCA6Bu) & p.lmask;
+        x ^= x >> p.h2;
+        x ^= p.s1;
+        x = __umul24(x, 0xD4EB2Fu) & p.lmask;
+    } else {
+        x = (x * 0x3779B1u) & p.lmask;
+        x ^= x >> p.h1;
+        x = (x * 0xEBCA6Bu) & p.lmask;
+        x ^= x >> p.h2;
+        x ^= p.s1;
+        x = (x * 0xD4EB2Fu) & p.lmask;
+    }
     x ^= x >> p.h1;
     return x;
 }
@@ -187,6 +198,20 @@ __device__ __forceinline__ char base_char(uint32_t c) { return (char)((0x5447434
 __device__ __forceinline__ bool sampled(uint32_t v, uint32_t salt, uint32_t shift)
 {
     return shift != 0 && (fmix32(v ^ salt) >> (32 - shift)) == 0;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup i runs on
+// XCD i % 8).  Renumbering gives XCD x the contiguous tile range
+// [x*per, (x+1)*per), so the buckets one XCD works on stay in its own L2.
+constexpr uint32_t N_XCD = 8;
+
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t n_tiles)
+{
+    if (!(n_tiles & 0x80000000u)) return blockIdx.x < n_tiles ? blockIdx.x : NONE;  // natural order
+    n_tiles &= 0x7FFFFFFFu;
+    const uint32_t per = (n_tiles + N_XCD - 1) / N_XCD;
+    const uint32_t t = (blockIdx.x % N_XCD) * per + blockIdx.x / N_XCD;
+    return t < n_tiles ? t : NONE;
 }
 
 // ---------------------------------------------------------------------------
@@ -465,9 +490,11 @@ __global__ __launch_bounds__(BLOCK) void k_edge_insert(const uint64_t *__restric
                                                        const ETile *__restrict__ tiles, const uint32_t *__restrict__ boff,
                                                        const uint64_t *__restrict__ toff,
                                                        const uint8_t *__restrict__ tbits, uint64_t *table, uint64_t pm0,
-                                                       uint64_t pm1)
+                                                       uint64_t pm1, uint32_t n_tiles)
 {
-    const ETile et = tiles[blockIdx.x];
+    const uint32_t ti = xcd_tile(n_tiles);
+    if (ti == NONE) return;
+    const ETile et = tiles[ti];
     const uint32_t base = boff[et.bucket];
     uint64_t *tb = table + toff[et.bucket];
     const uint32_t mask = (1u << tbits[et.bucket]) - 1u;
@@ -479,11 +506,10 @@ __global__ __launch_bounds__(BLOCK) void k_edge_insert(const uint64_t *__restric
         const uint64_t entry = ((uint64_t)tag << 32) | (uint64_t)(s - base);
         uint32_t pos = h & mask;
         for (uint32_t probe = 0; probe <= mask; probe++) {
-            uint64_t cur = tb[pos];
-            if (cur == EMPTY) {
-                cur = atomicCAS((unsigned long long *)&tb[pos], (unsigned long long)EMPTY, (unsigned long long)entry);
-                if (cur == EMPTY) break;
-            }
+            // claim first: the compare-and-swap returns the occupant anyway
+            const uint64_t cur =
+                atomicCAS((unsigned long long *)&tb[pos], (unsigned long long)EMPTY, (unsigned long long)entry);
+            if (cur == EMPTY) break;
             if ((uint32_t)((cur >> 32) & TAG_MASK) == tag) {
                 const Kmer<W> o = prefix_key<W>(load_kmer<W>(kmer, base + (uint32_t)cur, ~0ull), pm0, pm1);
                 if (key_eq<W>(o, key)) {
@@ -501,9 +527,11 @@ __global__ __launch_bounds__(BLOCK) void k_edge_lookup(const uint64_t *__restric
                                                        const ETile *__restrict__ tiles, const uint32_t *__restrict__ boff,
                                                        const uint64_t *__restrict__ toff,
                                                        const uint8_t *__restrict__ tbits, uint64_t *table, uint64_t pm0,
-                                                       uint64_t pm1, uint32_t *__restrict__ hit)
+                                                       uint64_t pm1, uint32_t *__restrict__ hit, uint32_t n_tiles)
 {
-    const ETile et = tiles[blockIdx.x];
+    const uint32_t ti = xcd_tile(n_tiles);
+    if (ti == NONE) return;
+    const ETile et = tiles[ti];
     const uint32_t base = boff[et.bucket];
     uint64_t *tb = table + toff[et.bucket];
     const uint32_t mask = (1u << tbits[et.bucket]) - 1u;
@@ -535,9 +563,11 @@ __global__ __launch_bounds__(BLOCK) void k_edge_lookup(const uint64_t *__restric
 __global__ __launch_bounds__(BLOCK) void k_edge_resolve(const ETile *__restrict__ tiles, const uint32_t *__restrict__ boff,
                                                         const uint64_t *__restrict__ toff, const uint64_t *table,
                                                         const uint32_t *__restrict__ hit, uint32_t *__restrict__ succ,
-                                                        uint32_t *__restrict__ pred)
+                                                        uint32_t *__restrict__ pred, uint32_t n_tiles)
 {
-    const ETile et = tiles[blockIdx.x];
+    const uint32_t ti = xcd_tile(n_tiles);
+    if (ti == NONE) return;
+    const ETile et = tiles[ti];
     const uint32_t base = boff[et.bucket];
     const uint64_t *tb = table + toff[et.bucket];
     for (uint32_t s = et.begin + threadIdx.x; s < et.end; s += BLOCK) {
@@ -556,6 +586,18 @@ __global__ __launch_bounds__(BLOCK) void k_edge_resolve(const ETile *__restrict_
 
 // ---------------------------------------------------------------------------
 // 3. contraction
+//
+// After the splice every live k-mer has at most one successor and one
+// predecessor, so the kept edges form disjoint simple paths (and, rarely,
+// isolated cycles).  Splitters are the path heads plus a hashed 2^-s sample
+// of the other vertices; a segment is a splitter and the vertices up to the
+// next splitter.  Segment lengths come from one walk per splitter, chains of
+// segments are ranked by pointer jumping over the sampled splitters only,
+// and the text is written by a second walk per segment.  No per-vertex
+// array is written.
+
+constexpr int CT_ITEMS = SCAN_ITEMS;
+constexpr int CT_TILE = SCAN_TILE;  // slots per contraction tile
 
 // the borrowed duplicate's out-edge moves to the home copy
 __global__ void k_splice(const uint32_t *__restrict__ pair_home, const uint32_t *__restrict__ pair_borrow, uint32_t x,
@@ -569,47 +611,110 @@ __global__ void k_splice(const uint32_t *__restrict__ pair_home, const uint32_t 
     if (t != NONE) pred[t] = h;
 }
 
-// every splitter (path head, or a sampled vertex) walks its segment
-__global__ __launch_bounds__(BLOCK) void k_walk(const uint32_t *__restrict__ succ, const uint32_t *__restrict__ pred,
-                                                const uint8_t *__restrict__ flags, uint32_t m, uint32_t salt,
-                                                uint32_t shift, uint32_t *__restrict__ seg,
-                                                uint32_t *__restrict__ rank, uint32_t *__restrict__ seglen,
-                                                uint32_t *__restrict__ n_split, uint32_t *__restrict__ err)
+// list index for every flagged lane: one atomic per wave
+__device__ __forceinline__ uint32_t wave_append(bool flag, uint32_t *counter)
 {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    bool split = false;
-    if (s < m && !(flags[s] & F_BORROW)) split = pred[s] == NONE || sampled(s, salt, shift);
-    const uint64_t bal = __ballot(split);
-    if (__lane_id() == 0 && bal) atomicAdd(n_split, (uint32_t)__popcll(bal));
-    if (!split) return;
-    uint32_t v = s, r = 0;
-    do {
-        seg[v] = s;
-        rank[v] = r++;
-        v = succ[v];
-        // in-degree <= 1 bounds every walk by m; the cap only guards the GPU
-        // against a corrupted successor array
-        if (r > m) {
-            atomicOr(err, 1u);
-            break;
-        }
-    } while (v != NONE && !sampled(v, salt, shift));
-    seglen[s] = r;
+    const uint64_t bal = __ballot(flag);
+    if (!bal) return NONE;
+    const int leader = __builtin_ctzll(bal);
+    uint32_t base = 0;
+    if (__lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(bal));
+    base = __shfl(base, leader, 64);
+    return base + (uint32_t)__popcll(bal & ((1ull << __lane_id()) - 1ull));
 }
 
-// sampled splitters that are not heads, with their preceding segment
-__global__ void k_chain_init(const uint32_t *__restrict__ pred, const uint8_t *__restrict__ flags,
-                             const uint32_t *__restrict__ seg, const uint32_t *__restrict__ seglen, uint32_t m,
-                             uint32_t salt, uint32_t shift, uint64_t *__restrict__ chain,
-                             uint32_t *__restrict__ list, uint32_t *__restrict__ n_list)
+// counters in the context's small buffer
+enum { C_LIST = 0, C_ERR = 1, C_ACTIVE = 2, C_NONCONV = 3, C_CYC = 4, C_CUT = 5, C_SPLIT = 6 };
+
+// one walk per splitter: segment length, next splitter; per tile the sum
+// of (1 << 32 | length) over heads whose unitig is a single segment (the
+// other heads add their count here and their length in k_chain_tail)
+__global__ __launch_bounds__(BLOCK) void k_seg_count(const uint32_t *__restrict__ succ, const uint32_t *__restrict__ pred,
+                                                     const uint8_t *__restrict__ flags, uint32_t m, uint32_t salt,
+                                                     uint32_t shift, uint32_t *__restrict__ seglen,
+                                                     uint32_t *__restrict__ segnext, uint32_t *__restrict__ prevseg,
+                                                     uint32_t *__restrict__ ulen, uint32_t *__restrict__ list,
+                                                     uint32_t *__restrict__ cnt, uint64_t *__restrict__ tile_sum,
+                                                     uint32_t n_tiles)
 {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= m || (flags[s] & F_BORROW)) return;
-    const uint32_t p = pred[s];
-    if (p == NONE || !sampled(s, salt, shift)) return;
-    const uint32_t a = seg[p];
-    chain[s] = (uint64_t)a | ((uint64_t)seglen[a] << 32);
-    list[atomicAdd(n_list, 1u)] = s;
+    __shared__ uint64_t red[WAVES];
+    const uint32_t ti = xcd_tile(n_tiles);
+    if (ti == NONE) return;
+    const uint64_t base = (uint64_t)ti * CT_TILE;
+    // splitters of this tile; sampled ones get list slots with one atomic
+    // per workgroup
+    bool hd[CT_ITEMS], rd[CT_ITEMS];
+    uint32_t nr = 0;
+#pragma unroll
+    for (int j = 0; j < CT_ITEMS; j++) {
+        const uint64_t s64 = base + (uint64_t)j * BLOCK + threadIdx.x;
+        hd[j] = rd[j] = false;
+        if (s64 < m && !(flags[s64] & F_BORROW)) {
+            hd[j] = pred[s64] == NONE;
+            rd[j] = !hd[j] && sampled((uint32_t)s64, salt, shift);
+        }
+        nr += rd[j];
+    }
+    __shared__ uint32_t wtot[WAVES];
+    __shared__ uint32_t bbase;
+    const uint32_t incl = wave_incl_scan(nr);
+    const int w = threadIdx.x / 64;
+    if (__lane_id() == 63) wtot[w] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int q = 0; q < WAVES; q++) t += wtot[q];
+        bbase = t ? atomicAdd(cnt + C_LIST, t) : 0u;
+    }
+    __syncthreads();
+    uint32_t li = bbase + incl - nr;
+    for (int q = 0; q < w; q++) li += wtot[q];
+    uint64_t acc = 0;
+    uint32_t nsplit = 0;
+    for (int j = 0; j < CT_ITEMS; j++) {
+        const uint32_t s = (uint32_t)(base + (uint64_t)j * BLOCK + threadIdx.x);
+        const bool head = hd[j], rnd = rd[j];
+        if (head || rnd) {
+            nsplit++;
+            uint32_t v = s, r = 0;
+            do {
+                r++;
+                v = succ[v];
+                // in-degree <= 1 bounds every walk by m; the cap only guards
+                // the GPU against a corrupted successor array
+                if (r > m) {
+                    atomicOr(cnt + C_ERR, 1u);
+                    v = NONE;
+                    break;
+                }
+            } while (v != NONE && !sampled(v, salt, shift));
+            seglen[s] = r;
+            segnext[s] = v;
+            if (v != NONE) prevseg[v] = s;
+            if (head) {
+                acc += 1ull << 32;
+                if (v == NONE) {
+                    ulen[s] = r;
+                    acc += r;
+                }
+            }
+            if (rnd) list[li++] = s;
+        }
+    }
+    acc = block_reduce(acc, red);
+    if (threadIdx.x == 0) tile_sum[ti] = acc;
+    for (int o = 32; o > 0; o >>= 1) nsplit += __shfl_xor(nsplit, o, 64);
+    if (__lane_id() == 0 && nsplit) atomicAdd(cnt + C_SPLIT, nsplit);
+}
+
+// sampled splitters: (preceding segment, its length)
+__global__ void k_chain_init(const uint32_t *__restrict__ list, uint32_t n, const uint32_t *__restrict__ prevseg,
+                             const uint32_t *__restrict__ seglen, uint64_t *__restrict__ chain)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t r = list[i], a = prevseg[r];
+    chain[r] = (uint64_t)a | ((uint64_t)seglen[a] << 32);
 }
 
 // one synchronous pointer-jumping round: (ancestor, distance) pairs
@@ -636,52 +741,62 @@ __global__ void k_chain_round(const uint32_t *__restrict__ list, uint32_t n, con
     if (__lane_id() == 0 && bal) atomicAdd(active, (uint32_t)__popcll(bal));
 }
 
-// (unitig head, position) of every live k-mer, written over (seg, rank);
-// tails record the unitig length at the head; unresolved k-mers are on cycles
-__global__ __launch_bounds__(BLOCK) void k_final(const uint32_t *__restrict__ succ, const uint32_t *__restrict__ pred,
-                                                 uint8_t *__restrict__ flags, const uint64_t *__restrict__ chain,
-                                                 uint32_t m, uint32_t *__restrict__ seg, uint32_t *__restrict__ rank,
-                                                 uint32_t *__restrict__ ulen, uint32_t *__restrict__ n_cyc)
+// the tail segment of a multi-segment unitig gives the unitig length
+__global__ void k_chain_tail(const uint32_t *__restrict__ list, uint32_t n, const uint32_t *__restrict__ pred,
+                             const uint64_t *__restrict__ chain, const uint32_t *__restrict__ seglen,
+                             const uint32_t *__restrict__ segnext, uint32_t *__restrict__ ulen,
+                             uint64_t *__restrict__ tile_sum, uint32_t *__restrict__ nonconv)
 {
-    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    bool cyc = false;
-    if (v < m && !(flags[v] & F_BORROW)) {
-        const uint32_t s = seg[v];
-        uint32_t root = NONE, pos = 0;
-        if (s == NONE) {
-            cyc = true;
-        } else if (pred[s] == NONE) {
-            root = s;
-            pos = rank[v];
-        } else {
-            const uint64_t c = chain[s];
-            const uint32_t a = (uint32_t)c;
-            if (pred[a] != NONE) {
-                cyc = true;
-            } else {
-                root = a;
-                pos = (uint32_t)(c >> 32) + rank[v];
-            }
-        }
-        if (cyc) {
-            flags[v] |= F_CYCLE;
-        } else {
-            seg[v] = root;
-            rank[v] = pos;
-            if (succ[v] == NONE) ulen[root] = pos + 1;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool bad = false;
+    if (i < n) {
+        const uint32_t r = list[i];
+        const uint64_t c = chain[r];
+        const uint32_t a = (uint32_t)c;
+        if (pred[a] != NONE) {
+            bad = true;  // no head behind this chain: an isolated cycle
+        } else if (segnext[r] == NONE) {
+            const uint32_t len = (uint32_t)(c >> 32) + seglen[r];
+            ulen[a] = len;
+            atomicAdd((unsigned long long *)&tile_sum[a / CT_TILE], (unsigned long long)len);
         }
     }
-    const uint64_t bal = __ballot(cyc);
-    if (__lane_id() == 0 && bal) atomicAdd(n_cyc, (uint32_t)__popcll(bal));
+    const uint64_t bal = __ballot(bad);
+    if (__lane_id() == 0 && bal) atomicAdd(nonconv, (uint32_t)__popcll(bal));
 }
 
-__global__ void k_cycle_init(const uint32_t *__restrict__ succ, const uint8_t *__restrict__ flags, uint32_t m,
+// ---- isolated cycles (only when some k-mers are not reached from a head) --
+
+// marks every k-mer of a segment that hangs off a head
+__global__ __launch_bounds__(BLOCK) void k_cover(const uint32_t *__restrict__ succ, const uint32_t *__restrict__ pred,
+                                                 uint8_t *__restrict__ flags, const uint64_t *__restrict__ chain,
+                                                 const uint32_t *__restrict__ seglen, uint32_t m, uint32_t salt,
+                                                 uint32_t shift)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= m || (flags[s] & F_BORROW)) return;
+    const bool head = pred[s] == NONE;
+    if (!head) {
+        if (!sampled(s, salt, shift)) return;
+        if (pred[(uint32_t)chain[s]] != NONE) return;
+    }
+    uint32_t v = s;
+    for (uint32_t i = 0, n = seglen[s]; i < n && v != NONE; i++) {
+        flags[v] |= F_VISIT;
+        v = succ[v];
+    }
+}
+
+__global__ void k_cycle_init(const uint32_t *__restrict__ succ, uint8_t *__restrict__ flags, uint32_t m,
                              uint64_t *__restrict__ lab, uint32_t *__restrict__ list, uint32_t *__restrict__ n_list)
 {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= m || !(flags[v] & F_CYCLE)) return;
+    const bool cyc = v < m && !(flags[v] & (F_BORROW | F_VISIT));
+    const uint32_t li = wave_append(cyc, n_list);
+    if (!cyc) return;
+    flags[v] |= F_CYCLE;
     lab[v] = (uint64_t)v | ((uint64_t)succ[v] << 32);
-    list[atomicAdd(n_list, 1u)] = v;
+    list[li] = v;
 }
 
 // min-label pointer jumping around each cycle: (smallest k-mer so far, jump)
@@ -701,6 +816,7 @@ __global__ void k_cycle_round(const uint32_t *__restrict__ list, uint32_t n, con
     lout[v] = (uint64_t)best | ((uint64_t)j2 << 32);
 }
 
+// each cycle loses the edge into its smallest k-mer
 __global__ void k_cycle_cut(const uint32_t *__restrict__ list, uint32_t n, const uint64_t *__restrict__ lab,
                             uint32_t *__restrict__ succ, uint32_t *__restrict__ pred, uint8_t *__restrict__ flags,
                             uint32_t *__restrict__ n_cut)
@@ -717,55 +833,137 @@ __global__ void k_cycle_cut(const uint32_t *__restrict__ list, uint32_t n, const
     }
 }
 
+__global__ void k_clear_flag(uint8_t *__restrict__ flags, uint32_t m, uint8_t bit)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (uint64_t)gridDim.x * blockDim.x)
+        flags[i] &= (uint8_t)~bit;
+}
+
 // ---------------------------------------------------------------------------
 // 4. emission
-
-// (heads << 32 | k-mers) of each unitig head, 0 elsewhere
-__global__ void k_unitig_vals(const uint32_t *__restrict__ pred, const uint8_t *__restrict__ flags,
-                              const uint32_t *__restrict__ ulen, uint32_t m, uint64_t *__restrict__ val)
-{
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= m) return;
-    const bool head = !(flags[s] & F_BORROW) && pred[s] == NONE;
-    val[s] = head ? ((1ull << 32) | (uint64_t)ulen[s]) : 0ull;
-}
-
-__global__ void k_order(const uint8_t *__restrict__ flags, const uint32_t *__restrict__ root,
-                        const uint32_t *__restrict__ pos, const uint32_t *__restrict__ pred,
-                        const uint64_t *__restrict__ uscan, uint32_t m, uint32_t *__restrict__ order,
-                        uint32_t *__restrict__ uroot)
-{
-    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= m || (flags[v] & F_BORROW)) return;
-    const uint32_t r = root[v];
-    const uint64_t us = uscan[r];
-    order[(uint32_t)us + pos[v]] = v;
-    if (pred[v] == NONE) uroot[(uint32_t)(us >> 32)] = v;
-}
+//
+// Unitig u (in order of head slot) occupies text bytes [len_<u + u*k,
+// ... + len_u + k): its first k-mer in full, the last character of every
+// later k-mer, '\n'.
 
 template <int W>
-__global__ __launch_bounds__(BLOCK) void k_emit(const uint64_t *__restrict__ kmer, const uint32_t *__restrict__ order,
-                                                const uint32_t *__restrict__ uroot, const uint64_t *__restrict__ uscan,
-                                                const uint32_t *__restrict__ ulen, uint32_t n_unitigs, uint32_t k,
-                                                char *__restrict__ text)
+__device__ __forceinline__ char last_char(const Kmer<W> &x, uint32_t k)
 {
-    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= n_unitigs) return;
-    const uint32_t s = uroot[u];
-    const uint32_t ks = (uint32_t)uscan[s], len = ulen[s];
-    char *o = text + (uint64_t)ks + (uint64_t)u * k;
-    const Kmer<W> x0 = load_kmer<W>(kmer, order[ks], ~0ull);
-    for (uint32_t i = 0; i < k; i++) o[i] = base_char((uint32_t)(x0.w[i / 32] >> (2 * (i % 32))) & 3u);
-    const uint32_t last = k - 1;
-    for (uint32_t p = 1; p < len; p++) {
-        const Kmer<W> x = load_kmer<W>(kmer, order[ks + p], ~0ull);
-        o[last + p] = base_char((uint32_t)(x.w[last / 32] >> (2 * (last % 32))) & 3u);
+    const uint32_t i = k - 1;
+    return base_char((uint32_t)(x.w[i / 32] >> (2 * (i % 32))) & 3u);
+}
+
+// per tile: exclusive prefix of (1 << 32 | length) over heads -> text
+// offsets; every head spells its own segment
+template <int W>
+__global__ __launch_bounds__(BLOCK) void k_head_emit(const uint64_t *__restrict__ kmer, const uint32_t *__restrict__ succ,
+                                                     const uint32_t *__restrict__ pred, const uint8_t *__restrict__ flags,
+                                                     const uint32_t *__restrict__ seglen,
+                                                     const uint32_t *__restrict__ segnext,
+                                                     const uint32_t *__restrict__ ulen,
+                                                     const uint64_t *__restrict__ tile_off, uint32_t m, uint32_t k,
+                                                     uint64_t *__restrict__ htoff, char *__restrict__ text,
+                                                     uint32_t n_tiles)
+{
+    __shared__ uint64_t tile[CT_TILE];
+    __shared__ uint64_t wsum[WAVES];
+    const uint32_t ti = xcd_tile(n_tiles);
+    if (ti == NONE) return;
+    const uint64_t base = (uint64_t)ti * CT_TILE;
+    bool hd[CT_ITEMS];
+#pragma unroll
+    for (int j = 0; j < CT_ITEMS; j++) {
+        const uint64_t s = base + (uint64_t)j * BLOCK + threadIdx.x;
+        hd[j] = s < m && !(flags[s] & F_BORROW) && pred[s] == NONE;
+        tile[j * BLOCK + threadIdx.x] = hd[j] ? ((1ull << 32) | ulen[s]) : 0ull;
     }
-    o[k - 1 + len] = '\n';
+    __syncthreads();
+    uint64_t loc[CT_ITEMS];
+    uint64_t run = 0;
+#pragma unroll
+    for (int j = 0; j < CT_ITEMS; j++) {
+        loc[j] = run;
+        run += tile[threadIdx.x * CT_ITEMS + j];
+    }
+    const uint64_t incl = wave_incl_scan(run);
+    const int w = threadIdx.x / 64;
+    if (__lane_id() == 63) wsum[w] = incl;
+    __syncthreads();
+    uint64_t wpre = 0;
+    for (int i = 0; i < w; i++) wpre += wsum[i];
+    const uint64_t excl = tile_off[ti] + wpre + incl - run;
+#pragma unroll
+    for (int j = 0; j < CT_ITEMS; j++) tile[threadIdx.x * CT_ITEMS + j] = excl + loc[j];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < CT_ITEMS; j++) {
+        if (!hd[j]) continue;
+        const uint32_t s = (uint32_t)(base + (uint64_t)j * BLOCK + threadIdx.x);
+        const uint64_t pre = tile[j * BLOCK + threadIdx.x];
+        const uint64_t off = (pre & 0xFFFFFFFFull) + (pre >> 32) * k;
+        const uint32_t len = seglen[s], nxt = segnext[s];
+        if (nxt != NONE) htoff[s] = off;
+        char *o = text + off;
+        const Kmer<W> x0 = load_kmer<W>(kmer, s, ~0ull);
+        for (uint32_t i = 0; i < k; i++) o[i] = base_char((uint32_t)(x0.w[i / 32] >> (2 * (i % 32))) & 3u);
+        uint32_t v = s;
+        for (uint32_t i = 1; i < len; i++) {
+            v = succ[v];
+            o[k - 1 + i] = last_char<W>(load_kmer<W>(kmer, v, ~0ull), k);
+        }
+        if (nxt == NONE) o[k - 1 + len] = '\n';
+    }
+}
+
+// segments of sampled splitters: positions from the ranked chain
+template <int W>
+__global__ void k_rand_emit(const uint64_t *__restrict__ kmer, const uint32_t *__restrict__ succ,
+                            const uint32_t *__restrict__ list, uint32_t n, const uint64_t *__restrict__ chain,
+                            const uint32_t *__restrict__ seglen, const uint32_t *__restrict__ segnext,
+                            const uint64_t *__restrict__ htoff, uint32_t k, char *__restrict__ text)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t r = list[i];
+    const uint64_t c = chain[r];
+    const uint32_t a = (uint32_t)c, d = (uint32_t)(c >> 32);
+    char *o = text + htoff[a] + (k - 1) + d;
+    const uint32_t len = seglen[r];
+    uint32_t v = r;
+    for (uint32_t p = 0; p < len; p++) {
+        o[p] = last_char<W>(load_kmer<W>(kmer, v, ~0ull), k);
+        v = succ[v];
+    }
+    if (segnext[r] == NONE) o[len] = '\n';
+}
+
+// text offset of each set's first unitig: the prefix at the set's first slot
+__global__ void k_set_offsets(const uint32_t *__restrict__ boff, uint32_t n_sets, uint32_t B,
+                              const uint32_t *__restrict__ pred, const uint8_t *__restrict__ flags,
+                              const uint32_t *__restrict__ ulen, const uint64_t *__restrict__ tile_off, uint32_t m,
+                              uint64_t total, uint64_t *__restrict__ out)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > n_sets) return;
+    const uint32_t sb = s < n_sets ? boff[(uint64_t)s * B] : m;
+    uint64_t acc = total;
+    if (sb < m) {
+        const uint32_t t = sb / CT_TILE;
+        acc = tile_off[t];
+        for (uint32_t x = t * CT_TILE; x < sb; x++)
+            if (!(flags[x] & F_BORROW) && pred[x] == NONE) acc += (1ull << 32) | ulen[x];
+    }
+    out[s] = acc;
 }
 
 // ---------------------------------------------------------------------------
 // host side
+
+// grid of whole XCD rounds covering n_tiles tiles (see xcd_tile)
+static uint32_t xcd_grid(uint64_t n_tiles)
+{
+    return (uint32_t)(N_XCD * ((n_tiles + N_XCD - 1) / N_XCD));
+}
 
 static uint32_t grid_for(uint64_t n, uint32_t block = BLOCK)
 {
@@ -807,8 +1005,11 @@ struct uc_ctx {
     // device buffers
     const uint64_t *kin = nullptr;  // k-mers in 64-bit layout (the caller's buffer or `in`)
     DBuf otable, in, stage, bkt, tiles, hist, xhist, scan_tmp, kmer, flags, src, pair_h, pair_b, boff, sethb, setts, etiles,
-        toff, tbits, table, hit, succ, pred, seg, rank, seglen, chainA, chainB, list, ulen, uscan, order, uroot, text,
-        small;
+        toff, tbits, table, hit, succ, pred, seglen, segnext, prevseg, chainA, chainB, cycB, list, clist, ulen, tsum,
+        htoff, setoff, text, small;
+    // contraction state
+    uint64_t *chain = nullptr;  // ranked chain entries (chainA or chainB)
+    uint32_t n_list = 0, salt = 0, shift = 5;
     hipEvent_t ev[6] = {};
 
     // per-kernel timing (UC_FLAG_PROFILE)
@@ -821,6 +1022,17 @@ struct uc_ctx {
     std::vector<std::pair<size_t, std::pair<hipEvent_t, hipEvent_t>>> pending;
     std::vector<hipEvent_t> evpool;
 };
+
+namespace {
+
+}  // namespace
+
+// tile-count argument of the tiled kernels: the top bit asks for the
+// XCD-contiguous renumbering of xcd_tile
+static uint32_t tile_arg(const uc_ctx *c, uint64_t n_tiles)
+{
+    return (uint32_t)n_tiles | ((c->opts.flags & UC_FLAG_XCD_MAP) ? 0x80000000u : 0u);
+}
 
 namespace {
 
@@ -1183,17 +1395,17 @@ int launch_edges(uc_ctx *c, size_t first, size_t count)
 {
     if (count == 0) return UC_OK;
     const ETile *tiles = P<ETile>(c->etiles) + first;
-    KLAUNCH(c, "k_edge_insert", k_edge_insert<W>, ((uint32_t)count, BLOCK, 0), P<uint64_t>(c->kmer), P<uint8_t>(c->flags), tiles,
+    KLAUNCH(c, "k_edge_insert", k_edge_insert<W>, (xcd_grid(count), BLOCK, 0), P<uint64_t>(c->kmer), P<uint8_t>(c->flags), tiles,
                                                                 P<uint32_t>(c->boff), P<uint64_t>(c->toff),
                                                                 P<uint8_t>(c->tbits), P<uint64_t>(c->table), c->pm0,
-                                                                c->pm1);
-    KLAUNCH(c, "k_edge_lookup", k_edge_lookup<W>, ((uint32_t)count, BLOCK, 0), P<uint64_t>(c->kmer), P<uint8_t>(c->flags), tiles,
+                                                                c->pm1, tile_arg(c, count));
+    KLAUNCH(c, "k_edge_lookup", k_edge_lookup<W>, (xcd_grid(count), BLOCK, 0), P<uint64_t>(c->kmer), P<uint8_t>(c->flags), tiles,
                                                                 P<uint32_t>(c->boff), P<uint64_t>(c->toff),
                                                                 P<uint8_t>(c->tbits), P<uint64_t>(c->table), c->pm0,
-                                                                c->pm1, P<uint32_t>(c->hit));
-    KLAUNCH(c, "k_edge_resolve", k_edge_resolve, ((uint32_t)count, BLOCK, 0), tiles, P<uint32_t>(c->boff), P<uint64_t>(c->toff),
+                                                                c->pm1, P<uint32_t>(c->hit), tile_arg(c, count));
+    KLAUNCH(c, "k_edge_resolve", k_edge_resolve, (xcd_grid(count), BLOCK, 0), tiles, P<uint32_t>(c->boff), P<uint64_t>(c->toff),
                                                              P<uint64_t>(c->table), P<uint32_t>(c->hit),
-                                                             P<uint32_t>(c->succ), P<uint32_t>(c->pred));
+                                                             P<uint32_t>(c->succ), P<uint32_t>(c->pred), tile_arg(c, count));
     return UC_OK;
 }
 
@@ -1294,70 +1506,93 @@ int do_edges(uc_ctx *c, uint32_t bucket)
 
 // ---- contraction ----------------------------------------------------------
 
-int rank_paths(uc_ctx *c, uint32_t salt, uint32_t shift, uint32_t *n_cyc_out)
+// one contraction pass: segment walks, chain ranking, unitig count/length
+// per tile, scan of the tiles.  *live = k-mers reached from a head,
+// *nonconv = sampled splitters whose chain has no head (on a cycle).
+int contract_pass(uc_ctx *c, uint64_t *live, uint32_t *nonconv)
 {
     const uint32_t m = (uint32_t)c->m;
-    uint32_t *cnt = P<uint32_t>(c->small);  // [0] splitters [1] list [2] active [3] cycles
+    const uint64_t T = (c->m + CT_TILE - 1) / CT_TILE;
+    uint32_t *cnt = P<uint32_t>(c->small);
     CHK(fill<uint32_t>(c, cnt, 8, 0u));
-    CHK(fill<uint32_t>(c, P<uint32_t>(c->seg), m, NONE));
-    KLAUNCH(c, "k_walk", k_walk, (grid_for(m), BLOCK, 0), P<uint32_t>(c->succ), P<uint32_t>(c->pred), P<uint8_t>(c->flags), m,
-                                                 salt, shift, P<uint32_t>(c->seg), P<uint32_t>(c->rank),
-                                                 P<uint32_t>(c->seglen), cnt + 0, cnt + 6);
-    KLAUNCH(c, "k_chain_init", k_chain_init, (grid_for(m), BLOCK, 0), P<uint32_t>(c->pred), P<uint8_t>(c->flags),
-                                                       P<uint32_t>(c->seg), P<uint32_t>(c->seglen), m, salt, shift,
-                                                       P<uint64_t>(c->chainA), P<uint32_t>(c->list), cnt + 1);
-    uint32_t h[2];
-    CHK(d2h(c, h, cnt, 8));
-    c->n_splitters = h[0];
-    const uint32_t nl = h[1];
+    KLAUNCH(c, "k_seg_count", k_seg_count, (xcd_grid(T), BLOCK, 0), P<uint32_t>(c->succ), P<uint32_t>(c->pred),
+            P<uint8_t>(c->flags), m, c->salt, c->shift, P<uint32_t>(c->seglen), P<uint32_t>(c->segnext),
+            P<uint32_t>(c->prevseg), P<uint32_t>(c->ulen), P<uint32_t>(c->list), cnt, P<uint64_t>(c->tsum),
+            tile_arg(c, T));
+    uint32_t h[8];
+    CHK(d2h(c, h, cnt, 32));
+    if (h[C_ERR]) return fail(c, UC_E_STATE, "successor array corrupted: a segment walk exceeded m steps");
+    c->n_list = h[C_LIST];
+    c->n_splitters = h[C_SPLIT];
+    const uint32_t nl = c->n_list;
     uint64_t *ca = P<uint64_t>(c->chainA), *cb = P<uint64_t>(c->chainB);
-    uint32_t rounds = 0;
+    uint32_t rounds = 0, bad = 0;
     if (nl) {
+        KLAUNCH(c, "k_chain_init", k_chain_init, (grid_for(nl), BLOCK, 0), P<uint32_t>(c->list), nl,
+                P<uint32_t>(c->prevseg), P<uint32_t>(c->seglen), ca);
+        // a chain holds at most nl segments: ceil(log2(nl)) + 1 rounds reach
+        // its head; chains still open after that run around a cycle
+        uint32_t cap = 1;
+        while ((1ull << cap) < nl) cap++;
+        cap += 1;
         for (;;) {
-            CHK(fill<uint32_t>(c, cnt + 2, 1, 0u));
-            KLAUNCH(c, "k_chain_round", k_chain_round, (grid_for(nl), BLOCK, 0), P<uint32_t>(c->list), nl, P<uint32_t>(c->pred), ca,
-                                                                 cb, cnt + 2);
+            CHK(fill<uint32_t>(c, cnt + C_ACTIVE, 1, 0u));
+            KLAUNCH(c, "k_chain_round", k_chain_round, (grid_for(nl), BLOCK, 0), P<uint32_t>(c->list), nl,
+                    P<uint32_t>(c->pred), ca, cb, cnt + C_ACTIVE);
             std::swap(ca, cb);
             rounds++;
             uint32_t active;
-            CHK(d2h(c, &active, cnt + 2, 4));
-            if (active == 0 || rounds > 40) break;  // 2^40 exceeds any chain; beyond: cycles
+            CHK(d2h(c, &active, cnt + C_ACTIVE, 4));
+            if (active == 0 || rounds >= cap) break;
         }
+        KLAUNCH(c, "k_chain_tail", k_chain_tail, (grid_for(nl), BLOCK, 0), P<uint32_t>(c->list), nl,
+                P<uint32_t>(c->pred), ca, P<uint32_t>(c->seglen), P<uint32_t>(c->segnext), P<uint32_t>(c->ulen),
+                P<uint64_t>(c->tsum), cnt + C_NONCONV);
+        CHK(d2h(c, &bad, cnt + C_NONCONV, 4));
     }
+    c->chain = ca;
     c->chain_rounds += rounds;
-    KLAUNCH(c, "k_final", k_final, (grid_for(m), BLOCK, 0), P<uint32_t>(c->succ), P<uint32_t>(c->pred), P<uint8_t>(c->flags),
-                                                  ca, m, P<uint32_t>(c->seg), P<uint32_t>(c->rank),
-                                                  P<uint32_t>(c->ulen), cnt + 3);
-    uint32_t res[4];
-    CHK(d2h(c, res, cnt + 3, 16));
-    if (res[3]) return fail(c, UC_E_STATE, "successor array corrupted: a segment walk exceeded m steps");
-    *n_cyc_out = res[0];
+    CHK(scan_excl<uint64_t>(c, P<uint64_t>(c->tsum), P<uint64_t>(c->tsum), T, P<uint64_t>(c->tsum) + T,
+                            c->scan_tmp));
+    uint64_t tot = 0;
+    CHK(d2h(c, &tot, P<uint64_t>(c->tsum) + T, 8));
+    c->n_unitigs = tot >> 32;
+    *live = tot & 0xFFFFFFFFull;
+    *nonconv = bad;
     return UC_OK;
 }
 
+// isolated cycles: every k-mer no head reaches lies on one; each cycle is
+// cut in front of its lexicographically smallest k-mer
 template <int W>
 int cut_cycles(uc_ctx *c, uint64_t *n_cut)
 {
     const uint32_t m = (uint32_t)c->m;
     uint32_t *cnt = P<uint32_t>(c->small);
-    CHK(fill<uint32_t>(c, cnt + 4, 1, 0u));
+    KLAUNCH(c, "k_cover", k_cover, (grid_for(m), BLOCK, 0), P<uint32_t>(c->succ), P<uint32_t>(c->pred),
+            P<uint8_t>(c->flags), c->chain, P<uint32_t>(c->seglen), m, c->salt, c->shift);
+    CHK(fill<uint32_t>(c, cnt + C_CYC, 2, 0u));
+    uint64_t *la = P<uint64_t>(c->chainA) == c->chain ? P<uint64_t>(c->chainB) : P<uint64_t>(c->chainA);
+    uint64_t *lb = P<uint64_t>(c->cycB);
     KLAUNCH(c, "k_cycle_init", k_cycle_init, (grid_for(m), BLOCK, 0), P<uint32_t>(c->succ), P<uint8_t>(c->flags), m,
-                                                       P<uint64_t>(c->chainA), P<uint32_t>(c->list), cnt + 4);
+            la, P<uint32_t>(c->clist), cnt + C_CYC);
     uint32_t nl;
-    CHK(d2h(c, &nl, cnt + 4, 4));
+    CHK(d2h(c, &nl, cnt + C_CYC, 4));
     uint32_t rounds = 1;
     while ((1ull << (rounds - 1)) < nl) rounds++;
-    uint64_t *ca = P<uint64_t>(c->chainA), *cb = P<uint64_t>(c->chainB);
-    for (uint32_t r = 0; r < rounds; r++) {
-        KLAUNCH(c, "k_cycle_round", k_cycle_round<W>, (grid_for(nl), BLOCK, 0), P<uint32_t>(c->list), nl, P<uint64_t>(c->kmer), ca,
-                                                                  cb);
-        std::swap(ca, cb);
+    for (uint32_t r = 0; r < rounds && nl; r++) {
+        KLAUNCH(c, "k_cycle_round", k_cycle_round<W>, (grid_for(nl), BLOCK, 0), P<uint32_t>(c->clist), nl,
+                P<uint64_t>(c->kmer), la, lb);
+        std::swap(la, lb);
     }
-    CHK(fill<uint32_t>(c, cnt + 5, 1, 0u));
-    KLAUNCH(c, "k_cycle_cut", k_cycle_cut, (grid_for(nl), BLOCK, 0), P<uint32_t>(c->list), nl, ca, P<uint32_t>(c->succ),
-                                                       P<uint32_t>(c->pred), P<uint8_t>(c->flags), cnt + 5);
+    if (nl) {
+        KLAUNCH(c, "k_cycle_cut", k_cycle_cut, (grid_for(nl), BLOCK, 0), P<uint32_t>(c->clist), nl, la,
+                P<uint32_t>(c->succ), P<uint32_t>(c->pred), P<uint8_t>(c->flags), cnt + C_CUT);
+    }
+    KLAUNCH(c, "k_clear_flag", k_clear_flag, (std::min<uint32_t>(grid_for(m), 65536), BLOCK, 0), P<uint8_t>(c->flags),
+            m, F_VISIT);
     uint32_t cuts = 0;
-    CHK(d2h(c, &cuts, cnt + 5, 4));
+    CHK(d2h(c, &cuts, cnt + C_CUT, 4));
     *n_cut = cuts;
     return UC_OK;
 }
@@ -1367,48 +1602,41 @@ int do_coarsen(uc_ctx *c)
     if (!c->edged) return fail(c, UC_E_STATE, "uc_coarsen before uc_find_edges");
     c->emitted = false;
     const uint32_t m = (uint32_t)c->m;
+    const uint64_t T = (c->m + CT_TILE - 1) / CT_TILE;
     CHK(ensure(c, c->small, 256));
-    CHK(ensure(c, c->seg, c->m * 4));
-    CHK(ensure(c, c->rank, c->m * 4));
     CHK(ensure(c, c->seglen, c->m * 4));
+    CHK(ensure(c, c->segnext, c->m * 4));
+    CHK(ensure(c, c->prevseg, c->m * 4));
+    CHK(ensure(c, c->ulen, c->m * 4));
     CHK(ensure(c, c->chainA, c->m * 8));
     CHK(ensure(c, c->chainB, c->m * 8));
     CHK(ensure(c, c->list, c->m * 4));
-    CHK(ensure(c, c->ulen, c->m * 4));
+    CHK(ensure(c, c->tsum, (T + 1) * 8));
     c->chain_rounds = 0;
     c->n_cycles = 0;
+    c->n_list = 0;
+    c->chain = P<uint64_t>(c->chainA);
+    c->salt = (uint32_t)(c->opts.seed * 0x9E3779B97F4A7C15ull >> 32);
+    c->shift = c->opts.sample_log2 ? c->opts.sample_log2 : 5;
     if (c->x) {
         KLAUNCH(c, "k_splice", k_splice, (grid_for(c->x), BLOCK, 0), P<uint32_t>(c->pair_h), P<uint32_t>(c->pair_b),
-                                                          (uint32_t)c->x, P<uint32_t>(c->succ), P<uint32_t>(c->pred));
+                (uint32_t)c->x, P<uint32_t>(c->succ), P<uint32_t>(c->pred));
     }
-    const uint32_t salt = (uint32_t)(c->opts.seed * 0x9E3779B97F4A7C15ull >> 32);
-    const uint32_t shift = c->opts.sample_log2 ? c->opts.sample_log2 : 5;
-    uint32_t n_cyc = 0;
-    uint64_t cut = 0;
-    if (m) CHK(rank_paths(c, salt, shift, &n_cyc));
-    if (n_cyc) {
-        // isolated cycles: cut each at its smallest k-mer, contract again
+    uint64_t live = 0;
+    uint32_t nonconv = 0;
+    if (m) CHK(contract_pass(c, &live, &nonconv));
+    if (live != c->n || nonconv) {
+        CHK(ensure(c, c->cycB, c->m * 8));
+        CHK(ensure(c, c->clist, c->m * 4));
+        uint64_t cut = 0;
         CHK(c->W == 1 ? cut_cycles<1>(c, &cut) : cut_cycles<2>(c, &cut));
-        uint32_t again = 0;
-        CHK(rank_paths(c, salt, shift, &again));
-        if (again) return fail(c, UC_E_STATE, "cycles left after cutting");
+        c->n_cycles = cut;
+        CHK(contract_pass(c, &live, &nonconv));
+        if (live != c->n || nonconv)
+            return fail(c, UC_E_STATE, "contraction lost k-mers: " + std::to_string(live) + " of " +
+                                           std::to_string(c->n) + " reached after cutting cycles");
     }
-    // unitig scan: (heads << 32 | k-mers) per head, exclusive over slots
-    CHK(ensure(c, c->uscan, (c->m + 1) * 8));
-    if (m) {
-        KLAUNCH(c, "k_unitig_vals", k_unitig_vals, (grid_for(m), BLOCK, 0), P<uint32_t>(c->pred), P<uint8_t>(c->flags),
-                                                            P<uint32_t>(c->ulen), m, P<uint64_t>(c->uscan));
-    }
-    CHK(scan_excl<uint64_t>(c, P<uint64_t>(c->uscan), P<uint64_t>(c->uscan), m, P<uint64_t>(c->uscan) + m,
-                            c->scan_tmp));
-    uint64_t tot = 0;
-    CHK(d2h(c, &tot, P<uint64_t>(c->uscan) + m, 8));
-    c->n_unitigs = tot >> 32;
-    const uint64_t live = tot & 0xFFFFFFFFull;
-    if (live != c->n) return fail(c, UC_E_STATE, "contraction lost k-mers: " + std::to_string(live) + " of " +
-                                                     std::to_string(c->n));
     c->text_bytes = live + c->n_unitigs * c->k;
-    c->n_cycles = cut;
     c->coarsened = true;
     return UC_OK;
 }
@@ -1417,29 +1645,30 @@ template <int W>
 int run_emit(uc_ctx *c)
 {
     const uint32_t m = (uint32_t)c->m;
-    CHK(ensure(c, c->order, c->n * 4));
-    CHK(ensure(c, c->uroot, c->n_unitigs * 4));
+    const uint64_t T = (c->m + CT_TILE - 1) / CT_TILE;
     CHK(ensure(c, c->text, c->text_bytes + 1));
+    CHK(ensure(c, c->htoff, c->m * 8));
     if (m) {
-        KLAUNCH(c, "k_order", k_order, (grid_for(m), BLOCK, 0), P<uint8_t>(c->flags), P<uint32_t>(c->seg), P<uint32_t>(c->rank),
-                                                      P<uint32_t>(c->pred), P<uint64_t>(c->uscan), m,
-                                                      P<uint32_t>(c->order), P<uint32_t>(c->uroot));
+        KLAUNCH(c, "k_head_emit", k_head_emit<W>, (xcd_grid(T), BLOCK, 0), P<uint64_t>(c->kmer), P<uint32_t>(c->succ),
+                P<uint32_t>(c->pred), P<uint8_t>(c->flags), P<uint32_t>(c->seglen), P<uint32_t>(c->segnext),
+                P<uint32_t>(c->ulen), P<uint64_t>(c->tsum), m, c->k, P<uint64_t>(c->htoff), P<char>(c->text),
+                tile_arg(c, T));
     }
-    if (c->n_unitigs) {
-        KLAUNCH(c, "k_emit", k_emit<W>, (grid_for(c->n_unitigs), BLOCK, 0), 
-            P<uint64_t>(c->kmer), P<uint32_t>(c->order), P<uint32_t>(c->uroot), P<uint64_t>(c->uscan),
-            P<uint32_t>(c->ulen), (uint32_t)c->n_unitigs, c->k, P<char>(c->text));
+    if (c->n_list) {
+        KLAUNCH(c, "k_rand_emit", k_rand_emit<W>, (grid_for(c->n_list), BLOCK, 0), P<uint64_t>(c->kmer),
+                P<uint32_t>(c->succ), P<uint32_t>(c->list), c->n_list, c->chain, P<uint32_t>(c->seglen),
+                P<uint32_t>(c->segnext), P<uint64_t>(c->htoff), c->k, P<char>(c->text));
     }
     // per-set text ranges: a set's unitigs start at its first bucket row
-    c->set_text.assign(c->n_sets + 1, c->text_bytes);
-    std::vector<uint64_t> us(c->n_sets, 0);
-    for (uint32_t s = 0; s < c->n_sets; s++) {
-        const uint32_t sb = c->boff_h[(uint64_t)s * c->B];
-        if (sb < m) {
-            CHK(d2h(c, &us[s], P<uint64_t>(c->uscan) + sb, 8));
-            c->set_text[s] = (us[s] & 0xFFFFFFFFull) + (us[s] >> 32) * c->k;
-        }
-    }
+    const uint64_t total = (c->n_unitigs << 32) | (c->text_bytes - c->n_unitigs * c->k);
+    CHK(ensure(c, c->setoff, (c->n_sets + 1) * 8));
+    KLAUNCH(c, "k_set_offsets", k_set_offsets, (grid_for(c->n_sets + 1), BLOCK, 0), P<uint32_t>(c->boff), c->n_sets,
+            c->B, P<uint32_t>(c->pred), P<uint8_t>(c->flags), P<uint32_t>(c->ulen), P<uint64_t>(c->tsum), m, total,
+            P<uint64_t>(c->setoff));
+    std::vector<uint64_t> so(c->n_sets + 1);
+    CHK(d2h(c, so.data(), c->setoff.p, (c->n_sets + 1) * 8));
+    c->set_text.resize(c->n_sets + 1);
+    for (uint32_t s = 0; s <= c->n_sets; s++) c->set_text[s] = (so[s] & 0xFFFFFFFFull) + (so[s] >> 32) * c->k;
     c->emitted = true;
     return UC_OK;
 }
@@ -1550,11 +1779,12 @@ int uc_ctx_destroy(uc_ctx *c)
     if (!c) return UC_OK;
     (void)hipSetDevice(c->opts.device);
     (void)hipStreamSynchronize(c->stream);
-    DBuf *bufs[] = {&c->otable, &c->in, &c->stage, &c->bkt, &c->tiles, &c->hist, &c->xhist, &c->scan_tmp, &c->kmer, &c->flags,
-                    &c->src, &c->pair_h, &c->pair_b, &c->boff, &c->sethb, &c->setts, &c->etiles, &c->toff,
-                    &c->tbits, &c->table, &c->hit, &c->succ, &c->pred, &c->seg, &c->rank, &c->seglen,
-                    &c->chainA, &c->chainB, &c->list, &c->ulen, &c->uscan, &c->order, &c->uroot, &c->text,
-                    &c->small};
+    DBuf *bufs[] = {&c->otable, &c->in,      &c->stage,  &c->bkt,     &c->tiles,  &c->hist,   &c->xhist,
+                    &c->scan_tmp, &c->kmer,  &c->flags,  &c->src,     &c->pair_h, &c->pair_b, &c->boff,
+                    &c->sethb,  &c->setts,   &c->etiles, &c->toff,    &c->tbits,  &c->table,  &c->hit,
+                    &c->succ,   &c->pred,    &c->seglen, &c->segnext, &c->prevseg, &c->chainA, &c->chainB,
+                    &c->cycB,   &c->list,    &c->clist,  &c->ulen,    &c->tsum,   &c->htoff,  &c->setoff,
+                    &c->text,   &c->small};
     for (DBuf *b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto &e : c->ev)

@@ -26,6 +26,7 @@ constexpr int HIT_SHIFT = 60;
 constexpr uint8_t F_LENT = 1;    // home copy of a crossing k-mer: its out-edge lives elsewhere
 constexpr uint8_t F_BORROW = 2;  // duplicate in the suffix bucket: carries the out-edge, dies at the splice
 constexpr uint8_t F_CYCLE = 4;   // on an isolated cycle (set during contraction)
+constexpr uint8_t F_VISIT = 8;   // reached from a path head (cycle detection only)
 
 constexpr int BLOCK = 256;
 constexpr int WAVES = BLOCK / 64;
