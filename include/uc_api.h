@@ -63,6 +63,7 @@ typedef struct {
 
 #define UC_FLAG_PROFILE 1u /* time every kernel launch with HIP events (uc_kernel_stat) */
 #define UC_FLAG_XCD_MAP 2u /* give each XCD a contiguous range of bucket tiles */
+#define UC_FLAG_KEEP_SRC 4u /* keep slot -> input k-mer ids for uc_bucket_copy */
 
 void uc_default_opts(uc_opts *o);
 int uc_ctx_create(const uc_opts *opts, uc_ctx **out);
@@ -118,12 +119,15 @@ typedef struct {
     uint32_t n_buckets;      /* buckets per set */
     uint32_t n_sets;
     uint64_t largest_bucket;
+    uint64_t n_fine_buckets;  /* fine buckets (coarse buckets split by further minimizer-order bits) */
+    uint64_t largest_fine_bucket;
 } uc_bucket_info;
 
 int uc_bucket_info_get(uc_ctx *ctx, uc_bucket_info *info);
 
 /* Host copies of the bucketing result for checking: bucket row offsets
- * (n_sets*n_buckets+1), slot -> input k-mer index (m), slot flags (m;
+ * (n_sets*n_buckets+1), slot -> input k-mer index (m; needs
+ * UC_FLAG_KEEP_SRC), slot flags (m;
  * bit0 = home copy of a crossing k-mer, bit1 = borrowed duplicate) and the
  * pairs (home slot, borrowed slot) (n_cross each).  Any pointer may be NULL. */
 int uc_bucket_copy(uc_ctx *ctx, uint64_t *bucket_rows, uint32_t *slot_src, uint8_t *slot_flags, uint32_t *pair_home,

@@ -25,6 +25,7 @@ UC_MODE_BATCHED = 0
 UC_MODE_PLAIN = 1
 UC_FLAG_PROFILE = 1
 UC_FLAG_XCD_MAP = 2
+UC_FLAG_KEEP_SRC = 4
 
 F_LENT = 1
 F_BORROW = 2
@@ -57,6 +58,8 @@ class UcBucketInfo(C.Structure):
         ("n_buckets", C.c_uint32),
         ("n_sets", C.c_uint32),
         ("largest_bucket", C.c_uint64),
+        ("n_fine_buckets", C.c_uint64),
+        ("largest_fine_bucket", C.c_uint64),
     ]
 
 
@@ -262,13 +265,14 @@ class Compactor:
     """One device context (uc_ctx).  Not thread-safe: one host thread per object."""
 
     def __init__(self, device=0, seed=0x5EED, n_buckets=64, mode=UC_MODE_BATCHED, load_pct=50, sample_log2=5,
-                 hbm_budget=0, profile=False, xcd_map=False):
+                 hbm_budget=0, profile=False, xcd_map=False, keep_src=False):
         L = lib()
         o = UcOpts()
         L.uc_default_opts(C.byref(o))
         o.device, o.seed, o.n_buckets, o.mode = device, seed, n_buckets, mode
         o.load_pct, o.sample_log2, o.hbm_budget = load_pct, sample_log2, hbm_budget
-        o.flags = (UC_FLAG_PROFILE if profile else 0) | (UC_FLAG_XCD_MAP if xcd_map else 0)
+        o.flags = ((UC_FLAG_PROFILE if profile else 0) | (UC_FLAG_XCD_MAP if xcd_map else 0)
+                   | (UC_FLAG_KEEP_SRC if keep_src else 0))
         self.opts = o
         self._ctx = _P()
         _check(L.uc_ctx_create(C.byref(o), C.byref(self._ctx)))

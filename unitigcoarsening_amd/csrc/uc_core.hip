@@ -331,11 +331,12 @@ __global__ __launch_bounds__(BLOCK) void k_scan_tile(const T *in, T *out, uint64
 
 template <int W>
 __global__ __launch_bounds__(BLOCK) void k_lmin_count(const uint64_t *__restrict__ kin, const BTile *__restrict__ tiles,
-                                                      LminParams p, uint64_t top_mask, uint32_t *__restrict__ bkt,
+                                                      LminParams p, uint64_t top_mask, uint2 *__restrict__ bkt,
                                                       uint32_t *__restrict__ hist, uint32_t *__restrict__ xhist)
 {
     extern __shared__ uint32_t cnt[];  // B buckets + 1 crossing counter
     const BTile t = tiles[blockIdx.x];
+    // fine bucket = order & (B*S - 1); its low log2(B) bits are the coarse bucket
     for (uint32_t i = threadIdx.x; i <= p.B; i += BLOCK) cnt[i] = 0;
     __syncthreads();
     const uint32_t bmask = p.B - 1;
@@ -348,7 +349,7 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_count(const uint64_t *__restrict
             uint32_t o1, o2;
             double_lmin<W>(x, p, o1, o2);
             const uint32_t bp = o1 & bmask, bs = o2 & bmask;
-            bkt[g] = bp | (bs << 16);
+            bkt[g] = make_uint2(o1 & t.fmask, o2 & t.fmask);
             atomicAdd(&cnt[bp], 1u);
             if (bp != bs) {
                 atomicAdd(&cnt[bs], 1u);
@@ -387,12 +388,13 @@ __device__ __forceinline__ uint64_t match_bucket(uint32_t v, uint32_t nbits, uin
 template <int W>
 __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restrict__ kin, const BTile *__restrict__ tiles,
                                                         uint32_t B, uint32_t nbits, uint64_t top_mask,
-                                                        const uint32_t *__restrict__ bkt,
+                                                        const uint2 *__restrict__ bkt,
                                                         const uint32_t *__restrict__ hscan,
                                                         const uint32_t *__restrict__ xscan, uint64_t *__restrict__ kout,
                                                         uint8_t *__restrict__ flags, uint32_t *__restrict__ src,
                                                         uint32_t *__restrict__ pair_home,
-                                                        uint32_t *__restrict__ pair_borrow)
+                                                        uint32_t *__restrict__ pair_borrow,
+                                                        uint16_t *__restrict__ sub)
 {
     extern __shared__ uint32_t sh[];
     uint32_t *run = sh;              // B+1 running counts
@@ -407,11 +409,13 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restri
         const uint32_t idx = j * BLOCK + threadIdx.x;
         const bool valid = idx < t.count;
         const uint64_t g = t.kbegin + idx;
-        uint32_t bp = 0, bs = 0;
+        uint32_t bp = 0, bs = 0, fp = 0, fs = 0;
         if (valid) {
-            const uint32_t c = bkt[g];
-            bp = c & 0xFFFFu;
-            bs = c >> 16;
+            const uint2 c = bkt[g];
+            fp = c.x >> nbits;  // fine index inside the coarse bucket
+            fs = c.y >> nbits;
+            bp = c.x & (B - 1);
+            bs = c.y & (B - 1);
         }
         const bool cross = valid && bp != bs;
         const uint64_t V = __ballot(valid), C = __ballot(cross);
@@ -439,14 +443,16 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restri
             const uint32_t sh_slot = hscan[t.hbase + (uint64_t)bp * t.ts + t.tlocal] + ph;
             store_kmer<W>(kout, sh_slot, x);
             flags[sh_slot] = cross ? F_LENT : 0;
-            src[sh_slot] = (uint32_t)g;
+            sub[sh_slot] = (uint16_t)fp;
+            if (src) src[sh_slot] = (uint32_t)g;
             if (cross) {
                 uint32_t pb = run[bs] + (uint32_t)__popcll(Mb & lt);
                 for (int q = 0; q < w; q++) pb += wcnt[q * (B + 1) + bs];
                 const uint32_t sb_slot = hscan[t.hbase + (uint64_t)bs * t.ts + t.tlocal] + pb;
                 store_kmer<W>(kout, sb_slot, x);
                 flags[sb_slot] = F_BORROW;
-                src[sb_slot] = (uint32_t)g;
+                sub[sb_slot] = (uint16_t)fs;
+                if (src) src[sb_slot] = (uint32_t)g;
                 uint32_t px = run[B] + (uint32_t)__popcll(C & lt);
                 for (int q = 0; q < w; q++) px += wcnt[q * (B + 1) + B];
                 pair_home[xbase + px] = sh_slot;
@@ -482,42 +488,206 @@ __global__ void k_gather_boff(const uint32_t *__restrict__ hscan, const uint64_t
     }
 }
 
+// ---- fine buckets: a stable counting sort of every coarse bucket's slots by
+// their fine index (the next bits of the same minimizer order)
+
+__global__ __launch_bounds__(BLOCK) void k_sub_count(const uint16_t *__restrict__ sub, const BTile *__restrict__ tiles,
+                                                     uint32_t *__restrict__ hist)
+{
+    extern __shared__ uint32_t cnt[];
+    const BTile t = tiles[blockIdx.x];
+    const uint32_t S = t.fmask;
+    for (uint32_t i = threadIdx.x; i < S; i += BLOCK) cnt[i] = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < t.count; i += BLOCK) atomicAdd(&cnt[sub[t.kbegin + i]], 1u);
+    __syncthreads();
+    for (uint32_t v = threadIdx.x; v < S; v += BLOCK) hist[t.hbase + (uint64_t)v * t.ts + t.tlocal] = cnt[v];
+}
+
+template <int W>
+__global__ __launch_bounds__(BLOCK) void k_sub_scatter(const BTile *__restrict__ tiles, const uint16_t *__restrict__ sub,
+                                                       const uint32_t *__restrict__ hscan, const uint64_t *__restrict__ kin,
+                                                       const uint8_t *__restrict__ fin, const uint32_t *__restrict__ sin,
+                                                       uint64_t *__restrict__ kout, uint8_t *__restrict__ fout,
+                                                       uint32_t *__restrict__ sout, uint32_t *__restrict__ perm)
+{
+    extern __shared__ uint32_t sh[];
+    const BTile t = tiles[blockIdx.x];
+    const uint32_t S = t.fmask;
+    const uint32_t nbits = (uint32_t)__builtin_ctz(S);
+    uint32_t *run = sh;          // S running counts
+    uint32_t *wcnt = sh + S;     // WAVES x S per-wave counts
+    for (uint32_t i = threadIdx.x; i < S * (WAVES + 1); i += BLOCK) sh[i] = 0;
+    __syncthreads();
+    const int lane = __lane_id(), w = threadIdx.x / 64;
+    const uint64_t lt = (1ull << lane) - 1ull, me = 1ull << lane;
+    const uint64_t zero[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t j0 = 0; j0 < t.count; j0 += BLOCK) {
+        const uint32_t idx = j0 + threadIdx.x;
+        const bool valid = idx < t.count;
+        const uint64_t s = t.kbegin + idx;
+        const uint32_t v = valid ? sub[s] : 0u;
+        const uint64_t V = __ballot(valid);
+        uint64_t BP[10];
+#pragma unroll
+        for (int q = 0; q < 10; q++) BP[q] = (uint32_t)q < nbits ? (uint64_t)__ballot(valid && ((v >> q) & 1u)) : 0ull;
+        uint64_t M = 0;
+        if (valid) {
+            M = match_bucket(v, nbits, V, 0ull, BP, zero);
+            if ((M & (0ull - M)) == me) wcnt[w * S + v] = (uint32_t)__popcll(M);
+        }
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = run[v] + (uint32_t)__popcll(M & lt);
+            for (int q = 0; q < w; q++) pos += wcnt[q * S + v];
+            const uint32_t ns = hscan[t.hbase + (uint64_t)v * t.ts + t.tlocal] + pos;
+            store_kmer<W>(kout, ns, load_kmer<W>(kin, s, ~0ull));
+            fout[ns] = fin[s];
+            if (sin) sout[ns] = sin[s];
+            perm[s] = ns;
+        }
+        __syncthreads();
+        for (uint32_t b = threadIdx.x; b < S; b += BLOCK) {
+            uint32_t a = 0;
+#pragma unroll
+            for (int q = 0; q < WAVES; q++) {
+                a += wcnt[q * S + b];
+                wcnt[q * S + b] = 0;
+            }
+            run[b] += a;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_pair_remap(uint32_t *__restrict__ pair_home, uint32_t *__restrict__ pair_borrow, uint32_t x,
+                             const uint32_t *__restrict__ perm)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= x) return;
+    pair_home[i] = perm[pair_home[i]];
+    pair_borrow[i] = perm[pair_borrow[i]];
+}
+
+// fine row starts: the scanned pass-2 histogram at each coarse bucket's first tile
+__global__ void k_gather_fine(const uint32_t *__restrict__ hscan, const uint64_t *__restrict__ cb_hbase,
+                              const uint32_t *__restrict__ cb_ts, const uint32_t *__restrict__ cb_first,
+                              uint32_t n_coarse, uint32_t m, uint32_t *__restrict__ fboff)
+{
+    const uint32_t g = blockIdx.x;
+    if (g == n_coarse) {
+        if (threadIdx.x == 0) fboff[cb_first[n_coarse]] = m;
+        return;
+    }
+    const uint32_t S = cb_first[g + 1] - cb_first[g];
+    for (uint32_t v = threadIdx.x; v < S; v += blockDim.x)
+        fboff[cb_first[g] + v] = hscan[cb_hbase[g] + (uint64_t)v * cb_ts[g]];
+}
+
 // ---------------------------------------------------------------------------
 // 2. edge finding
+//
+// One open-addressing table per bucket over the (k-1)-prefixes of the
+// bucket's home copies; 16-byte entries:
+//   key  k <= 32: the (k-1)-prefix itself (< 2^62, so never EMPTY);
+//        k > 32:  claim word (tag << 32 | owner), the key read back from the
+//        owner's k-mer on a tag match
+//   own  bucket-local owner slot (30 bits) | owner's last character << 30
+//   cnt  hits of suffix lookups (bits 0..15) | CNT_SHARED (prefix of more
+//        than one k-mer)
+// k_edge_insert claims prefixes, k_edge_lookup counts suffix hits and
+// remembers the hit position, k_edge_resolve keeps u -> v when the prefix
+// is unshared and was hit exactly once, writing link[u] = v | char(v) << 32
+// and v's has-predecessor bit.
+
+struct TEntry {
+    uint64_t key;
+    uint32_t own;
+    uint32_t cnt;
+};
+constexpr uint32_t CNT_SHARED = 0x80000000u;
+constexpr uint32_t OWN_MASK = 0x3FFFFFFFu;
+constexpr uint64_t NO_LINK = (uint64_t)NONE;
+
+__device__ __forceinline__ uint32_t fastrange(uint32_t h, uint32_t cap)
+{
+    return (uint32_t)(((uint64_t)h * cap) >> 32);
+}
+
+template <int W>
+__device__ __forceinline__ uint32_t char_at(const Kmer<W> &x, uint32_t i)
+{
+    return (uint32_t)(x.w[i / 32] >> (2 * (i % 32))) & 3u;
+}
+
+template <int W>
+__device__ __forceinline__ uint64_t claim_word(const Kmer<W> &key, uint32_t tag, uint32_t local)
+{
+    if constexpr (W == 1) return key.w[0];
+    return ((uint64_t)tag << 32) | local;
+}
+
+// does occupant `cur` hold `key`?
+template <int W>
+__device__ __forceinline__ bool same_key(uint64_t cur, const Kmer<W> &key, uint32_t tag,
+                                         const uint64_t *__restrict__ kmer, uint32_t base, uint64_t pm0, uint64_t pm1)
+{
+    if constexpr (W == 1) {
+        return cur == key.w[0];
+    } else {
+        if ((uint32_t)(cur >> 32) != tag) return false;
+        const Kmer<W> o = prefix_key<W>(load_kmer<W>(kmer, base + (uint32_t)cur, ~0ull), pm0, pm1);
+        return key_eq<W>(o, key);
+    }
+}
+
+__device__ __forceinline__ bool has_pred(const uint32_t *__restrict__ hp, uint32_t v)
+{
+    return (hp[v >> 5] >> (v & 31)) & 1u;
+}
+
+__global__ void k_table_clear(TEntry *__restrict__ t, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        reinterpret_cast<ulonglong2 *>(t)[i] = make_ulonglong2(EMPTY, 0ull);
+}
 
 template <int W>
 __global__ __launch_bounds__(BLOCK) void k_edge_insert(const uint64_t *__restrict__ kmer, const uint8_t *__restrict__ flags,
                                                        const ETile *__restrict__ tiles, const uint32_t *__restrict__ boff,
                                                        const uint64_t *__restrict__ toff,
-                                                       const uint8_t *__restrict__ tbits, uint64_t *table, uint64_t pm0,
-                                                       uint64_t pm1, uint32_t n_tiles)
+                                                       const uint32_t *__restrict__ tcap, TEntry *table, uint32_t k,
+                                                       uint64_t pm0, uint64_t pm1, uint32_t n_tiles)
 {
     const uint32_t ti = xcd_tile(n_tiles);
     if (ti == NONE) return;
     const ETile et = tiles[ti];
     const uint32_t base = boff[et.bucket];
-    uint64_t *tb = table + toff[et.bucket];
-    const uint32_t mask = (1u << tbits[et.bucket]) - 1u;
+    TEntry *tb = table + toff[et.bucket];
+    const uint32_t cap = tcap[et.bucket];
     for (uint32_t s = et.begin + threadIdx.x; s < et.end; s += BLOCK) {
         if (flags[s] & F_BORROW) continue;
-        const Kmer<W> key = prefix_key<W>(load_kmer<W>(kmer, s, ~0ull), pm0, pm1);
+        const Kmer<W> x = load_kmer<W>(kmer, s, ~0ull);
+        const Kmer<W> key = prefix_key<W>(x, pm0, pm1);
         uint32_t h, tag;
         key_hash<W>(key, h, tag);
-        const uint64_t entry = ((uint64_t)tag << 32) | (uint64_t)(s - base);
-        uint32_t pos = h & mask;
-        for (uint32_t probe = 0; probe <= mask; probe++) {
-            // claim first: the compare-and-swap returns the occupant anyway
+        const uint32_t local = s - base;
+        const uint64_t cw = claim_word<W>(key, tag, local);
+        uint32_t pos = fastrange(h, cap);
+        for (uint32_t probe = 0; probe < cap; probe++) {
+            // claim first: a compare-and-swap returns the occupant without
+            // pulling the line through a separate load
             const uint64_t cur =
-                atomicCAS((unsigned long long *)&tb[pos], (unsigned long long)EMPTY, (unsigned long long)entry);
-            if (cur == EMPTY) break;
-            if ((uint32_t)((cur >> 32) & TAG_MASK) == tag) {
-                const Kmer<W> o = prefix_key<W>(load_kmer<W>(kmer, base + (uint32_t)cur, ~0ull), pm0, pm1);
-                if (key_eq<W>(o, key)) {
-                    atomicOr((unsigned long long *)&tb[pos], (unsigned long long)SHARED_BIT);
-                    break;
-                }
+                atomicCAS((unsigned long long *)&tb[pos].key, (unsigned long long)EMPTY, (unsigned long long)cw);
+            if (cur == EMPTY) {
+                tb[pos].own = local | (char_at<W>(x, k - 1) << 30);
+                break;
             }
-            pos = (pos + 1) & mask;
+            if (same_key<W>(cur, key, tag, kmer, base, pm0, pm1)) {
+                atomicOr(&tb[pos].cnt, CNT_SHARED);
+                break;
+            }
+            pos = pos + 1 == cap ? 0 : pos + 1;
         }
     }
 }
@@ -526,34 +696,31 @@ template <int W>
 __global__ __launch_bounds__(BLOCK) void k_edge_lookup(const uint64_t *__restrict__ kmer, const uint8_t *__restrict__ flags,
                                                        const ETile *__restrict__ tiles, const uint32_t *__restrict__ boff,
                                                        const uint64_t *__restrict__ toff,
-                                                       const uint8_t *__restrict__ tbits, uint64_t *table, uint64_t pm0,
+                                                       const uint32_t *__restrict__ tcap, TEntry *table, uint64_t pm0,
                                                        uint64_t pm1, uint32_t *__restrict__ hit, uint32_t n_tiles)
 {
     const uint32_t ti = xcd_tile(n_tiles);
     if (ti == NONE) return;
     const ETile et = tiles[ti];
     const uint32_t base = boff[et.bucket];
-    uint64_t *tb = table + toff[et.bucket];
-    const uint32_t mask = (1u << tbits[et.bucket]) - 1u;
+    TEntry *tb = table + toff[et.bucket];
+    const uint32_t cap = tcap[et.bucket];
     for (uint32_t s = et.begin + threadIdx.x; s < et.end; s += BLOCK) {
         uint32_t res = NONE;
         if (!(flags[s] & F_LENT)) {
             const Kmer<W> key = suffix_key<W>(load_kmer<W>(kmer, s, ~0ull));
             uint32_t h, tag;
             key_hash<W>(key, h, tag);
-            uint32_t pos = h & mask;
-            for (uint32_t probe = 0; probe <= mask; probe++) {
-                const uint64_t cur = tb[pos];
+            uint32_t pos = fastrange(h, cap);
+            for (uint32_t probe = 0; probe < cap; probe++) {
+                const uint64_t cur = tb[pos].key;
                 if (cur == EMPTY) break;
-                if ((uint32_t)((cur >> 32) & TAG_MASK) == tag) {
-                    const Kmer<W> o = prefix_key<W>(load_kmer<W>(kmer, base + (uint32_t)cur, ~0ull), pm0, pm1);
-                    if (key_eq<W>(o, key)) {
-                        atomicAdd((unsigned long long *)&tb[pos], 1ull << HIT_SHIFT);
-                        res = pos;
-                        break;
-                    }
+                if (same_key<W>(cur, key, tag, kmer, base, pm0, pm1)) {
+                    atomicAdd(&tb[pos].cnt, 1u);
+                    res = pos;
+                    break;
                 }
-                pos = (pos + 1) & mask;
+                pos = pos + 1 == cap ? 0 : pos + 1;
             }
         }
         hit[s] = res;
@@ -561,26 +728,138 @@ __global__ __launch_bounds__(BLOCK) void k_edge_lookup(const uint64_t *__restric
 }
 
 __global__ __launch_bounds__(BLOCK) void k_edge_resolve(const ETile *__restrict__ tiles, const uint32_t *__restrict__ boff,
-                                                        const uint64_t *__restrict__ toff, const uint64_t *table,
-                                                        const uint32_t *__restrict__ hit, uint32_t *__restrict__ succ,
-                                                        uint32_t *__restrict__ pred, uint32_t n_tiles)
+                                                        const uint64_t *__restrict__ toff, const TEntry *table,
+                                                        const uint32_t *__restrict__ hit, uint64_t *__restrict__ link,
+                                                        uint32_t *__restrict__ hasp, uint32_t n_tiles)
 {
     const uint32_t ti = xcd_tile(n_tiles);
     if (ti == NONE) return;
     const ETile et = tiles[ti];
     const uint32_t base = boff[et.bucket];
-    const uint64_t *tb = table + toff[et.bucket];
+    const TEntry *tb = table + toff[et.bucket];
     for (uint32_t s = et.begin + threadIdx.x; s < et.end; s += BLOCK) {
         const uint32_t h = hit[s];
-        uint32_t v = NONE;
+        uint64_t l = NO_LINK;
         if (h != NONE) {
-            const uint64_t e = tb[h];
-            if (!(e & SHARED_BIT) && ((e >> HIT_SHIFT) & 7u) == 1u) {
-                v = base + (uint32_t)e;
-                pred[v] = s;
+            const uint2 e = *reinterpret_cast<const uint2 *>(&tb[h].own);
+            if (e.y == 1u) {  // unshared prefix, exactly one suffix hit
+                const uint32_t v = base + (e.x & OWN_MASK);
+                l = (uint64_t)v | ((uint64_t)(e.x >> 30) << 32);
+                atomicOr(&hasp[v >> 5], 1u << (v & 31));
             }
         }
-        succ[s] = v;
+        link[s] = l;
+    }
+}
+
+// ---- the LDS form: one workgroup stages a group of consecutive fine buckets
+// (k-mers, flags) in LDS and runs insert, lookup and resolve against an LDS
+// table.  Keys of different fine buckets never match (equal (k-1)-mers have
+// equal minimizers), so a group shares one table.  HBM sees only the
+// sequential k-mer/flag reads and link writes plus one bitmap word per 32
+// slots.
+//   LDS layout (nb = launch capacity, a power of two):
+//     km   Kmer<W>[nb]   staged k-mers
+//     tab  uint32[2nb]   owner (16 bits) | LDS_SHARED | hits << 24, or LDS_EMPTY
+//     hitp uint16[nb]    table position of each slot's suffix hit
+//     fl   uint8[nb]     staged flags
+//     hb   uint32[nb/32 + 2]  has-predecessor bits, aligned to global words
+
+constexpr uint32_t LDS_EMPTY = 0xFFFFFFFFu;
+constexpr uint32_t LDS_SHARED = 1u << 16;
+constexpr uint16_t LDS_NOHIT = 0xFFFFu;
+
+template <int W>
+__host__ __device__ constexpr size_t edge_lds_bytes(uint32_t nb)
+{
+    return (size_t)nb * (8 * W + 8 + 2 + 1) + ((size_t)nb / 32 + 2) * 4;
+}
+
+template <int W>
+__global__ __launch_bounds__(BLOCK) void k_edge_lds(const uint64_t *__restrict__ kmer, const uint8_t *__restrict__ flags,
+                                                    const EGroup *__restrict__ groups, uint32_t nb, uint32_t k,
+                                                    uint64_t pm0, uint64_t pm1, uint64_t *__restrict__ link,
+                                                    uint32_t *__restrict__ hasp)
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    Kmer<W> *km = reinterpret_cast<Kmer<W> *>(lds);
+    uint32_t *tab = reinterpret_cast<uint32_t *>(lds + (size_t)nb * 8 * W);
+    uint16_t *hitp = reinterpret_cast<uint16_t *>(lds + (size_t)nb * (8 * W + 8));
+    uint8_t *fl = lds + (size_t)nb * (8 * W + 10);
+    uint32_t *hb = reinterpret_cast<uint32_t *>(lds + (size_t)nb * (8 * W + 11));  // nb is a multiple of 64
+    const EGroup gr = groups[blockIdx.x];
+    const uint32_t n = gr.end - gr.begin;
+    uint32_t tcap = 64;
+    while (tcap < 2 * n) tcap <<= 1;
+    const uint32_t tmask = tcap - 1;
+    const uint32_t off0 = gr.begin & 31;
+    const uint32_t nw = (off0 + n + 31) / 32;
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+        km[i] = load_kmer<W>(kmer, gr.begin + i, ~0ull);
+        fl[i] = flags[gr.begin + i];
+    }
+    for (uint32_t i = threadIdx.x; i < tcap; i += BLOCK) tab[i] = LDS_EMPTY;
+    for (uint32_t i = threadIdx.x; i < nw; i += BLOCK) hb[i] = 0;
+    __syncthreads();
+    // prefixes of home copies
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+        if (fl[i] & F_BORROW) continue;
+        const Kmer<W> key = prefix_key<W>(km[i], pm0, pm1);
+        uint32_t h, tag;
+        key_hash<W>(key, h, tag);
+        uint32_t pos = h & tmask;
+        for (uint32_t probe = 0; probe < tcap; probe++) {
+            const uint32_t cur = atomicCAS(&tab[pos], LDS_EMPTY, i);
+            if (cur == LDS_EMPTY) break;
+            if (key_eq<W>(prefix_key<W>(km[cur & 0xFFFFu], pm0, pm1), key)) {
+                atomicOr(&tab[pos], LDS_SHARED);
+                break;
+            }
+            pos = (pos + 1) & tmask;
+        }
+    }
+    __syncthreads();
+    // suffixes of every copy that carries its out-edge here
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+        uint16_t hp = LDS_NOHIT;
+        if (!(fl[i] & F_LENT)) {
+            const Kmer<W> key = suffix_key<W>(km[i]);
+            uint32_t h, tag;
+            key_hash<W>(key, h, tag);
+            uint32_t pos = h & tmask;
+            for (uint32_t probe = 0; probe < tcap; probe++) {
+                const uint32_t e = tab[pos];
+                if (e == LDS_EMPTY) break;
+                if (key_eq<W>(prefix_key<W>(km[e & 0xFFFFu], pm0, pm1), key)) {
+                    atomicAdd(&tab[pos], 1u << 24);
+                    hp = (uint16_t)pos;
+                    break;
+                }
+                pos = (pos + 1) & tmask;
+            }
+        }
+        hitp[i] = hp;
+    }
+    __syncthreads();
+    // keep u -> v when v's prefix is unshared and was hit exactly once
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+        const uint16_t hp = hitp[i];
+        uint64_t l = NO_LINK;
+        if (hp != LDS_NOHIT) {
+            const uint32_t e = tab[hp];
+            if (!(e & LDS_SHARED) && (e >> 24) == 1u) {
+                const uint32_t v = e & 0xFFFFu;
+                l = (uint64_t)(gr.begin + v) | ((uint64_t)char_at<W>(km[v], k - 1) << 32);
+                atomicOr(&hb[(off0 + v) >> 5], 1u << ((off0 + v) & 31));
+            }
+        }
+        link[gr.begin + i] = l;
+    }
+    __syncthreads();
+    // boundary words may be shared with neighbouring groups
+    for (uint32_t j = threadIdx.x; j < nw; j += BLOCK) {
+        const uint32_t bits = hb[j];
+        if (bits) atomicOr(&hasp[(gr.begin >> 5) + j], bits);
     }
 }
 
@@ -593,22 +872,21 @@ __global__ __launch_bounds__(BLOCK) void k_edge_resolve(const ETile *__restrict_
 // of the other vertices; a segment is a splitter and the vertices up to the
 // next splitter.  Segment lengths come from one walk per splitter, chains of
 // segments are ranked by pointer jumping over the sampled splitters only,
-// and the text is written by a second walk per segment.  No per-vertex
-// array is written.
+// and the text is written by a second walk per segment.  A walk step reads
+// one 8-byte link (successor and its last character); no per-vertex array
+// is written.
 
 constexpr int CT_ITEMS = SCAN_ITEMS;
 constexpr int CT_TILE = SCAN_TILE;  // slots per contraction tile
 
-// the borrowed duplicate's out-edge moves to the home copy
+// the borrowed duplicate's out-edge (and its target's character) moves to
+// the home copy; the target keeps its has-predecessor bit
 __global__ void k_splice(const uint32_t *__restrict__ pair_home, const uint32_t *__restrict__ pair_borrow, uint32_t x,
-                         uint32_t *__restrict__ succ, uint32_t *__restrict__ pred)
+                         uint64_t *__restrict__ link)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= x) return;
-    const uint32_t h = pair_home[i], b = pair_borrow[i];
-    const uint32_t t = succ[b];
-    succ[h] = t;
-    if (t != NONE) pred[t] = h;
+    link[pair_home[i]] = link[pair_borrow[i]];
 }
 
 // list index for every flagged lane: one atomic per wave
@@ -626,10 +904,16 @@ __device__ __forceinline__ uint32_t wave_append(bool flag, uint32_t *counter)
 // counters in the context's small buffer
 enum { C_LIST = 0, C_ERR = 1, C_ACTIVE = 2, C_NONCONV = 3, C_CYC = 4, C_CUT = 5, C_SPLIT = 6 };
 
+__device__ __forceinline__ bool live_head(const uint8_t *__restrict__ flags, const uint32_t *__restrict__ hasp,
+                                          uint32_t s)
+{
+    return !(flags[s] & F_BORROW) && !has_pred(hasp, s);
+}
+
 // one walk per splitter: segment length, next splitter; per tile the sum
 // of (1 << 32 | length) over heads whose unitig is a single segment (the
 // other heads add their count here and their length in k_chain_tail)
-__global__ __launch_bounds__(BLOCK) void k_seg_count(const uint32_t *__restrict__ succ, const uint32_t *__restrict__ pred,
+__global__ __launch_bounds__(BLOCK) void k_seg_count(const uint64_t *__restrict__ link, const uint32_t *__restrict__ hasp,
                                                      const uint8_t *__restrict__ flags, uint32_t m, uint32_t salt,
                                                      uint32_t shift, uint32_t *__restrict__ seglen,
                                                      uint32_t *__restrict__ segnext, uint32_t *__restrict__ prevseg,
@@ -650,7 +934,7 @@ __global__ __launch_bounds__(BLOCK) void k_seg_count(const uint32_t *__restrict_
         const uint64_t s64 = base + (uint64_t)j * BLOCK + threadIdx.x;
         hd[j] = rd[j] = false;
         if (s64 < m && !(flags[s64] & F_BORROW)) {
-            hd[j] = pred[s64] == NONE;
+            hd[j] = !has_pred(hasp, (uint32_t)s64);
             rd[j] = !hd[j] && sampled((uint32_t)s64, salt, shift);
         }
         nr += rd[j];
@@ -679,9 +963,9 @@ __global__ __launch_bounds__(BLOCK) void k_seg_count(const uint32_t *__restrict_
             uint32_t v = s, r = 0;
             do {
                 r++;
-                v = succ[v];
+                v = (uint32_t)link[v];
                 // in-degree <= 1 bounds every walk by m; the cap only guards
-                // the GPU against a corrupted successor array
+                // the GPU against a corrupted link array
                 if (r > m) {
                     atomicOr(cnt + C_ERR, 1u);
                     v = NONE;
@@ -718,7 +1002,7 @@ __global__ void k_chain_init(const uint32_t *__restrict__ list, uint32_t n, cons
 }
 
 // one synchronous pointer-jumping round: (ancestor, distance) pairs
-__global__ void k_chain_round(const uint32_t *__restrict__ list, uint32_t n, const uint32_t *__restrict__ pred,
+__global__ void k_chain_round(const uint32_t *__restrict__ list, uint32_t n, const uint32_t *__restrict__ hasp,
                               const uint64_t *__restrict__ cin, uint64_t *__restrict__ cout,
                               uint32_t *__restrict__ active)
 {
@@ -728,13 +1012,13 @@ __global__ void k_chain_round(const uint32_t *__restrict__ list, uint32_t n, con
         const uint32_t r = list[i];
         const uint64_t c = cin[r];
         const uint32_t a = (uint32_t)c;
-        if (pred[a] == NONE) {
+        if (!has_pred(hasp, a)) {
             cout[r] = c;
         } else {
             const uint64_t c2 = cin[a];
             const uint32_t a2 = (uint32_t)c2;
             cout[r] = (uint64_t)a2 | ((uint64_t)((uint32_t)(c >> 32) + (uint32_t)(c2 >> 32)) << 32);
-            act = pred[a2] != NONE;
+            act = has_pred(hasp, a2);
         }
     }
     const uint64_t bal = __ballot(act);
@@ -742,7 +1026,7 @@ __global__ void k_chain_round(const uint32_t *__restrict__ list, uint32_t n, con
 }
 
 // the tail segment of a multi-segment unitig gives the unitig length
-__global__ void k_chain_tail(const uint32_t *__restrict__ list, uint32_t n, const uint32_t *__restrict__ pred,
+__global__ void k_chain_tail(const uint32_t *__restrict__ list, uint32_t n, const uint32_t *__restrict__ hasp,
                              const uint64_t *__restrict__ chain, const uint32_t *__restrict__ seglen,
                              const uint32_t *__restrict__ segnext, uint32_t *__restrict__ ulen,
                              uint64_t *__restrict__ tile_sum, uint32_t *__restrict__ nonconv)
@@ -753,7 +1037,7 @@ __global__ void k_chain_tail(const uint32_t *__restrict__ list, uint32_t n, cons
         const uint32_t r = list[i];
         const uint64_t c = chain[r];
         const uint32_t a = (uint32_t)c;
-        if (pred[a] != NONE) {
+        if (has_pred(hasp, a)) {
             bad = true;  // no head behind this chain: an isolated cycle
         } else if (segnext[r] == NONE) {
             const uint32_t len = (uint32_t)(c >> 32) + seglen[r];
@@ -768,26 +1052,25 @@ __global__ void k_chain_tail(const uint32_t *__restrict__ list, uint32_t n, cons
 // ---- isolated cycles (only when some k-mers are not reached from a head) --
 
 // marks every k-mer of a segment that hangs off a head
-__global__ __launch_bounds__(BLOCK) void k_cover(const uint32_t *__restrict__ succ, const uint32_t *__restrict__ pred,
+__global__ __launch_bounds__(BLOCK) void k_cover(const uint64_t *__restrict__ link, const uint32_t *__restrict__ hasp,
                                                  uint8_t *__restrict__ flags, const uint64_t *__restrict__ chain,
                                                  const uint32_t *__restrict__ seglen, uint32_t m, uint32_t salt,
                                                  uint32_t shift)
 {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= m || (flags[s] & F_BORROW)) return;
-    const bool head = pred[s] == NONE;
-    if (!head) {
+    if (has_pred(hasp, s)) {
         if (!sampled(s, salt, shift)) return;
-        if (pred[(uint32_t)chain[s]] != NONE) return;
+        if (has_pred(hasp, (uint32_t)chain[s])) return;
     }
     uint32_t v = s;
     for (uint32_t i = 0, n = seglen[s]; i < n && v != NONE; i++) {
         flags[v] |= F_VISIT;
-        v = succ[v];
+        v = (uint32_t)link[v];
     }
 }
 
-__global__ void k_cycle_init(const uint32_t *__restrict__ succ, uint8_t *__restrict__ flags, uint32_t m,
+__global__ void k_cycle_init(const uint64_t *__restrict__ link, uint8_t *__restrict__ flags, uint32_t m,
                              uint64_t *__restrict__ lab, uint32_t *__restrict__ list, uint32_t *__restrict__ n_list)
 {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -795,7 +1078,7 @@ __global__ void k_cycle_init(const uint32_t *__restrict__ succ, uint8_t *__restr
     const uint32_t li = wave_append(cyc, n_list);
     if (!cyc) return;
     flags[v] |= F_CYCLE;
-    lab[v] = (uint64_t)v | ((uint64_t)succ[v] << 32);
+    lab[v] = (uint64_t)v | ((uint64_t)(uint32_t)link[v] << 32);
     list[li] = v;
 }
 
@@ -816,19 +1099,28 @@ __global__ void k_cycle_round(const uint32_t *__restrict__ list, uint32_t n, con
     lout[v] = (uint64_t)best | ((uint64_t)j2 << 32);
 }
 
+// predecessors of cycle vertices (the main path keeps only a bitmap)
+__global__ void k_cycle_pred(const uint32_t *__restrict__ list, uint32_t n, const uint64_t *__restrict__ link,
+                             uint32_t *__restrict__ pred)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t v = list[i];
+    pred[(uint32_t)link[v]] = v;
+}
+
 // each cycle loses the edge into its smallest k-mer
 __global__ void k_cycle_cut(const uint32_t *__restrict__ list, uint32_t n, const uint64_t *__restrict__ lab,
-                            uint32_t *__restrict__ succ, uint32_t *__restrict__ pred, uint8_t *__restrict__ flags,
-                            uint32_t *__restrict__ n_cut)
+                            const uint32_t *__restrict__ pred, uint64_t *__restrict__ link,
+                            uint32_t *__restrict__ hasp, uint8_t *__restrict__ flags, uint32_t *__restrict__ n_cut)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t v = list[i];
     flags[v] &= (uint8_t)~F_CYCLE;
     if ((uint32_t)lab[v] == v) {
-        const uint32_t p = pred[v];
-        succ[p] = NONE;
-        pred[v] = NONE;
+        link[pred[v]] = NO_LINK;
+        atomicAnd(&hasp[v >> 5], ~(1u << (v & 31)));
         atomicAdd(n_cut, 1u);
     }
 }
@@ -846,18 +1138,11 @@ __global__ void k_clear_flag(uint8_t *__restrict__ flags, uint32_t m, uint8_t bi
 // ... + len_u + k): its first k-mer in full, the last character of every
 // later k-mer, '\n'.
 
-template <int W>
-__device__ __forceinline__ char last_char(const Kmer<W> &x, uint32_t k)
-{
-    const uint32_t i = k - 1;
-    return base_char((uint32_t)(x.w[i / 32] >> (2 * (i % 32))) & 3u);
-}
-
 // per tile: exclusive prefix of (1 << 32 | length) over heads -> text
 // offsets; every head spells its own segment
 template <int W>
-__global__ __launch_bounds__(BLOCK) void k_head_emit(const uint64_t *__restrict__ kmer, const uint32_t *__restrict__ succ,
-                                                     const uint32_t *__restrict__ pred, const uint8_t *__restrict__ flags,
+__global__ __launch_bounds__(BLOCK) void k_head_emit(const uint64_t *__restrict__ kmer, const uint64_t *__restrict__ link,
+                                                     const uint32_t *__restrict__ hasp, const uint8_t *__restrict__ flags,
                                                      const uint32_t *__restrict__ seglen,
                                                      const uint32_t *__restrict__ segnext,
                                                      const uint32_t *__restrict__ ulen,
@@ -874,7 +1159,7 @@ __global__ __launch_bounds__(BLOCK) void k_head_emit(const uint64_t *__restrict_
 #pragma unroll
     for (int j = 0; j < CT_ITEMS; j++) {
         const uint64_t s = base + (uint64_t)j * BLOCK + threadIdx.x;
-        hd[j] = s < m && !(flags[s] & F_BORROW) && pred[s] == NONE;
+        hd[j] = s < m && live_head(flags, hasp, (uint32_t)s);
         tile[j * BLOCK + threadIdx.x] = hd[j] ? ((1ull << 32) | ulen[s]) : 0ull;
     }
     __syncthreads();
@@ -905,11 +1190,12 @@ __global__ __launch_bounds__(BLOCK) void k_head_emit(const uint64_t *__restrict_
         if (nxt != NONE) htoff[s] = off;
         char *o = text + off;
         const Kmer<W> x0 = load_kmer<W>(kmer, s, ~0ull);
-        for (uint32_t i = 0; i < k; i++) o[i] = base_char((uint32_t)(x0.w[i / 32] >> (2 * (i % 32))) & 3u);
+        for (uint32_t i = 0; i < k; i++) o[i] = base_char(char_at<W>(x0, i));
         uint32_t v = s;
         for (uint32_t i = 1; i < len; i++) {
-            v = succ[v];
-            o[k - 1 + i] = last_char<W>(load_kmer<W>(kmer, v, ~0ull), k);
+            const uint64_t l = link[v];
+            o[k - 1 + i] = base_char((uint32_t)(l >> 32) & 3u);
+            v = (uint32_t)l;
         }
         if (nxt == NONE) o[k - 1 + len] = '\n';
     }
@@ -917,7 +1203,7 @@ __global__ __launch_bounds__(BLOCK) void k_head_emit(const uint64_t *__restrict_
 
 // segments of sampled splitters: positions from the ranked chain
 template <int W>
-__global__ void k_rand_emit(const uint64_t *__restrict__ kmer, const uint32_t *__restrict__ succ,
+__global__ void k_rand_emit(const uint64_t *__restrict__ kmer, const uint64_t *__restrict__ link,
                             const uint32_t *__restrict__ list, uint32_t n, const uint64_t *__restrict__ chain,
                             const uint32_t *__restrict__ seglen, const uint32_t *__restrict__ segnext,
                             const uint64_t *__restrict__ htoff, uint32_t k, char *__restrict__ text)
@@ -929,17 +1215,19 @@ __global__ void k_rand_emit(const uint64_t *__restrict__ kmer, const uint32_t *_
     const uint32_t a = (uint32_t)c, d = (uint32_t)(c >> 32);
     char *o = text + htoff[a] + (k - 1) + d;
     const uint32_t len = seglen[r];
+    o[0] = base_char(char_at<W>(load_kmer<W>(kmer, r, ~0ull), k - 1));
     uint32_t v = r;
-    for (uint32_t p = 0; p < len; p++) {
-        o[p] = last_char<W>(load_kmer<W>(kmer, v, ~0ull), k);
-        v = succ[v];
+    for (uint32_t p = 1; p < len; p++) {
+        const uint64_t l = link[v];
+        o[p] = base_char((uint32_t)(l >> 32) & 3u);
+        v = (uint32_t)l;
     }
     if (segnext[r] == NONE) o[len] = '\n';
 }
 
 // text offset of each set's first unitig: the prefix at the set's first slot
 __global__ void k_set_offsets(const uint32_t *__restrict__ boff, uint32_t n_sets, uint32_t B,
-                              const uint32_t *__restrict__ pred, const uint8_t *__restrict__ flags,
+                              const uint32_t *__restrict__ hasp, const uint8_t *__restrict__ flags,
                               const uint32_t *__restrict__ ulen, const uint64_t *__restrict__ tile_off, uint32_t m,
                               uint64_t total, uint64_t *__restrict__ out)
 {
@@ -951,9 +1239,16 @@ __global__ void k_set_offsets(const uint32_t *__restrict__ boff, uint32_t n_sets
         const uint32_t t = sb / CT_TILE;
         acc = tile_off[t];
         for (uint32_t x = t * CT_TILE; x < sb; x++)
-            if (!(flags[x] & F_BORROW) && pred[x] == NONE) acc += (1ull << 32) | ulen[x];
+            if (live_head(flags, hasp, x)) acc += (1ull << 32) | ulen[x];
     }
     out[s] = acc;
+}
+
+// successor slot of every slot (for checking)
+__global__ void k_link_succ(const uint64_t *__restrict__ link, uint32_t m, uint32_t *__restrict__ succ)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (uint64_t)gridDim.x * blockDim.x)
+        succ[i] = (uint32_t)link[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -991,11 +1286,18 @@ struct uc_ctx {
     uint64_t n = 0, m = 0, x = 0, largest = 0;
     uint64_t top_mask = ~0ull, pm0 = 0, pm1 = 0;
     std::vector<uint64_t> set_off;   // n_sets + 1
-    std::vector<uint32_t> boff_h;    // n_sets*B + 1
+    std::vector<uint32_t> boff_h;    // n_sets*B + 1 coarse bucket rows
+    std::vector<uint32_t> set_sub;   // fine buckets per coarse bucket, per set
+    std::vector<uint32_t> cb_first;  // first fine bucket of every coarse bucket
+    std::vector<uint32_t> fboff_h;   // fine bucket rows
     bool loaded = false, bucketed = false, edged = false, coarsened = false, emitted = false;
     bool edges_ready = false;
     std::vector<uint64_t> et_toff, et_first;
-    std::vector<uint8_t> et_bits, bucket_done;
+    std::vector<uint32_t> et_cap;
+    std::vector<uint8_t> bucket_done;
+    std::vector<size_t> grp_first, fb_first;  // per coarse bucket: first LDS group / oversized fine bucket
+    uint32_t lds_nb = 64;                     // LDS capacity of the edge launch
+    uint64_t n_fallback = 0, n_groups = 0;
     uint64_t table_slots = 0, n_done = 0;
     uint64_t n_unitigs = 0, n_cycles = 0, text_bytes = 0, n_splitters = 0;
     uint32_t chain_rounds = 0;
@@ -1004,8 +1306,9 @@ struct uc_ctx {
 
     // device buffers
     const uint64_t *kin = nullptr;  // k-mers in 64-bit layout (the caller's buffer or `in`)
-    DBuf otable, in, stage, bkt, tiles, hist, xhist, scan_tmp, kmer, flags, src, pair_h, pair_b, boff, sethb, setts, etiles,
-        toff, tbits, table, hit, succ, pred, seglen, segnext, prevseg, chainA, chainB, cycB, list, clist, ulen, tsum,
+    DBuf otable, in, stage, bkt, tiles, hist, xhist, kmer2, flags2, src2, sub, perm, tiles2, hist2, cbhb, cbts,
+        cbfirst, fboff, egroups, fbbase, scan_tmp, kmer, flags, src, pair_h, pair_b, boff, sethb, setts, etiles,
+        toff, tcap, table, hit, link, hasp, pred, seglen, segnext, prevseg, chainA, chainB, cycB, list, clist, ulen, tsum,
         htoff, setoff, text, small;
     // contraction state
     uint64_t *chain = nullptr;  // ranked chain entries (chainA or chainB)
@@ -1286,6 +1589,19 @@ template <int W>
 int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
 {
     const uint32_t B = c->B, S = c->n_sets;
+    const uint32_t W_ = W;
+    // fine buckets per coarse bucket, per set: about FINE_TARGET slots each
+    const uint64_t fine_target = W_ == 1 ? 512 : 384;
+    c->set_sub.assign(S, 1);
+    bool any_fine = false;
+    for (uint32_t s = 0; s < S; s++) {
+        const uint64_t ns = c->set_off[s + 1] - c->set_off[s];
+        const uint64_t want = (ns * 112 / 100 + (uint64_t)B * fine_target - 1) / ((uint64_t)B * fine_target);
+        uint32_t sub = 1;
+        while (sub < want && sub < MAX_SUB) sub <<= 1;
+        c->set_sub[s] = sub;
+        any_fine |= sub > 1;
+    }
     // tiles: every set gets at least one (possibly empty) tile so each bucket
     // row has a start in the histogram
     std::vector<BTile> tiles;
@@ -1305,6 +1621,8 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
             bt.set = s;
             bt.ts = ts;
             bt.tlocal = t;
+            bt.fmask = B * c->set_sub[s] - 1;
+            bt.pad = 0;
             tiles.push_back(bt);
         }
         hb += (uint64_t)B * ts;
@@ -1312,7 +1630,7 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     const uint64_t T = tiles.size();
     CHK(ensure(c, c->tiles, T * sizeof(BTile)));
     CHK(h2d(c, c->tiles.p, tiles.data(), T * sizeof(BTile)));
-    CHK(ensure(c, c->bkt, c->n * 4));
+    CHK(ensure(c, c->bkt, c->n * 8));
     CHK(ensure(c, c->hist, (hb + 1) * 4));
     CHK(ensure(c, c->xhist, (T + 1) * 4));
     CHK(ensure(c, c->sethb, S * 8));
@@ -1333,9 +1651,8 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     p.table = table;
 
     const uint64_t *kin = c->kin;
-    KLAUNCH(c, "k_lmin_count", k_lmin_count<W>, ((uint32_t)T, BLOCK, (B + 1) * 4), kin, P<BTile>(c->tiles), p, c->top_mask,
-                                                                     P<uint32_t>(c->bkt), P<uint32_t>(c->hist),
-                                                                     P<uint32_t>(c->xhist));
+    KLAUNCH(c, "k_lmin_count", k_lmin_count<W>, ((uint32_t)T, BLOCK, (B + 1) * 4), kin, P<BTile>(c->tiles), p,
+            c->top_mask, P<uint2>(c->bkt), P<uint32_t>(c->hist), P<uint32_t>(c->xhist));
     // totals at hist[hb] and xhist[T]
     CHK(scan_excl<uint32_t>(c, P<uint32_t>(c->hist), P<uint32_t>(c->hist), hb, P<uint32_t>(c->hist) + hb, c->scan_tmp));
     CHK(scan_excl<uint32_t>(c, P<uint32_t>(c->xhist), P<uint32_t>(c->xhist), T, P<uint32_t>(c->xhist) + T,
@@ -1347,25 +1664,93 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     c->x = tot[1];
     if (c->m >= 0xFFFFFFFFull) return fail(c, UC_E_LIMIT, "more than 2^32-2 placements");
     const uint64_t G = (uint64_t)S * B;
-    CHK(ensure(c, c->kmer, std::max<uint64_t>(c->m, 1) * W * 8));
-    CHK(ensure(c, c->flags, c->m + 64));
-    CHK(ensure(c, c->src, c->m * 4));
+    const bool keep_src = (c->opts.flags & UC_FLAG_KEEP_SRC) != 0;
+    DBuf &k1 = any_fine ? c->kmer2 : c->kmer, &f1 = any_fine ? c->flags2 : c->flags, &s1 = any_fine ? c->src2 : c->src;
+    CHK(ensure(c, k1, std::max<uint64_t>(c->m, 1) * W * 8));
+    CHK(ensure(c, f1, c->m + 64));
+    if (keep_src) CHK(ensure(c, s1, c->m * 4));
+    CHK(ensure(c, c->sub, c->m * 2 + 8));
     CHK(ensure(c, c->pair_h, c->x * 4));
     CHK(ensure(c, c->pair_b, c->x * 4));
     CHK(ensure(c, c->boff, (G + 1) * 4));
     const uint32_t nbits = (uint32_t)__builtin_ctz(B);
     const size_t lds = (size_t)(B + 1) * (WAVES + 1) * 4;
-    KLAUNCH(c, "k_lmin_scatter", k_lmin_scatter<W>, ((uint32_t)T, BLOCK, lds), 
-        kin, P<BTile>(c->tiles), B, nbits, c->top_mask, P<uint32_t>(c->bkt), P<uint32_t>(c->hist),
-        P<uint32_t>(c->xhist), P<uint64_t>(c->kmer), P<uint8_t>(c->flags), P<uint32_t>(c->src),
-        P<uint32_t>(c->pair_h), P<uint32_t>(c->pair_b));
-    KLAUNCH(c, "k_gather_boff", k_gather_boff, (grid_for(G + 1), BLOCK, 0), P<uint32_t>(c->hist), P<uint64_t>(c->sethb),
-                                                           P<uint32_t>(c->setts), S, B, (uint32_t)c->m,
-                                                           P<uint32_t>(c->boff));
+    KLAUNCH(c, "k_lmin_scatter", k_lmin_scatter<W>, ((uint32_t)T, BLOCK, lds), kin, P<BTile>(c->tiles), B, nbits,
+            c->top_mask, P<uint2>(c->bkt), P<uint32_t>(c->hist), P<uint32_t>(c->xhist), P<uint64_t>(k1),
+            P<uint8_t>(f1), keep_src ? P<uint32_t>(s1) : nullptr, P<uint32_t>(c->pair_h), P<uint32_t>(c->pair_b),
+            P<uint16_t>(c->sub));
+    KLAUNCH(c, "k_gather_boff", k_gather_boff, (grid_for(G + 1), BLOCK, 0), P<uint32_t>(c->hist),
+            P<uint64_t>(c->sethb), P<uint32_t>(c->setts), S, B, (uint32_t)c->m, P<uint32_t>(c->boff));
     c->boff_h.resize(G + 1);
     CHK(d2h(c, c->boff_h.data(), c->boff.p, (G + 1) * 4));
     c->largest = 0;
     for (uint64_t g = 0; g < G; g++) c->largest = std::max<uint64_t>(c->largest, c->boff_h[g + 1] - c->boff_h[g]);
+
+    // fine buckets: stable sort of each coarse bucket's slots by fine index
+    c->cb_first.assign(G + 1, 0);
+    for (uint64_t g = 0; g < G; g++) c->cb_first[g + 1] = c->cb_first[g] + c->set_sub[g / B];
+    const uint64_t F = c->cb_first[G];
+    if (!any_fine) {
+        c->fboff_h = c->boff_h;
+    } else {
+        std::vector<BTile> t2;
+        std::vector<uint64_t> cb_hbase(G);
+        std::vector<uint32_t> cb_ts(G);
+        uint64_t h2 = 0;
+        for (uint64_t g = 0; g < G; g++) {
+            const uint32_t Sg = c->set_sub[g / B];
+            const uint64_t b0 = c->boff_h[g], cnt = c->boff_h[g + 1] - b0;
+            const uint32_t ts = (uint32_t)std::max<uint64_t>(1, (cnt + SUB_TILE - 1) / SUB_TILE);
+            cb_hbase[g] = h2;
+            cb_ts[g] = ts;
+            for (uint32_t t = 0; t < ts; t++) {
+                BTile bt;
+                bt.kbegin = b0 + (uint64_t)t * SUB_TILE;
+                bt.hbase = h2;
+                bt.count = (uint32_t)std::min<uint64_t>(SUB_TILE, cnt - std::min<uint64_t>(cnt, (uint64_t)t * SUB_TILE));
+                bt.set = (uint32_t)g;
+                bt.ts = ts;
+                bt.tlocal = t;
+                bt.fmask = Sg;
+                bt.pad = 0;
+                t2.push_back(bt);
+            }
+            h2 += (uint64_t)Sg * ts;
+        }
+        const uint64_t T2 = t2.size();
+        CHK(ensure(c, c->tiles2, T2 * sizeof(BTile)));
+        CHK(h2d(c, c->tiles2.p, t2.data(), T2 * sizeof(BTile)));
+        CHK(ensure(c, c->hist2, (h2 + 1) * 4));
+        CHK(ensure(c, c->cbhb, G * 8));
+        CHK(ensure(c, c->cbts, G * 4));
+        CHK(ensure(c, c->cbfirst, (G + 1) * 4));
+        CHK(h2d(c, c->cbhb.p, cb_hbase.data(), G * 8));
+        CHK(h2d(c, c->cbts.p, cb_ts.data(), G * 4));
+        CHK(h2d(c, c->cbfirst.p, c->cb_first.data(), (G + 1) * 4));
+        uint32_t smax = 1;
+        for (uint32_t sb : c->set_sub) smax = std::max(smax, sb);
+        KLAUNCH(c, "k_sub_count", k_sub_count, ((uint32_t)T2, BLOCK, smax * 4), P<uint16_t>(c->sub),
+                P<BTile>(c->tiles2), P<uint32_t>(c->hist2));
+        CHK(scan_excl<uint32_t>(c, P<uint32_t>(c->hist2), P<uint32_t>(c->hist2), h2, P<uint32_t>(c->hist2) + h2,
+                                c->scan_tmp));
+        CHK(ensure(c, c->kmer, std::max<uint64_t>(c->m, 1) * W * 8));
+        CHK(ensure(c, c->flags, c->m + 64));
+        if (keep_src) CHK(ensure(c, c->src, c->m * 4));
+        CHK(ensure(c, c->perm, c->m * 4));
+        KLAUNCH(c, "k_sub_scatter", k_sub_scatter<W>, ((uint32_t)T2, BLOCK, (size_t)smax * (WAVES + 1) * 4),
+                P<BTile>(c->tiles2), P<uint16_t>(c->sub), P<uint32_t>(c->hist2), P<uint64_t>(c->kmer2),
+                P<uint8_t>(c->flags2), keep_src ? P<uint32_t>(c->src2) : nullptr, P<uint64_t>(c->kmer),
+                P<uint8_t>(c->flags), keep_src ? P<uint32_t>(c->src) : nullptr, P<uint32_t>(c->perm));
+        if (c->x)
+            KLAUNCH(c, "k_pair_remap", k_pair_remap, (grid_for(c->x), BLOCK, 0), P<uint32_t>(c->pair_h),
+                    P<uint32_t>(c->pair_b), (uint32_t)c->x, P<uint32_t>(c->perm));
+        CHK(ensure(c, c->fboff, (F + 1) * 4));
+        KLAUNCH(c, "k_gather_fine", k_gather_fine, ((uint32_t)(G + 1), 64, 0), P<uint32_t>(c->hist2),
+                P<uint64_t>(c->cbhb), P<uint32_t>(c->cbts), P<uint32_t>(c->cbfirst), (uint32_t)G, (uint32_t)c->m,
+                P<uint32_t>(c->fboff));
+        c->fboff_h.resize(F + 1);
+        CHK(d2h(c, c->fboff_h.data(), c->fboff.p, (F + 1) * 4));
+    }
     c->l = l;
     c->bucketed = true;
     return UC_OK;
@@ -1382,83 +1767,162 @@ int do_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
 
 // ---- edges -----------------------------------------------------------------
 
-uint32_t table_bits(uint64_t count, uint32_t load_pct)
+// entries of a table holding `count` keys at load_pct percent
+uint64_t table_cap(uint64_t count, uint32_t load_pct)
 {
-    uint64_t need = (count * 100 + load_pct - 1) / load_pct;
-    uint32_t b = 6;
-    while ((1ull << b) < need) b++;
-    return b;
+    return std::max<uint64_t>(64, (count * 100 + load_pct - 1) / load_pct);
 }
 
-template <int W>
-int launch_edges(uc_ctx *c, size_t first, size_t count)
+int clear_table(uc_ctx *c, TEntry *t, uint64_t n)
 {
-    if (count == 0) return UC_OK;
-    const ETile *tiles = P<ETile>(c->etiles) + first;
-    KLAUNCH(c, "k_edge_insert", k_edge_insert<W>, (xcd_grid(count), BLOCK, 0), P<uint64_t>(c->kmer), P<uint8_t>(c->flags), tiles,
-                                                                P<uint32_t>(c->boff), P<uint64_t>(c->toff),
-                                                                P<uint8_t>(c->tbits), P<uint64_t>(c->table), c->pm0,
-                                                                c->pm1, tile_arg(c, count));
-    KLAUNCH(c, "k_edge_lookup", k_edge_lookup<W>, (xcd_grid(count), BLOCK, 0), P<uint64_t>(c->kmer), P<uint8_t>(c->flags), tiles,
-                                                                P<uint32_t>(c->boff), P<uint64_t>(c->toff),
-                                                                P<uint8_t>(c->tbits), P<uint64_t>(c->table), c->pm0,
-                                                                c->pm1, P<uint32_t>(c->hit), tile_arg(c, count));
-    KLAUNCH(c, "k_edge_resolve", k_edge_resolve, (xcd_grid(count), BLOCK, 0), tiles, P<uint32_t>(c->boff), P<uint64_t>(c->toff),
-                                                             P<uint64_t>(c->table), P<uint32_t>(c->hit),
-                                                             P<uint32_t>(c->succ), P<uint32_t>(c->pred), tile_arg(c, count));
+    if (n == 0) return UC_OK;
+    KLAUNCH(c, "k_table_clear", k_table_clear, (std::min<uint32_t>(grid_for(n), 65536), BLOCK, 0), t, n);
     return UC_OK;
 }
 
-// table layout and bucket tiles, once per bucketing result
+template <int W>
+int launch_global_edges(uc_ctx *c, size_t first, size_t count)
+{
+    if (count == 0) return UC_OK;
+    const ETile *tiles = P<ETile>(c->etiles) + first;
+    KLAUNCH(c, "k_edge_insert", k_edge_insert<W>, (xcd_grid(count), BLOCK, 0), P<uint64_t>(c->kmer),
+            P<uint8_t>(c->flags), tiles, P<uint32_t>(c->fbbase), P<uint64_t>(c->toff), P<uint32_t>(c->tcap),
+            P<TEntry>(c->table), c->k, c->pm0, c->pm1, tile_arg(c, count));
+    KLAUNCH(c, "k_edge_lookup", k_edge_lookup<W>, (xcd_grid(count), BLOCK, 0), P<uint64_t>(c->kmer),
+            P<uint8_t>(c->flags), tiles, P<uint32_t>(c->fbbase), P<uint64_t>(c->toff), P<uint32_t>(c->tcap),
+            P<TEntry>(c->table), c->pm0, c->pm1, P<uint32_t>(c->hit), tile_arg(c, count));
+    KLAUNCH(c, "k_edge_resolve", k_edge_resolve, (xcd_grid(count), BLOCK, 0), tiles, P<uint32_t>(c->fbbase),
+            P<uint64_t>(c->toff), P<TEntry>(c->table), P<uint32_t>(c->hit), P<uint64_t>(c->link),
+            P<uint32_t>(c->hasp), tile_arg(c, count));
+    return UC_OK;
+}
+
+template <int W>
+int launch_lds_edges(uc_ctx *c, size_t first, size_t count)
+{
+    if (count == 0) return UC_OK;
+    KLAUNCH(c, "k_edge_lds", k_edge_lds<W>, ((uint32_t)count, BLOCK, edge_lds_bytes<W>(c->lds_nb)),
+            P<uint64_t>(c->kmer), P<uint8_t>(c->flags), P<EGroup>(c->egroups) + first, c->lds_nb, c->k, c->pm0,
+            c->pm1, P<uint64_t>(c->link), P<uint32_t>(c->hasp));
+    return UC_OK;
+}
+
+// Work layout of the edge phase, once per bucketing result: consecutive
+// fine buckets of one coarse bucket are packed into LDS groups of at most
+// LDS_GROUP slots; a fine bucket larger than that gets a table of its own in
+// HBM (the reference's layout: one open-addressing table per bucket,
+// assembler.h:113-150).
 int prepare_edges(uc_ctx *c)
 {
     if (c->edges_ready) return UC_OK;
     const uint64_t G = (uint64_t)c->n_sets * c->B;
     const uint32_t load = c->opts.load_pct ? c->opts.load_pct : 50;
     const bool plain = c->opts.mode == UC_MODE_PLAIN;
-    // batched: one table per bucket; plain: one shared table sized for the
-    // largest bucket (assembler.h:113-118, driver.cpp:465)
-    c->et_toff.assign(G + 1, 0);
-    c->et_bits.assign(G, 0);
-    uint64_t total = 0;
-    const uint32_t shared_bits = table_bits(std::max<uint64_t>(c->largest, 1), load);
-    if (shared_bits > 31) return fail(c, UC_E_LIMIT, "bucket too large for a 2^31-slot table");
+    const uint32_t group_max = LDS_GROUP;
+    std::vector<EGroup> groups;
+    std::vector<uint32_t> fb_base;
+    std::vector<uint64_t> fb_cnt;
+    c->grp_first.assign(G + 1, 0);
+    c->fb_first.assign(G + 1, 0);
+    uint32_t biggest_group = 0;
     for (uint64_t g = 0; g < G; g++) {
-        const uint64_t cnt = c->boff_h[g + 1] - c->boff_h[g];
-        c->et_bits[g] = plain ? (uint8_t)shared_bits : (uint8_t)table_bits(std::max<uint64_t>(cnt, 1), load);
-        c->et_toff[g] = plain ? 0 : total;
-        if (!plain) total += 1ull << c->et_bits[g];
+        c->grp_first[g] = groups.size();
+        c->fb_first[g] = fb_base.size();
+        uint32_t gb = NONE, ge = NONE;
+        for (uint32_t f = c->cb_first[g]; f < c->cb_first[g + 1]; f++) {
+            const uint32_t b0 = c->fboff_h[f], b1 = c->fboff_h[f + 1];
+            if (b1 == b0) continue;
+            if (b1 - b0 > group_max) {
+                fb_base.push_back(b0);
+                fb_cnt.push_back(b1 - b0);
+                continue;
+            }
+            if (gb != NONE && b1 - gb <= group_max && ge == b0) {
+                ge = b1;
+            } else {
+                if (gb != NONE) groups.push_back({gb, ge});
+                gb = b0;
+                ge = b1;
+            }
+        }
+        if (gb != NONE) groups.push_back({gb, ge});
     }
-    if (plain) total = 1ull << shared_bits;
+    c->grp_first[G] = groups.size();
+    c->fb_first[G] = fb_base.size();
+    for (const EGroup &e : groups) biggest_group = std::max(biggest_group, e.end - e.begin);
+    uint32_t nb = 64;
+    while (nb < biggest_group) nb <<= 1;
+    c->lds_nb = nb;
+    CHK(ensure(c, c->egroups, std::max<size_t>(groups.size(), 1) * sizeof(EGroup)));
+    CHK(h2d(c, c->egroups.p, groups.data(), groups.size() * sizeof(EGroup)));
+
+    // HBM tables for oversized fine buckets; plain form shares one table
+    // sized for the largest of them
+    const size_t NF = fb_base.size();
+    uint64_t fb_largest = 0;
+    for (uint64_t x : fb_cnt) fb_largest = std::max(fb_largest, x);
+    if (fb_largest > OWN_MASK) return fail(c, UC_E_LIMIT, "a bucket holds more than 2^30 slots");
+    c->et_toff.assign(NF + 1, 0);
+    c->et_cap.assign(NF, 0);
+    uint64_t total = 0;
+    const uint64_t shared_cap = table_cap(fb_largest, load);
+    if (shared_cap > 0xFFFFFFFFull) return fail(c, UC_E_LIMIT, "bucket too large for one table");
+    for (size_t f = 0; f < NF; f++) {
+        c->et_cap[f] = (uint32_t)(plain ? shared_cap : table_cap(fb_cnt[f], load));
+        c->et_toff[f] = plain ? 0 : total;
+        if (!plain) total += c->et_cap[f];
+    }
+    if (plain && NF) total = shared_cap;
     c->table_slots = total;
-    // tiles of at most EDGE_TILE slots, bucket by bucket
     std::vector<ETile> et;
-    c->et_first.assign(G + 1, 0);
-    for (uint64_t g = 0; g < G; g++) {
-        c->et_first[g] = et.size();
-        for (uint64_t b = c->boff_h[g]; b < c->boff_h[g + 1]; b += EDGE_TILE) {
+    c->et_first.assign(NF + 1, 0);
+    for (size_t f = 0; f < NF; f++) {
+        c->et_first[f] = et.size();
+        for (uint64_t b = fb_base[f]; b < fb_base[f] + fb_cnt[f]; b += EDGE_TILE) {
             ETile t;
-            t.bucket = (uint32_t)g;
+            t.bucket = (uint32_t)f;
             t.begin = (uint32_t)b;
-            t.end = (uint32_t)std::min<uint64_t>(c->boff_h[g + 1], b + EDGE_TILE);
+            t.end = (uint32_t)std::min<uint64_t>(fb_base[f] + fb_cnt[f], b + EDGE_TILE);
             t.pad = 0;
             et.push_back(t);
         }
     }
-    c->et_first[G] = et.size();
-    CHK(ensure(c, c->etiles, et.size() * sizeof(ETile)));
+    c->et_first[NF] = et.size();
+    CHK(ensure(c, c->etiles, std::max<size_t>(et.size(), 1) * sizeof(ETile)));
     CHK(h2d(c, c->etiles.p, et.data(), et.size() * sizeof(ETile)));
-    CHK(ensure(c, c->toff, (G + 1) * 8));
-    CHK(ensure(c, c->tbits, G + 8));
-    CHK(h2d(c, c->toff.p, c->et_toff.data(), (G + 1) * 8));
-    CHK(h2d(c, c->tbits.p, c->et_bits.data(), G));
-    CHK(ensure(c, c->table, total * 8));
+    CHK(ensure(c, c->fbbase, NF * 4 + 8));
+    CHK(h2d(c, c->fbbase.p, fb_base.data(), NF * 4));
+    CHK(ensure(c, c->toff, (NF + 1) * 8));
+    CHK(ensure(c, c->tcap, NF * 4 + 8));
+    CHK(h2d(c, c->toff.p, c->et_toff.data(), (NF + 1) * 8));
+    CHK(h2d(c, c->tcap.p, c->et_cap.data(), NF * 4));
+    CHK(ensure(c, c->table, total * sizeof(TEntry)));
     CHK(ensure(c, c->hit, c->m * 4));
-    CHK(ensure(c, c->succ, c->m * 4));
-    CHK(ensure(c, c->pred, c->m * 4));
+    CHK(ensure(c, c->link, c->m * 8));
+    CHK(ensure(c, c->hasp, (c->m + 31) / 32 * 4 + 8));
+    // has-predecessor bits: every bucket sets only its own slots' bits
+    CHK(fill<uint32_t>(c, P<uint32_t>(c->hasp), (c->m + 31) / 32, 0u));
+    c->n_fallback = NF;
+    c->n_groups = groups.size();
     c->bucket_done.assign(G, 0);
     c->n_done = 0;
     c->edges_ready = true;
+    return UC_OK;
+}
+
+// the oversized fine buckets [f0, f1) on the HBM tables
+template <int W>
+int run_fallback(uc_ctx *c, size_t f0, size_t f1)
+{
+    if (f0 == f1) return UC_OK;
+    if (c->opts.mode != UC_MODE_PLAIN) {
+        CHK(clear_table(c, P<TEntry>(c->table) + c->et_toff[f0], c->et_toff[f1 - 1] + c->et_cap[f1 - 1] - c->et_toff[f0]));
+        return launch_global_edges<W>(c, c->et_first[f0], c->et_first[f1] - c->et_first[f0]);
+    }
+    for (size_t f = f0; f < f1; f++) {
+        CHK(clear_table(c, P<TEntry>(c->table), c->et_cap[f]));
+        CHK(launch_global_edges<W>(c, c->et_first[f], c->et_first[f + 1] - c->et_first[f]));
+    }
     return UC_OK;
 }
 
@@ -1469,25 +1933,25 @@ int run_edges(uc_ctx *c, uint32_t bucket)
     const uint64_t G = (uint64_t)c->n_sets * c->B;
     const bool plain = c->opts.mode == UC_MODE_PLAIN;
     if (bucket == UC_ALL_BUCKETS) {
-        CHK(fill<uint32_t>(c, P<uint32_t>(c->pred), c->m, NONE));
+        CHK(fill<uint32_t>(c, P<uint32_t>(c->hasp), (c->m + 31) / 32, 0u));
         if (!plain) {
-            CHK(fill<uint64_t>(c, P<uint64_t>(c->table), c->table_slots, EMPTY));
-            CHK(launch_edges<W>(c, c->et_first[0], c->et_first[G] - c->et_first[0]));
+            CHK(launch_lds_edges<W>(c, 0, c->grp_first[G]));
+            CHK(run_fallback<W>(c, 0, c->fb_first[G]));
         } else {
+            // the reference's per-bucket loop (driver.cpp:479-521)
             for (uint64_t g = 0; g < G; g++) {
-                CHK(fill<uint64_t>(c, P<uint64_t>(c->table), 1ull << c->et_bits[g], EMPTY));
-                CHK(launch_edges<W>(c, c->et_first[g], c->et_first[g + 1] - c->et_first[g]));
+                CHK(launch_lds_edges<W>(c, c->grp_first[g], c->grp_first[g + 1] - c->grp_first[g]));
+                CHK(run_fallback<W>(c, c->fb_first[g], c->fb_first[g + 1]));
             }
         }
         std::fill(c->bucket_done.begin(), c->bucket_done.end(), 1);
         c->n_done = G;
     } else {
         if (bucket >= G) return fail(c, UC_E_ARG, "bucket id out of range");
-        // predecessors of this bucket's slots are written by this bucket only
-        const uint32_t b0 = c->boff_h[bucket], b1 = c->boff_h[bucket + 1];
-        CHK(fill<uint32_t>(c, P<uint32_t>(c->pred) + b0, b1 - b0, NONE));
-        CHK(fill<uint64_t>(c, P<uint64_t>(c->table) + c->et_toff[bucket], 1ull << c->et_bits[bucket], EMPTY));
-        CHK(launch_edges<W>(c, c->et_first[bucket], c->et_first[bucket + 1] - c->et_first[bucket]));
+        // this bucket sets only its own slots' has-predecessor bits, which
+        // start cleared from prepare_edges
+        CHK(launch_lds_edges<W>(c, c->grp_first[bucket], c->grp_first[bucket + 1] - c->grp_first[bucket]));
+        CHK(run_fallback<W>(c, c->fb_first[bucket], c->fb_first[bucket + 1]));
         if (!c->bucket_done[bucket]) {
             c->bucket_done[bucket] = 1;
             c->n_done++;
@@ -1515,7 +1979,7 @@ int contract_pass(uc_ctx *c, uint64_t *live, uint32_t *nonconv)
     const uint64_t T = (c->m + CT_TILE - 1) / CT_TILE;
     uint32_t *cnt = P<uint32_t>(c->small);
     CHK(fill<uint32_t>(c, cnt, 8, 0u));
-    KLAUNCH(c, "k_seg_count", k_seg_count, (xcd_grid(T), BLOCK, 0), P<uint32_t>(c->succ), P<uint32_t>(c->pred),
+    KLAUNCH(c, "k_seg_count", k_seg_count, (xcd_grid(T), BLOCK, 0), P<uint64_t>(c->link), P<uint32_t>(c->hasp),
             P<uint8_t>(c->flags), m, c->salt, c->shift, P<uint32_t>(c->seglen), P<uint32_t>(c->segnext),
             P<uint32_t>(c->prevseg), P<uint32_t>(c->ulen), P<uint32_t>(c->list), cnt, P<uint64_t>(c->tsum),
             tile_arg(c, T));
@@ -1538,7 +2002,7 @@ int contract_pass(uc_ctx *c, uint64_t *live, uint32_t *nonconv)
         for (;;) {
             CHK(fill<uint32_t>(c, cnt + C_ACTIVE, 1, 0u));
             KLAUNCH(c, "k_chain_round", k_chain_round, (grid_for(nl), BLOCK, 0), P<uint32_t>(c->list), nl,
-                    P<uint32_t>(c->pred), ca, cb, cnt + C_ACTIVE);
+                    P<uint32_t>(c->hasp), ca, cb, cnt + C_ACTIVE);
             std::swap(ca, cb);
             rounds++;
             uint32_t active;
@@ -1546,7 +2010,7 @@ int contract_pass(uc_ctx *c, uint64_t *live, uint32_t *nonconv)
             if (active == 0 || rounds >= cap) break;
         }
         KLAUNCH(c, "k_chain_tail", k_chain_tail, (grid_for(nl), BLOCK, 0), P<uint32_t>(c->list), nl,
-                P<uint32_t>(c->pred), ca, P<uint32_t>(c->seglen), P<uint32_t>(c->segnext), P<uint32_t>(c->ulen),
+                P<uint32_t>(c->hasp), ca, P<uint32_t>(c->seglen), P<uint32_t>(c->segnext), P<uint32_t>(c->ulen),
                 P<uint64_t>(c->tsum), cnt + C_NONCONV);
         CHK(d2h(c, &bad, cnt + C_NONCONV, 4));
     }
@@ -1569,12 +2033,12 @@ int cut_cycles(uc_ctx *c, uint64_t *n_cut)
 {
     const uint32_t m = (uint32_t)c->m;
     uint32_t *cnt = P<uint32_t>(c->small);
-    KLAUNCH(c, "k_cover", k_cover, (grid_for(m), BLOCK, 0), P<uint32_t>(c->succ), P<uint32_t>(c->pred),
+    KLAUNCH(c, "k_cover", k_cover, (grid_for(m), BLOCK, 0), P<uint64_t>(c->link), P<uint32_t>(c->hasp),
             P<uint8_t>(c->flags), c->chain, P<uint32_t>(c->seglen), m, c->salt, c->shift);
     CHK(fill<uint32_t>(c, cnt + C_CYC, 2, 0u));
     uint64_t *la = P<uint64_t>(c->chainA) == c->chain ? P<uint64_t>(c->chainB) : P<uint64_t>(c->chainA);
     uint64_t *lb = P<uint64_t>(c->cycB);
-    KLAUNCH(c, "k_cycle_init", k_cycle_init, (grid_for(m), BLOCK, 0), P<uint32_t>(c->succ), P<uint8_t>(c->flags), m,
+    KLAUNCH(c, "k_cycle_init", k_cycle_init, (grid_for(m), BLOCK, 0), P<uint64_t>(c->link), P<uint8_t>(c->flags), m,
             la, P<uint32_t>(c->clist), cnt + C_CYC);
     uint32_t nl;
     CHK(d2h(c, &nl, cnt + C_CYC, 4));
@@ -1586,8 +2050,10 @@ int cut_cycles(uc_ctx *c, uint64_t *n_cut)
         std::swap(la, lb);
     }
     if (nl) {
+        KLAUNCH(c, "k_cycle_pred", k_cycle_pred, (grid_for(nl), BLOCK, 0), P<uint32_t>(c->clist), nl,
+                P<uint64_t>(c->link), P<uint32_t>(c->pred));
         KLAUNCH(c, "k_cycle_cut", k_cycle_cut, (grid_for(nl), BLOCK, 0), P<uint32_t>(c->clist), nl, la,
-                P<uint32_t>(c->succ), P<uint32_t>(c->pred), P<uint8_t>(c->flags), cnt + C_CUT);
+                P<uint32_t>(c->pred), P<uint64_t>(c->link), P<uint32_t>(c->hasp), P<uint8_t>(c->flags), cnt + C_CUT);
     }
     KLAUNCH(c, "k_clear_flag", k_clear_flag, (std::min<uint32_t>(grid_for(m), 65536), BLOCK, 0), P<uint8_t>(c->flags),
             m, F_VISIT);
@@ -1620,7 +2086,7 @@ int do_coarsen(uc_ctx *c)
     c->shift = c->opts.sample_log2 ? c->opts.sample_log2 : 5;
     if (c->x) {
         KLAUNCH(c, "k_splice", k_splice, (grid_for(c->x), BLOCK, 0), P<uint32_t>(c->pair_h), P<uint32_t>(c->pair_b),
-                (uint32_t)c->x, P<uint32_t>(c->succ), P<uint32_t>(c->pred));
+                (uint32_t)c->x, P<uint64_t>(c->link));
     }
     uint64_t live = 0;
     uint32_t nonconv = 0;
@@ -1628,6 +2094,7 @@ int do_coarsen(uc_ctx *c)
     if (live != c->n || nonconv) {
         CHK(ensure(c, c->cycB, c->m * 8));
         CHK(ensure(c, c->clist, c->m * 4));
+        CHK(ensure(c, c->pred, c->m * 4));
         uint64_t cut = 0;
         CHK(c->W == 1 ? cut_cycles<1>(c, &cut) : cut_cycles<2>(c, &cut));
         c->n_cycles = cut;
@@ -1649,21 +2116,21 @@ int run_emit(uc_ctx *c)
     CHK(ensure(c, c->text, c->text_bytes + 1));
     CHK(ensure(c, c->htoff, c->m * 8));
     if (m) {
-        KLAUNCH(c, "k_head_emit", k_head_emit<W>, (xcd_grid(T), BLOCK, 0), P<uint64_t>(c->kmer), P<uint32_t>(c->succ),
-                P<uint32_t>(c->pred), P<uint8_t>(c->flags), P<uint32_t>(c->seglen), P<uint32_t>(c->segnext),
+        KLAUNCH(c, "k_head_emit", k_head_emit<W>, (xcd_grid(T), BLOCK, 0), P<uint64_t>(c->kmer), P<uint64_t>(c->link),
+                P<uint32_t>(c->hasp), P<uint8_t>(c->flags), P<uint32_t>(c->seglen), P<uint32_t>(c->segnext),
                 P<uint32_t>(c->ulen), P<uint64_t>(c->tsum), m, c->k, P<uint64_t>(c->htoff), P<char>(c->text),
                 tile_arg(c, T));
     }
     if (c->n_list) {
         KLAUNCH(c, "k_rand_emit", k_rand_emit<W>, (grid_for(c->n_list), BLOCK, 0), P<uint64_t>(c->kmer),
-                P<uint32_t>(c->succ), P<uint32_t>(c->list), c->n_list, c->chain, P<uint32_t>(c->seglen),
+                P<uint64_t>(c->link), P<uint32_t>(c->list), c->n_list, c->chain, P<uint32_t>(c->seglen),
                 P<uint32_t>(c->segnext), P<uint64_t>(c->htoff), c->k, P<char>(c->text));
     }
     // per-set text ranges: a set's unitigs start at its first bucket row
     const uint64_t total = (c->n_unitigs << 32) | (c->text_bytes - c->n_unitigs * c->k);
     CHK(ensure(c, c->setoff, (c->n_sets + 1) * 8));
     KLAUNCH(c, "k_set_offsets", k_set_offsets, (grid_for(c->n_sets + 1), BLOCK, 0), P<uint32_t>(c->boff), c->n_sets,
-            c->B, P<uint32_t>(c->pred), P<uint8_t>(c->flags), P<uint32_t>(c->ulen), P<uint64_t>(c->tsum), m, total,
+            c->B, P<uint32_t>(c->hasp), P<uint8_t>(c->flags), P<uint32_t>(c->ulen), P<uint64_t>(c->tsum), m, total,
             P<uint64_t>(c->setoff));
     std::vector<uint64_t> so(c->n_sets + 1);
     CHK(d2h(c, so.data(), c->setoff.p, (c->n_sets + 1) * 8));
@@ -1779,10 +2246,12 @@ int uc_ctx_destroy(uc_ctx *c)
     if (!c) return UC_OK;
     (void)hipSetDevice(c->opts.device);
     (void)hipStreamSynchronize(c->stream);
-    DBuf *bufs[] = {&c->otable, &c->in,      &c->stage,  &c->bkt,     &c->tiles,  &c->hist,   &c->xhist,
+    DBuf *bufs[] = {&c->kmer2,  &c->flags2,  &c->src2,   &c->sub,     &c->perm,   &c->tiles2, &c->hist2,
+                    &c->cbhb,   &c->cbts,    &c->cbfirst, &c->fboff,  &c->egroups, &c->fbbase,
+                    &c->otable, &c->in,      &c->stage,  &c->bkt,     &c->tiles,  &c->hist,   &c->xhist,
                     &c->scan_tmp, &c->kmer,  &c->flags,  &c->src,     &c->pair_h, &c->pair_b, &c->boff,
-                    &c->sethb,  &c->setts,   &c->etiles, &c->toff,    &c->tbits,  &c->table,  &c->hit,
-                    &c->succ,   &c->pred,    &c->seglen, &c->segnext, &c->prevseg, &c->chainA, &c->chainB,
+                    &c->sethb,  &c->setts,   &c->etiles, &c->toff,    &c->tcap,   &c->table,  &c->hit,
+                    &c->link,   &c->hasp,    &c->pred,    &c->seglen, &c->segnext, &c->prevseg, &c->chainA, &c->chainB,
                     &c->cycB,   &c->list,    &c->clist,  &c->ulen,    &c->tsum,   &c->htoff,  &c->setoff,
                     &c->text,   &c->small};
     for (DBuf *b : bufs)
@@ -1885,6 +2354,10 @@ int uc_bucket_info_get(uc_ctx *c, uc_bucket_info *info)
     info->n_buckets = c->B;
     info->n_sets = c->n_sets;
     info->largest_bucket = c->largest;
+    info->n_fine_buckets = c->fboff_h.empty() ? 0 : c->fboff_h.size() - 1;
+    info->largest_fine_bucket = 0;
+    for (size_t f = 0; f + 1 < c->fboff_h.size(); f++)
+        info->largest_fine_bucket = std::max<uint64_t>(info->largest_fine_bucket, c->fboff_h[f + 1] - c->fboff_h[f]);
     return UC_OK;
 }
 
@@ -1896,7 +2369,10 @@ int uc_bucket_copy(uc_ctx *c, uint64_t *bucket_rows, uint32_t *slot_src, uint8_t
     CHK(select_dev(c));
     if (bucket_rows)
         for (size_t i = 0; i < c->boff_h.size(); i++) bucket_rows[i] = c->boff_h[i];
-    if (slot_src) CHK(d2h(c, slot_src, c->src.p, c->m * 4));
+    if (slot_src) {
+        if (!(c->opts.flags & UC_FLAG_KEEP_SRC)) return fail(c, UC_E_STATE, "slot ids need UC_FLAG_KEEP_SRC");
+        CHK(d2h(c, slot_src, c->src.p, c->m * 4));
+    }
     if (slot_flags) {
         CHK(d2h(c, slot_flags, c->flags.p, c->m));
         for (uint64_t i = 0; i < c->m; i++) slot_flags[i] &= (F_LENT | F_BORROW);
@@ -1918,7 +2394,9 @@ int uc_succ_copy(uc_ctx *c, uint32_t *succ)
     if (!c || !succ) return UC_E_ARG;
     if (!c->edged) return fail(c, UC_E_STATE, "no edge result");
     CHK(select_dev(c));
-    return d2h(c, succ, c->succ.p, c->m * 4);
+    if (c->m) KLAUNCH(c, "k_link_succ", k_link_succ, (std::min<uint32_t>(grid_for(c->m), 65536), BLOCK, 0),
+                      P<uint64_t>(c->link), (uint32_t)c->m, P<uint32_t>(c->hit));
+    return d2h(c, succ, c->hit.p, c->m * 4);
 }
 
 int uc_coarsen(uc_ctx *c)

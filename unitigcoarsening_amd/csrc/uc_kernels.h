@@ -33,15 +33,24 @@ constexpr int WAVES = BLOCK / 64;
 constexpr int BKT_IPT = 4;                    // k-mers per thread in the bucketing kernels
 constexpr int BKT_TILE = BLOCK * BKT_IPT;     // k-mers per bucketing tile
 constexpr int EDGE_TILE = 1024;               // slots per edge-finding tile
-constexpr uint32_t MAX_BUCKETS = 1024;        // buckets per set
+constexpr uint32_t MAX_BUCKETS = 1024;        // coarse buckets per set
+constexpr uint32_t MAX_SUB = 1024;            // fine buckets per coarse bucket
+constexpr int SUB_TILE = 8192;                // slots per fine-sort tile
+constexpr uint32_t LDS_GROUP = 2048;          // slots per LDS edge-finding group
 
 struct BTile {              // bucketing tile (one workgroup)
-    uint64_t kbegin;        // first k-mer (global index)
-    uint64_t hbase;         // first histogram entry of the tile's set
-    uint32_t count;         // k-mers in the tile
-    uint32_t set;
-    uint32_t ts;            // tiles of the set
-    uint32_t tlocal;        // tile index inside the set
+    uint64_t kbegin;        // first k-mer (pass 1) or slot (pass 2), global index
+    uint64_t hbase;         // first histogram entry of the tile's set / coarse bucket
+    uint32_t count;         // items in the tile
+    uint32_t set;           // set (pass 1) or coarse bucket (pass 2)
+    uint32_t ts;            // tiles of the set / coarse bucket
+    uint32_t tlocal;        // tile index inside it
+    uint32_t fmask;         // pass 1: fine-bucket mask B*S-1 of the set; pass 2: S
+    uint32_t pad;
+};
+
+struct EGroup {             // edge-finding group: consecutive fine buckets of one
+    uint32_t begin, end;    // coarse bucket, staged in LDS by one workgroup
 };
 
 struct ETile {              // edge-finding tile: slots of one bucket
