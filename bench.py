@@ -46,28 +46,34 @@ def kernel_bytes(name, n, m, x, W, n_unitigs, text_bytes, k):
     n k-mers, m slots (k-mers + crossing duplicates), x crossing k-mers,
     8W bytes per packed k-mer."""
     kb = 8 * W
-    ins = m - x          # slots that insert their prefix (not borrowed)
-    look = m - x         # slots that look up their suffix (not lent)
     table = {
-        # read k-mer, write bucket code
-        "k_lmin_count": n * (kb + 4),
-        # read k-mer + code; write slot k-mer, flag, source id; pair list
-        "k_lmin_scatter": n * (kb + 4) + m * (kb + 1 + 4) + x * 8,
-        # read flag + k-mer; claim one 8-byte table slot per insert
-        "k_edge_insert": m * (1 + kb) + ins * 8,
-        # read flag + k-mer; read the hit slot, read the owner k-mer, add to
-        # the slot; write the hit index
-        "k_edge_lookup": m * (1 + kb + 4) + look * (8 + kb + 8),
-        # read hit, read the table slot, write succ and pred
-        "k_edge_resolve": m * (4 + 4) + look * 8 + n * 4,
-        # read flag + pred per slot, succ per k-mer; write length and next
-        # splitter per head, unitig length per head, one sum per tile
-        "k_seg_count": m * (1 + 4) + n * 4 + n_unitigs * (4 + 4 + 4),
-        # read flag + pred per slot, length per head; succ + k-mer per
-        # k-mer; write the text
-        "k_head_emit": m * (1 + 4) + n_unitigs * (4 + 4 + 4) + n * (4 + kb) + text_bytes,
+        # read k-mer; write both fine bucket codes
+        "k_lmin_count": n * (kb + 8),
+        # read k-mer + codes; write slot k-mer, flag, fine index; pair list
+        "k_lmin_scatter": n * (kb + 8) + m * (kb + 1 + 2) + x * 8,
+        # read the fine index of every slot
+        "k_sub_count": m * 2,
+        # read fine index, k-mer, flag; write k-mer, flag, slot permutation
+        "k_sub_scatter": m * (2 + kb + 1) + m * (kb + 1 + 4),
+        # read both pair entries and their permuted slots; write both
+        "k_pair_remap": x * (8 + 8 + 8),
+        # read k-mer + flag per slot; write the 8-byte link per slot and the
+        # has-predecessor bitmap
+        "k_edge_lds": m * (kb + 1) + m * 8 + m // 8,
+        # fallback tables for oversized fine buckets (16-byte entries): the
+        # same three steps over HBM
+        "k_edge_insert": m * (1 + kb) + (m - x) * 16,
+        "k_edge_lookup": m * (1 + kb + 4) + (m - x) * (16 + kb),
+        "k_edge_resolve": m * (4 + 8) + (m - x) * 16 + m // 8,
         # move the borrowed out-edge onto the home copy
-        "k_splice": x * (4 + 4 + 4 + 4 + 4),
+        "k_splice": x * (4 + 4 + 8 + 8),
+        # read flag + has-predecessor bit per slot, one 8-byte link per
+        # walked k-mer; write length, next splitter, unitig length per head
+        "k_seg_count": m * 1 + m // 8 + n * 8 + n_unitigs * (4 + 4 + 4),
+        # read flag + has-predecessor bit per slot, head k-mer and lengths;
+        # one 8-byte link per k-mer of the head segments (bounded by n);
+        # write the text
+        "k_head_emit": m * 1 + m // 8 + n_unitigs * (kb + 4 + 4 + 4) + n * 8 + text_bytes,
     }
     return table.get(name)
 
@@ -114,6 +120,143 @@ def cpu_baseline(sample_bp, seed, k, l, B, gpu_ctx_kwargs):
     }
 
 
+def sets_workload(n_sets, seed0, k, median=200_000, sigma=1.0, lo=10_000, hi=5_000_000):
+    """256 independent sets, sizes ~ LogNormal(ln median, sigma) clipped to
+    [lo, hi] k-mers, each from the medium repeat/error model (seeds seed0..)."""
+    import numpy as np
+    import unitigcoarsening_amd as uc
+
+    rng = np.random.default_rng(seed0)
+    sizes = np.clip(np.exp(rng.normal(np.log(median), sigma, n_sets)), lo, hi).astype(np.int64)
+    # ~0.97 distinct canonical k-mers per genome base under the medium model
+    return [uc.medium_model(seed0 + i, int(sz / 0.97) + k, k) for i, sz in enumerate(sizes)]
+
+
+def bench_sets(args):
+    """Batched form (one run over all sets) vs per-set plain runs."""
+    import unitigcoarsening_amd as uc
+
+    k, l, B = args.k, args.l, args.buckets
+    t = time.time()
+    sets = sets_workload(args.n_sets, 1000, k)
+    n = sum(s.shape[0] for s in sets)
+    log(f"sets workload: {len(sets)} sets, {n} k-mers in {time.time() - t:.1f}s")
+    tun = {"load_pct": args.load_pct, "sample_log2": args.sample_log2}
+    with uc.Compactor(n_buckets=B, **tun) as c:
+        for _ in range(max(args.warmup, 1)):
+            c.compact_sets(sets, k, l)
+        c.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            c.compact_sets(sets, k, l)
+        c.sync()
+        batched = (time.perf_counter() - t0) / args.steps
+    log(f"batched: {batched * 1e3:.1f} ms for all sets")
+    with uc.Compactor(n_buckets=B, mode=uc.UC_MODE_PLAIN, **tun) as c:
+        c.compact(sets[0], k, l)
+        c.sync()
+        t0 = time.perf_counter()
+        for s in sets:
+            c.compact(s, k, l)
+        c.sync()
+        plain = time.perf_counter() - t0
+    log(f"plain per-bucket, set by set: {plain * 1e3:.1f} ms")
+    return {"workload": f"{len(sets)} sets, log-normal sizes (median 200k, sigma 1, clipped 10k-5M k-mers), "
+                        f"medium repeat/error model, seeds 1000..{1000 + len(sets) - 1}, k={k}, l={l}, {B} buckets",
+            "n_kmers": n, "batched_ms": round(batched * 1e3, 3), "plain_ms": round(plain * 1e3, 3),
+            "batched_value": n / batched, "plain_value": n / plain, "batched_over_plain": round(plain / batched, 3),
+            "note": "host copies of the inputs are inside both timings (compact_sets / compact take host arrays)"}
+
+
+def serve_pool(n_inputs, seed0, lo=10_000, hi=2_000_000):
+    """Distinct job inputs for the request loop: sizes log-uniform in
+    [lo, hi] k-mers, k alternating 31 / 63, medium repeat/error model."""
+    import numpy as np
+    import unitigcoarsening_amd as uc
+
+    rng = np.random.default_rng(seed0)
+    pool = []
+    for i in range(n_inputs):
+        k = 31 if i % 2 == 0 else 63
+        size = int(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+        pool.append((uc.medium_model(seed0 + i, int(size / 0.97) + k, k), k, 11 if k == 31 else 13))
+    return pool
+
+
+def bench_serve(args):
+    """Request loop (uc_server): Poisson arrivals at a fraction of the
+    measured one-context capacity, jobs drawn from a pool of inputs;
+    latency = submit to result, over every job."""
+    import threading
+
+    import numpy as np
+    import unitigcoarsening_amd as uc
+
+    pool = serve_pool(16, 5000)
+    sizes = np.array([p[0].shape[0] for p in pool])
+    # one context, jobs back to back: service time per input
+    svc = []
+    with uc.Compactor(n_buckets=args.buckets, sample_log2=args.sample_log2) as c:
+        for km, k, l in pool:
+            c.compact(km, k, l)
+        for km, k, l in pool:
+            t0 = time.perf_counter()
+            c.compact(km, k, l)
+            svc.append(time.perf_counter() - t0)
+    mean_svc = float(np.mean(svc))
+    rng = np.random.default_rng(6)
+    picks = rng.integers(0, len(pool), args.serve_jobs)
+    rate = args.serve_load / mean_svc  # jobs per second offered
+    gaps = rng.exponential(1.0 / rate, args.serve_jobs)
+    lat, done_bytes = [], []
+    with uc.Server(n_slots=args.serve_slots, hbm_budget=args.serve_budget_gib << 30, n_buckets=args.buckets,
+                   sample_log2=args.sample_log2) as srv:
+        ids = []
+        waiter_err = []
+
+        def waiter():
+            i = 0
+            while i < args.serve_jobs:
+                while len(ids) <= i:
+                    time.sleep(0.0002)
+                try:
+                    text, qms, rms = srv.result(ids[i])
+                    lat.append(qms + rms)
+                    done_bytes.append(len(text))
+                except Exception as e:  # reported after the run
+                    waiter_err.append(repr(e))
+                i += 1
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t0 = time.perf_counter()
+        nxt = t0
+        for j in range(args.serve_jobs):
+            nxt += gaps[j]
+            dt = nxt - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            km, k, l = pool[picks[j]]
+            ids.append(srv.submit(km, k, l))
+        th.join()
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    if waiter_err:
+        raise RuntimeError(f"serving jobs failed: {waiter_err[:3]}")
+    lat = np.array(lat)
+    served = int(sizes[picks].sum())
+    log(f"serve: {args.serve_jobs} jobs in {wall:.2f}s, p50 {np.percentile(lat, 50):.2f} ms, "
+        f"p99 {np.percentile(lat, 99):.2f} ms")
+    return {"workload": f"{args.serve_jobs} jobs, Poisson arrivals at {args.serve_load:.2f} x one-context capacity, "
+                        f"drawn from 16 inputs of log-uniform 10k-2M k-mers (medium model, k 31/63 alternating, "
+                        f"seeds 5000..5015), {args.serve_slots} worker slots, budget {args.serve_budget_gib} GiB",
+            "offered_jobs_per_s": round(rate, 2), "mean_service_ms_one_context": round(1e3 * mean_svc, 3),
+            "kmers_per_s": served / wall, "p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3), "max_ms": round(float(lat.max()), 3),
+            "peak_held_gib": round(st["peak_held_bytes"] / 2 ** 30, 3),
+            "note": "latency includes the host copy of the input and of the result text"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +274,13 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
     ap.add_argument("--cpu-sample-bp", type=int, default=4_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sets", action="store_true", help="also run the 256-set batched vs plain comparison")
+    ap.add_argument("--n-sets", type=int, default=256)
+    ap.add_argument("--serve", action="store_true", help="also run the request-loop latency benchmark")
+    ap.add_argument("--serve-jobs", type=int, default=400)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-load", type=float, default=0.7, help="offered load, x one-context capacity")
+    ap.add_argument("--serve-budget-gib", type=int, default=64)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -209,6 +359,9 @@ def main():
         plain = {"value": n / pw, "ms_per_step": round(1e3 * pw, 3), "batched_over_plain": round(pw / (wall / args.steps), 3)}
         log(f"plain form: {1e3 * pw:.2f} ms/step")
 
+    sets = bench_sets(args) if args.sets else None
+    serve = bench_serve(args) if args.serve else None
+
     base = None
     if not args.no_cpu_baseline:
         log("cpu baseline (serial restatement of the reference pipeline) ...")
@@ -239,6 +392,8 @@ def main():
         "kernels": kernels,
         "roofline": roof,
         "plain": plain,
+        "sets": sets,
+        "serve": serve,
         "cpu_baseline": base,
     }
     print(json.dumps(out), flush=True)

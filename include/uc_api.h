@@ -206,6 +206,56 @@ int uc_kernel_stats_reset(uc_ctx *ctx);
 /* Blocks until the context's stream is idle. */
 int uc_sync(uc_ctx *ctx);
 
+/* Frees every device buffer the context holds (they grow back on demand). */
+int uc_ctx_trim(uc_ctx *ctx);
+
+/* Device bytes a context holds right now. */
+uint64_t uc_ctx_held_bytes(const uc_ctx *ctx);
+
+/* Upper estimate of the device bytes one job of n k-mers needs. */
+uint64_t uc_job_bytes_estimate(uint64_t n, uint32_t k);
+
+/* ---- serving --------------------------------------------------------------
+ * No counterpart in the reference, which runs one job per process.  A server
+ * owns n_slots worker contexts, each with its own HIP stream and host
+ * thread.  Jobs are queued in arrival order; a job starts when a worker is
+ * free and its estimated footprint fits in hbm_budget together with the
+ * memory the other workers hold (idle workers give their buffers back when
+ * that is what makes a job fit).  Jobs on different workers run
+ * concurrently on the device.  A job larger than the whole budget fails
+ * with UC_E_NOMEM. */
+
+typedef struct uc_server uc_server;
+
+typedef struct {
+    uc_opts ctx_opts;     /* options of every worker context */
+    uint32_t n_slots;     /* concurrent jobs */
+    uint64_t hbm_budget;  /* bytes all workers may hold together; 0 = no limit */
+} uc_server_opts;
+
+#define UC_JOB_QUEUED 0
+#define UC_JOB_RUNNING 1
+#define UC_JOB_DONE 2
+#define UC_JOB_FAILED 3
+
+int uc_server_create(const uc_server_opts *opts, uc_server **out);
+/* Waits for queued and running jobs, then frees everything. */
+int uc_server_destroy(uc_server *srv);
+/* Copies n packed k-mers (host memory) and queues the job. */
+int uc_submit(uc_server *srv, const uint32_t *packed, uint64_t n, uint32_t k, uint32_t l, uint64_t *job);
+/* Non-blocking state of a job (UC_JOB_*). */
+int uc_poll(uc_server *srv, uint64_t job, int *state);
+/* Blocks until the job is done or failed; returns the job's status. */
+int uc_wait(uc_server *srv, uint64_t job);
+/* Result of a finished job: text size (always set) and text (when dst has
+ * room), time queued and time running in milliseconds. */
+int uc_result(uc_server *srv, uint64_t job, char *dst, uint64_t cap, uint64_t *size, double *queue_ms,
+              double *run_ms);
+/* Forgets a finished job and its text. */
+int uc_release(uc_server *srv, uint64_t job);
+/* Peak device bytes the workers held together, and jobs completed. */
+int uc_server_stats(uc_server *srv, uint64_t *peak_held, uint64_t *jobs_done);
+
 #ifdef __cplusplus
 }
 #endif

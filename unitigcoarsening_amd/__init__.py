@@ -20,6 +20,7 @@ SYNTH_PATH = os.path.join(LIB_DIR, "libuc_synth.so")
 CLI_PATH = os.path.join(LIB_DIR, "uc_main")
 
 UC_OK = 0
+UC_E_ARG, UC_E_NOMEM, UC_E_HIP, UC_E_STATE, UC_E_IO, UC_E_FORMAT, UC_E_LIMIT = -1, -2, -3, -4, -5, -6, -7
 UC_ALL_BUCKETS = 0xFFFFFFFF
 UC_MODE_BATCHED = 0
 UC_MODE_PLAIN = 1
@@ -84,6 +85,17 @@ class UcTiming(C.Structure):
     ]
 
 
+class UcServerOpts(C.Structure):
+    _fields_ = [
+        ("ctx_opts", UcOpts),
+        ("n_slots", C.c_uint32),
+        ("hbm_budget", C.c_uint64),
+    ]
+
+
+UC_JOB_QUEUED, UC_JOB_RUNNING, UC_JOB_DONE, UC_JOB_FAILED = 0, 1, 2, 3
+
+
 class SynthParams(C.Structure):
     _fields_ = [
         ("seed", C.c_uint64),
@@ -132,6 +144,18 @@ API = {
     "uc_free": (None, [_P]),
     "uc_timing_get": (C.c_int, [_P, C.POINTER(UcTiming)]),
     "uc_sync": (C.c_int, [_P]),
+    "uc_ctx_trim": (C.c_int, [_P]),
+    "uc_ctx_held_bytes": (C.c_uint64, [_P]),
+    "uc_job_bytes_estimate": (C.c_uint64, [C.c_uint64, C.c_uint32]),
+    "uc_server_create": (C.c_int, [C.POINTER(UcServerOpts), C.POINTER(_P)]),
+    "uc_server_destroy": (C.c_int, [_P]),
+    "uc_submit": (C.c_int, [_P, _P, C.c_uint64, C.c_uint32, C.c_uint32, _U64P]),
+    "uc_poll": (C.c_int, [_P, C.c_uint64, C.POINTER(C.c_int)]),
+    "uc_wait": (C.c_int, [_P, C.c_uint64]),
+    "uc_result": (C.c_int, [_P, C.c_uint64, C.c_char_p, C.c_uint64, _U64P, C.POINTER(C.c_double),
+                            C.POINTER(C.c_double)]),
+    "uc_release": (C.c_int, [_P, C.c_uint64]),
+    "uc_server_stats": (C.c_int, [_P, _U64P, _U64P]),
 }
 
 _lib = None
@@ -422,6 +446,81 @@ class Compactor:
 
     def sync(self):
         _check(lib().uc_sync(self._ctx), self._ctx)
+
+    def trim(self):
+        """Free every device buffer the context holds (uc_ctx_trim)."""
+        _check(lib().uc_ctx_trim(self._ctx), self._ctx)
+
+    def held_bytes(self):
+        return lib().uc_ctx_held_bytes(self._ctx)
+
+
+def job_bytes_estimate(n, k):
+    """Upper bound of the device bytes one job of n k-mers needs (uc_job_bytes_estimate)."""
+    return lib().uc_job_bytes_estimate(n, k)
+
+
+class Server:
+    """Request loop (uc_server): n_slots worker contexts, FIFO admission under an HBM budget.
+
+    submit() copies the k-mers and returns a job id at once; result() waits and returns
+    (text, queue_ms, run_ms).  Safe to call from several Python threads."""
+
+    def __init__(self, n_slots=4, hbm_budget=0, device=0, seed=0x5EED, n_buckets=64, sample_log2=5):
+        L = lib()
+        o = UcServerOpts()
+        L.uc_default_opts(C.byref(o.ctx_opts))
+        o.ctx_opts.device, o.ctx_opts.seed = device, seed
+        o.ctx_opts.n_buckets, o.ctx_opts.sample_log2 = n_buckets, sample_log2
+        o.n_slots, o.hbm_budget = n_slots, hbm_budget
+        self._srv = _P()
+        _check(L.uc_server_create(C.byref(o), C.byref(self._srv)))
+
+    def close(self):
+        if self._srv:
+            lib().uc_server_destroy(self._srv)
+            self._srv = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, packed, k, l):
+        packed = np.ascontiguousarray(packed, dtype=np.uint32)
+        job = C.c_uint64()
+        _check(lib().uc_submit(self._srv, packed.ctypes.data, packed.shape[0], k, l, C.byref(job)))
+        return job.value
+
+    def poll(self, job):
+        st = C.c_int()
+        _check(lib().uc_poll(self._srv, job, C.byref(st)))
+        return st.value
+
+    def result(self, job, release=True):
+        L = lib()
+        L.uc_wait(self._srv, job)
+        size, qms, rms = C.c_uint64(), C.c_double(), C.c_double()
+        rc = L.uc_result(self._srv, job, None, 0, C.byref(size), C.byref(qms), C.byref(rms))
+        if rc == UC_OK:
+            buf = C.create_string_buffer(max(size.value, 1))
+            rc = L.uc_result(self._srv, job, buf, size.value, None, None, None)
+        if release:
+            L.uc_release(self._srv, job)
+        _check(rc)
+        return buf.raw[:size.value], qms.value, rms.value
+
+    def stats(self):
+        peak, done = C.c_uint64(), C.c_uint64()
+        _check(lib().uc_server_stats(self._srv, C.byref(peak), C.byref(done)))
+        return {"peak_held_bytes": peak.value, "jobs_done": done.value}
 
 
 def sorted_lines(text: bytes) -> list:

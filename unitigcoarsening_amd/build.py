@@ -60,6 +60,10 @@ def build_lib(verbose=False):
     _run(["g++", "-O3", "-std=c++17", "-fPIC", "-Wall", f"-I{INCLUDE}", "-c",
           os.path.join(CSRC, "uc_text.cpp"), "-o", text_o], verbose)
     objs.append(text_o)
+    srv_o = os.path.join(LIB, "uc_server.o")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-pthread", f"-I{INCLUDE}", "-c",
+          os.path.join(CSRC, "uc_server.cpp"), "-o", srv_o], verbose)
+    objs.append(srv_o)
     out = os.path.join(LIB, "libuc.so")
     tmp = out + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp, "-lpthread"], verbose)

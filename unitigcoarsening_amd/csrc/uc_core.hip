@@ -2473,6 +2473,45 @@ int uc_sync(uc_ctx *c)
     return UC_OK;
 }
 
+int uc_ctx_trim(uc_ctx *c)
+{
+    if (!c) return UC_E_ARG;
+    CHK(select_dev(c));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    DBuf *bufs[] = {&c->kmer2,  &c->flags2,  &c->src2,   &c->sub,     &c->perm,    &c->tiles2, &c->hist2,
+                    &c->cbhb,   &c->cbts,    &c->cbfirst, &c->fboff,  &c->egroups, &c->fbbase, &c->otable,
+                    &c->in,     &c->stage,   &c->bkt,    &c->tiles,   &c->hist,    &c->xhist,  &c->scan_tmp,
+                    &c->kmer,   &c->flags,   &c->src,    &c->pair_h,  &c->pair_b,  &c->boff,   &c->sethb,
+                    &c->setts,  &c->etiles,  &c->toff,   &c->tcap,    &c->table,   &c->hit,    &c->link,
+                    &c->hasp,   &c->pred,    &c->seglen, &c->segnext, &c->prevseg, &c->chainA, &c->chainB,
+                    &c->cycB,   &c->list,    &c->clist,  &c->ulen,    &c->tsum,    &c->htoff,  &c->setoff,
+                    &c->text,   &c->small};
+    for (DBuf *b : bufs) {
+        if (b->p) HIPCHK(c, hipFree(b->p));
+        b->p = nullptr;
+        b->cap = 0;
+    }
+    c->held = 0;
+    c->kin = nullptr;
+    reset_job(c);
+    return UC_OK;
+}
+
+uint64_t uc_ctx_held_bytes(const uc_ctx *c) { return c ? c->held : 0; }
+
+uint64_t uc_job_bytes_estimate(uint64_t n, uint32_t k)
+{
+    // per k-mer: input copy + bucket codes + the text of n isolated k-mers
+    // (the most unitigs a job can have); per slot (at most 1.25 n slots in
+    // the minimizer schemes used, crossing k-mers ~9% at k = 31): k-mers and
+    // flags twice, fine index, permutation, link, hit, bitmap, four segment
+    // arrays, two chain buffers, list, head offsets
+    const uint64_t W = k <= 32 ? 1 : 2;
+    const uint64_t per_kmer = 8 * W + 8 + (k + 1);
+    const uint64_t per_slot = 16 * W + 2 + 2 + 4 + 8 + 4 + 1 + 16 + 16 + 4 + 8;
+    return n * per_kmer + (n + n / 4) * per_slot + (64ull << 20);
+}
+
 }  // extern "C"
 
 namespace {
