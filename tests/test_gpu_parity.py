@@ -180,9 +180,14 @@ def test_order_table_matches_oracle(ctx):
 
 
 @pytest.mark.parametrize("use_table", [False, True])
-def test_bucketize_matches_oracle(use_table):
+@pytest.mark.parametrize("genome", [3_000, 100_000])
+def test_bucketize_matches_oracle(use_table, genome):
+    # The device buckets by B*S fine buckets (S per set, from its size): fine
+    # bucket f is the reference's bucket f with B*S buckets, and coarse bucket
+    # b holds the fine buckets f = j*B + b in order of j.  At S = 1 this is
+    # the reference's B-bucket layout itself.
     k, l, B = 31, 9, 64
-    kmers = uc.medium_model(31, 100_000, k)
+    kmers = uc.medium_model(31, genome, k)
     with uc.Compactor(n_buckets=B, seed=77, keep_src=True) as c:
         c.load(kmers, k)
         if use_table:
@@ -192,21 +197,37 @@ def test_bucketize_matches_oracle(use_table):
             order = oracle.lmer_order_table(l, 77)       # the inline bijection
             c.bucketize(l)
         rows, src, flags, ph, pb = c.bucket_arrays()
-    ob = oracle.bucketize(kmers, k, l, B, order)
-    assert np.array_equal(rows.astype(np.int64), ob["bucket_rows"].astype(np.int64))
+        info = c.bucket_info()
+    S = info.n_fine_buckets // B
+    Bf = B * S
+    log(f"bucketize n={kmers.shape[0]} S={S}")
+    assert (S == 1) == (genome == 3_000)
+    ob = oracle.bucketize(kmers, k, l, Bf, order)
+    orow = ob["bucket_rows"].astype(np.int64)
+    mins = np.array([oracle.double_lmin(kmers[i], k, l, order) for i in range(kmers.shape[0])], dtype=np.int64)
+    # fine bucket of every slot: prefix minimizer for home copies, suffix for borrowed ones
+    side = (flags == uc.F_BORROW).astype(np.int64)
+    fine = mins[src.astype(np.int64), side] % Bf
     for b in range(B):
-        g = np.sort(src[rows[b]:rows[b + 1]])
-        o = np.sort(ob["kmer_src"][ob["bucket_rows"][b]:ob["bucket_rows"][b + 1]])
-        assert np.array_equal(g, o.astype(np.uint32)), b
+        fs = [j * B + b for j in range(S)]
+        assert rows[b + 1] - rows[b] == sum(orow[f + 1] - orow[f] for f in fs), b
+        row_fine = fine[rows[b]:rows[b + 1]]
+        assert np.all(row_fine % B == b)
+        assert np.all(np.diff(row_fine // B) >= 0), b            # ordered by fine index
+        for f in fs:
+            g = np.sort(src[rows[b]:rows[b + 1]][row_fine == f])
+            o = np.sort(ob["kmer_src"][orow[f]:orow[f + 1]])
+            assert np.array_equal(g, o.astype(np.uint32)), (b, f)
     # every k-mer placed once, crossing k-mers twice
     cnt = np.bincount(src.astype(np.int64), minlength=kmers.shape[0])
     assert cnt.max() <= 2 and (cnt >= 1).all()
     # pairs: same k-mer, home copy flagged lent, duplicate flagged borrowed
     assert ph.shape[0] == (flags == uc.F_BORROW).sum() == (flags == uc.F_LENT).sum()
+    assert ph.shape[0] == ob["cross_rows"][-1]
     assert np.array_equal(src[ph], src[pb])
     assert np.all(flags[ph] == uc.F_LENT) and np.all(flags[pb] == uc.F_BORROW)
+    assert np.all(mins[src[ph], 0] % Bf != mins[src[ph], 1] % Bf)
     bucket_of = np.repeat(np.arange(B), np.diff(rows).astype(np.int64))
-    mins = np.array([oracle.double_lmin(kmers[i], k, l, order) for i in range(kmers.shape[0])])
     assert np.array_equal(bucket_of[ph], mins[src[ph], 0] % B)
     assert np.array_equal(bucket_of[pb], mins[src[pb], 1] % B)
 

@@ -336,7 +336,9 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_count(const uint64_t *__restrict
 {
     extern __shared__ uint32_t cnt[];  // B buckets + 1 crossing counter
     const BTile t = tiles[blockIdx.x];
-    // fine bucket = order & (B*S - 1); its low log2(B) bits are the coarse bucket
+    // fine bucket = order & (B*S - 1); its low log2(B) bits are the coarse
+    // bucket.  A k-mer crosses when its two fine buckets differ (both copies
+    // may then share a coarse bucket).
     for (uint32_t i = threadIdx.x; i <= p.B; i += BLOCK) cnt[i] = 0;
     __syncthreads();
     const uint32_t bmask = p.B - 1;
@@ -348,11 +350,11 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_count(const uint64_t *__restrict
             const Kmer<W> x = load_kmer<W>(kin, g, top_mask);
             uint32_t o1, o2;
             double_lmin<W>(x, p, o1, o2);
-            const uint32_t bp = o1 & bmask, bs = o2 & bmask;
-            bkt[g] = make_uint2(o1 & t.fmask, o2 & t.fmask);
-            atomicAdd(&cnt[bp], 1u);
-            if (bp != bs) {
-                atomicAdd(&cnt[bs], 1u);
+            const uint32_t f1 = o1 & t.fmask, f2 = o2 & t.fmask;
+            bkt[g] = make_uint2(f1, f2);
+            atomicAdd(&cnt[f1 & bmask], 1u);
+            if (f1 != f2) {
+                atomicAdd(&cnt[f2 & bmask], 1u);
                 atomicAdd(&cnt[p.B], 1u);
             }
         }
@@ -362,11 +364,12 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_count(const uint64_t *__restrict
     if (threadIdx.x == 0) xhist[blockIdx.x] = cnt[p.B];
 }
 
-// lanes whose home or borrow placement goes to bucket v
-__device__ __forceinline__ uint64_t match_bucket(uint32_t v, uint32_t nbits, uint64_t V, uint64_t C,
-                                                 const uint64_t *BP, const uint64_t *BS)
+// lanes whose home (mp) or borrow (ms) placement goes to bucket v
+__device__ __forceinline__ void match_bucket(uint32_t v, uint32_t nbits, uint64_t V, uint64_t C,
+                                             const uint64_t *BP, const uint64_t *BS, uint64_t &mp, uint64_t &ms)
 {
-    uint64_t mp = V, ms = C;
+    mp = V;
+    ms = C;
 #pragma unroll
     for (int j = 0; j < 10; j++) {
         if ((uint32_t)j < nbits) {
@@ -379,12 +382,12 @@ __device__ __forceinline__ uint64_t match_bucket(uint32_t v, uint32_t nbits, uin
             }
         }
     }
-    return mp | ms;
 }
 
 // Stable scatter into the bucket CSR: inside a tile, placements keep the
-// order of their k-mers; per wave the rank among lanes bound for the same
-// bucket comes from log2(B) ballots of the bucket bits.
+// order of their k-mers (home copy before borrowed copy when both land in
+// one bucket); per wave the rank among placements bound for the same bucket
+// comes from log2(B) ballots of the bucket bits.
 template <int W>
 __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restrict__ kin, const BTile *__restrict__ tiles,
                                                         uint32_t B, uint32_t nbits, uint64_t top_mask,
@@ -410,14 +413,15 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restri
         const bool valid = idx < t.count;
         const uint64_t g = t.kbegin + idx;
         uint32_t bp = 0, bs = 0, fp = 0, fs = 0;
+        bool cross = false;
         if (valid) {
             const uint2 c = bkt[g];
             fp = c.x >> nbits;  // fine index inside the coarse bucket
             fs = c.y >> nbits;
             bp = c.x & (B - 1);
             bs = c.y & (B - 1);
+            cross = c.x != c.y;
         }
-        const bool cross = valid && bp != bs;
         const uint64_t V = __ballot(valid), C = __ballot(cross);
         uint64_t BP[10], BS[10];
 #pragma unroll
@@ -425,20 +429,23 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restri
             BP[q] = (uint32_t)q < nbits ? (uint64_t)__ballot(valid && ((bp >> q) & 1u)) : 0ull;
             BS[q] = (uint32_t)q < nbits ? (uint64_t)__ballot(cross && ((bs >> q) & 1u)) : 0ull;
         }
-        uint64_t Mh = 0, Mb = 0;
+        // placements of lower lanes in the same bucket: homes (hp*) and borrows (hs*)
+        uint64_t hpH = 0, hsH = 0, hpB = 0, hsB = 0;
         if (valid) {
-            Mh = match_bucket(bp, nbits, V, C, BP, BS);
-            if ((Mh & (0ull - Mh)) == me) wcnt[w * (B + 1) + bp] = (uint32_t)__popcll(Mh);
+            match_bucket(bp, nbits, V, C, BP, BS, hpH, hsH);
+            const uint64_t M = hpH | hsH;
+            if ((M & (0ull - M)) == me) wcnt[w * (B + 1) + bp] = (uint32_t)(__popcll(hpH) + __popcll(hsH));
         }
         if (cross) {
-            Mb = match_bucket(bs, nbits, V, C, BP, BS);
-            if ((Mb & (0ull - Mb)) == me) wcnt[w * (B + 1) + bs] = (uint32_t)__popcll(Mb);
+            match_bucket(bs, nbits, V, C, BP, BS, hpB, hsB);
+            const uint64_t M = hpB | hsB;
+            if ((M & (0ull - M)) == me) wcnt[w * (B + 1) + bs] = (uint32_t)(__popcll(hpB) + __popcll(hsB));
         }
         if (C && (C & (0ull - C)) == me) wcnt[w * (B + 1) + B] = (uint32_t)__popcll(C);
         __syncthreads();
         if (valid) {
             const Kmer<W> x = load_kmer<W>(kin, g, top_mask);
-            uint32_t ph = run[bp] + (uint32_t)__popcll(Mh & lt);
+            uint32_t ph = run[bp] + (uint32_t)(__popcll(hpH & lt) + __popcll(hsH & lt));
             for (int q = 0; q < w; q++) ph += wcnt[q * (B + 1) + bp];
             const uint32_t sh_slot = hscan[t.hbase + (uint64_t)bp * t.ts + t.tlocal] + ph;
             store_kmer<W>(kout, sh_slot, x);
@@ -446,7 +453,7 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restri
             sub[sh_slot] = (uint16_t)fp;
             if (src) src[sh_slot] = (uint32_t)g;
             if (cross) {
-                uint32_t pb = run[bs] + (uint32_t)__popcll(Mb & lt);
+                uint32_t pb = run[bs] + (uint32_t)(__popcll(hpB & lt) + __popcll(hsB & lt)) + (bp == bs ? 1u : 0u);
                 for (int q = 0; q < w; q++) pb += wcnt[q * (B + 1) + bs];
                 const uint32_t sb_slot = hscan[t.hbase + (uint64_t)bs * t.ts + t.tlocal] + pb;
                 store_kmer<W>(kout, sb_slot, x);
@@ -533,7 +540,8 @@ __global__ __launch_bounds__(BLOCK) void k_sub_scatter(const BTile *__restrict__
         for (int q = 0; q < 10; q++) BP[q] = (uint32_t)q < nbits ? (uint64_t)__ballot(valid && ((v >> q) & 1u)) : 0ull;
         uint64_t M = 0;
         if (valid) {
-            M = match_bucket(v, nbits, V, 0ull, BP, zero);
+            uint64_t none;
+            match_bucket(v, nbits, V, 0ull, BP, zero, M, none);
             if ((M & (0ull - M)) == me) wcnt[w * S + v] = (uint32_t)__popcll(M);
         }
         __syncthreads();
