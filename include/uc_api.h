@@ -174,7 +174,9 @@ const char *uc_text_device_ptr(const uc_ctx *ctx);
 int uc_text_copy(uc_ctx *ctx, char *dst, uint64_t cap);
 uint64_t uc_set_text_offset(const uc_ctx *ctx, uint32_t set);
 
-/* Appends the text to a file (std::ofstream::app semantics, compact.h:7-37). */
+/* Appends the text to a file (std::ofstream::app semantics, compact.h:7-37).
+ * The copy out of HBM runs in 64 MiB pinned chunks, each overlapping the
+ * write of the one before. */
 int uc_text_append_file(uc_ctx *ctx, const char *path);
 
 /* ---- text input (driver.cpp:251-354) ---------------------------------- */
@@ -185,6 +187,23 @@ int uc_text_append_file(uc_ctx *ctx, const char *path);
 int uc_parse_kmer_text(const char *buf, uint64_t len, uint32_t k, uint32_t **packed, uint64_t *n);
 int uc_read_kmer_file(const char *path, uint32_t k, uint32_t **packed, uint64_t *n);
 void uc_free(void *p);
+
+/* Replaces driver.cpp:257-354 load_kmers + driver.cpp:227-249 compact_kmers
+ * on the device: the same text format and rules as uc_parse_kmer_text
+ * (n = min(count line, lines); a line shorter than k after dropping one '\r'
+ * before its end is UC_E_FORMAT), parsed and packed by kernels and loaded as
+ * one k-mer set, ready for uc_bucketize or uc_compact_loaded.  text is a
+ * host buffer, or a device buffer when on_device is nonzero. */
+int uc_load_text(uc_ctx *ctx, const char *text, uint64_t len, uint32_t k, int on_device);
+
+/* uc_load_text of a file, read through 64 MiB pinned chunks, each read
+ * overlapping the copy of the one before.  UC_E_IO if it cannot be opened
+ * (driver.cpp:262-265). */
+int uc_load_file(uc_ctx *ctx, const char *path, uint32_t k);
+
+/* The loaded k-mers in the reference packing (ceil(k/16) uint32 per
+ * k-mer); *n gets their count.  packed may be NULL to ask for the count. */
+int uc_loaded_copy(uc_ctx *ctx, uint32_t *packed, uint64_t cap, uint64_t *n);
 
 /* ---- timing ----------------------------------------------------------- */
 

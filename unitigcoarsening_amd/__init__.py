@@ -144,6 +144,9 @@ API = {
     "uc_free": (None, [_P]),
     "uc_timing_get": (C.c_int, [_P, C.POINTER(UcTiming)]),
     "uc_sync": (C.c_int, [_P]),
+    "uc_load_text": (C.c_int, [_P, C.c_char_p, C.c_uint64, C.c_uint32, C.c_int]),
+    "uc_load_file": (C.c_int, [_P, C.c_char_p, C.c_uint32]),
+    "uc_loaded_copy": (C.c_int, [_P, _U32P, C.c_uint64, _U64P]),
     "uc_ctx_trim": (C.c_int, [_P]),
     "uc_ctx_held_bytes": (C.c_uint64, [_P]),
     "uc_job_bytes_estimate": (C.c_uint64, [C.c_uint64, C.c_uint32]),
@@ -354,6 +357,23 @@ class Compactor:
         offs = np.zeros(len(sizes) + 1, dtype=np.uint64)
         offs[1:] = np.cumsum(sizes)
         _check(L.uc_load(self._ctx, packed.ctypes.data, offs.ctypes.data_as(_U64P), len(sizes), k, 0), self._ctx)
+
+    def load_text(self, text: bytes, k):
+        """Parse the k-mer text on the device (uc_load_text) and load it."""
+        _check(lib().uc_load_text(self._ctx, text, len(text), k, 0), self._ctx)
+
+    def load_file(self, path, k):
+        """Read and parse a k-mer file on the device (uc_load_file)."""
+        _check(lib().uc_load_file(self._ctx, os.fsencode(path), k), self._ctx)
+
+    def loaded(self, k):
+        """The loaded k-mers in the reference packing, shape (n, ceil(k/16))."""
+        L = lib()
+        n = C.c_uint64()
+        _check(L.uc_loaded_copy(self._ctx, None, 0, C.byref(n)), self._ctx)
+        out = np.zeros((max(n.value, 1), words_per_kmer(k)), np.uint32)
+        _check(L.uc_loaded_copy(self._ctx, out.ctypes.data_as(_U32P), n.value, None), self._ctx)
+        return out[:n.value]
 
     def compact_loaded(self, l):
         """Whole job on the k-mers of the last load(), already in device memory."""

@@ -1260,6 +1260,104 @@ __global__ void k_link_succ(const uint64_t *__restrict__ link, uint32_t m, uint3
 }
 
 // ---------------------------------------------------------------------------
+// 5. k-mer text on the device [driver.cpp:257-354 load_kmers,
+//    driver.cpp:227-249 compact_kmers]
+//
+// The text sits in HBM with at least TXT_PAD zero bytes after it.  Tiles of
+// TXT_TILE bytes, 16 per thread: k_txt_count counts the newlines of the body
+// per tile; after a scan, k_txt_pack finds every newline again with its
+// global index j and packs line j + 1 (the line that starts after it)
+// straight into the 64-bit k-mer layout.  Line 0 starts at the body.
+
+constexpr uint32_t TXT_TILE = BLOCK * 16;
+constexpr uint32_t TXT_PAD = TXT_TILE + 256;
+
+// bit b set iff byte b of the 16 at `pos` is '\n' and lo <= pos + b < hi
+__device__ __forceinline__ uint32_t nl_mask16(uint4 v, uint64_t pos, uint64_t lo, uint64_t hi)
+{
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+    uint32_t m = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const uint32_t t = w[j] ^ 0x0A0A0A0Au;  // zero bytes where '\n'
+        const uint32_t z = ~(((t & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | t | 0x7F7F7F7Fu);  // 0x80 per zero byte
+        const uint32_t nib = ((z >> 7) & 1u) | ((z >> 14) & 2u) | ((z >> 21) & 4u) | ((z >> 28) & 8u);
+        m |= nib << (4 * j);
+    }
+    if (pos < lo) m &= lo - pos >= 16 ? 0u : (0xFFFFu << (uint32_t)(lo - pos)) & 0xFFFFu;
+    if (pos + 16 > hi) m &= hi <= pos ? 0u : (0xFFFFu >> (uint32_t)(pos + 16 - hi));
+    return m;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_txt_count(const char *__restrict__ txt, uint64_t body, uint64_t len,
+                                                     uint64_t tile0, uint32_t *__restrict__ cnt)
+{
+    __shared__ uint32_t red[WAVES];
+    const uint64_t pos = (tile0 + blockIdx.x) * TXT_TILE + (uint64_t)threadIdx.x * 16;
+    const uint4 v = *reinterpret_cast<const uint4 *>(txt + pos);
+    const uint32_t c = (uint32_t)__popc(nl_mask16(v, pos, body, len));
+    const uint32_t t = block_reduce(c, red);
+    if (threadIdx.x == 0) cnt[blockIdx.x] = t;
+}
+
+__device__ __forceinline__ uint32_t txt_code(char ch)
+{
+    // driver.cpp:228-232: A 0, C 1, G 2, T 3, every other byte 0
+    return ch == 'C' ? 1u : ch == 'G' ? 2u : ch == 'T' ? 3u : 0u;
+}
+
+// line `idx` starting at `start`: its first k characters, or an error when
+// the line (without one '\r' before its end) is shorter than k
+template <int W>
+__device__ __forceinline__ void pack_line(const char *__restrict__ txt, uint64_t len, uint64_t start, uint64_t idx,
+                                          uint32_t k, uint64_t *__restrict__ out, uint32_t *__restrict__ err)
+{
+    uint64_t w[2] = {0, 0};
+    bool bad = false;
+    for (uint32_t i = 0; i < k; i++) {
+        const uint64_t p = start + i;
+        const char ch = p < len ? txt[p] : '\n';
+        if (ch == '\n') {
+            bad = true;
+            break;
+        }
+        w[i >> 5] |= (uint64_t)txt_code(ch) << (2 * (i & 31));
+    }
+    if (!bad) {
+        const char nx = start + k < len ? txt[start + k] : '\n';
+        if (nx == '\n' && txt[start + k - 1] == '\r') bad = true;
+    }
+    if (bad) atomicOr(err, 1u);
+#pragma unroll
+    for (int j = 0; j < W; j++) out[idx * W + j] = w[j];
+}
+
+template <int W>
+__global__ __launch_bounds__(BLOCK) void k_txt_pack(const char *__restrict__ txt, uint64_t body, uint64_t len,
+                                                    uint64_t tile0, const uint32_t *__restrict__ base, uint64_t n,
+                                                    uint32_t k, uint64_t *__restrict__ out, uint32_t *__restrict__ err)
+{
+    __shared__ uint32_t wsum[WAVES];
+    const uint64_t pos = (tile0 + blockIdx.x) * TXT_TILE + (uint64_t)threadIdx.x * 16;
+    const uint4 v = *reinterpret_cast<const uint4 *>(txt + pos);
+    uint32_t m = nl_mask16(v, pos, body, len);
+    const uint32_t c = (uint32_t)__popc(m);
+    const uint32_t incl = wave_incl_scan(c);
+    const int w = threadIdx.x / 64;
+    if (__lane_id() == 63) wsum[w] = incl;
+    __syncthreads();
+    uint64_t j = base[blockIdx.x] + incl - c;
+    for (int q = 0; q < w; q++) j += wsum[q];
+    if (blockIdx.x == 0 && threadIdx.x == 0 && n > 0) pack_line<W>(txt, len, body, 0, k, out, err);
+    while (m) {
+        const uint32_t b = (uint32_t)__builtin_ctz(m);
+        m &= m - 1;
+        if (j + 1 < n) pack_line<W>(txt, len, pos + b + 1, j + 1, k, out, err);
+        j++;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 
 // grid of whole XCD rounds covering n_tiles tiles (see xcd_tile)
@@ -1317,7 +1415,11 @@ struct uc_ctx {
     DBuf otable, in, stage, bkt, tiles, hist, xhist, kmer2, flags2, src2, sub, perm, tiles2, hist2, cbhb, cbts,
         cbfirst, fboff, egroups, fbbase, scan_tmp, kmer, flags, src, pair_h, pair_b, boff, sethb, setts, etiles,
         toff, tcap, table, hit, link, hasp, pred, seglen, segnext, prevseg, chainA, chainB, cycB, list, clist, ulen, tsum,
-        htoff, setoff, text, small;
+        htoff, setoff, text, small, txt, txtcnt;
+    // pinned host chunks of the file reader and writer
+    char *pin[2] = {nullptr, nullptr};
+    hipEvent_t pin_ev[2] = {nullptr, nullptr};
+    bool pin_used[2] = {false, false};
     // contraction state
     uint64_t *chain = nullptr;  // ranked chain entries (chainA or chainB)
     uint32_t n_list = 0, salt = 0, shift = 5;
@@ -1334,9 +1436,29 @@ struct uc_ctx {
     std::vector<hipEvent_t> evpool;
 };
 
-namespace {
+// every device buffer of a context
+static std::vector<DBuf *> all_bufs(uc_ctx *c)
+{
+    return {&c->kmer2,  &c->flags2,  &c->src2,   &c->sub,     &c->perm,    &c->tiles2, &c->hist2,
+            &c->cbhb,   &c->cbts,    &c->cbfirst, &c->fboff,  &c->egroups, &c->fbbase, &c->otable,
+            &c->in,     &c->stage,   &c->bkt,    &c->tiles,   &c->hist,    &c->xhist,  &c->scan_tmp,
+            &c->kmer,   &c->flags,   &c->src,    &c->pair_h,  &c->pair_b,  &c->boff,   &c->sethb,
+            &c->setts,  &c->etiles,  &c->toff,   &c->tcap,    &c->table,   &c->hit,    &c->link,
+            &c->hasp,   &c->pred,    &c->seglen, &c->segnext, &c->prevseg, &c->chainA, &c->chainB,
+            &c->cycB,   &c->list,    &c->clist,  &c->ulen,    &c->tsum,    &c->htoff,  &c->setoff,
+            &c->text,   &c->small,   &c->txt,    &c->txtcnt};
+}
 
-}  // namespace
+static void free_pinned(uc_ctx *c)
+{
+    for (int i = 0; i < 2; i++) {
+        if (c->pin[i]) (void)hipHostFree(c->pin[i]);
+        if (c->pin_ev[i]) (void)hipEventDestroy(c->pin_ev[i]);
+        c->pin[i] = nullptr;
+        c->pin_ev[i] = nullptr;
+        c->pin_used[i] = false;
+    }
+}
 
 // tile-count argument of the tiled kernels: the top bit asks for the
 // XCD-contiguous renumbering of xcd_tile
@@ -1537,9 +1659,9 @@ void reset_job(uc_ctx *c)
 
 // ---- load -----------------------------------------------------------------
 
-int do_load(uc_ctx *c, const uint32_t *packed, const uint64_t *set_offsets, uint32_t n_sets, uint32_t k, int on_device)
+// k, word layout, masks and set layout of a job
+int begin_job(uc_ctx *c, const uint64_t *set_offsets, uint32_t n_sets, uint32_t k)
 {
-    reset_job(c);
     if (k < 2 || k > 64) return fail(c, UC_E_ARG, "k must be in [2, 64]");
     if (n_sets == 0) return fail(c, UC_E_ARG, "n_sets must be >= 1");
     c->k = k;
@@ -1551,7 +1673,6 @@ int do_load(uc_ctx *c, const uint32_t *packed, const uint64_t *set_offsets, uint
     if (c->set_off[0] != 0) return fail(c, UC_E_ARG, "set offsets must start at 0");
     c->n = c->set_off[n_sets];
     if (c->n >= 0xFFFFFFFFull) return fail(c, UC_E_LIMIT, "more than 2^32-2 k-mers");
-    const uint32_t rw = (k + 15) / 16;
     const uint32_t W = c->W;
     const uint32_t top_bits = 2 * k - 64 * (W - 1);
     c->top_mask = top_bits >= 64 ? ~0ull : ((1ull << top_bits) - 1);
@@ -1563,6 +1684,15 @@ int do_load(uc_ctx *c, const uint32_t *packed, const uint64_t *set_offsets, uint
         c->pm0 = (1ull << pbits) - 1;
         c->pm1 = 0;
     }
+    return UC_OK;
+}
+
+int do_load(uc_ctx *c, const uint32_t *packed, const uint64_t *set_offsets, uint32_t n_sets, uint32_t k, int on_device)
+{
+    reset_job(c);
+    CHK(begin_job(c, set_offsets, n_sets, k));
+    const uint32_t rw = (k + 15) / 16;
+    const uint32_t W = c->W;
     const size_t in_bytes = (size_t)c->n * rw * 4;
     if (rw == 2 * W) {
         // the reference packing with an even word count is the 64-bit layout
@@ -1589,6 +1719,173 @@ int do_load(uc_ctx *c, const uint32_t *packed, const uint64_t *set_offsets, uint
     }
     c->loaded = true;
     return UC_OK;
+}
+
+// ---- k-mer text on the device ----------------------------------------------
+
+constexpr uint64_t PIN_CHUNK = 64ull << 20;  // bytes per pinned host chunk
+
+int pinned(uc_ctx *c)
+{
+    for (int i = 0; i < 2; i++) {
+        if (!c->pin[i]) {
+            HIPCHK(c, hipHostMalloc((void **)&c->pin[i], PIN_CHUNK, hipHostMallocDefault));
+            c->pin_used[i] = false;
+        }
+        if (!c->pin_ev[i]) HIPCHK(c, hipEventCreateWithFlags(&c->pin_ev[i], hipEventDisableTiming));
+    }
+    return UC_OK;
+}
+
+// parse the text held in c->txt (len bytes, TXT_PAD zero bytes after it)
+template <int W>
+int parse_text(uc_ctx *c, uint64_t len, uint32_t k)
+{
+    const char *txt = P<char>(c->txt);
+    // line 1: the k-mer count (driver.cpp:296-304), read in 4 KiB steps
+    std::vector<char> h;
+    uint64_t got = 0, nl = len;
+    while (got < len && nl == len) {
+        const uint64_t step = std::min<uint64_t>(len - got, 4096);
+        h.resize(got + step);
+        CHK(d2h(c, h.data() + got, txt + got, step));
+        const void *q = std::memchr(h.data() + got, '\n', step);
+        if (q) nl = (uint64_t)((const char *)q - h.data());
+        got += step;
+    }
+    uint64_t p = 0, count = 0;
+    bool digit = false;
+    while (p < got && (h[p] == ' ' || h[p] == '\t')) p++;
+    while (p < got && h[p] >= '0' && h[p] <= '9') {
+        count = count * 10 + (uint64_t)(h[p] - '0');
+        digit = true;
+        p++;
+    }
+    if (!digit) return fail(c, UC_E_FORMAT, "no k-mer count on the first line");
+    const uint64_t body = nl < len ? nl + 1 : len;
+    uint64_t lines = 0, n = 0, nt = 0, tile0 = 0;
+    if (body < len) {
+        char last = 0;
+        CHK(d2h(c, &last, txt + len - 1, 1));
+        tile0 = body / TXT_TILE;
+        nt = (len + TXT_TILE - 1) / TXT_TILE - tile0;
+        CHK(ensure(c, c->txtcnt, (nt + 2) * 4));
+        uint32_t *cnt = P<uint32_t>(c->txtcnt);
+        KLAUNCH(c, "k_txt_count", k_txt_count, ((uint32_t)nt, BLOCK, 0), txt, body, len, tile0, cnt);
+        CHK(scan_excl<uint32_t>(c, cnt, cnt, nt, cnt + nt, c->scan_tmp));
+        uint32_t total = 0;
+        CHK(d2h(c, &total, cnt + nt, 4));
+        lines = (uint64_t)total + (last != '\n' ? 1 : 0);
+        n = std::min<uint64_t>(count, lines);
+    }
+    const uint64_t offs[2] = {0, n};
+    CHK(begin_job(c, offs, 1, k));
+    CHK(ensure(c, c->in, std::max<uint64_t>(n, 1) * W * 8));
+    if (n) {
+        uint32_t *cnt = P<uint32_t>(c->txtcnt);
+        CHK(fill<uint32_t>(c, cnt + nt + 1, 1, 0u));
+        KLAUNCH(c, "k_txt_pack", k_txt_pack<W>, ((uint32_t)nt, BLOCK, 0), txt, body, len, tile0, cnt, n, k,
+                P<uint64_t>(c->in), cnt + nt + 1);
+        uint32_t err = 0;
+        CHK(d2h(c, &err, cnt + nt + 1, 4));
+        if (err) return fail(c, UC_E_FORMAT, "a k-mer line is shorter than k");
+    }
+    c->kin = P<uint64_t>(c->in);
+    c->loaded = true;
+    return UC_OK;
+}
+
+int parse_loaded_text(uc_ctx *c, uint64_t len, uint32_t k)
+{
+    if (k < 2 || k > 64) return fail(c, UC_E_ARG, "k must be in [2, 64]");
+    CHK(fill<uint8_t>(c, P<uint8_t>(c->txt) + len, TXT_PAD, (uint8_t)0));
+    return k <= 32 ? parse_text<1>(c, len, k) : parse_text<2>(c, len, k);
+}
+
+int do_load_text(uc_ctx *c, const char *text, uint64_t len, uint32_t k, int on_device)
+{
+    reset_job(c);
+    CHK(ensure(c, c->txt, len + TXT_PAD));
+    if (len) {
+        if (on_device)
+            HIPCHK(c, hipMemcpyAsync(c->txt.p, text, len, hipMemcpyDeviceToDevice, c->stream));
+        else
+            CHK(h2d(c, c->txt.p, text, len));
+    }
+    return parse_loaded_text(c, len, k);
+}
+
+// the file through two pinned chunks: reading chunk i+1 overlaps the copy of chunk i
+int do_load_file(uc_ctx *c, const char *path, uint32_t k)
+{
+    reset_job(c);
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return fail(c, UC_E_IO, std::string("could not open input file ") + path);
+    uint64_t len = 0;
+    if (std::fseek(f, 0, SEEK_END) == 0) {
+        const long sz = std::ftell(f);
+        if (sz >= 0) len = (uint64_t)sz;
+    }
+    std::rewind(f);
+    int rc = ensure(c, c->txt, len + TXT_PAD);
+    if (rc == UC_OK) rc = pinned(c);
+    for (uint64_t off = 0, i = 0; rc == UC_OK && off < len; i++) {
+        const int b = (int)(i & 1);
+        if (c->pin_used[b] && hipEventSynchronize(c->pin_ev[b]) != hipSuccess) {
+            rc = fail(c, UC_E_HIP, "hipEventSynchronize");
+            break;
+        }
+        const uint64_t step = std::min<uint64_t>(PIN_CHUNK, len - off);
+        if (std::fread(c->pin[b], 1, step, f) != step) {
+            rc = fail(c, UC_E_IO, std::string("could not read input file ") + path);
+            break;
+        }
+        if (hipMemcpyAsync(P<char>(c->txt) + off, c->pin[b], step, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+            hipEventRecord(c->pin_ev[b], c->stream) != hipSuccess) {
+            rc = fail(c, UC_E_HIP, "hipMemcpyAsync");
+            break;
+        }
+        c->pin_used[b] = true;
+        off += step;
+    }
+    std::fclose(f);
+    if (rc != UC_OK) {
+        (void)hipStreamSynchronize(c->stream);
+        return rc;
+    }
+    return parse_loaded_text(c, len, k);
+}
+
+// the text appended to a file: the copy of chunk i+1 overlaps the write of chunk i
+int append_text(uc_ctx *c, const char *path)
+{
+    FILE *f = std::fopen(path, "ab");
+    if (!f) return fail(c, UC_E_IO, std::string("could not open output file ") + path);
+    int rc = pinned(c);
+    const uint64_t N = c->text_bytes, nch = (N + PIN_CHUNK - 1) / PIN_CHUNK;
+    auto issue = [&](uint64_t i) -> int {
+        const int b = (int)(i & 1);
+        const uint64_t off = i * PIN_CHUNK, step = std::min<uint64_t>(PIN_CHUNK, N - off);
+        HIPCHK(c, hipMemcpyAsync(c->pin[b], P<char>(c->text) + off, step, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipEventRecord(c->pin_ev[b], c->stream));
+        c->pin_used[b] = true;
+        return UC_OK;
+    };
+    if (rc == UC_OK && nch) rc = issue(0);
+    for (uint64_t i = 0; rc == UC_OK && i < nch; i++) {
+        if (i + 1 < nch) rc = issue(i + 1);
+        if (rc != UC_OK) break;
+        const int b = (int)(i & 1);
+        if (hipEventSynchronize(c->pin_ev[b]) != hipSuccess) {
+            rc = fail(c, UC_E_HIP, "hipEventSynchronize");
+            break;
+        }
+        const uint64_t step = std::min<uint64_t>(PIN_CHUNK, N - i * PIN_CHUNK);
+        if (std::fwrite(c->pin[b], 1, step, f) != step) rc = fail(c, UC_E_IO, std::string("could not write ") + path);
+    }
+    if (std::fclose(f) != 0 && rc == UC_OK) rc = fail(c, UC_E_IO, std::string("could not write ") + path);
+    (void)hipStreamSynchronize(c->stream);
+    return rc;
 }
 
 // ---- bucketing ---------------------------------------------------------------
@@ -2254,16 +2551,9 @@ int uc_ctx_destroy(uc_ctx *c)
     if (!c) return UC_OK;
     (void)hipSetDevice(c->opts.device);
     (void)hipStreamSynchronize(c->stream);
-    DBuf *bufs[] = {&c->kmer2,  &c->flags2,  &c->src2,   &c->sub,     &c->perm,   &c->tiles2, &c->hist2,
-                    &c->cbhb,   &c->cbts,    &c->cbfirst, &c->fboff,  &c->egroups, &c->fbbase,
-                    &c->otable, &c->in,      &c->stage,  &c->bkt,     &c->tiles,  &c->hist,   &c->xhist,
-                    &c->scan_tmp, &c->kmer,  &c->flags,  &c->src,     &c->pair_h, &c->pair_b, &c->boff,
-                    &c->sethb,  &c->setts,   &c->etiles, &c->toff,    &c->tcap,   &c->table,  &c->hit,
-                    &c->link,   &c->hasp,    &c->pred,    &c->seglen, &c->segnext, &c->prevseg, &c->chainA, &c->chainB,
-                    &c->cycB,   &c->list,    &c->clist,  &c->ulen,    &c->tsum,   &c->htoff,  &c->setoff,
-                    &c->text,   &c->small};
-    for (DBuf *b : bufs)
+    for (DBuf *b : all_bufs(c))
         if (b->p) (void)hipFree(b->p);
+    free_pinned(c);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &p : c->pending) {
@@ -2456,13 +2746,38 @@ int uc_text_append_file(uc_ctx *c, const char *path)
 {
     if (!c || !path) return UC_E_ARG;
     if (!c->emitted) return fail(c, UC_E_STATE, "no text");
-    std::vector<char> h(c->text_bytes);
-    CHK(uc_text_copy(c, h.data(), h.size()));
-    std::ofstream of(path, std::ofstream::out | std::ofstream::app | std::ofstream::binary);
-    if (!of.is_open()) return fail(c, UC_E_IO, std::string("could not open output file ") + path);
-    of.write(h.data(), (std::streamsize)h.size());
-    of.close();
-    if (!of) return fail(c, UC_E_IO, std::string("could not write output file ") + path);
+    CHK(select_dev(c));
+    return append_text(c, path);
+}
+
+int uc_load_text(uc_ctx *c, const char *text, uint64_t len, uint32_t k, int on_device)
+{
+    if (!c || (!text && len)) return UC_E_ARG;
+    CHK(select_dev(c));
+    return do_load_text(c, text, len, k, on_device);
+}
+
+int uc_load_file(uc_ctx *c, const char *path, uint32_t k)
+{
+    if (!c || !path) return UC_E_ARG;
+    if (k < 2 || k > 64) return fail(c, UC_E_ARG, "k must be in [2, 64]");
+    CHK(select_dev(c));
+    return do_load_file(c, path, k);
+}
+
+int uc_loaded_copy(uc_ctx *c, uint32_t *packed, uint64_t cap, uint64_t *n)
+{
+    if (!c) return UC_E_ARG;
+    if (!c->loaded) return fail(c, UC_E_STATE, "nothing loaded");
+    if (n) *n = c->n;
+    if (!packed) return UC_OK;
+    if (cap < c->n) return fail(c, UC_E_ARG, "destination too small");
+    CHK(select_dev(c));
+    const uint32_t W = c->W, rw = (c->k + 15) / 16;
+    std::vector<uint64_t> h((size_t)c->n * W);
+    CHK(d2h(c, h.data(), c->kin, h.size() * 8));
+    for (uint64_t i = 0; i < c->n; i++)
+        for (uint32_t j = 0; j < rw; j++) packed[i * rw + j] = (uint32_t)(h[i * W + j / 2] >> (32 * (j & 1)));
     return UC_OK;
 }
 
@@ -2486,19 +2801,12 @@ int uc_ctx_trim(uc_ctx *c)
     if (!c) return UC_E_ARG;
     CHK(select_dev(c));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    DBuf *bufs[] = {&c->kmer2,  &c->flags2,  &c->src2,   &c->sub,     &c->perm,    &c->tiles2, &c->hist2,
-                    &c->cbhb,   &c->cbts,    &c->cbfirst, &c->fboff,  &c->egroups, &c->fbbase, &c->otable,
-                    &c->in,     &c->stage,   &c->bkt,    &c->tiles,   &c->hist,    &c->xhist,  &c->scan_tmp,
-                    &c->kmer,   &c->flags,   &c->src,    &c->pair_h,  &c->pair_b,  &c->boff,   &c->sethb,
-                    &c->setts,  &c->etiles,  &c->toff,   &c->tcap,    &c->table,   &c->hit,    &c->link,
-                    &c->hasp,   &c->pred,    &c->seglen, &c->segnext, &c->prevseg, &c->chainA, &c->chainB,
-                    &c->cycB,   &c->list,    &c->clist,  &c->ulen,    &c->tsum,    &c->htoff,  &c->setoff,
-                    &c->text,   &c->small};
-    for (DBuf *b : bufs) {
+    for (DBuf *b : all_bufs(c)) {
         if (b->p) HIPCHK(c, hipFree(b->p));
         b->p = nullptr;
         b->cap = 0;
     }
+    free_pinned(c);
     c->held = 0;
     c->kin = nullptr;
     reset_job(c);

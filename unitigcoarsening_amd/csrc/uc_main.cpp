@@ -7,6 +7,8 @@
 // the first k characters of each line are used), same output (one unitig per
 // line, appended to the output file), same usage and I/O failures
 // (driver.cpp:424-428, :262-265; compact.h:12-15).  Phase times go to stdout.
+// The text is parsed on the device (uc_load_file) and the unitigs are
+// appended through pinned chunks (uc_text_append_file).
 // Build options the reference fixes at compile time come from the
 // environment: UC_BUCKETS (64), UC_MODE (batched|plain), UC_SEED, UC_DEVICE.
 #include <chrono>
@@ -39,18 +41,13 @@ int main(int argc, char **argv)
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
     };
 
-    uint32_t *packed = nullptr;
-    uint64_t n = 0;
-    int rc = uc_read_kmer_file(in_path, k, &packed, &n);
-    if (rc == UC_E_IO) {
+    // the reference's open check comes first (driver.cpp:262-265)
+    if (FILE *f = std::fopen(in_path, "rb")) {
+        std::fclose(f);
+    } else {
         printf("Error: Could not open input file. Exiting ...\n");
         exit(1);
     }
-    if (rc != UC_OK) {
-        fprintf(stderr, "uc_main: %s reading %s\n", uc_strerror(rc), in_path);
-        return EXIT_FAILURE;
-    }
-    printf("Read %llu k-mers in %.3fs\n", (unsigned long long)n, secs(t0));
 
     uc_opts o;
     uc_default_opts(&o);
@@ -60,14 +57,26 @@ int main(int argc, char **argv)
     const char *mode = std::getenv("UC_MODE");
     if (mode && std::strcmp(mode, "plain") == 0) o.mode = UC_MODE_PLAIN;
     uc_ctx *ctx = nullptr;
-    rc = uc_ctx_create(&o, &ctx);
+    int rc = uc_ctx_create(&o, &ctx);
     if (rc != UC_OK) {
         fprintf(stderr, "uc_main: cannot create context: %s\n", uc_strerror(rc));
-        uc_free(packed);
         return EXIT_FAILURE;
     }
-    rc = uc_compact(ctx, packed, n, k, l, 0);
-    uc_free(packed);
+    // text -> HBM through pinned chunks, parsed and packed on the device
+    rc = uc_load_file(ctx, in_path, k);
+    if (rc == UC_E_IO) {
+        printf("Error: Could not open input file. Exiting ...\n");
+        exit(1);
+    }
+    uint64_t n = 0;
+    if (rc == UC_OK) rc = uc_loaded_copy(ctx, nullptr, 0, &n);
+    if (rc != UC_OK) {
+        fprintf(stderr, "uc_main: %s reading %s: %s\n", uc_strerror(rc), in_path, uc_last_error(ctx));
+        uc_ctx_destroy(ctx);
+        return EXIT_FAILURE;
+    }
+    printf("Read %llu k-mers in %.3fs\n", (unsigned long long)n, secs(t0));
+    rc = uc_compact_loaded(ctx, l);
     if (rc != UC_OK) {
         fprintf(stderr, "uc_main: %s: %s\n", uc_strerror(rc), uc_last_error(ctx));
         uc_ctx_destroy(ctx);
