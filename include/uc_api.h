@@ -201,6 +201,35 @@ int uc_load_text(uc_ctx *ctx, const char *text, uint64_t len, uint32_t k, int on
  * (driver.cpp:262-265). */
 int uc_load_file(uc_ctx *ctx, const char *path, uint32_t k);
 
+/* ---- packed binary k-mer files -----------------------------------------
+ * A 32-byte header, then n * ceil(k/16) uint32 words in the reference
+ * packing (the comp_vt buffer of driver.cpp:227-249, little-endian).  At
+ * 3.1e9 63-mers this is 50 GB where the text is about 200 GB. */
+
+#define UC_PACKED_MAGIC "UCKMERS1"
+
+typedef struct {
+    char magic[8];      /* UC_PACKED_MAGIC, no terminator */
+    uint32_t k;
+    uint32_t words;     /* ceil(k / 16) */
+    uint64_t n;
+    uint64_t reserved;  /* 0 */
+} uc_packed_header;
+
+int uc_write_packed_file(const char *path, const uint32_t *packed, uint64_t n, uint32_t k);
+/* UC_E_IO if it cannot be read, UC_E_FORMAT if the header is not one. */
+int uc_read_packed_header(const char *path, uc_packed_header *h);
+/* Streams a packed file into HBM through 64 MiB pinned chunks (read of
+ * chunk i+1 overlapping the copy of chunk i) and loads it as one k-mer
+ * set; *k gets its k. */
+int uc_load_packed_file(uc_ctx *ctx, const char *path, uint32_t *k);
+
+/* Device address of the loaded k-mers in the 64-bit layout (ceil(k/32)
+ * uint64 words per k-mer, char i at bits 2(i%32) of word i/32; for k in
+ * 17-32 and 49-64 these are the bytes of the reference packing), valid until
+ * the next load.  NULL when nothing is loaded. */
+const uint64_t *uc_loaded_device_ptr(const uc_ctx *ctx);
+
 /* The loaded k-mers in the reference packing (ceil(k/16) uint32 per
  * k-mer); *n gets their count.  packed may be NULL to ask for the count. */
 int uc_loaded_copy(uc_ctx *ctx, uint32_t *packed, uint64_t cap, uint64_t *n);

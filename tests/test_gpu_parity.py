@@ -223,7 +223,9 @@ def test_bucketize_matches_oracle(use_table, genome):
     assert cnt.max() <= 2 and (cnt >= 1).all()
     # pairs: same k-mer, home copy flagged lent, duplicate flagged borrowed
     assert ph.shape[0] == (flags == uc.F_BORROW).sum() == (flags == uc.F_LENT).sum()
-    assert ph.shape[0] == ob["cross_rows"][-1]
+    # the reference lists every crossing k-mer twice: in the (home, suffix)
+    # and the (suffix, home) cross list at the same offset
+    assert 2 * ph.shape[0] == ob["cross_rows"][-1]
     assert np.array_equal(src[ph], src[pb])
     assert np.all(flags[ph] == uc.F_LENT) and np.all(flags[pb] == uc.F_BORROW)
     assert np.all(mins[src[ph], 0] % Bf != mins[src[ph], 1] % Bf)

@@ -97,18 +97,22 @@ def test_large_text_and_compaction_match_host_path():
 def test_device_buffer_input():
     import ctypes as C
 
-    import torch
-
+    # plain HIP runtime calls for the device buffer
+    hip = C.CDLL("libamdhip64.so")
     kmers = uc.synth_kmers(5, 30_000, 47)
     text = uc.format_text(kmers, 47)
-    # an odd offset into a device allocation: the library copies it to its own padded buffer
-    dev = torch.zeros(len(text) + 3, dtype=torch.uint8, device="cuda")
-    dev[3:] = torch.frombuffer(bytearray(text), dtype=torch.uint8).cuda()
-    torch.cuda.synchronize()
-    with uc.Compactor() as c:
-        rc = uc.lib().uc_load_text(c._ctx, C.c_char_p(dev.data_ptr() + 3), len(text), 47, 1)
-        assert rc == uc.UC_OK
-        assert np.array_equal(c.loaded(47), kmers)
+    ptr = C.c_void_p()
+    assert hip.hipMalloc(C.byref(ptr), C.c_size_t(len(text) + 3)) == 0
+    try:
+        # an odd offset: the library copies the text into its own padded buffer
+        dst = C.c_void_p(ptr.value + 3)
+        assert hip.hipMemcpy(dst, C.c_char_p(text), C.c_size_t(len(text)), 1) == 0  # host to device
+        with uc.Compactor() as c:
+            rc = uc.lib().uc_load_text(c._ctx, C.cast(dst, C.c_char_p), len(text), 47, 1)
+            assert rc == uc.UC_OK
+            assert np.array_equal(c.loaded(47), kmers)
+    finally:
+        hip.hipFree(ptr)
 
 
 @pytest.mark.parametrize("name", ["hand_k4", "genome_k31", "genome_k63", "repeats_errors_k31"])
@@ -158,3 +162,31 @@ def test_writer_multi_chunk(tmp_path):
         out = tmp_path / "big.txt"
         _check_append(c, out)
     assert out.read_bytes() == want
+
+
+@pytest.mark.parametrize("k", [15, 31, 47, 63])
+def test_packed_file_load_and_cli(k, tmp_path):
+    import subprocess
+
+    kmers = uc.medium_model(40 + k, 60_000, k)
+    p = tmp_path / "in.ucb"
+    uc.write_packed_file(str(p), kmers, k)
+    l = 7 if k == 15 else 11
+    with uc.Compactor(n_buckets=64) as c:
+        assert c.load_packed_file(str(p)) == k
+        assert np.array_equal(c.loaded(k), kmers)
+        c.compact_loaded(l)
+        want = c.text()
+        assert want == c.compact(kmers, k, l)
+    out = tmp_path / "o.txt"
+    r = subprocess.run([uc.CLI_PATH, str(p), str(k), str(l), str(out)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert uc.sorted_lines(out.read_bytes()) == uc.sorted_lines(want)
+    # a truncated file is a format error
+    raw = p.read_bytes()
+    q = tmp_path / "cut.ucb"
+    q.write_bytes(raw[:-4])
+    with uc.Compactor() as c:
+        with pytest.raises(uc.UcError) as e:
+            c.load_packed_file(str(q))
+        assert e.value.status == uc.UC_E_FORMAT

@@ -95,3 +95,24 @@ def test_cli_missing_input(tmp_path):
                        capture_output=True, text=True)
     assert r.returncode == 1
     assert "Error: Could not open input file. Exiting ..." in r.stdout
+
+
+def test_packed_file_roundtrip(tmp_path):
+    for k in (4, 17, 31, 33, 63, 64):
+        kmers = uc.synth_kmers(k, 3000, k)
+        p = tmp_path / f"k{k}.ucb"
+        uc.write_packed_file(str(p), kmers, k)
+        h = uc.read_packed_header(str(p))
+        assert h == {"k": k, "words": uc.words_per_kmer(k), "n": kmers.shape[0]}
+        raw = p.read_bytes()
+        assert raw[:8] == b"UCKMERS1" and len(raw) == 32 + kmers.size * 4
+        assert np.array_equal(np.frombuffer(raw[32:], np.uint32).reshape(kmers.shape), kmers)
+    # text is not a packed file; a missing file is an I/O error
+    t = tmp_path / "t.txt"
+    t.write_bytes(b"1\nACGT\n")
+    with pytest.raises(uc.UcError) as e:
+        uc.read_packed_header(str(t))
+    assert e.value.status == uc.UC_E_FORMAT
+    with pytest.raises(uc.UcError) as e:
+        uc.read_packed_header(str(tmp_path / "none.ucb"))
+    assert e.value.status == uc.UC_E_IO

@@ -93,6 +93,16 @@ class UcServerOpts(C.Structure):
     ]
 
 
+class UcPackedHeader(C.Structure):
+    _fields_ = [
+        ("magic", C.c_char * 8),
+        ("k", C.c_uint32),
+        ("words", C.c_uint32),
+        ("n", C.c_uint64),
+        ("reserved", C.c_uint64),
+    ]
+
+
 UC_JOB_QUEUED, UC_JOB_RUNNING, UC_JOB_DONE, UC_JOB_FAILED = 0, 1, 2, 3
 
 
@@ -147,6 +157,10 @@ API = {
     "uc_load_text": (C.c_int, [_P, C.c_char_p, C.c_uint64, C.c_uint32, C.c_int]),
     "uc_load_file": (C.c_int, [_P, C.c_char_p, C.c_uint32]),
     "uc_loaded_copy": (C.c_int, [_P, _U32P, C.c_uint64, _U64P]),
+    "uc_loaded_device_ptr": (C.c_void_p, [_P]),
+    "uc_write_packed_file": (C.c_int, [C.c_char_p, _U32P, C.c_uint64, C.c_uint32]),
+    "uc_read_packed_header": (C.c_int, [C.c_char_p, C.POINTER(UcPackedHeader)]),
+    "uc_load_packed_file": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_uint32)]),
     "uc_ctx_trim": (C.c_int, [_P]),
     "uc_ctx_held_bytes": (C.c_uint64, [_P]),
     "uc_job_bytes_estimate": (C.c_uint64, [C.c_uint64, C.c_uint32]),
@@ -366,6 +380,12 @@ class Compactor:
         """Read and parse a k-mer file on the device (uc_load_file)."""
         _check(lib().uc_load_file(self._ctx, os.fsencode(path), k), self._ctx)
 
+    def load_packed_file(self, path):
+        """Stream a packed k-mer file into HBM (uc_load_packed_file); returns its k."""
+        k = C.c_uint32()
+        _check(lib().uc_load_packed_file(self._ctx, os.fsencode(path), C.byref(k)), self._ctx)
+        return k.value
+
     def loaded(self, k):
         """The loaded k-mers in the reference packing, shape (n, ceil(k/16))."""
         L = lib()
@@ -473,6 +493,18 @@ class Compactor:
 
     def held_bytes(self):
         return lib().uc_ctx_held_bytes(self._ctx)
+
+
+def write_packed_file(path, packed, k):
+    """Packed binary k-mer file (uc_write_packed_file)."""
+    packed = np.ascontiguousarray(packed, dtype=np.uint32).reshape(-1, words_per_kmer(k))
+    _check(lib().uc_write_packed_file(os.fsencode(path), packed.ctypes.data_as(_U32P), packed.shape[0], k))
+
+
+def read_packed_header(path):
+    h = UcPackedHeader()
+    _check(lib().uc_read_packed_header(os.fsencode(path), C.byref(h)))
+    return {"k": h.k, "words": h.words, "n": h.n}
 
 
 def job_bytes_estimate(n, k):

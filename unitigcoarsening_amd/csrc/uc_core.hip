@@ -1815,7 +1815,36 @@ int do_load_text(uc_ctx *c, const char *text, uint64_t len, uint32_t k, int on_d
     return parse_loaded_text(c, len, k);
 }
 
-// the file through two pinned chunks: reading chunk i+1 overlaps the copy of chunk i
+// `len` bytes of an open file into device memory through two pinned
+// chunks: reading chunk i+1 overlaps the copy of chunk i.  UC_E_FORMAT when
+// the file ends early.
+int stream_file(uc_ctx *c, FILE *f, uint64_t len, char *dst, const char *path)
+{
+    CHK(pinned(c));
+    int rc = UC_OK;
+    for (uint64_t off = 0, i = 0; off < len; i++) {
+        const int b = (int)(i & 1);
+        if (c->pin_used[b] && hipEventSynchronize(c->pin_ev[b]) != hipSuccess) {
+            rc = fail(c, UC_E_HIP, "hipEventSynchronize");
+            break;
+        }
+        const uint64_t step = std::min<uint64_t>(PIN_CHUNK, len - off);
+        if (std::fread(c->pin[b], 1, step, f) != step) {
+            rc = fail(c, std::feof(f) ? UC_E_FORMAT : UC_E_IO, std::string("short read of ") + path);
+            break;
+        }
+        if (hipMemcpyAsync(dst + off, c->pin[b], step, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+            hipEventRecord(c->pin_ev[b], c->stream) != hipSuccess) {
+            rc = fail(c, UC_E_HIP, "hipMemcpyAsync");
+            break;
+        }
+        c->pin_used[b] = true;
+        off += step;
+    }
+    if (rc != UC_OK) (void)hipStreamSynchronize(c->stream);
+    return rc;
+}
+
 int do_load_file(uc_ctx *c, const char *path, uint32_t k)
 {
     reset_job(c);
@@ -1828,32 +1857,43 @@ int do_load_file(uc_ctx *c, const char *path, uint32_t k)
     }
     std::rewind(f);
     int rc = ensure(c, c->txt, len + TXT_PAD);
-    if (rc == UC_OK) rc = pinned(c);
-    for (uint64_t off = 0, i = 0; rc == UC_OK && off < len; i++) {
-        const int b = (int)(i & 1);
-        if (c->pin_used[b] && hipEventSynchronize(c->pin_ev[b]) != hipSuccess) {
-            rc = fail(c, UC_E_HIP, "hipEventSynchronize");
-            break;
-        }
-        const uint64_t step = std::min<uint64_t>(PIN_CHUNK, len - off);
-        if (std::fread(c->pin[b], 1, step, f) != step) {
-            rc = fail(c, UC_E_IO, std::string("could not read input file ") + path);
-            break;
-        }
-        if (hipMemcpyAsync(P<char>(c->txt) + off, c->pin[b], step, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-            hipEventRecord(c->pin_ev[b], c->stream) != hipSuccess) {
-            rc = fail(c, UC_E_HIP, "hipMemcpyAsync");
-            break;
-        }
-        c->pin_used[b] = true;
-        off += step;
-    }
+    if (rc == UC_OK) rc = stream_file(c, f, len, P<char>(c->txt), path);
     std::fclose(f);
-    if (rc != UC_OK) {
-        (void)hipStreamSynchronize(c->stream);
-        return rc;
-    }
+    if (rc != UC_OK) return rc;
     return parse_loaded_text(c, len, k);
+}
+
+int do_load_packed_file(uc_ctx *c, const char *path, uint32_t *k_out)
+{
+    reset_job(c);
+    uc_packed_header h;
+    const int hr = uc_read_packed_header(path, &h);
+    if (hr == UC_E_IO) return fail(c, UC_E_IO, std::string("could not open input file ") + path);
+    if (hr != UC_OK) return fail(c, hr, std::string("not a packed k-mer file: ") + path);
+    const uint64_t offs[2] = {0, h.n};
+    CHK(begin_job(c, offs, 1, h.k));
+    const uint32_t rw = h.words, W = c->W;
+    const uint64_t bytes = h.n * rw * 4;
+    // even word counts are the 64-bit layout already; odd ones are widened
+    DBuf &dst = rw == 2 * W ? c->in : c->stage;
+    CHK(ensure(c, dst, std::max<uint64_t>(bytes, 8)));
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return fail(c, UC_E_IO, std::string("could not open input file ") + path);
+    int rc = std::fseek(f, (long)sizeof(uc_packed_header), SEEK_SET) == 0 ? UC_OK : fail(c, UC_E_IO, "seek");
+    if (rc == UC_OK) rc = stream_file(c, f, bytes, P<char>(dst), path);
+    std::fclose(f);
+    if (rc != UC_OK) return rc;
+    if (rw != 2 * W) {
+        CHK(ensure(c, c->in, std::max<uint64_t>(h.n, 1) * W * 8));
+        if (h.n)
+            KLAUNCH(c, "k_widen", k_widen, (std::min<uint32_t>(grid_for(h.n), 65536), BLOCK, 0), P<uint32_t>(c->stage),
+                    h.n, rw, W, P<uint64_t>(c->in));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->kin = P<uint64_t>(c->in);
+    c->loaded = true;
+    if (k_out) *k_out = h.k;
+    return UC_OK;
 }
 
 // the text appended to a file: the copy of chunk i+1 overlaps the write of chunk i
@@ -2764,6 +2804,15 @@ int uc_load_file(uc_ctx *c, const char *path, uint32_t k)
     CHK(select_dev(c));
     return do_load_file(c, path, k);
 }
+
+int uc_load_packed_file(uc_ctx *c, const char *path, uint32_t *k)
+{
+    if (!c || !path) return UC_E_ARG;
+    CHK(select_dev(c));
+    return do_load_packed_file(c, path, k);
+}
+
+const uint64_t *uc_loaded_device_ptr(const uc_ctx *c) { return (c && c->loaded) ? c->kin : nullptr; }
 
 int uc_loaded_copy(uc_ctx *c, uint32_t *packed, uint64_t cap, uint64_t *n)
 {

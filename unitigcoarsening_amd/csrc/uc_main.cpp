@@ -7,7 +7,8 @@
 // the first k characters of each line are used), same output (one unitig per
 // line, appended to the output file), same usage and I/O failures
 // (driver.cpp:424-428, :262-265; compact.h:12-15).  Phase times go to stdout.
-// The text is parsed on the device (uc_load_file) and the unitigs are
+// The text is parsed on the device (uc_load_file), a packed binary k-mer
+// file (uc_write_packed_file) is streamed in as it is, and the unitigs are
 // appended through pinned chunks (uc_text_append_file).
 // Build options the reference fixes at compile time come from the
 // environment: UC_BUCKETS (64), UC_MODE (batched|plain), UC_SEED, UC_DEVICE.
@@ -62,8 +63,20 @@ int main(int argc, char **argv)
         fprintf(stderr, "uc_main: cannot create context: %s\n", uc_strerror(rc));
         return EXIT_FAILURE;
     }
-    // text -> HBM through pinned chunks, parsed and packed on the device
-    rc = uc_load_file(ctx, in_path, k);
+    // a packed binary k-mer file streams straight into HBM; text goes to HBM
+    // through pinned chunks and is parsed and packed on the device
+    uc_packed_header ph;
+    if (uc_read_packed_header(in_path, &ph) == UC_OK) {
+        if (ph.k != k) {
+            fprintf(stderr, "uc_main: %s holds %u-mers, not %u-mers\n", in_path, ph.k, k);
+            uc_ctx_destroy(ctx);
+            return EXIT_FAILURE;
+        }
+        uint32_t fk = 0;
+        rc = uc_load_packed_file(ctx, in_path, &fk);
+    } else {
+        rc = uc_load_file(ctx, in_path, k);
+    }
     if (rc == UC_E_IO) {
         printf("Error: Could not open input file. Exiting ...\n");
         exit(1);

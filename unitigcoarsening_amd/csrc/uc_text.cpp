@@ -154,3 +154,40 @@ extern "C" int uc_read_kmer_file(const char *path, uint32_t k, uint32_t **packed
 }
 
 extern "C" void uc_free(void *p) { std::free(p); }
+
+// ---- packed binary k-mer files: a 32-byte header (magic, k, words per
+// k-mer, count, zero) and the reference packing, little-endian
+
+extern "C" int uc_write_packed_file(const char *path, const uint32_t *packed, uint64_t n, uint32_t k)
+{
+    if (!path || (!packed && n) || k < 2 || k > 64) return UC_E_ARG;
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return UC_E_IO;
+    uc_packed_header h;
+    std::memcpy(h.magic, UC_PACKED_MAGIC, 8);
+    h.k = k;
+    h.words = (k + 15) / 16;
+    h.n = n;
+    h.reserved = 0;
+    bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1;
+    const uint64_t words = n * h.words;
+    for (uint64_t off = 0; ok && off < words;) {
+        const uint64_t step = std::min<uint64_t>(words - off, 1ull << 24);
+        ok = std::fwrite(packed + off, 4, step, f) == step;
+        off += step;
+    }
+    ok = (std::fclose(f) == 0) && ok;
+    return ok ? UC_OK : UC_E_IO;
+}
+
+extern "C" int uc_read_packed_header(const char *path, uc_packed_header *h)
+{
+    if (!path || !h) return UC_E_ARG;
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return UC_E_IO;
+    const bool got = std::fread(h, sizeof(*h), 1, f) == 1;
+    std::fclose(f);
+    if (!got || std::memcmp(h->magic, UC_PACKED_MAGIC, 8) != 0) return UC_E_FORMAT;
+    if (h->k < 2 || h->k > 64 || h->words != (h->k + 15) / 16) return UC_E_FORMAT;
+    return UC_OK;
+}
