@@ -121,6 +121,9 @@ typedef struct {
     uint64_t largest_bucket;
     uint64_t n_fine_buckets;  /* fine buckets (coarse buckets split by further minimizer-order bits) */
     uint64_t largest_fine_bucket;
+    uint32_t bucket_tile;    /* k-mers per bucketing tile */
+    uint32_t fine_tile;      /* slots per fine-sort tile (0: no fine sort) */
+    uint32_t lds_limit;      /* LDS bytes a workgroup may use on the device */
 } uc_bucket_info;
 
 int uc_bucket_info_get(uc_ctx *ctx, uc_bucket_info *info);

@@ -61,6 +61,9 @@ class UcBucketInfo(C.Structure):
         ("largest_bucket", C.c_uint64),
         ("n_fine_buckets", C.c_uint64),
         ("largest_fine_bucket", C.c_uint64),
+        ("bucket_tile", C.c_uint32),
+        ("fine_tile", C.c_uint32),
+        ("lds_limit", C.c_uint32),
     ]
 
 

@@ -4,23 +4,27 @@
 //   1. bucketing   [lmin_buckets.h:288-433, driver.cpp:137-225]
 //      k_lmin_count: prefix/suffix l-minimizers per k-mer, per-tile bucket
 //      histogram in LDS; device scan; k_lmin_scatter: stable wavefront-
-//      ballot ranking, crossing k-mers duplicated into their suffix bucket,
-//      (home, borrow) pair list.
+//      ballot ranking, LDS-staged coalesced writes, crossing k-mers
+//      duplicated into their suffix bucket, (home, borrow) pair list;
+//      k_sub_count / k_sub_scatter: the same counting sort of each coarse
+//      bucket by the next bits of the minimizer order (fine buckets).
 //   2. edge finding [assembler.h:113-214]
-//      per bucket: k_edge_insert (CAS (k-1)-prefixes into an open-
-//      addressing table, mark shared prefixes), k_edge_lookup (probe with
-//      the (k-1)-suffix, count in-degree on the hit slot), k_edge_resolve
-//      (keep u->v iff the prefix is unshared and has exactly one hit).
+//      k_edge_lds: one workgroup per group of fine buckets, k-mers staged
+//      in LDS, insert / lookup / resolve against an LDS table; edges inside
+//      the group are chained into runs by pointer jumping in LDS.  Fine
+//      buckets too large for LDS use HBM tables (k_edge_insert / lookup /
+//      resolve).
 //   3. contraction [coarseners.h:68-485, heuristics.h:105-256,
 //      compact_graph.h, driver.cpp:527-587]
 //      k_splice re-attaches each borrowed duplicate's out-edge to its home
-//      copy; splitters (path heads + a hashed 2^-s sample) walk their
-//      segment (k_walk); segment chains are ranked by pointer jumping
-//      (k_chain_*); k_final gives every k-mer (unitig head, position).
-//      Isolated cycles are cut at their lexicographically smallest k-mer
-//      and the contraction repeats.
+//      copy; splitters (path heads + a hashed 2^-s sample of run heads)
+//      walk their segment run by run (k_seg_count); segment chains are
+//      ranked by pointer jumping (k_chain_*).  Isolated cycles are cut at
+//      their lexicographically smallest k-mer and the contraction repeats.
 //   4. emission    [compact.h:59-210, lmin_buckets.h:172-283]
-//      unitig scan, k_order (k-mers in path order), k_emit (text).
+//      k_head_emit / k_rand_emit give every run its text index;
+//      k_slot_emit writes every k-mer's character in parallel.
+//   5. text input  [driver.cpp:227-354]: k_txt_count / k_txt_pack.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -342,21 +346,17 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_count(const uint64_t *__restrict
     for (uint32_t i = threadIdx.x; i <= p.B; i += BLOCK) cnt[i] = 0;
     __syncthreads();
     const uint32_t bmask = p.B - 1;
-#pragma unroll
-    for (int j = 0; j < BKT_IPT; j++) {
-        const uint32_t idx = j * BLOCK + threadIdx.x;
-        if (idx < t.count) {
-            const uint64_t g = t.kbegin + idx;
-            const Kmer<W> x = load_kmer<W>(kin, g, top_mask);
-            uint32_t o1, o2;
-            double_lmin<W>(x, p, o1, o2);
-            const uint32_t f1 = o1 & t.fmask, f2 = o2 & t.fmask;
-            bkt[g] = make_uint2(f1, f2);
-            atomicAdd(&cnt[f1 & bmask], 1u);
-            if (f1 != f2) {
-                atomicAdd(&cnt[f2 & bmask], 1u);
-                atomicAdd(&cnt[p.B], 1u);
-            }
+    for (uint32_t idx = threadIdx.x; idx < t.count; idx += BLOCK) {
+        const uint64_t g = t.kbegin + idx;
+        const Kmer<W> x = load_kmer<W>(kin, g, top_mask);
+        uint32_t o1, o2;
+        double_lmin<W>(x, p, o1, o2);
+        const uint32_t f1 = o1 & t.fmask, f2 = o2 & t.fmask;
+        bkt[g] = make_uint2(f1, f2);
+        atomicAdd(&cnt[f1 & bmask], 1u);
+        if (f1 != f2) {
+            atomicAdd(&cnt[f2 & bmask], 1u);
+            atomicAdd(&cnt[p.B], 1u);
         }
     }
     __syncthreads();
@@ -384,10 +384,45 @@ __device__ __forceinline__ void match_bucket(uint32_t v, uint32_t nbits, uint64_
     }
 }
 
+// exclusive scan of a[0..n) in place (n <= 4 * BLOCK); returns the total
+__device__ uint32_t block_excl_scan(uint32_t *a, uint32_t n, uint32_t *wsum)
+{
+    const uint32_t per = (n + BLOCK - 1) / BLOCK;
+    const uint32_t b0 = threadIdx.x * per, b1 = min(n, b0 + per);
+    uint32_t loc = 0;
+    for (uint32_t i = b0; i < b1; i++) loc += a[i];
+    const uint32_t incl = wave_incl_scan(loc);
+    const int w = threadIdx.x / 64;
+    if (__lane_id() == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t pre = incl - loc, tot = 0;
+    for (int q = 0; q < WAVES; q++) {
+        if (q < w) pre += wsum[q];
+        tot += wsum[q];
+    }
+    for (uint32_t i = b0; i < b1; i++) {
+        const uint32_t v = a[i];
+        a[i] = pre;
+        pre += v;
+    }
+    __syncthreads();
+    return tot;
+}
+
 // Stable scatter into the bucket CSR: inside a tile, placements keep the
 // order of their k-mers (home copy before borrowed copy when both land in
 // one bucket); per wave the rank among placements bound for the same bucket
-// comes from log2(B) ballots of the bucket bits.
+// comes from log2(B) ballots of the bucket bits.  Placements are staged in
+// LDS sorted by bucket (tile starts from the scanned histogram) and written
+// out in that order, so consecutive lanes store consecutive addresses of one
+// bucket's run.
+template <int W>
+__host__ __device__ constexpr size_t lmin_lds_bytes(uint32_t tile, uint32_t B, bool src)
+{
+    const size_t P = 2 * (size_t)tile;  // at most two placements per k-mer
+    return P * 8 * W + (src ? P * 4 : 0) + ((size_t)(B + 1) * (WAVES + 1) + 2 * (size_t)B) * 4 + P * 4 + P;
+}
+
 template <int W>
 __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restrict__ kin, const BTile *__restrict__ tiles,
                                                         uint32_t B, uint32_t nbits, uint64_t top_mask,
@@ -397,19 +432,39 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restri
                                                         uint8_t *__restrict__ flags, uint32_t *__restrict__ src,
                                                         uint32_t *__restrict__ pair_home,
                                                         uint32_t *__restrict__ pair_borrow,
-                                                        uint16_t *__restrict__ sub)
+                                                        uint16_t *__restrict__ sub, uint32_t tile)
 {
-    extern __shared__ uint32_t sh[];
-    uint32_t *run = sh;              // B+1 running counts
-    uint32_t *wcnt = sh + (B + 1);   // WAVES x (B+1) per-wave counts
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ uint32_t wsum[WAVES];
+    const bool keep = src != nullptr;
+    const size_t P = 2 * (size_t)tile;
+    Kmer<W> *skm = reinterpret_cast<Kmer<W> *>(lds);
+    uint32_t *ssrc = reinterpret_cast<uint32_t *>(lds + P * 8 * W);
+    uint32_t *run = reinterpret_cast<uint32_t *>(lds + P * 8 * W + (keep ? P * 4 : 0));  // B+1 running counts
+    uint32_t *wcnt = run + (B + 1);                                                         // WAVES x (B+1)
+    uint32_t *tstart = wcnt + WAVES * (B + 1);
+    uint32_t *delta = tstart + B;
+    uint16_t *ssub = reinterpret_cast<uint16_t *>(delta + B);
+    uint16_t *sbk = ssub + P;
+    uint8_t *sfl = reinterpret_cast<uint8_t *>(sbk + P);
     const BTile t = tiles[blockIdx.x];
-    for (uint32_t i = threadIdx.x; i < (B + 1) * (WAVES + 1); i += BLOCK) sh[i] = 0;
+    for (uint32_t i = threadIdx.x; i < (B + 1) * (WAVES + 1); i += BLOCK) run[i] = 0;
+    // this tile's count per bucket is the next histogram entry minus its own
+    for (uint32_t v = threadIdx.x; v < B; v += BLOCK) {
+        const uint64_t hi = t.hbase + (uint64_t)v * t.ts + t.tlocal;
+        const uint32_t g0 = hscan[hi];
+        tstart[v] = hscan[hi + 1] - g0;
+        delta[v] = g0;
+    }
+    __syncthreads();
+    const uint32_t placed = block_excl_scan(tstart, B, wsum);
+    for (uint32_t v = threadIdx.x; v < B; v += BLOCK) delta[v] -= tstart[v];
     __syncthreads();
     const int lane = __lane_id(), w = threadIdx.x / 64;
     const uint64_t lt = (1ull << lane) - 1ull, me = 1ull << lane;
     const uint32_t xbase = xscan[blockIdx.x];
-    for (int j = 0; j < BKT_IPT; j++) {
-        const uint32_t idx = j * BLOCK + threadIdx.x;
+    for (uint32_t j0 = 0; j0 < t.count; j0 += BLOCK) {
+        const uint32_t idx = j0 + threadIdx.x;
         const bool valid = idx < t.count;
         const uint64_t g = t.kbegin + idx;
         uint32_t bp = 0, bs = 0, fp = 0, fs = 0;
@@ -445,38 +500,47 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restri
         __syncthreads();
         if (valid) {
             const Kmer<W> x = load_kmer<W>(kin, g, top_mask);
-            uint32_t ph = run[bp] + (uint32_t)(__popcll(hpH & lt) + __popcll(hsH & lt));
-            for (int q = 0; q < w; q++) ph += wcnt[q * (B + 1) + bp];
-            const uint32_t sh_slot = hscan[t.hbase + (uint64_t)bp * t.ts + t.tlocal] + ph;
-            store_kmer<W>(kout, sh_slot, x);
-            flags[sh_slot] = cross ? F_LENT : 0;
-            sub[sh_slot] = (uint16_t)fp;
-            if (src) src[sh_slot] = (uint32_t)g;
+            uint32_t lh = tstart[bp] + run[bp] + (uint32_t)(__popcll(hpH & lt) + __popcll(hsH & lt));
+            for (int q = 0; q < w; q++) lh += wcnt[q * (B + 1) + bp];
+            skm[lh] = x;
+            sfl[lh] = cross ? F_LENT : 0;
+            ssub[lh] = (uint16_t)fp;
+            sbk[lh] = (uint16_t)bp;
+            if (keep) ssrc[lh] = (uint32_t)g;
             if (cross) {
-                uint32_t pb = run[bs] + (uint32_t)(__popcll(hpB & lt) + __popcll(hsB & lt)) + (bp == bs ? 1u : 0u);
-                for (int q = 0; q < w; q++) pb += wcnt[q * (B + 1) + bs];
-                const uint32_t sb_slot = hscan[t.hbase + (uint64_t)bs * t.ts + t.tlocal] + pb;
-                store_kmer<W>(kout, sb_slot, x);
-                flags[sb_slot] = F_BORROW;
-                sub[sb_slot] = (uint16_t)fs;
-                if (src) src[sb_slot] = (uint32_t)g;
+                uint32_t lb = tstart[bs] + run[bs] + (uint32_t)(__popcll(hpB & lt) + __popcll(hsB & lt)) +
+                              (bp == bs ? 1u : 0u);
+                for (int q = 0; q < w; q++) lb += wcnt[q * (B + 1) + bs];
+                skm[lb] = x;
+                sfl[lb] = F_BORROW;
+                ssub[lb] = (uint16_t)fs;
+                sbk[lb] = (uint16_t)bs;
+                if (keep) ssrc[lb] = (uint32_t)g;
                 uint32_t px = run[B] + (uint32_t)__popcll(C & lt);
                 for (int q = 0; q < w; q++) px += wcnt[q * (B + 1) + B];
-                pair_home[xbase + px] = sh_slot;
-                pair_borrow[xbase + px] = sb_slot;
+                pair_home[xbase + px] = delta[bp] + lh;
+                pair_borrow[xbase + px] = delta[bs] + lb;
             }
         }
         __syncthreads();
         for (uint32_t v = threadIdx.x; v <= B; v += BLOCK) {
-            uint32_t s = 0;
+            uint32_t a = 0;
 #pragma unroll
             for (int q = 0; q < WAVES; q++) {
-                s += wcnt[q * (B + 1) + v];
+                a += wcnt[q * (B + 1) + v];
                 wcnt[q * (B + 1) + v] = 0;
             }
-            run[v] += s;
+            run[v] += a;
         }
         __syncthreads();
+    }
+    // write out in bucket order
+    for (uint32_t li = threadIdx.x; li < placed; li += BLOCK) {
+        const uint32_t ns = delta[sbk[li]] + li;
+        store_kmer<W>(kout, ns, skm[li]);
+        flags[ns] = sfl[li];
+        sub[ns] = ssub[li];
+        if (keep) src[ns] = ssrc[li];
     }
 }
 
@@ -511,60 +575,117 @@ __global__ __launch_bounds__(BLOCK) void k_sub_count(const uint16_t *__restrict_
     for (uint32_t v = threadIdx.x; v < S; v += BLOCK) hist[t.hbase + (uint64_t)v * t.ts + t.tlocal] = cnt[v];
 }
 
+// Pass-2 scatter with LDS staging.  A tile (`tile` slots of one coarse
+// bucket) is ranked per wave: each wave owns tile/WAVES consecutive slots,
+// counts its slots per fine bucket (ballot groups, one LDS add per group),
+// the counts are scanned once per tile, and a second sweep gives every slot
+// its stable position inside the tile, sorted by fine bucket.  The slots are
+// staged there in LDS and written out in that order, so consecutive lanes
+// store consecutive addresses of one fine bucket's run.
+template <int W>
+__host__ __device__ constexpr size_t sub_lds_bytes(uint32_t tile, uint32_t S, bool src)
+{
+    return (size_t)tile * 8 * W + (src ? (size_t)tile * 4 : 0) + ((size_t)WAVES * S + 2 * (size_t)S) * 4 +
+           (size_t)tile * 2 + tile;
+}
+
+
 template <int W>
 __global__ __launch_bounds__(BLOCK) void k_sub_scatter(const BTile *__restrict__ tiles, const uint16_t *__restrict__ sub,
                                                        const uint32_t *__restrict__ hscan, const uint64_t *__restrict__ kin,
                                                        const uint8_t *__restrict__ fin, const uint32_t *__restrict__ sin,
                                                        uint64_t *__restrict__ kout, uint8_t *__restrict__ fout,
-                                                       uint32_t *__restrict__ sout, uint32_t *__restrict__ perm)
+                                                       uint32_t *__restrict__ sout, uint32_t *__restrict__ perm,
+                                                       uint32_t tile)
 {
-    extern __shared__ uint32_t sh[];
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ uint32_t wsum[WAVES];
     const BTile t = tiles[blockIdx.x];
     const uint32_t S = t.fmask;
     const uint32_t nbits = (uint32_t)__builtin_ctz(S);
-    uint32_t *run = sh;          // S running counts
-    uint32_t *wcnt = sh + S;     // WAVES x S per-wave counts
-    for (uint32_t i = threadIdx.x; i < S * (WAVES + 1); i += BLOCK) sh[i] = 0;
-    __syncthreads();
+    const bool src = sin != nullptr;
+    Kmer<W> *skm = reinterpret_cast<Kmer<W> *>(lds);
+    uint32_t *ssrc = reinterpret_cast<uint32_t *>(lds + (size_t)tile * 8 * W);
+    uint32_t *wc = reinterpret_cast<uint32_t *>(lds + (size_t)tile * 8 * W + (src ? (size_t)tile * 4 : 0));
+    uint32_t *tstart = wc + WAVES * S;
+    uint32_t *delta = tstart + S;
+    uint16_t *ssub = reinterpret_cast<uint16_t *>(delta + S);
+    uint8_t *sfl = reinterpret_cast<uint8_t *>(ssub + tile);
     const int lane = __lane_id(), w = threadIdx.x / 64;
     const uint64_t lt = (1ull << lane) - 1ull, me = 1ull << lane;
     const uint64_t zero[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (uint32_t j0 = 0; j0 < t.count; j0 += BLOCK) {
-        const uint32_t idx = j0 + threadIdx.x;
+    const uint32_t chunk = tile / WAVES, c0 = (uint32_t)w * chunk;
+    uint32_t *mywc = wc + (uint32_t)w * S;
+    for (uint32_t i = threadIdx.x; i < WAVES * S; i += BLOCK) wc[i] = 0;
+    __syncthreads();
+    // 1. per-wave counts
+    for (uint32_t r = 0; r < chunk; r += 64) {
+        const uint32_t idx = c0 + r + (uint32_t)lane;
+        const bool valid = idx < t.count;
+        const uint32_t v = valid ? sub[t.kbegin + idx] : 0u;
+        const uint64_t V = __ballot(valid);
+        if (!V) break;
+        uint64_t BP[10];
+#pragma unroll
+        for (int q = 0; q < 10; q++) BP[q] = (uint32_t)q < nbits ? (uint64_t)__ballot(valid && ((v >> q) & 1u)) : 0ull;
+        uint64_t M = 0, none;
+        match_bucket(v, nbits, V, 0ull, BP, zero, M, none);
+        if (valid && (M & (0ull - M)) == me) atomicAdd(&mywc[v], (uint32_t)__popcll(M));
+    }
+    __syncthreads();
+    // 2. tile starts per fine bucket; per-wave bases; global minus local offsets
+    for (uint32_t v = threadIdx.x; v < S; v += BLOCK) {
+        uint32_t tot = 0;
+#pragma unroll
+        for (int q = 0; q < WAVES; q++) tot += wc[q * S + v];
+        tstart[v] = tot;
+    }
+    __syncthreads();
+    block_excl_scan(tstart, S, wsum);
+    for (uint32_t v = threadIdx.x; v < S; v += BLOCK) {
+        uint32_t b = tstart[v];
+#pragma unroll
+        for (int q = 0; q < WAVES; q++) {
+            const uint32_t cq = wc[q * S + v];
+            wc[q * S + v] = b;
+            b += cq;
+        }
+        delta[v] = hscan[t.hbase + (uint64_t)v * t.ts + t.tlocal] - tstart[v];
+    }
+    __syncthreads();
+    // 3. stable positions; stage the slots sorted by fine bucket
+    for (uint32_t r = 0; r < chunk; r += 64) {
+        const uint32_t idx = c0 + r + (uint32_t)lane;
         const bool valid = idx < t.count;
         const uint64_t s = t.kbegin + idx;
         const uint32_t v = valid ? sub[s] : 0u;
         const uint64_t V = __ballot(valid);
+        if (!V) break;
         uint64_t BP[10];
 #pragma unroll
         for (int q = 0; q < 10; q++) BP[q] = (uint32_t)q < nbits ? (uint64_t)__ballot(valid && ((v >> q) & 1u)) : 0ull;
-        uint64_t M = 0;
+        uint64_t M = 0, none;
+        match_bucket(v, nbits, V, 0ull, BP, zero, M, none);
+        uint32_t old = 0;
+        if (valid && (M & (0ull - M)) == me) old = atomicAdd(&mywc[v], (uint32_t)__popcll(M));
+        const int leader = valid ? __builtin_ctzll(M) : lane;
+        const uint32_t base = __shfl(old, leader, 64);
         if (valid) {
-            uint64_t none;
-            match_bucket(v, nbits, V, 0ull, BP, zero, M, none);
-            if ((M & (0ull - M)) == me) wcnt[w * S + v] = (uint32_t)__popcll(M);
+            const uint32_t li = base + (uint32_t)__popcll(M & lt);
+            skm[li] = load_kmer<W>(kin, s, ~0ull);
+            sfl[li] = fin[s];
+            ssub[li] = (uint16_t)v;
+            if (src) ssrc[li] = sin[s];
+            perm[s] = delta[v] + li;
         }
-        __syncthreads();
-        if (valid) {
-            uint32_t pos = run[v] + (uint32_t)__popcll(M & lt);
-            for (int q = 0; q < w; q++) pos += wcnt[q * S + v];
-            const uint32_t ns = hscan[t.hbase + (uint64_t)v * t.ts + t.tlocal] + pos;
-            store_kmer<W>(kout, ns, load_kmer<W>(kin, s, ~0ull));
-            fout[ns] = fin[s];
-            if (sin) sout[ns] = sin[s];
-            perm[s] = ns;
-        }
-        __syncthreads();
-        for (uint32_t b = threadIdx.x; b < S; b += BLOCK) {
-            uint32_t a = 0;
-#pragma unroll
-            for (int q = 0; q < WAVES; q++) {
-                a += wcnt[q * S + b];
-                wcnt[q * S + b] = 0;
-            }
-            run[b] += a;
-        }
-        __syncthreads();
+    }
+    __syncthreads();
+    // 4. write out in sorted order
+    for (uint32_t li = threadIdx.x; li < t.count; li += BLOCK) {
+        const uint32_t ns = delta[ssub[li]] + li;
+        store_kmer<W>(kout, ns, skm[li]);
+        fout[ns] = sfl[li];
+        if (src) sout[ns] = ssrc[li];
     }
 }
 
@@ -616,6 +737,20 @@ struct TEntry {
 constexpr uint32_t CNT_SHARED = 0x80000000u;
 constexpr uint32_t OWN_MASK = 0x3FFFFFFFu;
 constexpr uint64_t NO_LINK = (uint64_t)NONE;
+
+// Runs: maximal chains of edges u -> v with u and v in one LDS group and u
+// not a borrowed duplicate (such edges never leave a fine bucket).  Every
+// live k-mer belongs to one run: rhead = its first k-mer, roff = its
+// distance from it.  At run heads: rlen = k-mers in the run, rnext = first
+// k-mer of the next run on the path (set through the splice for runs that
+// end in a lent k-mer), NONE at a path end.
+struct RunOut {
+    uint32_t *rhead;
+    uint16_t *roff;
+    uint32_t *rlen;
+    uint32_t *rnext;
+    uint8_t *flags;
+};
 
 __device__ __forceinline__ uint32_t fastrange(uint32_t h, uint32_t cap)
 {
@@ -738,7 +873,7 @@ __global__ __launch_bounds__(BLOCK) void k_edge_lookup(const uint64_t *__restric
 __global__ __launch_bounds__(BLOCK) void k_edge_resolve(const ETile *__restrict__ tiles, const uint32_t *__restrict__ boff,
                                                         const uint64_t *__restrict__ toff, const TEntry *table,
                                                         const uint32_t *__restrict__ hit, uint64_t *__restrict__ link,
-                                                        uint32_t *__restrict__ hasp, uint32_t n_tiles)
+                                                        uint32_t *__restrict__ hasp, RunOut ro, uint32_t n_tiles)
 {
     const uint32_t ti = xcd_tile(n_tiles);
     if (ti == NONE) return;
@@ -757,6 +892,15 @@ __global__ __launch_bounds__(BLOCK) void k_edge_resolve(const ETile *__restrict_
             }
         }
         link[s] = l;
+        // an oversized fine bucket: every k-mer is a run of its own
+        const uint8_t f = ro.flags[s];
+        if (!(f & F_BORROW)) {
+            ro.flags[s] = (uint8_t)(f | F_RUNHEAD);
+            ro.rhead[s] = s;
+            ro.roff[s] = 0;
+            ro.rlen[s] = 1;
+            ro.rnext[s] = (uint32_t)l;
+        }
     }
 }
 
@@ -784,11 +928,12 @@ __host__ __device__ constexpr size_t edge_lds_bytes(uint32_t nb)
 }
 
 template <int W>
-__global__ __launch_bounds__(BLOCK) void k_edge_lds(const uint64_t *__restrict__ kmer, const uint8_t *__restrict__ flags,
-                                                    const EGroup *__restrict__ groups, uint32_t nb, uint32_t k,
-                                                    uint64_t pm0, uint64_t pm1, uint64_t *__restrict__ link,
-                                                    uint32_t *__restrict__ hasp)
+__global__ __launch_bounds__(BLOCK) void k_edge_lds(const uint64_t *__restrict__ kmer, const EGroup *__restrict__ groups,
+                                                    uint32_t nb, uint32_t k, uint64_t pm0, uint64_t pm1,
+                                                    uint64_t *__restrict__ link, uint32_t *__restrict__ hasp, RunOut ro)
 {
+    constexpr int EPT = LDS_GROUP / BLOCK;  // slots per thread (a group holds at most LDS_GROUP)
+    const uint8_t *flags = ro.flags;
     extern __shared__ __align__(16) unsigned char lds[];
     Kmer<W> *km = reinterpret_cast<Kmer<W> *>(lds);
     uint32_t *tab = reinterpret_cast<uint32_t *>(lds + (size_t)nb * 8 * W);
@@ -849,16 +994,23 @@ __global__ __launch_bounds__(BLOCK) void k_edge_lds(const uint64_t *__restrict__
         hitp[i] = hp;
     }
     __syncthreads();
-    // keep u -> v when v's prefix is unshared and was hit exactly once
-    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+    // keep u -> v when v's prefix is unshared and was hit exactly once; the
+    // edges out of k-mers that are not borrowed chain the runs
+    uint16_t tgt[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+        const uint32_t i = threadIdx.x + (uint32_t)e * BLOCK;
+        tgt[e] = LDS_NOHIT;
+        if (i >= n) continue;
         const uint16_t hp = hitp[i];
         uint64_t l = NO_LINK;
         if (hp != LDS_NOHIT) {
-            const uint32_t e = tab[hp];
-            if (!(e & LDS_SHARED) && (e >> 24) == 1u) {
-                const uint32_t v = e & 0xFFFFu;
+            const uint32_t en = tab[hp];
+            if (!(en & LDS_SHARED) && (en >> 24) == 1u) {
+                const uint32_t v = en & 0xFFFFu;
                 l = (uint64_t)(gr.begin + v) | ((uint64_t)char_at<W>(km[v], k - 1) << 32);
                 atomicOr(&hb[(off0 + v) >> 5], 1u << ((off0 + v) & 31));
+                if (!(fl[i] & F_BORROW)) tgt[e] = (uint16_t)v;
             }
         }
         link[gr.begin + i] = l;
@@ -868,6 +1020,67 @@ __global__ __launch_bounds__(BLOCK) void k_edge_lds(const uint64_t *__restrict__
     for (uint32_t j = threadIdx.x; j < nw; j += BLOCK) {
         const uint32_t bits = hb[j];
         if (bits) atomicOr(&hasp[(gr.begin >> 5) + j], bits);
+    }
+    // runs: run predecessor of every slot (hitp is free now)
+    uint16_t *predl = hitp;
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) predl[i] = LDS_NOHIT;
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < EPT; e++)
+        if (tgt[e] != LDS_NOHIT) predl[tgt[e]] = (uint16_t)(threadIdx.x + (uint32_t)e * BLOCK);
+    __syncthreads();
+    // pointer jumping toward the run heads: ancestor | distance << 16 (tab is free)
+    uint32_t *pd = tab;
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) pd[i] = predl[i] == LDS_NOHIT ? i : ((uint32_t)predl[i] | (1u << 16));
+    __syncthreads();
+    const uint32_t rounds = n > 1 ? 32u - (uint32_t)__builtin_clz(n - 1) : 0u;
+    for (uint32_t r = 0; r < rounds; r++) {
+        uint32_t nv[EPT];
+#pragma unroll
+        for (int e = 0; e < EPT; e++) {
+            const uint32_t i = threadIdx.x + (uint32_t)e * BLOCK;
+            nv[e] = 0;
+            if (i < n) {
+                const uint32_t x = pd[i], y = pd[x & 0xFFFFu];
+                nv[e] = (y & 0xFFFFu) | (((x >> 16) + (y >> 16)) << 16);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < EPT; e++) {
+            const uint32_t i = threadIdx.x + (uint32_t)e * BLOCK;
+            if (i < n) pd[i] = nv[e];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+        const uint32_t i = threadIdx.x + (uint32_t)e * BLOCK;
+        if (i >= n) continue;
+        const uint32_t g = gr.begin + i;
+        const uint8_t f = (uint8_t)(fl[i] & ~F_RUNHEAD);
+        if (f & F_BORROW) {
+            ro.flags[g] = f;
+            continue;
+        }
+        uint32_t a = pd[i] & 0xFFFFu, d = pd[i] >> 16;
+        // an ancestor with a predecessor: the slot lies on a cycle inside the
+        // group; such k-mers are runs of their own
+        const bool cyc = predl[a] != LDS_NOHIT;
+        if (cyc) {
+            a = i;
+            d = 0;
+        }
+        ro.flags[g] = (uint8_t)(f | (a == i ? F_RUNHEAD : 0));
+        ro.rhead[g] = gr.begin + a;
+        ro.roff[g] = (uint16_t)d;
+        if (cyc) {
+            ro.rlen[g] = 1;
+            ro.rnext[g] = gr.begin + tgt[e];
+        } else if (tgt[e] == LDS_NOHIT) {  // the run's last k-mer
+            ro.rlen[gr.begin + a] = d + 1;
+            ro.rnext[gr.begin + a] = NONE;
+        }
     }
 }
 
@@ -890,11 +1103,15 @@ constexpr int CT_TILE = SCAN_TILE;  // slots per contraction tile
 // the borrowed duplicate's out-edge (and its target's character) moves to
 // the home copy; the target keeps its has-predecessor bit
 __global__ void k_splice(const uint32_t *__restrict__ pair_home, const uint32_t *__restrict__ pair_borrow, uint32_t x,
-                         uint64_t *__restrict__ link)
+                         uint64_t *__restrict__ link, const uint32_t *__restrict__ rhead, uint32_t *__restrict__ rnext)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= x) return;
-    link[pair_home[i]] = link[pair_borrow[i]];
+    const uint32_t h = pair_home[i];
+    const uint64_t l = link[pair_borrow[i]];
+    link[h] = l;
+    // a lent k-mer ends its run: the run continues through the duplicate's edge
+    rnext[rhead[h]] = (uint32_t)l;
 }
 
 // list index for every flagged lane: one atomic per wave
@@ -921,7 +1138,8 @@ __device__ __forceinline__ bool live_head(const uint8_t *__restrict__ flags, con
 // one walk per splitter: segment length, next splitter; per tile the sum
 // of (1 << 32 | length) over heads whose unitig is a single segment (the
 // other heads add their count here and their length in k_chain_tail)
-__global__ __launch_bounds__(BLOCK) void k_seg_count(const uint64_t *__restrict__ link, const uint32_t *__restrict__ hasp,
+__global__ __launch_bounds__(BLOCK) void k_seg_count(const uint32_t *__restrict__ rlen, const uint32_t *__restrict__ rnext,
+                                                     const uint32_t *__restrict__ hasp,
                                                      const uint8_t *__restrict__ flags, uint32_t m, uint32_t salt,
                                                      uint32_t shift, uint32_t *__restrict__ seglen,
                                                      uint32_t *__restrict__ segnext, uint32_t *__restrict__ prevseg,
@@ -941,7 +1159,7 @@ __global__ __launch_bounds__(BLOCK) void k_seg_count(const uint64_t *__restrict_
     for (int j = 0; j < CT_ITEMS; j++) {
         const uint64_t s64 = base + (uint64_t)j * BLOCK + threadIdx.x;
         hd[j] = rd[j] = false;
-        if (s64 < m && !(flags[s64] & F_BORROW)) {
+        if (s64 < m && (flags[s64] & (F_BORROW | F_RUNHEAD)) == F_RUNHEAD) {
             hd[j] = !has_pred(hasp, (uint32_t)s64);
             rd[j] = !hd[j] && sampled((uint32_t)s64, salt, shift);
         }
@@ -968,13 +1186,13 @@ __global__ __launch_bounds__(BLOCK) void k_seg_count(const uint64_t *__restrict_
         const bool head = hd[j], rnd = rd[j];
         if (head || rnd) {
             nsplit++;
-            uint32_t v = s, r = 0;
+            uint32_t v = s, r = 0, steps = 0;
             do {
-                r++;
-                v = (uint32_t)link[v];
-                // in-degree <= 1 bounds every walk by m; the cap only guards
-                // the GPU against a corrupted link array
-                if (r > m) {
+                r += rlen[v];
+                v = rnext[v];
+                // in-degree <= 1 bounds every walk by m runs; the cap only
+                // guards the GPU against corrupted run arrays
+                if (++steps > m) {
                     atomicOr(cnt + C_ERR, 1u);
                     v = NONE;
                     break;
@@ -1059,34 +1277,39 @@ __global__ void k_chain_tail(const uint32_t *__restrict__ list, uint32_t n, cons
 
 // ---- isolated cycles (only when some k-mers are not reached from a head) --
 
-// marks every k-mer of a segment that hangs off a head
-__global__ __launch_bounds__(BLOCK) void k_cover(const uint64_t *__restrict__ link, const uint32_t *__restrict__ hasp,
+// marks every run of a segment that hangs off a head
+__global__ __launch_bounds__(BLOCK) void k_cover(const uint32_t *__restrict__ rnext, const uint32_t *__restrict__ hasp,
                                                  uint8_t *__restrict__ flags, const uint64_t *__restrict__ chain,
-                                                 const uint32_t *__restrict__ seglen, uint32_t m, uint32_t salt,
-                                                 uint32_t shift)
+                                                 uint32_t m, uint32_t salt, uint32_t shift)
 {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= m || (flags[s] & F_BORROW)) return;
+    if (s >= m || (flags[s] & (F_BORROW | F_RUNHEAD)) != F_RUNHEAD) return;
     if (has_pred(hasp, s)) {
         if (!sampled(s, salt, shift)) return;
         if (has_pred(hasp, (uint32_t)chain[s])) return;
     }
-    uint32_t v = s;
-    for (uint32_t i = 0, n = seglen[s]; i < n && v != NONE; i++) {
+    uint32_t v = s, steps = 0;
+    do {
         flags[v] |= F_VISIT;
-        v = (uint32_t)link[v];
-    }
+        v = rnext[v];
+    } while (v != NONE && !sampled(v, salt, shift) && ++steps <= m);
 }
 
-__global__ void k_cycle_init(const uint64_t *__restrict__ link, uint8_t *__restrict__ flags, uint32_t m,
+// k-mers whose run no head reaches lie on cycles; they become runs of their own
+__global__ void k_cycle_init(const uint64_t *__restrict__ link, uint8_t *__restrict__ flags, uint32_t m, RunOut ro,
                              uint64_t *__restrict__ lab, uint32_t *__restrict__ list, uint32_t *__restrict__ n_list)
 {
     const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool cyc = v < m && !(flags[v] & (F_BORROW | F_VISIT));
+    const bool cyc = v < m && !(flags[v] & F_BORROW) && !(flags[ro.rhead[v]] & F_VISIT);
     const uint32_t li = wave_append(cyc, n_list);
     if (!cyc) return;
-    flags[v] |= F_CYCLE;
-    lab[v] = (uint64_t)v | ((uint64_t)(uint32_t)link[v] << 32);
+    const uint64_t l = link[v];
+    flags[v] |= F_CYCLE | F_RUNHEAD;
+    ro.rhead[v] = v;
+    ro.roff[v] = 0;
+    ro.rlen[v] = 1;
+    ro.rnext[v] = (uint32_t)l;
+    lab[v] = (uint64_t)v | ((uint64_t)(uint32_t)l << 32);
     list[li] = v;
 }
 
@@ -1120,7 +1343,8 @@ __global__ void k_cycle_pred(const uint32_t *__restrict__ list, uint32_t n, cons
 // each cycle loses the edge into its smallest k-mer
 __global__ void k_cycle_cut(const uint32_t *__restrict__ list, uint32_t n, const uint64_t *__restrict__ lab,
                             const uint32_t *__restrict__ pred, uint64_t *__restrict__ link,
-                            uint32_t *__restrict__ hasp, uint8_t *__restrict__ flags, uint32_t *__restrict__ n_cut)
+                            uint32_t *__restrict__ rnext, uint32_t *__restrict__ hasp, uint8_t *__restrict__ flags,
+                            uint32_t *__restrict__ n_cut)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -1128,6 +1352,7 @@ __global__ void k_cycle_cut(const uint32_t *__restrict__ list, uint32_t n, const
     flags[v] &= (uint8_t)~F_CYCLE;
     if ((uint32_t)lab[v] == v) {
         link[pred[v]] = NO_LINK;
+        rnext[pred[v]] = NONE;
         atomicAnd(&hasp[v >> 5], ~(1u << (v & 31)));
         atomicAdd(n_cut, 1u);
     }
@@ -1147,26 +1372,28 @@ __global__ void k_clear_flag(uint8_t *__restrict__ flags, uint32_t m, uint8_t bi
 // later k-mer, '\n'.
 
 // per tile: exclusive prefix of (1 << 32 | length) over heads -> text
-// offsets; every head spells its own segment
+// offsets; every head writes its first k-mer and gives each run of its
+// segment the text index of the run's first k-mer's last character
 template <int W>
-__global__ __launch_bounds__(BLOCK) void k_head_emit(const uint64_t *__restrict__ kmer, const uint64_t *__restrict__ link,
+__global__ __launch_bounds__(BLOCK) void k_head_emit(const uint64_t *__restrict__ kmer, const uint32_t *__restrict__ rlen,
+                                                     const uint32_t *__restrict__ rnext,
                                                      const uint32_t *__restrict__ hasp, const uint8_t *__restrict__ flags,
-                                                     const uint32_t *__restrict__ seglen,
                                                      const uint32_t *__restrict__ segnext,
                                                      const uint32_t *__restrict__ ulen,
                                                      const uint64_t *__restrict__ tile_off, uint32_t m, uint32_t k,
-                                                     uint64_t *__restrict__ htoff, char *__restrict__ text,
+                                                     uint32_t salt, uint32_t shift, uint64_t *__restrict__ htoff,
+                                                     uint64_t *__restrict__ base, char *__restrict__ text,
                                                      uint32_t n_tiles)
 {
     __shared__ uint64_t tile[CT_TILE];
     __shared__ uint64_t wsum[WAVES];
     const uint32_t ti = xcd_tile(n_tiles);
     if (ti == NONE) return;
-    const uint64_t base = (uint64_t)ti * CT_TILE;
+    const uint64_t t0 = (uint64_t)ti * CT_TILE;
     bool hd[CT_ITEMS];
 #pragma unroll
     for (int j = 0; j < CT_ITEMS; j++) {
-        const uint64_t s = base + (uint64_t)j * BLOCK + threadIdx.x;
+        const uint64_t s = t0 + (uint64_t)j * BLOCK + threadIdx.x;
         hd[j] = s < m && live_head(flags, hasp, (uint32_t)s);
         tile[j * BLOCK + threadIdx.x] = hd[j] ? ((1ull << 32) | ulen[s]) : 0ull;
     }
@@ -1191,46 +1418,65 @@ __global__ __launch_bounds__(BLOCK) void k_head_emit(const uint64_t *__restrict_
 #pragma unroll
     for (int j = 0; j < CT_ITEMS; j++) {
         if (!hd[j]) continue;
-        const uint32_t s = (uint32_t)(base + (uint64_t)j * BLOCK + threadIdx.x);
+        const uint32_t s = (uint32_t)(t0 + (uint64_t)j * BLOCK + threadIdx.x);
         const uint64_t pre = tile[j * BLOCK + threadIdx.x];
         const uint64_t off = (pre & 0xFFFFFFFFull) + (pre >> 32) * k;
-        const uint32_t len = seglen[s], nxt = segnext[s];
-        if (nxt != NONE) htoff[s] = off;
+        if (segnext[s] != NONE) htoff[s] = off;
         char *o = text + off;
         const Kmer<W> x0 = load_kmer<W>(kmer, s, ~0ull);
         for (uint32_t i = 0; i < k; i++) o[i] = base_char(char_at<W>(x0, i));
-        uint32_t v = s;
-        for (uint32_t i = 1; i < len; i++) {
-            const uint64_t l = link[v];
-            o[k - 1 + i] = base_char((uint32_t)(l >> 32) & 3u);
-            v = (uint32_t)l;
-        }
-        if (nxt == NONE) o[k - 1 + len] = '\n';
+        uint64_t p = off + k - 1;
+        uint32_t v = s, steps = 0;
+        do {
+            base[v] = p;
+            p += rlen[v];
+            v = rnext[v];
+        } while (v != NONE && !sampled(v, salt, shift) && ++steps < m);
+        if (v == NONE) text[p] = '\n';
     }
 }
 
-// segments of sampled splitters: positions from the ranked chain
-template <int W>
-__global__ void k_rand_emit(const uint64_t *__restrict__ kmer, const uint64_t *__restrict__ link,
+// segments of sampled splitters: text indices from the ranked chain
+__global__ void k_rand_emit(const uint32_t *__restrict__ rlen, const uint32_t *__restrict__ rnext,
                             const uint32_t *__restrict__ list, uint32_t n, const uint64_t *__restrict__ chain,
-                            const uint32_t *__restrict__ seglen, const uint32_t *__restrict__ segnext,
-                            const uint64_t *__restrict__ htoff, uint32_t k, char *__restrict__ text)
+                            const uint64_t *__restrict__ htoff, uint32_t k, uint32_t salt, uint32_t shift,
+                            uint32_t n_slots, uint64_t *__restrict__ base, char *__restrict__ text)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t r = list[i];
     const uint64_t c = chain[r];
     const uint32_t a = (uint32_t)c, d = (uint32_t)(c >> 32);
-    char *o = text + htoff[a] + (k - 1) + d;
-    const uint32_t len = seglen[r];
-    o[0] = base_char(char_at<W>(load_kmer<W>(kmer, r, ~0ull), k - 1));
-    uint32_t v = r;
-    for (uint32_t p = 1; p < len; p++) {
-        const uint64_t l = link[v];
-        o[p] = base_char((uint32_t)(l >> 32) & 3u);
-        v = (uint32_t)l;
+    uint64_t p = htoff[a] + (k - 1) + d;
+    uint32_t v = r, steps = 0;
+    do {
+        base[v] = p;
+        p += rlen[v];
+        v = rnext[v];
+    } while (v != NONE && !sampled(v, salt, shift) && ++steps < n_slots);
+    if (v == NONE) text[p] = '\n';
+}
+
+// every live k-mer writes its last character at its run's index plus its
+// distance from the run's first k-mer
+template <int W>
+__global__ __launch_bounds__(BLOCK) void k_slot_emit(const uint64_t *__restrict__ kmer, const uint8_t *__restrict__ flags,
+                                                     const uint32_t *__restrict__ rhead,
+                                                     const uint16_t *__restrict__ roff,
+                                                     const uint64_t *__restrict__ base, uint32_t m, uint32_t k,
+                                                     uint64_t text_bytes, char *__restrict__ text,
+                                                     uint32_t *__restrict__ err)
+{
+    const uint32_t wl = (k - 1) / 32;
+    for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < m; s += (uint64_t)gridDim.x * blockDim.x) {
+        if (flags[s] & F_BORROW) continue;
+        const uint64_t word = kmer[s * W + wl];
+        const uint64_t p = base[rhead[s]] + roff[s];
+        if (p < text_bytes)
+            text[p] = base_char((uint32_t)(word >> (2 * ((k - 1) & 31))) & 3u);
+        else
+            atomicOr(err, 1u);
     }
-    if (segnext[r] == NONE) o[len] = '\n';
 }
 
 // text offset of each set's first unitig: the prefix at the set's first slot
@@ -1403,6 +1649,8 @@ struct uc_ctx {
     std::vector<uint8_t> bucket_done;
     std::vector<size_t> grp_first, fb_first;  // per coarse bucket: first LDS group / oversized fine bucket
     uint32_t lds_nb = 64;                     // LDS capacity of the edge launch
+    uint32_t lds_limit = 65536;               // dynamic LDS a workgroup may use on this device
+    uint32_t tile1 = 0, tile2 = 0;            // k-mers per bucketing tile, slots per fine-sort tile
     uint64_t n_fallback = 0, n_groups = 0;
     uint64_t table_slots = 0, n_done = 0;
     uint64_t n_unitigs = 0, n_cycles = 0, text_bytes = 0, n_splitters = 0;
@@ -1415,7 +1663,7 @@ struct uc_ctx {
     DBuf otable, in, stage, bkt, tiles, hist, xhist, kmer2, flags2, src2, sub, perm, tiles2, hist2, cbhb, cbts,
         cbfirst, fboff, egroups, fbbase, scan_tmp, kmer, flags, src, pair_h, pair_b, boff, sethb, setts, etiles,
         toff, tcap, table, hit, link, hasp, pred, seglen, segnext, prevseg, chainA, chainB, cycB, list, clist, ulen, tsum,
-        htoff, setoff, text, small, txt, txtcnt;
+        htoff, setoff, text, small, txt, txtcnt, rhead, roff, rlen, rnext, base;
     // pinned host chunks of the file reader and writer
     char *pin[2] = {nullptr, nullptr};
     hipEvent_t pin_ev[2] = {nullptr, nullptr};
@@ -1446,7 +1694,8 @@ static std::vector<DBuf *> all_bufs(uc_ctx *c)
             &c->setts,  &c->etiles,  &c->toff,   &c->tcap,    &c->table,   &c->hit,    &c->link,
             &c->hasp,   &c->pred,    &c->seglen, &c->segnext, &c->prevseg, &c->chainA, &c->chainB,
             &c->cycB,   &c->list,    &c->clist,  &c->ulen,    &c->tsum,    &c->htoff,  &c->setoff,
-            &c->text,   &c->small,   &c->txt,    &c->txtcnt};
+            &c->text,   &c->small,   &c->txt,    &c->txtcnt,  &c->rhead,   &c->roff,   &c->rlen,
+            &c->rnext,  &c->base};
 }
 
 static void free_pinned(uc_ctx *c)
@@ -1512,6 +1761,12 @@ template <typename T>
 T *P(DBuf &b)
 {
     return reinterpret_cast<T *>(b.p);
+}
+
+static RunOut run_out(uc_ctx *c)
+{
+    return RunOut{P<uint32_t>(c->rhead), P<uint16_t>(c->roff), P<uint32_t>(c->rlen), P<uint32_t>(c->rnext),
+                  P<uint8_t>(c->flags)};
 }
 
 // Times one kernel launch with a pair of events on the context's stream
@@ -1586,6 +1841,17 @@ int fill(uc_ctx *c, T *p, uint64_t n, T v)
 {
     if (n == 0) return UC_OK;
     KLAUNCH(c, "k_fill", k_fill<T>, (std::min<uint32_t>(grid_for(n), 65536), BLOCK, 0), p, n, v);
+    return UC_OK;
+}
+
+// dynamic LDS above the default 64 KiB must be asked for per kernel
+template <typename K>
+int allow_lds(uc_ctx *c, K kernel, size_t bytes)
+{
+    if (bytes <= 65536) return UC_OK;
+    if (bytes > c->lds_limit) return fail(c, UC_E_LIMIT, "kernel needs more LDS than the device has");
+    HIPCHK(c, hipFuncSetAttribute(reinterpret_cast<const void *>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)bytes));
     return UC_OK;
 }
 
@@ -1947,6 +2213,12 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
         c->set_sub[s] = sub;
         any_fine |= sub > 1;
     }
+    const bool keep_src = (c->opts.flags & UC_FLAG_KEEP_SRC) != 0;
+    c->tile2 = 0;
+    // k-mers per bucketing tile: as many as the staging LDS allows
+    uint32_t tile1 = W_ == 1 ? 2048 : 1024;
+    while (tile1 > 256 && lmin_lds_bytes<W>(tile1, B, keep_src) > c->lds_limit) tile1 >>= 1;
+    c->tile1 = tile1;
     // tiles: every set gets at least one (possibly empty) tile so each bucket
     // row has a start in the histogram
     std::vector<BTile> tiles;
@@ -1955,14 +2227,14 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     uint64_t hb = 0;
     for (uint32_t s = 0; s < S; s++) {
         const uint64_t ns = c->set_off[s + 1] - c->set_off[s];
-        const uint32_t ts = (uint32_t)std::max<uint64_t>(1, (ns + BKT_TILE - 1) / BKT_TILE);
+        const uint32_t ts = (uint32_t)std::max<uint64_t>(1, (ns + tile1 - 1) / tile1);
         set_hbase[s] = hb;
         set_ts[s] = ts;
         for (uint32_t t = 0; t < ts; t++) {
             BTile bt;
-            bt.kbegin = c->set_off[s] + (uint64_t)t * BKT_TILE;
+            bt.kbegin = c->set_off[s] + (uint64_t)t * tile1;
             bt.hbase = hb;
-            bt.count = (uint32_t)std::min<uint64_t>(BKT_TILE, ns - std::min<uint64_t>(ns, (uint64_t)t * BKT_TILE));
+            bt.count = (uint32_t)std::min<uint64_t>(tile1, ns - std::min<uint64_t>(ns, (uint64_t)t * tile1));
             bt.set = s;
             bt.ts = ts;
             bt.tlocal = t;
@@ -2009,7 +2281,6 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     c->x = tot[1];
     if (c->m >= 0xFFFFFFFFull) return fail(c, UC_E_LIMIT, "more than 2^32-2 placements");
     const uint64_t G = (uint64_t)S * B;
-    const bool keep_src = (c->opts.flags & UC_FLAG_KEEP_SRC) != 0;
     DBuf &k1 = any_fine ? c->kmer2 : c->kmer, &f1 = any_fine ? c->flags2 : c->flags, &s1 = any_fine ? c->src2 : c->src;
     CHK(ensure(c, k1, std::max<uint64_t>(c->m, 1) * W * 8));
     CHK(ensure(c, f1, c->m + 64));
@@ -2019,11 +2290,12 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     CHK(ensure(c, c->pair_b, c->x * 4));
     CHK(ensure(c, c->boff, (G + 1) * 4));
     const uint32_t nbits = (uint32_t)__builtin_ctz(B);
-    const size_t lds = (size_t)(B + 1) * (WAVES + 1) * 4;
+    const size_t lds = lmin_lds_bytes<W>(tile1, B, keep_src);
+    CHK(allow_lds(c, k_lmin_scatter<W>, lds));
     KLAUNCH(c, "k_lmin_scatter", k_lmin_scatter<W>, ((uint32_t)T, BLOCK, lds), kin, P<BTile>(c->tiles), B, nbits,
             c->top_mask, P<uint2>(c->bkt), P<uint32_t>(c->hist), P<uint32_t>(c->xhist), P<uint64_t>(k1),
             P<uint8_t>(f1), keep_src ? P<uint32_t>(s1) : nullptr, P<uint32_t>(c->pair_h), P<uint32_t>(c->pair_b),
-            P<uint16_t>(c->sub));
+            P<uint16_t>(c->sub), tile1);
     KLAUNCH(c, "k_gather_boff", k_gather_boff, (grid_for(G + 1), BLOCK, 0), P<uint32_t>(c->hist),
             P<uint64_t>(c->sethb), P<uint32_t>(c->setts), S, B, (uint32_t)c->m, P<uint32_t>(c->boff));
     c->boff_h.resize(G + 1);
@@ -2038,6 +2310,12 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     if (!any_fine) {
         c->fboff_h = c->boff_h;
     } else {
+        uint32_t smax = 1;
+        for (uint32_t sb : c->set_sub) smax = std::max(smax, sb);
+        // slots per fine-sort tile: as many as the staging LDS allows
+        uint32_t tile2 = W_ == 1 ? 8192 : 4096;
+        while (tile2 > 512 && sub_lds_bytes<W>(tile2, smax, keep_src) > c->lds_limit) tile2 >>= 1;
+        c->tile2 = tile2;
         std::vector<BTile> t2;
         std::vector<uint64_t> cb_hbase(G);
         std::vector<uint32_t> cb_ts(G);
@@ -2045,14 +2323,14 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
         for (uint64_t g = 0; g < G; g++) {
             const uint32_t Sg = c->set_sub[g / B];
             const uint64_t b0 = c->boff_h[g], cnt = c->boff_h[g + 1] - b0;
-            const uint32_t ts = (uint32_t)std::max<uint64_t>(1, (cnt + SUB_TILE - 1) / SUB_TILE);
+            const uint32_t ts = (uint32_t)std::max<uint64_t>(1, (cnt + tile2 - 1) / tile2);
             cb_hbase[g] = h2;
             cb_ts[g] = ts;
             for (uint32_t t = 0; t < ts; t++) {
                 BTile bt;
-                bt.kbegin = b0 + (uint64_t)t * SUB_TILE;
+                bt.kbegin = b0 + (uint64_t)t * tile2;
                 bt.hbase = h2;
-                bt.count = (uint32_t)std::min<uint64_t>(SUB_TILE, cnt - std::min<uint64_t>(cnt, (uint64_t)t * SUB_TILE));
+                bt.count = (uint32_t)std::min<uint64_t>(tile2, cnt - std::min<uint64_t>(cnt, (uint64_t)t * tile2));
                 bt.set = (uint32_t)g;
                 bt.ts = ts;
                 bt.tlocal = t;
@@ -2072,8 +2350,6 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
         CHK(h2d(c, c->cbhb.p, cb_hbase.data(), G * 8));
         CHK(h2d(c, c->cbts.p, cb_ts.data(), G * 4));
         CHK(h2d(c, c->cbfirst.p, c->cb_first.data(), (G + 1) * 4));
-        uint32_t smax = 1;
-        for (uint32_t sb : c->set_sub) smax = std::max(smax, sb);
         KLAUNCH(c, "k_sub_count", k_sub_count, ((uint32_t)T2, BLOCK, smax * 4), P<uint16_t>(c->sub),
                 P<BTile>(c->tiles2), P<uint32_t>(c->hist2));
         CHK(scan_excl<uint32_t>(c, P<uint32_t>(c->hist2), P<uint32_t>(c->hist2), h2, P<uint32_t>(c->hist2) + h2,
@@ -2082,10 +2358,12 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
         CHK(ensure(c, c->flags, c->m + 64));
         if (keep_src) CHK(ensure(c, c->src, c->m * 4));
         CHK(ensure(c, c->perm, c->m * 4));
-        KLAUNCH(c, "k_sub_scatter", k_sub_scatter<W>, ((uint32_t)T2, BLOCK, (size_t)smax * (WAVES + 1) * 4),
-                P<BTile>(c->tiles2), P<uint16_t>(c->sub), P<uint32_t>(c->hist2), P<uint64_t>(c->kmer2),
-                P<uint8_t>(c->flags2), keep_src ? P<uint32_t>(c->src2) : nullptr, P<uint64_t>(c->kmer),
-                P<uint8_t>(c->flags), keep_src ? P<uint32_t>(c->src) : nullptr, P<uint32_t>(c->perm));
+        const size_t lds2 = sub_lds_bytes<W>(tile2, smax, keep_src);
+        CHK(allow_lds(c, k_sub_scatter<W>, lds2));
+        KLAUNCH(c, "k_sub_scatter", k_sub_scatter<W>, ((uint32_t)T2, BLOCK, lds2), P<BTile>(c->tiles2),
+                P<uint16_t>(c->sub), P<uint32_t>(c->hist2), P<uint64_t>(c->kmer2), P<uint8_t>(c->flags2),
+                keep_src ? P<uint32_t>(c->src2) : nullptr, P<uint64_t>(c->kmer), P<uint8_t>(c->flags),
+                keep_src ? P<uint32_t>(c->src) : nullptr, P<uint32_t>(c->perm), tile2);
         if (c->x)
             KLAUNCH(c, "k_pair_remap", k_pair_remap, (grid_for(c->x), BLOCK, 0), P<uint32_t>(c->pair_h),
                     P<uint32_t>(c->pair_b), (uint32_t)c->x, P<uint32_t>(c->perm));
@@ -2138,7 +2416,7 @@ int launch_global_edges(uc_ctx *c, size_t first, size_t count)
             P<TEntry>(c->table), c->pm0, c->pm1, P<uint32_t>(c->hit), tile_arg(c, count));
     KLAUNCH(c, "k_edge_resolve", k_edge_resolve, (xcd_grid(count), BLOCK, 0), tiles, P<uint32_t>(c->fbbase),
             P<uint64_t>(c->toff), P<TEntry>(c->table), P<uint32_t>(c->hit), P<uint64_t>(c->link),
-            P<uint32_t>(c->hasp), tile_arg(c, count));
+            P<uint32_t>(c->hasp), run_out(c), tile_arg(c, count));
     return UC_OK;
 }
 
@@ -2147,8 +2425,8 @@ int launch_lds_edges(uc_ctx *c, size_t first, size_t count)
 {
     if (count == 0) return UC_OK;
     KLAUNCH(c, "k_edge_lds", k_edge_lds<W>, ((uint32_t)count, BLOCK, edge_lds_bytes<W>(c->lds_nb)),
-            P<uint64_t>(c->kmer), P<uint8_t>(c->flags), P<EGroup>(c->egroups) + first, c->lds_nb, c->k, c->pm0,
-            c->pm1, P<uint64_t>(c->link), P<uint32_t>(c->hasp));
+            P<uint64_t>(c->kmer), P<EGroup>(c->egroups) + first, c->lds_nb, c->k, c->pm0, c->pm1,
+            P<uint64_t>(c->link), P<uint32_t>(c->hasp), run_out(c));
     return UC_OK;
 }
 
@@ -2245,6 +2523,10 @@ int prepare_edges(uc_ctx *c)
     CHK(ensure(c, c->hit, c->m * 4));
     CHK(ensure(c, c->link, c->m * 8));
     CHK(ensure(c, c->hasp, (c->m + 31) / 32 * 4 + 8));
+    CHK(ensure(c, c->rhead, c->m * 4));
+    CHK(ensure(c, c->roff, c->m * 2 + 8));
+    CHK(ensure(c, c->rlen, c->m * 4));
+    CHK(ensure(c, c->rnext, c->m * 4));
     // has-predecessor bits: every bucket sets only its own slots' bits
     CHK(fill<uint32_t>(c, P<uint32_t>(c->hasp), (c->m + 31) / 32, 0u));
     c->n_fallback = NF;
@@ -2324,7 +2606,8 @@ int contract_pass(uc_ctx *c, uint64_t *live, uint32_t *nonconv)
     const uint64_t T = (c->m + CT_TILE - 1) / CT_TILE;
     uint32_t *cnt = P<uint32_t>(c->small);
     CHK(fill<uint32_t>(c, cnt, 8, 0u));
-    KLAUNCH(c, "k_seg_count", k_seg_count, (xcd_grid(T), BLOCK, 0), P<uint64_t>(c->link), P<uint32_t>(c->hasp),
+    KLAUNCH(c, "k_seg_count", k_seg_count, (xcd_grid(T), BLOCK, 0), P<uint32_t>(c->rlen), P<uint32_t>(c->rnext),
+            P<uint32_t>(c->hasp),
             P<uint8_t>(c->flags), m, c->salt, c->shift, P<uint32_t>(c->seglen), P<uint32_t>(c->segnext),
             P<uint32_t>(c->prevseg), P<uint32_t>(c->ulen), P<uint32_t>(c->list), cnt, P<uint64_t>(c->tsum),
             tile_arg(c, T));
@@ -2378,13 +2661,13 @@ int cut_cycles(uc_ctx *c, uint64_t *n_cut)
 {
     const uint32_t m = (uint32_t)c->m;
     uint32_t *cnt = P<uint32_t>(c->small);
-    KLAUNCH(c, "k_cover", k_cover, (grid_for(m), BLOCK, 0), P<uint64_t>(c->link), P<uint32_t>(c->hasp),
-            P<uint8_t>(c->flags), c->chain, P<uint32_t>(c->seglen), m, c->salt, c->shift);
+    KLAUNCH(c, "k_cover", k_cover, (grid_for(m), BLOCK, 0), P<uint32_t>(c->rnext), P<uint32_t>(c->hasp),
+            P<uint8_t>(c->flags), c->chain, m, c->salt, c->shift);
     CHK(fill<uint32_t>(c, cnt + C_CYC, 2, 0u));
     uint64_t *la = P<uint64_t>(c->chainA) == c->chain ? P<uint64_t>(c->chainB) : P<uint64_t>(c->chainA);
     uint64_t *lb = P<uint64_t>(c->cycB);
     KLAUNCH(c, "k_cycle_init", k_cycle_init, (grid_for(m), BLOCK, 0), P<uint64_t>(c->link), P<uint8_t>(c->flags), m,
-            la, P<uint32_t>(c->clist), cnt + C_CYC);
+            run_out(c), la, P<uint32_t>(c->clist), cnt + C_CYC);
     uint32_t nl;
     CHK(d2h(c, &nl, cnt + C_CYC, 4));
     uint32_t rounds = 1;
@@ -2398,7 +2681,8 @@ int cut_cycles(uc_ctx *c, uint64_t *n_cut)
         KLAUNCH(c, "k_cycle_pred", k_cycle_pred, (grid_for(nl), BLOCK, 0), P<uint32_t>(c->clist), nl,
                 P<uint64_t>(c->link), P<uint32_t>(c->pred));
         KLAUNCH(c, "k_cycle_cut", k_cycle_cut, (grid_for(nl), BLOCK, 0), P<uint32_t>(c->clist), nl, la,
-                P<uint32_t>(c->pred), P<uint64_t>(c->link), P<uint32_t>(c->hasp), P<uint8_t>(c->flags), cnt + C_CUT);
+                P<uint32_t>(c->pred), P<uint64_t>(c->link), P<uint32_t>(c->rnext), P<uint32_t>(c->hasp),
+                P<uint8_t>(c->flags), cnt + C_CUT);
     }
     KLAUNCH(c, "k_clear_flag", k_clear_flag, (std::min<uint32_t>(grid_for(m), 65536), BLOCK, 0), P<uint8_t>(c->flags),
             m, F_VISIT);
@@ -2431,7 +2715,7 @@ int do_coarsen(uc_ctx *c)
     c->shift = c->opts.sample_log2 ? c->opts.sample_log2 : 5;
     if (c->x) {
         KLAUNCH(c, "k_splice", k_splice, (grid_for(c->x), BLOCK, 0), P<uint32_t>(c->pair_h), P<uint32_t>(c->pair_b),
-                (uint32_t)c->x, P<uint64_t>(c->link));
+                (uint32_t)c->x, P<uint64_t>(c->link), P<uint32_t>(c->rhead), P<uint32_t>(c->rnext));
     }
     uint64_t live = 0;
     uint32_t nonconv = 0;
@@ -2460,16 +2744,24 @@ int run_emit(uc_ctx *c)
     const uint64_t T = (c->m + CT_TILE - 1) / CT_TILE;
     CHK(ensure(c, c->text, c->text_bytes + 1));
     CHK(ensure(c, c->htoff, c->m * 8));
+    CHK(ensure(c, c->base, c->m * 8));
     if (m) {
-        KLAUNCH(c, "k_head_emit", k_head_emit<W>, (xcd_grid(T), BLOCK, 0), P<uint64_t>(c->kmer), P<uint64_t>(c->link),
-                P<uint32_t>(c->hasp), P<uint8_t>(c->flags), P<uint32_t>(c->seglen), P<uint32_t>(c->segnext),
-                P<uint32_t>(c->ulen), P<uint64_t>(c->tsum), m, c->k, P<uint64_t>(c->htoff), P<char>(c->text),
-                tile_arg(c, T));
+        KLAUNCH(c, "k_head_emit", k_head_emit<W>, (xcd_grid(T), BLOCK, 0), P<uint64_t>(c->kmer), P<uint32_t>(c->rlen),
+                P<uint32_t>(c->rnext), P<uint32_t>(c->hasp), P<uint8_t>(c->flags), P<uint32_t>(c->segnext),
+                P<uint32_t>(c->ulen), P<uint64_t>(c->tsum), m, c->k, c->salt, c->shift, P<uint64_t>(c->htoff),
+                P<uint64_t>(c->base), P<char>(c->text), tile_arg(c, T));
     }
     if (c->n_list) {
-        KLAUNCH(c, "k_rand_emit", k_rand_emit<W>, (grid_for(c->n_list), BLOCK, 0), P<uint64_t>(c->kmer),
-                P<uint64_t>(c->link), P<uint32_t>(c->list), c->n_list, c->chain, P<uint32_t>(c->seglen),
-                P<uint32_t>(c->segnext), P<uint64_t>(c->htoff), c->k, P<char>(c->text));
+        KLAUNCH(c, "k_rand_emit", k_rand_emit, (grid_for(c->n_list), BLOCK, 0), P<uint32_t>(c->rlen),
+                P<uint32_t>(c->rnext), P<uint32_t>(c->list), c->n_list, c->chain, P<uint64_t>(c->htoff), c->k,
+                c->salt, c->shift, m, P<uint64_t>(c->base), P<char>(c->text));
+    }
+    uint32_t *cnt = P<uint32_t>(c->small);
+    if (m) {
+        CHK(fill<uint32_t>(c, cnt + C_ERR, 1, 0u));
+        KLAUNCH(c, "k_slot_emit", k_slot_emit<W>, (std::min<uint32_t>(grid_for(m), 65536), BLOCK, 0),
+                P<uint64_t>(c->kmer), P<uint8_t>(c->flags), P<uint32_t>(c->rhead), P<uint16_t>(c->roff),
+                P<uint64_t>(c->base), m, c->k, c->text_bytes, P<char>(c->text), cnt + C_ERR);
     }
     // per-set text ranges: a set's unitigs start at its first bucket row
     const uint64_t total = (c->n_unitigs << 32) | (c->text_bytes - c->n_unitigs * c->k);
@@ -2479,6 +2771,11 @@ int run_emit(uc_ctx *c)
             P<uint64_t>(c->setoff));
     std::vector<uint64_t> so(c->n_sets + 1);
     CHK(d2h(c, so.data(), c->setoff.p, (c->n_sets + 1) * 8));
+    if (m) {
+        uint32_t bad = 0;
+        CHK(d2h(c, &bad, cnt + C_ERR, 4));
+        if (bad) return fail(c, UC_E_STATE, "emission: a k-mer's text index fell outside the text");
+    }
     c->set_text.resize(c->n_sets + 1);
     for (uint32_t s = 0; s <= c->n_sets; s++) c->set_text[s] = (so[s] & 0xFFFFFFFFull) + (so[s] >> 32) * c->k;
     c->emitted = true;
@@ -2582,6 +2879,9 @@ int uc_ctx_create(const uc_opts *opts, uc_ctx **out)
         delete c;
         return UC_E_HIP;
     }
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, o.device) == hipSuccess && lds > 0)
+        c->lds_limit = (uint32_t)lds;
     *out = c;
     return UC_OK;
 }
@@ -2693,6 +2993,9 @@ int uc_bucket_info_get(uc_ctx *c, uc_bucket_info *info)
     info->n_sets = c->n_sets;
     info->largest_bucket = c->largest;
     info->n_fine_buckets = c->fboff_h.empty() ? 0 : c->fboff_h.size() - 1;
+    info->bucket_tile = c->tile1;
+    info->fine_tile = c->tile2;
+    info->lds_limit = c->lds_limit;
     info->largest_fine_bucket = 0;
     for (size_t f = 0; f + 1 < c->fboff_h.size(); f++)
         info->largest_fine_bucket = std::max<uint64_t>(info->largest_fine_bucket, c->fboff_h[f + 1] - c->fboff_h[f]);
@@ -2870,10 +3173,11 @@ uint64_t uc_job_bytes_estimate(uint64_t n, uint32_t k)
     // (the most unitigs a job can have); per slot (at most 1.25 n slots in
     // the minimizer schemes used, crossing k-mers ~9% at k = 31): k-mers and
     // flags twice, fine index, permutation, link, hit, bitmap, four segment
-    // arrays, two chain buffers, list, head offsets
+    // arrays, two chain buffers, list, head offsets, run head, offset,
+    // length, next run, text index
     const uint64_t W = k <= 32 ? 1 : 2;
     const uint64_t per_kmer = 8 * W + 8 + (k + 1);
-    const uint64_t per_slot = 16 * W + 2 + 2 + 4 + 8 + 4 + 1 + 16 + 16 + 4 + 8;
+    const uint64_t per_slot = 16 * W + 2 + 2 + 4 + 8 + 4 + 1 + 16 + 16 + 4 + 8 + 4 + 2 + 4 + 4 + 8;
     return n * per_kmer + (n + n / 4) * per_slot + (64ull << 20);
 }
 

@@ -27,15 +27,13 @@ constexpr uint8_t F_LENT = 1;    // home copy of a crossing k-mer: its out-edge 
 constexpr uint8_t F_BORROW = 2;  // duplicate in the suffix bucket: carries the out-edge, dies at the splice
 constexpr uint8_t F_CYCLE = 4;   // on an isolated cycle (set during contraction)
 constexpr uint8_t F_VISIT = 8;   // reached from a path head (cycle detection only)
+constexpr uint8_t F_RUNHEAD = 16;  // first k-mer of a run (a chain of edges inside one LDS group)
 
 constexpr int BLOCK = 256;
 constexpr int WAVES = BLOCK / 64;
-constexpr int BKT_IPT = 4;                    // k-mers per thread in the bucketing kernels
-constexpr int BKT_TILE = BLOCK * BKT_IPT;     // k-mers per bucketing tile
 constexpr int EDGE_TILE = 1024;               // slots per edge-finding tile
 constexpr uint32_t MAX_BUCKETS = 1024;        // coarse buckets per set
 constexpr uint32_t MAX_SUB = 1024;            // fine buckets per coarse bucket
-constexpr int SUB_TILE = 8192;                // slots per fine-sort tile
 constexpr uint32_t LDS_GROUP = 2048;          // slots per LDS edge-finding group
 
 struct BTile {              // bucketing tile (one workgroup)
