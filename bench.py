@@ -291,6 +291,7 @@ def main():
     ap.add_argument("--load-pct", type=int, default=50, help="hashmap load factor (percent)")
     ap.add_argument("--sample-log2", type=int, default=5, help="splitter sampling rate 2^-s")
     ap.add_argument("--xcd-map", action="store_true", help="XCD-contiguous tile order")
+    ap.add_argument("--autotune", action="store_true", help="tune the settings on the workload before timing")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
     ap.add_argument("--cpu-sample-bp", type=int, default=4_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -322,6 +323,14 @@ def main():
     tun = {"load_pct": args.load_pct, "sample_log2": args.sample_log2}
     c = uc.Compactor(n_buckets=B, mode=mode, profile=True, xcd_map=args.xcd_map, **tun)
     c.load(kmers, k)  # one H2D copy; every step starts from HBM
+    tune = None
+    if args.autotune:
+        t = time.time()
+        tune = c.autotune(l)
+        log(f"autotune: {tune} in {time.time() - t:.1f}s")
+        tun = {"load_pct": tune["load_pct"], "sample_log2": tune["sample_log2"], "fine_slots": tune["fine_slots"],
+               "group_slots": tune["group_slots"]}
+        B = tune["n_buckets"]
     for i in range(args.warmup):
         c.compact_loaded(l)
         log(f"warmup {i + 1}/{args.warmup}: {c.timing().total_ms:.2f} ms")
@@ -404,7 +413,7 @@ def main():
         "config": {
             "workload": f"medium: {args.genome_bp / 1e6:.0f} Mbp seeded genome (seed {args.seed}), 5% repeats of "
                         f"2-10 kbp copies with 0.1% substitutions, 1% error k-mers; distinct canonical {k}-mers",
-            "n_kmers": n, "k": k, "l": l, "buckets": B, "mode": args.mode, **tun,
+            "n_kmers": n, "k": k, "l": l, "buckets": B, "mode": args.mode, **tun, "autotuned": tune,
             "m_placements": b.m_placements, "n_cross": b.n_cross, "n_unitigs": ci.n_unitigs,
             "text_bytes": text_bytes,
         },

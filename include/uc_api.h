@@ -59,6 +59,9 @@ typedef struct {
     uint32_t sample_log2;  /* splitter sampling rate 2^-sample_log2 for path contraction; 0 = default */
     uint64_t hbm_budget;   /* bytes this context may allocate on the device; 0 = no limit */
     uint32_t flags;        /* UC_FLAG_* */
+    uint32_t fine_slots;   /* target slots per fine bucket (64..4096); 0 = default (512 for k <= 32, else 384) */
+    uint32_t group_slots;  /* most slots one LDS edge-finding workgroup takes (256..2048, power of two);
+                              0 = default 2048 */
 } uc_opts;
 
 #define UC_FLAG_PROFILE 1u /* time every kernel launch with HIP events (uc_kernel_stat) */
@@ -256,6 +259,24 @@ int uc_kernel_stats_reset(uc_ctx *ctx);
 
 /* Blocks until the context's stream is idle. */
 int uc_sync(uc_ctx *ctx);
+
+/* ---- autotuner ---------------------------------------------------------
+ * Times whole compactions of the loaded job (uc_load / uc_load_text / ...)
+ * under candidate settings, by coordinate descent over n_buckets,
+ * fine_slots (bucket-to-workgroup tiling), group_slots (LDS table size per
+ * workgroup), sample_log2 (contraction) and load_pct (hashmap load factor
+ * of the HBM tables), and keeps the fastest in the context for every later
+ * job.  The unitig set never depends on these settings; the line order
+ * follows n_buckets and fine_slots.  Each candidate runs `reps` times
+ * (0 = 2) after one warm-up; the minimum counts. */
+typedef struct {
+    uint32_t n_buckets, fine_slots, group_slots, sample_log2, load_pct;
+    uint32_t trials;       /* candidate settings timed */
+    float default_ms;      /* the context's settings before tuning */
+    float best_ms;
+} uc_tune_result;
+
+int uc_autotune(uc_ctx *ctx, uint32_t l, uint32_t reps, uc_tune_result *out);
 
 /* Frees every device buffer the context holds (they grow back on demand). */
 int uc_ctx_trim(uc_ctx *ctx);

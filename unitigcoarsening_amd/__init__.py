@@ -48,6 +48,8 @@ class UcOpts(C.Structure):
         ("sample_log2", C.c_uint32),
         ("hbm_budget", C.c_uint64),
         ("flags", C.c_uint32),
+        ("fine_slots", C.c_uint32),
+        ("group_slots", C.c_uint32),
     ]
 
 
@@ -85,6 +87,19 @@ class UcTiming(C.Structure):
         ("emit_ms", C.c_float),
         ("total_ms", C.c_float),
         ("device_bytes", C.c_uint64),
+    ]
+
+
+class UcTuneResult(C.Structure):
+    _fields_ = [
+        ("n_buckets", C.c_uint32),
+        ("fine_slots", C.c_uint32),
+        ("group_slots", C.c_uint32),
+        ("sample_log2", C.c_uint32),
+        ("load_pct", C.c_uint32),
+        ("trials", C.c_uint32),
+        ("default_ms", C.c_float),
+        ("best_ms", C.c_float),
     ]
 
 
@@ -164,6 +179,7 @@ API = {
     "uc_write_packed_file": (C.c_int, [C.c_char_p, _U32P, C.c_uint64, C.c_uint32]),
     "uc_read_packed_header": (C.c_int, [C.c_char_p, C.POINTER(UcPackedHeader)]),
     "uc_load_packed_file": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_uint32)]),
+    "uc_autotune": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.POINTER(UcTuneResult)]),
     "uc_ctx_trim": (C.c_int, [_P]),
     "uc_ctx_held_bytes": (C.c_uint64, [_P]),
     "uc_job_bytes_estimate": (C.c_uint64, [C.c_uint64, C.c_uint32]),
@@ -309,7 +325,7 @@ class Compactor:
     """One device context (uc_ctx).  Not thread-safe: one host thread per object."""
 
     def __init__(self, device=0, seed=0x5EED, n_buckets=64, mode=UC_MODE_BATCHED, load_pct=50, sample_log2=5,
-                 hbm_budget=0, profile=False, xcd_map=False, keep_src=False):
+                 hbm_budget=0, profile=False, xcd_map=False, keep_src=False, fine_slots=0, group_slots=0):
         L = lib()
         o = UcOpts()
         L.uc_default_opts(C.byref(o))
@@ -317,6 +333,7 @@ class Compactor:
         o.load_pct, o.sample_log2, o.hbm_budget = load_pct, sample_log2, hbm_budget
         o.flags = ((UC_FLAG_PROFILE if profile else 0) | (UC_FLAG_XCD_MAP if xcd_map else 0)
                    | (UC_FLAG_KEEP_SRC if keep_src else 0))
+        o.fine_slots, o.group_slots = fine_slots, group_slots
         self.opts = o
         self._ctx = _P()
         _check(L.uc_ctx_create(C.byref(o), C.byref(self._ctx)))
@@ -489,6 +506,12 @@ class Compactor:
 
     def sync(self):
         _check(lib().uc_sync(self._ctx), self._ctx)
+
+    def autotune(self, l, reps=2):
+        """Tune the settings on the loaded job (uc_autotune); returns the choice as a dict."""
+        r = UcTuneResult()
+        _check(lib().uc_autotune(self._ctx, l, reps, C.byref(r)), self._ctx)
+        return {f: getattr(r, f) for f, _ in UcTuneResult._fields_}
 
     def trim(self):
         """Free every device buffer the context holds (uc_ctx_trim)."""

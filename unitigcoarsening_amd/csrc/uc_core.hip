@@ -2202,7 +2202,7 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     const uint32_t B = c->B, S = c->n_sets;
     const uint32_t W_ = W;
     // fine buckets per coarse bucket, per set: about FINE_TARGET slots each
-    const uint64_t fine_target = W_ == 1 ? 512 : 384;
+    const uint64_t fine_target = c->opts.fine_slots ? c->opts.fine_slots : (W_ == 1 ? 512 : 384);
     c->set_sub.assign(S, 1);
     bool any_fine = false;
     for (uint32_t s = 0; s < S; s++) {
@@ -2441,7 +2441,7 @@ int prepare_edges(uc_ctx *c)
     const uint64_t G = (uint64_t)c->n_sets * c->B;
     const uint32_t load = c->opts.load_pct ? c->opts.load_pct : 50;
     const bool plain = c->opts.mode == UC_MODE_PLAIN;
-    const uint32_t group_max = LDS_GROUP;
+    const uint32_t group_max = c->opts.group_slots ? c->opts.group_slots : LDS_GROUP;
     std::vector<EGroup> groups;
     std::vector<uint32_t> fb_base;
     std::vector<uint64_t> fb_cnt;
@@ -2868,6 +2868,9 @@ int uc_ctx_create(const uc_opts *opts, uc_ctx **out)
     if (o.mode != UC_MODE_BATCHED && o.mode != UC_MODE_PLAIN) return UC_E_ARG;
     if (o.load_pct && (o.load_pct < 10 || o.load_pct > 90)) return UC_E_ARG;
     if (o.sample_log2 > 16) return UC_E_ARG;
+    if (o.fine_slots && (o.fine_slots < 64 || o.fine_slots > 4096)) return UC_E_ARG;
+    if (o.group_slots && (o.group_slots < 256 || o.group_slots > LDS_GROUP || (o.group_slots & (o.group_slots - 1))))
+        return UC_E_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return UC_E_HIP;
     if (o.device < 0 || o.device >= ndev) return UC_E_ARG;
@@ -3145,6 +3148,78 @@ int uc_sync(uc_ctx *c)
     if (!c) return UC_E_ARG;
     CHK(select_dev(c));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return UC_OK;
+}
+
+int uc_autotune(uc_ctx *c, uint32_t l, uint32_t reps, uc_tune_result *out)
+{
+    if (!c || !out) return UC_E_ARG;
+    if (!c->loaded) return fail(c, UC_E_STATE, "uc_autotune before a load");
+    CHK(select_dev(c));
+    if (reps == 0) reps = 2;
+    struct Cand {
+        uint32_t B, fine, group, s, load;
+    };
+    const uint32_t W = c->W;
+    Cand cur{c->B, c->opts.fine_slots ? c->opts.fine_slots : (W == 1 ? 512u : 384u),
+             c->opts.group_slots ? c->opts.group_slots : LDS_GROUP, c->opts.sample_log2 ? c->opts.sample_log2 : 5,
+             c->opts.load_pct ? c->opts.load_pct : 50};
+    uint32_t trials = 0;
+    auto apply = [&](const Cand &x) {
+        c->B = x.B;
+        c->opts.n_buckets = x.B;
+        c->opts.fine_slots = x.fine;
+        c->opts.group_slots = x.group;
+        c->opts.sample_log2 = x.s;
+        c->opts.load_pct = x.load;
+    };
+    auto time_of = [&](const Cand &x, float *ms) -> int {
+        apply(x);
+        trials++;
+        CHK(timed_phases(c, l));
+        float best = 1e30f;
+        for (uint32_t r = 0; r < reps; r++) {
+            CHK(timed_phases(c, l));
+            best = std::min(best, c->timing.total_ms);
+        }
+        *ms = best;
+        return UC_OK;
+    };
+    float best_ms;
+    CHK(time_of(cur, &best_ms));
+    const float default_ms = best_ms;
+    const std::vector<uint32_t> Bs = {32, 64, 128, 256}, fines = {256, 384, 512, 768, 1024},
+                                groups = {1024, 2048}, ss = {4, 5, 6, 7, 8, 9, 10}, loads = {35, 50, 70};
+    for (int pass = 0; pass < 2; pass++) {
+        bool moved = false;
+        for (int dim = 0; dim < 5; dim++) {
+            const std::vector<uint32_t> &vals = dim == 0 ? Bs : dim == 1 ? fines : dim == 2 ? groups : dim == 3 ? ss : loads;
+            for (uint32_t v : vals) {
+                Cand x = cur;
+                uint32_t &f = dim == 0 ? x.B : dim == 1 ? x.fine : dim == 2 ? x.group : dim == 3 ? x.s : x.load;
+                if (f == v) continue;
+                f = v;
+                if (x.B > MAX_BUCKETS) continue;
+                float ms;
+                CHK(time_of(x, &ms));
+                if (ms < best_ms * 0.99f) {
+                    best_ms = ms;
+                    cur = x;
+                    moved = true;
+                }
+            }
+        }
+        if (!moved) break;
+    }
+    apply(cur);
+    out->n_buckets = cur.B;
+    out->fine_slots = cur.fine;
+    out->group_slots = cur.group;
+    out->sample_log2 = cur.s;
+    out->load_pct = cur.load;
+    out->trials = trials;
+    out->default_ms = default_ms;
+    out->best_ms = best_ms;
     return UC_OK;
 }
 
