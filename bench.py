@@ -292,6 +292,9 @@ def main():
     ap.add_argument("--sample-log2", type=int, default=5, help="splitter sampling rate 2^-s")
     ap.add_argument("--xcd-map", action="store_true", help="XCD-contiguous tile order")
     ap.add_argument("--autotune", action="store_true", help="tune the settings on the workload before timing")
+    ap.add_argument("--direct-scatter", action="store_true", help="bucketing scatters without LDS staging")
+    ap.add_argument("--fine-slots", type=int, default=0)
+    ap.add_argument("--group-slots", type=int, default=0)
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
     ap.add_argument("--cpu-sample-bp", type=int, default=4_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -320,7 +323,8 @@ def main():
     log(f"workload: {n} canonical {k}-mers from {args.genome_bp} bp in {time.time() - t:.1f}s")
 
     mode = uc.UC_MODE_PLAIN if args.mode == "plain" else uc.UC_MODE_BATCHED
-    tun = {"load_pct": args.load_pct, "sample_log2": args.sample_log2}
+    tun = {"load_pct": args.load_pct, "sample_log2": args.sample_log2, "fine_slots": args.fine_slots,
+           "group_slots": args.group_slots, "direct_scatter": args.direct_scatter}
     c = uc.Compactor(n_buckets=B, mode=mode, profile=True, xcd_map=args.xcd_map, **tun)
     c.load(kmers, k)  # one H2D copy; every step starts from HBM
     tune = None
@@ -329,7 +333,7 @@ def main():
         tune = c.autotune(l)
         log(f"autotune: {tune} in {time.time() - t:.1f}s")
         tun = {"load_pct": tune["load_pct"], "sample_log2": tune["sample_log2"], "fine_slots": tune["fine_slots"],
-               "group_slots": tune["group_slots"]}
+               "group_slots": tune["group_slots"], "direct_scatter": args.direct_scatter}
         B = tune["n_buckets"]
     for i in range(args.warmup):
         c.compact_loaded(l)
@@ -415,7 +419,9 @@ def main():
                         f"2-10 kbp copies with 0.1% substitutions, 1% error k-mers; distinct canonical {k}-mers",
             "n_kmers": n, "k": k, "l": l, "buckets": B, "mode": args.mode, **tun, "autotuned": tune,
             "m_placements": b.m_placements, "n_cross": b.n_cross, "n_unitigs": ci.n_unitigs,
-            "text_bytes": text_bytes,
+            "text_bytes": text_bytes, "n_fine_buckets": b.n_fine_buckets,
+            "largest_fine_bucket": b.largest_fine_bucket, "bucket_tile": b.bucket_tile, "fine_tile": b.fine_tile,
+            "lds_limit": b.lds_limit, "n_splitters": ci.n_splitters, "chain_rounds": ci.chain_rounds,
         },
         "phases_ms_per_step": {kk: round(v / args.steps, 3) for kk, v in phase.items()},
         "kernels": kernels,

@@ -27,6 +27,7 @@ UC_MODE_PLAIN = 1
 UC_FLAG_PROFILE = 1
 UC_FLAG_XCD_MAP = 2
 UC_FLAG_KEEP_SRC = 4
+UC_FLAG_DIRECT_SCATTER = 8
 
 F_LENT = 1
 F_BORROW = 2
@@ -325,14 +326,15 @@ class Compactor:
     """One device context (uc_ctx).  Not thread-safe: one host thread per object."""
 
     def __init__(self, device=0, seed=0x5EED, n_buckets=64, mode=UC_MODE_BATCHED, load_pct=50, sample_log2=5,
-                 hbm_budget=0, profile=False, xcd_map=False, keep_src=False, fine_slots=0, group_slots=0):
+                 hbm_budget=0, profile=False, xcd_map=False, keep_src=False, fine_slots=0, group_slots=0,
+                 direct_scatter=False):
         L = lib()
         o = UcOpts()
         L.uc_default_opts(C.byref(o))
         o.device, o.seed, o.n_buckets, o.mode = device, seed, n_buckets, mode
         o.load_pct, o.sample_log2, o.hbm_budget = load_pct, sample_log2, hbm_budget
         o.flags = ((UC_FLAG_PROFILE if profile else 0) | (UC_FLAG_XCD_MAP if xcd_map else 0)
-                   | (UC_FLAG_KEEP_SRC if keep_src else 0))
+                   | (UC_FLAG_KEEP_SRC if keep_src else 0) | (UC_FLAG_DIRECT_SCATTER if direct_scatter else 0))
         o.fine_slots, o.group_slots = fine_slots, group_slots
         self.opts = o
         self._ctx = _P()

@@ -416,14 +416,19 @@ __device__ uint32_t block_excl_scan(uint32_t *a, uint32_t n, uint32_t *wsum)
 // LDS sorted by bucket (tile starts from the scanned histogram) and written
 // out in that order, so consecutive lanes store consecutive addresses of one
 // bucket's run.
-template <int W>
-__host__ __device__ constexpr size_t lmin_lds_bytes(uint32_t tile, uint32_t B, bool src)
+__host__ __device__ constexpr size_t lmin_book_bytes(uint32_t B)
 {
-    const size_t P = 2 * (size_t)tile;  // at most two placements per k-mer
-    return P * 8 * W + (src ? P * 4 : 0) + ((size_t)(B + 1) * (WAVES + 1) + 2 * (size_t)B) * 4 + P * 4 + P;
+    return (((size_t)(B + 1) * (WAVES + 1) + 2 * (size_t)B) * 4 + 15) / 16 * 16;
 }
 
 template <int W>
+__host__ __device__ constexpr size_t lmin_lds_bytes(uint32_t tile, uint32_t B, bool src, bool stage)
+{
+    const size_t P = 2 * (size_t)tile;  // at most two placements per k-mer
+    return lmin_book_bytes(B) + (stage ? P * 8 * W + (src ? P * 4 : 0) + P * 4 + P : 0);
+}
+
+template <int W, bool STAGE>
 __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restrict__ kin, const BTile *__restrict__ tiles,
                                                         uint32_t B, uint32_t nbits, uint64_t top_mask,
                                                         const uint2 *__restrict__ bkt,
@@ -438,13 +443,15 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restri
     __shared__ uint32_t wsum[WAVES];
     const bool keep = src != nullptr;
     const size_t P = 2 * (size_t)tile;
-    Kmer<W> *skm = reinterpret_cast<Kmer<W> *>(lds);
-    uint32_t *ssrc = reinterpret_cast<uint32_t *>(lds + P * 8 * W);
-    uint32_t *run = reinterpret_cast<uint32_t *>(lds + P * 8 * W + (keep ? P * 4 : 0));  // B+1 running counts
-    uint32_t *wcnt = run + (B + 1);                                                         // WAVES x (B+1)
+    uint32_t *run = reinterpret_cast<uint32_t *>(lds);  // B+1 running counts
+    uint32_t *wcnt = run + (B + 1);                      // WAVES x (B+1)
     uint32_t *tstart = wcnt + WAVES * (B + 1);
     uint32_t *delta = tstart + B;
-    uint16_t *ssub = reinterpret_cast<uint16_t *>(delta + B);
+    // staging (STAGE): placements sorted by bucket
+    unsigned char *st = lds + lmin_book_bytes(B);
+    Kmer<W> *skm = reinterpret_cast<Kmer<W> *>(st);
+    uint32_t *ssrc = reinterpret_cast<uint32_t *>(st + P * 8 * W);
+    uint16_t *ssub = reinterpret_cast<uint16_t *>(st + P * 8 * W + (keep ? P * 4 : 0));
     uint16_t *sbk = ssub + P;
     uint8_t *sfl = reinterpret_cast<uint8_t *>(sbk + P);
     const BTile t = tiles[blockIdx.x];
@@ -502,20 +509,36 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restri
             const Kmer<W> x = load_kmer<W>(kin, g, top_mask);
             uint32_t lh = tstart[bp] + run[bp] + (uint32_t)(__popcll(hpH & lt) + __popcll(hsH & lt));
             for (int q = 0; q < w; q++) lh += wcnt[q * (B + 1) + bp];
-            skm[lh] = x;
-            sfl[lh] = cross ? F_LENT : 0;
-            ssub[lh] = (uint16_t)fp;
-            sbk[lh] = (uint16_t)bp;
-            if (keep) ssrc[lh] = (uint32_t)g;
+            if constexpr (STAGE) {
+                skm[lh] = x;
+                sfl[lh] = cross ? F_LENT : 0;
+                ssub[lh] = (uint16_t)fp;
+                sbk[lh] = (uint16_t)bp;
+                if (keep) ssrc[lh] = (uint32_t)g;
+            } else {
+                const uint32_t ns = delta[bp] + lh;
+                store_kmer<W>(kout, ns, x);
+                flags[ns] = cross ? F_LENT : 0;
+                sub[ns] = (uint16_t)fp;
+                if (keep) src[ns] = (uint32_t)g;
+            }
             if (cross) {
                 uint32_t lb = tstart[bs] + run[bs] + (uint32_t)(__popcll(hpB & lt) + __popcll(hsB & lt)) +
                               (bp == bs ? 1u : 0u);
                 for (int q = 0; q < w; q++) lb += wcnt[q * (B + 1) + bs];
-                skm[lb] = x;
-                sfl[lb] = F_BORROW;
-                ssub[lb] = (uint16_t)fs;
-                sbk[lb] = (uint16_t)bs;
-                if (keep) ssrc[lb] = (uint32_t)g;
+                if constexpr (STAGE) {
+                    skm[lb] = x;
+                    sfl[lb] = F_BORROW;
+                    ssub[lb] = (uint16_t)fs;
+                    sbk[lb] = (uint16_t)bs;
+                    if (keep) ssrc[lb] = (uint32_t)g;
+                } else {
+                    const uint32_t ns = delta[bs] + lb;
+                    store_kmer<W>(kout, ns, x);
+                    flags[ns] = F_BORROW;
+                    sub[ns] = (uint16_t)fs;
+                    if (keep) src[ns] = (uint32_t)g;
+                }
                 uint32_t px = run[B] + (uint32_t)__popcll(C & lt);
                 for (int q = 0; q < w; q++) px += wcnt[q * (B + 1) + B];
                 pair_home[xbase + px] = delta[bp] + lh;
@@ -535,12 +558,14 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restri
         __syncthreads();
     }
     // write out in bucket order
-    for (uint32_t li = threadIdx.x; li < placed; li += BLOCK) {
-        const uint32_t ns = delta[sbk[li]] + li;
-        store_kmer<W>(kout, ns, skm[li]);
-        flags[ns] = sfl[li];
-        sub[ns] = ssub[li];
-        if (keep) src[ns] = ssrc[li];
+    if constexpr (STAGE) {
+        for (uint32_t li = threadIdx.x; li < placed; li += BLOCK) {
+            const uint32_t ns = delta[sbk[li]] + li;
+            store_kmer<W>(kout, ns, skm[li]);
+            flags[ns] = sfl[li];
+            sub[ns] = ssub[li];
+            if (keep) src[ns] = ssrc[li];
+        }
     }
 }
 
@@ -582,15 +607,19 @@ __global__ __launch_bounds__(BLOCK) void k_sub_count(const uint16_t *__restrict_
 // its stable position inside the tile, sorted by fine bucket.  The slots are
 // staged there in LDS and written out in that order, so consecutive lanes
 // store consecutive addresses of one fine bucket's run.
-template <int W>
-__host__ __device__ constexpr size_t sub_lds_bytes(uint32_t tile, uint32_t S, bool src)
+__host__ __device__ constexpr size_t sub_book_bytes(uint32_t S)
 {
-    return (size_t)tile * 8 * W + (src ? (size_t)tile * 4 : 0) + ((size_t)WAVES * S + 2 * (size_t)S) * 4 +
-           (size_t)tile * 2 + tile;
+    return (((size_t)WAVES * S + 2 * (size_t)S) * 4 + 15) / 16 * 16;
+}
+
+template <int W>
+__host__ __device__ constexpr size_t sub_lds_bytes(uint32_t tile, uint32_t S, bool src, bool stage)
+{
+    return sub_book_bytes(S) + (stage ? (size_t)tile * 8 * W + (src ? (size_t)tile * 4 : 0) + (size_t)tile * 2 + tile : 0);
 }
 
 
-template <int W>
+template <int W, bool STAGE>
 __global__ __launch_bounds__(BLOCK) void k_sub_scatter(const BTile *__restrict__ tiles, const uint16_t *__restrict__ sub,
                                                        const uint32_t *__restrict__ hscan, const uint64_t *__restrict__ kin,
                                                        const uint8_t *__restrict__ fin, const uint32_t *__restrict__ sin,
@@ -604,12 +633,14 @@ __global__ __launch_bounds__(BLOCK) void k_sub_scatter(const BTile *__restrict__
     const uint32_t S = t.fmask;
     const uint32_t nbits = (uint32_t)__builtin_ctz(S);
     const bool src = sin != nullptr;
-    Kmer<W> *skm = reinterpret_cast<Kmer<W> *>(lds);
-    uint32_t *ssrc = reinterpret_cast<uint32_t *>(lds + (size_t)tile * 8 * W);
-    uint32_t *wc = reinterpret_cast<uint32_t *>(lds + (size_t)tile * 8 * W + (src ? (size_t)tile * 4 : 0));
+    uint32_t *wc = reinterpret_cast<uint32_t *>(lds);  // WAVES x S
     uint32_t *tstart = wc + WAVES * S;
     uint32_t *delta = tstart + S;
-    uint16_t *ssub = reinterpret_cast<uint16_t *>(delta + S);
+    // staging (STAGE): slots sorted by fine bucket
+    unsigned char *st = lds + sub_book_bytes(S);
+    Kmer<W> *skm = reinterpret_cast<Kmer<W> *>(st);
+    uint32_t *ssrc = reinterpret_cast<uint32_t *>(st + (size_t)tile * 8 * W);
+    uint16_t *ssub = reinterpret_cast<uint16_t *>(st + (size_t)tile * 8 * W + (src ? (size_t)tile * 4 : 0));
     uint8_t *sfl = reinterpret_cast<uint8_t *>(ssub + tile);
     const int lane = __lane_id(), w = threadIdx.x / 64;
     const uint64_t lt = (1ull << lane) - 1ull, me = 1ull << lane;
@@ -672,20 +703,29 @@ __global__ __launch_bounds__(BLOCK) void k_sub_scatter(const BTile *__restrict__
         const uint32_t base = __shfl(old, leader, 64);
         if (valid) {
             const uint32_t li = base + (uint32_t)__popcll(M & lt);
-            skm[li] = load_kmer<W>(kin, s, ~0ull);
-            sfl[li] = fin[s];
-            ssub[li] = (uint16_t)v;
-            if (src) ssrc[li] = sin[s];
-            perm[s] = delta[v] + li;
+            const uint32_t ns = delta[v] + li;
+            if constexpr (STAGE) {
+                skm[li] = load_kmer<W>(kin, s, ~0ull);
+                sfl[li] = fin[s];
+                ssub[li] = (uint16_t)v;
+                if (src) ssrc[li] = sin[s];
+            } else {
+                store_kmer<W>(kout, ns, load_kmer<W>(kin, s, ~0ull));
+                fout[ns] = fin[s];
+                if (src) sout[ns] = sin[s];
+            }
+            perm[s] = ns;
         }
     }
-    __syncthreads();
     // 4. write out in sorted order
-    for (uint32_t li = threadIdx.x; li < t.count; li += BLOCK) {
-        const uint32_t ns = delta[ssub[li]] + li;
-        store_kmer<W>(kout, ns, skm[li]);
-        fout[ns] = sfl[li];
-        if (src) sout[ns] = ssrc[li];
+    if constexpr (STAGE) {
+        __syncthreads();
+        for (uint32_t li = threadIdx.x; li < t.count; li += BLOCK) {
+            const uint32_t ns = delta[ssub[li]] + li;
+            store_kmer<W>(kout, ns, skm[li]);
+            fout[ns] = sfl[li];
+            if (src) sout[ns] = ssrc[li];
+        }
     }
 }
 
@@ -1029,30 +1069,34 @@ __global__ __launch_bounds__(BLOCK) void k_edge_lds(const uint64_t *__restrict__
     for (int e = 0; e < EPT; e++)
         if (tgt[e] != LDS_NOHIT) predl[tgt[e]] = (uint16_t)(threadIdx.x + (uint32_t)e * BLOCK);
     __syncthreads();
-    // pointer jumping toward the run heads: ancestor | distance << 16 (tab is free)
+    // run walks inside LDS: every run head follows its run (a run ends where
+    // the minimizer changes, so runs are short).  tab is free: pd = run head |
+    // distance << 16 in its first half, successors in its second half.
     uint32_t *pd = tab;
-    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) pd[i] = predl[i] == LDS_NOHIT ? i : ((uint32_t)predl[i] | (1u << 16));
-    __syncthreads();
-    const uint32_t rounds = n > 1 ? 32u - (uint32_t)__builtin_clz(n - 1) : 0u;
-    for (uint32_t r = 0; r < rounds; r++) {
-        uint32_t nv[EPT];
+    uint16_t *succ = reinterpret_cast<uint16_t *>(tab + nb);
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) pd[i] = LDS_EMPTY;
 #pragma unroll
-        for (int e = 0; e < EPT; e++) {
-            const uint32_t i = threadIdx.x + (uint32_t)e * BLOCK;
-            nv[e] = 0;
-            if (i < n) {
-                const uint32_t x = pd[i], y = pd[x & 0xFFFFu];
-                nv[e] = (y & 0xFFFFu) | (((x >> 16) + (y >> 16)) << 16);
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int e = 0; e < EPT; e++) {
-            const uint32_t i = threadIdx.x + (uint32_t)e * BLOCK;
-            if (i < n) pd[i] = nv[e];
-        }
-        __syncthreads();
+    for (int e = 0; e < EPT; e++) {
+        const uint32_t i = threadIdx.x + (uint32_t)e * BLOCK;
+        if (i < n) succ[i] = tgt[e];
     }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+        const uint32_t i = threadIdx.x + (uint32_t)e * BLOCK;
+        if (i >= n || (fl[i] & F_BORROW) || predl[i] != LDS_NOHIT) continue;
+        uint32_t v = i, d = 0;
+        for (;;) {
+            pd[v] = i | (d << 16);
+            const uint16_t nx = succ[v];
+            if (nx == LDS_NOHIT || d + 1 >= n) break;
+            v = nx;
+            d++;
+        }
+        ro.rlen[gr.begin + i] = d + 1;
+        ro.rnext[gr.begin + i] = NONE;
+    }
+    __syncthreads();
 #pragma unroll
     for (int e = 0; e < EPT; e++) {
         const uint32_t i = threadIdx.x + (uint32_t)e * BLOCK;
@@ -1063,23 +1107,17 @@ __global__ __launch_bounds__(BLOCK) void k_edge_lds(const uint64_t *__restrict__
             ro.flags[g] = f;
             continue;
         }
-        uint32_t a = pd[i] & 0xFFFFu, d = pd[i] >> 16;
-        // an ancestor with a predecessor: the slot lies on a cycle inside the
-        // group; such k-mers are runs of their own
-        const bool cyc = predl[a] != LDS_NOHIT;
-        if (cyc) {
-            a = i;
-            d = 0;
-        }
+        // no run head reaches it: the slot lies on a cycle inside the group;
+        // such k-mers are runs of their own
+        const uint32_t x = pd[i];
+        const bool cyc = x == LDS_EMPTY;
+        const uint32_t a = cyc ? i : (x & 0xFFFFu), d = cyc ? 0u : (x >> 16);
         ro.flags[g] = (uint8_t)(f | (a == i ? F_RUNHEAD : 0));
         ro.rhead[g] = gr.begin + a;
         ro.roff[g] = (uint16_t)d;
         if (cyc) {
             ro.rlen[g] = 1;
             ro.rnext[g] = gr.begin + tgt[e];
-        } else if (tgt[e] == LDS_NOHIT) {  // the run's last k-mer
-            ro.rlen[gr.begin + a] = d + 1;
-            ro.rnext[gr.begin + a] = NONE;
         }
     }
 }
@@ -1089,13 +1127,13 @@ __global__ __launch_bounds__(BLOCK) void k_edge_lds(const uint64_t *__restrict__
 //
 // After the splice every live k-mer has at most one successor and one
 // predecessor, so the kept edges form disjoint simple paths (and, rarely,
-// isolated cycles).  Splitters are the path heads plus a hashed 2^-s sample
-// of the other vertices; a segment is a splitter and the vertices up to the
-// next splitter.  Segment lengths come from one walk per splitter, chains of
-// segments are ranked by pointer jumping over the sampled splitters only,
-// and the text is written by a second walk per segment.  A walk step reads
-// one 8-byte link (successor and its last character); no per-vertex array
-// is written.
+// isolated cycles), each a chain of runs.  Splitters are the path heads plus
+// a hashed 2^-s sample of the other run heads; a segment is a splitter and
+// the runs up to the next splitter.  Segment lengths come from one walk per
+// splitter (one step per run: its length and the next run), chains of
+// segments are ranked by pointer jumping over the sampled splitters only, a
+// second walk per segment gives every run its text index, and every k-mer
+// then writes its own character.
 
 constexpr int CT_ITEMS = SCAN_ITEMS;
 constexpr int CT_TILE = SCAN_TILE;  // slots per contraction tile
@@ -2215,9 +2253,10 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     }
     const bool keep_src = (c->opts.flags & UC_FLAG_KEEP_SRC) != 0;
     c->tile2 = 0;
+    const bool stage = !(c->opts.flags & UC_FLAG_DIRECT_SCATTER);
     // k-mers per bucketing tile: as many as the staging LDS allows
     uint32_t tile1 = W_ == 1 ? 2048 : 1024;
-    while (tile1 > 256 && lmin_lds_bytes<W>(tile1, B, keep_src) > c->lds_limit) tile1 >>= 1;
+    while (tile1 > 256 && lmin_lds_bytes<W>(tile1, B, keep_src, stage) > c->lds_limit) tile1 >>= 1;
     c->tile1 = tile1;
     // tiles: every set gets at least one (possibly empty) tile so each bucket
     // row has a start in the histogram
@@ -2290,12 +2329,20 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     CHK(ensure(c, c->pair_b, c->x * 4));
     CHK(ensure(c, c->boff, (G + 1) * 4));
     const uint32_t nbits = (uint32_t)__builtin_ctz(B);
-    const size_t lds = lmin_lds_bytes<W>(tile1, B, keep_src);
-    CHK(allow_lds(c, k_lmin_scatter<W>, lds));
-    KLAUNCH(c, "k_lmin_scatter", k_lmin_scatter<W>, ((uint32_t)T, BLOCK, lds), kin, P<BTile>(c->tiles), B, nbits,
-            c->top_mask, P<uint2>(c->bkt), P<uint32_t>(c->hist), P<uint32_t>(c->xhist), P<uint64_t>(k1),
-            P<uint8_t>(f1), keep_src ? P<uint32_t>(s1) : nullptr, P<uint32_t>(c->pair_h), P<uint32_t>(c->pair_b),
-            P<uint16_t>(c->sub), tile1);
+    const size_t lds = lmin_lds_bytes<W>(tile1, B, keep_src, stage);
+    if (stage) {
+        CHK(allow_lds(c, k_lmin_scatter<W, true>, lds));
+        KLAUNCH(c, "k_lmin_scatter", (k_lmin_scatter<W, true>), ((uint32_t)T, BLOCK, lds), kin, P<BTile>(c->tiles),
+                B, nbits, c->top_mask, P<uint2>(c->bkt), P<uint32_t>(c->hist), P<uint32_t>(c->xhist),
+                P<uint64_t>(k1), P<uint8_t>(f1), keep_src ? P<uint32_t>(s1) : nullptr, P<uint32_t>(c->pair_h),
+                P<uint32_t>(c->pair_b), P<uint16_t>(c->sub), tile1);
+    } else {
+        CHK(allow_lds(c, k_lmin_scatter<W, false>, lds));
+        KLAUNCH(c, "k_lmin_scatter", (k_lmin_scatter<W, false>), ((uint32_t)T, BLOCK, lds), kin, P<BTile>(c->tiles),
+                B, nbits, c->top_mask, P<uint2>(c->bkt), P<uint32_t>(c->hist), P<uint32_t>(c->xhist),
+                P<uint64_t>(k1), P<uint8_t>(f1), keep_src ? P<uint32_t>(s1) : nullptr, P<uint32_t>(c->pair_h),
+                P<uint32_t>(c->pair_b), P<uint16_t>(c->sub), tile1);
+    }
     KLAUNCH(c, "k_gather_boff", k_gather_boff, (grid_for(G + 1), BLOCK, 0), P<uint32_t>(c->hist),
             P<uint64_t>(c->sethb), P<uint32_t>(c->setts), S, B, (uint32_t)c->m, P<uint32_t>(c->boff));
     c->boff_h.resize(G + 1);
@@ -2314,7 +2361,7 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
         for (uint32_t sb : c->set_sub) smax = std::max(smax, sb);
         // slots per fine-sort tile: as many as the staging LDS allows
         uint32_t tile2 = W_ == 1 ? 8192 : 4096;
-        while (tile2 > 512 && sub_lds_bytes<W>(tile2, smax, keep_src) > c->lds_limit) tile2 >>= 1;
+        while (tile2 > 512 && sub_lds_bytes<W>(tile2, smax, keep_src, stage) > c->lds_limit) tile2 >>= 1;
         c->tile2 = tile2;
         std::vector<BTile> t2;
         std::vector<uint64_t> cb_hbase(G);
@@ -2358,12 +2405,20 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
         CHK(ensure(c, c->flags, c->m + 64));
         if (keep_src) CHK(ensure(c, c->src, c->m * 4));
         CHK(ensure(c, c->perm, c->m * 4));
-        const size_t lds2 = sub_lds_bytes<W>(tile2, smax, keep_src);
-        CHK(allow_lds(c, k_sub_scatter<W>, lds2));
-        KLAUNCH(c, "k_sub_scatter", k_sub_scatter<W>, ((uint32_t)T2, BLOCK, lds2), P<BTile>(c->tiles2),
-                P<uint16_t>(c->sub), P<uint32_t>(c->hist2), P<uint64_t>(c->kmer2), P<uint8_t>(c->flags2),
-                keep_src ? P<uint32_t>(c->src2) : nullptr, P<uint64_t>(c->kmer), P<uint8_t>(c->flags),
-                keep_src ? P<uint32_t>(c->src) : nullptr, P<uint32_t>(c->perm), tile2);
+        const size_t lds2 = sub_lds_bytes<W>(tile2, smax, keep_src, stage);
+        if (stage) {
+            CHK(allow_lds(c, k_sub_scatter<W, true>, lds2));
+            KLAUNCH(c, "k_sub_scatter", (k_sub_scatter<W, true>), ((uint32_t)T2, BLOCK, lds2), P<BTile>(c->tiles2),
+                    P<uint16_t>(c->sub), P<uint32_t>(c->hist2), P<uint64_t>(c->kmer2), P<uint8_t>(c->flags2),
+                    keep_src ? P<uint32_t>(c->src2) : nullptr, P<uint64_t>(c->kmer), P<uint8_t>(c->flags),
+                    keep_src ? P<uint32_t>(c->src) : nullptr, P<uint32_t>(c->perm), tile2);
+        } else {
+            CHK(allow_lds(c, k_sub_scatter<W, false>, lds2));
+            KLAUNCH(c, "k_sub_scatter", (k_sub_scatter<W, false>), ((uint32_t)T2, BLOCK, lds2), P<BTile>(c->tiles2),
+                    P<uint16_t>(c->sub), P<uint32_t>(c->hist2), P<uint64_t>(c->kmer2), P<uint8_t>(c->flags2),
+                    keep_src ? P<uint32_t>(c->src2) : nullptr, P<uint64_t>(c->kmer), P<uint8_t>(c->flags),
+                    keep_src ? P<uint32_t>(c->src) : nullptr, P<uint32_t>(c->perm), tile2);
+        }
         if (c->x)
             KLAUNCH(c, "k_pair_remap", k_pair_remap, (grid_for(c->x), BLOCK, 0), P<uint32_t>(c->pair_h),
                     P<uint32_t>(c->pair_b), (uint32_t)c->x, P<uint32_t>(c->perm));
