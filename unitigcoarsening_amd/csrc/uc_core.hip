@@ -333,7 +333,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_tile(const T *in, T *out, uint64
 // ---------------------------------------------------------------------------
 // 1. bucketing
 
-template <int W>
+template <int W, int IPT>
 __global__ __launch_bounds__(BLOCK) void k_lmin_count(const uint64_t *__restrict__ kin, const BTile *__restrict__ tiles,
                                                       LminParams p, uint64_t top_mask, uint2 *__restrict__ bkt,
                                                       uint32_t *__restrict__ hist, uint32_t *__restrict__ xhist)
@@ -346,13 +346,21 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_count(const uint64_t *__restrict
     for (uint32_t i = threadIdx.x; i <= p.B; i += BLOCK) cnt[i] = 0;
     __syncthreads();
     const uint32_t bmask = p.B - 1;
-    for (uint32_t idx = threadIdx.x; idx < t.count; idx += BLOCK) {
-        const uint64_t g = t.kbegin + idx;
-        const Kmer<W> x = load_kmer<W>(kin, g, top_mask);
+    // all of this thread's k-mers in flight at once
+    Kmer<W> xs[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+        const uint32_t idx = (uint32_t)j * BLOCK + threadIdx.x;
+        if (idx < t.count) xs[j] = load_kmer<W>(kin, t.kbegin + idx, top_mask);
+    }
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+        const uint32_t idx = (uint32_t)j * BLOCK + threadIdx.x;
+        if (idx >= t.count) continue;
         uint32_t o1, o2;
-        double_lmin<W>(x, p, o1, o2);
+        double_lmin<W>(xs[j], p, o1, o2);
         const uint32_t f1 = o1 & t.fmask, f2 = o2 & t.fmask;
-        bkt[g] = make_uint2(f1, f2);
+        bkt[t.kbegin + idx] = make_uint2(f1, f2);
         atomicAdd(&cnt[f1 & bmask], 1u);
         if (f1 != f2) {
             atomicAdd(&cnt[f2 & bmask], 1u);
@@ -428,7 +436,7 @@ __host__ __device__ constexpr size_t lmin_lds_bytes(uint32_t tile, uint32_t B, b
     return lmin_book_bytes(B) + (stage ? P * 8 * W + (src ? P * 4 : 0) + P * 4 + P : 0);
 }
 
-template <int W, bool STAGE>
+template <int W, bool STAGE, int IPT>
 __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restrict__ kin, const BTile *__restrict__ tiles,
                                                         uint32_t B, uint32_t nbits, uint64_t top_mask,
                                                         const uint2 *__restrict__ bkt,
@@ -455,6 +463,18 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restri
     uint16_t *sbk = ssub + P;
     uint8_t *sfl = reinterpret_cast<uint8_t *>(sbk + P);
     const BTile t = tiles[blockIdx.x];
+    // all of this thread's codes and k-mers in flight at once
+    uint2 codes[IPT];
+    Kmer<W> xs[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+        const uint32_t idx = (uint32_t)j * BLOCK + threadIdx.x;
+        codes[j] = make_uint2(0u, 0u);
+        if (idx < t.count) {
+            codes[j] = bkt[t.kbegin + idx];
+            xs[j] = load_kmer<W>(kin, t.kbegin + idx, top_mask);
+        }
+    }
     for (uint32_t i = threadIdx.x; i < (B + 1) * (WAVES + 1); i += BLOCK) run[i] = 0;
     // this tile's count per bucket is the next histogram entry minus its own
     for (uint32_t v = threadIdx.x; v < B; v += BLOCK) {
@@ -470,14 +490,16 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restri
     const int lane = __lane_id(), w = threadIdx.x / 64;
     const uint64_t lt = (1ull << lane) - 1ull, me = 1ull << lane;
     const uint32_t xbase = xscan[blockIdx.x];
-    for (uint32_t j0 = 0; j0 < t.count; j0 += BLOCK) {
-        const uint32_t idx = j0 + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+        if ((uint32_t)j * BLOCK >= t.count) break;  // uniform over the block
+        const uint32_t idx = (uint32_t)j * BLOCK + threadIdx.x;
         const bool valid = idx < t.count;
         const uint64_t g = t.kbegin + idx;
         uint32_t bp = 0, bs = 0, fp = 0, fs = 0;
         bool cross = false;
         if (valid) {
-            const uint2 c = bkt[g];
+            const uint2 c = codes[j];
             fp = c.x >> nbits;  // fine index inside the coarse bucket
             fs = c.y >> nbits;
             bp = c.x & (B - 1);
@@ -506,7 +528,7 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restri
         if (C && (C & (0ull - C)) == me) wcnt[w * (B + 1) + B] = (uint32_t)__popcll(C);
         __syncthreads();
         if (valid) {
-            const Kmer<W> x = load_kmer<W>(kin, g, top_mask);
+            const Kmer<W> x = xs[j];
             uint32_t lh = tstart[bp] + run[bp] + (uint32_t)(__popcll(hpH & lt) + __popcll(hsH & lt));
             for (int q = 0; q < w; q++) lh += wcnt[q * (B + 1) + bp];
             if constexpr (STAGE) {
@@ -619,14 +641,14 @@ __host__ __device__ constexpr size_t sub_lds_bytes(uint32_t tile, uint32_t S, bo
 }
 
 
-template <int W, bool STAGE>
+template <int W, bool STAGE, int IPT>
 __global__ __launch_bounds__(BLOCK) void k_sub_scatter(const BTile *__restrict__ tiles, const uint16_t *__restrict__ sub,
                                                        const uint32_t *__restrict__ hscan, const uint64_t *__restrict__ kin,
                                                        const uint8_t *__restrict__ fin, const uint32_t *__restrict__ sin,
                                                        uint64_t *__restrict__ kout, uint8_t *__restrict__ fout,
-                                                       uint32_t *__restrict__ sout, uint32_t *__restrict__ perm,
-                                                       uint32_t tile)
+                                                       uint32_t *__restrict__ sout, uint32_t *__restrict__ perm)
 {
+    constexpr uint32_t TILE = (uint32_t)IPT * BLOCK, CHUNK = TILE / WAVES;  // slots per tile / per wave
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ uint32_t wsum[WAVES];
     const BTile t = tiles[blockIdx.x];
@@ -639,30 +661,36 @@ __global__ __launch_bounds__(BLOCK) void k_sub_scatter(const BTile *__restrict__
     // staging (STAGE): slots sorted by fine bucket
     unsigned char *st = lds + sub_book_bytes(S);
     Kmer<W> *skm = reinterpret_cast<Kmer<W> *>(st);
-    uint32_t *ssrc = reinterpret_cast<uint32_t *>(st + (size_t)tile * 8 * W);
-    uint16_t *ssub = reinterpret_cast<uint16_t *>(st + (size_t)tile * 8 * W + (src ? (size_t)tile * 4 : 0));
-    uint8_t *sfl = reinterpret_cast<uint8_t *>(ssub + tile);
+    uint32_t *ssrc = reinterpret_cast<uint32_t *>(st + (size_t)TILE * 8 * W);
+    uint16_t *ssub = reinterpret_cast<uint16_t *>(st + (size_t)TILE * 8 * W + (src ? (size_t)TILE * 4 : 0));
+    uint8_t *sfl = reinterpret_cast<uint8_t *>(ssub + TILE);
     const int lane = __lane_id(), w = threadIdx.x / 64;
     const uint64_t lt = (1ull << lane) - 1ull, me = 1ull << lane;
     const uint64_t zero[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    const uint32_t chunk = tile / WAVES, c0 = (uint32_t)w * chunk;
+    const uint32_t c0 = (uint32_t)w * CHUNK;
     uint32_t *mywc = wc + (uint32_t)w * S;
+    // all of this lane's slots in flight at once: slot c0 + 64 j + lane
+    uint32_t vf[IPT];  // fine index | flag << 16
+    Kmer<W> xs[IPT];
+    uint32_t sr[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+        const uint32_t idx = c0 + 64u * j + (uint32_t)lane;
+        vf[j] = 0;
+        sr[j] = 0;
+        if (idx < t.count) {
+            const uint64_t s = t.kbegin + idx;
+            vf[j] = (uint32_t)sub[s] | ((uint32_t)fin[s] << 16);
+            xs[j] = load_kmer<W>(kin, s, ~0ull);
+            if (src) sr[j] = sin[s];
+        }
+    }
     for (uint32_t i = threadIdx.x; i < WAVES * S; i += BLOCK) wc[i] = 0;
     __syncthreads();
     // 1. per-wave counts
-    for (uint32_t r = 0; r < chunk; r += 64) {
-        const uint32_t idx = c0 + r + (uint32_t)lane;
-        const bool valid = idx < t.count;
-        const uint32_t v = valid ? sub[t.kbegin + idx] : 0u;
-        const uint64_t V = __ballot(valid);
-        if (!V) break;
-        uint64_t BP[10];
 #pragma unroll
-        for (int q = 0; q < 10; q++) BP[q] = (uint32_t)q < nbits ? (uint64_t)__ballot(valid && ((v >> q) & 1u)) : 0ull;
-        uint64_t M = 0, none;
-        match_bucket(v, nbits, V, 0ull, BP, zero, M, none);
-        if (valid && (M & (0ull - M)) == me) atomicAdd(&mywc[v], (uint32_t)__popcll(M));
-    }
+    for (int j = 0; j < IPT; j++)
+        if (c0 + 64u * j + (uint32_t)lane < t.count) atomicAdd(&mywc[vf[j] & 0xFFFFu], 1u);
     __syncthreads();
     // 2. tile starts per fine bucket; per-wave bases; global minus local offsets
     for (uint32_t v = threadIdx.x; v < S; v += BLOCK) {
@@ -684,14 +712,14 @@ __global__ __launch_bounds__(BLOCK) void k_sub_scatter(const BTile *__restrict__
         delta[v] = hscan[t.hbase + (uint64_t)v * t.ts + t.tlocal] - tstart[v];
     }
     __syncthreads();
-    // 3. stable positions; stage the slots sorted by fine bucket
-    for (uint32_t r = 0; r < chunk; r += 64) {
-        const uint32_t idx = c0 + r + (uint32_t)lane;
+    // 3. stable positions: per round, ballot groups of equal fine index
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+        const uint32_t idx = c0 + 64u * j + (uint32_t)lane;
         const bool valid = idx < t.count;
-        const uint64_t s = t.kbegin + idx;
-        const uint32_t v = valid ? sub[s] : 0u;
         const uint64_t V = __ballot(valid);
-        if (!V) break;
+        if (!V) break;  // uniform over the wave
+        const uint32_t v = vf[j] & 0xFFFFu;
         uint64_t BP[10];
 #pragma unroll
         for (int q = 0; q < 10; q++) BP[q] = (uint32_t)q < nbits ? (uint64_t)__ballot(valid && ((v >> q) & 1u)) : 0ull;
@@ -702,17 +730,18 @@ __global__ __launch_bounds__(BLOCK) void k_sub_scatter(const BTile *__restrict__
         const int leader = valid ? __builtin_ctzll(M) : lane;
         const uint32_t base = __shfl(old, leader, 64);
         if (valid) {
+            const uint64_t s = t.kbegin + idx;
             const uint32_t li = base + (uint32_t)__popcll(M & lt);
             const uint32_t ns = delta[v] + li;
             if constexpr (STAGE) {
-                skm[li] = load_kmer<W>(kin, s, ~0ull);
-                sfl[li] = fin[s];
+                skm[li] = xs[j];
+                sfl[li] = (uint8_t)(vf[j] >> 16);
                 ssub[li] = (uint16_t)v;
-                if (src) ssrc[li] = sin[s];
+                if (src) ssrc[li] = sr[j];
             } else {
-                store_kmer<W>(kout, ns, load_kmer<W>(kin, s, ~0ull));
-                fout[ns] = fin[s];
-                if (src) sout[ns] = sin[s];
+                store_kmer<W>(kout, ns, xs[j]);
+                fout[ns] = (uint8_t)(vf[j] >> 16);
+                if (src) sout[ns] = sr[j];
             }
             perm[s] = ns;
         }
@@ -2253,10 +2282,11 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     }
     const bool keep_src = (c->opts.flags & UC_FLAG_KEEP_SRC) != 0;
     c->tile2 = 0;
-    const bool stage = !(c->opts.flags & UC_FLAG_DIRECT_SCATTER);
-    // k-mers per bucketing tile: as many as the staging LDS allows
-    uint32_t tile1 = W_ == 1 ? 2048 : 1024;
-    while (tile1 > 256 && lmin_lds_bytes<W>(tile1, B, keep_src, stage) > c->lds_limit) tile1 >>= 1;
+    // k-mers per bucketing tile; LDS staging when the device has the room
+    constexpr int IPT1 = W == 1 ? 8 : 4;
+    const uint32_t tile1 = IPT1 * BLOCK;
+    const bool stage = !(c->opts.flags & UC_FLAG_DIRECT_SCATTER) &&
+                       lmin_lds_bytes<W>(tile1, B, keep_src, true) <= c->lds_limit;
     c->tile1 = tile1;
     // tiles: every set gets at least one (possibly empty) tile so each bucket
     // row has a start in the histogram
@@ -2307,7 +2337,7 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     p.table = table;
 
     const uint64_t *kin = c->kin;
-    KLAUNCH(c, "k_lmin_count", k_lmin_count<W>, ((uint32_t)T, BLOCK, (B + 1) * 4), kin, P<BTile>(c->tiles), p,
+    KLAUNCH(c, "k_lmin_count", (k_lmin_count<W, IPT1>), ((uint32_t)T, BLOCK, (B + 1) * 4), kin, P<BTile>(c->tiles), p,
             c->top_mask, P<uint2>(c->bkt), P<uint32_t>(c->hist), P<uint32_t>(c->xhist));
     // totals at hist[hb] and xhist[T]
     CHK(scan_excl<uint32_t>(c, P<uint32_t>(c->hist), P<uint32_t>(c->hist), hb, P<uint32_t>(c->hist) + hb, c->scan_tmp));
@@ -2331,14 +2361,14 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     const uint32_t nbits = (uint32_t)__builtin_ctz(B);
     const size_t lds = lmin_lds_bytes<W>(tile1, B, keep_src, stage);
     if (stage) {
-        CHK(allow_lds(c, k_lmin_scatter<W, true>, lds));
-        KLAUNCH(c, "k_lmin_scatter", (k_lmin_scatter<W, true>), ((uint32_t)T, BLOCK, lds), kin, P<BTile>(c->tiles),
+        CHK(allow_lds(c, k_lmin_scatter<W, true, IPT1>, lds));
+        KLAUNCH(c, "k_lmin_scatter", (k_lmin_scatter<W, true, IPT1>), ((uint32_t)T, BLOCK, lds), kin, P<BTile>(c->tiles),
                 B, nbits, c->top_mask, P<uint2>(c->bkt), P<uint32_t>(c->hist), P<uint32_t>(c->xhist),
                 P<uint64_t>(k1), P<uint8_t>(f1), keep_src ? P<uint32_t>(s1) : nullptr, P<uint32_t>(c->pair_h),
                 P<uint32_t>(c->pair_b), P<uint16_t>(c->sub), tile1);
     } else {
-        CHK(allow_lds(c, k_lmin_scatter<W, false>, lds));
-        KLAUNCH(c, "k_lmin_scatter", (k_lmin_scatter<W, false>), ((uint32_t)T, BLOCK, lds), kin, P<BTile>(c->tiles),
+        CHK(allow_lds(c, k_lmin_scatter<W, false, IPT1>, lds));
+        KLAUNCH(c, "k_lmin_scatter", (k_lmin_scatter<W, false, IPT1>), ((uint32_t)T, BLOCK, lds), kin, P<BTile>(c->tiles),
                 B, nbits, c->top_mask, P<uint2>(c->bkt), P<uint32_t>(c->hist), P<uint32_t>(c->xhist),
                 P<uint64_t>(k1), P<uint8_t>(f1), keep_src ? P<uint32_t>(s1) : nullptr, P<uint32_t>(c->pair_h),
                 P<uint32_t>(c->pair_b), P<uint16_t>(c->sub), tile1);
@@ -2359,9 +2389,11 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     } else {
         uint32_t smax = 1;
         for (uint32_t sb : c->set_sub) smax = std::max(smax, sb);
-        // slots per fine-sort tile: as many as the staging LDS allows
-        uint32_t tile2 = W_ == 1 ? 8192 : 4096;
-        while (tile2 > 512 && sub_lds_bytes<W>(tile2, smax, keep_src, stage) > c->lds_limit) tile2 >>= 1;
+        // slots per fine-sort tile; LDS staging when the device has the room
+        constexpr int IPT2 = W == 1 ? 16 : 8;
+        const uint32_t tile2 = IPT2 * BLOCK;
+        const bool stage2 = !(c->opts.flags & UC_FLAG_DIRECT_SCATTER) &&
+                            sub_lds_bytes<W>(tile2, smax, keep_src, true) <= c->lds_limit;
         c->tile2 = tile2;
         std::vector<BTile> t2;
         std::vector<uint64_t> cb_hbase(G);
@@ -2405,19 +2437,19 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
         CHK(ensure(c, c->flags, c->m + 64));
         if (keep_src) CHK(ensure(c, c->src, c->m * 4));
         CHK(ensure(c, c->perm, c->m * 4));
-        const size_t lds2 = sub_lds_bytes<W>(tile2, smax, keep_src, stage);
-        if (stage) {
-            CHK(allow_lds(c, k_sub_scatter<W, true>, lds2));
-            KLAUNCH(c, "k_sub_scatter", (k_sub_scatter<W, true>), ((uint32_t)T2, BLOCK, lds2), P<BTile>(c->tiles2),
-                    P<uint16_t>(c->sub), P<uint32_t>(c->hist2), P<uint64_t>(c->kmer2), P<uint8_t>(c->flags2),
-                    keep_src ? P<uint32_t>(c->src2) : nullptr, P<uint64_t>(c->kmer), P<uint8_t>(c->flags),
-                    keep_src ? P<uint32_t>(c->src) : nullptr, P<uint32_t>(c->perm), tile2);
+        const size_t lds2 = sub_lds_bytes<W>(tile2, smax, keep_src, stage2);
+        if (stage2) {
+            CHK(allow_lds(c, k_sub_scatter<W, true, IPT2>, lds2));
+            KLAUNCH(c, "k_sub_scatter", (k_sub_scatter<W, true, IPT2>), ((uint32_t)T2, BLOCK, lds2),
+                    P<BTile>(c->tiles2), P<uint16_t>(c->sub), P<uint32_t>(c->hist2), P<uint64_t>(c->kmer2),
+                    P<uint8_t>(c->flags2), keep_src ? P<uint32_t>(c->src2) : nullptr, P<uint64_t>(c->kmer),
+                    P<uint8_t>(c->flags), keep_src ? P<uint32_t>(c->src) : nullptr, P<uint32_t>(c->perm));
         } else {
-            CHK(allow_lds(c, k_sub_scatter<W, false>, lds2));
-            KLAUNCH(c, "k_sub_scatter", (k_sub_scatter<W, false>), ((uint32_t)T2, BLOCK, lds2), P<BTile>(c->tiles2),
-                    P<uint16_t>(c->sub), P<uint32_t>(c->hist2), P<uint64_t>(c->kmer2), P<uint8_t>(c->flags2),
-                    keep_src ? P<uint32_t>(c->src2) : nullptr, P<uint64_t>(c->kmer), P<uint8_t>(c->flags),
-                    keep_src ? P<uint32_t>(c->src) : nullptr, P<uint32_t>(c->perm), tile2);
+            CHK(allow_lds(c, k_sub_scatter<W, false, IPT2>, lds2));
+            KLAUNCH(c, "k_sub_scatter", (k_sub_scatter<W, false, IPT2>), ((uint32_t)T2, BLOCK, lds2),
+                    P<BTile>(c->tiles2), P<uint16_t>(c->sub), P<uint32_t>(c->hist2), P<uint64_t>(c->kmer2),
+                    P<uint8_t>(c->flags2), keep_src ? P<uint32_t>(c->src2) : nullptr, P<uint64_t>(c->kmer),
+                    P<uint8_t>(c->flags), keep_src ? P<uint32_t>(c->src) : nullptr, P<uint32_t>(c->perm));
         }
         if (c->x)
             KLAUNCH(c, "k_pair_remap", k_pair_remap, (grid_for(c->x), BLOCK, 0), P<uint32_t>(c->pair_h),
