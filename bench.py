@@ -289,9 +289,10 @@ def main():
     ap.add_argument("--buckets", type=int, default=64)
     ap.add_argument("--mode", choices=["batched", "plain"], default="batched")
     ap.add_argument("--load-pct", type=int, default=50, help="hashmap load factor (percent)")
-    ap.add_argument("--sample-log2", type=int, default=5, help="splitter sampling rate 2^-s")
+    ap.add_argument("--sample-log2", type=int, default=0, help="splitter sampling rate 2^-s (0: library default)")
     ap.add_argument("--xcd-map", action="store_true", help="XCD-contiguous tile order")
     ap.add_argument("--autotune", action="store_true", help="tune the settings on the workload before timing")
+    ap.add_argument("--tune-buckets", action="store_true", help="let the tuner change the bucket count too")
     ap.add_argument("--direct-scatter", action="store_true", help="bucketing scatters without LDS staging")
     ap.add_argument("--fine-slots", type=int, default=0)
     ap.add_argument("--group-slots", type=int, default=0)
@@ -330,7 +331,7 @@ def main():
     tune = None
     if args.autotune:
         t = time.time()
-        tune = c.autotune(l)
+        tune = c.autotune(l, keep_buckets=not args.tune_buckets)
         log(f"autotune: {tune} in {time.time() - t:.1f}s")
         tun = {"load_pct": tune["load_pct"], "sample_log2": tune["sample_log2"], "fine_slots": tune["fine_slots"],
                "group_slots": tune["group_slots"], "direct_scatter": args.direct_scatter}

@@ -56,7 +56,7 @@ typedef struct {
     uint32_t mode;         /* UC_MODE_BATCHED or UC_MODE_PLAIN */
     uint32_t load_pct;     /* hashmap load factor in percent (10..90; the reference uses <= 50,
                               assembler.h:113-118); 0 = default 50 */
-    uint32_t sample_log2;  /* splitter sampling rate 2^-sample_log2 for path contraction; 0 = default */
+    uint32_t sample_log2;  /* splitter sampling rate 2^-sample_log2 of the run heads; 0 = default 9 */
     uint64_t hbm_budget;   /* bytes this context may allocate on the device; 0 = no limit */
     uint32_t flags;        /* UC_FLAG_* */
     uint32_t fine_slots;   /* target slots per fine bucket (64..4096); 0 = default (512 for k <= 32, else 384) */
@@ -67,8 +67,8 @@ typedef struct {
 #define UC_FLAG_PROFILE 1u /* time every kernel launch with HIP events (uc_kernel_stat) */
 #define UC_FLAG_XCD_MAP 2u /* give each XCD a contiguous range of bucket tiles */
 #define UC_FLAG_KEEP_SRC 4u /* keep slot -> input k-mer ids for uc_bucket_copy */
-#define UC_FLAG_DIRECT_SCATTER 8u /* bucketing scatters write straight from registers instead of
-                                     staging each tile in LDS sorted by bucket */
+#define UC_FLAG_DIRECT_SCATTER 8u /* the fine-bucket sort writes straight from registers instead of
+                                     staging each tile in LDS sorted by fine bucket */
 
 void uc_default_opts(uc_opts *o);
 int uc_ctx_create(const uc_opts *opts, uc_ctx **out);
@@ -278,7 +278,9 @@ typedef struct {
     float best_ms;
 } uc_tune_result;
 
-int uc_autotune(uc_ctx *ctx, uint32_t l, uint32_t reps, uc_tune_result *out);
+#define UC_TUNE_KEEP_BUCKETS 1u /* leave n_buckets as it is (a workload that fixes it) */
+
+int uc_autotune(uc_ctx *ctx, uint32_t l, uint32_t reps, uint32_t tune_flags, uc_tune_result *out);
 
 /* Frees every device buffer the context holds (they grow back on demand). */
 int uc_ctx_trim(uc_ctx *ctx);

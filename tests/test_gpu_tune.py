@@ -20,7 +20,7 @@ def test_autotune_keeps_results(k, l):
         assert r["n_buckets"] in (32, 64, 128, 256)
         assert r["fine_slots"] in (256, 384, 512, 768, 1024)
         assert r["group_slots"] in (1024, 2048)
-        assert 4 <= r["sample_log2"] <= 10
+        assert 4 <= r["sample_log2"] <= 12
         assert r["best_ms"] <= r["default_ms"]
         assert uc.sorted_lines(c.compact(kmers, k, l)) == want
         c.load(kmers, k)
@@ -43,3 +43,12 @@ def test_fine_and_group_settings(fine, group):
         info = c.bucket_info()
     print(fine, group, info.n_fine_buckets, info.largest_fine_bucket, flush=True)
     assert uc.sorted_lines(got) == uc.sorted_lines(oracle.textbook(kmers, 31)[0])
+
+
+def test_autotune_keeps_bucket_count_when_asked():
+    kmers = uc.medium_model(5, 100_000, 31)
+    with uc.Compactor(n_buckets=16) as c:
+        c.load(kmers, 31)
+        r = c.autotune(11, reps=1, keep_buckets=True)
+        assert r["n_buckets"] == 16
+        assert uc.sorted_lines(c.compact(kmers, 31, 11)) == uc.sorted_lines(oracle.textbook(kmers, 31)[0])

@@ -28,6 +28,7 @@ UC_FLAG_PROFILE = 1
 UC_FLAG_XCD_MAP = 2
 UC_FLAG_KEEP_SRC = 4
 UC_FLAG_DIRECT_SCATTER = 8
+UC_TUNE_KEEP_BUCKETS = 1
 
 F_LENT = 1
 F_BORROW = 2
@@ -180,7 +181,7 @@ API = {
     "uc_write_packed_file": (C.c_int, [C.c_char_p, _U32P, C.c_uint64, C.c_uint32]),
     "uc_read_packed_header": (C.c_int, [C.c_char_p, C.POINTER(UcPackedHeader)]),
     "uc_load_packed_file": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_uint32)]),
-    "uc_autotune": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.POINTER(UcTuneResult)]),
+    "uc_autotune": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(UcTuneResult)]),
     "uc_ctx_trim": (C.c_int, [_P]),
     "uc_ctx_held_bytes": (C.c_uint64, [_P]),
     "uc_job_bytes_estimate": (C.c_uint64, [C.c_uint64, C.c_uint32]),
@@ -325,7 +326,7 @@ def encode(kmers, k: int) -> np.ndarray:
 class Compactor:
     """One device context (uc_ctx).  Not thread-safe: one host thread per object."""
 
-    def __init__(self, device=0, seed=0x5EED, n_buckets=64, mode=UC_MODE_BATCHED, load_pct=50, sample_log2=5,
+    def __init__(self, device=0, seed=0x5EED, n_buckets=64, mode=UC_MODE_BATCHED, load_pct=50, sample_log2=0,
                  hbm_budget=0, profile=False, xcd_map=False, keep_src=False, fine_slots=0, group_slots=0,
                  direct_scatter=False):
         L = lib()
@@ -509,10 +510,11 @@ class Compactor:
     def sync(self):
         _check(lib().uc_sync(self._ctx), self._ctx)
 
-    def autotune(self, l, reps=2):
+    def autotune(self, l, reps=2, keep_buckets=False):
         """Tune the settings on the loaded job (uc_autotune); returns the choice as a dict."""
         r = UcTuneResult()
-        _check(lib().uc_autotune(self._ctx, l, reps, C.byref(r)), self._ctx)
+        _check(lib().uc_autotune(self._ctx, l, reps, UC_TUNE_KEEP_BUCKETS if keep_buckets else 0, C.byref(r)),
+               self._ctx)
         return {f: getattr(r, f) for f, _ in UcTuneResult._fields_}
 
     def trim(self):
@@ -546,7 +548,7 @@ class Server:
     submit() copies the k-mers and returns a job id at once; result() waits and returns
     (text, queue_ms, run_ms).  Safe to call from several Python threads."""
 
-    def __init__(self, n_slots=4, hbm_budget=0, device=0, seed=0x5EED, n_buckets=64, sample_log2=5):
+    def __init__(self, n_slots=4, hbm_budget=0, device=0, seed=0x5EED, n_buckets=64, sample_log2=0):
         L = lib()
         o = UcServerOpts()
         L.uc_default_opts(C.byref(o.ctx_opts))

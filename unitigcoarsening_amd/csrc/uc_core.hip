@@ -420,23 +420,17 @@ __device__ uint32_t block_excl_scan(uint32_t *a, uint32_t n, uint32_t *wsum)
 // Stable scatter into the bucket CSR: inside a tile, placements keep the
 // order of their k-mers (home copy before borrowed copy when both land in
 // one bucket); per wave the rank among placements bound for the same bucket
-// comes from log2(B) ballots of the bucket bits.  Placements are staged in
-// LDS sorted by bucket (tile starts from the scanned histogram) and written
-// out in that order, so consecutive lanes store consecutive addresses of one
-// bucket's run.
-__host__ __device__ constexpr size_t lmin_book_bytes(uint32_t B)
+// comes from log2(B) ballots of the bucket bits.  Every thread's codes and
+// k-mers are loaded up front, so all of them are in flight at once.  (A
+// variant staging each tile in LDS sorted by bucket measured slower here:
+// with about 30 placements per bucket and tile the direct stores already
+// fill most of each line in L2.)
+__host__ __device__ constexpr size_t lmin_lds_bytes(uint32_t B)
 {
-    return (((size_t)(B + 1) * (WAVES + 1) + 2 * (size_t)B) * 4 + 15) / 16 * 16;
+    return ((size_t)(B + 1) * (WAVES + 1) + 2 * (size_t)B) * 4;
 }
 
-template <int W>
-__host__ __device__ constexpr size_t lmin_lds_bytes(uint32_t tile, uint32_t B, bool src, bool stage)
-{
-    const size_t P = 2 * (size_t)tile;  // at most two placements per k-mer
-    return lmin_book_bytes(B) + (stage ? P * 8 * W + (src ? P * 4 : 0) + P * 4 + P : 0);
-}
-
-template <int W, bool STAGE, int IPT>
+template <int W, int IPT>
 __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restrict__ kin, const BTile *__restrict__ tiles,
                                                         uint32_t B, uint32_t nbits, uint64_t top_mask,
                                                         const uint2 *__restrict__ bkt,
@@ -445,23 +439,15 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restri
                                                         uint8_t *__restrict__ flags, uint32_t *__restrict__ src,
                                                         uint32_t *__restrict__ pair_home,
                                                         uint32_t *__restrict__ pair_borrow,
-                                                        uint16_t *__restrict__ sub, uint32_t tile)
+                                                        uint16_t *__restrict__ sub)
 {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ uint32_t wsum[WAVES];
     const bool keep = src != nullptr;
-    const size_t P = 2 * (size_t)tile;
     uint32_t *run = reinterpret_cast<uint32_t *>(lds);  // B+1 running counts
     uint32_t *wcnt = run + (B + 1);                      // WAVES x (B+1)
     uint32_t *tstart = wcnt + WAVES * (B + 1);
     uint32_t *delta = tstart + B;
-    // staging (STAGE): placements sorted by bucket
-    unsigned char *st = lds + lmin_book_bytes(B);
-    Kmer<W> *skm = reinterpret_cast<Kmer<W> *>(st);
-    uint32_t *ssrc = reinterpret_cast<uint32_t *>(st + P * 8 * W);
-    uint16_t *ssub = reinterpret_cast<uint16_t *>(st + P * 8 * W + (keep ? P * 4 : 0));
-    uint16_t *sbk = ssub + P;
-    uint8_t *sfl = reinterpret_cast<uint8_t *>(sbk + P);
     const BTile t = tiles[blockIdx.x];
     // all of this thread's codes and k-mers in flight at once
     uint2 codes[IPT];
@@ -484,7 +470,7 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restri
         delta[v] = g0;
     }
     __syncthreads();
-    const uint32_t placed = block_excl_scan(tstart, B, wsum);
+    block_excl_scan(tstart, B, wsum);
     for (uint32_t v = threadIdx.x; v < B; v += BLOCK) delta[v] -= tstart[v];
     __syncthreads();
     const int lane = __lane_id(), w = threadIdx.x / 64;
@@ -531,13 +517,7 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restri
             const Kmer<W> x = xs[j];
             uint32_t lh = tstart[bp] + run[bp] + (uint32_t)(__popcll(hpH & lt) + __popcll(hsH & lt));
             for (int q = 0; q < w; q++) lh += wcnt[q * (B + 1) + bp];
-            if constexpr (STAGE) {
-                skm[lh] = x;
-                sfl[lh] = cross ? F_LENT : 0;
-                ssub[lh] = (uint16_t)fp;
-                sbk[lh] = (uint16_t)bp;
-                if (keep) ssrc[lh] = (uint32_t)g;
-            } else {
+            {
                 const uint32_t ns = delta[bp] + lh;
                 store_kmer<W>(kout, ns, x);
                 flags[ns] = cross ? F_LENT : 0;
@@ -548,13 +528,7 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restri
                 uint32_t lb = tstart[bs] + run[bs] + (uint32_t)(__popcll(hpB & lt) + __popcll(hsB & lt)) +
                               (bp == bs ? 1u : 0u);
                 for (int q = 0; q < w; q++) lb += wcnt[q * (B + 1) + bs];
-                if constexpr (STAGE) {
-                    skm[lb] = x;
-                    sfl[lb] = F_BORROW;
-                    ssub[lb] = (uint16_t)fs;
-                    sbk[lb] = (uint16_t)bs;
-                    if (keep) ssrc[lb] = (uint32_t)g;
-                } else {
+                {
                     const uint32_t ns = delta[bs] + lb;
                     store_kmer<W>(kout, ns, x);
                     flags[ns] = F_BORROW;
@@ -578,16 +552,6 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restri
             run[v] += a;
         }
         __syncthreads();
-    }
-    // write out in bucket order
-    if constexpr (STAGE) {
-        for (uint32_t li = threadIdx.x; li < placed; li += BLOCK) {
-            const uint32_t ns = delta[sbk[li]] + li;
-            store_kmer<W>(kout, ns, skm[li]);
-            flags[ns] = sfl[li];
-            sub[ns] = ssub[li];
-            if (keep) src[ns] = ssrc[li];
-        }
     }
 }
 
@@ -2282,11 +2246,9 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     }
     const bool keep_src = (c->opts.flags & UC_FLAG_KEEP_SRC) != 0;
     c->tile2 = 0;
-    // k-mers per bucketing tile; LDS staging when the device has the room
+    // k-mers per bucketing tile
     constexpr int IPT1 = W == 1 ? 8 : 4;
     const uint32_t tile1 = IPT1 * BLOCK;
-    const bool stage = !(c->opts.flags & UC_FLAG_DIRECT_SCATTER) &&
-                       lmin_lds_bytes<W>(tile1, B, keep_src, true) <= c->lds_limit;
     c->tile1 = tile1;
     // tiles: every set gets at least one (possibly empty) tile so each bucket
     // row has a start in the histogram
@@ -2359,20 +2321,11 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     CHK(ensure(c, c->pair_b, c->x * 4));
     CHK(ensure(c, c->boff, (G + 1) * 4));
     const uint32_t nbits = (uint32_t)__builtin_ctz(B);
-    const size_t lds = lmin_lds_bytes<W>(tile1, B, keep_src, stage);
-    if (stage) {
-        CHK(allow_lds(c, k_lmin_scatter<W, true, IPT1>, lds));
-        KLAUNCH(c, "k_lmin_scatter", (k_lmin_scatter<W, true, IPT1>), ((uint32_t)T, BLOCK, lds), kin, P<BTile>(c->tiles),
-                B, nbits, c->top_mask, P<uint2>(c->bkt), P<uint32_t>(c->hist), P<uint32_t>(c->xhist),
-                P<uint64_t>(k1), P<uint8_t>(f1), keep_src ? P<uint32_t>(s1) : nullptr, P<uint32_t>(c->pair_h),
-                P<uint32_t>(c->pair_b), P<uint16_t>(c->sub), tile1);
-    } else {
-        CHK(allow_lds(c, k_lmin_scatter<W, false, IPT1>, lds));
-        KLAUNCH(c, "k_lmin_scatter", (k_lmin_scatter<W, false, IPT1>), ((uint32_t)T, BLOCK, lds), kin, P<BTile>(c->tiles),
-                B, nbits, c->top_mask, P<uint2>(c->bkt), P<uint32_t>(c->hist), P<uint32_t>(c->xhist),
-                P<uint64_t>(k1), P<uint8_t>(f1), keep_src ? P<uint32_t>(s1) : nullptr, P<uint32_t>(c->pair_h),
-                P<uint32_t>(c->pair_b), P<uint16_t>(c->sub), tile1);
-    }
+    const size_t lds = lmin_lds_bytes(B);
+    KLAUNCH(c, "k_lmin_scatter", (k_lmin_scatter<W, IPT1>), ((uint32_t)T, BLOCK, lds), kin, P<BTile>(c->tiles), B,
+            nbits, c->top_mask, P<uint2>(c->bkt), P<uint32_t>(c->hist), P<uint32_t>(c->xhist), P<uint64_t>(k1),
+            P<uint8_t>(f1), keep_src ? P<uint32_t>(s1) : nullptr, P<uint32_t>(c->pair_h), P<uint32_t>(c->pair_b),
+            P<uint16_t>(c->sub));
     KLAUNCH(c, "k_gather_boff", k_gather_boff, (grid_for(G + 1), BLOCK, 0), P<uint32_t>(c->hist),
             P<uint64_t>(c->sethb), P<uint32_t>(c->setts), S, B, (uint32_t)c->m, P<uint32_t>(c->boff));
     c->boff_h.resize(G + 1);
@@ -2799,7 +2752,7 @@ int do_coarsen(uc_ctx *c)
     c->n_list = 0;
     c->chain = P<uint64_t>(c->chainA);
     c->salt = (uint32_t)(c->opts.seed * 0x9E3779B97F4A7C15ull >> 32);
-    c->shift = c->opts.sample_log2 ? c->opts.sample_log2 : 5;
+    c->shift = c->opts.sample_log2 ? c->opts.sample_log2 : DEFAULT_SAMPLE_LOG2;
     if (c->x) {
         KLAUNCH(c, "k_splice", k_splice, (grid_for(c->x), BLOCK, 0), P<uint32_t>(c->pair_h), P<uint32_t>(c->pair_b),
                 (uint32_t)c->x, P<uint64_t>(c->link), P<uint32_t>(c->rhead), P<uint32_t>(c->rnext));
@@ -2923,7 +2876,7 @@ void uc_default_opts(uc_opts *o)
     o->n_buckets = 64;
     o->mode = UC_MODE_BATCHED;
     o->load_pct = 50;
-    o->sample_log2 = 5;
+    o->sample_log2 = DEFAULT_SAMPLE_LOG2;
     o->hbm_budget = 0;
 }
 
@@ -3238,7 +3191,7 @@ int uc_sync(uc_ctx *c)
     return UC_OK;
 }
 
-int uc_autotune(uc_ctx *c, uint32_t l, uint32_t reps, uc_tune_result *out)
+int uc_autotune(uc_ctx *c, uint32_t l, uint32_t reps, uint32_t tune_flags, uc_tune_result *out)
 {
     if (!c || !out) return UC_E_ARG;
     if (!c->loaded) return fail(c, UC_E_STATE, "uc_autotune before a load");
@@ -3249,7 +3202,7 @@ int uc_autotune(uc_ctx *c, uint32_t l, uint32_t reps, uc_tune_result *out)
     };
     const uint32_t W = c->W;
     Cand cur{c->B, c->opts.fine_slots ? c->opts.fine_slots : (W == 1 ? 512u : 384u),
-             c->opts.group_slots ? c->opts.group_slots : LDS_GROUP, c->opts.sample_log2 ? c->opts.sample_log2 : 5,
+             c->opts.group_slots ? c->opts.group_slots : LDS_GROUP, c->opts.sample_log2 ? c->opts.sample_log2 : DEFAULT_SAMPLE_LOG2,
              c->opts.load_pct ? c->opts.load_pct : 50};
     uint32_t trials = 0;
     auto apply = [&](const Cand &x) {
@@ -3276,10 +3229,10 @@ int uc_autotune(uc_ctx *c, uint32_t l, uint32_t reps, uc_tune_result *out)
     CHK(time_of(cur, &best_ms));
     const float default_ms = best_ms;
     const std::vector<uint32_t> Bs = {32, 64, 128, 256}, fines = {256, 384, 512, 768, 1024},
-                                groups = {1024, 2048}, ss = {4, 5, 6, 7, 8, 9, 10}, loads = {35, 50, 70};
+                                groups = {1024, 2048}, ss = {4, 6, 8, 9, 10, 12}, loads = {35, 50, 70};
     for (int pass = 0; pass < 2; pass++) {
         bool moved = false;
-        for (int dim = 0; dim < 5; dim++) {
+        for (int dim = (tune_flags & UC_TUNE_KEEP_BUCKETS) ? 1 : 0; dim < 5; dim++) {
             const std::vector<uint32_t> &vals = dim == 0 ? Bs : dim == 1 ? fines : dim == 2 ? groups : dim == 3 ? ss : loads;
             for (uint32_t v : vals) {
                 Cand x = cur;
