@@ -40,11 +40,11 @@ def ensure_built():
         build_all()
 
 
-def kernel_bytes(name, n, m, x, W, n_unitigs, text_bytes, k):
+def kernel_bytes(name, n, m, x, W, n_unitigs, text_bytes, k, runs):
     """Algorithmic HBM bytes of one launch of `name` over the whole job:
     every array the kernel must touch, once, at its natural width.
     n k-mers, m slots (k-mers + crossing duplicates), x crossing k-mers,
-    8W bytes per packed k-mer."""
+    8W bytes per packed k-mer, U unitigs, T text bytes, R runs."""
     kb = 8 * W
     table = {
         # read k-mer; write both fine bucket codes
@@ -57,23 +57,25 @@ def kernel_bytes(name, n, m, x, W, n_unitigs, text_bytes, k):
         "k_sub_scatter": m * (2 + kb + 1) + m * (kb + 1 + 4),
         # read both pair entries and their permuted slots; write both
         "k_pair_remap": x * (8 + 8 + 8),
-        # read k-mer + flag per slot; write the 8-byte link per slot and the
-        # has-predecessor bitmap
-        "k_edge_lds": m * (kb + 1) + m * 8 + m // 8,
+        # read k-mer + flag per slot; write link, flag and run character per
+        # slot, the has-predecessor bitmap, a 16-byte record per run and the
+        # run of every lent k-mer
+        "k_edge_lds": m * (kb + 1) + m * (8 + 1 + 1) + m // 8 + runs * 16 + x * 4,
         # fallback tables for oversized fine buckets (16-byte entries): the
         # same three steps over HBM
         "k_edge_insert": m * (1 + kb) + (m - x) * 16,
         "k_edge_lookup": m * (1 + kb + 4) + (m - x) * (16 + kb),
         "k_edge_resolve": m * (4 + 8) + (m - x) * 16 + m // 8,
-        # move the borrowed out-edge onto the home copy
-        "k_splice": x * (4 + 4 + 8 + 8),
-        # read flag + has-predecessor bit per slot, one 8-byte link per
-        # walked k-mer; write length, next splitter, unitig length per head
-        "k_seg_count": m * 1 + m // 8 + n * 8 + n_unitigs * (4 + 4 + 4),
-        # read flag + has-predecessor bit per slot, head k-mer and lengths;
-        # one 8-byte link per k-mer of the head segments (bounded by n);
+        # pair entries, the duplicate's link, the lent k-mer's run; write its
+        # link and its run's next run
+        "k_splice": x * (4 + 4 + 8 + 4 + 8 + 4),
+        # read flag + has-predecessor bit per slot, one run record per run;
+        # write length, next splitter, unitig length per head
+        "k_seg_count": m * 1 + m // 8 + runs * 16 + n_unitigs * (4 + 4 + 4),
+        # read flag + has-predecessor bit per slot, head k-mer, unitig length
+        # and next splitter per head, one record and the characters per run;
         # write the text
-        "k_head_emit": m * 1 + m // 8 + n_unitigs * (kb + 4 + 4 + 4) + n * 8 + text_bytes,
+        "k_head_emit": m * 1 + m // 8 + n_unitigs * (kb + 4 + 4) + runs * 16 + n + text_bytes,
     }
     return table.get(name)
 
@@ -363,7 +365,7 @@ def main():
     avg_ms = dom_ms / max(dom_launches, 1)
     # algorithmic bytes per launch: per-job bytes spread over the launches of
     # one job (batched: one launch per job; plain: one per bucket)
-    per_job = kernel_bytes(dom_name, n, b.m_placements, b.n_cross, W, ci.n_unitigs, ci.text_bytes, k)
+    per_job = kernel_bytes(dom_name, n, b.m_placements, b.n_cross, W, ci.n_unitigs, ci.text_bytes, k, ci.n_runs)
     launches_per_job = dom_launches / args.steps
     roof = None
     if per_job is not None:
@@ -423,6 +425,7 @@ def main():
             "text_bytes": text_bytes, "n_fine_buckets": b.n_fine_buckets,
             "largest_fine_bucket": b.largest_fine_bucket, "bucket_tile": b.bucket_tile, "fine_tile": b.fine_tile,
             "lds_limit": b.lds_limit, "n_splitters": ci.n_splitters, "chain_rounds": ci.chain_rounds,
+            "n_runs": ci.n_runs,
         },
         "phases_ms_per_step": {kk: round(v / args.steps, 3) for kk, v in phase.items()},
         "kernels": kernels,

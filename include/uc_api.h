@@ -165,6 +165,7 @@ typedef struct {
     uint64_t text_bytes;     /* sum over unitigs of (length + k - 1 + 1) */
     uint64_t n_splitters;
     uint32_t chain_rounds;   /* pointer-jumping rounds over segment chains */
+    uint64_t n_runs;         /* runs: chains of edges inside one LDS group (every k-mer is in one) */
 } uc_coarsen_info;
 
 int uc_coarsen_info_get(uc_ctx *ctx, uc_coarsen_info *info);

@@ -78,6 +78,7 @@ class UcCoarsenInfo(C.Structure):
         ("text_bytes", C.c_uint64),
         ("n_splitters", C.c_uint64),
         ("chain_rounds", C.c_uint32),
+        ("n_runs", C.c_uint64),
     ]
 
 

@@ -22,8 +22,8 @@
 //      ranked by pointer jumping (k_chain_*).  Isolated cycles are cut at
 //      their lexicographically smallest k-mer and the contraction repeats.
 //   4. emission    [compact.h:59-210, lmin_buckets.h:172-283]
-//      k_head_emit / k_rand_emit give every run its text index;
-//      k_slot_emit writes every k-mer's character in parallel.
+//      k_head_emit / k_rand_emit write each unitig: its first k-mer, then
+//      the characters of its runs, which the edge kernel left in path order.
 //   5. text input  [driver.cpp:227-354]: k_txt_count / k_txt_pack.
 #include <hip/hip_runtime.h>
 
@@ -392,7 +392,7 @@ __device__ __forceinline__ void match_bucket(uint32_t v, uint32_t nbits, uint64_
     }
 }
 
-// exclusive scan of a[0..n) in place (n <= 4 * BLOCK); returns the total
+// exclusive scan of a[0..n) in place; returns the total
 __device__ uint32_t block_excl_scan(uint32_t *a, uint32_t n, uint32_t *wsum)
 {
     const uint32_t per = (n + BLOCK - 1) / BLOCK;
@@ -773,15 +773,16 @@ constexpr uint64_t NO_LINK = (uint64_t)NONE;
 
 // Runs: maximal chains of edges u -> v with u and v in one LDS group and u
 // not a borrowed duplicate (such edges never leave a fine bucket).  Every
-// live k-mer belongs to one run: rhead = its first k-mer, roff = its
-// distance from it.  At run heads: rlen = k-mers in the run, rnext = first
-// k-mer of the next run on the path (set through the splice for runs that
-// end in a lent k-mer), NONE at a path end.
+// live k-mer belongs to one run.  A run's record sits at its first k-mer's
+// slot: {next run's first k-mer or NONE, length, position of its characters
+// in rch}; rch holds the last character of every run member in path order
+// (run-major inside the group's own slot range).  A lent k-mer ends its run;
+// lhead gives its run's first k-mer so that the splice can continue the run
+// through the duplicate's edge.
 struct RunOut {
-    uint32_t *rhead;
-    uint16_t *roff;
-    uint32_t *rlen;
-    uint32_t *rnext;
+    uint4 *rec;
+    char *rch;
+    uint32_t *lhead;
     uint8_t *flags;
 };
 
@@ -906,7 +907,8 @@ __global__ __launch_bounds__(BLOCK) void k_edge_lookup(const uint64_t *__restric
 __global__ __launch_bounds__(BLOCK) void k_edge_resolve(const ETile *__restrict__ tiles, const uint32_t *__restrict__ boff,
                                                         const uint64_t *__restrict__ toff, const TEntry *table,
                                                         const uint32_t *__restrict__ hit, uint64_t *__restrict__ link,
-                                                        uint32_t *__restrict__ hasp, RunOut ro, uint32_t n_tiles)
+                                                        uint32_t *__restrict__ hasp, const uint64_t *__restrict__ kmer,
+                                                        uint32_t W, uint32_t k, RunOut ro, uint32_t n_tiles)
 {
     const uint32_t ti = xcd_tile(n_tiles);
     if (ti == NONE) return;
@@ -929,10 +931,10 @@ __global__ __launch_bounds__(BLOCK) void k_edge_resolve(const ETile *__restrict_
         const uint8_t f = ro.flags[s];
         if (!(f & F_BORROW)) {
             ro.flags[s] = (uint8_t)(f | F_RUNHEAD);
-            ro.rhead[s] = s;
-            ro.roff[s] = 0;
-            ro.rlen[s] = 1;
-            ro.rnext[s] = (uint32_t)l;
+            ro.rec[s] = make_uint4((uint32_t)l, 1u, s, 0u);
+            const uint64_t word = kmer[(uint64_t)s * W + (k - 1) / 32];
+            ro.rch[s] = base_char((uint32_t)(word >> (2 * ((k - 1) & 31))) & 3u);
+            if (f & F_LENT) ro.lhead[s] = s;
         }
     }
 }
@@ -967,6 +969,7 @@ __global__ __launch_bounds__(BLOCK) void k_edge_lds(const uint64_t *__restrict__
 {
     constexpr int EPT = LDS_GROUP / BLOCK;  // slots per thread (a group holds at most LDS_GROUP)
     const uint8_t *flags = ro.flags;
+    __shared__ uint32_t wsum[WAVES];
     extern __shared__ __align__(16) unsigned char lds[];
     Kmer<W> *km = reinterpret_cast<Kmer<W> *>(lds);
     uint32_t *tab = reinterpret_cast<uint32_t *>(lds + (size_t)nb * 8 * W);
@@ -1054,29 +1057,40 @@ __global__ __launch_bounds__(BLOCK) void k_edge_lds(const uint64_t *__restrict__
         const uint32_t bits = hb[j];
         if (bits) atomicOr(&hasp[(gr.begin >> 5) + j], bits);
     }
-    // runs: run predecessor of every slot (hitp is free now)
+    // runs: run predecessor of every slot (hitp is free now); the last
+    // character of every slot moves into its flag byte (bits 5-6) so that the
+    // k-mer staging area is free
     uint16_t *predl = hitp;
-    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) predl[i] = LDS_NOHIT;
+    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+        predl[i] = LDS_NOHIT;
+        fl[i] = (uint8_t)((fl[i] & 0x1Fu) | (char_at<W>(km[i], k - 1) << 5));
+    }
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < EPT; e++)
         if (tgt[e] != LDS_NOHIT) predl[tgt[e]] = (uint16_t)(threadIdx.x + (uint32_t)e * BLOCK);
-    __syncthreads();
     // run walks inside LDS: every run head follows its run (a run ends where
     // the minimizer changes, so runs are short).  tab is free: pd = run head |
-    // distance << 16 in its first half, successors in its second half.
+    // distance << 16 in its first half, successors in its second half; sc
+    // (over the k-mers) takes the run lengths, then their run-major starts.
     uint32_t *pd = tab;
     uint16_t *succ = reinterpret_cast<uint16_t *>(tab + nb);
-    for (uint32_t i = threadIdx.x; i < n; i += BLOCK) pd[i] = LDS_EMPTY;
+    uint32_t *sc = reinterpret_cast<uint32_t *>(km);
 #pragma unroll
     for (int e = 0; e < EPT; e++) {
         const uint32_t i = threadIdx.x + (uint32_t)e * BLOCK;
-        if (i < n) succ[i] = tgt[e];
+        if (i < n) {
+            succ[i] = tgt[e];
+            pd[i] = LDS_EMPTY;
+            sc[i] = 0;
+        }
     }
     __syncthreads();
+    uint16_t rl[EPT];
 #pragma unroll
     for (int e = 0; e < EPT; e++) {
         const uint32_t i = threadIdx.x + (uint32_t)e * BLOCK;
+        rl[e] = 0;
         if (i >= n || (fl[i] & F_BORROW) || predl[i] != LDS_NOHIT) continue;
         uint32_t v = i, d = 0;
         for (;;) {
@@ -1086,32 +1100,37 @@ __global__ __launch_bounds__(BLOCK) void k_edge_lds(const uint64_t *__restrict__
             v = nx;
             d++;
         }
-        ro.rlen[gr.begin + i] = d + 1;
-        ro.rnext[gr.begin + i] = NONE;
+        rl[e] = (uint16_t)(d + 1);
+        sc[i] = d + 1;
     }
     __syncthreads();
+    // no run head reaches it: the slot lies on a cycle inside the group; such
+    // k-mers are runs of their own
+#pragma unroll
+    for (int e = 0; e < EPT; e++) {
+        const uint32_t i = threadIdx.x + (uint32_t)e * BLOCK;
+        if (i < n && !(fl[i] & F_BORROW) && pd[i] == LDS_EMPTY) sc[i] = 1;
+    }
+    __syncthreads();
+    block_excl_scan(sc, n, wsum);
 #pragma unroll
     for (int e = 0; e < EPT; e++) {
         const uint32_t i = threadIdx.x + (uint32_t)e * BLOCK;
         if (i >= n) continue;
         const uint32_t g = gr.begin + i;
-        const uint8_t f = (uint8_t)(fl[i] & ~F_RUNHEAD);
+        const uint8_t f = (uint8_t)(fl[i] & 0x1Fu & ~F_RUNHEAD);
         if (f & F_BORROW) {
             ro.flags[g] = f;
             continue;
         }
-        // no run head reaches it: the slot lies on a cycle inside the group;
-        // such k-mers are runs of their own
         const uint32_t x = pd[i];
         const bool cyc = x == LDS_EMPTY;
         const uint32_t a = cyc ? i : (x & 0xFFFFu), d = cyc ? 0u : (x >> 16);
+        ro.rch[gr.begin + sc[a] + d] = base_char((fl[i] >> 5) & 3u);
         ro.flags[g] = (uint8_t)(f | (a == i ? F_RUNHEAD : 0));
-        ro.rhead[g] = gr.begin + a;
-        ro.roff[g] = (uint16_t)d;
-        if (cyc) {
-            ro.rlen[g] = 1;
-            ro.rnext[g] = gr.begin + tgt[e];
-        }
+        if (a == i)
+            ro.rec[g] = make_uint4(cyc ? gr.begin + tgt[e] : NONE, cyc ? 1u : (uint32_t)rl[e], gr.begin + sc[i], 0u);
+        if (f & F_LENT) ro.lhead[g] = gr.begin + a;
     }
 }
 
@@ -1134,7 +1153,7 @@ constexpr int CT_TILE = SCAN_TILE;  // slots per contraction tile
 // the borrowed duplicate's out-edge (and its target's character) moves to
 // the home copy; the target keeps its has-predecessor bit
 __global__ void k_splice(const uint32_t *__restrict__ pair_home, const uint32_t *__restrict__ pair_borrow, uint32_t x,
-                         uint64_t *__restrict__ link, const uint32_t *__restrict__ rhead, uint32_t *__restrict__ rnext)
+                         uint64_t *__restrict__ link, const uint32_t *__restrict__ lhead, uint4 *__restrict__ rec)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= x) return;
@@ -1142,7 +1161,7 @@ __global__ void k_splice(const uint32_t *__restrict__ pair_home, const uint32_t 
     const uint64_t l = link[pair_borrow[i]];
     link[h] = l;
     // a lent k-mer ends its run: the run continues through the duplicate's edge
-    rnext[rhead[h]] = (uint32_t)l;
+    reinterpret_cast<uint32_t *>(rec)[4ull * lhead[h]] = (uint32_t)l;
 }
 
 // list index for every flagged lane: one atomic per wave
@@ -1158,7 +1177,7 @@ __device__ __forceinline__ uint32_t wave_append(bool flag, uint32_t *counter)
 }
 
 // counters in the context's small buffer
-enum { C_LIST = 0, C_ERR = 1, C_ACTIVE = 2, C_NONCONV = 3, C_CYC = 4, C_CUT = 5, C_SPLIT = 6 };
+enum { C_LIST = 0, C_ERR = 1, C_ACTIVE = 2, C_NONCONV = 3, C_CYC = 4, C_CUT = 5, C_SPLIT = 6, C_RUNS = 7 };
 
 __device__ __forceinline__ bool live_head(const uint8_t *__restrict__ flags, const uint32_t *__restrict__ hasp,
                                           uint32_t s)
@@ -1169,8 +1188,7 @@ __device__ __forceinline__ bool live_head(const uint8_t *__restrict__ flags, con
 // one walk per splitter: segment length, next splitter; per tile the sum
 // of (1 << 32 | length) over heads whose unitig is a single segment (the
 // other heads add their count here and their length in k_chain_tail)
-__global__ __launch_bounds__(BLOCK) void k_seg_count(const uint32_t *__restrict__ rlen, const uint32_t *__restrict__ rnext,
-                                                     const uint32_t *__restrict__ hasp,
+__global__ __launch_bounds__(BLOCK) void k_seg_count(const uint4 *__restrict__ rec, const uint32_t *__restrict__ hasp,
                                                      const uint8_t *__restrict__ flags, uint32_t m, uint32_t salt,
                                                      uint32_t shift, uint32_t *__restrict__ seglen,
                                                      uint32_t *__restrict__ segnext, uint32_t *__restrict__ prevseg,
@@ -1185,17 +1203,23 @@ __global__ __launch_bounds__(BLOCK) void k_seg_count(const uint32_t *__restrict_
     // splitters of this tile; sampled ones get list slots with one atomic
     // per workgroup
     bool hd[CT_ITEMS], rd[CT_ITEMS];
-    uint32_t nr = 0;
+    uint32_t nr = 0, nruns = 0;
 #pragma unroll
     for (int j = 0; j < CT_ITEMS; j++) {
         const uint64_t s64 = base + (uint64_t)j * BLOCK + threadIdx.x;
         hd[j] = rd[j] = false;
         if (s64 < m && (flags[s64] & (F_BORROW | F_RUNHEAD)) == F_RUNHEAD) {
+            nruns++;
             hd[j] = !has_pred(hasp, (uint32_t)s64);
             rd[j] = !hd[j] && sampled((uint32_t)s64, salt, shift);
         }
         nr += rd[j];
     }
+    // every splitter's first run record in flight at once
+    uint4 first[CT_ITEMS];
+#pragma unroll
+    for (int j = 0; j < CT_ITEMS; j++)
+        if (hd[j] || rd[j]) first[j] = rec[base + (uint64_t)j * BLOCK + threadIdx.x];
     __shared__ uint32_t wtot[WAVES];
     __shared__ uint32_t bbase;
     const uint32_t incl = wave_incl_scan(nr);
@@ -1212,23 +1236,25 @@ __global__ __launch_bounds__(BLOCK) void k_seg_count(const uint32_t *__restrict_
     for (int q = 0; q < w; q++) li += wtot[q];
     uint64_t acc = 0;
     uint32_t nsplit = 0;
+#pragma unroll
     for (int j = 0; j < CT_ITEMS; j++) {
         const uint32_t s = (uint32_t)(base + (uint64_t)j * BLOCK + threadIdx.x);
         const bool head = hd[j], rnd = rd[j];
         if (head || rnd) {
             nsplit++;
-            uint32_t v = s, r = 0, steps = 0;
-            do {
-                r += rlen[v];
-                v = rnext[v];
+            uint32_t r = first[j].y, v = first[j].x, steps = 1;
+            while (v != NONE && !sampled(v, salt, shift)) {
+                const uint4 rr = rec[v];
+                r += rr.y;
+                v = rr.x;
                 // in-degree <= 1 bounds every walk by m runs; the cap only
-                // guards the GPU against corrupted run arrays
+                // guards the GPU against corrupted run records
                 if (++steps > m) {
                     atomicOr(cnt + C_ERR, 1u);
                     v = NONE;
                     break;
                 }
-            } while (v != NONE && !sampled(v, salt, shift));
+            }
             seglen[s] = r;
             segnext[s] = v;
             if (v != NONE) prevseg[v] = s;
@@ -1244,8 +1270,12 @@ __global__ __launch_bounds__(BLOCK) void k_seg_count(const uint32_t *__restrict_
     }
     acc = block_reduce(acc, red);
     if (threadIdx.x == 0) tile_sum[ti] = acc;
-    for (int o = 32; o > 0; o >>= 1) nsplit += __shfl_xor(nsplit, o, 64);
+    for (int o = 32; o > 0; o >>= 1) {
+        nsplit += __shfl_xor(nsplit, o, 64);
+        nruns += __shfl_xor(nruns, o, 64);
+    }
     if (__lane_id() == 0 && nsplit) atomicAdd(cnt + C_SPLIT, nsplit);
+    if (__lane_id() == 0 && nruns) atomicAdd(cnt + C_RUNS, nruns);
 }
 
 // sampled splitters: (preceding segment, its length)
@@ -1309,7 +1339,7 @@ __global__ void k_chain_tail(const uint32_t *__restrict__ list, uint32_t n, cons
 // ---- isolated cycles (only when some k-mers are not reached from a head) --
 
 // marks every run of a segment that hangs off a head
-__global__ __launch_bounds__(BLOCK) void k_cover(const uint32_t *__restrict__ rnext, const uint32_t *__restrict__ hasp,
+__global__ __launch_bounds__(BLOCK) void k_cover(const uint4 *__restrict__ rec, const uint32_t *__restrict__ hasp,
                                                  uint8_t *__restrict__ flags, const uint64_t *__restrict__ chain,
                                                  uint32_t m, uint32_t salt, uint32_t shift)
 {
@@ -1322,26 +1352,28 @@ __global__ __launch_bounds__(BLOCK) void k_cover(const uint32_t *__restrict__ rn
     uint32_t v = s, steps = 0;
     do {
         flags[v] |= F_VISIT;
-        v = rnext[v];
+        v = rec[v].x;
     } while (v != NONE && !sampled(v, salt, shift) && ++steps <= m);
 }
 
-// k-mers whose run no head reaches lie on cycles; they become runs of their own
+// runs that no head reaches lie on cycles; every member becomes a run of its
+// own (its character stays where it is in rch)
 __global__ void k_cycle_init(const uint64_t *__restrict__ link, uint8_t *__restrict__ flags, uint32_t m, RunOut ro,
                              uint64_t *__restrict__ lab, uint32_t *__restrict__ list, uint32_t *__restrict__ n_list)
 {
-    const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool cyc = v < m && !(flags[v] & F_BORROW) && !(flags[ro.rhead[v]] & F_VISIT);
-    const uint32_t li = wave_append(cyc, n_list);
-    if (!cyc) return;
-    const uint64_t l = link[v];
-    flags[v] |= F_CYCLE | F_RUNHEAD;
-    ro.rhead[v] = v;
-    ro.roff[v] = 0;
-    ro.rlen[v] = 1;
-    ro.rnext[v] = (uint32_t)l;
-    lab[v] = (uint64_t)v | ((uint64_t)(uint32_t)l << 32);
-    list[li] = v;
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    // (members converted by their run's head carry F_CYCLE and are skipped)
+    if (s >= m || (flags[s] & (F_BORROW | F_RUNHEAD | F_VISIT | F_CYCLE)) != F_RUNHEAD) return;
+    const uint4 r = ro.rec[s];
+    uint32_t u = s;
+    for (uint32_t j = 0; j < r.y && u != NONE; j++) {
+        const uint64_t l = link[u];
+        flags[u] |= F_CYCLE | F_RUNHEAD;
+        ro.rec[u] = make_uint4((uint32_t)l, 1u, r.z + j, 0u);
+        lab[u] = (uint64_t)u | ((uint64_t)(uint32_t)l << 32);
+        list[atomicAdd(n_list, 1u)] = u;
+        u = (uint32_t)l;
+    }
 }
 
 // min-label pointer jumping around each cycle: (smallest k-mer so far, jump)
@@ -1374,7 +1406,7 @@ __global__ void k_cycle_pred(const uint32_t *__restrict__ list, uint32_t n, cons
 // each cycle loses the edge into its smallest k-mer
 __global__ void k_cycle_cut(const uint32_t *__restrict__ list, uint32_t n, const uint64_t *__restrict__ lab,
                             const uint32_t *__restrict__ pred, uint64_t *__restrict__ link,
-                            uint32_t *__restrict__ rnext, uint32_t *__restrict__ hasp, uint8_t *__restrict__ flags,
+                            uint4 *__restrict__ rec, uint32_t *__restrict__ hasp, uint8_t *__restrict__ flags,
                             uint32_t *__restrict__ n_cut)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1383,7 +1415,7 @@ __global__ void k_cycle_cut(const uint32_t *__restrict__ list, uint32_t n, const
     flags[v] &= (uint8_t)~F_CYCLE;
     if ((uint32_t)lab[v] == v) {
         link[pred[v]] = NO_LINK;
-        rnext[pred[v]] = NONE;
+        rec[pred[v]].x = NONE;
         atomicAnd(&hasp[v >> 5], ~(1u << (v & 31)));
         atomicAdd(n_cut, 1u);
     }
@@ -1402,19 +1434,26 @@ __global__ void k_clear_flag(uint8_t *__restrict__ flags, uint32_t m, uint8_t bi
 // ... + len_u + k): its first k-mer in full, the last character of every
 // later k-mer, '\n'.
 
+// characters of one run from rch into the text
+__device__ __forceinline__ uint64_t copy_run(const char *__restrict__ rch, const uint4 &r, uint32_t skip, char *__restrict__ text,
+                                             uint64_t p)
+{
+    for (uint32_t i = skip; i < r.y; i++) text[p++] = rch[r.z + i];
+    return p;
+}
+
 // per tile: exclusive prefix of (1 << 32 | length) over heads -> text
-// offsets; every head writes its first k-mer and gives each run of its
-// segment the text index of the run's first k-mer's last character
+// offsets; every head writes its first k-mer, then the characters of each
+// later run of its segment
 template <int W>
-__global__ __launch_bounds__(BLOCK) void k_head_emit(const uint64_t *__restrict__ kmer, const uint32_t *__restrict__ rlen,
-                                                     const uint32_t *__restrict__ rnext,
-                                                     const uint32_t *__restrict__ hasp, const uint8_t *__restrict__ flags,
+__global__ __launch_bounds__(BLOCK) void k_head_emit(const uint64_t *__restrict__ kmer, const uint4 *__restrict__ rec,
+                                                     const char *__restrict__ rch, const uint32_t *__restrict__ hasp,
+                                                     const uint8_t *__restrict__ flags,
                                                      const uint32_t *__restrict__ segnext,
                                                      const uint32_t *__restrict__ ulen,
                                                      const uint64_t *__restrict__ tile_off, uint32_t m, uint32_t k,
                                                      uint32_t salt, uint32_t shift, uint64_t *__restrict__ htoff,
-                                                     uint64_t *__restrict__ base, char *__restrict__ text,
-                                                     uint32_t n_tiles)
+                                                     char *__restrict__ text, uint32_t n_tiles)
 {
     __shared__ uint64_t tile[CT_TILE];
     __shared__ uint64_t wsum[WAVES];
@@ -1456,58 +1495,38 @@ __global__ __launch_bounds__(BLOCK) void k_head_emit(const uint64_t *__restrict_
         char *o = text + off;
         const Kmer<W> x0 = load_kmer<W>(kmer, s, ~0ull);
         for (uint32_t i = 0; i < k; i++) o[i] = base_char(char_at<W>(x0, i));
-        uint64_t p = off + k - 1;
-        uint32_t v = s, steps = 0;
-        do {
-            base[v] = p;
-            p += rlen[v];
-            v = rnext[v];
-        } while (v != NONE && !sampled(v, salt, shift) && ++steps < m);
+        uint64_t p = off + k;
+        uint4 r = rec[s];
+        p = copy_run(rch, r, 1, text, p);  // the head's own character is the k-th above
+        uint32_t v = r.x, steps = 0;
+        while (v != NONE && !sampled(v, salt, shift) && ++steps < m) {
+            r = rec[v];
+            p = copy_run(rch, r, 0, text, p);
+            v = r.x;
+        }
         if (v == NONE) text[p] = '\n';
     }
 }
 
-// segments of sampled splitters: text indices from the ranked chain
-__global__ void k_rand_emit(const uint32_t *__restrict__ rlen, const uint32_t *__restrict__ rnext,
+// segments of sampled splitters: text positions from the ranked chain
+__global__ void k_rand_emit(const uint4 *__restrict__ rec, const char *__restrict__ rch,
                             const uint32_t *__restrict__ list, uint32_t n, const uint64_t *__restrict__ chain,
                             const uint64_t *__restrict__ htoff, uint32_t k, uint32_t salt, uint32_t shift,
-                            uint32_t n_slots, uint64_t *__restrict__ base, char *__restrict__ text)
+                            uint32_t n_slots, char *__restrict__ text)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint32_t r = list[i];
-    const uint64_t c = chain[r];
+    const uint32_t r0 = list[i];
+    const uint64_t c = chain[r0];
     const uint32_t a = (uint32_t)c, d = (uint32_t)(c >> 32);
     uint64_t p = htoff[a] + (k - 1) + d;
-    uint32_t v = r, steps = 0;
+    uint32_t v = r0, steps = 0;
     do {
-        base[v] = p;
-        p += rlen[v];
-        v = rnext[v];
+        const uint4 r = rec[v];
+        p = copy_run(rch, r, 0, text, p);
+        v = r.x;
     } while (v != NONE && !sampled(v, salt, shift) && ++steps < n_slots);
     if (v == NONE) text[p] = '\n';
-}
-
-// every live k-mer writes its last character at its run's index plus its
-// distance from the run's first k-mer
-template <int W>
-__global__ __launch_bounds__(BLOCK) void k_slot_emit(const uint64_t *__restrict__ kmer, const uint8_t *__restrict__ flags,
-                                                     const uint32_t *__restrict__ rhead,
-                                                     const uint16_t *__restrict__ roff,
-                                                     const uint64_t *__restrict__ base, uint32_t m, uint32_t k,
-                                                     uint64_t text_bytes, char *__restrict__ text,
-                                                     uint32_t *__restrict__ err)
-{
-    const uint32_t wl = (k - 1) / 32;
-    for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < m; s += (uint64_t)gridDim.x * blockDim.x) {
-        if (flags[s] & F_BORROW) continue;
-        const uint64_t word = kmer[s * W + wl];
-        const uint64_t p = base[rhead[s]] + roff[s];
-        if (p < text_bytes)
-            text[p] = base_char((uint32_t)(word >> (2 * ((k - 1) & 31))) & 3u);
-        else
-            atomicOr(err, 1u);
-    }
 }
 
 // text offset of each set's first unitig: the prefix at the set's first slot
@@ -1684,7 +1703,7 @@ struct uc_ctx {
     uint32_t tile1 = 0, tile2 = 0;            // k-mers per bucketing tile, slots per fine-sort tile
     uint64_t n_fallback = 0, n_groups = 0;
     uint64_t table_slots = 0, n_done = 0;
-    uint64_t n_unitigs = 0, n_cycles = 0, text_bytes = 0, n_splitters = 0;
+    uint64_t n_unitigs = 0, n_cycles = 0, text_bytes = 0, n_splitters = 0, n_runs = 0;
     uint32_t chain_rounds = 0;
     std::vector<uint64_t> set_text;  // n_sets + 1
     uc_timing timing{};
@@ -1694,7 +1713,7 @@ struct uc_ctx {
     DBuf otable, in, stage, bkt, tiles, hist, xhist, kmer2, flags2, src2, sub, perm, tiles2, hist2, cbhb, cbts,
         cbfirst, fboff, egroups, fbbase, scan_tmp, kmer, flags, src, pair_h, pair_b, boff, sethb, setts, etiles,
         toff, tcap, table, hit, link, hasp, pred, seglen, segnext, prevseg, chainA, chainB, cycB, list, clist, ulen, tsum,
-        htoff, setoff, text, small, txt, txtcnt, rhead, roff, rlen, rnext, base;
+        htoff, setoff, text, small, txt, txtcnt, rec, rch, lhead;
     // pinned host chunks of the file reader and writer
     char *pin[2] = {nullptr, nullptr};
     hipEvent_t pin_ev[2] = {nullptr, nullptr};
@@ -1725,8 +1744,7 @@ static std::vector<DBuf *> all_bufs(uc_ctx *c)
             &c->setts,  &c->etiles,  &c->toff,   &c->tcap,    &c->table,   &c->hit,    &c->link,
             &c->hasp,   &c->pred,    &c->seglen, &c->segnext, &c->prevseg, &c->chainA, &c->chainB,
             &c->cycB,   &c->list,    &c->clist,  &c->ulen,    &c->tsum,    &c->htoff,  &c->setoff,
-            &c->text,   &c->small,   &c->txt,    &c->txtcnt,  &c->rhead,   &c->roff,   &c->rlen,
-            &c->rnext,  &c->base};
+            &c->text,   &c->small,   &c->txt,    &c->txtcnt,  &c->rec,     &c->rch,    &c->lhead};
 }
 
 static void free_pinned(uc_ctx *c)
@@ -1796,8 +1814,7 @@ T *P(DBuf &b)
 
 static RunOut run_out(uc_ctx *c)
 {
-    return RunOut{P<uint32_t>(c->rhead), P<uint16_t>(c->roff), P<uint32_t>(c->rlen), P<uint32_t>(c->rnext),
-                  P<uint8_t>(c->flags)};
+    return RunOut{P<uint4>(c->rec), P<char>(c->rch), P<uint32_t>(c->lhead), P<uint8_t>(c->flags)};
 }
 
 // Times one kernel launch with a pair of events on the context's stream
@@ -2456,7 +2473,7 @@ int launch_global_edges(uc_ctx *c, size_t first, size_t count)
             P<TEntry>(c->table), c->pm0, c->pm1, P<uint32_t>(c->hit), tile_arg(c, count));
     KLAUNCH(c, "k_edge_resolve", k_edge_resolve, (xcd_grid(count), BLOCK, 0), tiles, P<uint32_t>(c->fbbase),
             P<uint64_t>(c->toff), P<TEntry>(c->table), P<uint32_t>(c->hit), P<uint64_t>(c->link),
-            P<uint32_t>(c->hasp), run_out(c), tile_arg(c, count));
+            P<uint32_t>(c->hasp), P<uint64_t>(c->kmer), c->W, c->k, run_out(c), tile_arg(c, count));
     return UC_OK;
 }
 
@@ -2563,10 +2580,9 @@ int prepare_edges(uc_ctx *c)
     CHK(ensure(c, c->hit, c->m * 4));
     CHK(ensure(c, c->link, c->m * 8));
     CHK(ensure(c, c->hasp, (c->m + 31) / 32 * 4 + 8));
-    CHK(ensure(c, c->rhead, c->m * 4));
-    CHK(ensure(c, c->roff, c->m * 2 + 8));
-    CHK(ensure(c, c->rlen, c->m * 4));
-    CHK(ensure(c, c->rnext, c->m * 4));
+    CHK(ensure(c, c->rec, c->m * 16 + 16));
+    CHK(ensure(c, c->rch, c->m + 16));
+    CHK(ensure(c, c->lhead, c->m * 4 + 16));
     // has-predecessor bits: every bucket sets only its own slots' bits
     CHK(fill<uint32_t>(c, P<uint32_t>(c->hasp), (c->m + 31) / 32, 0u));
     c->n_fallback = NF;
@@ -2646,8 +2662,7 @@ int contract_pass(uc_ctx *c, uint64_t *live, uint32_t *nonconv)
     const uint64_t T = (c->m + CT_TILE - 1) / CT_TILE;
     uint32_t *cnt = P<uint32_t>(c->small);
     CHK(fill<uint32_t>(c, cnt, 8, 0u));
-    KLAUNCH(c, "k_seg_count", k_seg_count, (xcd_grid(T), BLOCK, 0), P<uint32_t>(c->rlen), P<uint32_t>(c->rnext),
-            P<uint32_t>(c->hasp),
+    KLAUNCH(c, "k_seg_count", k_seg_count, (xcd_grid(T), BLOCK, 0), P<uint4>(c->rec), P<uint32_t>(c->hasp),
             P<uint8_t>(c->flags), m, c->salt, c->shift, P<uint32_t>(c->seglen), P<uint32_t>(c->segnext),
             P<uint32_t>(c->prevseg), P<uint32_t>(c->ulen), P<uint32_t>(c->list), cnt, P<uint64_t>(c->tsum),
             tile_arg(c, T));
@@ -2656,6 +2671,7 @@ int contract_pass(uc_ctx *c, uint64_t *live, uint32_t *nonconv)
     if (h[C_ERR]) return fail(c, UC_E_STATE, "successor array corrupted: a segment walk exceeded m steps");
     c->n_list = h[C_LIST];
     c->n_splitters = h[C_SPLIT];
+    c->n_runs = h[C_RUNS];
     const uint32_t nl = c->n_list;
     uint64_t *ca = P<uint64_t>(c->chainA), *cb = P<uint64_t>(c->chainB);
     uint32_t rounds = 0, bad = 0;
@@ -2701,7 +2717,7 @@ int cut_cycles(uc_ctx *c, uint64_t *n_cut)
 {
     const uint32_t m = (uint32_t)c->m;
     uint32_t *cnt = P<uint32_t>(c->small);
-    KLAUNCH(c, "k_cover", k_cover, (grid_for(m), BLOCK, 0), P<uint32_t>(c->rnext), P<uint32_t>(c->hasp),
+    KLAUNCH(c, "k_cover", k_cover, (grid_for(m), BLOCK, 0), P<uint4>(c->rec), P<uint32_t>(c->hasp),
             P<uint8_t>(c->flags), c->chain, m, c->salt, c->shift);
     CHK(fill<uint32_t>(c, cnt + C_CYC, 2, 0u));
     uint64_t *la = P<uint64_t>(c->chainA) == c->chain ? P<uint64_t>(c->chainB) : P<uint64_t>(c->chainA);
@@ -2721,7 +2737,7 @@ int cut_cycles(uc_ctx *c, uint64_t *n_cut)
         KLAUNCH(c, "k_cycle_pred", k_cycle_pred, (grid_for(nl), BLOCK, 0), P<uint32_t>(c->clist), nl,
                 P<uint64_t>(c->link), P<uint32_t>(c->pred));
         KLAUNCH(c, "k_cycle_cut", k_cycle_cut, (grid_for(nl), BLOCK, 0), P<uint32_t>(c->clist), nl, la,
-                P<uint32_t>(c->pred), P<uint64_t>(c->link), P<uint32_t>(c->rnext), P<uint32_t>(c->hasp),
+                P<uint32_t>(c->pred), P<uint64_t>(c->link), P<uint4>(c->rec), P<uint32_t>(c->hasp),
                 P<uint8_t>(c->flags), cnt + C_CUT);
     }
     KLAUNCH(c, "k_clear_flag", k_clear_flag, (std::min<uint32_t>(grid_for(m), 65536), BLOCK, 0), P<uint8_t>(c->flags),
@@ -2755,7 +2771,7 @@ int do_coarsen(uc_ctx *c)
     c->shift = c->opts.sample_log2 ? c->opts.sample_log2 : DEFAULT_SAMPLE_LOG2;
     if (c->x) {
         KLAUNCH(c, "k_splice", k_splice, (grid_for(c->x), BLOCK, 0), P<uint32_t>(c->pair_h), P<uint32_t>(c->pair_b),
-                (uint32_t)c->x, P<uint64_t>(c->link), P<uint32_t>(c->rhead), P<uint32_t>(c->rnext));
+                (uint32_t)c->x, P<uint64_t>(c->link), P<uint32_t>(c->lhead), P<uint4>(c->rec));
     }
     uint64_t live = 0;
     uint32_t nonconv = 0;
@@ -2784,24 +2800,16 @@ int run_emit(uc_ctx *c)
     const uint64_t T = (c->m + CT_TILE - 1) / CT_TILE;
     CHK(ensure(c, c->text, c->text_bytes + 1));
     CHK(ensure(c, c->htoff, c->m * 8));
-    CHK(ensure(c, c->base, c->m * 8));
     if (m) {
-        KLAUNCH(c, "k_head_emit", k_head_emit<W>, (xcd_grid(T), BLOCK, 0), P<uint64_t>(c->kmer), P<uint32_t>(c->rlen),
-                P<uint32_t>(c->rnext), P<uint32_t>(c->hasp), P<uint8_t>(c->flags), P<uint32_t>(c->segnext),
+        KLAUNCH(c, "k_head_emit", k_head_emit<W>, (xcd_grid(T), BLOCK, 0), P<uint64_t>(c->kmer), P<uint4>(c->rec),
+                P<char>(c->rch), P<uint32_t>(c->hasp), P<uint8_t>(c->flags), P<uint32_t>(c->segnext),
                 P<uint32_t>(c->ulen), P<uint64_t>(c->tsum), m, c->k, c->salt, c->shift, P<uint64_t>(c->htoff),
-                P<uint64_t>(c->base), P<char>(c->text), tile_arg(c, T));
+                P<char>(c->text), tile_arg(c, T));
     }
     if (c->n_list) {
-        KLAUNCH(c, "k_rand_emit", k_rand_emit, (grid_for(c->n_list), BLOCK, 0), P<uint32_t>(c->rlen),
-                P<uint32_t>(c->rnext), P<uint32_t>(c->list), c->n_list, c->chain, P<uint64_t>(c->htoff), c->k,
-                c->salt, c->shift, m, P<uint64_t>(c->base), P<char>(c->text));
-    }
-    uint32_t *cnt = P<uint32_t>(c->small);
-    if (m) {
-        CHK(fill<uint32_t>(c, cnt + C_ERR, 1, 0u));
-        KLAUNCH(c, "k_slot_emit", k_slot_emit<W>, (std::min<uint32_t>(grid_for(m), 65536), BLOCK, 0),
-                P<uint64_t>(c->kmer), P<uint8_t>(c->flags), P<uint32_t>(c->rhead), P<uint16_t>(c->roff),
-                P<uint64_t>(c->base), m, c->k, c->text_bytes, P<char>(c->text), cnt + C_ERR);
+        KLAUNCH(c, "k_rand_emit", k_rand_emit, (grid_for(c->n_list), BLOCK, 0), P<uint4>(c->rec), P<char>(c->rch),
+                P<uint32_t>(c->list), c->n_list, c->chain, P<uint64_t>(c->htoff), c->k, c->salt, c->shift, m,
+                P<char>(c->text));
     }
     // per-set text ranges: a set's unitigs start at its first bucket row
     const uint64_t total = (c->n_unitigs << 32) | (c->text_bytes - c->n_unitigs * c->k);
@@ -2811,11 +2819,6 @@ int run_emit(uc_ctx *c)
             P<uint64_t>(c->setoff));
     std::vector<uint64_t> so(c->n_sets + 1);
     CHK(d2h(c, so.data(), c->setoff.p, (c->n_sets + 1) * 8));
-    if (m) {
-        uint32_t bad = 0;
-        CHK(d2h(c, &bad, cnt + C_ERR, 4));
-        if (bad) return fail(c, UC_E_STATE, "emission: a k-mer's text index fell outside the text");
-    }
     c->set_text.resize(c->n_sets + 1);
     for (uint32_t s = 0; s <= c->n_sets; s++) c->set_text[s] = (so[s] & 0xFFFFFFFFull) + (so[s] >> 32) * c->k;
     c->emitted = true;
@@ -3098,6 +3101,7 @@ int uc_coarsen_info_get(uc_ctx *c, uc_coarsen_info *info)
     info->n_cycles = c->n_cycles;
     info->text_bytes = c->text_bytes;
     info->n_splitters = c->n_splitters;
+    info->n_runs = c->n_runs;
     info->chain_rounds = c->chain_rounds;
     return UC_OK;
 }
@@ -3288,11 +3292,11 @@ uint64_t uc_job_bytes_estimate(uint64_t n, uint32_t k)
     // (the most unitigs a job can have); per slot (at most 1.25 n slots in
     // the minimizer schemes used, crossing k-mers ~9% at k = 31): k-mers and
     // flags twice, fine index, permutation, link, hit, bitmap, four segment
-    // arrays, two chain buffers, list, head offsets, run head, offset,
-    // length, next run, text index
+    // arrays, two chain buffers, list, head offsets, run record, run
+    // characters, run head of lent k-mers
     const uint64_t W = k <= 32 ? 1 : 2;
     const uint64_t per_kmer = 8 * W + 8 + (k + 1);
-    const uint64_t per_slot = 16 * W + 2 + 2 + 4 + 8 + 4 + 1 + 16 + 16 + 4 + 8 + 4 + 2 + 4 + 4 + 8;
+    const uint64_t per_slot = 16 * W + 2 + 2 + 4 + 8 + 4 + 1 + 16 + 16 + 4 + 8 + 16 + 1 + 4;
     return n * per_kmer + (n + n / 4) * per_slot + (64ull << 20);
 }
 
