@@ -774,17 +774,31 @@ constexpr uint64_t NO_LINK = (uint64_t)NONE;
 // Runs: maximal chains of edges u -> v with u and v in one LDS group and u
 // not a borrowed duplicate (such edges never leave a fine bucket).  Every
 // live k-mer belongs to one run.  A run's record sits at its first k-mer's
-// slot: {next run's first k-mer or NONE, length, position of its characters
-// in rch}; rch holds the last character of every run member in path order
+// slot (next run, length, where its characters are); rch holds the last
+// character of every run member in path order
 // (run-major inside the group's own slot range).  A lent k-mer ends its run;
 // lhead gives its run's first k-mer so that the splice can continue the run
 // through the duplicate's edge.
 struct RunOut {
-    uint4 *rec;
+    uint64_t *rec;
     char *rch;
     uint32_t *lhead;
     uint8_t *flags;
 };
+
+// run record (8 bytes): next run's first k-mer | (length - 1) << 32 (11 bits,
+// a run never leaves its LDS group) | (position of its characters in rch
+// minus its own slot) << 43 (12 bits, signed: both lie in the group)
+__host__ __device__ __forceinline__ uint64_t make_rec(uint32_t next, uint32_t len, uint32_t slot, uint32_t pos)
+{
+    return (uint64_t)next | ((uint64_t)((len - 1) & 0x7FFu) << 32) | ((uint64_t)((pos - slot) & 0xFFFu) << 43);
+}
+__device__ __forceinline__ uint32_t rec_next(uint64_t r) { return (uint32_t)r; }
+__device__ __forceinline__ uint32_t rec_len(uint64_t r) { return (uint32_t)((r >> 32) & 0x7FFu) + 1u; }
+__device__ __forceinline__ uint32_t rec_pos(uint64_t r, uint32_t slot)
+{
+    return slot + (uint32_t)(((int32_t)((uint32_t)(r >> 43) & 0xFFFu) << 20) >> 20);
+}
 
 __device__ __forceinline__ uint32_t fastrange(uint32_t h, uint32_t cap)
 {
@@ -931,7 +945,7 @@ __global__ __launch_bounds__(BLOCK) void k_edge_resolve(const ETile *__restrict_
         const uint8_t f = ro.flags[s];
         if (!(f & F_BORROW)) {
             ro.flags[s] = (uint8_t)(f | F_RUNHEAD);
-            ro.rec[s] = make_uint4((uint32_t)l, 1u, s, 0u);
+            ro.rec[s] = make_rec((uint32_t)l, 1u, s, s);
             const uint64_t word = kmer[(uint64_t)s * W + (k - 1) / 32];
             ro.rch[s] = base_char((uint32_t)(word >> (2 * ((k - 1) & 31))) & 3u);
             if (f & F_LENT) ro.lhead[s] = s;
@@ -1129,7 +1143,7 @@ __global__ __launch_bounds__(BLOCK) void k_edge_lds(const uint64_t *__restrict__
         ro.rch[gr.begin + sc[a] + d] = base_char((fl[i] >> 5) & 3u);
         ro.flags[g] = (uint8_t)(f | (a == i ? F_RUNHEAD : 0));
         if (a == i)
-            ro.rec[g] = make_uint4(cyc ? gr.begin + tgt[e] : NONE, cyc ? 1u : (uint32_t)rl[e], gr.begin + sc[i], 0u);
+            ro.rec[g] = make_rec(cyc ? gr.begin + tgt[e] : NONE, cyc ? 1u : (uint32_t)rl[e], g, gr.begin + sc[i]);
         if (f & F_LENT) ro.lhead[g] = gr.begin + a;
     }
 }
@@ -1153,7 +1167,7 @@ constexpr int CT_TILE = SCAN_TILE;  // slots per contraction tile
 // the borrowed duplicate's out-edge (and its target's character) moves to
 // the home copy; the target keeps its has-predecessor bit
 __global__ void k_splice(const uint32_t *__restrict__ pair_home, const uint32_t *__restrict__ pair_borrow, uint32_t x,
-                         uint64_t *__restrict__ link, const uint32_t *__restrict__ lhead, uint4 *__restrict__ rec)
+                         uint64_t *__restrict__ link, const uint32_t *__restrict__ lhead, uint64_t *__restrict__ rec)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= x) return;
@@ -1161,7 +1175,7 @@ __global__ void k_splice(const uint32_t *__restrict__ pair_home, const uint32_t 
     const uint64_t l = link[pair_borrow[i]];
     link[h] = l;
     // a lent k-mer ends its run: the run continues through the duplicate's edge
-    reinterpret_cast<uint32_t *>(rec)[4ull * lhead[h]] = (uint32_t)l;
+    reinterpret_cast<uint32_t *>(rec)[2ull * lhead[h]] = (uint32_t)l;  // the record's next-run field
 }
 
 // list index for every flagged lane: one atomic per wave
@@ -1188,11 +1202,10 @@ __device__ __forceinline__ bool live_head(const uint8_t *__restrict__ flags, con
 // one walk per splitter: segment length, next splitter; per tile the sum
 // of (1 << 32 | length) over heads whose unitig is a single segment (the
 // other heads add their count here and their length in k_chain_tail)
-__global__ __launch_bounds__(BLOCK) void k_seg_count(const uint4 *__restrict__ rec, const uint32_t *__restrict__ hasp,
+__global__ __launch_bounds__(BLOCK) void k_seg_count(const uint64_t *__restrict__ rec, const uint32_t *__restrict__ hasp,
                                                      const uint8_t *__restrict__ flags, uint32_t m, uint32_t salt,
-                                                     uint32_t shift, uint32_t *__restrict__ seglen,
-                                                     uint32_t *__restrict__ segnext, uint32_t *__restrict__ prevseg,
-                                                     uint32_t *__restrict__ ulen, uint32_t *__restrict__ list,
+                                                     uint32_t shift, uint64_t *__restrict__ seg,
+                                                     uint32_t *__restrict__ prevseg, uint32_t *__restrict__ list,
                                                      uint32_t *__restrict__ cnt, uint64_t *__restrict__ tile_sum,
                                                      uint32_t n_tiles)
 {
@@ -1215,11 +1228,6 @@ __global__ __launch_bounds__(BLOCK) void k_seg_count(const uint4 *__restrict__ r
         }
         nr += rd[j];
     }
-    // every splitter's first run record in flight at once
-    uint4 first[CT_ITEMS];
-#pragma unroll
-    for (int j = 0; j < CT_ITEMS; j++)
-        if (hd[j] || rd[j]) first[j] = rec[base + (uint64_t)j * BLOCK + threadIdx.x];
     __shared__ uint32_t wtot[WAVES];
     __shared__ uint32_t bbase;
     const uint32_t incl = wave_incl_scan(nr);
@@ -1236,17 +1244,16 @@ __global__ __launch_bounds__(BLOCK) void k_seg_count(const uint4 *__restrict__ r
     for (int q = 0; q < w; q++) li += wtot[q];
     uint64_t acc = 0;
     uint32_t nsplit = 0;
-#pragma unroll
     for (int j = 0; j < CT_ITEMS; j++) {
         const uint32_t s = (uint32_t)(base + (uint64_t)j * BLOCK + threadIdx.x);
         const bool head = hd[j], rnd = rd[j];
         if (head || rnd) {
             nsplit++;
-            uint32_t r = first[j].y, v = first[j].x, steps = 1;
-            while (v != NONE && !sampled(v, salt, shift)) {
-                const uint4 rr = rec[v];
-                r += rr.y;
-                v = rr.x;
+            uint32_t v = s, r = 0, steps = 0;
+            do {
+                const uint64_t rr = rec[v];
+                r += rec_len(rr);
+                v = rec_next(rr);
                 // in-degree <= 1 bounds every walk by m runs; the cap only
                 // guards the GPU against corrupted run records
                 if (++steps > m) {
@@ -1254,16 +1261,12 @@ __global__ __launch_bounds__(BLOCK) void k_seg_count(const uint4 *__restrict__ r
                     v = NONE;
                     break;
                 }
-            }
-            seglen[s] = r;
-            segnext[s] = v;
+            } while (v != NONE && !sampled(v, salt, shift));
+            seg[s] = (uint64_t)r | ((uint64_t)v << 32);
             if (v != NONE) prevseg[v] = s;
             if (head) {
                 acc += 1ull << 32;
-                if (v == NONE) {
-                    ulen[s] = r;
-                    acc += r;
-                }
+                if (v == NONE) acc += r;
             }
             if (rnd) list[li++] = s;
         }
@@ -1280,12 +1283,12 @@ __global__ __launch_bounds__(BLOCK) void k_seg_count(const uint4 *__restrict__ r
 
 // sampled splitters: (preceding segment, its length)
 __global__ void k_chain_init(const uint32_t *__restrict__ list, uint32_t n, const uint32_t *__restrict__ prevseg,
-                             const uint32_t *__restrict__ seglen, uint64_t *__restrict__ chain)
+                             const uint64_t *__restrict__ seg, uint64_t *__restrict__ chain)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t r = list[i], a = prevseg[r];
-    chain[r] = (uint64_t)a | ((uint64_t)seglen[a] << 32);
+    chain[r] = (uint64_t)a | ((uint64_t)(uint32_t)seg[a] << 32);
 }
 
 // one synchronous pointer-jumping round: (ancestor, distance) pairs
@@ -1314,8 +1317,8 @@ __global__ void k_chain_round(const uint32_t *__restrict__ list, uint32_t n, con
 
 // the tail segment of a multi-segment unitig gives the unitig length
 __global__ void k_chain_tail(const uint32_t *__restrict__ list, uint32_t n, const uint32_t *__restrict__ hasp,
-                             const uint64_t *__restrict__ chain, const uint32_t *__restrict__ seglen,
-                             const uint32_t *__restrict__ segnext, uint32_t *__restrict__ ulen,
+                             const uint64_t *__restrict__ chain, const uint64_t *__restrict__ seg,
+                             uint32_t *__restrict__ ulen,
                              uint64_t *__restrict__ tile_sum, uint32_t *__restrict__ nonconv)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1326,8 +1329,8 @@ __global__ void k_chain_tail(const uint32_t *__restrict__ list, uint32_t n, cons
         const uint32_t a = (uint32_t)c;
         if (has_pred(hasp, a)) {
             bad = true;  // no head behind this chain: an isolated cycle
-        } else if (segnext[r] == NONE) {
-            const uint32_t len = (uint32_t)(c >> 32) + seglen[r];
+        } else if ((uint32_t)(seg[r] >> 32) == NONE) {
+            const uint32_t len = (uint32_t)(c >> 32) + (uint32_t)seg[r];
             ulen[a] = len;
             atomicAdd((unsigned long long *)&tile_sum[a / CT_TILE], (unsigned long long)len);
         }
@@ -1339,7 +1342,7 @@ __global__ void k_chain_tail(const uint32_t *__restrict__ list, uint32_t n, cons
 // ---- isolated cycles (only when some k-mers are not reached from a head) --
 
 // marks every run of a segment that hangs off a head
-__global__ __launch_bounds__(BLOCK) void k_cover(const uint4 *__restrict__ rec, const uint32_t *__restrict__ hasp,
+__global__ __launch_bounds__(BLOCK) void k_cover(const uint64_t *__restrict__ rec, const uint32_t *__restrict__ hasp,
                                                  uint8_t *__restrict__ flags, const uint64_t *__restrict__ chain,
                                                  uint32_t m, uint32_t salt, uint32_t shift)
 {
@@ -1352,7 +1355,7 @@ __global__ __launch_bounds__(BLOCK) void k_cover(const uint4 *__restrict__ rec, 
     uint32_t v = s, steps = 0;
     do {
         flags[v] |= F_VISIT;
-        v = rec[v].x;
+        v = rec_next(rec[v]);
     } while (v != NONE && !sampled(v, salt, shift) && ++steps <= m);
 }
 
@@ -1364,12 +1367,13 @@ __global__ void k_cycle_init(const uint64_t *__restrict__ link, uint8_t *__restr
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     // (members converted by their run's head carry F_CYCLE and are skipped)
     if (s >= m || (flags[s] & (F_BORROW | F_RUNHEAD | F_VISIT | F_CYCLE)) != F_RUNHEAD) return;
-    const uint4 r = ro.rec[s];
+    const uint64_t r = ro.rec[s];
+    const uint32_t len = rec_len(r), pos = rec_pos(r, s);
     uint32_t u = s;
-    for (uint32_t j = 0; j < r.y && u != NONE; j++) {
+    for (uint32_t j = 0; j < len && u != NONE; j++) {
         const uint64_t l = link[u];
         flags[u] |= F_CYCLE | F_RUNHEAD;
-        ro.rec[u] = make_uint4((uint32_t)l, 1u, r.z + j, 0u);
+        ro.rec[u] = make_rec((uint32_t)l, 1u, u, pos + j);
         lab[u] = (uint64_t)u | ((uint64_t)(uint32_t)l << 32);
         list[atomicAdd(n_list, 1u)] = u;
         u = (uint32_t)l;
@@ -1406,7 +1410,7 @@ __global__ void k_cycle_pred(const uint32_t *__restrict__ list, uint32_t n, cons
 // each cycle loses the edge into its smallest k-mer
 __global__ void k_cycle_cut(const uint32_t *__restrict__ list, uint32_t n, const uint64_t *__restrict__ lab,
                             const uint32_t *__restrict__ pred, uint64_t *__restrict__ link,
-                            uint4 *__restrict__ rec, uint32_t *__restrict__ hasp, uint8_t *__restrict__ flags,
+                            uint64_t *__restrict__ rec, uint32_t *__restrict__ hasp, uint8_t *__restrict__ flags,
                             uint32_t *__restrict__ n_cut)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1415,7 +1419,7 @@ __global__ void k_cycle_cut(const uint32_t *__restrict__ list, uint32_t n, const
     flags[v] &= (uint8_t)~F_CYCLE;
     if ((uint32_t)lab[v] == v) {
         link[pred[v]] = NO_LINK;
-        rec[pred[v]].x = NONE;
+        reinterpret_cast<uint32_t *>(rec)[2ull * pred[v]] = NONE;  // the record's next-run field
         atomicAnd(&hasp[v >> 5], ~(1u << (v & 31)));
         atomicAdd(n_cut, 1u);
     }
@@ -1434,11 +1438,21 @@ __global__ void k_clear_flag(uint8_t *__restrict__ flags, uint32_t m, uint8_t bi
 // ... + len_u + k): its first k-mer in full, the last character of every
 // later k-mer, '\n'.
 
-// characters of one run from rch into the text
-__device__ __forceinline__ uint64_t copy_run(const char *__restrict__ rch, const uint4 &r, uint32_t skip, char *__restrict__ text,
-                                             uint64_t p)
+// unitig length at a head: its segment's, or the chain's when it has more
+// than one segment
+__device__ __forceinline__ uint32_t unitig_len(const uint64_t *__restrict__ seg, const uint32_t *__restrict__ ulen,
+                                               uint32_t s)
 {
-    for (uint32_t i = skip; i < r.y; i++) text[p++] = rch[r.z + i];
+    const uint64_t sg = seg[s];
+    return (uint32_t)(sg >> 32) == NONE ? (uint32_t)sg : ulen[s];
+}
+
+// characters of one run from rch into the text
+__device__ __forceinline__ uint64_t copy_run(const char *__restrict__ rch, uint64_t r, uint32_t slot, uint32_t skip,
+                                             char *__restrict__ text, uint64_t p)
+{
+    const uint32_t len = rec_len(r), pos = rec_pos(r, slot);
+    for (uint32_t i = skip; i < len; i++) text[p++] = rch[pos + i];
     return p;
 }
 
@@ -1446,10 +1460,9 @@ __device__ __forceinline__ uint64_t copy_run(const char *__restrict__ rch, const
 // offsets; every head writes its first k-mer, then the characters of each
 // later run of its segment
 template <int W>
-__global__ __launch_bounds__(BLOCK) void k_head_emit(const uint64_t *__restrict__ kmer, const uint4 *__restrict__ rec,
+__global__ __launch_bounds__(BLOCK) void k_head_emit(const uint64_t *__restrict__ kmer, const uint64_t *__restrict__ rec,
                                                      const char *__restrict__ rch, const uint32_t *__restrict__ hasp,
-                                                     const uint8_t *__restrict__ flags,
-                                                     const uint32_t *__restrict__ segnext,
+                                                     const uint8_t *__restrict__ flags, const uint64_t *__restrict__ seg,
                                                      const uint32_t *__restrict__ ulen,
                                                      const uint64_t *__restrict__ tile_off, uint32_t m, uint32_t k,
                                                      uint32_t salt, uint32_t shift, uint64_t *__restrict__ htoff,
@@ -1465,7 +1478,7 @@ __global__ __launch_bounds__(BLOCK) void k_head_emit(const uint64_t *__restrict_
     for (int j = 0; j < CT_ITEMS; j++) {
         const uint64_t s = t0 + (uint64_t)j * BLOCK + threadIdx.x;
         hd[j] = s < m && live_head(flags, hasp, (uint32_t)s);
-        tile[j * BLOCK + threadIdx.x] = hd[j] ? ((1ull << 32) | ulen[s]) : 0ull;
+        tile[j * BLOCK + threadIdx.x] = hd[j] ? ((1ull << 32) | unitig_len(seg, ulen, (uint32_t)s)) : 0ull;
     }
     __syncthreads();
     uint64_t loc[CT_ITEMS];
@@ -1491,25 +1504,24 @@ __global__ __launch_bounds__(BLOCK) void k_head_emit(const uint64_t *__restrict_
         const uint32_t s = (uint32_t)(t0 + (uint64_t)j * BLOCK + threadIdx.x);
         const uint64_t pre = tile[j * BLOCK + threadIdx.x];
         const uint64_t off = (pre & 0xFFFFFFFFull) + (pre >> 32) * k;
-        if (segnext[s] != NONE) htoff[s] = off;
+        if ((uint32_t)(seg[s] >> 32) != NONE) htoff[s] = off;
         char *o = text + off;
         const Kmer<W> x0 = load_kmer<W>(kmer, s, ~0ull);
         for (uint32_t i = 0; i < k; i++) o[i] = base_char(char_at<W>(x0, i));
-        uint64_t p = off + k;
-        uint4 r = rec[s];
-        p = copy_run(rch, r, 1, text, p);  // the head's own character is the k-th above
-        uint32_t v = r.x, steps = 0;
+        uint64_t r = rec[s];
+        uint64_t p = copy_run(rch, r, s, 1, text, off + k);  // the head's own character is the k-th above
+        uint32_t v = rec_next(r), steps = 0;
         while (v != NONE && !sampled(v, salt, shift) && ++steps < m) {
             r = rec[v];
-            p = copy_run(rch, r, 0, text, p);
-            v = r.x;
+            p = copy_run(rch, r, v, 0, text, p);
+            v = rec_next(r);
         }
         if (v == NONE) text[p] = '\n';
     }
 }
 
 // segments of sampled splitters: text positions from the ranked chain
-__global__ void k_rand_emit(const uint4 *__restrict__ rec, const char *__restrict__ rch,
+__global__ void k_rand_emit(const uint64_t *__restrict__ rec, const char *__restrict__ rch,
                             const uint32_t *__restrict__ list, uint32_t n, const uint64_t *__restrict__ chain,
                             const uint64_t *__restrict__ htoff, uint32_t k, uint32_t salt, uint32_t shift,
                             uint32_t n_slots, char *__restrict__ text)
@@ -1522,9 +1534,9 @@ __global__ void k_rand_emit(const uint4 *__restrict__ rec, const char *__restric
     uint64_t p = htoff[a] + (k - 1) + d;
     uint32_t v = r0, steps = 0;
     do {
-        const uint4 r = rec[v];
-        p = copy_run(rch, r, 0, text, p);
-        v = r.x;
+        const uint64_t r = rec[v];
+        p = copy_run(rch, r, v, 0, text, p);
+        v = rec_next(r);
     } while (v != NONE && !sampled(v, salt, shift) && ++steps < n_slots);
     if (v == NONE) text[p] = '\n';
 }
@@ -1532,7 +1544,8 @@ __global__ void k_rand_emit(const uint4 *__restrict__ rec, const char *__restric
 // text offset of each set's first unitig: the prefix at the set's first slot
 __global__ void k_set_offsets(const uint32_t *__restrict__ boff, uint32_t n_sets, uint32_t B,
                               const uint32_t *__restrict__ hasp, const uint8_t *__restrict__ flags,
-                              const uint32_t *__restrict__ ulen, const uint64_t *__restrict__ tile_off, uint32_t m,
+                              const uint64_t *__restrict__ seg, const uint32_t *__restrict__ ulen,
+                              const uint64_t *__restrict__ tile_off, uint32_t m,
                               uint64_t total, uint64_t *__restrict__ out)
 {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1543,7 +1556,7 @@ __global__ void k_set_offsets(const uint32_t *__restrict__ boff, uint32_t n_sets
         const uint32_t t = sb / CT_TILE;
         acc = tile_off[t];
         for (uint32_t x = t * CT_TILE; x < sb; x++)
-            if (live_head(flags, hasp, x)) acc += (1ull << 32) | ulen[x];
+            if (live_head(flags, hasp, x)) acc += (1ull << 32) | unitig_len(seg, ulen, x);
     }
     out[s] = acc;
 }
@@ -1712,7 +1725,7 @@ struct uc_ctx {
     const uint64_t *kin = nullptr;  // k-mers in 64-bit layout (the caller's buffer or `in`)
     DBuf otable, in, stage, bkt, tiles, hist, xhist, kmer2, flags2, src2, sub, perm, tiles2, hist2, cbhb, cbts,
         cbfirst, fboff, egroups, fbbase, scan_tmp, kmer, flags, src, pair_h, pair_b, boff, sethb, setts, etiles,
-        toff, tcap, table, hit, link, hasp, pred, seglen, segnext, prevseg, chainA, chainB, cycB, list, clist, ulen, tsum,
+        toff, tcap, table, hit, link, hasp, pred, seg, prevseg, chainA, chainB, cycB, list, clist, ulen, tsum,
         htoff, setoff, text, small, txt, txtcnt, rec, rch, lhead;
     // pinned host chunks of the file reader and writer
     char *pin[2] = {nullptr, nullptr};
@@ -1742,7 +1755,7 @@ static std::vector<DBuf *> all_bufs(uc_ctx *c)
             &c->in,     &c->stage,   &c->bkt,    &c->tiles,   &c->hist,    &c->xhist,  &c->scan_tmp,
             &c->kmer,   &c->flags,   &c->src,    &c->pair_h,  &c->pair_b,  &c->boff,   &c->sethb,
             &c->setts,  &c->etiles,  &c->toff,   &c->tcap,    &c->table,   &c->hit,    &c->link,
-            &c->hasp,   &c->pred,    &c->seglen, &c->segnext, &c->prevseg, &c->chainA, &c->chainB,
+            &c->hasp,   &c->pred,    &c->seg,    &c->prevseg, &c->chainA, &c->chainB,
             &c->cycB,   &c->list,    &c->clist,  &c->ulen,    &c->tsum,    &c->htoff,  &c->setoff,
             &c->text,   &c->small,   &c->txt,    &c->txtcnt,  &c->rec,     &c->rch,    &c->lhead};
 }
@@ -1814,7 +1827,7 @@ T *P(DBuf &b)
 
 static RunOut run_out(uc_ctx *c)
 {
-    return RunOut{P<uint4>(c->rec), P<char>(c->rch), P<uint32_t>(c->lhead), P<uint8_t>(c->flags)};
+    return RunOut{P<uint64_t>(c->rec), P<char>(c->rch), P<uint32_t>(c->lhead), P<uint8_t>(c->flags)};
 }
 
 // Times one kernel launch with a pair of events on the context's stream
@@ -2580,7 +2593,7 @@ int prepare_edges(uc_ctx *c)
     CHK(ensure(c, c->hit, c->m * 4));
     CHK(ensure(c, c->link, c->m * 8));
     CHK(ensure(c, c->hasp, (c->m + 31) / 32 * 4 + 8));
-    CHK(ensure(c, c->rec, c->m * 16 + 16));
+    CHK(ensure(c, c->rec, c->m * 8 + 16));
     CHK(ensure(c, c->rch, c->m + 16));
     CHK(ensure(c, c->lhead, c->m * 4 + 16));
     // has-predecessor bits: every bucket sets only its own slots' bits
@@ -2662,10 +2675,9 @@ int contract_pass(uc_ctx *c, uint64_t *live, uint32_t *nonconv)
     const uint64_t T = (c->m + CT_TILE - 1) / CT_TILE;
     uint32_t *cnt = P<uint32_t>(c->small);
     CHK(fill<uint32_t>(c, cnt, 8, 0u));
-    KLAUNCH(c, "k_seg_count", k_seg_count, (xcd_grid(T), BLOCK, 0), P<uint4>(c->rec), P<uint32_t>(c->hasp),
-            P<uint8_t>(c->flags), m, c->salt, c->shift, P<uint32_t>(c->seglen), P<uint32_t>(c->segnext),
-            P<uint32_t>(c->prevseg), P<uint32_t>(c->ulen), P<uint32_t>(c->list), cnt, P<uint64_t>(c->tsum),
-            tile_arg(c, T));
+    KLAUNCH(c, "k_seg_count", k_seg_count, (xcd_grid(T), BLOCK, 0), P<uint64_t>(c->rec), P<uint32_t>(c->hasp),
+            P<uint8_t>(c->flags), m, c->salt, c->shift, P<uint64_t>(c->seg), P<uint32_t>(c->prevseg),
+            P<uint32_t>(c->list), cnt, P<uint64_t>(c->tsum), tile_arg(c, T));
     uint32_t h[8];
     CHK(d2h(c, h, cnt, 32));
     if (h[C_ERR]) return fail(c, UC_E_STATE, "successor array corrupted: a segment walk exceeded m steps");
@@ -2677,7 +2689,7 @@ int contract_pass(uc_ctx *c, uint64_t *live, uint32_t *nonconv)
     uint32_t rounds = 0, bad = 0;
     if (nl) {
         KLAUNCH(c, "k_chain_init", k_chain_init, (grid_for(nl), BLOCK, 0), P<uint32_t>(c->list), nl,
-                P<uint32_t>(c->prevseg), P<uint32_t>(c->seglen), ca);
+                P<uint32_t>(c->prevseg), P<uint64_t>(c->seg), ca);
         // a chain holds at most nl segments: ceil(log2(nl)) + 1 rounds reach
         // its head; chains still open after that run around a cycle
         uint32_t cap = 1;
@@ -2694,7 +2706,7 @@ int contract_pass(uc_ctx *c, uint64_t *live, uint32_t *nonconv)
             if (active == 0 || rounds >= cap) break;
         }
         KLAUNCH(c, "k_chain_tail", k_chain_tail, (grid_for(nl), BLOCK, 0), P<uint32_t>(c->list), nl,
-                P<uint32_t>(c->hasp), ca, P<uint32_t>(c->seglen), P<uint32_t>(c->segnext), P<uint32_t>(c->ulen),
+                P<uint32_t>(c->hasp), ca, P<uint64_t>(c->seg), P<uint32_t>(c->ulen),
                 P<uint64_t>(c->tsum), cnt + C_NONCONV);
         CHK(d2h(c, &bad, cnt + C_NONCONV, 4));
     }
@@ -2717,7 +2729,7 @@ int cut_cycles(uc_ctx *c, uint64_t *n_cut)
 {
     const uint32_t m = (uint32_t)c->m;
     uint32_t *cnt = P<uint32_t>(c->small);
-    KLAUNCH(c, "k_cover", k_cover, (grid_for(m), BLOCK, 0), P<uint4>(c->rec), P<uint32_t>(c->hasp),
+    KLAUNCH(c, "k_cover", k_cover, (grid_for(m), BLOCK, 0), P<uint64_t>(c->rec), P<uint32_t>(c->hasp),
             P<uint8_t>(c->flags), c->chain, m, c->salt, c->shift);
     CHK(fill<uint32_t>(c, cnt + C_CYC, 2, 0u));
     uint64_t *la = P<uint64_t>(c->chainA) == c->chain ? P<uint64_t>(c->chainB) : P<uint64_t>(c->chainA);
@@ -2737,7 +2749,7 @@ int cut_cycles(uc_ctx *c, uint64_t *n_cut)
         KLAUNCH(c, "k_cycle_pred", k_cycle_pred, (grid_for(nl), BLOCK, 0), P<uint32_t>(c->clist), nl,
                 P<uint64_t>(c->link), P<uint32_t>(c->pred));
         KLAUNCH(c, "k_cycle_cut", k_cycle_cut, (grid_for(nl), BLOCK, 0), P<uint32_t>(c->clist), nl, la,
-                P<uint32_t>(c->pred), P<uint64_t>(c->link), P<uint4>(c->rec), P<uint32_t>(c->hasp),
+                P<uint32_t>(c->pred), P<uint64_t>(c->link), P<uint64_t>(c->rec), P<uint32_t>(c->hasp),
                 P<uint8_t>(c->flags), cnt + C_CUT);
     }
     KLAUNCH(c, "k_clear_flag", k_clear_flag, (std::min<uint32_t>(grid_for(m), 65536), BLOCK, 0), P<uint8_t>(c->flags),
@@ -2755,8 +2767,7 @@ int do_coarsen(uc_ctx *c)
     const uint32_t m = (uint32_t)c->m;
     const uint64_t T = (c->m + CT_TILE - 1) / CT_TILE;
     CHK(ensure(c, c->small, 256));
-    CHK(ensure(c, c->seglen, c->m * 4));
-    CHK(ensure(c, c->segnext, c->m * 4));
+    CHK(ensure(c, c->seg, c->m * 8));
     CHK(ensure(c, c->prevseg, c->m * 4));
     CHK(ensure(c, c->ulen, c->m * 4));
     CHK(ensure(c, c->chainA, c->m * 8));
@@ -2771,7 +2782,7 @@ int do_coarsen(uc_ctx *c)
     c->shift = c->opts.sample_log2 ? c->opts.sample_log2 : DEFAULT_SAMPLE_LOG2;
     if (c->x) {
         KLAUNCH(c, "k_splice", k_splice, (grid_for(c->x), BLOCK, 0), P<uint32_t>(c->pair_h), P<uint32_t>(c->pair_b),
-                (uint32_t)c->x, P<uint64_t>(c->link), P<uint32_t>(c->lhead), P<uint4>(c->rec));
+                (uint32_t)c->x, P<uint64_t>(c->link), P<uint32_t>(c->lhead), P<uint64_t>(c->rec));
     }
     uint64_t live = 0;
     uint32_t nonconv = 0;
@@ -2801,13 +2812,13 @@ int run_emit(uc_ctx *c)
     CHK(ensure(c, c->text, c->text_bytes + 1));
     CHK(ensure(c, c->htoff, c->m * 8));
     if (m) {
-        KLAUNCH(c, "k_head_emit", k_head_emit<W>, (xcd_grid(T), BLOCK, 0), P<uint64_t>(c->kmer), P<uint4>(c->rec),
-                P<char>(c->rch), P<uint32_t>(c->hasp), P<uint8_t>(c->flags), P<uint32_t>(c->segnext),
+        KLAUNCH(c, "k_head_emit", k_head_emit<W>, (xcd_grid(T), BLOCK, 0), P<uint64_t>(c->kmer), P<uint64_t>(c->rec),
+                P<char>(c->rch), P<uint32_t>(c->hasp), P<uint8_t>(c->flags), P<uint64_t>(c->seg),
                 P<uint32_t>(c->ulen), P<uint64_t>(c->tsum), m, c->k, c->salt, c->shift, P<uint64_t>(c->htoff),
                 P<char>(c->text), tile_arg(c, T));
     }
     if (c->n_list) {
-        KLAUNCH(c, "k_rand_emit", k_rand_emit, (grid_for(c->n_list), BLOCK, 0), P<uint4>(c->rec), P<char>(c->rch),
+        KLAUNCH(c, "k_rand_emit", k_rand_emit, (grid_for(c->n_list), BLOCK, 0), P<uint64_t>(c->rec), P<char>(c->rch),
                 P<uint32_t>(c->list), c->n_list, c->chain, P<uint64_t>(c->htoff), c->k, c->salt, c->shift, m,
                 P<char>(c->text));
     }
@@ -2815,7 +2826,8 @@ int run_emit(uc_ctx *c)
     const uint64_t total = (c->n_unitigs << 32) | (c->text_bytes - c->n_unitigs * c->k);
     CHK(ensure(c, c->setoff, (c->n_sets + 1) * 8));
     KLAUNCH(c, "k_set_offsets", k_set_offsets, (grid_for(c->n_sets + 1), BLOCK, 0), P<uint32_t>(c->boff), c->n_sets,
-            c->B, P<uint32_t>(c->hasp), P<uint8_t>(c->flags), P<uint32_t>(c->ulen), P<uint64_t>(c->tsum), m, total,
+            c->B, P<uint32_t>(c->hasp), P<uint8_t>(c->flags), P<uint64_t>(c->seg), P<uint32_t>(c->ulen),
+            P<uint64_t>(c->tsum), m, total,
             P<uint64_t>(c->setoff));
     std::vector<uint64_t> so(c->n_sets + 1);
     CHK(d2h(c, so.data(), c->setoff.p, (c->n_sets + 1) * 8));
@@ -3296,7 +3308,7 @@ uint64_t uc_job_bytes_estimate(uint64_t n, uint32_t k)
     // characters, run head of lent k-mers
     const uint64_t W = k <= 32 ? 1 : 2;
     const uint64_t per_kmer = 8 * W + 8 + (k + 1);
-    const uint64_t per_slot = 16 * W + 2 + 2 + 4 + 8 + 4 + 1 + 16 + 16 + 4 + 8 + 16 + 1 + 4;
+    const uint64_t per_slot = 16 * W + 2 + 2 + 4 + 8 + 4 + 1 + 16 + 16 + 4 + 8 + 8 + 1 + 4;
     return n * per_kmer + (n + n / 4) * per_slot + (64ull << 20);
 }
 
