@@ -1193,6 +1193,15 @@ __device__ __forceinline__ uint32_t wave_append(bool flag, uint32_t *counter)
 // counters in the context's small buffer
 enum { C_LIST = 0, C_ERR = 1, C_ACTIVE = 2, C_NONCONV = 3, C_CYC = 4, C_CUT = 5, C_SPLIT = 6, C_RUNS = 7 };
 
+// flags and has-predecessor bits of the 8 slots s0 .. s0+7 (s0 a multiple of
+// 8; the flag array has 64 bytes of slack past its end)
+__device__ __forceinline__ void slot8(const uint8_t *__restrict__ flags, const uint32_t *__restrict__ hasp, uint64_t s0,
+                                      uint64_t &f8, uint32_t &hp8)
+{
+    f8 = *reinterpret_cast<const uint64_t *>(flags + s0);
+    hp8 = (hasp[s0 >> 5] >> (s0 & 31)) & 0xFFu;
+}
+
 __device__ __forceinline__ bool live_head(const uint8_t *__restrict__ flags, const uint32_t *__restrict__ hasp,
                                           uint32_t s)
 {
@@ -1212,22 +1221,28 @@ __global__ __launch_bounds__(BLOCK) void k_seg_count(const uint64_t *__restrict_
     __shared__ uint64_t red[WAVES];
     const uint32_t ti = xcd_tile(n_tiles);
     if (ti == NONE) return;
-    const uint64_t base = (uint64_t)ti * CT_TILE;
-    // splitters of this tile; sampled ones get list slots with one atomic
-    // per workgroup
+    // this thread's 8 consecutive slots: their flags and has-predecessor bits
+    // in two loads
+    const uint64_t s0 = (uint64_t)ti * CT_TILE + (uint64_t)threadIdx.x * CT_ITEMS;
+    uint64_t f8 = 0;
+    uint32_t hp8 = 0;
+    if (s0 < m) slot8(flags, hasp, s0, f8, hp8);
+    // splitters; sampled ones get list slots with one atomic per workgroup
     bool hd[CT_ITEMS], rd[CT_ITEMS];
     uint32_t nr = 0, nruns = 0;
 #pragma unroll
     for (int j = 0; j < CT_ITEMS; j++) {
-        const uint64_t s64 = base + (uint64_t)j * BLOCK + threadIdx.x;
-        hd[j] = rd[j] = false;
-        if (s64 < m && (flags[s64] & (F_BORROW | F_RUNHEAD)) == F_RUNHEAD) {
-            nruns++;
-            hd[j] = !has_pred(hasp, (uint32_t)s64);
-            rd[j] = !hd[j] && sampled((uint32_t)s64, salt, shift);
-        }
+        const uint64_t s = s0 + j;
+        const bool cand = s < m && ((uint32_t)(f8 >> (8 * j)) & (F_BORROW | F_RUNHEAD)) == F_RUNHEAD;
+        hd[j] = cand && !((hp8 >> j) & 1u);
+        rd[j] = cand && !hd[j] && sampled((uint32_t)s, salt, shift);
+        nruns += cand;
         nr += rd[j];
     }
+    // every splitter's first run record in flight at once
+    uint64_t first[CT_ITEMS];
+#pragma unroll
+    for (int j = 0; j < CT_ITEMS; j++) first[j] = (hd[j] || rd[j]) ? rec[s0 + j] : 0ull;
     __shared__ uint32_t wtot[WAVES];
     __shared__ uint32_t bbase;
     const uint32_t incl = wave_incl_scan(nr);
@@ -1245,31 +1260,29 @@ __global__ __launch_bounds__(BLOCK) void k_seg_count(const uint64_t *__restrict_
     uint64_t acc = 0;
     uint32_t nsplit = 0;
     for (int j = 0; j < CT_ITEMS; j++) {
-        const uint32_t s = (uint32_t)(base + (uint64_t)j * BLOCK + threadIdx.x);
-        const bool head = hd[j], rnd = rd[j];
-        if (head || rnd) {
-            nsplit++;
-            uint32_t v = s, r = 0, steps = 0;
-            do {
-                const uint64_t rr = rec[v];
-                r += rec_len(rr);
-                v = rec_next(rr);
-                // in-degree <= 1 bounds every walk by m runs; the cap only
-                // guards the GPU against corrupted run records
-                if (++steps > m) {
-                    atomicOr(cnt + C_ERR, 1u);
-                    v = NONE;
-                    break;
-                }
-            } while (v != NONE && !sampled(v, salt, shift));
-            seg[s] = (uint64_t)r | ((uint64_t)v << 32);
-            if (v != NONE) prevseg[v] = s;
-            if (head) {
-                acc += 1ull << 32;
-                if (v == NONE) acc += r;
+        if (!(hd[j] || rd[j])) continue;
+        const uint32_t s = (uint32_t)(s0 + j);
+        nsplit++;
+        uint32_t r = rec_len(first[j]), v = rec_next(first[j]), steps = 1;
+        while (v != NONE && !sampled(v, salt, shift)) {
+            const uint64_t rr = rec[v];
+            r += rec_len(rr);
+            v = rec_next(rr);
+            // in-degree <= 1 bounds every walk by m runs; the cap only guards
+            // the GPU against corrupted run records
+            if (++steps > m) {
+                atomicOr(cnt + C_ERR, 1u);
+                v = NONE;
+                break;
             }
-            if (rnd) list[li++] = s;
         }
+        seg[s] = (uint64_t)r | ((uint64_t)v << 32);
+        if (v != NONE) prevseg[v] = s;
+        if (hd[j]) {
+            acc += 1ull << 32;
+            if (v == NONE) acc += r;
+        }
+        if (rd[j]) list[li++] = s;
     }
     acc = block_reduce(acc, red);
     if (threadIdx.x == 0) tile_sum[ti] = acc;
@@ -1468,25 +1481,31 @@ __global__ __launch_bounds__(BLOCK) void k_head_emit(const uint64_t *__restrict_
                                                      uint32_t salt, uint32_t shift, uint64_t *__restrict__ htoff,
                                                      char *__restrict__ text, uint32_t n_tiles)
 {
-    __shared__ uint64_t tile[CT_TILE];
     __shared__ uint64_t wsum[WAVES];
     const uint32_t ti = xcd_tile(n_tiles);
     if (ti == NONE) return;
-    const uint64_t t0 = (uint64_t)ti * CT_TILE;
+    // this thread's 8 consecutive slots
+    const uint64_t s0 = (uint64_t)ti * CT_TILE + (uint64_t)threadIdx.x * CT_ITEMS;
+    uint64_t f8 = 0;
+    uint32_t hp8 = 0;
+    if (s0 < m) slot8(flags, hasp, s0, f8, hp8);
     bool hd[CT_ITEMS];
+    uint64_t sg[CT_ITEMS];
 #pragma unroll
     for (int j = 0; j < CT_ITEMS; j++) {
-        const uint64_t s = t0 + (uint64_t)j * BLOCK + threadIdx.x;
-        hd[j] = s < m && live_head(flags, hasp, (uint32_t)s);
-        tile[j * BLOCK + threadIdx.x] = hd[j] ? ((1ull << 32) | unitig_len(seg, ulen, (uint32_t)s)) : 0ull;
+        hd[j] = s0 + j < m && !((uint32_t)(f8 >> (8 * j)) & F_BORROW) && !((hp8 >> j) & 1u);
+        sg[j] = hd[j] ? seg[s0 + j] : 0ull;
     }
-    __syncthreads();
+    // (1 << 32 | unitig length) per head: exclusive prefix over the tile
     uint64_t loc[CT_ITEMS];
     uint64_t run = 0;
 #pragma unroll
     for (int j = 0; j < CT_ITEMS; j++) {
         loc[j] = run;
-        run += tile[threadIdx.x * CT_ITEMS + j];
+        if (hd[j]) {
+            const uint32_t len = (uint32_t)(sg[j] >> 32) == NONE ? (uint32_t)sg[j] : ulen[s0 + j];
+            run += (1ull << 32) | len;
+        }
     }
     const uint64_t incl = wave_incl_scan(run);
     const int w = threadIdx.x / 64;
@@ -1495,16 +1514,12 @@ __global__ __launch_bounds__(BLOCK) void k_head_emit(const uint64_t *__restrict_
     uint64_t wpre = 0;
     for (int i = 0; i < w; i++) wpre += wsum[i];
     const uint64_t excl = tile_off[ti] + wpre + incl - run;
-#pragma unroll
-    for (int j = 0; j < CT_ITEMS; j++) tile[threadIdx.x * CT_ITEMS + j] = excl + loc[j];
-    __syncthreads();
-#pragma unroll
     for (int j = 0; j < CT_ITEMS; j++) {
         if (!hd[j]) continue;
-        const uint32_t s = (uint32_t)(t0 + (uint64_t)j * BLOCK + threadIdx.x);
-        const uint64_t pre = tile[j * BLOCK + threadIdx.x];
+        const uint32_t s = (uint32_t)(s0 + j);
+        const uint64_t pre = excl + loc[j];
         const uint64_t off = (pre & 0xFFFFFFFFull) + (pre >> 32) * k;
-        if ((uint32_t)(seg[s] >> 32) != NONE) htoff[s] = off;
+        if ((uint32_t)(sg[j] >> 32) != NONE) htoff[s] = off;
         char *o = text + off;
         const Kmer<W> x0 = load_kmer<W>(kmer, s, ~0ull);
         for (uint32_t i = 0; i < k; i++) o[i] = base_char(char_at<W>(x0, i));
