@@ -109,8 +109,10 @@ __device__ __forceinline__ Kmer<W> suffix_key(const Kmer<W> &x)
 template <int W>
 __device__ __forceinline__ bool key_eq(const Kmer<W> &a, const Kmer<W> &b)
 {
-    if constexpr (W == 1) return a.w[0] == b.w[0];
-    return a.w[0] == b.w[0] && a.w[1] == b.w[1];
+    if constexpr (W == 1)
+        return a.w[0] == b.w[0];
+    else
+        return a.w[0] == b.w[0] && a.w[1] == b.w[1];
 }
 
 // table position hash and an independent 28-bit tag
@@ -140,10 +142,13 @@ __device__ __forceinline__ bool lex_less(const Kmer<W> &a, const Kmer<W> &b)
 {
     // both values sit in the low 2k bits; reversing all 64*W bits puts
     // character 0 in the top bits of both, so numeric order is lexicographic
-    if constexpr (W == 1) return rev_chars64(a.w[0]) < rev_chars64(b.w[0]);
-    const uint64_t ah = rev_chars64(a.w[0]), bh = rev_chars64(b.w[0]);
-    if (ah != bh) return ah < bh;
-    return rev_chars64(a.w[1]) < rev_chars64(b.w[1]);
+    if constexpr (W == 1) {
+        return rev_chars64(a.w[0]) < rev_chars64(b.w[0]);
+    } else {
+        const uint64_t ah = rev_chars64(a.w[0]), bh = rev_chars64(b.w[0]);
+        if (ah != bh) return ah < bh;
+        return rev_chars64(a.w[1]) < rev_chars64(b.w[1]);
+    }
 }
 
 // seeded bijection on 2l-bit l-mer codes (same function as the oracle's
@@ -1191,16 +1196,9 @@ __device__ __forceinline__ uint32_t wave_append(bool flag, uint32_t *counter)
 }
 
 // counters in the context's small buffer
-enum { C_LIST = 0, C_ERR = 1, C_ACTIVE = 2, C_NONCONV = 3, C_CYC = 4, C_CUT = 5, C_SPLIT = 6, C_RUNS = 7 };
-
-// flags and has-predecessor bits of the 8 slots s0 .. s0+7 (s0 a multiple of
-// 8; the flag array has 64 bytes of slack past its end)
-__device__ __forceinline__ void slot8(const uint8_t *__restrict__ flags, const uint32_t *__restrict__ hasp, uint64_t s0,
-                                      uint64_t &f8, uint32_t &hp8)
-{
-    f8 = *reinterpret_cast<const uint64_t *>(flags + s0);
-    hp8 = (hasp[s0 >> 5] >> (s0 & 31)) & 0xFFu;
-}
+// (uint32 indices: one 128-byte line each, so that atomics on different
+// counters never share a line; C_TOT holds two uint64 totals)
+enum { C_LIST = 0, C_ERR = 32, C_ACTIVE = 64, C_NONCONV = 96, C_CYC = 128, C_CUT = 160, C_TOT = 192, C_WORDS = 224 };
 
 __device__ __forceinline__ bool live_head(const uint8_t *__restrict__ flags, const uint32_t *__restrict__ hasp,
                                           uint32_t s)
@@ -1216,33 +1214,39 @@ __global__ __launch_bounds__(BLOCK) void k_seg_count(const uint64_t *__restrict_
                                                      uint32_t shift, uint64_t *__restrict__ seg,
                                                      uint32_t *__restrict__ prevseg, uint32_t *__restrict__ list,
                                                      uint32_t *__restrict__ cnt, uint64_t *__restrict__ tile_sum,
-                                                     uint32_t n_tiles)
+                                                     uint64_t *__restrict__ tile_cnt, uint32_t n_tiles)
 {
     __shared__ uint64_t red[WAVES];
     const uint32_t ti = xcd_tile(n_tiles);
     if (ti == NONE) return;
-    // this thread's 8 consecutive slots: their flags and has-predecessor bits
-    // in two loads
-    const uint64_t s0 = (uint64_t)ti * CT_TILE + (uint64_t)threadIdx.x * CT_ITEMS;
-    uint64_t f8 = 0;
-    uint32_t hp8 = 0;
-    if (s0 < m) slot8(flags, hasp, s0, f8, hp8);
+    const uint64_t base = (uint64_t)ti * CT_TILE;
+    // flags and has-predecessor words of this thread's slots, all in flight
+    // at once (the flag array has slack past its end, so no load is guarded)
+    uint8_t f[CT_ITEMS];
+    uint32_t hw[CT_ITEMS];
+#pragma unroll
+    for (int j = 0; j < CT_ITEMS; j++) {
+        const uint64_t s = base + (uint64_t)j * BLOCK + threadIdx.x;
+        f[j] = s < m ? flags[s] : (uint8_t)F_BORROW;
+        hw[j] = hasp[(s < m ? s : 0) >> 5];
+    }
     // splitters; sampled ones get list slots with one atomic per workgroup
     bool hd[CT_ITEMS], rd[CT_ITEMS];
     uint32_t nr = 0, nruns = 0;
 #pragma unroll
     for (int j = 0; j < CT_ITEMS; j++) {
-        const uint64_t s = s0 + j;
-        const bool cand = s < m && ((uint32_t)(f8 >> (8 * j)) & (F_BORROW | F_RUNHEAD)) == F_RUNHEAD;
-        hd[j] = cand && !((hp8 >> j) & 1u);
-        rd[j] = cand && !hd[j] && sampled((uint32_t)s, salt, shift);
+        const uint32_t s = (uint32_t)(base + (uint64_t)j * BLOCK + threadIdx.x);
+        const bool cand = (f[j] & (F_BORROW | F_RUNHEAD)) == F_RUNHEAD;
+        hd[j] = cand && !((hw[j] >> (s & 31)) & 1u);
+        rd[j] = cand && !hd[j] && sampled(s, salt, shift);
         nruns += cand;
         nr += rd[j];
     }
     // every splitter's first run record in flight at once
     uint64_t first[CT_ITEMS];
 #pragma unroll
-    for (int j = 0; j < CT_ITEMS; j++) first[j] = (hd[j] || rd[j]) ? rec[s0 + j] : 0ull;
+    for (int j = 0; j < CT_ITEMS; j++)
+        first[j] = (hd[j] || rd[j]) ? rec[base + (uint64_t)j * BLOCK + threadIdx.x] : 0ull;
     __shared__ uint32_t wtot[WAVES];
     __shared__ uint32_t bbase;
     const uint32_t incl = wave_incl_scan(nr);
@@ -1261,7 +1265,7 @@ __global__ __launch_bounds__(BLOCK) void k_seg_count(const uint64_t *__restrict_
     uint32_t nsplit = 0;
     for (int j = 0; j < CT_ITEMS; j++) {
         if (!(hd[j] || rd[j])) continue;
-        const uint32_t s = (uint32_t)(s0 + j);
+        const uint32_t s = (uint32_t)(base + (uint64_t)j * BLOCK + threadIdx.x);
         nsplit++;
         uint32_t r = rec_len(first[j]), v = rec_next(first[j]), steps = 1;
         while (v != NONE && !sampled(v, salt, shift)) {
@@ -1284,14 +1288,14 @@ __global__ __launch_bounds__(BLOCK) void k_seg_count(const uint64_t *__restrict_
         }
         if (rd[j]) list[li++] = s;
     }
+    // per-tile sums, no global atomics: (heads << 32 | head-segment lengths)
+    // and (splitters << 32 | runs)
     acc = block_reduce(acc, red);
-    if (threadIdx.x == 0) tile_sum[ti] = acc;
-    for (int o = 32; o > 0; o >>= 1) {
-        nsplit += __shfl_xor(nsplit, o, 64);
-        nruns += __shfl_xor(nruns, o, 64);
+    const uint64_t sr = block_reduce(((uint64_t)nsplit << 32) | nruns, red);
+    if (threadIdx.x == 0) {
+        tile_sum[ti] = acc;
+        tile_cnt[ti] = sr;
     }
-    if (__lane_id() == 0 && nsplit) atomicAdd(cnt + C_SPLIT, nsplit);
-    if (__lane_id() == 0 && nruns) atomicAdd(cnt + C_RUNS, nruns);
 }
 
 // sampled splitters: (preceding segment, its length)
@@ -1481,31 +1485,25 @@ __global__ __launch_bounds__(BLOCK) void k_head_emit(const uint64_t *__restrict_
                                                      uint32_t salt, uint32_t shift, uint64_t *__restrict__ htoff,
                                                      char *__restrict__ text, uint32_t n_tiles)
 {
+    __shared__ uint64_t tile[CT_TILE];
     __shared__ uint64_t wsum[WAVES];
     const uint32_t ti = xcd_tile(n_tiles);
     if (ti == NONE) return;
-    // this thread's 8 consecutive slots
-    const uint64_t s0 = (uint64_t)ti * CT_TILE + (uint64_t)threadIdx.x * CT_ITEMS;
-    uint64_t f8 = 0;
-    uint32_t hp8 = 0;
-    if (s0 < m) slot8(flags, hasp, s0, f8, hp8);
+    const uint64_t t0 = (uint64_t)ti * CT_TILE;
     bool hd[CT_ITEMS];
-    uint64_t sg[CT_ITEMS];
 #pragma unroll
     for (int j = 0; j < CT_ITEMS; j++) {
-        hd[j] = s0 + j < m && !((uint32_t)(f8 >> (8 * j)) & F_BORROW) && !((hp8 >> j) & 1u);
-        sg[j] = hd[j] ? seg[s0 + j] : 0ull;
+        const uint64_t s = t0 + (uint64_t)j * BLOCK + threadIdx.x;
+        hd[j] = s < m && live_head(flags, hasp, (uint32_t)s);
+        tile[j * BLOCK + threadIdx.x] = hd[j] ? ((1ull << 32) | unitig_len(seg, ulen, (uint32_t)s)) : 0ull;
     }
-    // (1 << 32 | unitig length) per head: exclusive prefix over the tile
+    __syncthreads();
     uint64_t loc[CT_ITEMS];
     uint64_t run = 0;
 #pragma unroll
     for (int j = 0; j < CT_ITEMS; j++) {
         loc[j] = run;
-        if (hd[j]) {
-            const uint32_t len = (uint32_t)(sg[j] >> 32) == NONE ? (uint32_t)sg[j] : ulen[s0 + j];
-            run += (1ull << 32) | len;
-        }
+        run += tile[threadIdx.x * CT_ITEMS + j];
     }
     const uint64_t incl = wave_incl_scan(run);
     const int w = threadIdx.x / 64;
@@ -1514,12 +1512,16 @@ __global__ __launch_bounds__(BLOCK) void k_head_emit(const uint64_t *__restrict_
     uint64_t wpre = 0;
     for (int i = 0; i < w; i++) wpre += wsum[i];
     const uint64_t excl = tile_off[ti] + wpre + incl - run;
+#pragma unroll
+    for (int j = 0; j < CT_ITEMS; j++) tile[threadIdx.x * CT_ITEMS + j] = excl + loc[j];
+    __syncthreads();
+#pragma unroll
     for (int j = 0; j < CT_ITEMS; j++) {
         if (!hd[j]) continue;
-        const uint32_t s = (uint32_t)(s0 + j);
-        const uint64_t pre = excl + loc[j];
+        const uint32_t s = (uint32_t)(t0 + (uint64_t)j * BLOCK + threadIdx.x);
+        const uint64_t pre = tile[j * BLOCK + threadIdx.x];
         const uint64_t off = (pre & 0xFFFFFFFFull) + (pre >> 32) * k;
-        if ((uint32_t)(sg[j] >> 32) != NONE) htoff[s] = off;
+        if ((uint32_t)(seg[s] >> 32) != NONE) htoff[s] = off;
         char *o = text + off;
         const Kmer<W> x0 = load_kmer<W>(kmer, s, ~0ull);
         for (uint32_t i = 0; i < k; i++) o[i] = base_char(char_at<W>(x0, i));
@@ -1741,7 +1743,7 @@ struct uc_ctx {
     DBuf otable, in, stage, bkt, tiles, hist, xhist, kmer2, flags2, src2, sub, perm, tiles2, hist2, cbhb, cbts,
         cbfirst, fboff, egroups, fbbase, scan_tmp, kmer, flags, src, pair_h, pair_b, boff, sethb, setts, etiles,
         toff, tcap, table, hit, link, hasp, pred, seg, prevseg, chainA, chainB, cycB, list, clist, ulen, tsum,
-        htoff, setoff, text, small, txt, txtcnt, rec, rch, lhead;
+        htoff, setoff, text, small, txt, txtcnt, rec, rch, lhead, tcnt;
     // pinned host chunks of the file reader and writer
     char *pin[2] = {nullptr, nullptr};
     hipEvent_t pin_ev[2] = {nullptr, nullptr};
@@ -1772,7 +1774,8 @@ static std::vector<DBuf *> all_bufs(uc_ctx *c)
             &c->setts,  &c->etiles,  &c->toff,   &c->tcap,    &c->table,   &c->hit,    &c->link,
             &c->hasp,   &c->pred,    &c->seg,    &c->prevseg, &c->chainA, &c->chainB,
             &c->cycB,   &c->list,    &c->clist,  &c->ulen,    &c->tsum,    &c->htoff,  &c->setoff,
-            &c->text,   &c->small,   &c->txt,    &c->txtcnt,  &c->rec,     &c->rch,    &c->lhead};
+            &c->text,   &c->small,   &c->txt,    &c->txtcnt,  &c->rec,     &c->rch,    &c->lhead,
+            &c->tcnt};
 }
 
 static void free_pinned(uc_ctx *c)
@@ -2689,16 +2692,20 @@ int contract_pass(uc_ctx *c, uint64_t *live, uint32_t *nonconv)
     const uint32_t m = (uint32_t)c->m;
     const uint64_t T = (c->m + CT_TILE - 1) / CT_TILE;
     uint32_t *cnt = P<uint32_t>(c->small);
-    CHK(fill<uint32_t>(c, cnt, 8, 0u));
+    CHK(fill<uint32_t>(c, cnt, C_WORDS, 0u));
     KLAUNCH(c, "k_seg_count", k_seg_count, (xcd_grid(T), BLOCK, 0), P<uint64_t>(c->rec), P<uint32_t>(c->hasp),
             P<uint8_t>(c->flags), m, c->salt, c->shift, P<uint64_t>(c->seg), P<uint32_t>(c->prevseg),
-            P<uint32_t>(c->list), cnt, P<uint64_t>(c->tsum), tile_arg(c, T));
-    uint32_t h[8];
-    CHK(d2h(c, h, cnt, 32));
+            P<uint32_t>(c->list), cnt, P<uint64_t>(c->tsum), P<uint64_t>(c->tcnt), tile_arg(c, T));
+    CHK(scan_excl<uint64_t>(c, P<uint64_t>(c->tcnt), P<uint64_t>(c->tcnt), T,
+                            reinterpret_cast<uint64_t *>(cnt + C_TOT), c->scan_tmp));
+    uint32_t h[C_WORDS];
+    CHK(d2h(c, h, cnt, sizeof(h)));
     if (h[C_ERR]) return fail(c, UC_E_STATE, "successor array corrupted: a segment walk exceeded m steps");
     c->n_list = h[C_LIST];
-    c->n_splitters = h[C_SPLIT];
-    c->n_runs = h[C_RUNS];
+    uint64_t st;
+    std::memcpy(&st, h + C_TOT, 8);
+    c->n_splitters = st >> 32;
+    c->n_runs = st & 0xFFFFFFFFull;
     const uint32_t nl = c->n_list;
     uint64_t *ca = P<uint64_t>(c->chainA), *cb = P<uint64_t>(c->chainB);
     uint32_t rounds = 0, bad = 0;
@@ -2746,7 +2753,8 @@ int cut_cycles(uc_ctx *c, uint64_t *n_cut)
     uint32_t *cnt = P<uint32_t>(c->small);
     KLAUNCH(c, "k_cover", k_cover, (grid_for(m), BLOCK, 0), P<uint64_t>(c->rec), P<uint32_t>(c->hasp),
             P<uint8_t>(c->flags), c->chain, m, c->salt, c->shift);
-    CHK(fill<uint32_t>(c, cnt + C_CYC, 2, 0u));
+    CHK(fill<uint32_t>(c, cnt + C_CYC, 1, 0u));
+    CHK(fill<uint32_t>(c, cnt + C_CUT, 1, 0u));
     uint64_t *la = P<uint64_t>(c->chainA) == c->chain ? P<uint64_t>(c->chainB) : P<uint64_t>(c->chainA);
     uint64_t *lb = P<uint64_t>(c->cycB);
     KLAUNCH(c, "k_cycle_init", k_cycle_init, (grid_for(m), BLOCK, 0), P<uint64_t>(c->link), P<uint8_t>(c->flags), m,
@@ -2781,7 +2789,8 @@ int do_coarsen(uc_ctx *c)
     c->emitted = false;
     const uint32_t m = (uint32_t)c->m;
     const uint64_t T = (c->m + CT_TILE - 1) / CT_TILE;
-    CHK(ensure(c, c->small, 256));
+    CHK(ensure(c, c->small, C_WORDS * 4));
+    CHK(ensure(c, c->tcnt, (T + 1) * 8));
     CHK(ensure(c, c->seg, c->m * 8));
     CHK(ensure(c, c->prevseg, c->m * 4));
     CHK(ensure(c, c->ulen, c->m * 4));
