@@ -578,17 +578,149 @@ __global__ void k_gather_boff(const uint32_t *__restrict__ hscan, const uint64_t
 // ---- fine buckets: a stable counting sort of every coarse bucket's slots by
 // their fine index (the next bits of the same minimizer order)
 
+// Pass-2 tiles are planned on the device from the coarse rows: coarse
+// bucket g (S_g fine buckets, ts_g = max(1, ceil(rows / tile)) tiles) owns
+// tiles [tfirst_g, tfirst_g + ts_g) and histogram entries [hbase_g, hbase_g +
+// ts_g * S_g), tile-major (entry t * S_g + v) so that every tile writes its
+// counts contiguously.  Tiles past the last one are empty (fmask 0).
+
+// one workgroup: tiles and histogram entries per coarse bucket, prefixes
+__global__ __launch_bounds__(BLOCK) void k_plan2_prefix(const uint32_t *__restrict__ boff,
+                                                        const uint32_t *__restrict__ cb_first, uint32_t G, uint32_t tile,
+                                                        uint32_t *__restrict__ cb_ts, uint32_t *__restrict__ cb_tfirst,
+                                                        uint64_t *__restrict__ cb_hbase)
+{
+    __shared__ uint64_t wsum[WAVES], hsum[WAVES];
+    __shared__ uint64_t tcarry, hcarry;
+    if (threadIdx.x == 0) tcarry = hcarry = 0;
+    __syncthreads();
+    const int w = threadIdx.x / 64;
+    for (uint32_t g0 = 0; g0 < G; g0 += BLOCK) {
+        const uint32_t g = g0 + threadIdx.x;
+        uint64_t ts = 0, hs = 0;
+        if (g < G) {
+            const uint32_t rows = boff[g + 1] - boff[g];
+            ts = rows ? (rows + tile - 1) / tile : 1u;
+            hs = ts * (cb_first[g + 1] - cb_first[g]);
+        }
+        const uint64_t ti = wave_incl_scan(ts), hi = wave_incl_scan(hs);
+        if (__lane_id() == 63) {
+            wsum[w] = ti;
+            hsum[w] = hi;
+        }
+        __syncthreads();
+        uint64_t tp = tcarry + ti - ts, hp = hcarry + hi - hs, tt = 0, ht = 0;
+        for (int q = 0; q < WAVES; q++) {
+            if (q < w) {
+                tp += wsum[q];
+                hp += hsum[q];
+            }
+            tt += wsum[q];
+            ht += hsum[q];
+        }
+        if (g < G) {
+            cb_ts[g] = (uint32_t)ts;
+            cb_tfirst[g] = (uint32_t)tp;
+            cb_hbase[g] = hp;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            tcarry += tt;
+            hcarry += ht;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        cb_tfirst[G] = (uint32_t)tcarry;
+        cb_hbase[G] = hcarry;
+    }
+}
+
+// one thread per tile slot up to the upper bound
+__global__ void k_plan2_tiles(const uint32_t *__restrict__ boff, const uint32_t *__restrict__ cb_first,
+                              const uint32_t *__restrict__ cb_ts, const uint32_t *__restrict__ cb_tfirst,
+                              const uint64_t *__restrict__ cb_hbase, uint32_t G, uint32_t tile, uint32_t n_tiles,
+                              BTile *__restrict__ tiles)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_tiles) return;
+    BTile bt;
+    bt.pad = 0;
+    if (i >= cb_tfirst[G]) {
+        bt.kbegin = 0;
+        bt.hbase = 0;
+        bt.count = bt.set = bt.ts = bt.tlocal = bt.fmask = 0;
+        tiles[i] = bt;
+        return;
+    }
+    uint32_t lo = 0, hi = G;  // last g with cb_tfirst[g] <= i
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (cb_tfirst[mid] <= i) lo = mid; else hi = mid;
+    }
+    const uint32_t g = lo, t = i - cb_tfirst[g], rows = boff[g + 1] - boff[g];
+    bt.kbegin = (uint64_t)boff[g] + (uint64_t)t * tile;
+    bt.hbase = cb_hbase[g];
+    bt.count = rows > t * tile ? min(tile, rows - t * tile) : 0u;
+    bt.set = g;
+    bt.ts = cb_ts[g];
+    bt.tlocal = t;
+    bt.fmask = cb_first[g + 1] - cb_first[g];
+    tiles[i] = bt;
+}
+
 __global__ __launch_bounds__(BLOCK) void k_sub_count(const uint16_t *__restrict__ sub, const BTile *__restrict__ tiles,
                                                      uint32_t *__restrict__ hist)
 {
     extern __shared__ uint32_t cnt[];
     const BTile t = tiles[blockIdx.x];
     const uint32_t S = t.fmask;
+    if (S == 0) return;  // past the last tile
     for (uint32_t i = threadIdx.x; i < S; i += BLOCK) cnt[i] = 0;
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < t.count; i += BLOCK) atomicAdd(&cnt[sub[t.kbegin + i]], 1u);
     __syncthreads();
-    for (uint32_t v = threadIdx.x; v < S; v += BLOCK) hist[t.hbase + (uint64_t)v * t.ts + t.tlocal] = cnt[v];
+    for (uint32_t v = threadIdx.x; v < S; v += BLOCK) hist[t.hbase + (uint64_t)t.tlocal * S + v] = cnt[v];
+}
+
+// per coarse bucket: each fine bucket's counts over the tiles become
+// exclusive prefixes (in place), and the fine rows follow from the column
+// totals: fboff[first_g + v] = boff[g] + totals of fine buckets < v
+__global__ __launch_bounds__(BLOCK) void k_sub_prefix(uint32_t *__restrict__ hist, const uint64_t *__restrict__ cb_hbase,
+                                                      const uint32_t *__restrict__ cb_ts,
+                                                      const uint32_t *__restrict__ cb_first,
+                                                      const uint32_t *__restrict__ boff, uint32_t G, uint32_t m,
+                                                      uint32_t *__restrict__ fboff)
+{
+    __shared__ uint32_t tot[MAX_SUB];
+    __shared__ uint32_t wsum[WAVES];
+    const uint32_t g = blockIdx.x;
+    const uint32_t f0 = cb_first[g], S = cb_first[g + 1] - f0, ts = cb_ts[g];
+    const uint64_t hb = cb_hbase[g];
+    for (uint32_t v = threadIdx.x; v < S; v += BLOCK) {
+        uint32_t run = 0;
+        uint32_t t = 0;
+        for (; t + 4 <= ts; t += 4) {
+            uint32_t c[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) c[q] = hist[hb + (uint64_t)(t + q) * S + v];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                hist[hb + (uint64_t)(t + q) * S + v] = run;
+                run += c[q];
+            }
+        }
+        for (; t < ts; t++) {
+            const uint32_t c = hist[hb + (uint64_t)t * S + v];
+            hist[hb + (uint64_t)t * S + v] = run;
+            run += c;
+        }
+        tot[v] = run;
+    }
+    __syncthreads();
+    block_excl_scan(tot, S, wsum);
+    for (uint32_t v = threadIdx.x; v < S; v += BLOCK) fboff[f0 + v] = boff[g] + tot[v];
+    if (g == G - 1 && threadIdx.x == 0) fboff[cb_first[G]] = m;
 }
 
 // Pass-2 scatter with LDS staging.  A tile (`tile` slots of one coarse
@@ -612,7 +744,9 @@ __host__ __device__ constexpr size_t sub_lds_bytes(uint32_t tile, uint32_t S, bo
 
 template <int W, bool STAGE, int IPT>
 __global__ __launch_bounds__(BLOCK) void k_sub_scatter(const BTile *__restrict__ tiles, const uint16_t *__restrict__ sub,
-                                                       const uint32_t *__restrict__ hscan, const uint64_t *__restrict__ kin,
+                                                       const uint32_t *__restrict__ hscan,
+                                                       const uint32_t *__restrict__ fboff,
+                                                       const uint32_t *__restrict__ cb_first, const uint64_t *__restrict__ kin,
                                                        const uint8_t *__restrict__ fin, const uint32_t *__restrict__ sin,
                                                        uint64_t *__restrict__ kout, uint8_t *__restrict__ fout,
                                                        uint32_t *__restrict__ sout, uint32_t *__restrict__ perm)
@@ -622,6 +756,7 @@ __global__ __launch_bounds__(BLOCK) void k_sub_scatter(const BTile *__restrict__
     __shared__ uint32_t wsum[WAVES];
     const BTile t = tiles[blockIdx.x];
     const uint32_t S = t.fmask;
+    if (S == 0) return;  // past the last tile
     const uint32_t nbits = (uint32_t)__builtin_ctz(S);
     const bool src = sin != nullptr;
     uint32_t *wc = reinterpret_cast<uint32_t *>(lds);  // WAVES x S
@@ -678,7 +813,7 @@ __global__ __launch_bounds__(BLOCK) void k_sub_scatter(const BTile *__restrict__
             wc[q * S + v] = b;
             b += cq;
         }
-        delta[v] = hscan[t.hbase + (uint64_t)v * t.ts + t.tlocal] - tstart[v];
+        delta[v] = fboff[cb_first[t.set] + v] + hscan[t.hbase + (uint64_t)t.tlocal * S + v] - tstart[v];
     }
     __syncthreads();
     // 3. stable positions: per round, ballot groups of equal fine index
@@ -734,21 +869,6 @@ __global__ void k_pair_remap(uint32_t *__restrict__ pair_home, uint32_t *__restr
     if (i >= x) return;
     pair_home[i] = perm[pair_home[i]];
     pair_borrow[i] = perm[pair_borrow[i]];
-}
-
-// fine row starts: the scanned pass-2 histogram at each coarse bucket's first tile
-__global__ void k_gather_fine(const uint32_t *__restrict__ hscan, const uint64_t *__restrict__ cb_hbase,
-                              const uint32_t *__restrict__ cb_ts, const uint32_t *__restrict__ cb_first,
-                              uint32_t n_coarse, uint32_t m, uint32_t *__restrict__ fboff)
-{
-    const uint32_t g = blockIdx.x;
-    if (g == n_coarse) {
-        if (threadIdx.x == 0) fboff[cb_first[n_coarse]] = m;
-        return;
-    }
-    const uint32_t S = cb_first[g + 1] - cb_first[g];
-    for (uint32_t v = threadIdx.x; v < S; v += blockDim.x)
-        fboff[cb_first[g] + v] = hscan[cb_hbase[g] + (uint64_t)v * cb_ts[g]];
 }
 
 // ---------------------------------------------------------------------------
@@ -958,6 +1078,46 @@ __global__ __launch_bounds__(BLOCK) void k_edge_resolve(const ETile *__restrict_
     }
 }
 
+// LDS groups, planned on the device: one workgroup per coarse bucket reads
+// its fine rows, and its first thread packs consecutive fine buckets
+// greedily into groups of at most gmax slots.  Groups are stored like the
+// fine buckets (entry cb_first[g] + j); unused entries stay empty.  Fine
+// buckets above gmax go to the oversized list (HBM tables).
+__global__ __launch_bounds__(BLOCK) void k_plan_groups(const uint32_t *__restrict__ fboff,
+                                                       const uint32_t *__restrict__ cb_first, uint32_t gmax,
+                                                       EGroup *__restrict__ groups, uint32_t *__restrict__ over,
+                                                       uint32_t *__restrict__ n_over)
+{
+    __shared__ uint32_t rows[MAX_SUB + 1];
+    __shared__ EGroup gl[MAX_SUB];
+    __shared__ uint32_t ng;
+    const uint32_t g = blockIdx.x, f0 = cb_first[g], S = cb_first[g + 1] - f0;
+    for (uint32_t i = threadIdx.x; i <= S; i += BLOCK) rows[i] = fboff[f0 + i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t n = 0, gb = NONE, ge = NONE;
+        for (uint32_t v = 0; v < S; v++) {
+            const uint32_t b0 = rows[v], b1 = rows[v + 1];
+            if (b1 == b0) continue;
+            if (b1 - b0 > gmax) {
+                over[atomicAdd(n_over, 1u)] = f0 + v;
+                continue;
+            }
+            if (gb != NONE && b1 - gb <= gmax && ge == b0) {
+                ge = b1;
+            } else {
+                if (gb != NONE) gl[n++] = EGroup{gb, ge};
+                gb = b0;
+                ge = b1;
+            }
+        }
+        if (gb != NONE) gl[n++] = EGroup{gb, ge};
+        ng = n;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < S; i += BLOCK) groups[f0 + i] = i < ng ? gl[i] : EGroup{0u, 0u};
+}
+
 // ---- the LDS form: one workgroup stages a group of consecutive fine buckets
 // (k-mers, flags) in LDS and runs insert, lookup and resolve against an LDS
 // table.  Keys of different fine buckets never match (equal (k-1)-mers have
@@ -997,6 +1157,7 @@ __global__ __launch_bounds__(BLOCK) void k_edge_lds(const uint64_t *__restrict__
     uint32_t *hb = reinterpret_cast<uint32_t *>(lds + (size_t)nb * (8 * W + 11));  // nb is a multiple of 64
     const EGroup gr = groups[blockIdx.x];
     const uint32_t n = gr.end - gr.begin;
+    if (n == 0) return;  // an unused group entry
     uint32_t tcap = 64;
     while (tcap < 2 * n) tcap <<= 1;
     const uint32_t tmask = tcap - 1;
@@ -1198,7 +1359,10 @@ __device__ __forceinline__ uint32_t wave_append(bool flag, uint32_t *counter)
 // counters in the context's small buffer
 // (uint32 indices: one 128-byte line each, so that atomics on different
 // counters never share a line; C_TOT holds two uint64 totals)
-enum { C_LIST = 0, C_ERR = 32, C_ACTIVE = 64, C_NONCONV = 96, C_CYC = 128, C_CUT = 160, C_TOT = 192, C_WORDS = 224 };
+enum {
+    C_LIST = 0, C_ERR = 32, C_ACTIVE = 64, C_NONCONV = 96, C_CYC = 128, C_CUT = 160, C_TOT = 192, C_OVER = 224,
+    C_WORDS = 256
+};
 
 __device__ __forceinline__ bool live_head(const uint8_t *__restrict__ flags, const uint32_t *__restrict__ hasp,
                                           uint32_t s)
@@ -1729,6 +1893,7 @@ struct uc_ctx {
     std::vector<uint8_t> bucket_done;
     std::vector<size_t> grp_first, fb_first;  // per coarse bucket: first LDS group / oversized fine bucket
     uint32_t lds_nb = 64;                     // LDS capacity of the edge launch
+    bool rows_valid = false;                  // boff_h / fboff_h mirror the device rows
     uint32_t lds_limit = 65536;               // dynamic LDS a workgroup may use on this device
     uint32_t tile1 = 0, tile2 = 0;            // k-mers per bucketing tile, slots per fine-sort tile
     uint64_t n_fallback = 0, n_groups = 0;
@@ -1743,7 +1908,7 @@ struct uc_ctx {
     DBuf otable, in, stage, bkt, tiles, hist, xhist, kmer2, flags2, src2, sub, perm, tiles2, hist2, cbhb, cbts,
         cbfirst, fboff, egroups, fbbase, scan_tmp, kmer, flags, src, pair_h, pair_b, boff, sethb, setts, etiles,
         toff, tcap, table, hit, link, hasp, pred, seg, prevseg, chainA, chainB, cycB, list, clist, ulen, tsum,
-        htoff, setoff, text, small, txt, txtcnt, rec, rch, lhead, tcnt;
+        htoff, setoff, text, small, txt, txtcnt, rec, rch, lhead, tcnt, cbtf, over;
     // pinned host chunks of the file reader and writer
     char *pin[2] = {nullptr, nullptr};
     hipEvent_t pin_ev[2] = {nullptr, nullptr};
@@ -1775,7 +1940,7 @@ static std::vector<DBuf *> all_bufs(uc_ctx *c)
             &c->hasp,   &c->pred,    &c->seg,    &c->prevseg, &c->chainA, &c->chainB,
             &c->cycB,   &c->list,    &c->clist,  &c->ulen,    &c->tsum,    &c->htoff,  &c->setoff,
             &c->text,   &c->small,   &c->txt,    &c->txtcnt,  &c->rec,     &c->rch,    &c->lhead,
-            &c->tcnt};
+            &c->tcnt,   &c->cbtf,    &c->over};
 }
 
 static void free_pinned(uc_ctx *c)
@@ -1986,6 +2151,21 @@ int d2h(uc_ctx *c, void *dst, const void *src, size_t bytes)
     return UC_OK;
 }
 
+// host mirrors of the coarse and fine bucket rows, fetched when asked for
+int host_rows(uc_ctx *c)
+{
+    if (c->rows_valid) return UC_OK;
+    const uint64_t G = (uint64_t)c->n_sets * c->B, F = c->cb_first.empty() ? 0 : c->cb_first[G];
+    c->boff_h.resize(G + 1);
+    c->fboff_h.resize(F + 1);
+    CHK(d2h(c, c->boff_h.data(), c->boff.p, (G + 1) * 4));
+    CHK(d2h(c, c->fboff_h.data(), c->fboff.p, (F + 1) * 4));
+    c->largest = 0;
+    for (uint64_t g = 0; g < G; g++) c->largest = std::max<uint64_t>(c->largest, c->boff_h[g + 1] - c->boff_h[g]);
+    c->rows_valid = true;
+    return UC_OK;
+}
+
 int h2d(uc_ctx *c, void *dst, const void *src, size_t bytes)
 {
     if (bytes == 0) return UC_OK;
@@ -2000,6 +2180,7 @@ void reset_job(uc_ctx *c)
     c->n_unitigs = c->n_cycles = c->text_bytes = c->n_splitters = 0;
     c->chain_rounds = 0;
     c->m = c->x = c->largest = 0;
+    c->rows_valid = false;
 }
 
 // ---- load -----------------------------------------------------------------
@@ -2349,13 +2530,16 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     const uint64_t *kin = c->kin;
     KLAUNCH(c, "k_lmin_count", (k_lmin_count<W, IPT1>), ((uint32_t)T, BLOCK, (B + 1) * 4), kin, P<BTile>(c->tiles), p,
             c->top_mask, P<uint2>(c->bkt), P<uint32_t>(c->hist), P<uint32_t>(c->xhist));
-    // totals at hist[hb] and xhist[T]
+    // totals at hist[hb] and xhist[T], copies of both side by side for one read
+    CHK(ensure(c, c->small, C_WORDS * 4));
+    uint32_t *cnt = P<uint32_t>(c->small);
     CHK(scan_excl<uint32_t>(c, P<uint32_t>(c->hist), P<uint32_t>(c->hist), hb, P<uint32_t>(c->hist) + hb, c->scan_tmp));
     CHK(scan_excl<uint32_t>(c, P<uint32_t>(c->xhist), P<uint32_t>(c->xhist), T, P<uint32_t>(c->xhist) + T,
                             c->scan_tmp));
+    HIPCHK(c, hipMemcpyAsync(cnt + C_TOT, P<uint32_t>(c->hist) + hb, 4, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(cnt + C_TOT + 1, P<uint32_t>(c->xhist) + T, 4, hipMemcpyDeviceToDevice, c->stream));
     uint32_t tot[2];
-    CHK(d2h(c, &tot[0], P<uint32_t>(c->hist) + hb, 4));
-    CHK(d2h(c, &tot[1], P<uint32_t>(c->xhist) + T, 4));
+    CHK(d2h(c, tot, cnt + C_TOT, 8));
     c->m = tot[0];
     c->x = tot[1];
     if (c->m >= 0xFFFFFFFFull) return fail(c, UC_E_LIMIT, "more than 2^32-2 placements");
@@ -2376,17 +2560,18 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
             P<uint16_t>(c->sub));
     KLAUNCH(c, "k_gather_boff", k_gather_boff, (grid_for(G + 1), BLOCK, 0), P<uint32_t>(c->hist),
             P<uint64_t>(c->sethb), P<uint32_t>(c->setts), S, B, (uint32_t)c->m, P<uint32_t>(c->boff));
-    c->boff_h.resize(G + 1);
-    CHK(d2h(c, c->boff_h.data(), c->boff.p, (G + 1) * 4));
-    c->largest = 0;
-    for (uint64_t g = 0; g < G; g++) c->largest = std::max<uint64_t>(c->largest, c->boff_h[g + 1] - c->boff_h[g]);
+    c->rows_valid = false;
 
-    // fine buckets: stable sort of each coarse bucket's slots by fine index
+    // fine buckets: stable sort of each coarse bucket's slots by fine index,
+    // planned and run on the device (no host round trip)
     c->cb_first.assign(G + 1, 0);
     for (uint64_t g = 0; g < G; g++) c->cb_first[g + 1] = c->cb_first[g] + c->set_sub[g / B];
     const uint64_t F = c->cb_first[G];
+    CHK(ensure(c, c->cbfirst, (G + 1) * 4));
+    CHK(h2d(c, c->cbfirst.p, c->cb_first.data(), (G + 1) * 4));
+    CHK(ensure(c, c->fboff, (F + 1) * 4));
     if (!any_fine) {
-        c->fboff_h = c->boff_h;
+        HIPCHK(c, hipMemcpyAsync(c->fboff.p, c->boff.p, (G + 1) * 4, hipMemcpyDeviceToDevice, c->stream));
     } else {
         uint32_t smax = 1;
         for (uint32_t sb : c->set_sub) smax = std::max(smax, sb);
@@ -2396,44 +2581,24 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
         const bool stage2 = !(c->opts.flags & UC_FLAG_DIRECT_SCATTER) &&
                             sub_lds_bytes<W>(tile2, smax, keep_src, true) <= c->lds_limit;
         c->tile2 = tile2;
-        std::vector<BTile> t2;
-        std::vector<uint64_t> cb_hbase(G);
-        std::vector<uint32_t> cb_ts(G);
-        uint64_t h2 = 0;
-        for (uint64_t g = 0; g < G; g++) {
-            const uint32_t Sg = c->set_sub[g / B];
-            const uint64_t b0 = c->boff_h[g], cnt = c->boff_h[g + 1] - b0;
-            const uint32_t ts = (uint32_t)std::max<uint64_t>(1, (cnt + tile2 - 1) / tile2);
-            cb_hbase[g] = h2;
-            cb_ts[g] = ts;
-            for (uint32_t t = 0; t < ts; t++) {
-                BTile bt;
-                bt.kbegin = b0 + (uint64_t)t * tile2;
-                bt.hbase = h2;
-                bt.count = (uint32_t)std::min<uint64_t>(tile2, cnt - std::min<uint64_t>(cnt, (uint64_t)t * tile2));
-                bt.set = (uint32_t)g;
-                bt.ts = ts;
-                bt.tlocal = t;
-                bt.fmask = Sg;
-                bt.pad = 0;
-                t2.push_back(bt);
-            }
-            h2 += (uint64_t)Sg * ts;
-        }
-        const uint64_t T2 = t2.size();
+        // tiles: at most one partial tile per coarse bucket beyond m / tile2
+        const uint64_t T2 = G + (c->m + tile2 - 1) / tile2;
+        const uint64_t h2 = (uint64_t)smax * T2;  // histogram entries, upper bound
         CHK(ensure(c, c->tiles2, T2 * sizeof(BTile)));
-        CHK(h2d(c, c->tiles2.p, t2.data(), T2 * sizeof(BTile)));
         CHK(ensure(c, c->hist2, (h2 + 1) * 4));
-        CHK(ensure(c, c->cbhb, G * 8));
-        CHK(ensure(c, c->cbts, G * 4));
-        CHK(ensure(c, c->cbfirst, (G + 1) * 4));
-        CHK(h2d(c, c->cbhb.p, cb_hbase.data(), G * 8));
-        CHK(h2d(c, c->cbts.p, cb_ts.data(), G * 4));
-        CHK(h2d(c, c->cbfirst.p, c->cb_first.data(), (G + 1) * 4));
+        CHK(ensure(c, c->cbhb, (G + 1) * 8));
+        CHK(ensure(c, c->cbts, (G + 1) * 4));
+        CHK(ensure(c, c->cbtf, (G + 1) * 4));
+        KLAUNCH(c, "k_plan2_prefix", k_plan2_prefix, (1, BLOCK, 0), P<uint32_t>(c->boff), P<uint32_t>(c->cbfirst),
+                (uint32_t)G, tile2, P<uint32_t>(c->cbts), P<uint32_t>(c->cbtf), P<uint64_t>(c->cbhb));
+        KLAUNCH(c, "k_plan2_tiles", k_plan2_tiles, (grid_for(T2), BLOCK, 0), P<uint32_t>(c->boff),
+                P<uint32_t>(c->cbfirst), P<uint32_t>(c->cbts), P<uint32_t>(c->cbtf), P<uint64_t>(c->cbhb), (uint32_t)G,
+                tile2, (uint32_t)T2, P<BTile>(c->tiles2));
         KLAUNCH(c, "k_sub_count", k_sub_count, ((uint32_t)T2, BLOCK, smax * 4), P<uint16_t>(c->sub),
                 P<BTile>(c->tiles2), P<uint32_t>(c->hist2));
-        CHK(scan_excl<uint32_t>(c, P<uint32_t>(c->hist2), P<uint32_t>(c->hist2), h2, P<uint32_t>(c->hist2) + h2,
-                                c->scan_tmp));
+        KLAUNCH(c, "k_sub_prefix", k_sub_prefix, ((uint32_t)G, BLOCK, 0), P<uint32_t>(c->hist2), P<uint64_t>(c->cbhb),
+                P<uint32_t>(c->cbts), P<uint32_t>(c->cbfirst), P<uint32_t>(c->boff), (uint32_t)G, (uint32_t)c->m,
+                P<uint32_t>(c->fboff));
         CHK(ensure(c, c->kmer, std::max<uint64_t>(c->m, 1) * W * 8));
         CHK(ensure(c, c->flags, c->m + 64));
         if (keep_src) CHK(ensure(c, c->src, c->m * 4));
@@ -2442,25 +2607,21 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
         if (stage2) {
             CHK(allow_lds(c, k_sub_scatter<W, true, IPT2>, lds2));
             KLAUNCH(c, "k_sub_scatter", (k_sub_scatter<W, true, IPT2>), ((uint32_t)T2, BLOCK, lds2),
-                    P<BTile>(c->tiles2), P<uint16_t>(c->sub), P<uint32_t>(c->hist2), P<uint64_t>(c->kmer2),
-                    P<uint8_t>(c->flags2), keep_src ? P<uint32_t>(c->src2) : nullptr, P<uint64_t>(c->kmer),
-                    P<uint8_t>(c->flags), keep_src ? P<uint32_t>(c->src) : nullptr, P<uint32_t>(c->perm));
+                    P<BTile>(c->tiles2), P<uint16_t>(c->sub), P<uint32_t>(c->hist2), P<uint32_t>(c->fboff),
+                    P<uint32_t>(c->cbfirst), P<uint64_t>(c->kmer2), P<uint8_t>(c->flags2),
+                    keep_src ? P<uint32_t>(c->src2) : nullptr, P<uint64_t>(c->kmer), P<uint8_t>(c->flags),
+                    keep_src ? P<uint32_t>(c->src) : nullptr, P<uint32_t>(c->perm));
         } else {
             CHK(allow_lds(c, k_sub_scatter<W, false, IPT2>, lds2));
             KLAUNCH(c, "k_sub_scatter", (k_sub_scatter<W, false, IPT2>), ((uint32_t)T2, BLOCK, lds2),
-                    P<BTile>(c->tiles2), P<uint16_t>(c->sub), P<uint32_t>(c->hist2), P<uint64_t>(c->kmer2),
-                    P<uint8_t>(c->flags2), keep_src ? P<uint32_t>(c->src2) : nullptr, P<uint64_t>(c->kmer),
-                    P<uint8_t>(c->flags), keep_src ? P<uint32_t>(c->src) : nullptr, P<uint32_t>(c->perm));
+                    P<BTile>(c->tiles2), P<uint16_t>(c->sub), P<uint32_t>(c->hist2), P<uint32_t>(c->fboff),
+                    P<uint32_t>(c->cbfirst), P<uint64_t>(c->kmer2), P<uint8_t>(c->flags2),
+                    keep_src ? P<uint32_t>(c->src2) : nullptr, P<uint64_t>(c->kmer), P<uint8_t>(c->flags),
+                    keep_src ? P<uint32_t>(c->src) : nullptr, P<uint32_t>(c->perm));
         }
         if (c->x)
             KLAUNCH(c, "k_pair_remap", k_pair_remap, (grid_for(c->x), BLOCK, 0), P<uint32_t>(c->pair_h),
                     P<uint32_t>(c->pair_b), (uint32_t)c->x, P<uint32_t>(c->perm));
-        CHK(ensure(c, c->fboff, (F + 1) * 4));
-        KLAUNCH(c, "k_gather_fine", k_gather_fine, ((uint32_t)(G + 1), 64, 0), P<uint32_t>(c->hist2),
-                P<uint64_t>(c->cbhb), P<uint32_t>(c->cbts), P<uint32_t>(c->cbfirst), (uint32_t)G, (uint32_t)c->m,
-                P<uint32_t>(c->fboff));
-        c->fboff_h.resize(F + 1);
-        CHK(d2h(c, c->fboff_h.data(), c->fboff.p, (F + 1) * 4));
     }
     c->l = l;
     c->bucketed = true;
@@ -2530,42 +2691,37 @@ int prepare_edges(uc_ctx *c)
     const uint32_t load = c->opts.load_pct ? c->opts.load_pct : 50;
     const bool plain = c->opts.mode == UC_MODE_PLAIN;
     const uint32_t group_max = c->opts.group_slots ? c->opts.group_slots : LDS_GROUP;
-    std::vector<EGroup> groups;
+    const uint64_t F = c->cb_first[G];
+    // groups planned on the device, indexed like the fine buckets: coarse
+    // bucket g's groups are entries [cb_first[g], cb_first[g + 1])
+    CHK(ensure(c, c->egroups, std::max<uint64_t>(F, 1) * sizeof(EGroup)));
+    CHK(ensure(c, c->over, F * 4 + 4));
+    CHK(ensure(c, c->small, C_WORDS * 4));
+    uint32_t *cnt = P<uint32_t>(c->small);
+    CHK(fill<uint32_t>(c, cnt + C_OVER, 1, 0u));
+    if (G)
+        KLAUNCH(c, "k_plan_groups", k_plan_groups, ((uint32_t)G, BLOCK, 0), P<uint32_t>(c->fboff),
+                P<uint32_t>(c->cbfirst), group_max, P<EGroup>(c->egroups), P<uint32_t>(c->over), cnt + C_OVER);
+    uint32_t n_over = 0;
+    CHK(d2h(c, &n_over, cnt + C_OVER, 4));
+    c->lds_nb = group_max;
+    c->grp_first.assign(c->cb_first.begin(), c->cb_first.end());
+    // oversized fine buckets (rare): their rows from the host mirror
     std::vector<uint32_t> fb_base;
     std::vector<uint64_t> fb_cnt;
-    c->grp_first.assign(G + 1, 0);
     c->fb_first.assign(G + 1, 0);
-    uint32_t biggest_group = 0;
-    for (uint64_t g = 0; g < G; g++) {
-        c->grp_first[g] = groups.size();
-        c->fb_first[g] = fb_base.size();
-        uint32_t gb = NONE, ge = NONE;
-        for (uint32_t f = c->cb_first[g]; f < c->cb_first[g + 1]; f++) {
-            const uint32_t b0 = c->fboff_h[f], b1 = c->fboff_h[f + 1];
-            if (b1 == b0) continue;
-            if (b1 - b0 > group_max) {
-                fb_base.push_back(b0);
-                fb_cnt.push_back(b1 - b0);
-                continue;
-            }
-            if (gb != NONE && b1 - gb <= group_max && ge == b0) {
-                ge = b1;
-            } else {
-                if (gb != NONE) groups.push_back({gb, ge});
-                gb = b0;
-                ge = b1;
-            }
+    if (n_over) {
+        std::vector<uint32_t> ov(n_over);
+        CHK(d2h(c, ov.data(), c->over.p, n_over * 4ull));
+        std::sort(ov.begin(), ov.end());
+        CHK(host_rows(c));
+        for (uint32_t f : ov) {
+            fb_base.push_back(c->fboff_h[f]);
+            fb_cnt.push_back(c->fboff_h[f + 1] - c->fboff_h[f]);
         }
-        if (gb != NONE) groups.push_back({gb, ge});
+        for (uint64_t g = 0; g <= G; g++)
+            c->fb_first[g] = (size_t)(std::lower_bound(ov.begin(), ov.end(), c->cb_first[g]) - ov.begin());
     }
-    c->grp_first[G] = groups.size();
-    c->fb_first[G] = fb_base.size();
-    for (const EGroup &e : groups) biggest_group = std::max(biggest_group, e.end - e.begin);
-    uint32_t nb = 64;
-    while (nb < biggest_group) nb <<= 1;
-    c->lds_nb = nb;
-    CHK(ensure(c, c->egroups, std::max<size_t>(groups.size(), 1) * sizeof(EGroup)));
-    CHK(h2d(c, c->egroups.p, groups.data(), groups.size() * sizeof(EGroup)));
 
     // HBM tables for oversized fine buckets; plain form shares one table
     // sized for the largest of them
@@ -2617,7 +2773,7 @@ int prepare_edges(uc_ctx *c)
     // has-predecessor bits: every bucket sets only its own slots' bits
     CHK(fill<uint32_t>(c, P<uint32_t>(c->hasp), (c->m + 31) / 32, 0u));
     c->n_fallback = NF;
-    c->n_groups = groups.size();
+    c->n_groups = F;
     c->bucket_done.assign(G, 0);
     c->n_done = 0;
     c->edges_ready = true;
@@ -2730,14 +2886,17 @@ int contract_pass(uc_ctx *c, uint64_t *live, uint32_t *nonconv)
         KLAUNCH(c, "k_chain_tail", k_chain_tail, (grid_for(nl), BLOCK, 0), P<uint32_t>(c->list), nl,
                 P<uint32_t>(c->hasp), ca, P<uint64_t>(c->seg), P<uint32_t>(c->ulen),
                 P<uint64_t>(c->tsum), cnt + C_NONCONV);
-        CHK(d2h(c, &bad, cnt + C_NONCONV, 4));
     }
     c->chain = ca;
     c->chain_rounds += rounds;
-    CHK(scan_excl<uint64_t>(c, P<uint64_t>(c->tsum), P<uint64_t>(c->tsum), T, P<uint64_t>(c->tsum) + T,
-                            c->scan_tmp));
-    uint64_t tot = 0;
-    CHK(d2h(c, &tot, P<uint64_t>(c->tsum) + T, 8));
+    // the unitig totals land beside the non-converged count: one read for both
+    uint64_t *ttot = reinterpret_cast<uint64_t *>(cnt + C_TOT + 2);
+    CHK(scan_excl<uint64_t>(c, P<uint64_t>(c->tsum), P<uint64_t>(c->tsum), T, ttot, c->scan_tmp));
+    uint32_t hh[C_TOT + 4 - C_NONCONV];
+    CHK(d2h(c, hh, cnt + C_NONCONV, sizeof(hh)));
+    bad = hh[0];
+    uint64_t tot;
+    std::memcpy(&tot, hh + (C_TOT + 2 - C_NONCONV), 8);
     c->n_unitigs = tot >> 32;
     *live = tot & 0xFFFFFFFFull;
     *nonconv = bad;
@@ -3068,6 +3227,8 @@ int uc_bucket_info_get(uc_ctx *c, uc_bucket_info *info)
 {
     if (!c || !info) return UC_E_ARG;
     if (!c->bucketed) return fail(c, UC_E_STATE, "no bucketing result");
+    CHK(select_dev(c));
+    CHK(host_rows(c));
     info->n_kmers = c->n;
     info->m_placements = c->m;
     info->n_cross = c->x;
@@ -3090,6 +3251,7 @@ int uc_bucket_copy(uc_ctx *c, uint64_t *bucket_rows, uint32_t *slot_src, uint8_t
     if (!c) return UC_E_ARG;
     if (!c->bucketed) return fail(c, UC_E_STATE, "no bucketing result");
     CHK(select_dev(c));
+    CHK(host_rows(c));
     if (bucket_rows)
         for (size_t i = 0; i < c->boff_h.size(); i++) bucket_rows[i] = c->boff_h[i];
     if (slot_src) {
