@@ -1078,44 +1078,34 @@ __global__ __launch_bounds__(BLOCK) void k_edge_resolve(const ETile *__restrict_
     }
 }
 
-// LDS groups, planned on the device: one workgroup per coarse bucket reads
-// its fine rows, and its first thread packs consecutive fine buckets
-// greedily into groups of at most gmax slots.  Groups are stored like the
-// fine buckets (entry cb_first[g] + j); unused entries stay empty.  Fine
+// LDS groups, planned on the device: one workgroup per coarse bucket; its
+// thread j takes the fine buckets 2j and 2j+1 as one group when they fit in
+// gmax slots together, else as a group each.  Groups are stored like the
+// fine buckets (entry cb_first[g] + v); unused entries stay empty.  Fine
 // buckets above gmax go to the oversized list (HBM tables).
 __global__ __launch_bounds__(BLOCK) void k_plan_groups(const uint32_t *__restrict__ fboff,
                                                        const uint32_t *__restrict__ cb_first, uint32_t gmax,
                                                        EGroup *__restrict__ groups, uint32_t *__restrict__ over,
                                                        uint32_t *__restrict__ n_over)
 {
-    __shared__ uint32_t rows[MAX_SUB + 1];
-    __shared__ EGroup gl[MAX_SUB];
-    __shared__ uint32_t ng;
     const uint32_t g = blockIdx.x, f0 = cb_first[g], S = cb_first[g + 1] - f0;
-    for (uint32_t i = threadIdx.x; i <= S; i += BLOCK) rows[i] = fboff[f0 + i];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t n = 0, gb = NONE, ge = NONE;
-        for (uint32_t v = 0; v < S; v++) {
-            const uint32_t b0 = rows[v], b1 = rows[v + 1];
-            if (b1 == b0) continue;
-            if (b1 - b0 > gmax) {
-                over[atomicAdd(n_over, 1u)] = f0 + v;
-                continue;
-            }
-            if (gb != NONE && b1 - gb <= gmax && ge == b0) {
-                ge = b1;
-            } else {
-                if (gb != NONE) gl[n++] = EGroup{gb, ge};
-                gb = b0;
-                ge = b1;
-            }
+    for (uint32_t v = 2 * threadIdx.x; v < S; v += 2 * BLOCK) {
+        const uint32_t r0 = fboff[f0 + v], r1 = fboff[f0 + v + 1];
+        const bool two = v + 1 < S;
+        const uint32_t r2 = two ? fboff[f0 + v + 2] : r1;
+        const uint32_t a = r1 - r0, b = r2 - r1;
+        EGroup ga{0u, 0u}, gb{0u, 0u};
+        if (a > gmax) over[atomicAdd(n_over, 1u)] = f0 + v;
+        if (two && b > gmax) over[atomicAdd(n_over, 1u)] = f0 + v + 1;
+        if (a <= gmax && b <= gmax && a + b <= gmax) {
+            ga = EGroup{r0, r2};
+        } else {
+            if (a <= gmax) ga = EGroup{r0, r1};
+            if (b <= gmax) gb = EGroup{r1, r2};
         }
-        if (gb != NONE) gl[n++] = EGroup{gb, ge};
-        ng = n;
+        groups[f0 + v] = ga;
+        if (two) groups[f0 + v + 1] = gb;
     }
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < S; i += BLOCK) groups[f0 + i] = i < ng ? gl[i] : EGroup{0u, 0u};
 }
 
 // ---- the LDS form: one workgroup stages a group of consecutive fine buckets
