@@ -12,7 +12,10 @@ roofline = the dominant kernel's algorithmic bytes per launch over its
 average launch time (HIP events on the library's stream).  cpu_baseline =
 the serial restatement of the reference pipeline (oracle/refpipe.c) on a
 bounded sample of the same model, on one host core, checked against the
-GPU's output for that sample.
+GPU's output for that sample.  Beside it: "plain" (the same job with one
+launch per bucket), "sets" (256 uneven sets, batched form vs per-set plain
+runs, both from HBM) and "serve" (the request loop under Poisson arrivals:
+p50 / p99 latency and k-mers/s).
 """
 import argparse
 import json
@@ -300,9 +303,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
     ap.add_argument("--cpu-sample-bp", type=int, default=4_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--sets", action="store_true", help="also run the 256-set batched vs plain comparison")
+    ap.add_argument("--no-sets", action="store_true", help="skip the 256-set batched vs plain comparison")
     ap.add_argument("--n-sets", type=int, default=256)
-    ap.add_argument("--serve", action="store_true", help="also run the request-loop latency benchmark")
+    ap.add_argument("--no-serve", action="store_true", help="skip the request-loop latency benchmark")
     ap.add_argument("--serve-jobs", type=int, default=400)
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-load", type=float, default=0.7, help="offered load, x one-context capacity")
@@ -394,8 +397,8 @@ def main():
         plain = {"value": n / pw, "ms_per_step": round(1e3 * pw, 3), "batched_over_plain": round(pw / (wall / args.steps), 3)}
         log(f"plain form: {1e3 * pw:.2f} ms/step")
 
-    sets = bench_sets(args) if args.sets else None
-    serve = bench_serve(args) if args.serve else None
+    sets = None if args.no_sets else bench_sets(args)
+    serve = None if args.no_serve else bench_serve(args)
 
     base = None
     if not args.no_cpu_baseline:

@@ -22,6 +22,12 @@ typedef struct {
     uint64_t seg_max;
     double repeat_sub;    /* per-base substitution rate inside a copy */
     double error_frac;    /* fraction of true k-mers given one substituted extra copy */
+    /* repeat families (copies_max > 0): each family copies one source
+     * segment c times, c log-uniform in [copies_min, copies_max], with a
+     * per-family substitution rate uniform in [div_min, div_max] (replaces
+     * repeat_sub); copies_max = 0 keeps one copy per segment */
+    uint32_t copies_min, copies_max;
+    double div_min, div_max;
 } uc_synth_params;
 
 /* Distinct canonical k-mers, lexicographically sorted, reference packing

@@ -137,6 +137,10 @@ class SynthParams(C.Structure):
         ("seg_max", C.c_uint64),
         ("repeat_sub", C.c_double),
         ("error_frac", C.c_double),
+        ("copies_min", C.c_uint32),
+        ("copies_max", C.c_uint32),
+        ("div_min", C.c_double),
+        ("div_max", C.c_double),
     ]
 
 
@@ -251,11 +255,13 @@ def _check(rc, ctx=None):
 
 
 def synth_kmers(seed, genome_len, k, repeat_frac=0.0, seg_min=2000, seg_max=10000, repeat_sub=0.001,
-                error_frac=0.0) -> np.ndarray:
+                error_frac=0.0, copies=None, divergence=None) -> np.ndarray:
     """Distinct canonical k-mers of a seeded synthetic genome, reference packing
     (uint32 array of shape (n, ceil(k/16)))."""
     S = synth_lib()
-    p = SynthParams(seed, genome_len, k, repeat_frac, seg_min, seg_max, repeat_sub, error_frac)
+    cmin, cmax = copies if copies else (0, 0)
+    dmin, dmax = divergence if divergence else (0.0, 0.0)
+    p = SynthParams(seed, genome_len, k, repeat_frac, seg_min, seg_max, repeat_sub, error_frac, cmin, cmax, dmin, dmax)
     ptr = _U32P()
     n = C.c_uint64()
     rc = S.uc_synth_kmers(C.byref(p), C.byref(ptr), C.byref(n))
@@ -272,6 +278,14 @@ def medium_model(seed, genome_len, k=31):
     2-10 kbp segments with 0.1% substitutions, 1% error k-mers."""
     return synth_kmers(seed, genome_len, k, repeat_frac=0.05, seg_min=2000, seg_max=10000, repeat_sub=0.001,
                        error_frac=0.01)
+
+
+def large_model(seed, genome_len, k=63):
+    """The repeat and error model of the large single instance: 45% repeat
+    content in families of 300 bp - 6 kbp segments, copy number log-uniform
+    2-1000, per-family divergence 0.5-15%; 0.5% error k-mers."""
+    return synth_kmers(seed, genome_len, k, repeat_frac=0.45, seg_min=300, seg_max=6000, error_frac=0.005,
+                       copies=(2, 1000), divergence=(0.005, 0.15))
 
 
 def format_text(packed: np.ndarray, k: int) -> bytes:

@@ -14,6 +14,7 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -115,16 +116,28 @@ int run(const uc_synth_params *p, uint32_t **out, uint64_t *n_out)
     for (uint64_t i = 0; i < G; i++) g[i] = (uint8_t)(rng.next() >> 62);
     if (p->repeat_frac > 0 && p->seg_max >= p->seg_min && p->seg_min > 0 && p->seg_max < G) {
         uint64_t covered = 0, target = (uint64_t)(p->repeat_frac * (double)G);
+        const bool families = p->copies_max > 0;
         while (covered < target) {
             uint64_t len = p->seg_min + rng.below(p->seg_max - p->seg_min + 1);
-            uint64_t src = rng.below(G - len + 1), dst = rng.below(G - len + 1);
+            uint64_t src = rng.below(G - len + 1);
             std::vector<uint8_t> seg(g.begin() + src, g.begin() + src + len);
-            for (uint64_t i = 0; i < len; i++) {
-                uint8_t c = seg[i];
-                if (p->repeat_sub > 0 && rng.uniform() < p->repeat_sub) c = (uint8_t)((c + 1 + rng.below(3)) & 3);
-                g[dst + i] = c;
+            uint64_t copies = 1;
+            double sub = p->repeat_sub;
+            if (families) {
+                const double lo = std::log((double)std::max(1u, p->copies_min));
+                const double hi = std::log((double)std::max(p->copies_min, p->copies_max));
+                copies = (uint64_t)std::llround(std::exp(lo + (hi - lo) * rng.uniform()));
+                sub = p->div_min + (p->div_max - p->div_min) * rng.uniform();
             }
-            covered += len;
+            for (uint64_t r = 0; r < copies && covered < target; r++) {
+                const uint64_t dst = rng.below(G - len + 1);
+                for (uint64_t i = 0; i < len; i++) {
+                    uint8_t c = seg[i];
+                    if (sub > 0 && rng.uniform() < sub) c = (uint8_t)((c + 1 + rng.below(3)) & 3);
+                    g[dst + i] = c;
+                }
+                covered += len;
+            }
         }
     }
     const K mask = lex_mask<K>(k);

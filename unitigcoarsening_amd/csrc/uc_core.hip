@@ -1080,9 +1080,12 @@ __global__ __launch_bounds__(BLOCK) void k_edge_resolve(const ETile *__restrict_
 
 // LDS groups, planned on the device: one workgroup per coarse bucket; its
 // thread j takes the fine buckets 2j and 2j+1 as one group when they fit in
-// gmax slots together, else as a group each.  Groups are stored like the
-// fine buckets (entry cb_first[g] + v); unused entries stay empty.  Fine
-// buckets above gmax go to the oversized list (HBM tables).
+// gmax slots together, else as a group each.  A coarse bucket's S entries of
+// the group array hold the pair groups first (entry j) and the second halves
+// of split pairs after them (entry S/2 + j), so that the busy workgroups are
+// consecutive (workgroups are dealt round-robin to the XCDs: alternating
+// busy and empty ones would load half of them).  Unused entries stay empty.
+// Fine buckets above gmax go to the oversized list (HBM tables).
 __global__ __launch_bounds__(BLOCK) void k_plan_groups(const uint32_t *__restrict__ fboff,
                                                        const uint32_t *__restrict__ cb_first, uint32_t gmax,
                                                        EGroup *__restrict__ groups, uint32_t *__restrict__ over,
@@ -1103,8 +1106,8 @@ __global__ __launch_bounds__(BLOCK) void k_plan_groups(const uint32_t *__restric
             if (a <= gmax) ga = EGroup{r0, r1};
             if (b <= gmax) gb = EGroup{r1, r2};
         }
-        groups[f0 + v] = ga;
-        if (two) groups[f0 + v + 1] = gb;
+        groups[f0 + v / 2] = ga;
+        if (two) groups[f0 + S / 2 + v / 2] = gb;
     }
 }
 
