@@ -94,15 +94,29 @@ def traffic_for(name):
         return None
 
 
-def cpu_baseline(sample_bp, seed, k, l, B, gpu_ctx_kwargs):
-    """Serial restatement of the reference pipeline on a bounded sample."""
-    import numpy as np  # noqa: F401
+def workload(args, genome_bp=None):
+    """(k-mers, description) of the configured workload."""
+    import unitigcoarsening_amd as uc
 
+    bp = genome_bp or args.genome_bp
+    if args.large:
+        return (uc.large_model(args.seed, bp, args.k),
+                f"large instance at reduced size: {bp / 1e6:.0f} Mbp seeded genome (seed {args.seed}), 45% repeat "
+                f"families (300 bp - 6 kbp segments, copy number 2-1000, divergence 0.5-15%), 0.5% error k-mers; "
+                f"distinct canonical {args.k}-mers")
+    return (uc.medium_model(args.seed, bp, args.k),
+            f"medium: {bp / 1e6:.0f} Mbp seeded genome (seed {args.seed}), 5% repeats of 2-10 kbp copies with 0.1% "
+            f"substitutions, 1% error k-mers; distinct canonical {args.k}-mers")
+
+
+def cpu_baseline(args, l, B, gpu_ctx_kwargs):
+    """Serial restatement of the reference pipeline on a bounded sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     import unitigcoarsening_amd as uc
 
-    kmers = uc.medium_model(seed, sample_bp, k)
+    sample_bp, seed, k = args.cpu_sample_bp, args.seed, args.k
+    kmers, desc = workload(args, sample_bp)
     order = oracle.generate_permutation(l, seed)
     t0 = time.perf_counter()
     text, stats = oracle.refpipe(kmers, k, l, B, order)
@@ -115,9 +129,8 @@ def cpu_baseline(sample_bp, seed, k, l, B, gpu_ctx_kwargs):
         "unit": "k-mers/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"medium model at {sample_bp / 1e6:.0f} Mbp (seed {seed}): {kmers.shape[0]} canonical {k}-mers, "
-                  f"l={l}, {B} buckets; oracle/refpipe.c (serial restatement of the reference pipeline), "
-                  f"{wall:.1f} s",
+        "sample": f"{desc}: {kmers.shape[0]} k-mers, l={l}, {B} buckets; oracle/refpipe.c (serial restatement "
+                  f"of the reference pipeline), {wall:.1f} s",
         "phases_s": {kk: round(stats[kk], 3) for kk in ("bucketing", "assemble", "coarsen", "write_intra",
                                                           "minitigs", "cross_finish")},
         "parity_with_gpu": parity,
@@ -310,7 +323,22 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-load", type=float, default=0.7, help="offered load, x one-context capacity")
     ap.add_argument("--serve-budget-gib", type=int, default=64)
+    ap.add_argument("--large", action="store_true",
+                    help="the large instance's model (k=63, l=13, 256 buckets, repeat families) at --genome-bp")
     args = ap.parse_args()
+    if args.large:
+        # defaults of the large instance unless given explicitly
+        if "--k" not in sys.argv:
+            args.k = 63
+        if "--l" not in sys.argv:
+            args.l = 13
+        if "--buckets" not in sys.argv:
+            args.buckets = 256
+        if "--seed" not in sys.argv:
+            args.seed = 4
+        if "--genome-bp" not in sys.argv:
+            args.genome_bp = 300_000_000
+        args.no_sets = args.no_serve = True
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the path shards by minimizer bucket "
@@ -322,7 +350,7 @@ def main():
 
     k, l, B = args.k, args.l, args.buckets
     t = time.time()
-    kmers = uc.medium_model(args.seed, args.genome_bp, k)
+    kmers, desc = workload(args)
     n = kmers.shape[0]
     W = 1 if k <= 32 else 2
     log(f"workload: {n} canonical {k}-mers from {args.genome_bp} bp in {time.time() - t:.1f}s")
@@ -403,7 +431,7 @@ def main():
     base = None
     if not args.no_cpu_baseline:
         log("cpu baseline (serial restatement of the reference pipeline) ...")
-        base = cpu_baseline(args.cpu_sample_bp, args.seed, k, l, B, {"n_buckets": B})
+        base = cpu_baseline(args, l, B, {"n_buckets": B})
         log(f"cpu baseline: {base['value'] / 1e6:.3f} M k-mers/s, parity with GPU: {base['parity_with_gpu']}")
 
     out = {
@@ -420,8 +448,7 @@ def main():
         "dtype": "u64",
         "data": "synthetic",
         "config": {
-            "workload": f"medium: {args.genome_bp / 1e6:.0f} Mbp seeded genome (seed {args.seed}), 5% repeats of "
-                        f"2-10 kbp copies with 0.1% substitutions, 1% error k-mers; distinct canonical {k}-mers",
+            "workload": desc,
             "n_kmers": n, "k": k, "l": l, "buckets": B, "mode": args.mode, **tun, "autotuned": tune,
             "m_placements": b.m_placements, "n_cross": b.n_cross, "n_unitigs": ci.n_unitigs,
             "text_bytes": text_bytes, "n_fine_buckets": b.n_fine_buckets,
