@@ -933,7 +933,9 @@ __device__ __forceinline__ uint32_t fastrange(uint32_t h, uint32_t cap)
 template <int W>
 __device__ __forceinline__ uint32_t char_at(const Kmer<W> &x, uint32_t i)
 {
-    return (uint32_t)(x.w[i / 32] >> (2 * (i % 32))) & 3u;
+    // a select rather than an indexed read keeps x in registers
+    const uint64_t v = W == 1 || i < 32 ? x.w[0] : x.w[W - 1];
+    return (uint32_t)(v >> (2 * (i % 32))) & 3u;
 }
 
 template <int W>
@@ -1621,19 +1623,109 @@ __device__ __forceinline__ uint32_t unitig_len(const uint64_t *__restrict__ seg,
     return (uint32_t)(sg >> 32) == NONE ? (uint32_t)sg : ulen[s];
 }
 
-// characters of one run from rch into the text
-__device__ __forceinline__ uint64_t copy_run(const char *__restrict__ rch, uint64_t r, uint32_t slot, uint32_t skip,
-                                             char *__restrict__ text, uint64_t p)
+// LDS writes of one wave visible to the whole wave (no block barrier)
+__device__ __forceinline__ void wave_sync()
 {
-    const uint32_t len = rec_len(r), pos = rec_pos(r, slot);
-    for (uint32_t i = skip; i < len; i++) text[p++] = rch[pos + i];
-    return p;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// per tile: exclusive prefix of (1 << 32 | length) over heads -> text
-// offsets; every head writes its first k-mer, then the characters of each
-// later run of its segment
+// four characters from four 2-bit codes (the low byte of b, first code in
+// the low bits), first character in the low byte
+__device__ __forceinline__ uint32_t chars4(uint32_t b)
+{
+    uint32_t r = 0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) r |= (uint32_t)(uint8_t)base_char((b >> (2 * q)) & 3u) << (8 * q);
+    return r;
+}
+
+// characters i .. i + 3 of x (i + 3 < 32 W)
 template <int W>
+__device__ __forceinline__ uint32_t kmer_chars4(const Kmer<W> &x, uint32_t i)
+{
+    const uint32_t sh = 2 * (i % 32);
+    if constexpr (W == 1) {
+        return chars4((uint32_t)(x.w[0] >> sh));
+    } else {
+        const bool hi = i >= 32;
+        uint64_t v = (hi ? x.w[1] : x.w[0]) >> sh;
+        if (!hi && sh > 56) v |= x.w[1] << (64 - sh);
+        return chars4((uint32_t)v);
+    }
+}
+
+// the k characters of x at o, whose address is `at` modulo 4: single bytes
+// up to the first 4-byte boundary and after the last, 4-byte words between
+// (other unitigs own the neighbouring bytes)
+template <int W>
+__device__ __forceinline__ void put_kmer(char *o, uint64_t at, const Kmer<W> &x, uint32_t k)
+{
+    uint32_t lead = (4u - (uint32_t)(at & 3u)) & 3u;
+    if (lead > k) lead = k;
+    uint32_t i = 0;
+    for (; i < lead; i++) o[i] = base_char(char_at<W>(x, i));
+    for (; i + 4 <= k; i += 4) *reinterpret_cast<uint32_t *>(o + i) = kmer_chars4<W>(x, i);
+    for (; i < k; i++) o[i] = base_char(char_at<W>(x, i));
+}
+
+// byte i (0..15) of a 16-byte line
+__device__ __forceinline__ char line_byte(const uint4 &v, uint32_t i)
+{
+    const uint32_t q = i >> 2;
+    const uint32_t d = q == 0 ? v.x : q == 1 ? v.y : q == 2 ? v.z : v.w;
+    return (char)(d >> (8 * (i & 3u)));
+}
+
+// rch[pos, pos + len) to o, reading whole aligned 16-byte lines (rch is
+// padded to a line past its last character)
+__device__ __forceinline__ void copy_chars(char *o, const char *__restrict__ rch, uint32_t pos, uint32_t len)
+{
+    for (uint32_t i = 0; i < len;) {
+        const uint32_t q = (pos + i) & ~15u, b0 = pos + i - q;
+        const uint4 v = *reinterpret_cast<const uint4 *>(rch + q);
+        const uint32_t nb = 16 - b0 < len - i ? 16 - b0 : len - i;
+        for (uint32_t t = 0; t < nb; t++) o[i + t] = line_byte(v, b0 + t);
+        i += nb;
+    }
+}
+
+// one head's segment at o (address `at` modulo 4): first k-mer x, the later
+// characters of its run (record r), the characters of every later run up to
+// the next sampled splitter, '\n' when the unitig ends there
+template <int W>
+__device__ __forceinline__ void put_head(char *o, uint64_t at, const Kmer<W> &x, uint64_t r,
+                                         const uint64_t *__restrict__ rec, const char *__restrict__ rch, uint32_t s,
+                                         uint32_t k, uint32_t m, uint32_t salt, uint32_t shift)
+{
+    put_kmer<W>(o, at, x, k);
+    uint32_t len = rec_len(r), p = k;
+    copy_chars(o + p, rch, rec_pos(r, s) + 1, len - 1);  // the head's own character is the k-th above
+    p += len - 1;
+    uint32_t v = rec_next(r), steps = 0;
+    while (v != NONE && !sampled(v, salt, shift) && ++steps < m) {
+        r = rec[v];
+        len = rec_len(r);
+        copy_chars(o + p, rch, rec_pos(r, v), len);
+        p += len;
+        v = rec_next(r);
+    }
+    if (v == NONE) o[p] = '\n';
+}
+
+// LDS staging bytes per wave in k_head_emit (a range that does not fit is
+// written directly)
+constexpr uint32_t EMIT_STAGE = 4096;
+
+// per tile: exclusive prefix of (1 << 32 | length) over heads -> text
+// offsets.  Every head writes its segment (put_head).  The 64 slots a wave
+// takes at a time own one contiguous text range; when every head there is a
+// whole unitig (a single segment) and the range fits the wave's LDS stage,
+// the heads write into LDS and the range goes out as aligned 16-byte stores.
+// Otherwise each head writes its bytes to HBM itself.  A thread's next item
+// (record and k-mer) is loaded while it writes the current one.
+template <int W, bool STAGE>
 __global__ __launch_bounds__(BLOCK) void k_head_emit(const uint64_t *__restrict__ kmer, const uint64_t *__restrict__ rec,
                                                      const char *__restrict__ rch, const uint32_t *__restrict__ hasp,
                                                      const uint8_t *__restrict__ flags, const uint64_t *__restrict__ seg,
@@ -1642,17 +1734,26 @@ __global__ __launch_bounds__(BLOCK) void k_head_emit(const uint64_t *__restrict_
                                                      uint32_t salt, uint32_t shift, uint64_t *__restrict__ htoff,
                                                      char *__restrict__ text, uint32_t n_tiles)
 {
-    __shared__ uint64_t tile[CT_TILE];
+    constexpr uint32_t STG = EMIT_STAGE;
+    __shared__ uint64_t tile[CT_TILE + 1];  // prefixes in slot order, then the tile's end
     __shared__ uint64_t wsum[WAVES];
+    __shared__ __align__(16) char stage[STAGE ? WAVES : 1][STAGE ? STG : 16];
     const uint32_t ti = xcd_tile(n_tiles);
     if (ti == NONE) return;
     const uint64_t t0 = (uint64_t)ti * CT_TILE;
-    bool hd[CT_ITEMS];
+    uint32_t hd = 0, multi = 0;  // per item: live head; head of a chain of several segments
 #pragma unroll
     for (int j = 0; j < CT_ITEMS; j++) {
         const uint64_t s = t0 + (uint64_t)j * BLOCK + threadIdx.x;
-        hd[j] = s < m && live_head(flags, hasp, (uint32_t)s);
-        tile[j * BLOCK + threadIdx.x] = hd[j] ? ((1ull << 32) | unitig_len(seg, ulen, (uint32_t)s)) : 0ull;
+        uint64_t v = 0;
+        if (s < m && live_head(flags, hasp, (uint32_t)s)) {
+            const uint64_t sg = seg[s];
+            const bool one = (uint32_t)(sg >> 32) == NONE;
+            v = (1ull << 32) | (one ? (uint32_t)sg : ulen[s]);
+            hd |= 1u << j;
+            multi |= (one ? 0u : 1u) << j;
+        }
+        tile[j * BLOCK + threadIdx.x] = v;
     }
     __syncthreads();
     uint64_t loc[CT_ITEMS];
@@ -1671,26 +1772,50 @@ __global__ __launch_bounds__(BLOCK) void k_head_emit(const uint64_t *__restrict_
     const uint64_t excl = tile_off[ti] + wpre + incl - run;
 #pragma unroll
     for (int j = 0; j < CT_ITEMS; j++) tile[threadIdx.x * CT_ITEMS + j] = excl + loc[j];
+    if (threadIdx.x == BLOCK - 1) tile[CT_TILE] = excl + run;
     __syncthreads();
-#pragma unroll
-    for (int j = 0; j < CT_ITEMS; j++) {
-        if (!hd[j]) continue;
-        const uint32_t s = (uint32_t)(t0 + (uint64_t)j * BLOCK + threadIdx.x);
+    char *stg = stage[STAGE ? w : 0];
+    const uint32_t lane = __lane_id();
+    const uint32_t s0 = (uint32_t)t0 + threadIdx.x;
+    uint64_t nr = 0;
+    Kmer<W> nx{};
+    if (hd & 1u) {
+        nr = rec[s0];
+        nx = load_kmer<W>(kmer, s0, ~0ull);
+    }
+#pragma unroll 1
+    for (uint32_t j = 0; j < (uint32_t)CT_ITEMS; j++) {
+        const uint32_t s = s0 + j * BLOCK;
+        const uint64_t r = nr;
+        const Kmer<W> x = nx;
+        if ((hd >> (j + 1)) & 1u) {
+            nr = rec[s + BLOCK];
+            nx = load_kmer<W>(kmer, s + BLOCK, ~0ull);
+        }
+        const bool h = (hd >> j) & 1u, mu = (multi >> j) & 1u;
         const uint64_t pre = tile[j * BLOCK + threadIdx.x];
         const uint64_t off = (pre & 0xFFFFFFFFull) + (pre >> 32) * k;
-        if ((uint32_t)(seg[s] >> 32) != NONE) htoff[s] = off;
-        char *o = text + off;
-        const Kmer<W> x0 = load_kmer<W>(kmer, s, ~0ull);
-        for (uint32_t i = 0; i < k; i++) o[i] = base_char(char_at<W>(x0, i));
-        uint64_t r = rec[s];
-        uint64_t p = copy_run(rch, r, s, 1, text, off + k);  // the head's own character is the k-th above
-        uint32_t v = rec_next(r), steps = 0;
-        while (v != NONE && !sampled(v, salt, shift) && ++steps < m) {
-            r = rec[v];
-            p = copy_run(rch, r, v, 0, text, p);
-            v = rec_next(r);
+        // the wave's text range [r0, r1): from its first slot to the slot after its last
+        const uint32_t li = j * BLOCK + (uint32_t)w * 64;
+        const uint64_t p0 = tile[li], p1 = tile[li + 64];
+        const uint64_t r0 = (p0 & 0xFFFFFFFFull) + (p0 >> 32) * k, r1 = (p1 & 0xFFFFFFFFull) + (p1 >> 32) * k;
+        const uint64_t a = r0 & ~15ull;  // staged from r0's 16-byte line
+        if (STAGE && __ballot(mu) == 0 && r1 - a <= STG) {  // uniform over the wave
+            if (h) put_head<W>(stg + (off - a), off, x, r, rec, rch, s, k, m, salt, shift);
+            wave_sync();
+            // whole lines [f, e) as 16-byte stores; the partial first and last
+            // lines byte by byte, one lane per byte
+            const uint64_t f = (r0 + 15) >> 4, e = r1 >> 4;
+            for (uint64_t c = f + lane; c < e; c += 64)
+                *reinterpret_cast<uint4 *>(text + 16 * c) = *reinterpret_cast<const uint4 *>(stg + (16 * c - a));
+            const uint64_t hb = lane < 16 ? r0 + lane : 16 * e + (lane - 16);
+            const uint64_t hlim = lane < 16 ? (16 * f < r1 ? 16 * f : r1) : r1;
+            if (lane < 32 && hb < hlim && (lane < 16 || e >= f)) text[hb] = stg[hb - a];
+            wave_sync();
+        } else if (h) {
+            if (mu) htoff[s] = off;
+            put_head<W>(text + off, off, x, r, rec, rch, s, k, m, salt, shift);
         }
-        if (v == NONE) text[p] = '\n';
     }
 }
 
@@ -1709,7 +1834,8 @@ __global__ void k_rand_emit(const uint64_t *__restrict__ rec, const char *__rest
     uint32_t v = r0, steps = 0;
     do {
         const uint64_t r = rec[v];
-        p = copy_run(rch, r, v, 0, text, p);
+        copy_chars(text + p, rch, rec_pos(r, v), rec_len(r));
+        p += rec_len(r);
         v = rec_next(r);
     } while (v != NONE && !sampled(v, salt, shift) && ++steps < n_slots);
     if (v == NONE) text[p] = '\n';
@@ -2988,8 +3114,10 @@ int run_emit(uc_ctx *c)
     CHK(ensure(c, c->text, c->text_bytes + 1));
     CHK(ensure(c, c->htoff, c->m * 8));
     if (m) {
-        KLAUNCH(c, "k_head_emit", k_head_emit<W>, (xcd_grid(T), BLOCK, 0), P<uint64_t>(c->kmer), P<uint64_t>(c->rec),
-                P<char>(c->rch), P<uint32_t>(c->hasp), P<uint8_t>(c->flags), P<uint64_t>(c->seg),
+        // one-word k-mers go straight to HBM; wider ones are staged in LDS
+        // (measured faster each way on MI355X)
+        KLAUNCH(c, "k_head_emit", (k_head_emit<W, (W > 1)>), (xcd_grid(T), BLOCK, 0), P<uint64_t>(c->kmer),
+                P<uint64_t>(c->rec), P<char>(c->rch), P<uint32_t>(c->hasp), P<uint8_t>(c->flags), P<uint64_t>(c->seg),
                 P<uint32_t>(c->ulen), P<uint64_t>(c->tsum), m, c->k, c->salt, c->shift, P<uint64_t>(c->htoff),
                 P<char>(c->text), tile_arg(c, T));
     }
