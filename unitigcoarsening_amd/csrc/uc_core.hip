@@ -560,6 +560,34 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restri
     }
 }
 
+// pass-1 tiles from the per-set plan: set s owns tiles [tfirst[s],
+// tfirst[s + 1]) (at least one, possibly empty, so that every bucket row has
+// a start in the histogram) and histogram entries from B * tfirst[s]
+__global__ void k_plan1_tiles(const uint64_t *__restrict__ set_off, const uint64_t *__restrict__ tfirst,
+                              const uint32_t *__restrict__ set_sub, uint32_t n_sets, uint32_t B, uint32_t tile,
+                              uint64_t T, BTile *__restrict__ tiles)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    uint32_t lo = 0, hi = n_sets;  // the set with tfirst[lo] <= t < tfirst[lo + 1]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (tfirst[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    const uint64_t ns = set_off[lo + 1] - set_off[lo], tl = t - tfirst[lo], skip = tl * tile;
+    BTile bt;
+    bt.kbegin = set_off[lo] + skip;
+    bt.hbase = (uint64_t)B * tfirst[lo];
+    bt.count = (uint32_t)(ns > skip ? (ns - skip < tile ? ns - skip : tile) : 0);
+    bt.set = lo;
+    bt.ts = (uint32_t)(tfirst[lo + 1] - tfirst[lo]);
+    bt.tlocal = (uint32_t)tl;
+    bt.fmask = B * set_sub[lo] - 1;
+    bt.pad = 0;
+    tiles[t] = bt;
+}
+
 // bucket row starts: boff[set*B + b] = scanned histogram at the set's first tile
 __global__ void k_gather_boff(const uint32_t *__restrict__ hscan, const uint64_t *__restrict__ set_hbase,
                               const uint32_t *__restrict__ set_ts, uint32_t n_sets, uint32_t B, uint32_t m,
@@ -2027,7 +2055,7 @@ struct uc_ctx {
     DBuf otable, in, stage, bkt, tiles, hist, xhist, kmer2, flags2, src2, sub, perm, tiles2, hist2, cbhb, cbts,
         cbfirst, fboff, egroups, fbbase, scan_tmp, kmer, flags, src, pair_h, pair_b, boff, sethb, setts, etiles,
         toff, tcap, table, hit, link, hasp, pred, seg, prevseg, chainA, chainB, cycB, list, clist, ulen, tsum,
-        htoff, setoff, text, small, txt, txtcnt, rec, rch, lhead, tcnt, cbtf, over;
+        htoff, setoff, text, small, txt, txtcnt, rec, rch, lhead, tcnt, cbtf, over, setplan;
     // pinned host chunks of the file reader and writer
     char *pin[2] = {nullptr, nullptr};
     hipEvent_t pin_ev[2] = {nullptr, nullptr};
@@ -2059,7 +2087,7 @@ static std::vector<DBuf *> all_bufs(uc_ctx *c)
             &c->hasp,   &c->pred,    &c->seg,    &c->prevseg, &c->chainA, &c->chainB,
             &c->cycB,   &c->list,    &c->clist,  &c->ulen,    &c->tsum,    &c->htoff,  &c->setoff,
             &c->text,   &c->small,   &c->txt,    &c->txtcnt,  &c->rec,     &c->rch,    &c->lhead,
-            &c->tcnt,   &c->cbtf,    &c->over};
+            &c->tcnt,   &c->cbtf,    &c->over,   &c->setplan};
 }
 
 static void free_pinned(uc_ctx *c)
@@ -2599,33 +2627,31 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     const uint32_t tile1 = IPT1 * BLOCK;
     c->tile1 = tile1;
     // tiles: every set gets at least one (possibly empty) tile so each bucket
-    // row has a start in the histogram
-    std::vector<BTile> tiles;
+    // row has a start in the histogram; the tiles are filled on the device
+    std::vector<uint64_t> set_plan(3 * (uint64_t)S + 2);  // set offsets, tile prefix, then the sets' fine counts
     std::vector<uint64_t> set_hbase(S);
     std::vector<uint32_t> set_ts(S);
-    uint64_t hb = 0;
+    uint64_t T = 0;
     for (uint32_t s = 0; s < S; s++) {
         const uint64_t ns = c->set_off[s + 1] - c->set_off[s];
         const uint32_t ts = (uint32_t)std::max<uint64_t>(1, (ns + tile1 - 1) / tile1);
-        set_hbase[s] = hb;
+        set_plan[s] = c->set_off[s];
+        set_plan[S + 1 + s] = T;
+        set_hbase[s] = (uint64_t)B * T;
         set_ts[s] = ts;
-        for (uint32_t t = 0; t < ts; t++) {
-            BTile bt;
-            bt.kbegin = c->set_off[s] + (uint64_t)t * tile1;
-            bt.hbase = hb;
-            bt.count = (uint32_t)std::min<uint64_t>(tile1, ns - std::min<uint64_t>(ns, (uint64_t)t * tile1));
-            bt.set = s;
-            bt.ts = ts;
-            bt.tlocal = t;
-            bt.fmask = B * c->set_sub[s] - 1;
-            bt.pad = 0;
-            tiles.push_back(bt);
-        }
-        hb += (uint64_t)B * ts;
+        T += ts;
     }
-    const uint64_t T = tiles.size();
+    set_plan[S] = c->set_off[S];
+    set_plan[2 * (uint64_t)S + 1] = T;
+    uint32_t *sub_h = reinterpret_cast<uint32_t *>(set_plan.data() + 2 * (uint64_t)S + 2);
+    for (uint32_t s = 0; s < S; s++) sub_h[s] = c->set_sub[s];
+    const uint64_t hb = (uint64_t)B * T;
+    CHK(ensure(c, c->setplan, set_plan.size() * 8));
+    CHK(h2d(c, c->setplan.p, set_plan.data(), set_plan.size() * 8));
     CHK(ensure(c, c->tiles, T * sizeof(BTile)));
-    CHK(h2d(c, c->tiles.p, tiles.data(), T * sizeof(BTile)));
+    const uint64_t *sp = P<uint64_t>(c->setplan);
+    KLAUNCH(c, "k_plan1_tiles", k_plan1_tiles, (grid_for(T), BLOCK, 0), sp, sp + S + 1,
+            reinterpret_cast<const uint32_t *>(sp + 2 * (uint64_t)S + 2), S, B, tile1, T, P<BTile>(c->tiles));
     CHK(ensure(c, c->bkt, c->n * 8));
     CHK(ensure(c, c->hist, (hb + 1) * 4));
     CHK(ensure(c, c->xhist, (T + 1) * 4));
