@@ -67,8 +67,8 @@ typedef struct {
 #define UC_FLAG_PROFILE 1u /* time every kernel launch with HIP events (uc_kernel_stat) */
 #define UC_FLAG_XCD_MAP 2u /* give each XCD a contiguous range of bucket tiles */
 #define UC_FLAG_KEEP_SRC 4u /* keep slot -> input k-mer ids for uc_bucket_copy */
-#define UC_FLAG_DIRECT_SCATTER 8u /* the fine-bucket sort writes straight from registers instead of
-                                     staging each tile in LDS sorted by fine bucket */
+#define UC_FLAG_DIRECT_SCATTER 8u /* both bucketing scatters write straight from registers instead of
+                                     staging each tile in LDS sorted by bucket */
 
 void uc_default_opts(uc_opts *o);
 int uc_ctx_create(const uc_opts *opts, uc_ctx **out);

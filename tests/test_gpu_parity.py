@@ -234,6 +234,34 @@ def test_bucketize_matches_oracle(use_table, genome):
     assert np.array_equal(bucket_of[pb], mins[src[pb], 1] % B)
 
 
+@pytest.mark.parametrize("k,l,genome", [(31, 11, 200_000), (63, 13, 120_000), (15, 13, 0)])
+def test_staged_and_direct_scatter_agree(k, l, genome):
+    # LDS-staged scatters (the default) and direct stores from registers
+    # give the same bucket CSR, pairs and text.  k = 15, l = 13 on random
+    # k-mers makes about two thirds of them cross, more than a pass-1 tile's
+    # stage holds, so those tiles take the direct path inside the staged kernel.
+    if genome:
+        kmers = uc.medium_model(50 + k, genome, k)
+    else:
+        kmers = uc.encode(rand_set(random.Random(9), 60_000, k), k)
+    out = []
+    for direct in (False, True):
+        with uc.Compactor(n_buckets=64, seed=5, keep_src=True, direct_scatter=direct) as c:
+            c.load(kmers, k)
+            c.bucketize(l)
+            arrays = c.bucket_arrays()
+            log(f"k={k} l={l} direct={direct} m={c.bucket_info().m_placements} x={c.bucket_info().n_cross}")
+            text = c.compact(kmers, k, l)
+        out.append((arrays, text))
+    (a0, t0), (a1, t1) = out
+    for x, y in zip(a0, a1):
+        assert np.array_equal(x, y)
+    assert t0 == t1
+    assert uc.sorted_lines(t0) == textbook_sorted(kmers, k)
+    if not genome:
+        assert a0[3].shape[0] > kmers.shape[0] // 2  # most k-mers crossed
+
+
 def test_edges_match_textbook_succ():
     k = 31
     kmers = uc.medium_model(41, 120_000, k)

@@ -426,16 +426,27 @@ __device__ uint32_t block_excl_scan(uint32_t *a, uint32_t n, uint32_t *wsum)
 // order of their k-mers (home copy before borrowed copy when both land in
 // one bucket); per wave the rank among placements bound for the same bucket
 // comes from log2(B) ballots of the bucket bits.  Every thread's codes and
-// k-mers are loaded up front, so all of them are in flight at once.  (A
-// variant staging each tile in LDS sorted by bucket measured slower here:
-// with about 30 placements per bucket and tile the direct stores already
-// fill most of each line in L2.)
+// k-mers are loaded up front, so all of them are in flight at once.  With
+// STAGE, the tile's placements are gathered in LDS in bucket order and leave
+// as runs of consecutive slots; a tile with more placements than the stage
+// holds (more than a quarter of its k-mers crossing) stores them directly.
 __host__ __device__ constexpr size_t lmin_lds_bytes(uint32_t B)
 {
     return ((size_t)(B + 1) * (WAVES + 1) + 2 * (size_t)B) * 4;
 }
 
-template <int W, int IPT>
+// placements one pass-1 tile may stage: its k-mers plus a quarter
+__host__ __device__ constexpr uint32_t lmin_stage_cap(uint32_t tile) { return tile + tile / 4; }
+
+__host__ __device__ constexpr size_t lmin_stage_off(uint32_t B) { return (lmin_lds_bytes(B) + 15) & ~(size_t)15; }
+
+template <int W>
+__host__ __device__ constexpr size_t lmin_stage_bytes(uint32_t B, uint32_t tile, bool keep)
+{
+    return lmin_stage_off(B) + (size_t)lmin_stage_cap(tile) * (8 * W + 2 + 2 + 1 + (keep ? 4 : 0));
+}
+
+template <int W, int IPT, bool STAGE>
 __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restrict__ kin, const BTile *__restrict__ tiles,
                                                         uint32_t B, uint32_t nbits, uint64_t top_mask,
                                                         const uint2 *__restrict__ bkt,
@@ -444,7 +455,7 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restri
                                                         uint8_t *__restrict__ flags, uint32_t *__restrict__ src,
                                                         uint32_t *__restrict__ pair_home,
                                                         uint32_t *__restrict__ pair_borrow,
-                                                        uint16_t *__restrict__ sub)
+                                                        uint16_t *__restrict__ sub, uint32_t n_tiles)
 {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ uint32_t wsum[WAVES];
@@ -453,7 +464,9 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restri
     uint32_t *wcnt = run + (B + 1);                      // WAVES x (B+1)
     uint32_t *tstart = wcnt + WAVES * (B + 1);
     uint32_t *delta = tstart + B;
-    const BTile t = tiles[blockIdx.x];
+    const uint32_t ti = xcd_tile(n_tiles);
+    if (ti == NONE) return;
+    const BTile t = tiles[ti];
     // all of this thread's codes and k-mers in flight at once
     uint2 codes[IPT];
     Kmer<W> xs[IPT];
@@ -480,7 +493,16 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restri
     __syncthreads();
     const int lane = __lane_id(), w = threadIdx.x / 64;
     const uint64_t lt = (1ull << lane) - 1ull, me = 1ull << lane;
-    const uint32_t xbase = xscan[blockIdx.x];
+    const uint32_t xbase = xscan[ti];
+    // staged placements: k-mers, fine index, bucket, flags (and sources)
+    constexpr uint32_t CAP = lmin_stage_cap(IPT * BLOCK);
+    Kmer<W> *skm = reinterpret_cast<Kmer<W> *>(lds + lmin_stage_off(B));
+    uint16_t *ssub = reinterpret_cast<uint16_t *>(skm + CAP);
+    uint16_t *sbk = ssub + CAP;
+    uint8_t *sfl = reinterpret_cast<uint8_t *>(sbk + CAP);
+    uint32_t *ssrc = reinterpret_cast<uint32_t *>(sfl + CAP);  // 4-byte aligned: CAP is a multiple of 4
+    const uint32_t placed = t.count + (xscan[ti + 1] - xbase);
+    const bool staged = STAGE && placed <= CAP;  // uniform over the block
 #pragma unroll
     for (int j = 0; j < IPT; j++) {
         if ((uint32_t)j * BLOCK >= t.count) break;  // uniform over the block
@@ -522,7 +544,13 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restri
             const Kmer<W> x = xs[j];
             uint32_t lh = tstart[bp] + run[bp] + (uint32_t)(__popcll(hpH & lt) + __popcll(hsH & lt));
             for (int q = 0; q < w; q++) lh += wcnt[q * (B + 1) + bp];
-            {
+            if (staged) {
+                skm[lh] = x;
+                ssub[lh] = (uint16_t)fp;
+                sbk[lh] = (uint16_t)bp;
+                sfl[lh] = cross ? F_LENT : 0;
+                if (keep) ssrc[lh] = (uint32_t)g;
+            } else {
                 const uint32_t ns = delta[bp] + lh;
                 store_kmer<W>(kout, ns, x);
                 flags[ns] = cross ? F_LENT : 0;
@@ -533,7 +561,13 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restri
                 uint32_t lb = tstart[bs] + run[bs] + (uint32_t)(__popcll(hpB & lt) + __popcll(hsB & lt)) +
                               (bp == bs ? 1u : 0u);
                 for (int q = 0; q < w; q++) lb += wcnt[q * (B + 1) + bs];
-                {
+                if (staged) {
+                    skm[lb] = x;
+                    ssub[lb] = (uint16_t)fs;
+                    sbk[lb] = (uint16_t)bs;
+                    sfl[lb] = F_BORROW;
+                    if (keep) ssrc[lb] = (uint32_t)g;
+                } else {
                     const uint32_t ns = delta[bs] + lb;
                     store_kmer<W>(kout, ns, x);
                     flags[ns] = F_BORROW;
@@ -557,6 +591,17 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restri
             run[v] += a;
         }
         __syncthreads();
+    }
+    // staged: slot li of the tile's bucket-ordered placements goes to
+    // delta[bucket] + li, so consecutive li of one bucket are consecutive slots
+    if (staged) {
+        for (uint32_t li = threadIdx.x; li < placed; li += BLOCK) {
+            const uint32_t ns = delta[sbk[li]] + li;
+            store_kmer<W>(kout, ns, skm[li]);
+            flags[ns] = sfl[li];
+            sub[ns] = ssub[li];
+            if (keep) src[ns] = ssrc[li];
+        }
     }
 }
 
@@ -777,12 +822,15 @@ __global__ __launch_bounds__(BLOCK) void k_sub_scatter(const BTile *__restrict__
                                                        const uint32_t *__restrict__ cb_first, const uint64_t *__restrict__ kin,
                                                        const uint8_t *__restrict__ fin, const uint32_t *__restrict__ sin,
                                                        uint64_t *__restrict__ kout, uint8_t *__restrict__ fout,
-                                                       uint32_t *__restrict__ sout, uint32_t *__restrict__ perm)
+                                                       uint32_t *__restrict__ sout, uint32_t *__restrict__ perm,
+                                                       uint32_t n_tiles)
 {
     constexpr uint32_t TILE = (uint32_t)IPT * BLOCK, CHUNK = TILE / WAVES;  // slots per tile / per wave
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ uint32_t wsum[WAVES];
-    const BTile t = tiles[blockIdx.x];
+    const uint32_t ti = xcd_tile(n_tiles);
+    if (ti == NONE) return;
+    const BTile t = tiles[ti];
     const uint32_t S = t.fmask;
     if (S == 0) return;  // past the last tile
     const uint32_t nbits = (uint32_t)__builtin_ctz(S);
@@ -2108,6 +2156,15 @@ static uint32_t tile_arg(const uc_ctx *c, uint64_t n_tiles)
     return (uint32_t)n_tiles | ((c->opts.flags & UC_FLAG_XCD_MAP) ? 0x80000000u : 0u);
 }
 
+// XCD-contiguous tile order whatever the flags: the scatters, whose
+// neighbouring tiles write neighbouring bytes of every bucket (one L2 then
+// merges the partial lines; measured 1.9 -> 1.3 GB and 1.4 -> 0.7 GB
+// written by the two passes on 48 M k-mers)
+static uint32_t tile_arg_xcd(uint64_t n_tiles)
+{
+    return (uint32_t)n_tiles | 0x80000000u;
+}
+
 namespace {
 
 int fail(uc_ctx *c, int code, const std::string &msg)
@@ -2698,11 +2755,23 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     CHK(ensure(c, c->pair_b, c->x * 4));
     CHK(ensure(c, c->boff, (G + 1) * 4));
     const uint32_t nbits = (uint32_t)__builtin_ctz(B);
-    const size_t lds = lmin_lds_bytes(B);
-    KLAUNCH(c, "k_lmin_scatter", (k_lmin_scatter<W, IPT1>), ((uint32_t)T, BLOCK, lds), kin, P<BTile>(c->tiles), B,
-            nbits, c->top_mask, P<uint2>(c->bkt), P<uint32_t>(c->hist), P<uint32_t>(c->xhist), P<uint64_t>(k1),
-            P<uint8_t>(f1), keep_src ? P<uint32_t>(s1) : nullptr, P<uint32_t>(c->pair_h), P<uint32_t>(c->pair_b),
-            P<uint16_t>(c->sub));
+    const bool stage1 = !(c->opts.flags & UC_FLAG_DIRECT_SCATTER) &&
+                        lmin_stage_bytes<W>(B, tile1, keep_src) <= c->lds_limit;
+    if (stage1) {
+        const size_t lds = lmin_stage_bytes<W>(B, tile1, keep_src);
+        CHK(allow_lds(c, k_lmin_scatter<W, IPT1, true>, lds));
+        KLAUNCH(c, "k_lmin_scatter", (k_lmin_scatter<W, IPT1, true>), (xcd_grid(T), BLOCK, lds), kin,
+                P<BTile>(c->tiles), B, nbits, c->top_mask, P<uint2>(c->bkt), P<uint32_t>(c->hist),
+                P<uint32_t>(c->xhist), P<uint64_t>(k1), P<uint8_t>(f1), keep_src ? P<uint32_t>(s1) : nullptr,
+                P<uint32_t>(c->pair_h), P<uint32_t>(c->pair_b), P<uint16_t>(c->sub), tile_arg_xcd(T));
+    } else {
+        const size_t lds = lmin_lds_bytes(B);
+        CHK(allow_lds(c, k_lmin_scatter<W, IPT1, false>, lds));
+        KLAUNCH(c, "k_lmin_scatter", (k_lmin_scatter<W, IPT1, false>), (xcd_grid(T), BLOCK, lds), kin,
+                P<BTile>(c->tiles), B, nbits, c->top_mask, P<uint2>(c->bkt), P<uint32_t>(c->hist),
+                P<uint32_t>(c->xhist), P<uint64_t>(k1), P<uint8_t>(f1), keep_src ? P<uint32_t>(s1) : nullptr,
+                P<uint32_t>(c->pair_h), P<uint32_t>(c->pair_b), P<uint16_t>(c->sub), tile_arg_xcd(T));
+    }
     KLAUNCH(c, "k_gather_boff", k_gather_boff, (grid_for(G + 1), BLOCK, 0), P<uint32_t>(c->hist),
             P<uint64_t>(c->sethb), P<uint32_t>(c->setts), S, B, (uint32_t)c->m, P<uint32_t>(c->boff));
     c->rows_valid = false;
@@ -2723,7 +2792,9 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
         // slots per fine-sort tile; LDS staging when the device has the room
         constexpr int IPT2 = W == 1 ? 16 : 8;
         const uint32_t tile2 = IPT2 * BLOCK;
-        const bool stage2 = !(c->opts.flags & UC_FLAG_DIRECT_SCATTER) &&
+        // (two-word k-mers measured faster unstaged here: 4.9 vs 5.5 ms on
+        // 303 M slots; one-word ones staged: 0.82 vs 0.85 ms on 53 M)
+        const bool stage2 = W == 1 && !(c->opts.flags & UC_FLAG_DIRECT_SCATTER) &&
                             sub_lds_bytes<W>(tile2, smax, keep_src, true) <= c->lds_limit;
         c->tile2 = tile2;
         // tiles: at most one partial tile per coarse bucket beyond m / tile2
@@ -2751,18 +2822,18 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
         const size_t lds2 = sub_lds_bytes<W>(tile2, smax, keep_src, stage2);
         if (stage2) {
             CHK(allow_lds(c, k_sub_scatter<W, true, IPT2>, lds2));
-            KLAUNCH(c, "k_sub_scatter", (k_sub_scatter<W, true, IPT2>), ((uint32_t)T2, BLOCK, lds2),
+            KLAUNCH(c, "k_sub_scatter", (k_sub_scatter<W, true, IPT2>), (xcd_grid(T2), BLOCK, lds2),
                     P<BTile>(c->tiles2), P<uint16_t>(c->sub), P<uint32_t>(c->hist2), P<uint32_t>(c->fboff),
                     P<uint32_t>(c->cbfirst), P<uint64_t>(c->kmer2), P<uint8_t>(c->flags2),
                     keep_src ? P<uint32_t>(c->src2) : nullptr, P<uint64_t>(c->kmer), P<uint8_t>(c->flags),
-                    keep_src ? P<uint32_t>(c->src) : nullptr, P<uint32_t>(c->perm));
+                    keep_src ? P<uint32_t>(c->src) : nullptr, P<uint32_t>(c->perm), tile_arg_xcd(T2));
         } else {
             CHK(allow_lds(c, k_sub_scatter<W, false, IPT2>, lds2));
-            KLAUNCH(c, "k_sub_scatter", (k_sub_scatter<W, false, IPT2>), ((uint32_t)T2, BLOCK, lds2),
+            KLAUNCH(c, "k_sub_scatter", (k_sub_scatter<W, false, IPT2>), (xcd_grid(T2), BLOCK, lds2),
                     P<BTile>(c->tiles2), P<uint16_t>(c->sub), P<uint32_t>(c->hist2), P<uint32_t>(c->fboff),
                     P<uint32_t>(c->cbfirst), P<uint64_t>(c->kmer2), P<uint8_t>(c->flags2),
                     keep_src ? P<uint32_t>(c->src2) : nullptr, P<uint64_t>(c->kmer), P<uint8_t>(c->flags),
-                    keep_src ? P<uint32_t>(c->src) : nullptr, P<uint32_t>(c->perm));
+                    keep_src ? P<uint32_t>(c->src) : nullptr, P<uint32_t>(c->perm), tile_arg_xcd(T2));
         }
         if (c->x)
             KLAUNCH(c, "k_pair_remap", k_pair_remap, (grid_for(c->x), BLOCK, 0), P<uint32_t>(c->pair_h),
