@@ -151,51 +151,77 @@ __device__ __forceinline__ bool lex_less(const Kmer<W> &a, const Kmer<W> &b)
     }
 }
 
+// How the order of an l-mer code is found: a caller's table, or the seeded
+// bijection computed with 24-bit multiplies (2l <= 24: the full-rate
+// v_mul_u32_u24, same residues modulo 2^(2l)) or 32-bit ones.  The mode is a
+// template argument so that the hot loop has no branch on it (with a runtime
+// branch the compiler merged both forms into the quarter-rate 32-bit one).
+enum { LM_TABLE = 0, LM_MUL24 = 1, LM_MUL32 = 2 };
+
+__host__ __device__ constexpr int lmin_mode(uint32_t l, bool table)
+{
+    return table ? LM_TABLE : (2 * l <= 24 ? LM_MUL24 : LM_MUL32);
+}
+
 // seeded bijection on 2l-bit l-mer codes (same function as the oracle's
 // orc_lmer_order): xor seed, three rounds of odd multiply + xorshift
+template <int MODE>
+__device__ __forceinline__ uint32_t lmer_order_m(uint32_t code, const LminParams &p)
+{
+    if constexpr (MODE == LM_TABLE) {
+        return p.table[code];
+    } else {
+        uint32_t x = (code ^ p.s0) & p.lmask;
+        if constexpr (MODE == LM_MUL24) {
+            x = __umul24(x, 0x3779B1u) & p.lmask;
+            x ^= x >> p.h1;
+            x = __umul24(x, 0xEBCA6Bu) & p.lmask;
+            x ^= (x >> p.h2) ^ p.s1;
+            x = __umul24(x, 0xD4EB2Fu) & p.lmask;
+        } else {
+            x = (x * 0x3779B1u) & p.lmask;
+            x ^= x >> p.h1;
+            x = (x * 0xEBCA6Bu) & p.lmask;
+            x ^= (x >> p.h2) ^ p.s1;
+            x = (x * 0xD4EB2Fu) & p.lmask;
+        }
+        x ^= x >> p.h1;
+        return x;
+    }
+}
+
 __device__ __forceinline__ uint32_t lmer_order(uint32_t code, const LminParams &p)
 {
-    if (p.table) return p.table[code];
-    uint32_t x = (code ^ p.s0) & p.lmask;
-    if (p.lmask < (1u << 24)) {
-        // codes and constants below 2^24: the full-rate 24-bit multiply
-        // gives the same residue modulo 2^(2l)
-        x = __umul24(x, 0x3779B1u) & p.lmask;
-        x ^= x >> p.h1;
-        x = __umul24(x, 0xEBCA6Bu) & p.lmask;
-        x ^= x >> p.h2;
-        x ^= p.s1;
-        x = __umul24(x, 0xD4EB2Fu) & p.lmask;
-    } else {
-        x = (x * 0x3779B1u) & p.lmask;
-        x ^= x >> p.h1;
-        x = (x * 0xEBCA6Bu) & p.lmask;
-        x ^= x >> p.h2;
-        x ^= p.s1;
-        x = (x * 0xD4EB2Fu) & p.lmask;
-    }
-    x ^= x >> p.h1;
-    return x;
+    if (p.table) return lmer_order_m<LM_TABLE>(code, p);
+    return 2 * p.l <= 24 ? lmer_order_m<LM_MUL24>(code, p) : lmer_order_m<LM_MUL32>(code, p);
 }
 
 // lmin_buckets.h:67-94: minimum order over the l-mers of the (k-1)-prefix
 // (o1) and of the (k-1)-suffix (o2); the l-mer code keeps its first
-// character most significant.
-template <int W>
+// character most significant.  Characters are read 16 to a 32-bit word; the
+// loop over a word's characters keeps a run-time (uniform) trip count, so
+// the code stays small whatever k is.
+template <int W, int MODE>
 __device__ __forceinline__ void double_lmin(const Kmer<W> &x, const LminParams &p, uint32_t &o1, uint32_t &o2)
 {
     uint32_t lmer = 0;
     o1 = NONE;
     o2 = NONE;
+    const uint32_t k = p.k, l = p.l;
 #pragma unroll
-    for (int i = 0; i < 32 * W; i++) {
-        if ((uint32_t)i < p.k) {
-            const uint32_t c = (uint32_t)(x.w[i / 32] >> (2 * (i % 32))) & 3u;
-            lmer = ((lmer << 2) | c) & p.lmask;
-            if ((uint32_t)i + 1 >= p.l) {
-                const uint32_t o = lmer_order(lmer, p);
-                if ((uint32_t)i + 1 < p.k) o1 = min(o1, o);
-                if ((uint32_t)i >= p.l) o2 = min(o2, o);
+    for (int q = 0; q < 2 * W; q++) {
+        const uint32_t i0 = 16u * q;
+        if (i0 >= k) break;  // uniform
+        const uint32_t w = (uint32_t)(x.w[q / 2] >> (32 * (q % 2)));
+        const uint32_t n = k - i0 < 16u ? k - i0 : 16u;
+#pragma unroll 1
+        for (uint32_t j = 0; j < n; j++) {
+            const uint32_t i = i0 + j;
+            lmer = ((lmer << 2) | ((w >> (2 * j)) & 3u)) & p.lmask;
+            if (i + 1 >= l) {  // uniform
+                const uint32_t o = lmer_order_m<MODE>(lmer, p);
+                if (i + 1 < k) o1 = min(o1, o);
+                if (i >= l) o2 = min(o2, o);
             }
         }
     }
@@ -338,7 +364,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_tile(const T *in, T *out, uint64
 // ---------------------------------------------------------------------------
 // 1. bucketing
 
-template <int W, int IPT>
+template <int W, int IPT, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_lmin_count(const uint64_t *__restrict__ kin, const BTile *__restrict__ tiles,
                                                       LminParams p, uint64_t top_mask, uint2 *__restrict__ bkt,
                                                       uint32_t *__restrict__ hist, uint32_t *__restrict__ xhist)
@@ -363,7 +389,7 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_count(const uint64_t *__restrict
         const uint32_t idx = (uint32_t)j * BLOCK + threadIdx.x;
         if (idx >= t.count) continue;
         uint32_t o1, o2;
-        double_lmin<W>(xs[j], p, o1, o2);
+        double_lmin<W, MODE>(xs[j], p, o1, o2);
         const uint32_t f1 = o1 & t.fmask, f2 = o2 & t.fmask;
         bkt[t.kbegin + idx] = make_uint2(f1, f2);
         atomicAdd(&cnt[f1 & bmask], 1u);
@@ -2730,8 +2756,20 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     p.table = table;
 
     const uint64_t *kin = c->kin;
-    KLAUNCH(c, "k_lmin_count", (k_lmin_count<W, IPT1>), ((uint32_t)T, BLOCK, (B + 1) * 4), kin, P<BTile>(c->tiles), p,
-            c->top_mask, P<uint2>(c->bkt), P<uint32_t>(c->hist), P<uint32_t>(c->xhist));
+    switch (lmin_mode(l, table != nullptr)) {
+    case LM_TABLE:
+        KLAUNCH(c, "k_lmin_count", (k_lmin_count<W, IPT1, LM_TABLE>), ((uint32_t)T, BLOCK, (B + 1) * 4), kin,
+                P<BTile>(c->tiles), p, c->top_mask, P<uint2>(c->bkt), P<uint32_t>(c->hist), P<uint32_t>(c->xhist));
+        break;
+    case LM_MUL24:
+        KLAUNCH(c, "k_lmin_count", (k_lmin_count<W, IPT1, LM_MUL24>), ((uint32_t)T, BLOCK, (B + 1) * 4), kin,
+                P<BTile>(c->tiles), p, c->top_mask, P<uint2>(c->bkt), P<uint32_t>(c->hist), P<uint32_t>(c->xhist));
+        break;
+    default:
+        KLAUNCH(c, "k_lmin_count", (k_lmin_count<W, IPT1, LM_MUL32>), ((uint32_t)T, BLOCK, (B + 1) * 4), kin,
+                P<BTile>(c->tiles), p, c->top_mask, P<uint2>(c->bkt), P<uint32_t>(c->hist), P<uint32_t>(c->xhist));
+        break;
+    }
     // totals at hist[hb] and xhist[T], copies of both side by side for one read
     CHK(ensure(c, c->small, C_WORDS * 4));
     uint32_t *cnt = P<uint32_t>(c->small);
