@@ -791,39 +791,56 @@ __global__ __launch_bounds__(BLOCK) void k_sub_count(const uint16_t *__restrict_
 }
 
 // per coarse bucket: each fine bucket's counts over the tiles become
-// exclusive prefixes (in place), and the fine rows follow from the column
-// totals: fboff[first_g + v] = boff[g] + totals of fine buckets < v
-__global__ __launch_bounds__(BLOCK) void k_sub_prefix(uint32_t *__restrict__ hist, const uint64_t *__restrict__ cb_hbase,
-                                                      const uint32_t *__restrict__ cb_ts,
-                                                      const uint32_t *__restrict__ cb_first,
-                                                      const uint32_t *__restrict__ boff, uint32_t G, uint32_t m,
-                                                      uint32_t *__restrict__ fboff)
+// exclusive prefixes (in place), and the column totals go to colsum.  A block
+// takes 64 columns (fine buckets) of one coarse bucket, lane = column; its
+// waves split the tiles, so that G * S / 64 blocks share the work.
+__global__ __launch_bounds__(BLOCK) void k_sub_colscan(uint32_t *__restrict__ hist, const uint64_t *__restrict__ cb_hbase,
+                                                       const uint32_t *__restrict__ cb_ts,
+                                                       const uint32_t *__restrict__ cb_first, uint32_t cpb,
+                                                       uint32_t *__restrict__ colsum)
 {
-    __shared__ uint32_t tot[MAX_SUB];
-    __shared__ uint32_t wsum[WAVES];
-    const uint32_t g = blockIdx.x;
-    const uint32_t f0 = cb_first[g], S = cb_first[g + 1] - f0, ts = cb_ts[g];
+    __shared__ uint32_t part[WAVES][64];
+    const uint32_t g = blockIdx.x / cpb, cb = blockIdx.x % cpb;
+    const uint32_t f0 = cb_first[g], S = cb_first[g + 1] - f0;
+    if (cb * 64 >= S) return;  // uniform over the block
+    const uint32_t ts = cb_ts[g];
     const uint64_t hb = cb_hbase[g];
-    for (uint32_t v = threadIdx.x; v < S; v += BLOCK) {
-        uint32_t run = 0;
-        uint32_t t = 0;
-        for (; t + 4 <= ts; t += 4) {
-            uint32_t c[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) c[q] = hist[hb + (uint64_t)(t + q) * S + v];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                hist[hb + (uint64_t)(t + q) * S + v] = run;
-                run += c[q];
-            }
-        }
-        for (; t < ts; t++) {
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x / 64;
+    const uint32_t v = cb * 64 + lane;
+    const bool on = v < S;
+    const uint32_t per = (ts + WAVES - 1) / WAVES, t0 = min(ts, w * per), t1 = min(ts, t0 + per);
+    uint32_t sum = 0;
+    if (on) {
+#pragma unroll 8
+        for (uint32_t t = t0; t < t1; t++) sum += hist[hb + (uint64_t)t * S + v];
+    }
+    part[w][lane] = sum;
+    __syncthreads();
+    uint32_t run = 0;
+    for (uint32_t q = 0; q < w; q++) run += part[q][lane];
+    if (on) {
+#pragma unroll 8
+        for (uint32_t t = t0; t < t1; t++) {
             const uint32_t c = hist[hb + (uint64_t)t * S + v];
             hist[hb + (uint64_t)t * S + v] = run;
             run += c;
         }
-        tot[v] = run;
+        if (w == WAVES - 1) colsum[f0 + v] = run;
     }
+}
+
+// fine rows from the column totals: fboff[first_g + v] = boff[g] + totals of
+// fine buckets < v of coarse bucket g
+__global__ __launch_bounds__(BLOCK) void k_sub_rows(const uint32_t *__restrict__ colsum,
+                                                    const uint32_t *__restrict__ cb_first,
+                                                    const uint32_t *__restrict__ boff, uint32_t G, uint32_t m,
+                                                    uint32_t *__restrict__ fboff)
+{
+    __shared__ uint32_t tot[MAX_SUB];
+    __shared__ uint32_t wsum[WAVES];
+    const uint32_t g = blockIdx.x;
+    const uint32_t f0 = cb_first[g], S = cb_first[g + 1] - f0;
+    for (uint32_t v = threadIdx.x; v < S; v += BLOCK) tot[v] = colsum[f0 + v];
     __syncthreads();
     block_excl_scan(tot, S, wsum);
     for (uint32_t v = threadIdx.x; v < S; v += BLOCK) fboff[f0 + v] = boff[g] + tot[v];
@@ -2177,7 +2194,7 @@ struct uc_ctx {
     DBuf otable, in, stage, bkt, tiles, hist, xhist, kmer2, flags2, src2, sub, perm, tiles2, hist2, cbhb, cbts,
         cbfirst, fboff, egroups, fbbase, scan_tmp, kmer, flags, src, pair_h, pair_b, boff, sethb, setts, etiles,
         toff, tcap, table, hit, link, hasp, pred, seg, prevseg, chainA, chainB, cycB, list, clist, ulen, tsum,
-        htoff, setoff, text, small, txt, txtcnt, rec, rch, lhead, tcnt, cbtf, over, setplan;
+        htoff, setoff, text, small, txt, txtcnt, rec, rch, lhead, tcnt, cbtf, over, setplan, colsum;
     // pinned host chunks of the file reader and writer
     char *pin[2] = {nullptr, nullptr};
     hipEvent_t pin_ev[2] = {nullptr, nullptr};
@@ -2209,7 +2226,7 @@ static std::vector<DBuf *> all_bufs(uc_ctx *c)
             &c->hasp,   &c->pred,    &c->seg,    &c->prevseg, &c->chainA, &c->chainB,
             &c->cycB,   &c->list,    &c->clist,  &c->ulen,    &c->tsum,    &c->htoff,  &c->setoff,
             &c->text,   &c->small,   &c->txt,    &c->txtcnt,  &c->rec,     &c->rch,    &c->lhead,
-            &c->tcnt,   &c->cbtf,    &c->over,   &c->setplan};
+            &c->tcnt,   &c->cbtf,    &c->over,   &c->setplan, &c->colsum};
 }
 
 static void free_pinned(uc_ctx *c)
@@ -2876,7 +2893,9 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
         uint32_t smax = 1;
         for (uint32_t sb : c->set_sub) smax = std::max(smax, sb);
         // slots per fine-sort tile; LDS staging when the device has the room
-        constexpr int IPT2 = W == 1 ? 16 : 8;
+        // (2048-slot tiles: half the LDS of 4096, so more tiles in flight per
+        // CU; measured 0.82 -> 0.71 ms for the staged one-word scatter)
+        constexpr int IPT2 = 8;
         const uint32_t tile2 = IPT2 * BLOCK;
         // (two-word k-mers measured faster unstaged here: 4.9 vs 5.5 ms on
         // 303 M slots; one-word ones staged: 0.82 vs 0.85 ms on 53 M)
@@ -2898,9 +2917,12 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
                 tile2, (uint32_t)T2, P<BTile>(c->tiles2));
         KLAUNCH(c, "k_sub_count", k_sub_count, ((uint32_t)T2, BLOCK, smax * 4), P<uint16_t>(c->sub),
                 P<BTile>(c->tiles2), P<uint32_t>(c->hist2));
-        KLAUNCH(c, "k_sub_prefix", k_sub_prefix, ((uint32_t)G, BLOCK, 0), P<uint32_t>(c->hist2), P<uint64_t>(c->cbhb),
-                P<uint32_t>(c->cbts), P<uint32_t>(c->cbfirst), P<uint32_t>(c->boff), (uint32_t)G, (uint32_t)c->m,
-                P<uint32_t>(c->fboff));
+        const uint32_t cpb = (smax + 63) / 64;  // column blocks per coarse bucket
+        CHK(ensure(c, c->colsum, F * 4 + 4));
+        KLAUNCH(c, "k_sub_colscan", k_sub_colscan, ((uint32_t)(G * cpb), BLOCK, 0), P<uint32_t>(c->hist2),
+                P<uint64_t>(c->cbhb), P<uint32_t>(c->cbts), P<uint32_t>(c->cbfirst), cpb, P<uint32_t>(c->colsum));
+        KLAUNCH(c, "k_sub_rows", k_sub_rows, ((uint32_t)G, BLOCK, 0), P<uint32_t>(c->colsum), P<uint32_t>(c->cbfirst),
+                P<uint32_t>(c->boff), (uint32_t)G, (uint32_t)c->m, P<uint32_t>(c->fboff));
         CHK(ensure(c, c->kmer, std::max<uint64_t>(c->m, 1) * W * 8));
         CHK(ensure(c, c->flags, c->m + 64));
         if (keep_src) CHK(ensure(c, c->src, c->m * 4));
