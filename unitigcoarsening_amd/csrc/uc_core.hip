@@ -1909,12 +1909,26 @@ __global__ __launch_bounds__(BLOCK) void k_head_emit(const uint64_t *__restrict_
     if (ti == NONE) return;
     const uint64_t t0 = (uint64_t)ti * CT_TILE;
     uint32_t hd = 0, multi = 0;  // per item: live head; head of a chain of several segments
+    // flags, predecessor bits and segments of all items in flight at once
+    // (segments of slots that are no heads are read and ignored)
+    uint32_t fh[CT_ITEMS];
+    uint64_t sgs[CT_ITEMS];
+#pragma unroll
+    for (int j = 0; j < CT_ITEMS; j++) {
+        const uint64_t s = t0 + (uint64_t)j * BLOCK + threadIdx.x;
+        fh[j] = F_BORROW;
+        sgs[j] = 0;
+        if (s < m) {
+            fh[j] = (uint32_t)flags[s] | (((hasp[s >> 5] >> (s & 31)) & 1u) << 8);
+            sgs[j] = seg[s];
+        }
+    }
 #pragma unroll
     for (int j = 0; j < CT_ITEMS; j++) {
         const uint64_t s = t0 + (uint64_t)j * BLOCK + threadIdx.x;
         uint64_t v = 0;
-        if (s < m && live_head(flags, hasp, (uint32_t)s)) {
-            const uint64_t sg = seg[s];
+        if (!(fh[j] & (F_BORROW | 0x100u))) {  // a live head: not borrowed, no predecessor
+            const uint64_t sg = sgs[j];
             const bool one = (uint32_t)(sg >> 32) == NONE;
             v = (1ull << 32) | (one ? (uint32_t)sg : ulen[s]);
             hd |= 1u << j;
