@@ -1892,8 +1892,8 @@ constexpr uint32_t EMIT_STAGE = 4096;
 // the heads write into LDS and the range goes out as aligned 16-byte stores.
 // Otherwise each head writes its bytes to HBM itself.  A thread's next item
 // (record and k-mer) is loaded while it writes the current one.
-template <int W, bool STAGE>
-__global__ __launch_bounds__(BLOCK) void k_head_emit(const uint64_t *__restrict__ kmer, const uint64_t *__restrict__ rec,
+template <int W, bool STAGE, int NT>
+__global__ __launch_bounds__(NT) void k_head_emit(const uint64_t *__restrict__ kmer, const uint64_t *__restrict__ rec,
                                                      const char *__restrict__ rch, const uint32_t *__restrict__ hasp,
                                                      const uint8_t *__restrict__ flags, const uint64_t *__restrict__ seg,
                                                      const uint32_t *__restrict__ ulen,
@@ -1902,20 +1902,21 @@ __global__ __launch_bounds__(BLOCK) void k_head_emit(const uint64_t *__restrict_
                                                      char *__restrict__ text, uint32_t n_tiles)
 {
     constexpr uint32_t STG = EMIT_STAGE;
+    constexpr int NW = NT / 64, CTI = CT_TILE / NT;  // waves; items per thread
     __shared__ uint64_t tile[CT_TILE + 1];  // prefixes in slot order, then the tile's end
-    __shared__ uint64_t wsum[WAVES];
-    __shared__ __align__(16) char stage[STAGE ? WAVES : 1][STAGE ? STG : 16];
+    __shared__ uint64_t wsum[NW];
+    __shared__ __align__(16) char stage[STAGE ? NW : 1][STAGE ? STG : 16];
     const uint32_t ti = xcd_tile(n_tiles);
     if (ti == NONE) return;
     const uint64_t t0 = (uint64_t)ti * CT_TILE;
     uint32_t hd = 0, multi = 0;  // per item: live head; head of a chain of several segments
     // flags, predecessor bits and segments of all items in flight at once
     // (segments of slots that are no heads are read and ignored)
-    uint32_t fh[CT_ITEMS];
-    uint64_t sgs[CT_ITEMS];
+    uint32_t fh[CTI];
+    uint64_t sgs[CTI];
 #pragma unroll
-    for (int j = 0; j < CT_ITEMS; j++) {
-        const uint64_t s = t0 + (uint64_t)j * BLOCK + threadIdx.x;
+    for (int j = 0; j < CTI; j++) {
+        const uint64_t s = t0 + (uint64_t)j * NT + threadIdx.x;
         fh[j] = F_BORROW;
         sgs[j] = 0;
         if (s < m) {
@@ -1924,8 +1925,8 @@ __global__ __launch_bounds__(BLOCK) void k_head_emit(const uint64_t *__restrict_
         }
     }
 #pragma unroll
-    for (int j = 0; j < CT_ITEMS; j++) {
-        const uint64_t s = t0 + (uint64_t)j * BLOCK + threadIdx.x;
+    for (int j = 0; j < CTI; j++) {
+        const uint64_t s = t0 + (uint64_t)j * NT + threadIdx.x;
         uint64_t v = 0;
         if (!(fh[j] & (F_BORROW | 0x100u))) {  // a live head: not borrowed, no predecessor
             const uint64_t sg = sgs[j];
@@ -1934,15 +1935,15 @@ __global__ __launch_bounds__(BLOCK) void k_head_emit(const uint64_t *__restrict_
             hd |= 1u << j;
             multi |= (one ? 0u : 1u) << j;
         }
-        tile[j * BLOCK + threadIdx.x] = v;
+        tile[j * NT + threadIdx.x] = v;
     }
     __syncthreads();
-    uint64_t loc[CT_ITEMS];
+    uint64_t loc[CTI];
     uint64_t run = 0;
 #pragma unroll
-    for (int j = 0; j < CT_ITEMS; j++) {
+    for (int j = 0; j < CTI; j++) {
         loc[j] = run;
-        run += tile[threadIdx.x * CT_ITEMS + j];
+        run += tile[threadIdx.x * CTI + j];
     }
     const uint64_t incl = wave_incl_scan(run);
     const int w = threadIdx.x / 64;
@@ -1952,8 +1953,8 @@ __global__ __launch_bounds__(BLOCK) void k_head_emit(const uint64_t *__restrict_
     for (int i = 0; i < w; i++) wpre += wsum[i];
     const uint64_t excl = tile_off[ti] + wpre + incl - run;
 #pragma unroll
-    for (int j = 0; j < CT_ITEMS; j++) tile[threadIdx.x * CT_ITEMS + j] = excl + loc[j];
-    if (threadIdx.x == BLOCK - 1) tile[CT_TILE] = excl + run;
+    for (int j = 0; j < CTI; j++) tile[threadIdx.x * CTI + j] = excl + loc[j];
+    if (threadIdx.x == NT - 1) tile[CT_TILE] = excl + run;
     __syncthreads();
     char *stg = stage[STAGE ? w : 0];
     const uint32_t lane = __lane_id();
@@ -1965,19 +1966,19 @@ __global__ __launch_bounds__(BLOCK) void k_head_emit(const uint64_t *__restrict_
         nx = load_kmer<W>(kmer, s0, ~0ull);
     }
 #pragma unroll 1
-    for (uint32_t j = 0; j < (uint32_t)CT_ITEMS; j++) {
-        const uint32_t s = s0 + j * BLOCK;
+    for (uint32_t j = 0; j < (uint32_t)CTI; j++) {
+        const uint32_t s = s0 + j * NT;
         const uint64_t r = nr;
         const Kmer<W> x = nx;
         if ((hd >> (j + 1)) & 1u) {
-            nr = rec[s + BLOCK];
-            nx = load_kmer<W>(kmer, s + BLOCK, ~0ull);
+            nr = rec[s + NT];
+            nx = load_kmer<W>(kmer, s + NT, ~0ull);
         }
         const bool h = (hd >> j) & 1u, mu = (multi >> j) & 1u;
-        const uint64_t pre = tile[j * BLOCK + threadIdx.x];
+        const uint64_t pre = tile[j * NT + threadIdx.x];
         const uint64_t off = (pre & 0xFFFFFFFFull) + (pre >> 32) * k;
         // the wave's text range [r0, r1): from its first slot to the slot after its last
-        const uint32_t li = j * BLOCK + (uint32_t)w * 64;
+        const uint32_t li = j * NT + (uint32_t)w * 64;
         const uint64_t p0 = tile[li], p1 = tile[li + 64];
         const uint64_t r0 = (p0 & 0xFFFFFFFFull) + (p0 >> 32) * k, r1 = (p1 & 0xFFFFFFFFull) + (p1 >> 32) * k;
         const uint64_t a = r0 & ~15ull;  // staged from r0's 16-byte line
@@ -3337,9 +3338,11 @@ int run_emit(uc_ctx *c)
     CHK(ensure(c, c->text, c->text_bytes + 1));
     CHK(ensure(c, c->htoff, c->m * 8));
     if (m) {
-        // one-word k-mers go straight to HBM; wider ones are staged in LDS
-        // (measured faster each way on MI355X)
-        KLAUNCH(c, "k_head_emit", (k_head_emit<W, (W > 1)>), (xcd_grid(T), BLOCK, 0), P<uint64_t>(c->kmer),
+        // one-word k-mers go straight to HBM from 256-thread groups; wider
+        // ones are staged in LDS by 512-thread groups (each measured faster
+        // than the other choices on MI355X)
+        constexpr int NT = W == 1 ? 256 : 512;
+        KLAUNCH(c, "k_head_emit", (k_head_emit<W, (W > 1), NT>), (xcd_grid(T), NT, 0), P<uint64_t>(c->kmer),
                 P<uint64_t>(c->rec), P<char>(c->rch), P<uint32_t>(c->hasp), P<uint8_t>(c->flags), P<uint64_t>(c->seg),
                 P<uint32_t>(c->ulen), P<uint64_t>(c->tsum), m, c->k, c->salt, c->shift, P<uint64_t>(c->htoff),
                 P<char>(c->text), tile_arg(c, T));
