@@ -453,7 +453,8 @@ def main():
             "m_placements": b.m_placements, "n_cross": b.n_cross, "n_unitigs": ci.n_unitigs,
             "text_bytes": text_bytes, "n_fine_buckets": b.n_fine_buckets,
             "largest_fine_bucket": b.largest_fine_bucket, "bucket_tile": b.bucket_tile, "fine_tile": b.fine_tile,
-            "lds_limit": b.lds_limit, "n_splitters": ci.n_splitters, "chain_rounds": ci.chain_rounds,
+            "lds_limit": b.lds_limit, "n_oversized_fine": b.n_oversized, "oversized_slots": b.oversized_slots,
+            "n_splitters": ci.n_splitters, "chain_rounds": ci.chain_rounds,
             "n_runs": ci.n_runs,
         },
         "phases_ms_per_step": {kk: round(v / args.steps, 3) for kk, v in phase.items()},

@@ -129,6 +129,8 @@ typedef struct {
     uint32_t bucket_tile;    /* k-mers per bucketing tile */
     uint32_t fine_tile;      /* slots per fine-sort tile (0: no fine sort) */
     uint32_t lds_limit;      /* LDS bytes a workgroup may use on the device */
+    uint32_t n_oversized;    /* fine buckets above the LDS group size: edge finding uses HBM tables */
+    uint64_t oversized_slots; /* slots in those fine buckets */
 } uc_bucket_info;
 
 int uc_bucket_info_get(uc_ctx *ctx, uc_bucket_info *info);

@@ -68,6 +68,8 @@ class UcBucketInfo(C.Structure):
         ("bucket_tile", C.c_uint32),
         ("fine_tile", C.c_uint32),
         ("lds_limit", C.c_uint32),
+        ("n_oversized", C.c_uint32),
+        ("oversized_slots", C.c_uint64),
     ]
 
 

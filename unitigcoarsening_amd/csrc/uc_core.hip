@@ -3587,8 +3587,17 @@ int uc_bucket_info_get(uc_ctx *c, uc_bucket_info *info)
     info->fine_tile = c->tile2;
     info->lds_limit = c->lds_limit;
     info->largest_fine_bucket = 0;
-    for (size_t f = 0; f + 1 < c->fboff_h.size(); f++)
-        info->largest_fine_bucket = std::max<uint64_t>(info->largest_fine_bucket, c->fboff_h[f + 1] - c->fboff_h[f]);
+    info->n_oversized = 0;
+    info->oversized_slots = 0;
+    const uint64_t group_max = c->opts.group_slots ? c->opts.group_slots : LDS_GROUP;
+    for (size_t f = 0; f + 1 < c->fboff_h.size(); f++) {
+        const uint64_t sz = c->fboff_h[f + 1] - c->fboff_h[f];
+        info->largest_fine_bucket = std::max<uint64_t>(info->largest_fine_bucket, sz);
+        if (sz > group_max) {
+            info->n_oversized++;
+            info->oversized_slots += sz;
+        }
+    }
     return UC_OK;
 }
 
