@@ -1038,18 +1038,19 @@ struct RunOut {
     uint8_t *flags;
 };
 
-// run record (8 bytes): next run's first k-mer | (length - 1) << 32 (11 bits,
-// a run never leaves its LDS group) | (position of its characters in rch
-// minus its own slot) << 43 (12 bits, signed: both lie in the group)
+// run record (8 bytes): next run's first k-mer | (length - 1) << 32 (12 bits,
+// a run never leaves its LDS group of at most 4096 slots) | (position of its
+// characters in rch minus its own slot) << 44 (13 bits, signed: both lie in
+// the group)
 __host__ __device__ __forceinline__ uint64_t make_rec(uint32_t next, uint32_t len, uint32_t slot, uint32_t pos)
 {
-    return (uint64_t)next | ((uint64_t)((len - 1) & 0x7FFu) << 32) | ((uint64_t)((pos - slot) & 0xFFFu) << 43);
+    return (uint64_t)next | ((uint64_t)((len - 1) & 0xFFFu) << 32) | ((uint64_t)((pos - slot) & 0x1FFFu) << 44);
 }
 __device__ __forceinline__ uint32_t rec_next(uint64_t r) { return (uint32_t)r; }
-__device__ __forceinline__ uint32_t rec_len(uint64_t r) { return (uint32_t)((r >> 32) & 0x7FFu) + 1u; }
+__device__ __forceinline__ uint32_t rec_len(uint64_t r) { return (uint32_t)((r >> 32) & 0xFFFu) + 1u; }
 __device__ __forceinline__ uint32_t rec_pos(uint64_t r, uint32_t slot)
 {
-    return slot + (uint32_t)(((int32_t)((uint32_t)(r >> 43) & 0xFFFu) << 20) >> 20);
+    return slot + (uint32_t)(((int32_t)((uint32_t)(r >> 44) & 0x1FFFu) << 19) >> 19);
 }
 
 __device__ __forceinline__ uint32_t fastrange(uint32_t h, uint32_t cap)
@@ -1263,12 +1264,12 @@ __host__ __device__ constexpr size_t edge_lds_bytes(uint32_t nb)
     return (size_t)nb * (8 * W + 8 + 2 + 1) + ((size_t)nb / 32 + 2) * 4;
 }
 
-template <int W, int EB>
+template <int W, int EB, int GMAX>
 __global__ __launch_bounds__(EB) void k_edge_lds(const uint64_t *__restrict__ kmer, const EGroup *__restrict__ groups,
                                                     uint32_t nb, uint32_t k, uint64_t pm0, uint64_t pm1,
                                                     uint64_t *__restrict__ link, uint32_t *__restrict__ hasp, RunOut ro)
 {
-    constexpr int EPT = LDS_GROUP / EB;  // slots per thread (a group holds at most LDS_GROUP)
+    constexpr int EPT = GMAX / EB;  // slots per thread (a group holds at most GMAX)
     const uint8_t *flags = ro.flags;
     __shared__ uint32_t wsum[EB / 64];
     extern __shared__ __align__(16) unsigned char lds[];
@@ -2192,12 +2193,12 @@ struct uc_ctx {
     std::vector<uint64_t> et_toff, et_first;
     std::vector<uint32_t> et_cap;
     std::vector<uint8_t> bucket_done;
-    std::vector<size_t> grp_first, fb_first;  // per coarse bucket: first LDS group / oversized fine bucket
+    std::vector<size_t> grp_first, fb_first, bg_first;  // per coarse bucket: first LDS group / HBM-table / big fine bucket
     uint32_t lds_nb = 64;                     // LDS capacity of the edge launch
     bool rows_valid = false;                  // boff_h / fboff_h mirror the device rows
     uint32_t lds_limit = 65536;               // dynamic LDS a workgroup may use on this device
     uint32_t tile1 = 0, tile2 = 0;            // k-mers per bucketing tile, slots per fine-sort tile
-    uint64_t n_fallback = 0, n_groups = 0;
+    uint64_t n_fallback = 0, n_groups = 0, n_big = 0;
     uint64_t table_slots = 0, n_done = 0;
     uint64_t n_unitigs = 0, n_cycles = 0, text_bytes = 0, n_splitters = 0, n_runs = 0;
     uint32_t chain_rounds = 0;
@@ -2209,7 +2210,7 @@ struct uc_ctx {
     DBuf otable, in, stage, bkt, tiles, hist, xhist, kmer2, flags2, src2, sub, perm, tiles2, hist2, cbhb, cbts,
         cbfirst, fboff, egroups, fbbase, scan_tmp, kmer, flags, src, pair_h, pair_b, boff, sethb, setts, etiles,
         toff, tcap, table, hit, link, hasp, pred, seg, prevseg, chainA, chainB, cycB, list, clist, ulen, tsum,
-        htoff, setoff, text, small, txt, txtcnt, rec, rch, lhead, tcnt, cbtf, over, setplan, colsum;
+        htoff, setoff, text, small, txt, txtcnt, rec, rch, lhead, tcnt, cbtf, over, setplan, colsum, bgroups;
     // pinned host chunks of the file reader and writer
     char *pin[2] = {nullptr, nullptr};
     hipEvent_t pin_ev[2] = {nullptr, nullptr};
@@ -2241,7 +2242,7 @@ static std::vector<DBuf *> all_bufs(uc_ctx *c)
             &c->hasp,   &c->pred,    &c->seg,    &c->prevseg, &c->chainA, &c->chainB,
             &c->cycB,   &c->list,    &c->clist,  &c->ulen,    &c->tsum,    &c->htoff,  &c->setoff,
             &c->text,   &c->small,   &c->txt,    &c->txtcnt,  &c->rec,     &c->rch,    &c->lhead,
-            &c->tcnt,   &c->cbtf,    &c->over,   &c->setplan, &c->colsum};
+            &c->tcnt,   &c->cbtf,    &c->over,   &c->setplan, &c->colsum, &c->bgroups};
 }
 
 static void free_pinned(uc_ctx *c)
@@ -3016,8 +3017,22 @@ int launch_lds_edges(uc_ctx *c, size_t first, size_t count)
     // of its phases (measured: one-word k-mers 1.07 -> 0.81 ms at 512 threads,
     // two-word 11.4 -> 6.1 ms at 1024, against 256)
     constexpr int EB = W == 1 ? 512 : 1024;
-    KLAUNCH(c, "k_edge_lds", (k_edge_lds<W, EB>), ((uint32_t)count, EB, edge_lds_bytes<W>(c->lds_nb)),
+    KLAUNCH(c, "k_edge_lds", (k_edge_lds<W, EB, LDS_GROUP>), ((uint32_t)count, EB, edge_lds_bytes<W>(c->lds_nb)),
             P<uint64_t>(c->kmer), P<EGroup>(c->egroups) + first, c->lds_nb, c->k, c->pm0, c->pm1,
+            P<uint64_t>(c->link), P<uint32_t>(c->hasp), run_out(c));
+    return UC_OK;
+}
+
+// single fine buckets above the group size but at most BIG_GROUP slots: one
+// 1024-thread workgroup each, with the LDS area of BIG_GROUP slots
+template <int W>
+int launch_big_edges(uc_ctx *c, size_t first, size_t count)
+{
+    if (count == 0) return UC_OK;
+    const size_t lds = edge_lds_bytes<W>(BIG_GROUP);
+    CHK(allow_lds(c, k_edge_lds<W, 1024, BIG_GROUP>, lds));
+    KLAUNCH(c, "k_edge_lds_big", (k_edge_lds<W, 1024, BIG_GROUP>), ((uint32_t)count, 1024, lds),
+            P<uint64_t>(c->kmer), P<EGroup>(c->bgroups) + first, BIG_GROUP, c->k, c->pm0, c->pm1,
             P<uint64_t>(c->link), P<uint32_t>(c->hasp), run_out(c));
     return UC_OK;
 }
@@ -3049,22 +3064,41 @@ int prepare_edges(uc_ctx *c)
     CHK(d2h(c, &n_over, cnt + C_OVER, 4));
     c->lds_nb = group_max;
     c->grp_first.assign(c->cb_first.begin(), c->cb_first.end());
-    // oversized fine buckets (rare): their rows from the host mirror
+    // oversized fine buckets: their rows from the host mirror.  Those of at
+    // most BIG_GROUP slots run alone in a larger LDS area; the rest get HBM
+    // tables.
     std::vector<uint32_t> fb_base;
     std::vector<uint64_t> fb_cnt;
+    std::vector<EGroup> big;
     c->fb_first.assign(G + 1, 0);
+    c->bg_first.assign(G + 1, 0);
     if (n_over) {
         std::vector<uint32_t> ov(n_over);
         CHK(d2h(c, ov.data(), c->over.p, n_over * 4ull));
         std::sort(ov.begin(), ov.end());
         CHK(host_rows(c));
+        const size_t big_lds = (size_t)BIG_GROUP * (8 * c->W + 11) + ((size_t)BIG_GROUP / 32 + 2) * 4;
+        const bool big_ok = big_lds <= c->lds_limit;
+        std::vector<uint32_t> ovh, ovb;  // HBM-table and big-LDS fine buckets
         for (uint32_t f : ov) {
-            fb_base.push_back(c->fboff_h[f]);
-            fb_cnt.push_back(c->fboff_h[f + 1] - c->fboff_h[f]);
+            const uint64_t sz = c->fboff_h[f + 1] - c->fboff_h[f];
+            if (big_ok && sz <= BIG_GROUP) {
+                ovb.push_back(f);
+                big.push_back(EGroup{c->fboff_h[f], c->fboff_h[f + 1]});
+            } else {
+                ovh.push_back(f);
+                fb_base.push_back(c->fboff_h[f]);
+                fb_cnt.push_back(sz);
+            }
         }
-        for (uint64_t g = 0; g <= G; g++)
-            c->fb_first[g] = (size_t)(std::lower_bound(ov.begin(), ov.end(), c->cb_first[g]) - ov.begin());
+        for (uint64_t g = 0; g <= G; g++) {
+            c->fb_first[g] = (size_t)(std::lower_bound(ovh.begin(), ovh.end(), c->cb_first[g]) - ovh.begin());
+            c->bg_first[g] = (size_t)(std::lower_bound(ovb.begin(), ovb.end(), c->cb_first[g]) - ovb.begin());
+        }
     }
+    CHK(ensure(c, c->bgroups, std::max<size_t>(big.size(), 1) * sizeof(EGroup)));
+    if (!big.empty()) CHK(h2d(c, c->bgroups.p, big.data(), big.size() * sizeof(EGroup)));
+    c->n_big = big.size();
 
     // HBM tables for oversized fine buckets; plain form shares one table
     // sized for the largest of them
@@ -3149,11 +3183,13 @@ int run_edges(uc_ctx *c, uint32_t bucket)
         CHK(fill<uint32_t>(c, P<uint32_t>(c->hasp), (c->m + 31) / 32, 0u));
         if (!plain) {
             CHK(launch_lds_edges<W>(c, 0, c->grp_first[G]));
+            CHK(launch_big_edges<W>(c, 0, c->bg_first[G]));
             CHK(run_fallback<W>(c, 0, c->fb_first[G]));
         } else {
             // the reference's per-bucket loop (driver.cpp:479-521)
             for (uint64_t g = 0; g < G; g++) {
                 CHK(launch_lds_edges<W>(c, c->grp_first[g], c->grp_first[g + 1] - c->grp_first[g]));
+                CHK(launch_big_edges<W>(c, c->bg_first[g], c->bg_first[g + 1] - c->bg_first[g]));
                 CHK(run_fallback<W>(c, c->fb_first[g], c->fb_first[g + 1]));
             }
         }
@@ -3164,6 +3200,7 @@ int run_edges(uc_ctx *c, uint32_t bucket)
         // this bucket sets only its own slots' has-predecessor bits, which
         // start cleared from prepare_edges
         CHK(launch_lds_edges<W>(c, c->grp_first[bucket], c->grp_first[bucket + 1] - c->grp_first[bucket]));
+        CHK(launch_big_edges<W>(c, c->bg_first[bucket], c->bg_first[bucket + 1] - c->bg_first[bucket]));
         CHK(run_fallback<W>(c, c->fb_first[bucket], c->fb_first[bucket + 1]));
         if (!c->bucket_done[bucket]) {
             c->bucket_done[bucket] = 1;

@@ -35,6 +35,7 @@ constexpr int EDGE_TILE = 1024;               // slots per edge-finding tile
 constexpr uint32_t MAX_BUCKETS = 1024;        // coarse buckets per set
 constexpr uint32_t MAX_SUB = 1024;            // fine buckets per coarse bucket
 constexpr uint32_t LDS_GROUP = 2048;          // slots per LDS edge-finding group
+constexpr uint32_t BIG_GROUP = 4096;          // a lone fine bucket up to this size still runs in LDS
 constexpr uint32_t DEFAULT_SAMPLE_LOG2 = 9;   // splitter sampling 2^-9 of the run heads
 
 struct BTile {              // bucketing tile (one workgroup)
