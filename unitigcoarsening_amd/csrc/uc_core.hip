@@ -464,24 +464,27 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t *a, uint32_t n, uin
 // STAGE, the tile's placements are gathered in LDS in bucket order and leave
 // as runs of consecutive slots; a tile with more placements than the stage
 // holds (more than a quarter of its k-mers crossing) stores them directly.
-__host__ __device__ constexpr size_t lmin_lds_bytes(uint32_t B)
+__host__ __device__ constexpr size_t lmin_lds_bytes(uint32_t B, uint32_t nw = WAVES)
 {
-    return ((size_t)(B + 1) * (WAVES + 1) + 2 * (size_t)B) * 4;
+    return ((size_t)(B + 1) * (nw + 1) + 2 * (size_t)B) * 4;
 }
 
 // placements one pass-1 tile may stage: its k-mers plus a quarter
 __host__ __device__ constexpr uint32_t lmin_stage_cap(uint32_t tile) { return tile + tile / 4; }
 
-__host__ __device__ constexpr size_t lmin_stage_off(uint32_t B) { return (lmin_lds_bytes(B) + 15) & ~(size_t)15; }
-
-template <int W>
-__host__ __device__ constexpr size_t lmin_stage_bytes(uint32_t B, uint32_t tile, bool keep)
+__host__ __device__ constexpr size_t lmin_stage_off(uint32_t B, uint32_t nw = WAVES)
 {
-    return lmin_stage_off(B) + (size_t)lmin_stage_cap(tile) * (8 * W + 2 + 2 + 1 + (keep ? 4 : 0));
+    return (lmin_lds_bytes(B, nw) + 15) & ~(size_t)15;
 }
 
-template <int W, int IPT, bool STAGE>
-__global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restrict__ kin, const BTile *__restrict__ tiles,
+template <int W>
+__host__ __device__ constexpr size_t lmin_stage_bytes(uint32_t B, uint32_t tile, bool keep, uint32_t nw = WAVES)
+{
+    return lmin_stage_off(B, nw) + (size_t)lmin_stage_cap(tile) * (8 * W + 2 + 2 + 1 + (keep ? 4 : 0));
+}
+
+template <int W, int IPT, bool STAGE, int NT>
+__global__ __launch_bounds__(NT) void k_lmin_scatter(const uint64_t *__restrict__ kin, const BTile *__restrict__ tiles,
                                                         uint32_t B, uint32_t nbits, uint64_t top_mask,
                                                         const uint2 *__restrict__ bkt,
                                                         const uint32_t *__restrict__ hscan,
@@ -491,12 +494,13 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restri
                                                         uint32_t *__restrict__ pair_borrow,
                                                         uint16_t *__restrict__ sub, uint32_t n_tiles)
 {
+    constexpr int NW = NT / 64;
     extern __shared__ __align__(16) unsigned char lds[];
-    __shared__ uint32_t wsum[WAVES];
+    __shared__ uint32_t wsum[NW];
     const bool keep = src != nullptr;
     uint32_t *run = reinterpret_cast<uint32_t *>(lds);  // B+1 running counts
-    uint32_t *wcnt = run + (B + 1);                      // WAVES x (B+1)
-    uint32_t *tstart = wcnt + WAVES * (B + 1);
+    uint32_t *wcnt = run + (B + 1);                      // NW x (B+1)
+    uint32_t *tstart = wcnt + NW * (B + 1);
     uint32_t *delta = tstart + B;
     const uint32_t ti = xcd_tile(n_tiles);
     if (ti == NONE) return;
@@ -506,31 +510,31 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restri
     Kmer<W> xs[IPT];
 #pragma unroll
     for (int j = 0; j < IPT; j++) {
-        const uint32_t idx = (uint32_t)j * BLOCK + threadIdx.x;
+        const uint32_t idx = (uint32_t)j * NT + threadIdx.x;
         codes[j] = make_uint2(0u, 0u);
         if (idx < t.count) {
             codes[j] = bkt[t.kbegin + idx];
             xs[j] = load_kmer<W>(kin, t.kbegin + idx, top_mask);
         }
     }
-    for (uint32_t i = threadIdx.x; i < (B + 1) * (WAVES + 1); i += BLOCK) run[i] = 0;
+    for (uint32_t i = threadIdx.x; i < (B + 1) * (NW + 1); i += NT) run[i] = 0;
     // this tile's count per bucket is the next histogram entry minus its own
-    for (uint32_t v = threadIdx.x; v < B; v += BLOCK) {
+    for (uint32_t v = threadIdx.x; v < B; v += NT) {
         const uint64_t hi = t.hbase + (uint64_t)v * t.ts + t.tlocal;
         const uint32_t g0 = hscan[hi];
         tstart[v] = hscan[hi + 1] - g0;
         delta[v] = g0;
     }
     __syncthreads();
-    block_excl_scan(tstart, B, wsum);
-    for (uint32_t v = threadIdx.x; v < B; v += BLOCK) delta[v] -= tstart[v];
+    block_excl_scan_n<NT>(tstart, B, wsum);
+    for (uint32_t v = threadIdx.x; v < B; v += NT) delta[v] -= tstart[v];
     __syncthreads();
     const int lane = __lane_id(), w = threadIdx.x / 64;
     const uint64_t lt = (1ull << lane) - 1ull, me = 1ull << lane;
     const uint32_t xbase = xscan[ti];
     // staged placements: k-mers, fine index, bucket, flags (and sources)
-    constexpr uint32_t CAP = lmin_stage_cap(IPT * BLOCK);
-    Kmer<W> *skm = reinterpret_cast<Kmer<W> *>(lds + lmin_stage_off(B));
+    constexpr uint32_t CAP = lmin_stage_cap(IPT * NT);
+    Kmer<W> *skm = reinterpret_cast<Kmer<W> *>(lds + lmin_stage_off(B, NW));
     uint16_t *ssub = reinterpret_cast<uint16_t *>(skm + CAP);
     uint16_t *sbk = ssub + CAP;
     uint8_t *sfl = reinterpret_cast<uint8_t *>(sbk + CAP);
@@ -539,8 +543,8 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restri
     const bool staged = STAGE && placed <= CAP;  // uniform over the block
 #pragma unroll
     for (int j = 0; j < IPT; j++) {
-        if ((uint32_t)j * BLOCK >= t.count) break;  // uniform over the block
-        const uint32_t idx = (uint32_t)j * BLOCK + threadIdx.x;
+        if ((uint32_t)j * NT >= t.count) break;  // uniform over the block
+        const uint32_t idx = (uint32_t)j * NT + threadIdx.x;
         const bool valid = idx < t.count;
         const uint64_t g = t.kbegin + idx;
         uint32_t bp = 0, bs = 0, fp = 0, fs = 0;
@@ -615,10 +619,10 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restri
             }
         }
         __syncthreads();
-        for (uint32_t v = threadIdx.x; v <= B; v += BLOCK) {
+        for (uint32_t v = threadIdx.x; v <= B; v += NT) {
             uint32_t a = 0;
 #pragma unroll
-            for (int q = 0; q < WAVES; q++) {
+            for (int q = 0; q < NW; q++) {
                 a += wcnt[q * (B + 1) + v];
                 wcnt[q * (B + 1) + v] = 0;
             }
@@ -629,7 +633,7 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_scatter(const uint64_t *__restri
     // staged: slot li of the tile's bucket-ordered placements goes to
     // delta[bucket] + li, so consecutive li of one bucket are consecutive slots
     if (staged) {
-        for (uint32_t li = threadIdx.x; li < placed; li += BLOCK) {
+        for (uint32_t li = threadIdx.x; li < placed; li += NT) {
             const uint32_t ns = delta[sbk[li]] + li;
             store_kmer<W>(kout, ns, skm[li]);
             flags[ns] = sfl[li];
@@ -854,20 +858,21 @@ __global__ __launch_bounds__(BLOCK) void k_sub_rows(const uint32_t *__restrict__
 // its stable position inside the tile, sorted by fine bucket.  The slots are
 // staged there in LDS and written out in that order, so consecutive lanes
 // store consecutive addresses of one fine bucket's run.
-__host__ __device__ constexpr size_t sub_book_bytes(uint32_t S)
+__host__ __device__ constexpr size_t sub_book_bytes(uint32_t S, uint32_t nw = WAVES)
 {
-    return (((size_t)WAVES * S + 2 * (size_t)S) * 4 + 15) / 16 * 16;
+    return (((size_t)nw * S + 2 * (size_t)S) * 4 + 15) / 16 * 16;
 }
 
 template <int W>
-__host__ __device__ constexpr size_t sub_lds_bytes(uint32_t tile, uint32_t S, bool src, bool stage)
+__host__ __device__ constexpr size_t sub_lds_bytes(uint32_t tile, uint32_t S, bool src, bool stage, uint32_t nw = WAVES)
 {
-    return sub_book_bytes(S) + (stage ? (size_t)tile * 8 * W + (src ? (size_t)tile * 4 : 0) + (size_t)tile * 2 + tile : 0);
+    return sub_book_bytes(S, nw) +
+           (stage ? (size_t)tile * 8 * W + (src ? (size_t)tile * 4 : 0) + (size_t)tile * 2 + tile : 0);
 }
 
 
-template <int W, bool STAGE, int IPT>
-__global__ __launch_bounds__(BLOCK) void k_sub_scatter(const BTile *__restrict__ tiles, const uint16_t *__restrict__ sub,
+template <int W, bool STAGE, int IPT, int NT>
+__global__ __launch_bounds__(NT) void k_sub_scatter(const BTile *__restrict__ tiles, const uint16_t *__restrict__ sub,
                                                        const uint32_t *__restrict__ hscan,
                                                        const uint32_t *__restrict__ fboff,
                                                        const uint32_t *__restrict__ cb_first, const uint64_t *__restrict__ kin,
@@ -876,9 +881,10 @@ __global__ __launch_bounds__(BLOCK) void k_sub_scatter(const BTile *__restrict__
                                                        uint32_t *__restrict__ sout, uint32_t *__restrict__ perm,
                                                        uint32_t n_tiles)
 {
-    constexpr uint32_t TILE = (uint32_t)IPT * BLOCK, CHUNK = TILE / WAVES;  // slots per tile / per wave
+    constexpr int NW = NT / 64;
+    constexpr uint32_t TILE = (uint32_t)IPT * NT, CHUNK = TILE / NW;  // slots per tile / per wave
     extern __shared__ __align__(16) unsigned char lds[];
-    __shared__ uint32_t wsum[WAVES];
+    __shared__ uint32_t wsum[NW];
     const uint32_t ti = xcd_tile(n_tiles);
     if (ti == NONE) return;
     const BTile t = tiles[ti];
@@ -886,11 +892,11 @@ __global__ __launch_bounds__(BLOCK) void k_sub_scatter(const BTile *__restrict__
     if (S == 0) return;  // past the last tile
     const uint32_t nbits = (uint32_t)__builtin_ctz(S);
     const bool src = sin != nullptr;
-    uint32_t *wc = reinterpret_cast<uint32_t *>(lds);  // WAVES x S
-    uint32_t *tstart = wc + WAVES * S;
+    uint32_t *wc = reinterpret_cast<uint32_t *>(lds);  // NW x S
+    uint32_t *tstart = wc + NW * S;
     uint32_t *delta = tstart + S;
     // staging (STAGE): slots sorted by fine bucket
-    unsigned char *st = lds + sub_book_bytes(S);
+    unsigned char *st = lds + sub_book_bytes(S, NW);
     Kmer<W> *skm = reinterpret_cast<Kmer<W> *>(st);
     uint32_t *ssrc = reinterpret_cast<uint32_t *>(st + (size_t)TILE * 8 * W);
     uint16_t *ssub = reinterpret_cast<uint16_t *>(st + (size_t)TILE * 8 * W + (src ? (size_t)TILE * 4 : 0));
@@ -916,7 +922,7 @@ __global__ __launch_bounds__(BLOCK) void k_sub_scatter(const BTile *__restrict__
             if (src) sr[j] = sin[s];
         }
     }
-    for (uint32_t i = threadIdx.x; i < WAVES * S; i += BLOCK) wc[i] = 0;
+    for (uint32_t i = threadIdx.x; i < NW * S; i += NT) wc[i] = 0;
     __syncthreads();
     // 1. per-wave counts
 #pragma unroll
@@ -924,18 +930,18 @@ __global__ __launch_bounds__(BLOCK) void k_sub_scatter(const BTile *__restrict__
         if (c0 + 64u * j + (uint32_t)lane < t.count) atomicAdd(&mywc[vf[j] & 0xFFFFu], 1u);
     __syncthreads();
     // 2. tile starts per fine bucket; per-wave bases; global minus local offsets
-    for (uint32_t v = threadIdx.x; v < S; v += BLOCK) {
+    for (uint32_t v = threadIdx.x; v < S; v += NT) {
         uint32_t tot = 0;
 #pragma unroll
-        for (int q = 0; q < WAVES; q++) tot += wc[q * S + v];
+        for (int q = 0; q < NW; q++) tot += wc[q * S + v];
         tstart[v] = tot;
     }
     __syncthreads();
-    block_excl_scan(tstart, S, wsum);
-    for (uint32_t v = threadIdx.x; v < S; v += BLOCK) {
+    block_excl_scan_n<NT>(tstart, S, wsum);
+    for (uint32_t v = threadIdx.x; v < S; v += NT) {
         uint32_t b = tstart[v];
 #pragma unroll
-        for (int q = 0; q < WAVES; q++) {
+        for (int q = 0; q < NW; q++) {
             const uint32_t cq = wc[q * S + v];
             wc[q * S + v] = b;
             b += cq;
@@ -980,7 +986,7 @@ __global__ __launch_bounds__(BLOCK) void k_sub_scatter(const BTile *__restrict__
     // 4. write out in sorted order
     if constexpr (STAGE) {
         __syncthreads();
-        for (uint32_t li = threadIdx.x; li < t.count; li += BLOCK) {
+        for (uint32_t li = threadIdx.x; li < t.count; li += NT) {
             const uint32_t ns = delta[ssub[li]] + li;
             store_kmer<W>(kout, ns, skm[li]);
             fout[ns] = sfl[li];
@@ -2874,23 +2880,23 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     CHK(ensure(c, c->pair_b, c->x * 4));
     CHK(ensure(c, c->boff, (G + 1) * 4));
     const uint32_t nbits = (uint32_t)__builtin_ctz(B);
+    // 512 threads per 2048-k-mer tile (measured against 256 and 1024: the
+    // staged form holds four tiles per CU, and 8 waves each hide more latency)
+    constexpr int NT1 = 512;
     const bool stage1 = !(c->opts.flags & UC_FLAG_DIRECT_SCATTER) &&
-                        lmin_stage_bytes<W>(B, tile1, keep_src) <= c->lds_limit;
-    if (stage1) {
-        const size_t lds = lmin_stage_bytes<W>(B, tile1, keep_src);
-        CHK(allow_lds(c, k_lmin_scatter<W, IPT1, true>, lds));
-        KLAUNCH(c, "k_lmin_scatter", (k_lmin_scatter<W, IPT1, true>), (xcd_grid(T), BLOCK, lds), kin,
-                P<BTile>(c->tiles), B, nbits, c->top_mask, P<uint2>(c->bkt), P<uint32_t>(c->hist),
-                P<uint32_t>(c->xhist), P<uint64_t>(k1), P<uint8_t>(f1), keep_src ? P<uint32_t>(s1) : nullptr,
-                P<uint32_t>(c->pair_h), P<uint32_t>(c->pair_b), P<uint16_t>(c->sub), tile_arg_xcd(T));
-    } else {
-        const size_t lds = lmin_lds_bytes(B);
-        CHK(allow_lds(c, k_lmin_scatter<W, IPT1, false>, lds));
-        KLAUNCH(c, "k_lmin_scatter", (k_lmin_scatter<W, IPT1, false>), (xcd_grid(T), BLOCK, lds), kin,
-                P<BTile>(c->tiles), B, nbits, c->top_mask, P<uint2>(c->bkt), P<uint32_t>(c->hist),
-                P<uint32_t>(c->xhist), P<uint64_t>(k1), P<uint8_t>(f1), keep_src ? P<uint32_t>(s1) : nullptr,
-                P<uint32_t>(c->pair_h), P<uint32_t>(c->pair_b), P<uint16_t>(c->sub), tile_arg_xcd(T));
-    }
+                        lmin_stage_bytes<W>(B, tile1, keep_src, NT1 / 64) <= c->lds_limit;
+#define UC_S1(ST)                                                                                                   \
+    do {                                                                                                           \
+        const size_t lds = ST ? lmin_stage_bytes<W>(B, tile1, keep_src, NT1 / 64) : lmin_lds_bytes(B, NT1 / 64);      \
+        CHK(allow_lds(c, k_lmin_scatter<W, IPT1 * BLOCK / NT1, ST, NT1>, lds));                                     \
+        KLAUNCH(c, "k_lmin_scatter", (k_lmin_scatter<W, IPT1 * BLOCK / NT1, ST, NT1>), (xcd_grid(T), NT1, lds), kin, \
+                P<BTile>(c->tiles), B, nbits, c->top_mask, P<uint2>(c->bkt), P<uint32_t>(c->hist),                  \
+                P<uint32_t>(c->xhist), P<uint64_t>(k1), P<uint8_t>(f1), keep_src ? P<uint32_t>(s1) : nullptr,       \
+                P<uint32_t>(c->pair_h), P<uint32_t>(c->pair_b), P<uint16_t>(c->sub), tile_arg_xcd(T));             \
+    } while (0)
+    if (stage1) UC_S1(true);
+    else UC_S1(false);
+#undef UC_S1
     KLAUNCH(c, "k_gather_boff", k_gather_boff, (grid_for(G + 1), BLOCK, 0), P<uint32_t>(c->hist),
             P<uint64_t>(c->sethb), P<uint32_t>(c->setts), S, B, (uint32_t)c->m, P<uint32_t>(c->boff));
     c->rows_valid = false;
@@ -2915,8 +2921,11 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
         const uint32_t tile2 = IPT2 * BLOCK;
         // (two-word k-mers measured faster unstaged here: 4.9 vs 5.5 ms on
         // 303 M slots; one-word ones staged: 0.82 vs 0.85 ms on 53 M)
+        // threads per 2048-slot tile: 512 for the staged one-word form, 1024
+        // for the direct two-word form (each measured best of 256/512/1024)
+        constexpr int NT2 = W == 1 ? 512 : 1024;
         const bool stage2 = W == 1 && !(c->opts.flags & UC_FLAG_DIRECT_SCATTER) &&
-                            sub_lds_bytes<W>(tile2, smax, keep_src, true) <= c->lds_limit;
+                            sub_lds_bytes<W>(tile2, smax, keep_src, true, NT2 / 64) <= c->lds_limit;
         c->tile2 = tile2;
         // tiles: at most one partial tile per coarse bucket beyond m / tile2
         const uint64_t T2 = G + (c->m + tile2 - 1) / tile2;
@@ -2943,22 +2952,19 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
         CHK(ensure(c, c->flags, c->m + 64));
         if (keep_src) CHK(ensure(c, c->src, c->m * 4));
         CHK(ensure(c, c->perm, c->m * 4));
-        const size_t lds2 = sub_lds_bytes<W>(tile2, smax, keep_src, stage2);
-        if (stage2) {
-            CHK(allow_lds(c, k_sub_scatter<W, true, IPT2>, lds2));
-            KLAUNCH(c, "k_sub_scatter", (k_sub_scatter<W, true, IPT2>), (xcd_grid(T2), BLOCK, lds2),
-                    P<BTile>(c->tiles2), P<uint16_t>(c->sub), P<uint32_t>(c->hist2), P<uint32_t>(c->fboff),
-                    P<uint32_t>(c->cbfirst), P<uint64_t>(c->kmer2), P<uint8_t>(c->flags2),
-                    keep_src ? P<uint32_t>(c->src2) : nullptr, P<uint64_t>(c->kmer), P<uint8_t>(c->flags),
-                    keep_src ? P<uint32_t>(c->src) : nullptr, P<uint32_t>(c->perm), tile_arg_xcd(T2));
-        } else {
-            CHK(allow_lds(c, k_sub_scatter<W, false, IPT2>, lds2));
-            KLAUNCH(c, "k_sub_scatter", (k_sub_scatter<W, false, IPT2>), (xcd_grid(T2), BLOCK, lds2),
-                    P<BTile>(c->tiles2), P<uint16_t>(c->sub), P<uint32_t>(c->hist2), P<uint32_t>(c->fboff),
-                    P<uint32_t>(c->cbfirst), P<uint64_t>(c->kmer2), P<uint8_t>(c->flags2),
-                    keep_src ? P<uint32_t>(c->src2) : nullptr, P<uint64_t>(c->kmer), P<uint8_t>(c->flags),
-                    keep_src ? P<uint32_t>(c->src) : nullptr, P<uint32_t>(c->perm), tile_arg_xcd(T2));
-        }
+#define UC_S2(ST)                                                                                                   \
+    do {                                                                                                           \
+        const size_t lds2 = sub_lds_bytes<W>(tile2, smax, keep_src, ST, NT2 / 64);                                  \
+        CHK(allow_lds(c, k_sub_scatter<W, ST, IPT2 * BLOCK / NT2, NT2>, lds2));                                    \
+        KLAUNCH(c, "k_sub_scatter", (k_sub_scatter<W, ST, IPT2 * BLOCK / NT2, NT2>), (xcd_grid(T2), NT2, lds2),      \
+                P<BTile>(c->tiles2), P<uint16_t>(c->sub), P<uint32_t>(c->hist2), P<uint32_t>(c->fboff),             \
+                P<uint32_t>(c->cbfirst), P<uint64_t>(c->kmer2), P<uint8_t>(c->flags2),                              \
+                keep_src ? P<uint32_t>(c->src2) : nullptr, P<uint64_t>(c->kmer), P<uint8_t>(c->flags),               \
+                keep_src ? P<uint32_t>(c->src) : nullptr, P<uint32_t>(c->perm), tile_arg_xcd(T2));                  \
+    } while (0)
+        if (stage2) UC_S2(true);
+        else UC_S2(false);
+#undef UC_S2
         if (c->x)
             KLAUNCH(c, "k_pair_remap", k_pair_remap, (grid_for(c->x), BLOCK, 0), P<uint32_t>(c->pair_h),
                     P<uint32_t>(c->pair_b), (uint32_t)c->x, P<uint32_t>(c->perm));
