@@ -261,6 +261,10 @@ int uc_timing_get(uc_ctx *ctx, uc_timing *t);
  * stream.  Returns UC_E_ARG past the last kernel. */
 int uc_kernel_stat(uc_ctx *ctx, uint32_t idx, char *name, uint32_t name_cap, double *total_ms, uint64_t *launches);
 int uc_kernel_stats_reset(uc_ctx *ctx);
+/* With UC_FLAG_PROFILE: time only the launches of the named kernel (NULL or
+ * "" for every kernel again).  Two events per launch of one kernel disturb a
+ * timed run far less than two around every launch. */
+int uc_kernel_stats_select(uc_ctx *ctx, const char *name);
 
 /* Blocks until the context's stream is idle. */
 int uc_sync(uc_ctx *ctx);

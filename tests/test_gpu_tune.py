@@ -52,3 +52,26 @@ def test_autotune_keeps_bucket_count_when_asked():
         r = c.autotune(11, reps=1, keep_buckets=True)
         assert r["n_buckets"] == 16
         assert uc.sorted_lines(c.compact(kmers, 31, 11)) == uc.sorted_lines(oracle.textbook(kmers, 31)[0])
+
+
+def test_kernel_stats_select_times_one_kernel():
+    # every launch timed, then only the selected kernel; the text is unchanged
+    kmers = uc.medium_model(4, 200_000, 31)
+    with uc.Compactor(profile=True) as c:
+        c.load(kmers, 31)
+        c.compact_loaded(11)
+        want = c.text()
+        full = c.kernel_stats()
+        assert {"k_lmin_count", "k_edge_lds", "k_head_emit"} <= set(full)
+        c.kernel_stats_select("k_head_emit")
+        c.kernel_stats_reset()
+        for _ in range(2):
+            c.compact_loaded(11)
+        one = c.kernel_stats()
+        assert one["k_head_emit"][1] == 2 and one["k_head_emit"][0] > 0
+        assert all(nl == 0 for name, (ms, nl) in one.items() if name != "k_head_emit")
+        assert c.text() == want
+        c.kernel_stats_select(None)
+        c.kernel_stats_reset()
+        c.compact_loaded(11)
+        assert c.kernel_stats()["k_lmin_count"][1] == 1

@@ -160,6 +160,7 @@ API = {
     "uc_compact_loaded": (C.c_int, [_P, C.c_uint32]),
     "uc_kernel_stat": (C.c_int, [_P, C.c_uint32, C.c_char_p, C.c_uint32, C.POINTER(C.c_double), _U64P]),
     "uc_kernel_stats_reset": (C.c_int, [_P]),
+    "uc_kernel_stats_select": (C.c_int, [_P, C.c_char_p]),
     "uc_compact_sets": (C.c_int, [_P, _P, _U64P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int]),
     "uc_lmer_order_table": (C.c_int, [_P, C.c_uint32, _P, C.c_int]),
     "uc_load": (C.c_int, [_P, _P, _U64P, C.c_uint32, C.c_uint32, C.c_int]),
@@ -454,6 +455,10 @@ class Compactor:
 
     def kernel_stats_reset(self):
         _check(lib().uc_kernel_stats_reset(self._ctx), self._ctx)
+
+    def kernel_stats_select(self, name=None):
+        """Time only the named kernel (None: every kernel) from now on."""
+        _check(lib().uc_kernel_stats_select(self._ctx, (name or "").encode()), self._ctx)
 
     def bucketize(self, l, order_table=None):
         """order_table: None (inline seeded bijection) or a host array of 4^l orders."""

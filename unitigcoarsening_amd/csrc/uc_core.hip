@@ -2233,6 +2233,7 @@ struct uc_ctx {
         uint64_t launches = 0;
     };
     std::vector<KStat> kstats;
+    std::string prof_only;  // time only this kernel ("" for all)
     std::vector<std::pair<size_t, std::pair<hipEvent_t, hipEvent_t>>> pending;
     std::vector<hipEvent_t> evpool;
 };
@@ -2338,7 +2339,7 @@ struct Prof {
     hipEvent_t a = nullptr, b = nullptr;
     Prof(uc_ctx *ctx, const char *name) : c(ctx)
     {
-        if (!(c->opts.flags & UC_FLAG_PROFILE)) {
+        if (!(c->opts.flags & UC_FLAG_PROFILE) || (!c->prof_only.empty() && c->prof_only != name)) {
             c = nullptr;
             return;
         }
@@ -2359,7 +2360,8 @@ struct Prof {
         if (!c->evpool.empty()) {
             e = c->evpool.back();
             c->evpool.pop_back();
-        } else if (hipEventCreate(&e) != hipSuccess) {
+        } else if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) {
+            // timing only: no cache writeback and invalidation at every kernel boundary
             e = nullptr;
         }
         return e;
@@ -3567,6 +3569,14 @@ int uc_kernel_stat(uc_ctx *c, uint32_t idx, char *name, uint32_t name_cap, doubl
     }
     if (total_ms) *total_ms = k.ms;
     if (launches) *launches = k.launches;
+    return UC_OK;
+}
+
+int uc_kernel_stats_select(uc_ctx *c, const char *name)
+{
+    if (!c) return UC_E_ARG;
+    CHK(prof_flush(c));
+    c->prof_only = name ? name : "";
     return UC_OK;
 }
 
