@@ -63,17 +63,16 @@ def kernel_bytes(name, n, m, x, W, n_unitigs, text_bytes, k, runs):
         # read both pair entries and their permuted slots; write both
         "k_pair_remap": x * (8 + 8 + 8),
         # read k-mer + flag per slot; write link, flag and run character per
-        # slot, the has-predecessor bitmap, an 8-byte record per run and the
-        # run of every lent k-mer
-        "k_edge_lds": m * (kb + 1) + m * (8 + 1 + 1) + m // 8 + runs * 8 + x * 4,
+        # slot, the has-predecessor bitmap and an 8-byte record per run
+        "k_edge_lds": m * (kb + 1) + m * (8 + 1 + 1) + m // 8 + runs * 8,
         # fallback tables for oversized fine buckets (16-byte entries): the
         # same three steps over HBM
         "k_edge_insert": m * (1 + kb) + (m - x) * 16,
         "k_edge_lookup": m * (1 + kb + 4) + (m - x) * (16 + kb),
         "k_edge_resolve": m * (4 + 8) + (m - x) * 16 + m // 8,
-        # pair entries, the duplicate's link, the lent k-mer's run; write its
-        # link and its run record's next run
-        "k_splice": x * (4 + 4 + 8 + 4 + 8 + 4),
+        # pair entries, the duplicate's link, the lent k-mer's link (its run
+        # head); write that link and its run record's next run
+        "k_splice": x * (4 + 4 + 8 + 8 + 8 + 4),
         # read flag + has-predecessor bit per slot, one run record per run;
         # write the segment (length, next splitter) per head
         "k_seg_count": m * 1 + m // 8 + runs * 8 + n_unitigs * 8,

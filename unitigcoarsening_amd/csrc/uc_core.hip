@@ -1034,13 +1034,14 @@ constexpr uint64_t NO_LINK = (uint64_t)NONE;
 // live k-mer belongs to one run.  A run's record sits at its first k-mer's
 // slot (next run, length, where its characters are); rch holds the last
 // character of every run member in path order
-// (run-major inside the group's own slot range).  A lent k-mer ends its run;
-// lhead gives its run's first k-mer so that the splice can continue the run
-// through the duplicate's edge.
+// (run-major inside the group's own slot range).  A lent k-mer ends its run
+// and has no out-edge of its own: its link holds NONE in the low half and its
+// run's first k-mer in the high half, so that the splice can continue the run
+// through the duplicate's edge.  (Every other link is the successor in the
+// low half; nothing reads a link's high half but the splice.)
 struct RunOut {
     uint64_t *rec;
     char *rch;
-    uint32_t *lhead;
     uint8_t *flags;
 };
 
@@ -1201,15 +1202,15 @@ __global__ __launch_bounds__(BLOCK) void k_edge_resolve(const ETile *__restrict_
                 atomicOr(&hasp[v >> 5], 1u << (v & 31));
             }
         }
-        link[s] = l;
-        // an oversized fine bucket: every k-mer is a run of its own
+        // an oversized fine bucket: every k-mer is a run of its own (a lent
+        // one keeps itself as its run head in the link's high half)
         const uint8_t f = ro.flags[s];
+        link[s] = (f & F_LENT) ? (NO_LINK | ((uint64_t)s << 32)) : l;
         if (!(f & F_BORROW)) {
             ro.flags[s] = (uint8_t)(f | F_RUNHEAD);
             ro.rec[s] = make_rec((uint32_t)l, 1u, s, s);
             const uint64_t word = kmer[(uint64_t)s * W + (k - 1) / 32];
             ro.rch[s] = base_char((uint32_t)(word >> (2 * ((k - 1) & 31))) & 3u);
-            if (f & F_LENT) ro.lhead[s] = s;
         }
     }
 }
@@ -1479,7 +1480,7 @@ __global__ __launch_bounds__(EB) void k_edge_lds(const uint64_t *__restrict__ km
         ro.flags[g] = (uint8_t)(f | (a == i ? F_RUNHEAD : 0));
         if (a == i)
             ro.rec[g] = make_rec(cyc ? gr.begin + tgt[e] : NONE, cyc ? 1u : (uint32_t)rl[e], g, gr.begin + sc[i]);
-        if (f & F_LENT) ro.lhead[g] = gr.begin + a;
+        if (f & F_LENT) link[g] = NO_LINK | ((uint64_t)(gr.begin + a) << 32);
     }
 }
 
@@ -1502,15 +1503,16 @@ constexpr int CT_TILE = SCAN_TILE;  // slots per contraction tile
 // the borrowed duplicate's out-edge (and its target's character) moves to
 // the home copy; the target keeps its has-predecessor bit
 __global__ void k_splice(const uint32_t *__restrict__ pair_home, const uint32_t *__restrict__ pair_borrow, uint32_t x,
-                         uint64_t *__restrict__ link, const uint32_t *__restrict__ lhead, uint64_t *__restrict__ rec)
+                         uint64_t *__restrict__ link, uint64_t *__restrict__ rec)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= x) return;
     const uint32_t h = pair_home[i];
     const uint64_t l = link[pair_borrow[i]];
-    link[h] = l;
+    const uint64_t head = link[h] >> 32;  // the lent k-mer's run head
+    link[h] = (l & 0xFFFFFFFFull) | (head << 32);
     // a lent k-mer ends its run: the run continues through the duplicate's edge
-    reinterpret_cast<uint32_t *>(rec)[2ull * lhead[h]] = (uint32_t)l;  // the record's next-run field
+    reinterpret_cast<uint32_t *>(rec)[2ull * head] = (uint32_t)l;  // the record's next-run field
 }
 
 // list index for every flagged lane: one atomic per wave
@@ -2216,7 +2218,7 @@ struct uc_ctx {
     DBuf otable, in, stage, bkt, tiles, hist, xhist, kmer2, flags2, src2, sub, perm, tiles2, hist2, cbhb, cbts,
         cbfirst, fboff, egroups, fbbase, scan_tmp, kmer, flags, src, pair_h, pair_b, boff, sethb, setts, etiles,
         toff, tcap, table, hit, link, hasp, pred, seg, prevseg, chainA, chainB, cycB, list, clist, ulen, tsum,
-        htoff, setoff, text, small, txt, txtcnt, rec, rch, lhead, tcnt, cbtf, over, setplan, colsum, bgroups;
+        htoff, setoff, text, small, txt, txtcnt, rec, rch, tcnt, cbtf, over, setplan, colsum, bgroups;
     // pinned host chunks of the file reader and writer
     char *pin[2] = {nullptr, nullptr};
     hipEvent_t pin_ev[2] = {nullptr, nullptr};
@@ -2248,7 +2250,7 @@ static std::vector<DBuf *> all_bufs(uc_ctx *c)
             &c->setts,  &c->etiles,  &c->toff,   &c->tcap,    &c->table,   &c->hit,    &c->link,
             &c->hasp,   &c->pred,    &c->seg,    &c->prevseg, &c->chainA, &c->chainB,
             &c->cycB,   &c->list,    &c->clist,  &c->ulen,    &c->tsum,    &c->htoff,  &c->setoff,
-            &c->text,   &c->small,   &c->txt,    &c->txtcnt,  &c->rec,     &c->rch,    &c->lhead,
+            &c->text,   &c->small,   &c->txt,    &c->txtcnt,  &c->rec,     &c->rch,
             &c->tcnt,   &c->cbtf,    &c->over,   &c->setplan, &c->colsum, &c->bgroups};
 }
 
@@ -2328,7 +2330,7 @@ T *P(DBuf &b)
 
 static RunOut run_out(uc_ctx *c)
 {
-    return RunOut{P<uint64_t>(c->rec), P<char>(c->rch), P<uint32_t>(c->lhead), P<uint8_t>(c->flags)};
+    return RunOut{P<uint64_t>(c->rec), P<char>(c->rch), P<uint8_t>(c->flags)};
 }
 
 // Times one kernel launch with a pair of events on the context's stream
@@ -3154,7 +3156,6 @@ int prepare_edges(uc_ctx *c)
     CHK(ensure(c, c->hasp, (c->m + 31) / 32 * 4 + 8));
     CHK(ensure(c, c->rec, c->m * 8 + 16));
     CHK(ensure(c, c->rch, c->m + 16));
-    CHK(ensure(c, c->lhead, c->m * 4 + 16));
     // has-predecessor bits: every bucket sets only its own slots' bits
     CHK(fill<uint32_t>(c, P<uint32_t>(c->hasp), (c->m + 31) / 32, 0u));
     c->n_fallback = NF;
@@ -3353,7 +3354,7 @@ int do_coarsen(uc_ctx *c)
     c->shift = c->opts.sample_log2 ? c->opts.sample_log2 : DEFAULT_SAMPLE_LOG2;
     if (c->x) {
         KLAUNCH(c, "k_splice", k_splice, (grid_for(c->x), BLOCK, 0), P<uint32_t>(c->pair_h), P<uint32_t>(c->pair_b),
-                (uint32_t)c->x, P<uint64_t>(c->link), P<uint32_t>(c->lhead), P<uint64_t>(c->rec));
+                (uint32_t)c->x, P<uint64_t>(c->link), P<uint64_t>(c->rec));
     }
     uint64_t live = 0;
     uint32_t nonconv = 0;
