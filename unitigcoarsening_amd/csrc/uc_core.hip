@@ -33,6 +33,7 @@
 #include <cstring>
 #include <fstream>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/uc_api.h"
@@ -198,33 +199,36 @@ __device__ __forceinline__ uint32_t lmer_order(uint32_t code, const LminParams &
 
 // lmin_buckets.h:67-94: minimum order over the l-mers of the (k-1)-prefix
 // (o1) and of the (k-1)-suffix (o2); the l-mer code keeps its first
-// character most significant.  Characters are read 16 to a 32-bit word; the
-// loop over a word's characters keeps a run-time (uniform) trip count, so
+// character most significant.  Both share the middle l-mers: o1 = min(first,
+// middle), o2 = min(middle, last).  Characters are read 16 to a 32-bit word;
+// the loop over a word's characters keeps a run-time (uniform) trip count, so
 // the code stays small whatever k is.
 template <int W, int MODE>
 __device__ __forceinline__ void double_lmin(const Kmer<W> &x, const LminParams &p, uint32_t &o1, uint32_t &o2)
 {
-    uint32_t lmer = 0;
-    o1 = NONE;
-    o2 = NONE;
     const uint32_t k = p.k, l = p.l;
+    uint32_t lmer = 0, mid = NONE;
+    // characters [i0, i1) into the rolling code; hash (into mid) when HASH
+    auto scan = [&](uint32_t i0, uint32_t i1, auto hash) {
 #pragma unroll
-    for (int q = 0; q < 2 * W; q++) {
-        const uint32_t i0 = 16u * q;
-        if (i0 >= k) break;  // uniform
-        const uint32_t w = (uint32_t)(x.w[q / 2] >> (32 * (q % 2)));
-        const uint32_t n = k - i0 < 16u ? k - i0 : 16u;
+        for (int q = 0; q < 2 * W; q++) {
+            const uint32_t b = 16u * q;
+            const uint32_t lo = i0 > b ? i0 : b, hi = i1 < b + 16u ? i1 : b + 16u;
+            const uint32_t w = (uint32_t)(x.w[q / 2] >> (32 * (q % 2)));
 #pragma unroll 1
-        for (uint32_t j = 0; j < n; j++) {
-            const uint32_t i = i0 + j;
-            lmer = ((lmer << 2) | ((w >> (2 * j)) & 3u)) & p.lmask;
-            if (i + 1 >= l) {  // uniform
-                const uint32_t o = lmer_order_m<MODE>(lmer, p);
-                if (i + 1 < k) o1 = min(o1, o);
-                if (i >= l) o2 = min(o2, o);
+            for (uint32_t i = lo; i < hi; i++) {  // uniform trip count
+                lmer = ((lmer << 2) | ((w >> (2 * (i - b))) & 3u)) & p.lmask;
+                if constexpr (decltype(hash)::value) mid = min(mid, lmer_order_m<MODE>(lmer, p));
             }
         }
-    }
+    };
+    scan(0, l, std::false_type{});
+    const uint32_t first = lmer_order_m<MODE>(lmer, p);
+    scan(l, k - 1, std::true_type{});
+    scan(k - 1, k, std::false_type{});
+    const uint32_t last = lmer_order_m<MODE>(lmer, p);
+    o1 = min(first, mid);
+    o2 = min(mid, last);
 }
 
 // 2-bit code -> 'A','C','G','T' (bytes 0x41, 0x43, 0x47, 0x54)
