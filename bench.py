@@ -315,7 +315,8 @@ def main():
     ap.add_argument("--fine-slots", type=int, default=0)
     ap.add_argument("--group-slots", type=int, default=0)
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-form comparison")
-    ap.add_argument("--cpu-sample-bp", type=int, default=25_000_000)
+    ap.add_argument("--cpu-sample-bp", type=int, default=50_000_000,
+                    help="genome size of the CPU baseline's sample (default: the whole medium workload, ~10 s)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sets", action="store_true", help="skip the 256-set batched vs plain comparison")
     ap.add_argument("--n-sets", type=int, default=256)
