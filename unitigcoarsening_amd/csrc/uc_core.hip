@@ -1253,8 +1253,8 @@ __global__ __launch_bounds__(BLOCK) void k_plan_groups(const uint32_t *__restric
 }
 
 // ---- the LDS form: one workgroup stages a group of consecutive fine buckets
-// (k-mers, flags) in LDS and runs insert, lookup and resolve against an LDS
-// table.  Keys of different fine buckets never match (equal (k-1)-mers have
+// (k-mers) in LDS and runs insert, lookup and resolve against an LDS table;
+// each thread keeps its own slots' flags and last characters in registers.  Keys of different fine buckets never match (equal (k-1)-mers have
 // equal minimizers), so a group shares one table.  HBM sees only the
 // sequential k-mer/flag reads and link writes plus one bitmap word per 32
 // slots.
@@ -1262,7 +1262,6 @@ __global__ __launch_bounds__(BLOCK) void k_plan_groups(const uint32_t *__restric
 //     km   Kmer<W>[nb]   staged k-mers
 //     tab  uint32[2nb]   owner (16 bits) | LDS_SHARED | hits << 24, or LDS_EMPTY
 //     hitp uint16[nb]    table position of each slot's suffix hit
-//     fl   uint8[nb]     staged flags
 //     hb   uint32[nb/32 + 2]  has-predecessor bits, aligned to global words
 
 constexpr uint32_t LDS_EMPTY = 0xFFFFFFFFu;
@@ -1272,7 +1271,7 @@ constexpr uint16_t LDS_NOHIT = 0xFFFFu;
 template <int W>
 __host__ __device__ constexpr size_t edge_lds_bytes(uint32_t nb)
 {
-    return (size_t)nb * (8 * W + 8 + 2 + 1) + ((size_t)nb / 32 + 2) * 4;
+    return (size_t)nb * (8 * W + 8 + 2) + ((size_t)nb / 32 + 2) * 4;
 }
 
 template <int W, int EB, int GMAX>
@@ -1287,8 +1286,7 @@ __global__ __launch_bounds__(EB) void k_edge_lds(const uint64_t *__restrict__ km
     Kmer<W> *km = reinterpret_cast<Kmer<W> *>(lds);
     uint32_t *tab = reinterpret_cast<uint32_t *>(lds + (size_t)nb * 8 * W);
     uint16_t *hitp = reinterpret_cast<uint16_t *>(lds + (size_t)nb * (8 * W + 8));
-    uint8_t *fl = lds + (size_t)nb * (8 * W + 10);
-    uint32_t *hb = reinterpret_cast<uint32_t *>(lds + (size_t)nb * (8 * W + 11));  // nb is a multiple of 64
+    uint32_t *hb = reinterpret_cast<uint32_t *>(lds + (size_t)nb * (8 * W + 10));  // nb is a multiple of 64
     const EGroup gr = groups[blockIdx.x];
     const uint32_t n = gr.end - gr.begin;
     if (n == 0) return;  // an unused group entry
@@ -1310,12 +1308,13 @@ __global__ __launch_bounds__(EB) void k_edge_lds(const uint64_t *__restrict__ km
             fs |= (uint64_t)flags[gr.begin + i] << (8 * e);
         }
     }
+    uint32_t lc = 0;  // last character of item e at bits 2e
 #pragma unroll
     for (int e = 0; e < EPT; e++) {
         const uint32_t i = threadIdx.x + (uint32_t)e * EB;
         if (i < n) {
             km[i] = xs[e];
-            fl[i] = (uint8_t)(fs >> (8 * e));
+            lc |= char_at<W>(xs[e], k - 1) << (2 * e);
         }
     }
     for (uint32_t i = threadIdx.x; i < tcap; i += EB) tab[i] = LDS_EMPTY;
@@ -1398,9 +1397,9 @@ __global__ __launch_bounds__(EB) void k_edge_lds(const uint64_t *__restrict__ km
             const uint32_t en = tab[hp];
             if (!(en & LDS_SHARED) && (en >> 24) == 1u) {
                 const uint32_t v = en & 0xFFFFu;
-                l = (uint64_t)(gr.begin + v) | ((uint64_t)char_at<W>(km[v], k - 1) << 32);
+                l = (uint64_t)(gr.begin + v);
                 atomicOr(&hb[(off0 + v) >> 5], 1u << ((off0 + v) & 31));
-                if (!(fl[i] & F_BORROW)) tgt[e] = (uint16_t)v;
+                if (!((fs >> (8 * e)) & F_BORROW)) tgt[e] = (uint16_t)v;
             }
         }
         link[gr.begin + i] = l;
@@ -1411,14 +1410,10 @@ __global__ __launch_bounds__(EB) void k_edge_lds(const uint64_t *__restrict__ km
         const uint32_t bits = hb[j];
         if (bits) atomicOr(&hasp[(gr.begin >> 5) + j], bits);
     }
-    // runs: run predecessor of every slot (hitp is free now); the last
-    // character of every slot moves into its flag byte (bits 5-6) so that the
-    // k-mer staging area is free
+    // runs: run predecessor of every slot (hitp is free now; the flags and
+    // last characters stay in registers, so the k-mer staging area is free)
     uint16_t *predl = hitp;
-    for (uint32_t i = threadIdx.x; i < n; i += EB) {
-        predl[i] = LDS_NOHIT;
-        fl[i] = (uint8_t)((fl[i] & 0x1Fu) | (char_at<W>(km[i], k - 1) << 5));
-    }
+    for (uint32_t i = threadIdx.x; i < n; i += EB) predl[i] = LDS_NOHIT;
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < EPT; e++)
@@ -1445,7 +1440,7 @@ __global__ __launch_bounds__(EB) void k_edge_lds(const uint64_t *__restrict__ km
     for (int e = 0; e < EPT; e++) {
         const uint32_t i = threadIdx.x + (uint32_t)e * EB;
         rl[e] = 0;
-        if (i >= n || (fl[i] & F_BORROW) || predl[i] != LDS_NOHIT) continue;
+        if (i >= n || ((fs >> (8 * e)) & F_BORROW) || predl[i] != LDS_NOHIT) continue;
         uint32_t v = i, d = 0;
         for (;;) {
             pd[v] = i | (d << 16);
@@ -1463,7 +1458,7 @@ __global__ __launch_bounds__(EB) void k_edge_lds(const uint64_t *__restrict__ km
 #pragma unroll
     for (int e = 0; e < EPT; e++) {
         const uint32_t i = threadIdx.x + (uint32_t)e * EB;
-        if (i < n && !(fl[i] & F_BORROW) && pd[i] == LDS_EMPTY) sc[i] = 1;
+        if (i < n && !((fs >> (8 * e)) & F_BORROW) && pd[i] == LDS_EMPTY) sc[i] = 1;
     }
     __syncthreads();
     block_excl_scan_n<EB>(sc, n, wsum);
@@ -1472,7 +1467,7 @@ __global__ __launch_bounds__(EB) void k_edge_lds(const uint64_t *__restrict__ km
         const uint32_t i = threadIdx.x + (uint32_t)e * EB;
         if (i >= n) continue;
         const uint32_t g = gr.begin + i;
-        const uint8_t f = (uint8_t)(fl[i] & 0x1Fu & ~F_RUNHEAD);
+        const uint8_t f = (uint8_t)((fs >> (8 * e)) & 0x1Fu & ~F_RUNHEAD);
         if (f & F_BORROW) {
             ro.flags[g] = f;
             continue;
@@ -1480,7 +1475,7 @@ __global__ __launch_bounds__(EB) void k_edge_lds(const uint64_t *__restrict__ km
         const uint32_t x = pd[i];
         const bool cyc = x == LDS_EMPTY;
         const uint32_t a = cyc ? i : (x & 0xFFFFu), d = cyc ? 0u : (x >> 16);
-        ro.rch[gr.begin + sc[a] + d] = base_char((fl[i] >> 5) & 3u);
+        ro.rch[gr.begin + sc[a] + d] = base_char((lc >> (2 * e)) & 3u);
         ro.flags[g] = (uint8_t)(f | (a == i ? F_RUNHEAD : 0));
         if (a == i)
             ro.rec[g] = make_rec(cyc ? gr.begin + tgt[e] : NONE, cyc ? 1u : (uint32_t)rl[e], g, gr.begin + sc[i]);
@@ -1496,16 +1491,15 @@ __global__ __launch_bounds__(EB) void k_edge_lds(const uint64_t *__restrict__ km
 // isolated cycles), each a chain of runs.  Splitters are the path heads plus
 // a hashed 2^-s sample of the other run heads; a segment is a splitter and
 // the runs up to the next splitter.  Segment lengths come from one walk per
-// splitter (one step per run: its length and the next run), chains of
-// segments are ranked by pointer jumping over the sampled splitters only, a
-// second walk per segment gives every run its text index, and every k-mer
-// then writes its own character.
+// splitter (one step per run: its length and the next run), and chains of
+// segments are ranked by pointer jumping over the sampled splitters only.
+// Emission then walks each segment again, copying its runs' characters.
 
 constexpr int CT_ITEMS = SCAN_ITEMS;
 constexpr int CT_TILE = SCAN_TILE;  // slots per contraction tile
 
-// the borrowed duplicate's out-edge (and its target's character) moves to
-// the home copy; the target keeps its has-predecessor bit
+// the borrowed duplicate's out-edge moves to the home copy; the target keeps
+// its has-predecessor bit
 __global__ void k_splice(const uint32_t *__restrict__ pair_home, const uint32_t *__restrict__ pair_borrow, uint32_t x,
                          uint64_t *__restrict__ link, uint64_t *__restrict__ rec)
 {
@@ -3091,7 +3085,7 @@ int prepare_edges(uc_ctx *c)
         CHK(d2h(c, ov.data(), c->over.p, n_over * 4ull));
         std::sort(ov.begin(), ov.end());
         CHK(host_rows(c));
-        const size_t big_lds = (size_t)BIG_GROUP * (8 * c->W + 11) + ((size_t)BIG_GROUP / 32 + 2) * 4;
+        const size_t big_lds = (size_t)BIG_GROUP * (8 * c->W + 10) + ((size_t)BIG_GROUP / 32 + 2) * 4;
         const bool big_ok = big_lds <= c->lds_limit;
         std::vector<uint32_t> ovh, ovb;  // HBM-table and big-LDS fine buckets
         for (uint32_t f : ov) {
