@@ -418,6 +418,7 @@ def main():
     kernels = {name: {"ms_per_step": round(ms / warm_steps, 4), "launches_per_step": nl / warm_steps}
                for name, (ms, nl) in sorted(ks.items(), key=lambda kv: -kv[1][0])}
     text_bytes = c.text_size()
+    device_gib = round(c.held_bytes() / 2 ** 30, 2)
     c.close()
 
     plain = None
@@ -464,7 +465,7 @@ def main():
             "text_bytes": text_bytes, "n_fine_buckets": b.n_fine_buckets,
             "largest_fine_bucket": b.largest_fine_bucket, "bucket_tile": b.bucket_tile, "fine_tile": b.fine_tile,
             "lds_limit": b.lds_limit, "n_oversized_fine": b.n_oversized, "oversized_slots": b.oversized_slots,
-            "n_splitters": ci.n_splitters, "chain_rounds": ci.chain_rounds,
+            "n_splitters": ci.n_splitters, "chain_rounds": ci.chain_rounds, "device_gib_held": device_gib,
             "n_runs": ci.n_runs,
         },
         "phases_ms_per_step": {kk: round(v / args.steps, 3) for kk, v in phase.items()},
