@@ -700,17 +700,19 @@ __global__ void k_gather_boff(const uint32_t *__restrict__ hscan, const uint64_t
 // counts contiguously.  Tiles past the last one are empty (fmask 0).
 
 // one workgroup: tiles and histogram entries per coarse bucket, prefixes
-__global__ __launch_bounds__(BLOCK) void k_plan2_prefix(const uint32_t *__restrict__ boff,
+constexpr int PLAN_NT = 1024;  // one workgroup walks all coarse buckets: the widest one
+
+__global__ __launch_bounds__(PLAN_NT) void k_plan2_prefix(const uint32_t *__restrict__ boff,
                                                         const uint32_t *__restrict__ cb_first, uint32_t G, uint32_t tile,
                                                         uint32_t *__restrict__ cb_ts, uint32_t *__restrict__ cb_tfirst,
                                                         uint64_t *__restrict__ cb_hbase)
 {
-    __shared__ uint64_t wsum[WAVES], hsum[WAVES];
+    __shared__ uint64_t wsum[PLAN_NT / 64], hsum[PLAN_NT / 64];
     __shared__ uint64_t tcarry, hcarry;
     if (threadIdx.x == 0) tcarry = hcarry = 0;
     __syncthreads();
     const int w = threadIdx.x / 64;
-    for (uint32_t g0 = 0; g0 < G; g0 += BLOCK) {
+    for (uint32_t g0 = 0; g0 < G; g0 += PLAN_NT) {
         const uint32_t g = g0 + threadIdx.x;
         uint64_t ts = 0, hs = 0;
         if (g < G) {
@@ -725,7 +727,7 @@ __global__ __launch_bounds__(BLOCK) void k_plan2_prefix(const uint32_t *__restri
         }
         __syncthreads();
         uint64_t tp = tcarry + ti - ts, hp = hcarry + hi - hs, tt = 0, ht = 0;
-        for (int q = 0; q < WAVES; q++) {
+        for (int q = 0; q < PLAN_NT / 64; q++) {
             if (q < w) {
                 tp += wsum[q];
                 hp += hsum[q];
@@ -2031,23 +2033,29 @@ __global__ void k_rand_emit(const uint64_t *__restrict__ rec, const char *__rest
 }
 
 // text offset of each set's first unitig: the prefix at the set's first slot
-__global__ void k_set_offsets(const uint32_t *__restrict__ boff, uint32_t n_sets, uint32_t B,
-                              const uint32_t *__restrict__ hasp, const uint8_t *__restrict__ flags,
-                              const uint64_t *__restrict__ seg, const uint32_t *__restrict__ ulen,
-                              const uint64_t *__restrict__ tile_off, uint32_t m,
-                              uint64_t total, uint64_t *__restrict__ out)
+// (one workgroup per set: the heads between the start of the set's first
+// slot's contraction tile and that slot)
+__global__ __launch_bounds__(BLOCK) void k_set_offsets(const uint32_t *__restrict__ boff, uint32_t n_sets,
+                                                       uint32_t B, const uint32_t *__restrict__ hasp,
+                                                       const uint8_t *__restrict__ flags,
+                                                       const uint64_t *__restrict__ seg,
+                                                       const uint32_t *__restrict__ ulen,
+                                                       const uint64_t *__restrict__ tile_off, uint32_t m,
+                                                       uint64_t total, uint64_t *__restrict__ out)
 {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s > n_sets) return;
+    __shared__ uint64_t red[WAVES];
+    const uint32_t s = blockIdx.x;
     const uint32_t sb = s < n_sets ? boff[(uint64_t)s * B] : m;
-    uint64_t acc = total;
-    if (sb < m) {
-        const uint32_t t = sb / CT_TILE;
-        acc = tile_off[t];
-        for (uint32_t x = t * CT_TILE; x < sb; x++)
-            if (live_head(flags, hasp, x)) acc += (1ull << 32) | unitig_len(seg, ulen, x);
+    if (sb >= m) {
+        if (threadIdx.x == 0) out[s] = total;
+        return;
     }
-    out[s] = acc;
+    const uint32_t t = sb / CT_TILE;
+    uint64_t acc = 0;
+    for (uint32_t x = t * CT_TILE + threadIdx.x; x < sb; x += BLOCK)
+        if (live_head(flags, hasp, x)) acc += (1ull << 32) | unitig_len(seg, ulen, x);
+    acc = block_reduce(acc, red);
+    if (threadIdx.x == 0) out[s] = tile_off[t] + acc;
 }
 
 // successor slot of every slot (for checking)
@@ -2937,7 +2945,7 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
         CHK(ensure(c, c->cbhb, (G + 1) * 8));
         CHK(ensure(c, c->cbts, (G + 1) * 4));
         CHK(ensure(c, c->cbtf, (G + 1) * 4));
-        KLAUNCH(c, "k_plan2_prefix", k_plan2_prefix, (1, BLOCK, 0), P<uint32_t>(c->boff), P<uint32_t>(c->cbfirst),
+        KLAUNCH(c, "k_plan2_prefix", k_plan2_prefix, (1, PLAN_NT, 0), P<uint32_t>(c->boff), P<uint32_t>(c->cbfirst),
                 (uint32_t)G, tile2, P<uint32_t>(c->cbts), P<uint32_t>(c->cbtf), P<uint64_t>(c->cbhb));
         KLAUNCH(c, "k_plan2_tiles", k_plan2_tiles, (grid_for(T2), BLOCK, 0), P<uint32_t>(c->boff),
                 P<uint32_t>(c->cbfirst), P<uint32_t>(c->cbts), P<uint32_t>(c->cbtf), P<uint64_t>(c->cbhb), (uint32_t)G,
@@ -3399,7 +3407,7 @@ int run_emit(uc_ctx *c)
     // per-set text ranges: a set's unitigs start at its first bucket row
     const uint64_t total = (c->n_unitigs << 32) | (c->text_bytes - c->n_unitigs * c->k);
     CHK(ensure(c, c->setoff, (c->n_sets + 1) * 8));
-    KLAUNCH(c, "k_set_offsets", k_set_offsets, (grid_for(c->n_sets + 1), BLOCK, 0), P<uint32_t>(c->boff), c->n_sets,
+    KLAUNCH(c, "k_set_offsets", k_set_offsets, (c->n_sets + 1, BLOCK, 0), P<uint32_t>(c->boff), c->n_sets,
             c->B, P<uint32_t>(c->hasp), P<uint8_t>(c->flags), P<uint64_t>(c->seg), P<uint32_t>(c->ulen),
             P<uint64_t>(c->tsum), m, total,
             P<uint64_t>(c->setoff));
