@@ -1222,41 +1222,85 @@ __global__ __launch_bounds__(BLOCK) void k_edge_resolve(const ETile *__restrict_
 }
 
 // LDS groups, planned on the device: one workgroup per coarse bucket; its
-// thread j takes the fine buckets 2j and 2j+1 as one group when they fit in
-// gmax slots together, else as a group each.  A coarse bucket's S entries of
-// the group array hold the pair groups first (entry j) and the second halves
-// of split pairs after them (entry S/2 + j), so that the busy workgroups are
-// consecutive (workgroups are dealt round-robin to the XCDs: alternating
-// busy and empty ones would load half of them).  Unused entries stay empty.
-// Fine buckets above gmax go to the oversized list (HBM tables).
-__global__ __launch_bounds__(BLOCK) void k_plan_groups(const uint32_t *__restrict__ fboff,
-                                                       const uint32_t *__restrict__ cb_first, uint32_t gmax,
-                                                       EGroup *__restrict__ groups, uint32_t *__restrict__ over,
-                                                       uint32_t *__restrict__ n_over)
+// thread j packs the fine buckets 8j .. 8j+7 in order into groups of at most
+// gmax slots (a group closes when the next fine bucket would overflow it).
+// k_plan_count counts each coarse bucket's groups (and lists the fine buckets
+// above gmax); after a scan of the counts k_plan_write lays every group out
+// in one dense list, so that the edge kernel's workgroups are all busy
+// (workgroups are dealt round-robin to the XCDs: runs of empty entries
+// between the coarse buckets had left some XCDs idle).
+constexpr uint32_t PLAN_RUN = 8;  // fine buckets per planning thread
+static_assert(MAX_SUB <= PLAN_RUN * BLOCK, "one run of fine buckets per planning thread");
+
+__device__ __forceinline__ uint32_t pack_groups(const uint32_t *__restrict__ fboff, uint32_t f0, uint32_t v0,
+                                                uint32_t v1, uint32_t gmax, EGroup *out, uint32_t *__restrict__ over,
+                                                uint32_t *__restrict__ n_over)
 {
-    const uint32_t g = blockIdx.x, f0 = cb_first[g], S = cb_first[g + 1] - f0;
-    for (uint32_t v = 2 * threadIdx.x; v < S; v += 2 * BLOCK) {
+    uint32_t ng = 0, gb = 0, ge = 0;
+    bool open = false;
+    for (uint32_t v = v0; v < v1; v++) {
         const uint32_t r0 = fboff[f0 + v], r1 = fboff[f0 + v + 1];
-        const bool two = v + 1 < S;
-        const uint32_t r2 = two ? fboff[f0 + v + 2] : r1;
-        const uint32_t a = r1 - r0, b = r2 - r1;
-        EGroup ga{0u, 0u}, gb{0u, 0u};
-        if (a > gmax) over[atomicAdd(n_over, 1u)] = f0 + v;
-        if (two && b > gmax) over[atomicAdd(n_over, 1u)] = f0 + v + 1;
-        if (a <= gmax && b <= gmax && a + b <= gmax) {
-            ga = EGroup{r0, r2};
-        } else {
-            if (a <= gmax) ga = EGroup{r0, r1};
-            if (b <= gmax) gb = EGroup{r1, r2};
+        if (r1 - r0 > gmax) {
+            if (over) over[atomicAdd(n_over, 1u)] = f0 + v;
+            if (open && out) out[ng] = EGroup{gb, ge};
+            ng += open;
+            open = false;
+            continue;
         }
-        groups[f0 + v / 2] = ga;
-        if (two) groups[f0 + S / 2 + v / 2] = gb;
+        if (open && r1 - gb <= gmax) {
+            ge = r1;
+        } else if (r1 == r0) {
+            continue;  // an empty fine bucket opens no group
+        } else {
+            if (open && out) out[ng] = EGroup{gb, ge};
+            ng += open;
+            gb = r0;
+            ge = r1;
+            open = true;
+        }
     }
+    if (open && out) out[ng] = EGroup{gb, ge};
+    return ng + open;
+}
+
+// per coarse bucket: its group count (block sum of the threads' counts)
+__global__ __launch_bounds__(BLOCK) void k_plan_count(const uint32_t *__restrict__ fboff,
+                                                      const uint32_t *__restrict__ cb_first, uint32_t gmax,
+                                                      uint32_t *__restrict__ gcount, uint32_t *__restrict__ over,
+                                                      uint32_t *__restrict__ n_over)
+{
+    __shared__ uint32_t red[WAVES];
+    const uint32_t g = blockIdx.x, f0 = cb_first[g], S = cb_first[g + 1] - f0;
+    // S <= MAX_SUB <= PLAN_RUN * BLOCK: one run of fine buckets per thread
+    const uint32_t v0 = PLAN_RUN * threadIdx.x, v1 = v0 + PLAN_RUN < S ? v0 + PLAN_RUN : S;
+    const uint32_t ng = v0 < S ? pack_groups(fboff, f0, v0, v1, gmax, nullptr, over, n_over) : 0u;
+    const uint32_t tot = block_reduce(ng, red);
+    if (threadIdx.x == 0) gcount[g] = tot;
+}
+
+// per coarse bucket: its groups from its scanned first entry
+__global__ __launch_bounds__(BLOCK) void k_plan_write(const uint32_t *__restrict__ fboff,
+                                                      const uint32_t *__restrict__ cb_first, uint32_t gmax,
+                                                      const uint32_t *__restrict__ gfirst,
+                                                      EGroup *__restrict__ groups)
+{
+    __shared__ uint32_t wsum[WAVES];
+    const uint32_t g = blockIdx.x, f0 = cb_first[g], S = cb_first[g + 1] - f0;
+    const uint32_t v0 = PLAN_RUN * threadIdx.x, v1 = v0 + PLAN_RUN < S ? v0 + PLAN_RUN : S;
+    const uint32_t ng = v0 < S ? pack_groups(fboff, f0, v0, v1, gmax, nullptr, nullptr, nullptr) : 0u;
+    const uint32_t incl = wave_incl_scan(ng);
+    const int w = threadIdx.x / 64;
+    if (__lane_id() == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t base = gfirst[g] + incl - ng;
+    for (int q = 0; q < w; q++) base += wsum[q];
+    if (ng) pack_groups(fboff, f0, v0, v1, gmax, groups + base, nullptr, nullptr);
 }
 
 // ---- the LDS form: one workgroup stages a group of consecutive fine buckets
 // (k-mers) in LDS and runs insert, lookup and resolve against an LDS table;
-// each thread keeps its own slots' flags and last characters in registers.  Keys of different fine buckets never match (equal (k-1)-mers have
+// each thread keeps its own slots' flags and last characters in registers.
+// Keys of different fine buckets never match (equal (k-1)-mers have
 // equal minimizers), so a group shares one table.  HBM sees only the
 // sequential k-mer/flag reads and link writes plus one bitmap word per 32
 // slots.
@@ -2224,7 +2268,7 @@ struct uc_ctx {
     DBuf otable, in, stage, bkt, tiles, hist, xhist, kmer2, flags2, src2, sub, perm, tiles2, hist2, cbhb, cbts,
         cbfirst, fboff, egroups, fbbase, scan_tmp, kmer, flags, src, pair_h, pair_b, boff, sethb, setts, etiles,
         toff, tcap, table, hit, link, hasp, pred, seg, prevseg, chainA, chainB, cycB, list, clist, ulen, tsum,
-        htoff, setoff, text, small, txt, txtcnt, rec, rch, tcnt, cbtf, over, setplan, colsum, bgroups;
+        htoff, setoff, text, small, txt, txtcnt, rec, rch, tcnt, cbtf, over, setplan, colsum, bgroups, gfirst;
     // pinned host chunks of the file reader and writer
     char *pin[2] = {nullptr, nullptr};
     hipEvent_t pin_ev[2] = {nullptr, nullptr};
@@ -2257,7 +2301,7 @@ static std::vector<DBuf *> all_bufs(uc_ctx *c)
             &c->hasp,   &c->pred,    &c->seg,    &c->prevseg, &c->chainA, &c->chainB,
             &c->cycB,   &c->list,    &c->clist,  &c->ulen,    &c->tsum,    &c->htoff,  &c->setoff,
             &c->text,   &c->small,   &c->txt,    &c->txtcnt,  &c->rec,     &c->rch,
-            &c->tcnt,   &c->cbtf,    &c->over,   &c->setplan, &c->colsum, &c->bgroups};
+            &c->tcnt,   &c->cbtf,    &c->over,   &c->setplan, &c->colsum, &c->bgroups, &c->gfirst};
 }
 
 static void free_pinned(uc_ctx *c)
@@ -3066,20 +3110,28 @@ int prepare_edges(uc_ctx *c)
     const bool plain = c->opts.mode == UC_MODE_PLAIN;
     const uint32_t group_max = c->opts.group_slots ? c->opts.group_slots : LDS_GROUP;
     const uint64_t F = c->cb_first[G];
-    // groups planned on the device, indexed like the fine buckets: coarse
-    // bucket g's groups are entries [cb_first[g], cb_first[g + 1])
+    // groups planned on the device in one dense list: coarse bucket g's
+    // groups are entries [gfirst[g], gfirst[g + 1])
     CHK(ensure(c, c->egroups, std::max<uint64_t>(F, 1) * sizeof(EGroup)));
     CHK(ensure(c, c->over, F * 4 + 4));
+    CHK(ensure(c, c->gfirst, (G + 1) * 4));
     CHK(ensure(c, c->small, C_WORDS * 4));
     uint32_t *cnt = P<uint32_t>(c->small);
     CHK(fill<uint32_t>(c, cnt + C_OVER, 1, 0u));
-    if (G)
-        KLAUNCH(c, "k_plan_groups", k_plan_groups, ((uint32_t)G, BLOCK, 0), P<uint32_t>(c->fboff),
-                P<uint32_t>(c->cbfirst), group_max, P<EGroup>(c->egroups), P<uint32_t>(c->over), cnt + C_OVER);
+    std::vector<uint32_t> gf(G + 1, 0);
+    if (G) {
+        uint32_t *gfd = P<uint32_t>(c->gfirst);
+        KLAUNCH(c, "k_plan_count", k_plan_count, ((uint32_t)G, BLOCK, 0), P<uint32_t>(c->fboff),
+                P<uint32_t>(c->cbfirst), group_max, gfd, P<uint32_t>(c->over), cnt + C_OVER);
+        CHK(scan_excl<uint32_t>(c, gfd, gfd, G, gfd + G, c->scan_tmp));
+        KLAUNCH(c, "k_plan_write", k_plan_write, ((uint32_t)G, BLOCK, 0), P<uint32_t>(c->fboff),
+                P<uint32_t>(c->cbfirst), group_max, gfd, P<EGroup>(c->egroups));
+        CHK(d2h(c, gf.data(), gfd, (G + 1) * 4));
+    }
     uint32_t n_over = 0;
     CHK(d2h(c, &n_over, cnt + C_OVER, 4));
     c->lds_nb = group_max;
-    c->grp_first.assign(c->cb_first.begin(), c->cb_first.end());
+    c->grp_first.assign(gf.begin(), gf.end());
     // oversized fine buckets: their rows from the host mirror.  Those of at
     // most BIG_GROUP slots run alone in a larger LDS area; the rest get HBM
     // tables.
