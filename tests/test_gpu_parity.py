@@ -328,25 +328,30 @@ def test_long_path_chain_ranking():
     assert uc.sorted_lines(got) == textbook_sorted(uc.encode(kmers, k), k)
 
 
-@pytest.mark.parametrize("l,B,mode", [(3, 2, uc.UC_MODE_BATCHED), (4, 4, uc.UC_MODE_PLAIN), (5, 8, uc.UC_MODE_BATCHED)])
-def test_oversized_fine_buckets_use_hbm_tables(l, B, mode):
-    # few distinct minimizers -> fine buckets above the LDS group size; those
-    # of at most 4096 slots run alone in a larger LDS area, bigger ones go
-    # through the HBM-table kernels, the rest through LDS groups
+@pytest.mark.parametrize("l,B,mode", [(1, 2, uc.UC_MODE_BATCHED), (1, 2, uc.UC_MODE_PLAIN), (3, 2, uc.UC_MODE_BATCHED),
+                                      (4, 4, uc.UC_MODE_PLAIN), (5, 8, uc.UC_MODE_BATCHED)])
+def test_oversized_fine_buckets(l, B, mode):
+    # few distinct minimizers -> fine buckets above the LDS group size: those
+    # of at most 4096 slots run alone in a larger LDS area, those of at most
+    # 65535 in LDS hash slices, bigger ones (l = 1: four minimizers) through
+    # the HBM-table kernels; the rest through LDS groups
     kmers = uc.medium_model(77 + l, 300_000, 31)
     with uc.Compactor(n_buckets=B, mode=mode) as c:
         got = c.compact(kmers, 31, l)
         info = c.bucket_info()
     log(f"l={l} B={B} largest fine bucket {info.largest_fine_bucket} oversized {info.n_oversized}")
     assert info.largest_fine_bucket > 2048 and info.n_oversized > 0
+    if l == 1:
+        assert info.largest_fine_bucket > 65535
     assert uc.sorted_lines(got) == textbook_sorted(kmers, 31)
 
 
-@pytest.mark.parametrize("k,l,mode", [(31, 11, uc.UC_MODE_BATCHED), (63, 13, uc.UC_MODE_PLAIN), (31, 3, uc.UC_MODE_BATCHED)])
-def test_small_groups_route_fine_buckets_to_all_three_paths(k, l, mode):
+@pytest.mark.parametrize("k,l,mode", [(31, 11, uc.UC_MODE_BATCHED), (63, 13, uc.UC_MODE_PLAIN), (31, 3, uc.UC_MODE_BATCHED),
+                                      (63, 2, uc.UC_MODE_BATCHED)])
+def test_small_groups_route_fine_buckets_to_every_path(k, l, mode):
     # with 256-slot LDS groups most fine buckets are "oversized": those up to
-    # 4096 slots take the single-bucket LDS kernel, larger ones (l = 3) the
-    # HBM tables; the text is the textbook one either way
+    # 4096 slots take the single-bucket LDS kernel, larger ones (l <= 3) the
+    # LDS hash slices; the text is the textbook one either way
     kmers = uc.medium_model(91 + k + l, 200_000, k)
     with uc.Compactor(n_buckets=16, mode=mode, group_slots=256) as c:
         got = c.compact(kmers, k, l)
@@ -354,6 +359,6 @@ def test_small_groups_route_fine_buckets_to_all_three_paths(k, l, mode):
     log(f"k={k} l={l} largest fine bucket {info.largest_fine_bucket} oversized {info.n_oversized} "
         f"slots {info.oversized_slots}")
     assert info.n_oversized > 0
-    if l == 3:
+    if l <= 3:
         assert info.largest_fine_bucket > 4096
     assert uc.sorted_lines(got) == textbook_sorted(kmers, k)

@@ -1529,6 +1529,118 @@ __global__ __launch_bounds__(EB) void k_edge_lds(const uint64_t *__restrict__ km
     }
 }
 
+// ---- hash slices: a fine bucket above BIG_GROUP (and at most SLICE_MAX)
+// slots is split by the hash of the (k-1)-mer key into P slices of about
+// SLICE_KEYS prefix keys; one workgroup per slice streams the whole bucket
+// (from L2) three times: insert the prefixes of its slice into an LDS table,
+// count the suffix hits of its slice, and resolve.  Equal keys share a
+// slice, so every edge is found in exactly one workgroup.  Table entry: owner
+// (16 bits) | LDS_SHARED | 7-bit tag << 17 | hits << 24.  Every k-mer is a
+// run of its own, as in the HBM-table form.
+constexpr uint32_t SLICE_TAGS = 0x7Fu << 17;
+
+template <int W>
+__device__ __forceinline__ bool slice_probe(const uint32_t *tab, const uint64_t *__restrict__ kmer, uint32_t begin,
+                                            const Kmer<W> &key, uint32_t h, uint32_t tg, uint64_t pm0, uint64_t pm1,
+                                            uint32_t &pos)
+{
+    pos = h & (SLICE_TAB - 1);
+    for (uint32_t probe = 0; probe < SLICE_TAB; probe++) {
+        const uint32_t e = tab[pos];
+        if (e == LDS_EMPTY) return false;
+        if ((e & SLICE_TAGS) == tg &&
+            key_eq<W>(prefix_key<W>(load_kmer<W>(kmer, begin + (e & 0xFFFFu), ~0ull), pm0, pm1), key))
+            return true;
+        pos = (pos + 1) & (SLICE_TAB - 1);
+    }
+    return false;
+}
+
+template <int W>
+__global__ __launch_bounds__(1024) void k_edge_slice(const uint64_t *__restrict__ kmer,
+                                                     const SliceTask *__restrict__ tasks, uint32_t k, uint64_t pm0,
+                                                     uint64_t pm1, uint64_t *__restrict__ link,
+                                                     uint32_t *__restrict__ hasp, RunOut ro, uint32_t *__restrict__ err)
+{
+    constexpr uint32_t NT = 1024;
+    __shared__ uint32_t tab[SLICE_TAB];
+    const SliceTask t = tasks[blockIdx.x];
+    const uint32_t n = t.end - t.begin;
+    for (uint32_t i = threadIdx.x; i < SLICE_TAB; i += NT) tab[i] = LDS_EMPTY;
+    __syncthreads();
+    // 1. prefixes of home copies in this slice
+    for (uint32_t i = threadIdx.x; i < n; i += NT) {
+        const uint32_t s = t.begin + i;
+        if (ro.flags[s] & F_BORROW) continue;
+        const Kmer<W> key = prefix_key<W>(load_kmer<W>(kmer, s, ~0ull), pm0, pm1);
+        uint32_t h, tag;
+        key_hash<W>(key, h, tag);
+        if (fastrange(h, t.P) != t.p) continue;
+        const uint32_t tg = (tag & 0x7Fu) << 17;
+        uint32_t pos = h & (SLICE_TAB - 1), probe = 0;
+        for (; probe < SLICE_TAB; probe++) {
+            const uint32_t cur = atomicCAS(&tab[pos], LDS_EMPTY, i | tg);
+            if (cur == LDS_EMPTY) break;
+            if ((cur & SLICE_TAGS) == tg &&
+                key_eq<W>(prefix_key<W>(load_kmer<W>(kmer, t.begin + (cur & 0xFFFFu), ~0ull), pm0, pm1), key)) {
+                atomicOr(&tab[pos], LDS_SHARED);
+                break;
+            }
+            pos = (pos + 1) & (SLICE_TAB - 1);
+        }
+        if (probe == SLICE_TAB) atomicOr(err, 1u);  // table full: not expected at SLICE_KEYS per slice
+    }
+    __syncthreads();
+    // 2. suffix hits of the copies that carry their out-edge here, this slice
+    for (uint32_t i = threadIdx.x; i < n; i += NT) {
+        const uint32_t s = t.begin + i;
+        if (ro.flags[s] & F_LENT) continue;
+        const Kmer<W> key = suffix_key<W>(load_kmer<W>(kmer, s, ~0ull));
+        uint32_t h, tag, pos;
+        key_hash<W>(key, h, tag);
+        if (fastrange(h, t.P) != t.p) continue;
+        if (slice_probe<W>(tab, kmer, t.begin, key, h, (tag & 0x7Fu) << 17, pm0, pm1, pos))
+            atomicAdd(&tab[pos], 1u << 24);
+    }
+    __syncthreads();
+    // 3. links and singleton runs: a lent k-mer belongs to its prefix's
+    // slice, every other k-mer to its suffix's
+    for (uint32_t i = threadIdx.x; i < n; i += NT) {
+        const uint32_t s = t.begin + i;
+        const uint8_t f = ro.flags[s];
+        const Kmer<W> x = load_kmer<W>(kmer, s, ~0ull);
+        uint32_t h, tag;
+        if (f & F_LENT) {
+            key_hash<W>(prefix_key<W>(x, pm0, pm1), h, tag);
+            if (fastrange(h, t.P) != t.p) continue;
+            link[s] = NO_LINK | ((uint64_t)s << 32);  // its own run head, for the splice
+            ro.flags[s] = (uint8_t)(f | F_RUNHEAD);
+            ro.rec[s] = make_rec(NONE, 1u, s, s);
+            ro.rch[s] = base_char(char_at<W>(x, k - 1));
+            continue;
+        }
+        const Kmer<W> key = suffix_key<W>(x);
+        key_hash<W>(key, h, tag);
+        if (fastrange(h, t.P) != t.p) continue;
+        uint64_t l = NO_LINK;
+        uint32_t pos;
+        if (slice_probe<W>(tab, kmer, t.begin, key, h, (tag & 0x7Fu) << 17, pm0, pm1, pos)) {
+            const uint32_t e = tab[pos];
+            if (!(e & LDS_SHARED) && (e >> 24) == 1u) {  // unshared prefix, exactly one suffix hit
+                const uint32_t v = t.begin + (e & 0xFFFFu);
+                l = v;
+                atomicOr(&hasp[v >> 5], 1u << (v & 31));
+            }
+        }
+        link[s] = l;
+        if (!(f & F_BORROW)) {
+            ro.flags[s] = (uint8_t)(f | F_RUNHEAD);
+            ro.rec[s] = make_rec((uint32_t)l, 1u, s, s);
+            ro.rch[s] = base_char(char_at<W>(x, k - 1));
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // 3. contraction
 //
@@ -1576,7 +1688,7 @@ __device__ __forceinline__ uint32_t wave_append(bool flag, uint32_t *counter)
 // counters never share a line; C_TOT holds two uint64 totals)
 enum {
     C_LIST = 0, C_ERR = 32, C_ACTIVE = 64, C_NONCONV = 96, C_CYC = 128, C_CUT = 160, C_TOT = 192, C_OVER = 224,
-    C_WORDS = 256
+    C_SLICE = 256, C_WORDS = 288
 };
 
 __device__ __forceinline__ bool live_head(const uint8_t *__restrict__ flags, const uint32_t *__restrict__ hasp,
@@ -2251,7 +2363,8 @@ struct uc_ctx {
     std::vector<uint64_t> et_toff, et_first;
     std::vector<uint32_t> et_cap;
     std::vector<uint8_t> bucket_done;
-    std::vector<size_t> grp_first, fb_first, bg_first;  // per coarse bucket: first LDS group / HBM-table / big fine bucket
+    std::vector<size_t> grp_first, fb_first, bg_first, sl_first;  // per coarse bucket: first LDS group / HBM-table
+                                                                 // bucket / big fine bucket / slice task
     uint32_t lds_nb = 64;                     // LDS capacity of the edge launch
     bool rows_valid = false;                  // boff_h / fboff_h mirror the device rows
     uint32_t lds_limit = 65536;               // dynamic LDS a workgroup may use on this device
@@ -2268,7 +2381,7 @@ struct uc_ctx {
     DBuf otable, in, stage, bkt, tiles, hist, xhist, kmer2, flags2, src2, sub, perm, tiles2, hist2, cbhb, cbts,
         cbfirst, fboff, egroups, fbbase, scan_tmp, kmer, flags, src, pair_h, pair_b, boff, sethb, setts, etiles,
         toff, tcap, table, hit, link, hasp, pred, seg, prevseg, chainA, chainB, cycB, list, clist, ulen, tsum,
-        htoff, setoff, text, small, txt, txtcnt, rec, rch, tcnt, cbtf, over, setplan, colsum, bgroups, gfirst;
+        htoff, setoff, text, small, txt, txtcnt, rec, rch, tcnt, cbtf, over, setplan, colsum, bgroups, gfirst, stasks;
     // pinned host chunks of the file reader and writer
     char *pin[2] = {nullptr, nullptr};
     hipEvent_t pin_ev[2] = {nullptr, nullptr};
@@ -2301,7 +2414,7 @@ static std::vector<DBuf *> all_bufs(uc_ctx *c)
             &c->hasp,   &c->pred,    &c->seg,    &c->prevseg, &c->chainA, &c->chainB,
             &c->cycB,   &c->list,    &c->clist,  &c->ulen,    &c->tsum,    &c->htoff,  &c->setoff,
             &c->text,   &c->small,   &c->txt,    &c->txtcnt,  &c->rec,     &c->rch,
-            &c->tcnt,   &c->cbtf,    &c->over,   &c->setplan, &c->colsum, &c->bgroups, &c->gfirst};
+            &c->tcnt,   &c->cbtf,    &c->over,   &c->setplan, &c->colsum, &c->bgroups, &c->gfirst, &c->stasks};
 }
 
 static void free_pinned(uc_ctx *c)
@@ -3083,6 +3196,18 @@ int launch_lds_edges(uc_ctx *c, size_t first, size_t count)
     return UC_OK;
 }
 
+// fine buckets above BIG_GROUP and at most SLICE_MAX slots: one workgroup
+// per hash slice
+template <int W>
+int launch_slice_edges(uc_ctx *c, size_t first, size_t count)
+{
+    if (count == 0) return UC_OK;
+    KLAUNCH(c, "k_edge_slice", k_edge_slice<W>, ((uint32_t)count, 1024, 0), P<uint64_t>(c->kmer),
+            P<SliceTask>(c->stasks) + first, c->k, c->pm0, c->pm1, P<uint64_t>(c->link), P<uint32_t>(c->hasp),
+            run_out(c), P<uint32_t>(c->small) + C_SLICE);
+    return UC_OK;
+}
+
 // single fine buckets above the group size but at most BIG_GROUP slots: one
 // 1024-thread workgroup each, with the LDS area of BIG_GROUP slots
 template <int W>
@@ -3138,8 +3263,10 @@ int prepare_edges(uc_ctx *c)
     std::vector<uint32_t> fb_base;
     std::vector<uint64_t> fb_cnt;
     std::vector<EGroup> big;
+    std::vector<SliceTask> slices;
     c->fb_first.assign(G + 1, 0);
     c->bg_first.assign(G + 1, 0);
+    c->sl_first.assign(G + 1, 0);
     if (n_over) {
         std::vector<uint32_t> ov(n_over);
         CHK(d2h(c, ov.data(), c->over.p, n_over * 4ull));
@@ -3147,23 +3274,34 @@ int prepare_edges(uc_ctx *c)
         CHK(host_rows(c));
         const size_t big_lds = (size_t)BIG_GROUP * (8 * c->W + 10) + ((size_t)BIG_GROUP / 32 + 2) * 4;
         const bool big_ok = big_lds <= c->lds_limit;
-        std::vector<uint32_t> ovh, ovb;  // HBM-table and big-LDS fine buckets
+        std::vector<uint32_t> ovh, ovb, ovs;  // HBM-table, big-LDS and sliced fine buckets
+        std::vector<size_t> s_first;            // first slice task of each sliced fine bucket
         for (uint32_t f : ov) {
             const uint64_t sz = c->fboff_h[f + 1] - c->fboff_h[f];
             if (big_ok && sz <= BIG_GROUP) {
                 ovb.push_back(f);
                 big.push_back(EGroup{c->fboff_h[f], c->fboff_h[f + 1]});
+            } else if (sz <= SLICE_MAX) {
+                ovs.push_back(f);
+                s_first.push_back(slices.size());
+                const uint32_t P = (uint32_t)((sz + SLICE_KEYS - 1) / SLICE_KEYS);
+                for (uint32_t q = 0; q < P; q++) slices.push_back(SliceTask{c->fboff_h[f], c->fboff_h[f + 1], q, P});
             } else {
                 ovh.push_back(f);
                 fb_base.push_back(c->fboff_h[f]);
                 fb_cnt.push_back(sz);
             }
         }
+        s_first.push_back(slices.size());
         for (uint64_t g = 0; g <= G; g++) {
             c->fb_first[g] = (size_t)(std::lower_bound(ovh.begin(), ovh.end(), c->cb_first[g]) - ovh.begin());
             c->bg_first[g] = (size_t)(std::lower_bound(ovb.begin(), ovb.end(), c->cb_first[g]) - ovb.begin());
+            c->sl_first[g] = s_first[std::lower_bound(ovs.begin(), ovs.end(), c->cb_first[g]) - ovs.begin()];
         }
     }
+    CHK(ensure(c, c->stasks, std::max<size_t>(slices.size(), 1) * sizeof(SliceTask)));
+    if (!slices.empty()) CHK(h2d(c, c->stasks.p, slices.data(), slices.size() * sizeof(SliceTask)));
+    CHK(fill<uint32_t>(c, cnt + C_SLICE, 1, 0u));
     CHK(ensure(c, c->bgroups, std::max<size_t>(big.size(), 1) * sizeof(EGroup)));
     if (!big.empty()) CHK(h2d(c, c->bgroups.p, big.data(), big.size() * sizeof(EGroup)));
     c->n_big = big.size();
@@ -3251,12 +3389,14 @@ int run_edges(uc_ctx *c, uint32_t bucket)
         if (!plain) {
             CHK(launch_lds_edges<W>(c, 0, c->grp_first[G]));
             CHK(launch_big_edges<W>(c, 0, c->bg_first[G]));
+            CHK(launch_slice_edges<W>(c, 0, c->sl_first[G]));
             CHK(run_fallback<W>(c, 0, c->fb_first[G]));
         } else {
             // the reference's per-bucket loop (driver.cpp:479-521)
             for (uint64_t g = 0; g < G; g++) {
                 CHK(launch_lds_edges<W>(c, c->grp_first[g], c->grp_first[g + 1] - c->grp_first[g]));
                 CHK(launch_big_edges<W>(c, c->bg_first[g], c->bg_first[g + 1] - c->bg_first[g]));
+                CHK(launch_slice_edges<W>(c, c->sl_first[g], c->sl_first[g + 1] - c->sl_first[g]));
                 CHK(run_fallback<W>(c, c->fb_first[g], c->fb_first[g + 1]));
             }
         }
@@ -3268,6 +3408,7 @@ int run_edges(uc_ctx *c, uint32_t bucket)
         // start cleared from prepare_edges
         CHK(launch_lds_edges<W>(c, c->grp_first[bucket], c->grp_first[bucket + 1] - c->grp_first[bucket]));
         CHK(launch_big_edges<W>(c, c->bg_first[bucket], c->bg_first[bucket + 1] - c->bg_first[bucket]));
+        CHK(launch_slice_edges<W>(c, c->sl_first[bucket], c->sl_first[bucket + 1] - c->sl_first[bucket]));
         CHK(run_fallback<W>(c, c->fb_first[bucket], c->fb_first[bucket + 1]));
         if (!c->bucket_done[bucket]) {
             c->bucket_done[bucket] = 1;
@@ -3295,7 +3436,7 @@ int contract_pass(uc_ctx *c, uint64_t *live, uint32_t *nonconv)
     const uint32_t m = (uint32_t)c->m;
     const uint64_t T = (c->m + CT_TILE - 1) / CT_TILE;
     uint32_t *cnt = P<uint32_t>(c->small);
-    CHK(fill<uint32_t>(c, cnt, C_WORDS, 0u));
+    CHK(fill<uint32_t>(c, cnt, C_SLICE, 0u));  // (the edge phase's slice flag stays)
     KLAUNCH(c, "k_seg_count", k_seg_count, (xcd_grid(T), BLOCK, 0), P<uint64_t>(c->rec), P<uint32_t>(c->hasp),
             P<uint8_t>(c->flags), m, c->salt, c->shift, P<uint64_t>(c->seg), P<uint32_t>(c->prevseg),
             P<uint32_t>(c->list), cnt, P<uint64_t>(c->tsum), P<uint64_t>(c->tcnt), tile_arg(c, T));
@@ -3304,6 +3445,7 @@ int contract_pass(uc_ctx *c, uint64_t *live, uint32_t *nonconv)
     uint32_t h[C_WORDS];
     CHK(d2h(c, h, cnt, sizeof(h)));
     if (h[C_ERR]) return fail(c, UC_E_STATE, "successor array corrupted: a segment walk exceeded m steps");
+    if (h[C_SLICE]) return fail(c, UC_E_STATE, "an edge-finding slice table overflowed");
     c->n_list = h[C_LIST];
     uint64_t st;
     std::memcpy(&st, h + C_TOT, 8);

@@ -36,6 +36,9 @@ constexpr uint32_t MAX_BUCKETS = 1024;        // coarse buckets per set
 constexpr uint32_t MAX_SUB = 1024;            // fine buckets per coarse bucket
 constexpr uint32_t LDS_GROUP = 2048;          // slots per LDS edge-finding group
 constexpr uint32_t BIG_GROUP = 4096;          // a lone fine bucket up to this size still runs in LDS
+constexpr uint32_t SLICE_MAX = 65535;         // larger fine buckets up to this size run in LDS hash slices
+constexpr uint32_t SLICE_KEYS = 2048;         // prefix keys per slice (slices per bucket: size / this)
+constexpr uint32_t SLICE_TAB = 8192;          // LDS table entries of a slice
 constexpr uint32_t DEFAULT_SAMPLE_LOG2 = 9;   // splitter sampling 2^-9 of the run heads
 
 struct BTile {              // bucketing tile (one workgroup)
@@ -51,6 +54,11 @@ struct BTile {              // bucketing tile (one workgroup)
 
 struct EGroup {             // edge-finding group: consecutive fine buckets of one
     uint32_t begin, end;    // coarse bucket, staged in LDS by one workgroup
+};
+
+struct SliceTask {          // one hash slice of one large fine bucket
+    uint32_t begin, end;    // the fine bucket's slots
+    uint32_t p, P;          // slice p of P: the keys whose hash falls in it
 };
 
 struct ETile {              // edge-finding tile: slots of one bucket
