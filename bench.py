@@ -221,6 +221,64 @@ def serve_pool(n_inputs, seed0, lo=10_000, hi=2_000_000):
     return pool
 
 
+def copy_ceiling():
+    """Stream-copy ceiling of the device: the best of five 1 GiB copies by
+    uc_copy_probe (a plain copy kernel), read + write bytes per second."""
+    import subprocess
+
+    import unitigcoarsening_amd as uc
+
+    exe = os.path.join(os.path.dirname(uc.LIB_PATH), "uc_copy_probe")
+    try:
+        r = subprocess.run([exe, "1024", "5"], capture_output=True, text=True, timeout=60)
+    except (OSError, subprocess.SubprocessError):
+        return None
+    rates = [float(line.split()[-2]) for line in r.stdout.splitlines() if line.startswith("copy ")]
+    return max(rates[1:] or rates) if r.returncode == 0 and rates else None
+
+
+def bench_file(args, kmers, l, B):
+    """End to end, file to file: the workload as reference k-mer text in a
+    file, read and parsed on the device (pinned chunks), compacted, and the
+    unitig text appended to an output file."""
+    import tempfile
+
+    import unitigcoarsening_amd as uc
+
+    k = args.k
+    text = uc.format_text(kmers, k)
+    d = tempfile.mkdtemp(prefix="uc_bench_")
+    src, dst = os.path.join(d, "kmers.txt"), os.path.join(d, "unitigs.txt")
+    try:
+        with open(src, "wb") as f:
+            f.write(text)
+        in_bytes = len(text)
+        del text
+        with uc.Compactor(n_buckets=B) as c:
+            c.load_file(src, k)  # warm: page cache, pinned buffers, device buffers
+            c.compact_loaded(l)
+            c.sync()
+            t0 = time.perf_counter()
+            c.load_file(src, k)
+            c.compact_loaded(l)
+            c.append_text_file(dst)
+            c.sync()
+            wall = time.perf_counter() - t0
+        out_bytes = os.path.getsize(dst)
+        return {"ms": round(wall * 1e3, 2), "kmers_per_s": kmers.shape[0] / wall, "input_bytes": in_bytes,
+                "output_bytes": out_bytes,
+                "note": "reference text in, unitig text out, on the device's host; the input file is in the page "
+                        "cache (read once before); the output is appended to an empty file"}
+    except (OSError, RuntimeError) as e:
+        log(f"file to file skipped: {e}")
+        return None
+    finally:
+        for f in (src, dst):
+            if os.path.exists(f):
+                os.remove(f)
+        os.rmdir(d)
+
+
 def bench_serve(args):
     """Request loop (uc_server): Poisson arrivals at a fraction of the
     measured one-context capacity, jobs drawn from a pool of inputs;
@@ -321,6 +379,8 @@ def main():
     ap.add_argument("--no-sets", action="store_true", help="skip the 256-set batched vs plain comparison")
     ap.add_argument("--n-sets", type=int, default=256)
     ap.add_argument("--no-serve", action="store_true", help="skip the request-loop latency benchmark")
+    ap.add_argument("--no-file", action="store_true", help="skip the file-to-file run")
+    ap.add_argument("--no-ceiling", action="store_true", help="skip the stream-copy ceiling measurement")
     ap.add_argument("--serve-jobs", type=int, default=400)
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-load", type=float, default=0.7, help="offered load, x one-context capacity")
@@ -438,6 +498,13 @@ def main():
 
     sets = None if args.no_sets else bench_sets(args)
     serve = None if args.no_serve else bench_serve(args)
+    f2f = None if args.no_file else bench_file(args, kmers, l, B)
+    if f2f:
+        log(f"file to file: {f2f['ms']:.1f} ms, {f2f['kmers_per_s'] / 1e6:.1f} M k-mers/s")
+    ceiling = None if args.no_ceiling else copy_ceiling()
+    if roof is not None and ceiling:
+        roof["copy_ceiling"] = {"achieved_copy_gbs": round(ceiling, 1), "frac": round(roof["achieved"] / ceiling, 4),
+                                "note": "stream-copy ceiling of this device (uc_copy_probe, 1 GiB read + write)"}
 
     base = None
     if not args.no_cpu_baseline:
@@ -476,6 +543,7 @@ def main():
         "plain": plain,
         "sets": sets,
         "serve": serve,
+        "file_to_file": f2f,
         "cpu_baseline": base,
     }
     print(json.dumps(out), flush=True)

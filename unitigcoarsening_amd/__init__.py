@@ -524,6 +524,10 @@ class Compactor:
     def text_size(self):
         return lib().uc_text_size(self._ctx)
 
+    def append_text_file(self, path):
+        """Append the unitig text to the file at `path` (uc_text_append_file)."""
+        _check(lib().uc_text_append_file(self._ctx, os.fsencode(path)), self._ctx)
+
     def timing(self):
         t = UcTiming()
         _check(lib().uc_timing_get(self._ctx, C.byref(t)), self._ctx)
