@@ -73,12 +73,13 @@ def kernel_bytes(name, n, m, x, W, n_unitigs, text_bytes, k, runs):
         # pair entries, the duplicate's link, the lent k-mer's link (its run
         # head); write that link and its run record's next run
         "k_splice": x * (4 + 4 + 8 + 8 + 8 + 4),
-        # read flag + has-predecessor bit per slot, one run record per run;
-        # write the segment (length, next splitter) per head
-        "k_seg_count": m * 1 + m // 8 + runs * 8 + n_unitigs * 8,
-        # read flag + has-predecessor bit per slot, head k-mer and segment per
-        # head, one record and the characters per run; write the text
-        "k_head_emit": m * 1 + m // 8 + n_unitigs * (kb + 8) + runs * 8 + n + text_bytes,
+        # read flag + has-predecessor bit per slot, one run record per run
+        # (the list entries of heads of several runs are not counted)
+        "k_seg_count": m * 1 + m // 8 + runs * 8,
+        # read flag + has-predecessor bit per slot, head k-mer per head, one
+        # record and the characters per run; write the text (list entries
+        # of heads of several runs are not counted)
+        "k_head_emit": m * 1 + m // 8 + n_unitigs * kb + runs * 8 + n + text_bytes,
     }
     return table.get(name)
 

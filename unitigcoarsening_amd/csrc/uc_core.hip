@@ -1655,6 +1655,7 @@ __global__ __launch_bounds__(1024) void k_edge_slice(const uint64_t *__restrict_
 
 constexpr int CT_ITEMS = SCAN_ITEMS;
 constexpr int CT_TILE = SCAN_TILE;  // slots per contraction tile
+static_assert(CT_TILE <= 2048, "a tile's list entries hold slot offsets in 11 bits");
 
 // the borrowed duplicate's out-edge moves to the home copy; the target keeps
 // its has-predecessor bit
@@ -1697,13 +1698,25 @@ __device__ __forceinline__ bool live_head(const uint8_t *__restrict__ flags, con
     return !(flags[s] & F_BORROW) && !has_pred(hasp, s);
 }
 
+// heads whose first run has a next run, per tile: at mlist[tile start + i],
+// i < mcnt[tile], (slot - tile start) | several segments << 11 | segment
+// length << 32.  Every other head's unitig is its first run.
+__device__ __forceinline__ uint64_t mlist_entry(uint32_t off, bool multi, uint32_t len)
+{
+    return (uint64_t)off | (multi ? 1ull << 11 : 0ull) | ((uint64_t)len << 32);
+}
+
 // one walk per splitter: segment length, next splitter; per tile the sum
 // of (1 << 32 | length) over heads whose unitig is a single segment (the
-// other heads add their count here and their length in k_chain_tail)
+// other heads add their count here and their length in k_chain_tail).
+// Segment records are kept where the chain kernels read them (segments that
+// end at or start from a sampled splitter); heads of several runs go to the
+// tile's list, written contiguously
 __global__ __launch_bounds__(BLOCK) void k_seg_count(const uint64_t *__restrict__ rec, const uint32_t *__restrict__ hasp,
                                                      const uint8_t *__restrict__ flags, uint32_t m, uint32_t salt,
                                                      uint32_t shift, uint64_t *__restrict__ seg,
                                                      uint32_t *__restrict__ prevseg, uint32_t *__restrict__ list,
+                                                     uint64_t *__restrict__ mlist, uint32_t *__restrict__ mcnt,
                                                      uint32_t *__restrict__ cnt, uint64_t *__restrict__ tile_sum,
                                                      uint64_t *__restrict__ tile_cnt, uint32_t n_tiles)
 {
@@ -1738,20 +1751,28 @@ __global__ __launch_bounds__(BLOCK) void k_seg_count(const uint64_t *__restrict_
 #pragma unroll
     for (int j = 0; j < CT_ITEMS; j++)
         first[j] = (hd[j] || rd[j]) ? rec[base + (uint64_t)j * BLOCK + threadIdx.x] : 0ull;
-    __shared__ uint32_t wtot[WAVES];
+    uint32_t nm = 0;
+#pragma unroll
+    for (int j = 0; j < CT_ITEMS; j++) nm += hd[j] && rec_next(first[j]) != NONE;
+    // one scan ranks both: sampled splitters (low half), list heads (high)
+    __shared__ uint64_t wtot[WAVES];
     __shared__ uint32_t bbase;
-    const uint32_t incl = wave_incl_scan(nr);
+    const uint64_t both = ((uint64_t)nm << 32) | nr;
+    const uint64_t incl = wave_incl_scan(both);
     const int w = threadIdx.x / 64;
     if (__lane_id() == 63) wtot[w] = incl;
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint32_t t = 0;
+        uint64_t t = 0;
         for (int q = 0; q < WAVES; q++) t += wtot[q];
-        bbase = t ? atomicAdd(cnt + C_LIST, t) : 0u;
+        bbase = (uint32_t)t ? atomicAdd(cnt + C_LIST, (uint32_t)t) : 0u;
+        mcnt[ti] = (uint32_t)(t >> 32);
     }
     __syncthreads();
-    uint32_t li = bbase + incl - nr;
-    for (int q = 0; q < w; q++) li += wtot[q];
+    uint64_t pre = incl - both;
+    for (int q = 0; q < w; q++) pre += wtot[q];
+    uint32_t li = bbase + (uint32_t)pre;
+    uint64_t mi = base + (pre >> 32);
     uint64_t acc = 0;
     uint32_t nsplit = 0;
     for (int j = 0; j < CT_ITEMS; j++) {
@@ -1771,7 +1792,8 @@ __global__ __launch_bounds__(BLOCK) void k_seg_count(const uint64_t *__restrict_
                 break;
             }
         }
-        seg[s] = (uint64_t)r | ((uint64_t)v << 32);
+        if (rd[j] || v != NONE) seg[s] = (uint64_t)r | ((uint64_t)v << 32);
+        if (hd[j] && rec_next(first[j]) != NONE) mlist[mi++] = mlist_entry(s - (uint32_t)base, v != NONE, r);
         if (v != NONE) prevseg[v] = s;
         if (hd[j]) {
             acc += 1ull << 32;
@@ -1946,13 +1968,11 @@ __global__ void k_clear_flag(uint8_t *__restrict__ flags, uint32_t m, uint8_t bi
 // ... + len_u + k): its first k-mer in full, the last character of every
 // later k-mer, '\n'.
 
-// unitig length at a head: its segment's, or the chain's when it has more
-// than one segment
-__device__ __forceinline__ uint32_t unitig_len(const uint64_t *__restrict__ seg, const uint32_t *__restrict__ ulen,
-                                               uint32_t s)
+// unitig length of a list head (entry e at tile start t0): its segment's,
+// or the chain's when the unitig has several segments
+__device__ __forceinline__ uint32_t mlist_len(uint64_t e, const uint32_t *__restrict__ ulen, uint64_t t0)
 {
-    const uint64_t sg = seg[s];
-    return (uint32_t)(sg >> 32) == NONE ? (uint32_t)sg : ulen[s];
+    return (e >> 11) & 1u ? ulen[t0 + (e & 0x7FFu)] : (uint32_t)(e >> 32);
 }
 
 // LDS writes of one wave visible to the whole wave (no block barrier)
@@ -2060,7 +2080,8 @@ constexpr uint32_t EMIT_STAGE = 4096;
 template <int W, bool STAGE, int NT>
 __global__ __launch_bounds__(NT) void k_head_emit(const uint64_t *__restrict__ kmer, const uint64_t *__restrict__ rec,
                                                      const char *__restrict__ rch, const uint32_t *__restrict__ hasp,
-                                                     const uint8_t *__restrict__ flags, const uint64_t *__restrict__ seg,
+                                                     const uint8_t *__restrict__ flags,
+                                                     const uint64_t *__restrict__ mlist, const uint32_t *__restrict__ mcnt,
                                                      const uint32_t *__restrict__ ulen,
                                                      const uint64_t *__restrict__ tile_off, uint32_t m, uint32_t k,
                                                      uint32_t salt, uint32_t shift, uint64_t *__restrict__ htoff,
@@ -2075,40 +2096,51 @@ __global__ __launch_bounds__(NT) void k_head_emit(const uint64_t *__restrict__ k
     if (ti == NONE) return;
     const uint64_t t0 = (uint64_t)ti * CT_TILE;
     uint32_t hd = 0, multi = 0;  // per item: live head; head of a chain of several segments
-    // flags, predecessor bits and segments of all items in flight at once
-    // (segments of slots that are no heads are read and ignored)
+    // the tile's list of heads of several runs, its entries, and every
+    // item's flags, predecessor bit and run record, all in flight at once
+    const uint32_t nml = mcnt[ti];
     uint32_t fh[CTI];
-    uint64_t sgs[CTI];
+    uint64_t hr[CTI];
 #pragma unroll
     for (int j = 0; j < CTI; j++) {
         const uint64_t s = t0 + (uint64_t)j * NT + threadIdx.x;
-        fh[j] = F_BORROW;
-        sgs[j] = 0;
-        if (s < m) {
-            fh[j] = (uint32_t)flags[s] | (((hasp[s >> 5] >> (s & 31)) & 1u) << 8);
-            sgs[j] = seg[s];
-        }
+        fh[j] = s < m ? (uint32_t)flags[s] | (((hasp[s >> 5] >> (s & 31)) & 1u) << 8) : (uint32_t)F_BORROW;
+        hr[j] = s < m ? rec[s] : 0ull;
     }
+    uint64_t ml[CTI];
 #pragma unroll
     for (int j = 0; j < CTI; j++) {
-        const uint64_t s = t0 + (uint64_t)j * NT + threadIdx.x;
+        const uint32_t e = (uint32_t)j * NT + threadIdx.x;
+        ml[j] = e < nml ? mlist[t0 + e] : 0ull;
+    }
+    // a live head (not borrowed, no predecessor) whose run is its unitig has
+    // its length now; the list fills in the others
+#pragma unroll
+    for (int j = 0; j < CTI; j++) {
         uint64_t v = 0;
-        if (!(fh[j] & (F_BORROW | 0x100u))) {  // a live head: not borrowed, no predecessor
-            const uint64_t sg = sgs[j];
-            const bool one = (uint32_t)(sg >> 32) == NONE;
-            v = (1ull << 32) | (one ? (uint32_t)sg : ulen[s]);
+        if (!(fh[j] & (F_BORROW | 0x100u))) {
             hd |= 1u << j;
-            multi |= (one ? 0u : 1u) << j;
+            if (rec_next(hr[j]) == NONE) v = (1ull << 32) | rec_len(hr[j]);
         }
         tile[j * NT + threadIdx.x] = v;
     }
     __syncthreads();
+    constexpr uint64_t MULTI = 1ull << 63;  // marks a list head of several segments until the scan
+#pragma unroll
+    for (int j = 0; j < CTI; j++) {
+        const uint32_t e = (uint32_t)j * NT + threadIdx.x;
+        if (e < nml)
+            tile[ml[j] & 0x7FFu] = (1ull << 32) | mlist_len(ml[j], ulen, t0) | ((ml[j] >> 11) & 1u ? MULTI : 0ull);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < CTI; j++) multi |= (tile[j * NT + threadIdx.x] & MULTI ? 1u : 0u) << j;
     uint64_t loc[CTI];
     uint64_t run = 0;
 #pragma unroll
     for (int j = 0; j < CTI; j++) {
         loc[j] = run;
-        run += tile[threadIdx.x * CTI + j];
+        run += tile[threadIdx.x * CTI + j] & ~MULTI;
     }
     const uint64_t incl = wave_incl_scan(run);
     const int w = threadIdx.x / 64;
@@ -2124,11 +2156,11 @@ __global__ __launch_bounds__(NT) void k_head_emit(const uint64_t *__restrict__ k
     char *stg = stage[STAGE ? w : 0];
     const uint32_t lane = __lane_id();
     const uint32_t s0 = (uint32_t)t0 + threadIdx.x;
-    uint64_t nr = 0;
     Kmer<W> nx{};
+    uint64_t nr = 0;
     if (hd & 1u) {
-        nr = rec[s0];
         nx = load_kmer<W>(kmer, s0, ~0ull);
+        nr = rec[s0];
     }
 #pragma unroll 1
     for (uint32_t j = 0; j < (uint32_t)CTI; j++) {
@@ -2136,8 +2168,8 @@ __global__ __launch_bounds__(NT) void k_head_emit(const uint64_t *__restrict__ k
         const uint64_t r = nr;
         const Kmer<W> x = nx;
         if ((hd >> (j + 1)) & 1u) {
-            nr = rec[s + NT];
             nx = load_kmer<W>(kmer, s + NT, ~0ull);
+            nr = rec[s + NT];
         }
         const bool h = (hd >> j) & 1u, mu = (multi >> j) & 1u;
         const uint64_t pre = tile[j * NT + threadIdx.x];
@@ -2194,7 +2226,9 @@ __global__ void k_rand_emit(const uint64_t *__restrict__ rec, const char *__rest
 __global__ __launch_bounds__(BLOCK) void k_set_offsets(const uint32_t *__restrict__ boff, uint32_t n_sets,
                                                        uint32_t B, const uint32_t *__restrict__ hasp,
                                                        const uint8_t *__restrict__ flags,
-                                                       const uint64_t *__restrict__ seg,
+                                                       const uint64_t *__restrict__ rec,
+                                                       const uint64_t *__restrict__ mlist,
+                                                       const uint32_t *__restrict__ mcnt,
                                                        const uint32_t *__restrict__ ulen,
                                                        const uint64_t *__restrict__ tile_off, uint32_t m,
                                                        uint64_t total, uint64_t *__restrict__ out)
@@ -2207,9 +2241,18 @@ __global__ __launch_bounds__(BLOCK) void k_set_offsets(const uint32_t *__restric
         return;
     }
     const uint32_t t = sb / CT_TILE;
+    const uint64_t t0 = (uint64_t)t * CT_TILE;
     uint64_t acc = 0;
-    for (uint32_t x = t * CT_TILE + threadIdx.x; x < sb; x += BLOCK)
-        if (live_head(flags, hasp, x)) acc += (1ull << 32) | unitig_len(seg, ulen, x);
+    for (uint32_t x = (uint32_t)t0 + threadIdx.x; x < sb; x += BLOCK)
+        if (live_head(flags, hasp, x)) {
+            const uint64_t r = rec[x];
+            acc += (1ull << 32) | (rec_next(r) == NONE ? rec_len(r) : 0u);
+        }
+    // heads of several runs: their lengths from the tile's list
+    for (uint32_t e = threadIdx.x; e < mcnt[t]; e += BLOCK) {
+        const uint64_t v = mlist[t0 + e];
+        if (t0 + (v & 0x7FFu) < sb) acc += mlist_len(v, ulen, t0);
+    }
     acc = block_reduce(acc, red);
     if (threadIdx.x == 0) out[s] = tile_off[t] + acc;
 }
@@ -2381,7 +2424,8 @@ struct uc_ctx {
     DBuf otable, in, stage, bkt, tiles, hist, xhist, kmer2, flags2, src2, sub, perm, tiles2, hist2, cbhb, cbts,
         cbfirst, fboff, egroups, fbbase, scan_tmp, kmer, flags, src, pair_h, pair_b, boff, sethb, setts, etiles,
         toff, tcap, table, hit, link, hasp, pred, seg, prevseg, chainA, chainB, cycB, list, clist, ulen, tsum,
-        htoff, setoff, text, small, txt, txtcnt, rec, rch, tcnt, cbtf, over, setplan, colsum, bgroups, gfirst, stasks;
+        htoff, setoff, text, small, txt, txtcnt, rec, rch, tcnt, cbtf, over, setplan, colsum, bgroups, gfirst, stasks,
+        mlist, mcnt;
     // pinned host chunks of the file reader and writer
     char *pin[2] = {nullptr, nullptr};
     hipEvent_t pin_ev[2] = {nullptr, nullptr};
@@ -2414,7 +2458,8 @@ static std::vector<DBuf *> all_bufs(uc_ctx *c)
             &c->hasp,   &c->pred,    &c->seg,    &c->prevseg, &c->chainA, &c->chainB,
             &c->cycB,   &c->list,    &c->clist,  &c->ulen,    &c->tsum,    &c->htoff,  &c->setoff,
             &c->text,   &c->small,   &c->txt,    &c->txtcnt,  &c->rec,     &c->rch,
-            &c->tcnt,   &c->cbtf,    &c->over,   &c->setplan, &c->colsum, &c->bgroups, &c->gfirst, &c->stasks};
+            &c->tcnt,   &c->cbtf,    &c->over,   &c->setplan, &c->colsum, &c->bgroups, &c->gfirst, &c->stasks,
+            &c->mlist,  &c->mcnt};
 }
 
 static void free_pinned(uc_ctx *c)
@@ -3439,7 +3484,8 @@ int contract_pass(uc_ctx *c, uint64_t *live, uint32_t *nonconv)
     CHK(fill<uint32_t>(c, cnt, C_SLICE, 0u));  // (the edge phase's slice flag stays)
     KLAUNCH(c, "k_seg_count", k_seg_count, (xcd_grid(T), BLOCK, 0), P<uint64_t>(c->rec), P<uint32_t>(c->hasp),
             P<uint8_t>(c->flags), m, c->salt, c->shift, P<uint64_t>(c->seg), P<uint32_t>(c->prevseg),
-            P<uint32_t>(c->list), cnt, P<uint64_t>(c->tsum), P<uint64_t>(c->tcnt), tile_arg(c, T));
+            P<uint32_t>(c->list), P<uint64_t>(c->mlist), P<uint32_t>(c->mcnt), cnt, P<uint64_t>(c->tsum),
+            P<uint64_t>(c->tcnt), tile_arg(c, T));
     CHK(scan_excl<uint64_t>(c, P<uint64_t>(c->tcnt), P<uint64_t>(c->tcnt), T,
                             reinterpret_cast<uint64_t *>(cnt + C_TOT), c->scan_tmp));
     uint32_t h[C_WORDS];
@@ -3540,6 +3586,8 @@ int do_coarsen(uc_ctx *c)
     CHK(ensure(c, c->small, C_WORDS * 4));
     CHK(ensure(c, c->tcnt, (T + 1) * 8));
     CHK(ensure(c, c->seg, c->m * 8));
+    CHK(ensure(c, c->mlist, c->m * 8));
+    CHK(ensure(c, c->mcnt, (T + 1) * 4));
     CHK(ensure(c, c->prevseg, c->m * 4));
     CHK(ensure(c, c->ulen, c->m * 4));
     CHK(ensure(c, c->chainA, c->m * 8));
@@ -3589,9 +3637,9 @@ int run_emit(uc_ctx *c)
         // than the other choices on MI355X)
         constexpr int NT = W == 1 ? 256 : 512;
         KLAUNCH(c, "k_head_emit", (k_head_emit<W, (W > 1), NT>), (xcd_grid(T), NT, 0), P<uint64_t>(c->kmer),
-                P<uint64_t>(c->rec), P<char>(c->rch), P<uint32_t>(c->hasp), P<uint8_t>(c->flags), P<uint64_t>(c->seg),
-                P<uint32_t>(c->ulen), P<uint64_t>(c->tsum), m, c->k, c->salt, c->shift, P<uint64_t>(c->htoff),
-                P<char>(c->text), tile_arg(c, T));
+                P<uint64_t>(c->rec), P<char>(c->rch), P<uint32_t>(c->hasp), P<uint8_t>(c->flags),
+                P<uint64_t>(c->mlist), P<uint32_t>(c->mcnt), P<uint32_t>(c->ulen), P<uint64_t>(c->tsum), m, c->k,
+                c->salt, c->shift, P<uint64_t>(c->htoff), P<char>(c->text), tile_arg(c, T));
     }
     if (c->n_list) {
         KLAUNCH(c, "k_rand_emit", k_rand_emit, (grid_for(c->n_list), BLOCK, 0), P<uint64_t>(c->rec), P<char>(c->rch),
@@ -3602,9 +3650,8 @@ int run_emit(uc_ctx *c)
     const uint64_t total = (c->n_unitigs << 32) | (c->text_bytes - c->n_unitigs * c->k);
     CHK(ensure(c, c->setoff, (c->n_sets + 1) * 8));
     KLAUNCH(c, "k_set_offsets", k_set_offsets, (c->n_sets + 1, BLOCK, 0), P<uint32_t>(c->boff), c->n_sets,
-            c->B, P<uint32_t>(c->hasp), P<uint8_t>(c->flags), P<uint64_t>(c->seg), P<uint32_t>(c->ulen),
-            P<uint64_t>(c->tsum), m, total,
-            P<uint64_t>(c->setoff));
+            c->B, P<uint32_t>(c->hasp), P<uint8_t>(c->flags), P<uint64_t>(c->rec), P<uint64_t>(c->mlist),
+            P<uint32_t>(c->mcnt), P<uint32_t>(c->ulen), P<uint64_t>(c->tsum), m, total, P<uint64_t>(c->setoff));
     std::vector<uint64_t> so(c->n_sets + 1);
     CHK(d2h(c, so.data(), c->setoff.p, (c->n_sets + 1) * 8));
     c->set_text.resize(c->n_sets + 1);
@@ -4101,10 +4148,10 @@ uint64_t uc_job_bytes_estimate(uint64_t n, uint32_t k)
     // the minimizer schemes used, crossing k-mers ~9% at k = 31): k-mers and
     // flags twice, fine index, permutation, link, hit, bitmap, four segment
     // arrays, two chain buffers, list, head offsets, run record, run
-    // characters, run head of lent k-mers
+    // characters, run head of lent k-mers, the per-tile head lists
     const uint64_t W = k <= 32 ? 1 : 2;
     const uint64_t per_kmer = 8 * W + 8 + (k + 1);
-    const uint64_t per_slot = 16 * W + 2 + 2 + 4 + 8 + 4 + 1 + 16 + 16 + 4 + 8 + 8 + 1 + 4;
+    const uint64_t per_slot = 16 * W + 2 + 2 + 4 + 8 + 4 + 1 + 16 + 16 + 4 + 8 + 8 + 1 + 4 + 8;
     return n * per_kmer + (n + n / 4) * per_slot + (64ull << 20);
 }
 
