@@ -164,6 +164,22 @@ def test_writer_multi_chunk(tmp_path):
     assert out.read_bytes() == want
 
 
+def test_text_copy_in_pinned_pieces_matches_writer(tmp_path):
+    # uc_text_copy moves texts of 1 MiB and more through two pinned 4 MiB
+    # pieces; a text of a few pieces with a partial last one must equal
+    # what the file writer (64 MiB chunks) appends, and the oracle's lines
+    kmers = uc.medium_model(21, 700_000, 31)
+    with uc.Compactor() as c:
+        got = c.compact(kmers, 31, 11)
+        log(f"text {len(got)} bytes")
+        assert len(got) > (8 << 20) and len(got) % (4 << 20)
+        out = tmp_path / "o.txt"
+        _check_append(c, out)
+    assert out.read_bytes() == got
+    want, _, _ = oracle.textbook(kmers, 31)
+    assert uc.sorted_lines(got) == uc.sorted_lines(want)
+
+
 @pytest.mark.parametrize("k", [15, 31, 47, 63])
 def test_packed_file_load_and_cli(k, tmp_path):
     import subprocess
