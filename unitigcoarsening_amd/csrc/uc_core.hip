@@ -1811,24 +1811,32 @@ __global__ __launch_bounds__(BLOCK) void k_seg_count(const uint64_t *__restrict_
     }
 }
 
+// The chain kernels run over the sampled splitters, whose count the device
+// holds (*np): a fixed grid strides over them, so no host read of the count
+// stands between the segment walks and the chain rounds.
+
 // sampled splitters: (preceding segment, its length)
-__global__ void k_chain_init(const uint32_t *__restrict__ list, uint32_t n, const uint32_t *__restrict__ prevseg,
-                             const uint64_t *__restrict__ seg, uint64_t *__restrict__ chain)
+__global__ void k_chain_init(const uint32_t *__restrict__ list, const uint32_t *__restrict__ np,
+                             const uint32_t *__restrict__ prevseg, const uint64_t *__restrict__ seg,
+                             uint64_t *__restrict__ chain)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t r = list[i], a = prevseg[r];
-    chain[r] = (uint64_t)a | ((uint64_t)(uint32_t)seg[a] << 32);
+    const uint32_t n = *np;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t r = list[i], a = prevseg[r];
+        chain[r] = (uint64_t)a | ((uint64_t)(uint32_t)seg[a] << 32);
+    }
 }
 
-// one synchronous pointer-jumping round: (ancestor, distance) pairs
-__global__ void k_chain_round(const uint32_t *__restrict__ list, uint32_t n, const uint32_t *__restrict__ hasp,
-                              const uint64_t *__restrict__ cin, uint64_t *__restrict__ cout,
-                              uint32_t *__restrict__ active)
+// one synchronous pointer-jumping round: (ancestor, distance) pairs; a
+// round after convergence copies every pair through unchanged.  *active
+// ends non-zero when some chain has not reached its head.
+__global__ void k_chain_round(const uint32_t *__restrict__ list, const uint32_t *__restrict__ np,
+                              const uint32_t *__restrict__ hasp, const uint64_t *__restrict__ cin,
+                              uint64_t *__restrict__ cout, uint32_t *__restrict__ active)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = *np;
     bool act = false;
-    if (i < n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t r = list[i];
         const uint64_t c = cin[r];
         const uint32_t a = (uint32_t)c;
@@ -1838,7 +1846,7 @@ __global__ void k_chain_round(const uint32_t *__restrict__ list, uint32_t n, con
             const uint64_t c2 = cin[a];
             const uint32_t a2 = (uint32_t)c2;
             cout[r] = (uint64_t)a2 | ((uint64_t)((uint32_t)(c >> 32) + (uint32_t)(c2 >> 32)) << 32);
-            act = has_pred(hasp, a2);
+            act |= has_pred(hasp, a2);
         }
     }
     const uint64_t bal = __ballot(act);
@@ -1846,14 +1854,14 @@ __global__ void k_chain_round(const uint32_t *__restrict__ list, uint32_t n, con
 }
 
 // the tail segment of a multi-segment unitig gives the unitig length
-__global__ void k_chain_tail(const uint32_t *__restrict__ list, uint32_t n, const uint32_t *__restrict__ hasp,
-                             const uint64_t *__restrict__ chain, const uint64_t *__restrict__ seg,
-                             uint32_t *__restrict__ ulen,
+__global__ void k_chain_tail(const uint32_t *__restrict__ list, const uint32_t *__restrict__ np,
+                             const uint32_t *__restrict__ hasp, const uint64_t *__restrict__ chain,
+                             const uint64_t *__restrict__ seg, uint32_t *__restrict__ ulen,
                              uint64_t *__restrict__ tile_sum, uint32_t *__restrict__ nonconv)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = *np;
     bool bad = false;
-    if (i < n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t r = list[i];
         const uint64_t c = chain[r];
         const uint32_t a = (uint32_t)c;
@@ -3339,7 +3347,7 @@ int prepare_edges(uc_ctx *c)
     // groups are entries [gfirst[g], gfirst[g + 1])
     CHK(ensure(c, c->egroups, std::max<uint64_t>(F, 1) * sizeof(EGroup)));
     CHK(ensure(c, c->over, F * 4 + 4));
-    CHK(ensure(c, c->gfirst, (G + 1) * 4));
+    CHK(ensure(c, c->gfirst, (G + 2) * 4));
     CHK(ensure(c, c->small, C_WORDS * 4));
     uint32_t *cnt = P<uint32_t>(c->small);
     CHK(fill<uint32_t>(c, cnt + C_OVER, 1, 0u));
@@ -3351,10 +3359,18 @@ int prepare_edges(uc_ctx *c)
         CHK(scan_excl<uint32_t>(c, gfd, gfd, G, gfd + G, c->scan_tmp));
         KLAUNCH(c, "k_plan_write", k_plan_write, ((uint32_t)G, BLOCK, 0), P<uint32_t>(c->fboff),
                 P<uint32_t>(c->cbfirst), group_max, gfd, P<EGroup>(c->egroups));
-        CHK(d2h(c, gf.data(), gfd, (G + 1) * 4));
+        // the oversized count lands after the group offsets: one read for both
+        HIPCHK(c, hipMemcpyAsync(gfd + G + 1, cnt + C_OVER, 4, hipMemcpyDeviceToDevice, c->stream));
+        gf.resize(G + 2);
+        CHK(d2h(c, gf.data(), gfd, (G + 2) * 4));
     }
     uint32_t n_over = 0;
-    CHK(d2h(c, &n_over, cnt + C_OVER, 4));
+    if (G) {
+        n_over = gf[G + 1];
+        gf.resize(G + 1);
+    } else {
+        CHK(d2h(c, &n_over, cnt + C_OVER, 4));
+    }
     c->lds_nb = group_max;
     c->grp_first.assign(gf.begin(), gf.end());
     // oversized fine buckets: their rows from the host mirror.  Those of at
@@ -3531,7 +3547,16 @@ int do_edges(uc_ctx *c, uint32_t bucket)
 // one contraction pass: segment walks, chain ranking, unitig count/length
 // per tile, scan of the tiles.  *live = k-mers reached from a head,
 // *nonconv = sampled splitters whose chain has no head (on a cycle).
-int contract_pass(uc_ctx *c, uint64_t *live, uint32_t *nonconv)
+// chain rounds launched before the host looks at the active counters; a
+// chain of up to 2^(SPEC_ROUNDS - 1) sampled segments converges within them
+constexpr uint32_t SPEC_ROUNDS = 2;
+constexpr uint32_t CHAIN_BLOCKS = 1024;  // grid of the chain kernels (they stride)
+
+// Segment walks, chain ranking and unitig totals.  The first attempt (spec)
+// reads the device's counters once, at the end, after SPEC_ROUNDS chain
+// rounds; if a chain is still open then, the pass runs again from the walks
+// with a host-checked round loop (everything it writes is rewritten).
+int contract_pass(uc_ctx *c, uint64_t *live, uint32_t *nonconv, bool spec = true)
 {
     const uint32_t m = (uint32_t)c->m;
     const uint64_t T = (c->m + CT_TILE - 1) / CT_TILE;
@@ -3543,53 +3568,77 @@ int contract_pass(uc_ctx *c, uint64_t *live, uint32_t *nonconv)
             P<uint64_t>(c->tcnt), tile_arg(c, T));
     CHK(scan_excl<uint64_t>(c, P<uint64_t>(c->tcnt), P<uint64_t>(c->tcnt), T,
                             reinterpret_cast<uint64_t *>(cnt + C_TOT), c->scan_tmp));
+    const uint32_t *np = cnt + C_LIST;
+    uint64_t *ca = P<uint64_t>(c->chainA), *cb = P<uint64_t>(c->chainB);
     uint32_t h[C_WORDS];
-    CHK(d2h(c, h, cnt, sizeof(h)));
-    if (h[C_ERR]) return fail(c, UC_E_STATE, "successor array corrupted: a segment walk exceeded m steps");
-    if (h[C_SLICE]) return fail(c, UC_E_STATE, "an edge-finding slice table overflowed");
-    c->n_list = h[C_LIST];
+    uint32_t rounds = 0;
+    auto read_counts = [&]() -> int {
+        CHK(d2h(c, h, cnt, sizeof(h)));
+        if (h[C_ERR]) return fail(c, UC_E_STATE, "successor array corrupted: a segment walk exceeded m steps");
+        if (h[C_SLICE]) return fail(c, UC_E_STATE, "an edge-finding slice table overflowed");
+        return UC_OK;
+    };
+    if (spec) {
+        const uint32_t g = std::max<uint32_t>(1, std::min<uint32_t>(grid_for(m), CHAIN_BLOCKS));
+        KLAUNCH(c, "k_chain_init", k_chain_init, (g, BLOCK, 0), P<uint32_t>(c->list), np, P<uint32_t>(c->prevseg),
+                P<uint64_t>(c->seg), ca);
+        for (uint32_t r = 0; r < SPEC_ROUNDS; r++) {
+            KLAUNCH(c, "k_chain_round", k_chain_round, (g, BLOCK, 0), P<uint32_t>(c->list), np,
+                    P<uint32_t>(c->hasp), ca, cb, cnt + C_ACTIVE + r);
+            std::swap(ca, cb);
+        }
+        KLAUNCH(c, "k_chain_tail", k_chain_tail, (g, BLOCK, 0), P<uint32_t>(c->list), np, P<uint32_t>(c->hasp),
+                ca, P<uint64_t>(c->seg), P<uint32_t>(c->ulen), P<uint64_t>(c->tsum), cnt + C_NONCONV);
+    } else {
+        CHK(read_counts());
+        const uint32_t nl = h[C_LIST];
+        if (nl) {
+            const uint32_t g = std::min<uint32_t>(grid_for(nl), CHAIN_BLOCKS);
+            KLAUNCH(c, "k_chain_init", k_chain_init, (g, BLOCK, 0), P<uint32_t>(c->list), np,
+                    P<uint32_t>(c->prevseg), P<uint64_t>(c->seg), ca);
+            // a chain holds at most nl segments: ceil(log2(nl)) + 1 rounds
+            // reach its head; chains still open after that run around a cycle
+            uint32_t cap = 1;
+            while ((1ull << cap) < nl) cap++;
+            cap += 1;
+            for (;;) {
+                CHK(fill<uint32_t>(c, cnt + C_ACTIVE, 1, 0u));
+                KLAUNCH(c, "k_chain_round", k_chain_round, (g, BLOCK, 0), P<uint32_t>(c->list), np,
+                        P<uint32_t>(c->hasp), ca, cb, cnt + C_ACTIVE);
+                std::swap(ca, cb);
+                rounds++;
+                uint32_t active;
+                CHK(d2h(c, &active, cnt + C_ACTIVE, 4));
+                if (active == 0 || rounds >= cap) break;
+            }
+            KLAUNCH(c, "k_chain_tail", k_chain_tail, (g, BLOCK, 0), P<uint32_t>(c->list), np,
+                    P<uint32_t>(c->hasp), ca, P<uint64_t>(c->seg), P<uint32_t>(c->ulen), P<uint64_t>(c->tsum),
+                    cnt + C_NONCONV);
+        }
+    }
+    // the unitig totals land beside the other counters: one read for all
+    uint64_t *ttot = reinterpret_cast<uint64_t *>(cnt + C_TOT + 2);
+    CHK(scan_excl<uint64_t>(c, P<uint64_t>(c->tsum), P<uint64_t>(c->tsum), T, ttot, c->scan_tmp));
+    CHK(read_counts());
+    const uint32_t nl = h[C_LIST];
+    if (spec) {
+        // rounds to convergence: the first whose pairs all reached a head
+        while (rounds < SPEC_ROUNDS && h[C_ACTIVE + rounds]) rounds++;
+        if (nl && rounds == SPEC_ROUNDS) return contract_pass(c, live, nonconv, false);
+        rounds = nl ? rounds + 1 : 0;
+    }
+    c->n_list = nl;
     uint64_t st;
     std::memcpy(&st, h + C_TOT, 8);
     c->n_splitters = st >> 32;
     c->n_runs = st & 0xFFFFFFFFull;
-    const uint32_t nl = c->n_list;
-    uint64_t *ca = P<uint64_t>(c->chainA), *cb = P<uint64_t>(c->chainB);
-    uint32_t rounds = 0, bad = 0;
-    if (nl) {
-        KLAUNCH(c, "k_chain_init", k_chain_init, (grid_for(nl), BLOCK, 0), P<uint32_t>(c->list), nl,
-                P<uint32_t>(c->prevseg), P<uint64_t>(c->seg), ca);
-        // a chain holds at most nl segments: ceil(log2(nl)) + 1 rounds reach
-        // its head; chains still open after that run around a cycle
-        uint32_t cap = 1;
-        while ((1ull << cap) < nl) cap++;
-        cap += 1;
-        for (;;) {
-            CHK(fill<uint32_t>(c, cnt + C_ACTIVE, 1, 0u));
-            KLAUNCH(c, "k_chain_round", k_chain_round, (grid_for(nl), BLOCK, 0), P<uint32_t>(c->list), nl,
-                    P<uint32_t>(c->hasp), ca, cb, cnt + C_ACTIVE);
-            std::swap(ca, cb);
-            rounds++;
-            uint32_t active;
-            CHK(d2h(c, &active, cnt + C_ACTIVE, 4));
-            if (active == 0 || rounds >= cap) break;
-        }
-        KLAUNCH(c, "k_chain_tail", k_chain_tail, (grid_for(nl), BLOCK, 0), P<uint32_t>(c->list), nl,
-                P<uint32_t>(c->hasp), ca, P<uint64_t>(c->seg), P<uint32_t>(c->ulen),
-                P<uint64_t>(c->tsum), cnt + C_NONCONV);
-    }
     c->chain = ca;
     c->chain_rounds += rounds;
-    // the unitig totals land beside the non-converged count: one read for both
-    uint64_t *ttot = reinterpret_cast<uint64_t *>(cnt + C_TOT + 2);
-    CHK(scan_excl<uint64_t>(c, P<uint64_t>(c->tsum), P<uint64_t>(c->tsum), T, ttot, c->scan_tmp));
-    uint32_t hh[C_TOT + 4 - C_NONCONV];
-    CHK(d2h(c, hh, cnt + C_NONCONV, sizeof(hh)));
-    bad = hh[0];
     uint64_t tot;
-    std::memcpy(&tot, hh + (C_TOT + 2 - C_NONCONV), 8);
+    std::memcpy(&tot, h + C_TOT + 2, 8);
     c->n_unitigs = tot >> 32;
     *live = tot & 0xFFFFFFFFull;
-    *nonconv = bad;
+    *nonconv = h[C_NONCONV];
     return UC_OK;
 }
 
@@ -3701,6 +3750,13 @@ int run_emit(uc_ctx *c)
                 P<uint32_t>(c->list), c->n_list, c->chain, P<uint64_t>(c->htoff), c->k, c->salt, c->shift, m,
                 P<char>(c->text));
     }
+    c->set_text.resize(c->n_sets + 1);
+    if (c->n_sets == 1) {  // one set: the whole text
+        c->set_text[0] = 0;
+        c->set_text[1] = c->text_bytes;
+        c->emitted = true;
+        return UC_OK;
+    }
     // per-set text ranges: a set's unitigs start at its first bucket row
     const uint64_t total = (c->n_unitigs << 32) | (c->text_bytes - c->n_unitigs * c->k);
     CHK(ensure(c, c->setoff, (c->n_sets + 1) * 8));
@@ -3709,7 +3765,6 @@ int run_emit(uc_ctx *c)
             P<uint32_t>(c->mcnt), P<uint32_t>(c->ulen), P<uint64_t>(c->tsum), m, total, P<uint64_t>(c->setoff));
     std::vector<uint64_t> so(c->n_sets + 1);
     CHK(d2h(c, so.data(), c->setoff.p, (c->n_sets + 1) * 8));
-    c->set_text.resize(c->n_sets + 1);
     for (uint32_t s = 0; s <= c->n_sets; s++) c->set_text[s] = (so[s] & 0xFFFFFFFFull) + (so[s] >> 32) * c->k;
     c->emitted = true;
     return UC_OK;

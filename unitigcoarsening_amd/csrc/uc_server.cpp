@@ -60,13 +60,19 @@ struct uc_server {
     }
 
     // called with mu held by idle worker `self`: can the queue head run here?
+    // A worker keeps its buffers from job to job (no reallocation while the
+    // budget allows); they can grow by at most the job's estimate.  When the
+    // budget is short, this worker's buffers go back first, then those of
+    // idle workers.
     bool admit(size_t self, Job *j)
     {
         const uint64_t budget = opts.hbm_budget;
         if (!budget) return true;
-        const uint64_t need = std::max<uint64_t>(j->estimate, 0);
+        const uint64_t need = j->estimate;
+        uc_ctx *mine = workers[self].ctx;
+        if (held_by_others(self) + uc_ctx_held_bytes(mine) + need <= budget) return true;
+        if (uc_ctx_held_bytes(mine)) uc_ctx_trim(mine);
         if (held_by_others(self) + need <= budget) return true;
-        // idle workers give their buffers back
         for (size_t i = 0; i < workers.size(); i++)
             if (i != self && !workers[i].busy && uc_ctx_held_bytes(workers[i].ctx)) uc_ctx_trim(workers[i].ctx);
         return held_by_others(self) + need <= budget;
@@ -87,9 +93,7 @@ struct uc_server {
             Job *j = queue.front();
             queue.pop_front();
             w.busy = true;
-            // this worker's own buffers are part of what it may use
-            if (opts.hbm_budget && uc_ctx_held_bytes(w.ctx) > j->estimate) uc_ctx_trim(w.ctx);
-            w.reserved = j->estimate;
+            w.reserved = uc_ctx_held_bytes(w.ctx) + j->estimate;  // what admit() counted
             j->state = UC_JOB_RUNNING;
             j->started = Clock::now();
             lk.unlock();
