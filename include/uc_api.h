@@ -69,6 +69,8 @@ typedef struct {
 #define UC_FLAG_KEEP_SRC 4u /* keep slot -> input k-mer ids for uc_bucket_copy */
 #define UC_FLAG_DIRECT_SCATTER 8u /* both bucketing scatters write straight from registers instead of
                                      staging each tile in LDS sorted by bucket */
+#define UC_FLAG_EAGER 16u /* launch the kernels of a contraction pass one by one instead of replaying
+                             them as a captured HIP graph */
 
 void uc_default_opts(uc_opts *o);
 int uc_ctx_create(const uc_opts *opts, uc_ctx **out);

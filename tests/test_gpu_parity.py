@@ -324,8 +324,29 @@ def test_long_path_chain_ranking():
     with uc.Compactor(n_buckets=64, sample_log2=4) as c:
         got = c.compact(uc.encode(kmers, k), k, 11)
         info = c.coarsen_info()
-    assert info.chain_rounds > 0
+    # chains of thousands of segments: more than the two speculative rounds,
+    # so the host-checked round loop ran
+    assert info.chain_rounds > 2
     assert uc.sorted_lines(got) == textbook_sorted(uc.encode(kmers, k), k)
+
+
+def test_pass_graph_replays_match_eager():
+    # the contraction pass is captured once and replayed while sizes and
+    # buffers repeat; every replay, with eager work on another context in
+    # between, must give the eager context's text byte for byte
+    jobs = [(uc.medium_model(31, 120_000, 31), 31, 11), (uc.medium_model(32, 60_000, 63), 63, 13),
+            (uc.medium_model(33, 3_000, 31), 31, 11)]
+    with uc.Compactor() as g, uc.Compactor(eager=True) as e:
+        for km, k, l in jobs:
+            want = e.compact(km, k, l)
+            g.load(km, k)
+            for rep in range(3):
+                log(f"n={km.shape[0]} k={k} replay {rep}")
+                g.compact_loaded(l)
+                assert g.text() == want
+                e.compact(jobs[2][0], 31, 11)
+            tb, n_unitigs, _ = oracle.textbook(km, k)
+            assert uc.sorted_lines(want) == uc.sorted_lines(tb)
 
 
 @pytest.mark.parametrize("l,B,mode", [(1, 2, uc.UC_MODE_BATCHED), (1, 2, uc.UC_MODE_PLAIN), (3, 2, uc.UC_MODE_BATCHED),

@@ -28,6 +28,7 @@ UC_FLAG_PROFILE = 1
 UC_FLAG_XCD_MAP = 2
 UC_FLAG_KEEP_SRC = 4
 UC_FLAG_DIRECT_SCATTER = 8
+UC_FLAG_EAGER = 16
 UC_TUNE_KEEP_BUCKETS = 1
 
 F_LENT = 1
@@ -346,14 +347,15 @@ class Compactor:
 
     def __init__(self, device=0, seed=0x5EED, n_buckets=64, mode=UC_MODE_BATCHED, load_pct=50, sample_log2=0,
                  hbm_budget=0, profile=False, xcd_map=False, keep_src=False, fine_slots=0, group_slots=0,
-                 direct_scatter=False):
+                 direct_scatter=False, eager=False):
         L = lib()
         o = UcOpts()
         L.uc_default_opts(C.byref(o))
         o.device, o.seed, o.n_buckets, o.mode = device, seed, n_buckets, mode
         o.load_pct, o.sample_log2, o.hbm_budget = load_pct, sample_log2, hbm_budget
         o.flags = ((UC_FLAG_PROFILE if profile else 0) | (UC_FLAG_XCD_MAP if xcd_map else 0)
-                   | (UC_FLAG_KEEP_SRC if keep_src else 0) | (UC_FLAG_DIRECT_SCATTER if direct_scatter else 0))
+                   | (UC_FLAG_KEEP_SRC if keep_src else 0) | (UC_FLAG_DIRECT_SCATTER if direct_scatter else 0)
+                   | (UC_FLAG_EAGER if eager else 0))
         o.fine_slots, o.group_slots = fine_slots, group_slots
         self.opts = o
         self._ctx = _P()

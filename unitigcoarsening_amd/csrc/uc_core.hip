@@ -2442,6 +2442,11 @@ struct uc_ctx {
     char *spin[2] = {nullptr, nullptr};
     hipEvent_t spin_ev[2] = {nullptr, nullptr};
     bool spin_used[2] = {false, false};
+    // the captured launches of a contraction pass, replayed while the key
+    // (sizes, parameters, buffer addresses) stays the same
+    hipGraphExec_t cgraph = nullptr;
+    std::vector<uint64_t> cgraph_key;
+    bool capturing = false;
     // contraction state
     uint64_t *chain = nullptr;  // ranked chain entries (chainA or chainB)
     uint32_t n_list = 0, salt = 0, shift = 5;
@@ -2566,7 +2571,8 @@ struct Prof {
     hipEvent_t a = nullptr, b = nullptr;
     Prof(uc_ctx *ctx, const char *name) : c(ctx)
     {
-        if (!(c->opts.flags & UC_FLAG_PROFILE) || (!c->prof_only.empty() && c->prof_only != name)) {
+        if (!(c->opts.flags & UC_FLAG_PROFILE) || c->capturing ||
+            (!c->prof_only.empty() && c->prof_only != name)) {
             c = nullptr;
             return;
         }
@@ -3556,20 +3562,67 @@ constexpr uint32_t CHAIN_BLOCKS = 1024;  // grid of the chain kernels (they stri
 // reads the device's counters once, at the end, after SPEC_ROUNDS chain
 // rounds; if a chain is still open then, the pass runs again from the walks
 // with a host-checked round loop (everything it writes is rewritten).
+// kernels of a contraction pass, which the graph form captures
+static const char *const PASS_KERNELS[] = {"k_fill",       "k_seg_count",   "k_scan_reduce", "k_scan_tile",
+                                           "k_chain_init", "k_chain_round", "k_chain_tail"};
+
+// the graph form unless asked for eager launches, or unless one of the
+// pass's kernels is to be timed (captured launches carry no events)
+static bool pass_graph(const uc_ctx *c)
+{
+    if (c->opts.flags & UC_FLAG_EAGER) return false;
+    if (!(c->opts.flags & UC_FLAG_PROFILE)) return true;
+    if (c->prof_only.empty()) return false;
+    for (const char *n : PASS_KERNELS)
+        if (c->prof_only == n) return false;
+    return true;
+}
+
+void drop_pass_graph(uc_ctx *c)
+{
+    if (c->cgraph) (void)hipGraphExecDestroy(c->cgraph);
+    c->cgraph = nullptr;
+    c->cgraph_key.clear();
+}
+
 int contract_pass(uc_ctx *c, uint64_t *live, uint32_t *nonconv, bool spec = true)
 {
     const uint32_t m = (uint32_t)c->m;
     const uint64_t T = (c->m + CT_TILE - 1) / CT_TILE;
     uint32_t *cnt = P<uint32_t>(c->small);
-    CHK(fill<uint32_t>(c, cnt, C_SLICE, 0u));  // (the edge phase's slice flag stays)
-    KLAUNCH(c, "k_seg_count", k_seg_count, (xcd_grid(T), BLOCK, 0), P<uint64_t>(c->rec), P<uint32_t>(c->hasp),
-            P<uint8_t>(c->flags), m, c->salt, c->shift, P<uint64_t>(c->seg), P<uint32_t>(c->prevseg),
-            P<uint32_t>(c->list), P<uint64_t>(c->mlist), P<uint32_t>(c->mcnt), cnt, P<uint64_t>(c->tsum),
-            P<uint64_t>(c->tcnt), tile_arg(c, T));
-    CHK(scan_excl<uint64_t>(c, P<uint64_t>(c->tcnt), P<uint64_t>(c->tcnt), T,
-                            reinterpret_cast<uint64_t *>(cnt + C_TOT), c->scan_tmp));
     const uint32_t *np = cnt + C_LIST;
     uint64_t *ca = P<uint64_t>(c->chainA), *cb = P<uint64_t>(c->chainB);
+    // walks, the splitter/run scan, then (spec) chain ranking and the
+    // unitig-length scan; everything up to the counter read
+    auto walks = [&]() -> int {
+        CHK(fill<uint32_t>(c, cnt, C_SLICE, 0u));  // (the edge phase's slice flag stays)
+        KLAUNCH(c, "k_seg_count", k_seg_count, (xcd_grid(T), BLOCK, 0), P<uint64_t>(c->rec), P<uint32_t>(c->hasp),
+                P<uint8_t>(c->flags), m, c->salt, c->shift, P<uint64_t>(c->seg), P<uint32_t>(c->prevseg),
+                P<uint32_t>(c->list), P<uint64_t>(c->mlist), P<uint32_t>(c->mcnt), cnt, P<uint64_t>(c->tsum),
+                P<uint64_t>(c->tcnt), tile_arg(c, T));
+        CHK(scan_excl<uint64_t>(c, P<uint64_t>(c->tcnt), P<uint64_t>(c->tcnt), T,
+                                reinterpret_cast<uint64_t *>(cnt + C_TOT), c->scan_tmp));
+        return UC_OK;
+    };
+    auto totals = [&]() -> int {
+        uint64_t *ttot = reinterpret_cast<uint64_t *>(cnt + C_TOT + 2);
+        return scan_excl<uint64_t>(c, P<uint64_t>(c->tsum), P<uint64_t>(c->tsum), T, ttot, c->scan_tmp);
+    };
+    auto spec_pass = [&]() -> int {
+        CHK(walks());
+        uint64_t *a = ca, *b = cb;
+        const uint32_t g = std::max<uint32_t>(1, std::min<uint32_t>(grid_for(m), CHAIN_BLOCKS));
+        KLAUNCH(c, "k_chain_init", k_chain_init, (g, BLOCK, 0), P<uint32_t>(c->list), np, P<uint32_t>(c->prevseg),
+                P<uint64_t>(c->seg), a);
+        for (uint32_t r = 0; r < SPEC_ROUNDS; r++) {
+            KLAUNCH(c, "k_chain_round", k_chain_round, (g, BLOCK, 0), P<uint32_t>(c->list), np,
+                    P<uint32_t>(c->hasp), a, b, cnt + C_ACTIVE + r);
+            std::swap(a, b);
+        }
+        KLAUNCH(c, "k_chain_tail", k_chain_tail, (g, BLOCK, 0), P<uint32_t>(c->list), np, P<uint32_t>(c->hasp), a,
+                P<uint64_t>(c->seg), P<uint32_t>(c->ulen), P<uint64_t>(c->tsum), cnt + C_NONCONV);
+        return totals();
+    };
     uint32_t h[C_WORDS];
     uint32_t rounds = 0;
     auto read_counts = [&]() -> int {
@@ -3578,18 +3631,39 @@ int contract_pass(uc_ctx *c, uint64_t *live, uint32_t *nonconv, bool spec = true
         if (h[C_SLICE]) return fail(c, UC_E_STATE, "an edge-finding slice table overflowed");
         return UC_OK;
     };
-    if (spec) {
-        const uint32_t g = std::max<uint32_t>(1, std::min<uint32_t>(grid_for(m), CHAIN_BLOCKS));
-        KLAUNCH(c, "k_chain_init", k_chain_init, (g, BLOCK, 0), P<uint32_t>(c->list), np, P<uint32_t>(c->prevseg),
-                P<uint64_t>(c->seg), ca);
-        for (uint32_t r = 0; r < SPEC_ROUNDS; r++) {
-            KLAUNCH(c, "k_chain_round", k_chain_round, (g, BLOCK, 0), P<uint32_t>(c->list), np,
-                    P<uint32_t>(c->hasp), ca, cb, cnt + C_ACTIVE + r);
-            std::swap(ca, cb);
+    if (spec && pass_graph(c)) {
+        // no allocation or host wait may happen inside the capture
+        CHK(ensure(c, c->scan_tmp, scan_scratch<uint64_t>(T) + 256));
+        const std::vector<uint64_t> key = {
+            m, c->salt, c->shift, tile_arg(c, T), (uint64_t)c->rec.p, (uint64_t)c->hasp.p, (uint64_t)c->flags.p,
+            (uint64_t)c->seg.p, (uint64_t)c->prevseg.p, (uint64_t)c->list.p, (uint64_t)c->mlist.p,
+            (uint64_t)c->mcnt.p, (uint64_t)c->small.p, (uint64_t)c->tsum.p, (uint64_t)c->tcnt.p,
+            (uint64_t)c->chainA.p, (uint64_t)c->chainB.p, (uint64_t)c->ulen.p, (uint64_t)c->scan_tmp.p};
+        if (!c->cgraph || key != c->cgraph_key) {
+            drop_pass_graph(c);
+            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            c->capturing = true;
+            const int rc = spec_pass();
+            c->capturing = false;
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(c->stream, &g);
+            if (rc != UC_OK || e != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc != UC_OK ? rc : fail(c, UC_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+            }
+            const hipError_t ei = hipGraphInstantiate(&c->cgraph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess) {
+                c->cgraph = nullptr;
+                return fail(c, UC_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+            }
+            c->cgraph_key = key;
         }
-        KLAUNCH(c, "k_chain_tail", k_chain_tail, (g, BLOCK, 0), P<uint32_t>(c->list), np, P<uint32_t>(c->hasp),
-                ca, P<uint64_t>(c->seg), P<uint32_t>(c->ulen), P<uint64_t>(c->tsum), cnt + C_NONCONV);
+        HIPCHK(c, hipGraphLaunch(c->cgraph, c->stream));
+    } else if (spec) {
+        CHK(spec_pass());
     } else {
+        CHK(walks());
         CHK(read_counts());
         const uint32_t nl = h[C_LIST];
         if (nl) {
@@ -3615,10 +3689,10 @@ int contract_pass(uc_ctx *c, uint64_t *live, uint32_t *nonconv, bool spec = true
                     P<uint32_t>(c->hasp), ca, P<uint64_t>(c->seg), P<uint32_t>(c->ulen), P<uint64_t>(c->tsum),
                     cnt + C_NONCONV);
         }
+        CHK(totals());
     }
+    if (spec && SPEC_ROUNDS % 2) std::swap(ca, cb);  // where the last speculative round wrote
     // the unitig totals land beside the other counters: one read for all
-    uint64_t *ttot = reinterpret_cast<uint64_t *>(cnt + C_TOT + 2);
-    CHK(scan_excl<uint64_t>(c, P<uint64_t>(c->tsum), P<uint64_t>(c->tsum), T, ttot, c->scan_tmp));
     CHK(read_counts());
     const uint32_t nl = h[C_LIST];
     if (spec) {
@@ -3882,6 +3956,7 @@ int uc_ctx_destroy(uc_ctx *c)
     if (!c) return UC_OK;
     (void)hipSetDevice(c->opts.device);
     (void)hipStreamSynchronize(c->stream);
+    drop_pass_graph(c);
     for (DBuf *b : all_bufs(c))
         if (b->p) (void)hipFree(b->p);
     free_pinned(c);
@@ -4237,6 +4312,7 @@ int uc_ctx_trim(uc_ctx *c)
     if (!c) return UC_E_ARG;
     CHK(select_dev(c));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    drop_pass_graph(c);
     for (DBuf *b : all_bufs(c)) {
         if (b->p) HIPCHK(c, hipFree(b->p));
         b->p = nullptr;
