@@ -126,6 +126,7 @@ def cpu_baseline(args, l, B, gpu_ctx_kwargs):
     with uc.Compactor(**gpu_ctx_kwargs) as c:
         got = c.compact(kmers, k, l)
     parity = uc.sorted_lines(got) == uc.sorted_lines(text)
+    par = parallel_cpu_baseline(args, l, B)
     return {
         "value": kmers.shape[0] / wall,
         "unit": "k-mers/s",
@@ -136,7 +137,34 @@ def cpu_baseline(args, l, B, gpu_ctx_kwargs):
         "phases_s": {kk: round(stats[kk], 3) for kk in ("bucketing", "assemble", "coarsen", "write_intra",
                                                           "minitigs", "cross_finish")},
         "parity_with_gpu": parity,
+        "parallel": par,
     }
+
+
+def parallel_cpu_baseline(args, l, B, workers=16):
+    """The same serial restatement on the host's cores at once: one
+    independent sample of a fifth of the serial sample's genome per worker
+    process (oracle/parallel_baseline.py), k-mers over the wall time from
+    the first start to the last finish."""
+    import subprocess
+
+    workers = min(workers, os.cpu_count() or 1)
+    bp = max(args.cpu_sample_bp // 5, 100_000)
+    model = "large" if args.large else "medium"
+    cmd = [sys.executable, os.path.join(ROOT, "oracle", "parallel_baseline.py"), str(workers), str(bp), str(args.k),
+           str(l), str(B), str(args.seed + 100), model]
+    try:
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
+        res = json.loads(r.stdout.strip().splitlines()[-1]) if r.returncode == 0 else None
+    except (OSError, subprocess.SubprocessError, ValueError, IndexError):
+        res = None
+    if res is None:
+        return None
+    log(f"cpu baseline, {workers} processes: {res['kmers_per_s'] / 1e6:.2f} M k-mers/s")
+    return {"value": res["kmers_per_s"], "unit": "k-mers/s", "cores": workers, "kind": "port",
+            "sample": f"{workers} independent {model}-model samples of {bp / 1e6:.0f} Mbp (seeds {args.seed + 100}.."
+                      f"{args.seed + 99 + workers}), {res['kmers']} k-mers in all, one serial refpipe.c run per "
+                      f"process, {res['wall_s']:.1f} s from the first start to the last finish"}
 
 
 def sets_workload(n_sets, seed0, k, median=200_000, sigma=1.0, lo=10_000, hi=5_000_000):

@@ -2,9 +2,11 @@
 every declared symbol, the host text parser matches the oracle's parser,
 and the CLI keeps the reference's usage and I/O failure behaviour."""
 import ctypes as C
+import json
 import os
 import re
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -116,3 +118,16 @@ def test_packed_file_roundtrip(tmp_path):
     with pytest.raises(uc.UcError) as e:
         uc.read_packed_header(str(tmp_path / "none.ucb"))
     assert e.value.status == uc.UC_E_IO
+
+
+def test_parallel_cpu_baseline_runs_independent_samples():
+    # the multi-process CPU baseline: every worker's k-mers are counted and
+    # the rate is their sum over the common wall time
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "oracle", "parallel_baseline.py"), "2", "100000", "31", "11",
+                        "16", "5", "medium"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    want = sum(uc.medium_model(5 + i, 100_000, 31).shape[0] for i in range(2))
+    assert res["workers"] == 2 and res["kmers"] == want
+    assert res["kmers_per_s"] == pytest.approx(res["kmers"] / res["wall_s"])
