@@ -64,7 +64,7 @@ def kernel_bytes(name, n, m, x, W, n_unitigs, text_bytes, k, runs):
         "k_pair_remap": x * (8 + 8 + 8),
         # read k-mer + flag per slot; write link, flag and run character per
         # slot, the has-predecessor bitmap and an 8-byte record per run
-        "k_edge_lds": m * (kb + 1) + m * (8 + 1 + 1) + m // 8 + runs * 8,
+        "k_edge_lds": m * (kb + 1) + m * (4 + 1 + 1) + m // 8 + runs * 8,
         # fallback tables for oversized fine buckets (16-byte entries): the
         # same three steps over HBM
         "k_edge_insert": m * (1 + kb) + (m - x) * 16,
@@ -72,7 +72,7 @@ def kernel_bytes(name, n, m, x, W, n_unitigs, text_bytes, k, runs):
         "k_edge_resolve": m * (4 + 8) + (m - x) * 16 + m // 8,
         # pair entries, the duplicate's link, the lent k-mer's link (its run
         # head); write that link and its run record's next run
-        "k_splice": x * (4 + 4 + 8 + 8 + 8 + 4),
+        "k_splice": x * (4 + 4 + 4 + 4 + 4 + 4),
         # read flag + has-predecessor bit per slot, one run record per run
         # (the list entries of heads of several runs are not counted)
         "k_seg_count": m * 1 + m // 8 + runs * 8,

@@ -1023,7 +1023,7 @@ __global__ void k_pair_remap(uint32_t *__restrict__ pair_home, uint32_t *__restr
 //        than one k-mer)
 // k_edge_insert claims prefixes, k_edge_lookup counts suffix hits and
 // remembers the hit position, k_edge_resolve keeps u -> v when the prefix
-// is unshared and was hit exactly once, writing link[u] = v | char(v) << 32
+// is unshared and was hit exactly once, writing link[u] = v
 // and v's has-predecessor bit.
 
 struct TEntry {
@@ -1033,7 +1033,7 @@ struct TEntry {
 };
 constexpr uint32_t CNT_SHARED = 0x80000000u;
 constexpr uint32_t OWN_MASK = 0x3FFFFFFFu;
-constexpr uint64_t NO_LINK = (uint64_t)NONE;
+constexpr uint32_t NO_LINK = NONE;
 
 // Runs: maximal chains of edges u -> v with u and v in one LDS group and u
 // not a borrowed duplicate (such edges never leave a fine bucket).  Every
@@ -1188,7 +1188,7 @@ __global__ __launch_bounds__(BLOCK) void k_edge_lookup(const uint64_t *__restric
 
 __global__ __launch_bounds__(BLOCK) void k_edge_resolve(const ETile *__restrict__ tiles, const uint32_t *__restrict__ boff,
                                                         const uint64_t *__restrict__ toff, const TEntry *table,
-                                                        const uint32_t *__restrict__ hit, uint64_t *__restrict__ link,
+                                                        const uint32_t *__restrict__ hit, uint32_t *__restrict__ link,
                                                         uint32_t *__restrict__ hasp, const uint64_t *__restrict__ kmer,
                                                         uint32_t W, uint32_t k, RunOut ro, uint32_t n_tiles)
 {
@@ -1199,19 +1199,19 @@ __global__ __launch_bounds__(BLOCK) void k_edge_resolve(const ETile *__restrict_
     const TEntry *tb = table + toff[et.bucket];
     for (uint32_t s = et.begin + threadIdx.x; s < et.end; s += BLOCK) {
         const uint32_t h = hit[s];
-        uint64_t l = NO_LINK;
+        uint32_t l = NO_LINK;
         if (h != NONE) {
             const uint2 e = *reinterpret_cast<const uint2 *>(&tb[h].own);
             if (e.y == 1u) {  // unshared prefix, exactly one suffix hit
                 const uint32_t v = base + (e.x & OWN_MASK);
-                l = (uint64_t)v | ((uint64_t)(e.x >> 30) << 32);
+                l = v;
                 atomicOr(&hasp[v >> 5], 1u << (v & 31));
             }
         }
         // an oversized fine bucket: every k-mer is a run of its own (a lent
-        // one keeps itself as its run head in the link's high half)
+        // one's link holds its run head, itself, until the splice)
         const uint8_t f = ro.flags[s];
-        link[s] = (f & F_LENT) ? (NO_LINK | ((uint64_t)s << 32)) : l;
+        link[s] = (f & F_LENT) ? s : l;
         if (!(f & F_BORROW)) {
             ro.flags[s] = (uint8_t)(f | F_RUNHEAD);
             ro.rec[s] = make_rec((uint32_t)l, 1u, s, s);
@@ -1323,7 +1323,7 @@ __host__ __device__ constexpr size_t edge_lds_bytes(uint32_t nb)
 template <int W, int EB, int GMAX>
 __global__ __launch_bounds__(EB) void k_edge_lds(const uint64_t *__restrict__ kmer, const EGroup *__restrict__ groups,
                                                     uint32_t nb, uint32_t k, uint64_t pm0, uint64_t pm1,
-                                                    uint64_t *__restrict__ link, uint32_t *__restrict__ hasp, RunOut ro)
+                                                    uint32_t *__restrict__ link, uint32_t *__restrict__ hasp, RunOut ro)
 {
     constexpr int EPT = GMAX / EB;  // slots per thread (a group holds at most GMAX)
     const uint8_t *flags = ro.flags;
@@ -1438,12 +1438,12 @@ __global__ __launch_bounds__(EB) void k_edge_lds(const uint64_t *__restrict__ km
         tgt[e] = LDS_NOHIT;
         if (i >= n) continue;
         const uint16_t hp = hitp[i];
-        uint64_t l = NO_LINK;
+        uint32_t l = NO_LINK;
         if (hp != LDS_NOHIT) {
             const uint32_t en = tab[hp];
             if (!(en & LDS_SHARED) && (en >> 24) == 1u) {
                 const uint32_t v = en & 0xFFFFu;
-                l = (uint64_t)(gr.begin + v);
+                l = gr.begin + v;
                 atomicOr(&hb[(off0 + v) >> 5], 1u << ((off0 + v) & 31));
                 if (!((fs >> (8 * e)) & F_BORROW)) tgt[e] = (uint16_t)v;
             }
@@ -1525,7 +1525,7 @@ __global__ __launch_bounds__(EB) void k_edge_lds(const uint64_t *__restrict__ km
         ro.flags[g] = (uint8_t)(f | (a == i ? F_RUNHEAD : 0));
         if (a == i)
             ro.rec[g] = make_rec(cyc ? gr.begin + tgt[e] : NONE, cyc ? 1u : (uint32_t)rl[e], g, gr.begin + sc[i]);
-        if (f & F_LENT) link[g] = NO_LINK | ((uint64_t)(gr.begin + a) << 32);
+        if (f & F_LENT) link[g] = gr.begin + a;  // its run head, until the splice
     }
 }
 
@@ -1559,7 +1559,7 @@ __device__ __forceinline__ bool slice_probe(const uint32_t *tab, const uint64_t 
 template <int W>
 __global__ __launch_bounds__(1024) void k_edge_slice(const uint64_t *__restrict__ kmer,
                                                      const SliceTask *__restrict__ tasks, uint32_t k, uint64_t pm0,
-                                                     uint64_t pm1, uint64_t *__restrict__ link,
+                                                     uint64_t pm1, uint32_t *__restrict__ link,
                                                      uint32_t *__restrict__ hasp, RunOut ro, uint32_t *__restrict__ err)
 {
     constexpr uint32_t NT = 1024;
@@ -1613,7 +1613,7 @@ __global__ __launch_bounds__(1024) void k_edge_slice(const uint64_t *__restrict_
         if (f & F_LENT) {
             key_hash<W>(prefix_key<W>(x, pm0, pm1), h, tag);
             if (fastrange(h, t.P) != t.p) continue;
-            link[s] = NO_LINK | ((uint64_t)s << 32);  // its own run head, for the splice
+            link[s] = s;  // its own run head, until the splice
             ro.flags[s] = (uint8_t)(f | F_RUNHEAD);
             ro.rec[s] = make_rec(NONE, 1u, s, s);
             ro.rch[s] = base_char(char_at<W>(x, k - 1));
@@ -1622,7 +1622,7 @@ __global__ __launch_bounds__(1024) void k_edge_slice(const uint64_t *__restrict_
         const Kmer<W> key = suffix_key<W>(x);
         key_hash<W>(key, h, tag);
         if (fastrange(h, t.P) != t.p) continue;
-        uint64_t l = NO_LINK;
+        uint32_t l = NO_LINK;
         uint32_t pos;
         if (slice_probe<W>(tab, kmer, t.begin, key, h, (tag & 0x7Fu) << 17, pm0, pm1, pos)) {
             const uint32_t e = tab[pos];
@@ -1660,16 +1660,16 @@ static_assert(CT_TILE <= 2048, "a tile's list entries hold slot offsets in 11 bi
 // the borrowed duplicate's out-edge moves to the home copy; the target keeps
 // its has-predecessor bit
 __global__ void k_splice(const uint32_t *__restrict__ pair_home, const uint32_t *__restrict__ pair_borrow, uint32_t x,
-                         uint64_t *__restrict__ link, uint64_t *__restrict__ rec)
+                         uint32_t *__restrict__ link, uint64_t *__restrict__ rec)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= x) return;
     const uint32_t h = pair_home[i];
-    const uint64_t l = link[pair_borrow[i]];
-    const uint64_t head = link[h] >> 32;  // the lent k-mer's run head
-    link[h] = (l & 0xFFFFFFFFull) | (head << 32);
+    const uint32_t l = link[pair_borrow[i]];
+    const uint32_t head = link[h];  // the lent k-mer's run head
+    link[h] = l;
     // a lent k-mer ends its run: the run continues through the duplicate's edge
-    reinterpret_cast<uint32_t *>(rec)[2ull * head] = (uint32_t)l;  // the record's next-run field
+    reinterpret_cast<uint32_t *>(rec)[2ull * head] = l;  // the record's next-run field
 }
 
 // list index for every flagged lane: one atomic per wave
@@ -1899,7 +1899,7 @@ __global__ __launch_bounds__(BLOCK) void k_cover(const uint64_t *__restrict__ re
 
 // runs that no head reaches lie on cycles; every member becomes a run of its
 // own (its character stays where it is in rch)
-__global__ void k_cycle_init(const uint64_t *__restrict__ link, uint8_t *__restrict__ flags, uint32_t m, RunOut ro,
+__global__ void k_cycle_init(const uint32_t *__restrict__ link, uint8_t *__restrict__ flags, uint32_t m, RunOut ro,
                              uint64_t *__restrict__ lab, uint32_t *__restrict__ list, uint32_t *__restrict__ n_list)
 {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1909,7 +1909,7 @@ __global__ void k_cycle_init(const uint64_t *__restrict__ link, uint8_t *__restr
     const uint32_t len = rec_len(r), pos = rec_pos(r, s);
     uint32_t u = s;
     for (uint32_t j = 0; j < len && u != NONE; j++) {
-        const uint64_t l = link[u];
+        const uint32_t l = link[u];
         flags[u] |= F_CYCLE | F_RUNHEAD;
         ro.rec[u] = make_rec((uint32_t)l, 1u, u, pos + j);
         lab[u] = (uint64_t)u | ((uint64_t)(uint32_t)l << 32);
@@ -1936,7 +1936,7 @@ __global__ void k_cycle_round(const uint32_t *__restrict__ list, uint32_t n, con
 }
 
 // predecessors of cycle vertices (the main path keeps only a bitmap)
-__global__ void k_cycle_pred(const uint32_t *__restrict__ list, uint32_t n, const uint64_t *__restrict__ link,
+__global__ void k_cycle_pred(const uint32_t *__restrict__ list, uint32_t n, const uint32_t *__restrict__ link,
                              uint32_t *__restrict__ pred)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1947,7 +1947,7 @@ __global__ void k_cycle_pred(const uint32_t *__restrict__ list, uint32_t n, cons
 
 // each cycle loses the edge into its smallest k-mer
 __global__ void k_cycle_cut(const uint32_t *__restrict__ list, uint32_t n, const uint64_t *__restrict__ lab,
-                            const uint32_t *__restrict__ pred, uint64_t *__restrict__ link,
+                            const uint32_t *__restrict__ pred, uint32_t *__restrict__ link,
                             uint64_t *__restrict__ rec, uint32_t *__restrict__ hasp, uint8_t *__restrict__ flags,
                             uint32_t *__restrict__ n_cut)
 {
@@ -2265,11 +2265,13 @@ __global__ __launch_bounds__(BLOCK) void k_set_offsets(const uint32_t *__restric
     if (threadIdx.x == 0) out[s] = tile_off[t] + acc;
 }
 
-// successor slot of every slot (for checking)
-__global__ void k_link_succ(const uint64_t *__restrict__ link, uint32_t m, uint32_t *__restrict__ succ)
+// successor slot of every slot (for checking); before the splice a lent
+// k-mer's link holds its run head and it has no successor yet
+__global__ void k_link_succ(const uint32_t *__restrict__ link, const uint8_t *__restrict__ flags, uint32_t m,
+                            bool lent_heads, uint32_t *__restrict__ succ)
 {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (uint64_t)gridDim.x * blockDim.x)
-        succ[i] = (uint32_t)link[i];
+        succ[i] = lent_heads && (flags[i] & F_LENT) ? NONE : link[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -2410,6 +2412,7 @@ struct uc_ctx {
     std::vector<uint32_t> cb_first;  // first fine bucket of every coarse bucket
     std::vector<uint32_t> fboff_h;   // fine bucket rows
     bool loaded = false, bucketed = false, edged = false, coarsened = false, emitted = false;
+    bool spliced = false;  // lent k-mers' links hold successors, no longer run heads
     bool edges_ready = false;
     std::vector<uint64_t> et_toff, et_first;
     std::vector<uint32_t> et_cap;
@@ -3291,7 +3294,7 @@ int launch_global_edges(uc_ctx *c, size_t first, size_t count)
             P<uint8_t>(c->flags), tiles, P<uint32_t>(c->fbbase), P<uint64_t>(c->toff), P<uint32_t>(c->tcap),
             P<TEntry>(c->table), c->pm0, c->pm1, P<uint32_t>(c->hit), tile_arg(c, count));
     KLAUNCH(c, "k_edge_resolve", k_edge_resolve, (xcd_grid(count), BLOCK, 0), tiles, P<uint32_t>(c->fbbase),
-            P<uint64_t>(c->toff), P<TEntry>(c->table), P<uint32_t>(c->hit), P<uint64_t>(c->link),
+            P<uint64_t>(c->toff), P<TEntry>(c->table), P<uint32_t>(c->hit), P<uint32_t>(c->link),
             P<uint32_t>(c->hasp), P<uint64_t>(c->kmer), c->W, c->k, run_out(c), tile_arg(c, count));
     return UC_OK;
 }
@@ -3306,7 +3309,7 @@ int launch_lds_edges(uc_ctx *c, size_t first, size_t count)
     constexpr int EB = W == 1 ? 512 : 1024;
     KLAUNCH(c, "k_edge_lds", (k_edge_lds<W, EB, LDS_GROUP>), ((uint32_t)count, EB, edge_lds_bytes<W>(c->lds_nb)),
             P<uint64_t>(c->kmer), P<EGroup>(c->egroups) + first, c->lds_nb, c->k, c->pm0, c->pm1,
-            P<uint64_t>(c->link), P<uint32_t>(c->hasp), run_out(c));
+            P<uint32_t>(c->link), P<uint32_t>(c->hasp), run_out(c));
     return UC_OK;
 }
 
@@ -3317,7 +3320,7 @@ int launch_slice_edges(uc_ctx *c, size_t first, size_t count)
 {
     if (count == 0) return UC_OK;
     KLAUNCH(c, "k_edge_slice", k_edge_slice<W>, ((uint32_t)count, 1024, 0), P<uint64_t>(c->kmer),
-            P<SliceTask>(c->stasks) + first, c->k, c->pm0, c->pm1, P<uint64_t>(c->link), P<uint32_t>(c->hasp),
+            P<SliceTask>(c->stasks) + first, c->k, c->pm0, c->pm1, P<uint32_t>(c->link), P<uint32_t>(c->hasp),
             run_out(c), P<uint32_t>(c->small) + C_SLICE);
     return UC_OK;
 }
@@ -3332,7 +3335,7 @@ int launch_big_edges(uc_ctx *c, size_t first, size_t count)
     CHK(allow_lds(c, k_edge_lds<W, 1024, BIG_GROUP>, lds));
     KLAUNCH(c, "k_edge_lds_big", (k_edge_lds<W, 1024, BIG_GROUP>), ((uint32_t)count, 1024, lds),
             P<uint64_t>(c->kmer), P<EGroup>(c->bgroups) + first, BIG_GROUP, c->k, c->pm0, c->pm1,
-            P<uint64_t>(c->link), P<uint32_t>(c->hasp), run_out(c));
+            P<uint32_t>(c->link), P<uint32_t>(c->hasp), run_out(c));
     return UC_OK;
 }
 
@@ -3470,7 +3473,7 @@ int prepare_edges(uc_ctx *c)
     CHK(h2d(c, c->tcap.p, c->et_cap.data(), NF * 4));
     CHK(ensure(c, c->table, total * sizeof(TEntry)));
     CHK(ensure(c, c->hit, c->m * 4));
-    CHK(ensure(c, c->link, c->m * 8));
+    CHK(ensure(c, c->link, c->m * 4));
     CHK(ensure(c, c->hasp, (c->m + 31) / 32 * 4 + 8));
     CHK(ensure(c, c->rec, c->m * 8 + 16));
     CHK(ensure(c, c->rch, c->m + 16));
@@ -3507,6 +3510,7 @@ int run_edges(uc_ctx *c, uint32_t bucket)
     const uint64_t G = (uint64_t)c->n_sets * c->B;
     const bool plain = c->opts.mode == UC_MODE_PLAIN;
     if (bucket == UC_ALL_BUCKETS) {
+        c->spliced = false;
         CHK(fill<uint32_t>(c, P<uint32_t>(c->hasp), (c->m + 31) / 32, 0u));
         if (!plain) {
             CHK(launch_lds_edges<W>(c, 0, c->grp_first[G]));
@@ -3527,7 +3531,13 @@ int run_edges(uc_ctx *c, uint32_t bucket)
     } else {
         if (bucket >= G) return fail(c, UC_E_ARG, "bucket id out of range");
         // this bucket sets only its own slots' has-predecessor bits, which
-        // start cleared from prepare_edges
+        // start cleared from prepare_edges.  After a contraction (which
+        // consumed the lent k-mers' run heads) every bucket is found again.
+        if (c->spliced) {
+            std::fill(c->bucket_done.begin(), c->bucket_done.end(), 0);
+            c->n_done = 0;
+            c->spliced = false;
+        }
         CHK(launch_lds_edges<W>(c, c->grp_first[bucket], c->grp_first[bucket + 1] - c->grp_first[bucket]));
         CHK(launch_big_edges<W>(c, c->bg_first[bucket], c->bg_first[bucket + 1] - c->bg_first[bucket]));
         CHK(launch_slice_edges<W>(c, c->sl_first[bucket], c->sl_first[bucket + 1] - c->sl_first[bucket]));
@@ -3729,7 +3739,7 @@ int cut_cycles(uc_ctx *c, uint64_t *n_cut)
     CHK(fill<uint32_t>(c, cnt + C_CUT, 1, 0u));
     uint64_t *la = P<uint64_t>(c->chainA) == c->chain ? P<uint64_t>(c->chainB) : P<uint64_t>(c->chainA);
     uint64_t *lb = P<uint64_t>(c->cycB);
-    KLAUNCH(c, "k_cycle_init", k_cycle_init, (grid_for(m), BLOCK, 0), P<uint64_t>(c->link), P<uint8_t>(c->flags), m,
+    KLAUNCH(c, "k_cycle_init", k_cycle_init, (grid_for(m), BLOCK, 0), P<uint32_t>(c->link), P<uint8_t>(c->flags), m,
             run_out(c), la, P<uint32_t>(c->clist), cnt + C_CYC);
     uint32_t nl;
     CHK(d2h(c, &nl, cnt + C_CYC, 4));
@@ -3742,9 +3752,9 @@ int cut_cycles(uc_ctx *c, uint64_t *n_cut)
     }
     if (nl) {
         KLAUNCH(c, "k_cycle_pred", k_cycle_pred, (grid_for(nl), BLOCK, 0), P<uint32_t>(c->clist), nl,
-                P<uint64_t>(c->link), P<uint32_t>(c->pred));
+                P<uint32_t>(c->link), P<uint32_t>(c->pred));
         KLAUNCH(c, "k_cycle_cut", k_cycle_cut, (grid_for(nl), BLOCK, 0), P<uint32_t>(c->clist), nl, la,
-                P<uint32_t>(c->pred), P<uint64_t>(c->link), P<uint64_t>(c->rec), P<uint32_t>(c->hasp),
+                P<uint32_t>(c->pred), P<uint32_t>(c->link), P<uint64_t>(c->rec), P<uint32_t>(c->hasp),
                 P<uint8_t>(c->flags), cnt + C_CUT);
     }
     KLAUNCH(c, "k_clear_flag", k_clear_flag, (std::min<uint32_t>(grid_for(m), 65536), BLOCK, 0), P<uint8_t>(c->flags),
@@ -3778,10 +3788,11 @@ int do_coarsen(uc_ctx *c)
     c->chain = P<uint64_t>(c->chainA);
     c->salt = (uint32_t)(c->opts.seed * 0x9E3779B97F4A7C15ull >> 32);
     c->shift = c->opts.sample_log2 ? c->opts.sample_log2 : DEFAULT_SAMPLE_LOG2;
-    if (c->x) {
+    if (c->x && !c->spliced) {  // (once per edge result: it consumes the lent k-mers' run heads)
         KLAUNCH(c, "k_splice", k_splice, (grid_for(c->x), BLOCK, 0), P<uint32_t>(c->pair_h), P<uint32_t>(c->pair_b),
-                (uint32_t)c->x, P<uint64_t>(c->link), P<uint64_t>(c->rec));
+                (uint32_t)c->x, P<uint32_t>(c->link), P<uint64_t>(c->rec));
     }
+    c->spliced = true;
     uint64_t live = 0;
     uint32_t nonconv = 0;
     if (m) CHK(contract_pass(c, &live, &nonconv));
@@ -4122,7 +4133,7 @@ int uc_succ_copy(uc_ctx *c, uint32_t *succ)
     if (!c->edged) return fail(c, UC_E_STATE, "no edge result");
     CHK(select_dev(c));
     if (c->m) KLAUNCH(c, "k_link_succ", k_link_succ, (std::min<uint32_t>(grid_for(c->m), 65536), BLOCK, 0),
-                      P<uint64_t>(c->link), (uint32_t)c->m, P<uint32_t>(c->hit));
+                      P<uint32_t>(c->link), P<uint8_t>(c->flags), (uint32_t)c->m, !c->spliced, P<uint32_t>(c->hit));
     return d2h(c, succ, c->hit.p, c->m * 4);
 }
 
