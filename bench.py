@@ -138,7 +138,22 @@ def cpu_baseline(args, l, B, gpu_ctx_kwargs):
                                                           "minitigs", "cross_finish")},
         "parity_with_gpu": parity,
         "parallel": par,
+        "host": host_info(),
     }
+
+
+def host_info():
+    """nproc, CPU model and RAM of the host the CPU baseline ran on."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        with open("/proc/cpuinfo") as f:
+            info["cpu_model"] = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), None)
+        with open("/proc/meminfo") as f:
+            kb = next((int(ln.split()[1]) for ln in f if ln.startswith("MemTotal")), 0)
+        info["ram_gib"] = round(kb / 2 ** 20, 1)
+    except OSError:
+        pass
+    return info
 
 
 def parallel_cpu_baseline(args, l, B, workers=16):
@@ -499,6 +514,7 @@ def main():
 
     b = c.bucket_info()
     ci = c.coarsen_info()
+    edges_ms = phase["edges_ms"] / args.steps
     dom_ms, dom_launches = c.kernel_stats()[dom_name]
     avg_ms = dom_ms / max(dom_launches, 1)
     # algorithmic bytes per launch: per-job bytes spread over the launches of
@@ -518,6 +534,19 @@ def main():
     text_bytes = c.text_size()
     device_gib = round(c.held_bytes() / 2 ** 30, 2)
     c.close()
+
+    # hash-table probes of the same job, counted in a step of its own (the
+    # timed steps count nothing)
+    with uc.Compactor(n_buckets=B, mode=mode, xcd_map=args.xcd_map, eager=args.eager, count_probes=True,
+                      **tun) as cp:
+        cp.load(kmers, args.k)
+        cp.compact_loaded(l)
+        probes = cp.edge_probes()
+    hashmap = {"probes_per_step": probes, "keys_per_step": 2 * (b.m_placements - b.n_cross),
+               "probes_per_s": probes / (edges_ms * 1e-3) if edges_ms > 0 else None,
+               "note": "slot inspections of the insert and lookup probes over every hash table of the edge phase, "
+                       "counted in one extra step, over the timed steps' edge-phase time (HIP events); keys = "
+                       "inserts (all but borrowed copies) + lookups (all but lent copies)"}
 
     plain = None
     if not args.no_plain and mode == uc.UC_MODE_BATCHED:
@@ -580,6 +609,7 @@ def main():
         "roofline": roof,
         "plain": plain,
         "sets": sets,
+        "hashmap": hashmap,
         "serve": serve,
         "file_to_file": f2f,
         "cpu_baseline": base,

@@ -71,6 +71,7 @@ typedef struct {
                                      staging each tile in LDS sorted by bucket */
 #define UC_FLAG_EAGER 16u /* launch the kernels of a contraction pass one by one instead of replaying
                              them as a captured HIP graph */
+#define UC_FLAG_COUNT_PROBES 32u /* count the hash-table probes of the edge phase (uc_edge_probes) */
 
 void uc_default_opts(uc_opts *o);
 int uc_ctx_create(const uc_opts *opts, uc_ctx **out);
@@ -154,6 +155,11 @@ int uc_find_edges(uc_ctx *ctx, uint32_t bucket);
 
 /* Host copy of the successor array (m entries, UINT32_MAX = none), slot ids. */
 int uc_succ_copy(uc_ctx *ctx, uint32_t *succ);
+/* Hash-table slot inspections (insert and lookup probes, over every table
+ * kind) of the last edge phase, with UC_FLAG_COUNT_PROBES (UC_E_STATE
+ * without).  No counterpart in the reference: the hashmap probes/s figure
+ * of the measurement. */
+int uc_edge_probes(uc_ctx *ctx, uint64_t *probes);
 
 /* Replaces coarseners.h:388-485 coarsen_de_bruijn_full_cycle(_final),
  * heuristics.h:205-256 coarsen_HEC, compact_graph.h and the cross-graph

@@ -330,6 +330,24 @@ def test_long_path_chain_ranking():
     assert uc.sorted_lines(got) == textbook_sorted(uc.encode(kmers, k), k)
 
 
+@pytest.mark.parametrize("l,B", [(11, 64), (1, 2), (4, 4)])
+def test_edge_probe_count_bounds(l, B):
+    # every insert and every lookup inspects at least one slot; at the load
+    # factors used (at most one half) the average stays far below four
+    kmers = uc.medium_model(91, 200_000, 31)
+    with uc.Compactor(n_buckets=B, count_probes=True) as c:
+        c.compact(kmers, 31, l)
+        b = c.bucket_info()
+        probes = c.edge_probes()
+    with uc.Compactor(n_buckets=B) as c:
+        c.compact(kmers, 31, l)
+        with pytest.raises(uc.UcError):
+            c.edge_probes()  # not counted without the flag
+    keys = 2 * (b.m_placements - b.n_cross)
+    log(f"l={l} B={B} probes={probes} keys={keys} oversized={b.n_oversized}")
+    assert keys <= probes <= 4 * keys
+
+
 def test_pass_graph_replays_match_eager():
     # the contraction pass is captured once and replayed while sizes and
     # buffers repeat; every replay, with eager work on another context in

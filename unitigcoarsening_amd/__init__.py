@@ -29,6 +29,7 @@ UC_FLAG_XCD_MAP = 2
 UC_FLAG_KEEP_SRC = 4
 UC_FLAG_DIRECT_SCATTER = 8
 UC_FLAG_EAGER = 16
+UC_FLAG_COUNT_PROBES = 32
 UC_TUNE_KEEP_BUCKETS = 1
 
 F_LENT = 1
@@ -178,6 +179,7 @@ API = {
     "uc_text_copy": (C.c_int, [_P, C.c_char_p, C.c_uint64]),
     "uc_set_text_offset": (C.c_uint64, [_P, C.c_uint32]),
     "uc_text_append_file": (C.c_int, [_P, C.c_char_p]),
+    "uc_edge_probes": (C.c_int, [_P, _U64P]),
     "uc_parse_kmer_text": (C.c_int, [C.c_char_p, C.c_uint64, C.c_uint32, C.POINTER(_U32P), _U64P]),
     "uc_read_kmer_file": (C.c_int, [C.c_char_p, C.c_uint32, C.POINTER(_U32P), _U64P]),
     "uc_free": (None, [_P]),
@@ -347,7 +349,7 @@ class Compactor:
 
     def __init__(self, device=0, seed=0x5EED, n_buckets=64, mode=UC_MODE_BATCHED, load_pct=50, sample_log2=0,
                  hbm_budget=0, profile=False, xcd_map=False, keep_src=False, fine_slots=0, group_slots=0,
-                 direct_scatter=False, eager=False):
+                 direct_scatter=False, eager=False, count_probes=False):
         L = lib()
         o = UcOpts()
         L.uc_default_opts(C.byref(o))
@@ -355,7 +357,7 @@ class Compactor:
         o.load_pct, o.sample_log2, o.hbm_budget = load_pct, sample_log2, hbm_budget
         o.flags = ((UC_FLAG_PROFILE if profile else 0) | (UC_FLAG_XCD_MAP if xcd_map else 0)
                    | (UC_FLAG_KEEP_SRC if keep_src else 0) | (UC_FLAG_DIRECT_SCATTER if direct_scatter else 0)
-                   | (UC_FLAG_EAGER if eager else 0))
+                   | (UC_FLAG_EAGER if eager else 0) | (UC_FLAG_COUNT_PROBES if count_probes else 0))
         o.fine_slots, o.group_slots = fine_slots, group_slots
         self.opts = o
         self._ctx = _P()
@@ -525,6 +527,12 @@ class Compactor:
 
     def text_size(self):
         return lib().uc_text_size(self._ctx)
+
+    def edge_probes(self):
+        """Hash-table slot inspections of the last edge phase (uc_edge_probes)."""
+        v = C.c_uint64()
+        _check(lib().uc_edge_probes(self._ctx, C.byref(v)), self._ctx)
+        return v.value
 
     def append_text_file(self, path):
         """Append the unitig text to the file at `path` (uc_text_append_file)."""

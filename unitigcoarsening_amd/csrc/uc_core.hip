@@ -1111,13 +1111,35 @@ __global__ void k_table_clear(TEntry *__restrict__ t, uint64_t n)
         reinterpret_cast<ulonglong2 *>(t)[i] = make_ulonglong2(EMPTY, 0ull);
 }
 
-template <int W>
+// Hash-table slot inspections (probes) of the edge phase, counted per
+// thread when `probes` is given (UC_FLAG_COUNT_PROBES) and added with one
+// atomic per workgroup, spread over PROBE_SLOTS counters so that workgroups
+// on different XCDs rarely meet on one (every thread of the workgroup calls
+// this; red holds one word per wave)
+constexpr uint32_t PROBE_SLOTS = 256;
+template <int NT>
+__device__ __forceinline__ void add_probes(unsigned long long *probes, uint32_t np, uint32_t *red)
+{
+    for (int o = 32; o > 0; o >>= 1) np += __shfl_xor(np, o, 64);
+    if (__lane_id() == 0) red[threadIdx.x / 64] = np;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (int w = 0; w < NT / 64; w++) t += red[w];
+        atomicAdd(probes + blockIdx.x % PROBE_SLOTS, (unsigned long long)t);
+    }
+}
+
+template <int W, bool CNT>
 __global__ __launch_bounds__(BLOCK) void k_edge_insert(const uint64_t *__restrict__ kmer, const uint8_t *__restrict__ flags,
                                                        const ETile *__restrict__ tiles, const uint32_t *__restrict__ boff,
                                                        const uint64_t *__restrict__ toff,
                                                        const uint32_t *__restrict__ tcap, TEntry *table, uint32_t k,
-                                                       uint64_t pm0, uint64_t pm1, uint32_t n_tiles)
+                                                       uint64_t pm0, uint64_t pm1, uint32_t n_tiles,
+                                                       unsigned long long *__restrict__ probes)
 {
+    __shared__ uint32_t pred_[BLOCK / 64];
+    uint32_t np = 0;
     const uint32_t ti = xcd_tile(n_tiles);
     if (ti == NONE) return;
     const ETile et = tiles[ti];
@@ -1134,6 +1156,7 @@ __global__ __launch_bounds__(BLOCK) void k_edge_insert(const uint64_t *__restric
         const uint64_t cw = claim_word<W>(key, tag, local);
         uint32_t pos = fastrange(h, cap);
         for (uint32_t probe = 0; probe < cap; probe++) {
+            if (CNT) np++;
             // claim first: a compare-and-swap returns the occupant without
             // pulling the line through a separate load
             const uint64_t cur =
@@ -1149,15 +1172,19 @@ __global__ __launch_bounds__(BLOCK) void k_edge_insert(const uint64_t *__restric
             pos = pos + 1 == cap ? 0 : pos + 1;
         }
     }
+    if (CNT) add_probes<BLOCK>(probes, np, pred_);
 }
 
-template <int W>
+template <int W, bool CNT>
 __global__ __launch_bounds__(BLOCK) void k_edge_lookup(const uint64_t *__restrict__ kmer, const uint8_t *__restrict__ flags,
                                                        const ETile *__restrict__ tiles, const uint32_t *__restrict__ boff,
                                                        const uint64_t *__restrict__ toff,
                                                        const uint32_t *__restrict__ tcap, TEntry *table, uint64_t pm0,
-                                                       uint64_t pm1, uint32_t *__restrict__ hit, uint32_t n_tiles)
+                                                       uint64_t pm1, uint32_t *__restrict__ hit, uint32_t n_tiles,
+                                                       unsigned long long *__restrict__ probes)
 {
+    __shared__ uint32_t pred_[BLOCK / 64];
+    uint32_t np = 0;
     const uint32_t ti = xcd_tile(n_tiles);
     if (ti == NONE) return;
     const ETile et = tiles[ti];
@@ -1172,6 +1199,7 @@ __global__ __launch_bounds__(BLOCK) void k_edge_lookup(const uint64_t *__restric
             key_hash<W>(key, h, tag);
             uint32_t pos = fastrange(h, cap);
             for (uint32_t probe = 0; probe < cap; probe++) {
+                if (CNT) np++;
                 const uint64_t cur = tb[pos].key;
                 if (cur == EMPTY) break;
                 if (same_key<W>(cur, key, tag, kmer, base, pm0, pm1)) {
@@ -1184,6 +1212,7 @@ __global__ __launch_bounds__(BLOCK) void k_edge_lookup(const uint64_t *__restric
         }
         hit[s] = res;
     }
+    if (CNT) add_probes<BLOCK>(probes, np, pred_);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_edge_resolve(const ETile *__restrict__ tiles, const uint32_t *__restrict__ boff,
@@ -1320,12 +1349,15 @@ __host__ __device__ constexpr size_t edge_lds_bytes(uint32_t nb)
     return (size_t)nb * (8 * W + 8 + 2) + ((size_t)nb / 32 + 2) * 4;
 }
 
-template <int W, int EB, int GMAX>
+template <int W, int EB, int GMAX, bool CNT>
 __global__ __launch_bounds__(EB) void k_edge_lds(const uint64_t *__restrict__ kmer, const EGroup *__restrict__ groups,
                                                     uint32_t nb, uint32_t k, uint64_t pm0, uint64_t pm1,
-                                                    uint32_t *__restrict__ link, uint32_t *__restrict__ hasp, RunOut ro)
+                                                    uint32_t *__restrict__ link, uint32_t *__restrict__ hasp, RunOut ro,
+                                                    unsigned long long *__restrict__ probes)
 {
     constexpr int EPT = GMAX / EB;  // slots per thread (a group holds at most GMAX)
+    __shared__ uint32_t pred_[EB / 64];
+    uint32_t np = 0;
     const uint8_t *flags = ro.flags;
     __shared__ uint32_t wsum[EB / 64];
     extern __shared__ __align__(16) unsigned char lds[];
@@ -1386,6 +1418,7 @@ __global__ __launch_bounds__(EB) void k_edge_lds(const uint64_t *__restrict__ km
         const Kmer<W> key = prefix_key<W>(xs[e], pm0, pm1);
         uint32_t pos = hpos[e];
         for (uint32_t probe = 0; probe < tcap; probe++) {
+            if (CNT) np++;
             const uint32_t cur = atomicCAS(&tab[pos], LDS_EMPTY, i);
             if (cur == LDS_EMPTY) break;
             if (key_eq<W>(prefix_key<W>(km[cur & 0xFFFFu], pm0, pm1), key)) {
@@ -1416,6 +1449,7 @@ __global__ __launch_bounds__(EB) void k_edge_lds(const uint64_t *__restrict__ km
             const Kmer<W> key = suffix_key<W>(xs[e]);
             uint32_t pos = hpos[e];
             for (uint32_t probe = 0; probe < tcap; probe++) {
+                if (CNT) np++;
                 const uint32_t en = tab[pos];
                 if (en == LDS_EMPTY) break;
                 if (key_eq<W>(prefix_key<W>(km[en & 0xFFFFu], pm0, pm1), key)) {
@@ -1527,6 +1561,7 @@ __global__ __launch_bounds__(EB) void k_edge_lds(const uint64_t *__restrict__ km
             ro.rec[g] = make_rec(cyc ? gr.begin + tgt[e] : NONE, cyc ? 1u : (uint32_t)rl[e], g, gr.begin + sc[i]);
         if (f & F_LENT) link[g] = gr.begin + a;  // its run head, until the splice
     }
+    if (CNT) add_probes<EB>(probes, np, pred_);
 }
 
 // ---- hash slices: a fine bucket above BIG_GROUP (and at most SLICE_MAX)
@@ -1539,13 +1574,14 @@ __global__ __launch_bounds__(EB) void k_edge_lds(const uint64_t *__restrict__ km
 // run of its own, as in the HBM-table form.
 constexpr uint32_t SLICE_TAGS = 0x7Fu << 17;
 
-template <int W>
+template <int W, bool CNT>
 __device__ __forceinline__ bool slice_probe(const uint32_t *tab, const uint64_t *__restrict__ kmer, uint32_t begin,
                                             const Kmer<W> &key, uint32_t h, uint32_t tg, uint64_t pm0, uint64_t pm1,
-                                            uint32_t &pos)
+                                            uint32_t &pos, uint32_t &np)
 {
     pos = h & (SLICE_TAB - 1);
     for (uint32_t probe = 0; probe < SLICE_TAB; probe++) {
+        if (CNT) np++;
         const uint32_t e = tab[pos];
         if (e == LDS_EMPTY) return false;
         if ((e & SLICE_TAGS) == tg &&
@@ -1556,14 +1592,17 @@ __device__ __forceinline__ bool slice_probe(const uint32_t *tab, const uint64_t 
     return false;
 }
 
-template <int W>
+template <int W, bool CNT>
 __global__ __launch_bounds__(1024) void k_edge_slice(const uint64_t *__restrict__ kmer,
                                                      const SliceTask *__restrict__ tasks, uint32_t k, uint64_t pm0,
                                                      uint64_t pm1, uint32_t *__restrict__ link,
-                                                     uint32_t *__restrict__ hasp, RunOut ro, uint32_t *__restrict__ err)
+                                                     uint32_t *__restrict__ hasp, RunOut ro, uint32_t *__restrict__ err,
+                                                     unsigned long long *__restrict__ probes)
 {
     constexpr uint32_t NT = 1024;
     __shared__ uint32_t tab[SLICE_TAB];
+    __shared__ uint32_t pred_[NT / 64];
+    uint32_t np = 0;
     const SliceTask t = tasks[blockIdx.x];
     const uint32_t n = t.end - t.begin;
     for (uint32_t i = threadIdx.x; i < SLICE_TAB; i += NT) tab[i] = LDS_EMPTY;
@@ -1579,6 +1618,7 @@ __global__ __launch_bounds__(1024) void k_edge_slice(const uint64_t *__restrict_
         const uint32_t tg = (tag & 0x7Fu) << 17;
         uint32_t pos = h & (SLICE_TAB - 1), probe = 0;
         for (; probe < SLICE_TAB; probe++) {
+            if (CNT) np++;
             const uint32_t cur = atomicCAS(&tab[pos], LDS_EMPTY, i | tg);
             if (cur == LDS_EMPTY) break;
             if ((cur & SLICE_TAGS) == tg &&
@@ -1599,7 +1639,7 @@ __global__ __launch_bounds__(1024) void k_edge_slice(const uint64_t *__restrict_
         uint32_t h, tag, pos;
         key_hash<W>(key, h, tag);
         if (fastrange(h, t.P) != t.p) continue;
-        if (slice_probe<W>(tab, kmer, t.begin, key, h, (tag & 0x7Fu) << 17, pm0, pm1, pos))
+        if (slice_probe<W, CNT>(tab, kmer, t.begin, key, h, (tag & 0x7Fu) << 17, pm0, pm1, pos, np))
             atomicAdd(&tab[pos], 1u << 24);
     }
     __syncthreads();
@@ -1624,7 +1664,7 @@ __global__ __launch_bounds__(1024) void k_edge_slice(const uint64_t *__restrict_
         if (fastrange(h, t.P) != t.p) continue;
         uint32_t l = NO_LINK;
         uint32_t pos;
-        if (slice_probe<W>(tab, kmer, t.begin, key, h, (tag & 0x7Fu) << 17, pm0, pm1, pos)) {
+        if (slice_probe<W, CNT>(tab, kmer, t.begin, key, h, (tag & 0x7Fu) << 17, pm0, pm1, pos, np)) {
             const uint32_t e = tab[pos];
             if (!(e & LDS_SHARED) && (e >> 24) == 1u) {  // unshared prefix, exactly one suffix hit
                 const uint32_t v = t.begin + (e & 0xFFFFu);
@@ -1639,6 +1679,7 @@ __global__ __launch_bounds__(1024) void k_edge_slice(const uint64_t *__restrict_
             ro.rch[s] = base_char(char_at<W>(x, k - 1));
         }
     }
+    if (CNT) add_probes<NT>(probes, np, pred_);
 }
 
 // ---------------------------------------------------------------------------
@@ -2436,7 +2477,7 @@ struct uc_ctx {
         cbfirst, fboff, egroups, fbbase, scan_tmp, kmer, flags, src, pair_h, pair_b, boff, sethb, setts, etiles,
         toff, tcap, table, hit, link, hasp, pred, seg, prevseg, chainA, chainB, cycB, list, clist, ulen, tsum,
         htoff, setoff, text, small, txt, txtcnt, rec, rch, tcnt, cbtf, over, setplan, colsum, bgroups, gfirst, stasks,
-        mlist, mcnt;
+        mlist, mcnt, probes;
     // pinned host chunks of the file reader and writer
     char *pin[2] = {nullptr, nullptr};
     hipEvent_t pin_ev[2] = {nullptr, nullptr};
@@ -2479,7 +2520,7 @@ static std::vector<DBuf *> all_bufs(uc_ctx *c)
             &c->cycB,   &c->list,    &c->clist,  &c->ulen,    &c->tsum,    &c->htoff,  &c->setoff,
             &c->text,   &c->small,   &c->txt,    &c->txtcnt,  &c->rec,     &c->rch,
             &c->tcnt,   &c->cbtf,    &c->over,   &c->setplan, &c->colsum, &c->bgroups, &c->gfirst, &c->stasks,
-            &c->mlist,  &c->mcnt};
+            &c->mlist,  &c->mcnt,    &c->probes};
 }
 
 static void free_pinned(uc_ctx *c)
@@ -3282,61 +3323,107 @@ int clear_table(uc_ctx *c, TEntry *t, uint64_t n)
     return UC_OK;
 }
 
-template <int W>
-int launch_global_edges(uc_ctx *c, size_t first, size_t count)
+// the edge phase's probe counters, when they are kept
+static unsigned long long *probe_counter(uc_ctx *c)
+{
+    if (!(c->opts.flags & UC_FLAG_COUNT_PROBES)) return nullptr;
+    return reinterpret_cast<unsigned long long *>(c->probes.p);
+}
+
+static int clear_probes(uc_ctx *c)
+{
+    if (!(c->opts.flags & UC_FLAG_COUNT_PROBES)) return UC_OK;
+    CHK(ensure(c, c->probes, PROBE_SLOTS * 8));
+    return fill<uint32_t>(c, P<uint32_t>(c->probes), 2 * PROBE_SLOTS, 0u);
+}
+
+template <int W, bool CNT>
+int launch_global_edges_(uc_ctx *c, size_t first, size_t count)
 {
     if (count == 0) return UC_OK;
     const ETile *tiles = P<ETile>(c->etiles) + first;
-    KLAUNCH(c, "k_edge_insert", k_edge_insert<W>, (xcd_grid(count), BLOCK, 0), P<uint64_t>(c->kmer),
+    KLAUNCH(c, "k_edge_insert", (k_edge_insert<W, CNT>), (xcd_grid(count), BLOCK, 0), P<uint64_t>(c->kmer),
             P<uint8_t>(c->flags), tiles, P<uint32_t>(c->fbbase), P<uint64_t>(c->toff), P<uint32_t>(c->tcap),
-            P<TEntry>(c->table), c->k, c->pm0, c->pm1, tile_arg(c, count));
-    KLAUNCH(c, "k_edge_lookup", k_edge_lookup<W>, (xcd_grid(count), BLOCK, 0), P<uint64_t>(c->kmer),
+            P<TEntry>(c->table), c->k, c->pm0, c->pm1, tile_arg(c, count), probe_counter(c));
+    KLAUNCH(c, "k_edge_lookup", (k_edge_lookup<W, CNT>), (xcd_grid(count), BLOCK, 0), P<uint64_t>(c->kmer),
             P<uint8_t>(c->flags), tiles, P<uint32_t>(c->fbbase), P<uint64_t>(c->toff), P<uint32_t>(c->tcap),
-            P<TEntry>(c->table), c->pm0, c->pm1, P<uint32_t>(c->hit), tile_arg(c, count));
+            P<TEntry>(c->table), c->pm0, c->pm1, P<uint32_t>(c->hit), tile_arg(c, count), probe_counter(c));
     KLAUNCH(c, "k_edge_resolve", k_edge_resolve, (xcd_grid(count), BLOCK, 0), tiles, P<uint32_t>(c->fbbase),
             P<uint64_t>(c->toff), P<TEntry>(c->table), P<uint32_t>(c->hit), P<uint32_t>(c->link),
             P<uint32_t>(c->hasp), P<uint64_t>(c->kmer), c->W, c->k, run_out(c), tile_arg(c, count));
     return UC_OK;
 }
 
-template <int W>
-int launch_lds_edges(uc_ctx *c, size_t first, size_t count)
+template <int W, bool CNT>
+int launch_lds_edges_(uc_ctx *c, size_t first, size_t count)
 {
     if (count == 0) return UC_OK;
     // threads per group: more waves per group hide the LDS and HBM latency
     // of its phases (measured: one-word k-mers 1.07 -> 0.81 ms at 512 threads,
     // two-word 11.4 -> 6.1 ms at 1024, against 256)
     constexpr int EB = W == 1 ? 512 : 1024;
-    KLAUNCH(c, "k_edge_lds", (k_edge_lds<W, EB, LDS_GROUP>), ((uint32_t)count, EB, edge_lds_bytes<W>(c->lds_nb)),
+    KLAUNCH(c, "k_edge_lds", (k_edge_lds<W, EB, LDS_GROUP, CNT>), ((uint32_t)count, EB, edge_lds_bytes<W>(c->lds_nb)),
             P<uint64_t>(c->kmer), P<EGroup>(c->egroups) + first, c->lds_nb, c->k, c->pm0, c->pm1,
-            P<uint32_t>(c->link), P<uint32_t>(c->hasp), run_out(c));
+            P<uint32_t>(c->link), P<uint32_t>(c->hasp), run_out(c), probe_counter(c));
     return UC_OK;
 }
 
 // fine buckets above BIG_GROUP and at most SLICE_MAX slots: one workgroup
 // per hash slice
-template <int W>
-int launch_slice_edges(uc_ctx *c, size_t first, size_t count)
+template <int W, bool CNT>
+int launch_slice_edges_(uc_ctx *c, size_t first, size_t count)
 {
     if (count == 0) return UC_OK;
-    KLAUNCH(c, "k_edge_slice", k_edge_slice<W>, ((uint32_t)count, 1024, 0), P<uint64_t>(c->kmer),
+    KLAUNCH(c, "k_edge_slice", (k_edge_slice<W, CNT>), ((uint32_t)count, 1024, 0), P<uint64_t>(c->kmer),
             P<SliceTask>(c->stasks) + first, c->k, c->pm0, c->pm1, P<uint32_t>(c->link), P<uint32_t>(c->hasp),
-            run_out(c), P<uint32_t>(c->small) + C_SLICE);
+            run_out(c), P<uint32_t>(c->small) + C_SLICE, probe_counter(c));
     return UC_OK;
 }
 
 // single fine buckets above the group size but at most BIG_GROUP slots: one
 // 1024-thread workgroup each, with the LDS area of BIG_GROUP slots
-template <int W>
-int launch_big_edges(uc_ctx *c, size_t first, size_t count)
+template <int W, bool CNT>
+int launch_big_edges_(uc_ctx *c, size_t first, size_t count)
 {
     if (count == 0) return UC_OK;
     const size_t lds = edge_lds_bytes<W>(BIG_GROUP);
-    CHK(allow_lds(c, k_edge_lds<W, 1024, BIG_GROUP>, lds));
-    KLAUNCH(c, "k_edge_lds_big", (k_edge_lds<W, 1024, BIG_GROUP>), ((uint32_t)count, 1024, lds),
+    CHK(allow_lds(c, (k_edge_lds<W, 1024, BIG_GROUP, CNT>), lds));
+    KLAUNCH(c, "k_edge_lds_big", (k_edge_lds<W, 1024, BIG_GROUP, CNT>), ((uint32_t)count, 1024, lds),
             P<uint64_t>(c->kmer), P<EGroup>(c->bgroups) + first, BIG_GROUP, c->k, c->pm0, c->pm1,
-            P<uint32_t>(c->link), P<uint32_t>(c->hasp), run_out(c));
+            P<uint32_t>(c->link), P<uint32_t>(c->hasp), run_out(c), probe_counter(c));
     return UC_OK;
+}
+
+// launch_global_edges with the probe-counting kernels when probes are counted
+template <int W>
+int launch_global_edges(uc_ctx *c, size_t first, size_t count)
+{
+    return probe_counter(c) ? launch_global_edges_<W, true>(c, first, count)
+                            : launch_global_edges_<W, false>(c, first, count);
+}
+
+// launch_lds_edges with the probe-counting kernels when probes are counted
+template <int W>
+int launch_lds_edges(uc_ctx *c, size_t first, size_t count)
+{
+    return probe_counter(c) ? launch_lds_edges_<W, true>(c, first, count)
+                            : launch_lds_edges_<W, false>(c, first, count);
+}
+
+// launch_slice_edges with the probe-counting kernels when probes are counted
+template <int W>
+int launch_slice_edges(uc_ctx *c, size_t first, size_t count)
+{
+    return probe_counter(c) ? launch_slice_edges_<W, true>(c, first, count)
+                            : launch_slice_edges_<W, false>(c, first, count);
+}
+
+// launch_big_edges with the probe-counting kernels when probes are counted
+template <int W>
+int launch_big_edges(uc_ctx *c, size_t first, size_t count)
+{
+    return probe_counter(c) ? launch_big_edges_<W, true>(c, first, count)
+                            : launch_big_edges_<W, false>(c, first, count);
 }
 
 // Work layout of the edge phase, once per bucketing result: consecutive
@@ -3360,6 +3447,7 @@ int prepare_edges(uc_ctx *c)
     CHK(ensure(c, c->small, C_WORDS * 4));
     uint32_t *cnt = P<uint32_t>(c->small);
     CHK(fill<uint32_t>(c, cnt + C_OVER, 1, 0u));
+    CHK(clear_probes(c));
     std::vector<uint32_t> gf(G + 1, 0);
     if (G) {
         uint32_t *gfd = P<uint32_t>(c->gfirst);
@@ -3512,6 +3600,7 @@ int run_edges(uc_ctx *c, uint32_t bucket)
     if (bucket == UC_ALL_BUCKETS) {
         c->spliced = false;
         CHK(fill<uint32_t>(c, P<uint32_t>(c->hasp), (c->m + 31) / 32, 0u));
+        CHK(clear_probes(c));
         if (!plain) {
             CHK(launch_lds_edges<W>(c, 0, c->grp_first[G]));
             CHK(launch_big_edges<W>(c, 0, c->bg_first[G]));
@@ -4125,6 +4214,19 @@ int uc_find_edges(uc_ctx *c, uint32_t bucket)
     if (!c) return UC_E_ARG;
     CHK(select_dev(c));
     return do_edges(c, bucket);
+}
+
+int uc_edge_probes(uc_ctx *c, uint64_t *probes)
+{
+    if (!c || !probes) return UC_E_ARG;
+    if (!c->edged) return fail(c, UC_E_STATE, "no edge result");
+    if (!(c->opts.flags & UC_FLAG_COUNT_PROBES)) return fail(c, UC_E_STATE, "probes are counted with UC_FLAG_COUNT_PROBES");
+    CHK(select_dev(c));
+    uint64_t v[PROBE_SLOTS];
+    CHK(d2h(c, v, c->probes.p, sizeof(v)));
+    *probes = 0;
+    for (uint64_t x : v) *probes += x;
+    return UC_OK;
 }
 
 int uc_succ_copy(uc_ctx *c, uint32_t *succ)
