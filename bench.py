@@ -84,6 +84,24 @@ def kernel_bytes(name, n, m, x, W, n_unitigs, text_bytes, k, runs):
     return table.get(name)
 
 
+def phase_roofline(n, m, k, text_bytes, ms):
+    """Compulsory HBM bytes of each phase by the reference's data layout
+    (W = 4 * ceil(k / 16) bytes per packed k-mer, T text bytes) over the
+    phase's time: bucketing B1 = nW + mW, edge finding B2 = m(4W + 4),
+    contraction B3 = 12m, emission B4 = mW + T; the north star's
+    contraction phase is B3 + B4."""
+    W = 4 * ((k + 15) // 16)
+    rows = {"bucketing": (n * W + m * W, ms["bucketize_ms"]), "edge_finding": (m * (4 * W + 4), ms["edges_ms"]),
+            "contraction": (12 * m, ms["coarsen_ms"]), "emission": (m * W + text_bytes, ms["emit_ms"]),
+            "contraction_and_emission": (12 * m + m * W + text_bytes, ms["coarsen_ms"] + ms["emit_ms"])}
+    out = {}
+    for name, (nb, t) in rows.items():
+        gbs = nb / (t * 1e-3) / 1e9 if t > 0 else None
+        out[name] = {"bytes": int(nb), "ms": round(t, 3), "achieved_gbs": round(gbs, 1) if gbs else None,
+                     "frac": round(gbs / HBM_PEAK_GBS, 4) if gbs else None}
+    return out
+
+
 def traffic_for(name):
     """Measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE,
     scaled by the copy-kernel calibration) when a summary was committed."""
@@ -609,6 +627,8 @@ def main():
         "roofline": roof,
         "plain": plain,
         "sets": sets,
+        "phase_roofline": phase_roofline(n, b.m_placements, args.k, text_bytes,
+                                         {kk: v / args.steps for kk, v in phase.items()}),
         "hashmap": hashmap,
         "serve": serve,
         "file_to_file": f2f,
