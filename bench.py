@@ -102,16 +102,20 @@ def phase_roofline(n, m, k, text_bytes, ms):
     return out
 
 
-def traffic_for(name):
+def traffic_for(name, n, k, l, buckets):
     """Measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE,
-    scaled by the copy-kernel calibration) when a summary was committed."""
+    scaled by the copy-kernel calibration) when a summary of this same
+    workload was committed, else None."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(p) as f:
             t = json.load(f)
-        return t.get("kernels", {}).get(name, {}).get("bytes_per_launch")
     except (OSError, ValueError):
         return None
+    w = t.get("workload", {})
+    if (w.get("n_kmers"), w.get("k"), w.get("l"), w.get("buckets")) != (n, k, l, buckets):
+        return None
+    return t.get("kernels", {}).get(name, {}).get("bytes_per_launch")
 
 
 def workload(args, genome_bp=None):
@@ -544,7 +548,7 @@ def main():
         per_launch = per_job / launches_per_job
         achieved = per_launch / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_for(dom_name),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_for(dom_name, n, k, l, B),
                 "kernel": dom_name, "avg_launch_ms": round(avg_ms, 4),
                 "algorithmic_bytes_per_launch": int(per_launch)}
     kernels = {name: {"ms_per_step": round(ms / warm_steps, 4), "launches_per_step": nl / warm_steps}

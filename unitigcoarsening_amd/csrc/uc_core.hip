@@ -2116,8 +2116,12 @@ __device__ __forceinline__ void put_head(char *o, uint64_t at, const Kmer<W> &x,
 }
 
 // LDS staging bytes per wave in k_head_emit (a range that does not fit is
-// written directly)
-constexpr uint32_t EMIT_STAGE = 4096;
+// written directly): 2 KiB for one-word k-mers, whose 256-thread groups
+// would lose residency with more (measured on MI355X: 2 KiB 0.95 ms, 3 KiB
+// 1.07 ms, unstaged 1.05 ms on the 50 Mbp job), 4 KiB for two-word k-mers
+// (2 KiB: 8.8 against 8.4 ms on the 300 Mbp instance)
+template <int W>
+constexpr uint32_t emit_stage() { return W == 1 ? 2048 : 4096; }
 
 // per tile: exclusive prefix of (1 << 32 | length) over heads -> text
 // offsets.  Every head writes its segment (put_head).  The 64 slots a wave
@@ -2136,7 +2140,7 @@ __global__ __launch_bounds__(NT) void k_head_emit(const uint64_t *__restrict__ k
                                                      uint32_t salt, uint32_t shift, uint64_t *__restrict__ htoff,
                                                      char *__restrict__ text, uint32_t n_tiles)
 {
-    constexpr uint32_t STG = EMIT_STAGE;
+    constexpr uint32_t STG = emit_stage<W>();
     constexpr int NW = NT / 64, CTI = CT_TILE / NT;  // waves; items per thread
     __shared__ uint64_t tile[CT_TILE + 1];  // prefixes in slot order, then the tile's end
     __shared__ uint64_t wsum[NW];
@@ -3910,11 +3914,11 @@ int run_emit(uc_ctx *c)
     CHK(ensure(c, c->text, c->text_bytes + 1));
     CHK(ensure(c, c->htoff, c->m * 8));
     if (m) {
-        // one-word k-mers go straight to HBM from 256-thread groups; wider
-        // ones are staged in LDS by 512-thread groups (each measured faster
-        // than the other choices on MI355X)
+        // one-word k-mers from 256-thread groups, wider ones from 512-thread
+        // groups, both staged in LDS (each measured faster than the other
+        // choices on MI355X)
         constexpr int NT = W == 1 ? 256 : 512;
-        KLAUNCH(c, "k_head_emit", (k_head_emit<W, (W > 1), NT>), (xcd_grid(T), NT, 0), P<uint64_t>(c->kmer),
+        KLAUNCH(c, "k_head_emit", (k_head_emit<W, true, NT>), (xcd_grid(T), NT, 0), P<uint64_t>(c->kmer),
                 P<uint64_t>(c->rec), P<char>(c->rch), P<uint32_t>(c->hasp), P<uint8_t>(c->flags),
                 P<uint64_t>(c->mlist), P<uint32_t>(c->mcnt), P<uint32_t>(c->ulen), P<uint64_t>(c->tsum), m, c->k,
                 c->salt, c->shift, P<uint64_t>(c->htoff), P<char>(c->text), tile_arg(c, T));
