@@ -88,8 +88,8 @@ def phase_roofline(n, m, k, text_bytes, ms):
     """Compulsory HBM bytes of each phase by the reference's data layout
     (W = 4 * ceil(k / 16) bytes per packed k-mer, T text bytes) over the
     phase's time: bucketing B1 = nW + mW, edge finding B2 = m(4W + 4),
-    contraction B3 = 12m, emission B4 = mW + T; the north star's
-    contraction phase is B3 + B4."""
+    contraction B3 = 12m, emission B4 = mW + T; contraction together with
+    emission is B3 + B4."""
     W = 4 * ((k + 15) // 16)
     rows = {"bucketing": (n * W + m * W, ms["bucketize_ms"]), "edge_finding": (m * (4 * W + 4), ms["edges_ms"]),
             "contraction": (12 * m, ms["coarsen_ms"]), "emission": (m * W + text_bytes, ms["emit_ms"]),
@@ -477,7 +477,7 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the path shards by minimizer bucket "
-              "across GPUs by design but this build runs and reports a single MI355X.", file=sys.stderr)
+              "across GPUs by design but this library runs and reports a single MI355X.", file=sys.stderr)
         sys.exit(2)
 
     ensure_built()

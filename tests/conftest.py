@@ -21,7 +21,7 @@ def _built():
 
 
 def pytest_sessionstart(session):
-    # build only when nothing is built (the GPU box runs on prebuilt files)
+    # build only when nothing is built (a machine without the toolchain runs on prebuilt files)
     if not _built():
         from unitigcoarsening_amd.build import build_all
 

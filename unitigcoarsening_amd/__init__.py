@@ -651,7 +651,7 @@ _COMP = str.maketrans("ACGT", "TGCA")
 
 def canonical_sorted(text: bytes) -> list:
     """Each unitig in canonical orientation (min of it and its reverse
-    complement), then sorted -- the comparison the north star names."""
+    complement), then sorted: the multiset comparison of unitig sets."""
     out = []
     for s in text.decode().splitlines():
         r = s.translate(_COMP)[::-1]
