@@ -131,3 +131,12 @@ def test_parallel_cpu_baseline_runs_independent_samples():
     want = sum(uc.medium_model(5 + i, 100_000, 31).shape[0] for i in range(2))
     assert res["workers"] == 2 and res["kmers"] == want
     assert res["kmers_per_s"] == pytest.approx(res["kmers"] / res["wall_s"])
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # the bench line is one GPU's; a request for more exits with a message
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr and r.stdout == ""
