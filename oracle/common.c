@@ -82,11 +82,11 @@ int orc_parse_text(const char *buf, size_t len, uint32_t k, uint32_t **packed, u
     return ORC_OK;
 }
 
-/* Seeded bijection on 2l-bit codes: xor with the seed, then three rounds of
- * (multiply by an odd constant modulo 2^(2l), xorshift right).  Every step
- * is invertible on [0, 4^l), so the composition is a permutation.  The
- * constants are odd and below 2^24 so a GPU may use 24-bit multiplies when
- * 2l <= 24 and get the same residues. */
+/* Seeded bijection on 2l-bit codes: xor with the seed, then two rounds of
+ * (multiply by an odd constant modulo 2^(2l), xorshift right), then xor with
+ * the seed's high word.  Every step is invertible on [0, 4^l), so the
+ * composition is a permutation.  The constants are odd and below 2^24 so a
+ * GPU may use 24-bit multiplies when 2l <= 24 and get the same residues. */
 uint32_t orc_lmer_order(uint32_t code, uint32_t l, uint64_t seed)
 {
     uint32_t bits = 2 * l;
@@ -98,7 +98,5 @@ uint32_t orc_lmer_order(uint32_t code, uint32_t l, uint64_t seed)
     x = (x * ORC_MIX_C2) & mask;
     x ^= x >> h2;
     x ^= (uint32_t)(seed >> 32) & mask;
-    x = (x * ORC_MIX_C3) & mask;
-    x ^= x >> h1;
     return x;
 }

@@ -7,7 +7,6 @@
 /* odd multipliers below 2^24 of the l-mer order bijection */
 #define ORC_MIX_C1 0x3779B1u
 #define ORC_MIX_C2 0xEBCA6Bu
-#define ORC_MIX_C3 0xD4EB2Fu
 
 typedef unsigned __int128 orc_u128;
 

@@ -165,7 +165,10 @@ __host__ __device__ constexpr int lmin_mode(uint32_t l, bool table)
 }
 
 // seeded bijection on 2l-bit l-mer codes (same function as the oracle's
-// orc_lmer_order): xor seed, three rounds of odd multiply + xorshift
+// orc_lmer_order): xor seed, two rounds of odd multiply + xorshift, xor the
+// second seed word.  A third round cost 25% of the minimizer scan (for
+// 2l > 24 its multiplies are quarter-rate v_mul_lo_u32) and left the bucket
+// sizes where they were.
 template <int MODE>
 __device__ __forceinline__ uint32_t lmer_order_m(uint32_t code, const LminParams &p)
 {
@@ -177,17 +180,12 @@ __device__ __forceinline__ uint32_t lmer_order_m(uint32_t code, const LminParams
             x = __umul24(x, 0x3779B1u) & p.lmask;
             x ^= x >> p.h1;
             x = __umul24(x, 0xEBCA6Bu) & p.lmask;
-            x ^= (x >> p.h2) ^ p.s1;
-            x = __umul24(x, 0xD4EB2Fu) & p.lmask;
         } else {
             x = (x * 0x3779B1u) & p.lmask;
             x ^= x >> p.h1;
             x = (x * 0xEBCA6Bu) & p.lmask;
-            x ^= (x >> p.h2) ^ p.s1;
-            x = (x * 0xD4EB2Fu) & p.lmask;
         }
-        x ^= x >> p.h1;
-        return x;
+        return x ^ (x >> p.h2) ^ p.s1;
     }
 }
 
