@@ -121,12 +121,15 @@ template <int W>
 __device__ __forceinline__ void key_hash(const Kmer<W> &x, uint32_t &h, uint32_t &tag)
 {
     const uint32_t a = (uint32_t)x.w[0], b = (uint32_t)(x.w[0] >> 32);
+    // the position: one finalizer over the words folded by odd multiplies
+    // (a finalizer per word measured 2% slower in edge finding on two-word
+    // k-mers); the tag keeps its own chain
     if constexpr (W == 1) {
-        h = fmix32(a ^ fmix32(b ^ 0x9E3779B9u));
+        h = fmix32(a ^ (b * 0x9E3779B9u));
         tag = fmix32(b ^ fmix32(a ^ 0x7FEB352Du)) & TAG_MASK;
     } else {
         const uint32_t c = (uint32_t)x.w[1], d = (uint32_t)(x.w[1] >> 32);
-        h = fmix32(a ^ fmix32(b ^ fmix32(c ^ fmix32(d ^ 0x9E3779B9u))));
+        h = fmix32(a ^ (b * 0x9E3779B9u) ^ fmix32(c ^ (d * 0x85EBCA77u)));
         tag = fmix32(d ^ fmix32(c ^ fmix32(b ^ fmix32(a ^ 0x7FEB352Du)))) & TAG_MASK;
     }
 }
