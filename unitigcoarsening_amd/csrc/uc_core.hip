@@ -178,7 +178,7 @@ __device__ __forceinline__ uint32_t lmer_order_m(uint32_t code, const LminParams
     if constexpr (MODE == LM_TABLE) {
         return p.table[code];
     } else {
-        uint32_t x = (code ^ p.s0) & p.lmask;
+        uint32_t x = code ^ p.s0;  // both below 4^l: no mask needed
         if constexpr (MODE == LM_MUL24) {
             x = __umul24(x, 0x3779B1u) & p.lmask;
             x ^= x >> p.h1;
