@@ -263,25 +263,38 @@ def bench_sets(args):
             for _ in range(reps):
                 run_plain()
             plain = (time.perf_counter() - t0) / reps
-    log(f"plain per-bucket, set by set: {plain * 1e3:.2f} ms")
+            # the outputs, untimed: every set's batched text against its plain run
+            batched_text = cb.text()
+            bounds = [L.uc_set_text_offset(cb._ctx, i) for i in range(len(sizes) + 1)]
+            equal = 0
+            for i, sz in enumerate(sizes):
+                rc = L.uc_compact(cp._ctx, C.c_void_p(base + int(offs[i]) * 8 * W), sz, k, l, 1)
+                if rc != 0:
+                    raise uc.UcError(rc, L.uc_last_error(cp._ctx).decode())
+                equal += cp.text() == batched_text[bounds[i]:bounds[i + 1]]
+    log(f"plain per-bucket, set by set: {plain * 1e3:.2f} ms; {equal} of {len(sizes)} set texts byte-equal")
+    if equal != len(sizes):
+        raise RuntimeError(f"batched and plain texts differ on {len(sizes) - equal} sets")
     return {"workload": f"{len(sizes)} sets, log-normal sizes (median 200k, sigma 1, clipped 10k-5M k-mers), "
                         f"medium repeat/error model, seeds 1000..{1000 + len(sizes) - 1}, k={k}, l={l}, {B} buckets",
             "n_kmers": n, "batched_ms": round(batched * 1e3, 3), "plain_ms": round(plain * 1e3, 3),
             "batched_value": n / batched, "plain_value": n / plain, "batched_over_plain": round(plain / batched, 3),
+            "texts_byte_equal": f"{equal} of {len(sizes)} sets",
             "note": "inputs resident in HBM for both forms; the plain form runs uc_compact per set on device "
-                    "pointers with one launch per bucket"}
+                    "pointers with one launch per bucket; after timing, every set's batched text is compared "
+                    "byte for byte with its plain run"}
 
 
-def serve_pool(n_inputs, seed0, lo=10_000, hi=2_000_000):
+def serve_pool(n_inputs, seed0, lo=100_000, hi=50_000_000):
     """Distinct job inputs for the request loop: sizes log-uniform in
-    [lo, hi] k-mers, k alternating 31 / 63, medium repeat/error model."""
+    [lo, hi] k-mers, k drawn from {31, 63}, medium repeat/error model."""
     import numpy as np
     import unitigcoarsening_amd as uc
 
     rng = np.random.default_rng(seed0)
     pool = []
     for i in range(n_inputs):
-        k = 31 if i % 2 == 0 else 63
+        k = int(rng.choice([31, 63]))
         size = int(np.exp(rng.uniform(np.log(lo), np.log(hi))))
         pool.append((uc.medium_model(seed0 + i, int(size / 0.97) + k, k), k, 11 if k == 31 else 13))
     return pool
@@ -345,38 +358,25 @@ def bench_file(args, kmers, l, B):
         os.rmdir(d)
 
 
-def bench_serve(args):
-    """Request loop (uc_server): Poisson arrivals at a fraction of the
-    measured one-context capacity, jobs drawn from a pool of inputs;
-    latency = submit to result, over every job."""
+def serve_run(args, pool, sizes, n_slots, load, n_jobs, mean_svc, seed):
+    """Poisson arrivals at `load` x one context's capacity onto a server of
+    n_slots workers; latency = submit to result of every job."""
     import threading
 
     import numpy as np
     import unitigcoarsening_amd as uc
 
-    pool = serve_pool(16, 5000)
-    sizes = np.array([p[0].shape[0] for p in pool])
-    # one context, jobs back to back: service time per input
-    svc = []
-    with uc.Compactor(n_buckets=args.buckets, sample_log2=args.sample_log2) as c:
-        for km, k, l in pool:
-            c.compact(km, k, l)
-        for km, k, l in pool:
-            t0 = time.perf_counter()
-            c.compact(km, k, l)
-            svc.append(time.perf_counter() - t0)
-    mean_svc = float(np.mean(svc))
-    rng = np.random.default_rng(6)
-    picks = rng.integers(0, len(pool), args.serve_jobs)
-    rate = args.serve_load / mean_svc  # jobs per second offered
-    gaps = rng.exponential(1.0 / rate, args.serve_jobs)
-    lat, done_bytes = [], []
-    with uc.Server(n_slots=args.serve_slots, hbm_budget=args.serve_budget_gib << 30, n_buckets=args.buckets,
+    rng = np.random.default_rng(seed)
+    picks = rng.integers(0, len(pool), n_jobs)
+    rate = load / mean_svc  # jobs per second offered
+    gaps = rng.exponential(1.0 / rate, n_jobs)
+    lat = []
+    with uc.Server(n_slots=n_slots, hbm_budget=int(args.serve_budget_gb * 1e9), n_buckets=args.buckets,
                    sample_log2=args.sample_log2) as srv:
         # steady state: every input once per slot, largest first, so each
         # worker context has grown its buffers before the timed arrivals
         order = sorted(pool, key=lambda p: -uc.job_bytes_estimate(p[0].shape[0], p[1]))
-        warm = [srv.submit(km, k, l) for km, k, l in order for _ in range(args.serve_slots)]
+        warm = [srv.submit(km, k, l) for km, k, l in order for _ in range(n_slots)]
         for i in warm:
             srv.result(i)
         ids = []
@@ -384,13 +384,12 @@ def bench_serve(args):
 
         def waiter():
             i = 0
-            while i < args.serve_jobs:
+            while i < n_jobs:
                 while len(ids) <= i:
                     time.sleep(0.0002)
                 try:
                     text, qms, rms = srv.result(ids[i])
                     lat.append(qms + rms)
-                    done_bytes.append(len(text))
                 except Exception as e:  # reported after the run
                     waiter_err.append(repr(e))
                 i += 1
@@ -399,7 +398,7 @@ def bench_serve(args):
         th.start()
         t0 = time.perf_counter()
         nxt = t0
-        for j in range(args.serve_jobs):
+        for j in range(n_jobs):
             nxt += gaps[j]
             dt = nxt - time.perf_counter()
             if dt > 0:
@@ -413,16 +412,56 @@ def bench_serve(args):
         raise RuntimeError(f"serving jobs failed: {waiter_err[:3]}")
     lat = np.array(lat)
     served = int(sizes[picks].sum())
-    log(f"serve: {args.serve_jobs} jobs in {wall:.2f}s, p50 {np.percentile(lat, 50):.2f} ms, "
-        f"p99 {np.percentile(lat, 99):.2f} ms")
-    return {"workload": f"{args.serve_jobs} jobs, Poisson arrivals at {args.serve_load:.2f} x one-context capacity, "
-                        f"drawn from 16 inputs of log-uniform 10k-2M k-mers (medium model, k 31/63 alternating, "
-                        f"seeds 5000..5015), {args.serve_slots} worker slots, budget {args.serve_budget_gib} GiB",
-            "offered_jobs_per_s": round(rate, 2), "mean_service_ms_one_context": round(1e3 * mean_svc, 3),
+    log(f"serve {n_slots} slot(s) at {load:.2f}x: {n_jobs} jobs in {wall:.2f}s, {served / wall / 1e6:.1f} M k-mers/s, "
+        f"p50 {np.percentile(lat, 50):.2f} ms, p99 {np.percentile(lat, 99):.2f} ms")
+    return {"n_slots": n_slots, "offered_load": load, "offered_jobs_per_s": round(rate, 2), "jobs": n_jobs,
             "kmers_per_s": served / wall, "p50_ms": round(float(np.percentile(lat, 50)), 3),
             "p99_ms": round(float(np.percentile(lat, 99)), 3), "max_ms": round(float(lat.max()), 3),
-            "peak_held_gib": round(st["peak_held_bytes"] / 2 ** 30, 3),
-            "note": "latency includes the host copy of the input and of the result text"}
+            "peak_held_gib": round(st["peak_held_bytes"] / 2 ** 30, 3)}
+
+
+def bench_serve(args):
+    """Request loop (uc_server) at the serving configuration: inputs of
+    log-uniform 100k-50M k-mers with k drawn from {31, 63}, an HBM budget
+    of 288 GB, Poisson arrivals at a fixed fraction of one context's
+    capacity; then a saturation point (1.5x that capacity) with one worker
+    slot and with four."""
+    import numpy as np
+    import unitigcoarsening_amd as uc
+
+    t = time.time()
+    pool = serve_pool(args.serve_inputs, 5000)
+    sizes = np.array([p[0].shape[0] for p in pool])
+    log(f"serve pool: {len(pool)} inputs, {sizes.sum()} k-mers, sizes {sizes.min()}..{sizes.max()} "
+        f"in {time.time() - t:.1f}s")
+    # one context, jobs back to back, through the same host boundary as the
+    # server (input copied in, text copied out): service time per input
+    svc = []
+    with uc.Compactor(n_buckets=args.buckets, sample_log2=args.sample_log2) as c:
+        for km, k, l in pool:
+            c.compact(km, k, l)
+        for km, k, l in pool:
+            t0 = time.perf_counter()
+            c.compact(km, k, l)
+            svc.append(time.perf_counter() - t0)
+    mean_svc = float(np.mean(svc))
+    main_run = serve_run(args, pool, sizes, args.serve_slots, args.serve_load, args.serve_jobs, mean_svc, 6)
+    sat = None
+    if args.serve_saturation_jobs:
+        sat = {f"slots_{ns}": serve_run(args, pool, sizes, ns, 1.5, args.serve_saturation_jobs, mean_svc, 7)
+               for ns in (1, args.serve_slots)}
+        sat["slots_ratio"] = round(sat[f"slots_{args.serve_slots}"]["kmers_per_s"] / sat["slots_1"]["kmers_per_s"], 3)
+    return {"workload": f"{args.serve_jobs} jobs, Poisson arrivals at {args.serve_load:.2f} x one-context capacity, "
+                        f"drawn from {len(pool)} inputs of log-uniform 100k-50M k-mers (medium model, k drawn from "
+                        f"{{31, 63}}, seeds 5000..{5000 + len(pool) - 1}), {args.serve_slots} worker slots, budget "
+                        f"{args.serve_budget_gb:g} GB",
+            "pool_kmers": [int(x) for x in sizes], "pool_k": [p[1] for p in pool],
+            "mean_service_ms_one_context": round(1e3 * mean_svc, 3),
+            **{kk: v for kk, v in main_run.items() if kk not in ("n_slots",)},
+            "saturation": sat,
+            "note": "latency is submit to result and includes the host copies of the input and of the result "
+                    "text; the saturation point offers 1.5x one context's capacity to one worker slot and to "
+                    f"{args.serve_slots}: served k-mers/s above the one-slot figure is work overlapped on the device"}
 
 
 def main():
@@ -454,10 +493,13 @@ def main():
     ap.add_argument("--no-file", action="store_true", help="skip the file-to-file run")
     ap.add_argument("--eager", action="store_true", help="launch the contraction pass kernel by kernel, no graph")
     ap.add_argument("--no-ceiling", action="store_true", help="skip the stream-copy ceiling measurement")
-    ap.add_argument("--serve-jobs", type=int, default=400)
+    ap.add_argument("--serve-jobs", type=int, default=300)
+    ap.add_argument("--serve-inputs", type=int, default=16, help="distinct job inputs drawn from")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-load", type=float, default=0.7, help="offered load, x one-context capacity")
-    ap.add_argument("--serve-budget-gib", type=int, default=64)
+    ap.add_argument("--serve-saturation-jobs", type=int, default=150,
+                    help="jobs of each saturation run (1.5x load, 1 slot and --serve-slots); 0 skips them")
+    ap.add_argument("--serve-budget-gb", type=float, default=288.0)
     ap.add_argument("--large", action="store_true",
                     help="the large instance's model (k=63, l=13, 256 buckets, repeat families) at --genome-bp")
     args = ap.parse_args()

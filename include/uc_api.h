@@ -134,6 +134,11 @@ typedef struct {
     uint32_t lds_limit;      /* LDS bytes a workgroup may use on the device */
     uint32_t n_oversized;    /* fine buckets above the LDS group size: edge finding uses HBM tables */
     uint64_t oversized_slots; /* slots in those fine buckets */
+    /* how the edge phase takes the oversized fine buckets (0 until it was planned):
+       alone in a 4096-slot LDS area, split into LDS hash slices, or in HBM tables */
+    uint64_t n_big_lds;
+    uint64_t n_slice_tasks;
+    uint64_t n_hbm_tables;
 } uc_bucket_info;
 
 int uc_bucket_info_get(uc_ctx *ctx, uc_bucket_info *info);
@@ -303,8 +308,16 @@ int uc_ctx_trim(uc_ctx *ctx);
 /* Device bytes a context holds right now. */
 uint64_t uc_ctx_held_bytes(const uc_ctx *ctx);
 
-/* Upper estimate of the device bytes one job of n k-mers needs. */
+/* Upper bound of the device bytes one job of n k-mers needs, whatever its
+ * crossing rate (at most 2n placements). */
 uint64_t uc_job_bytes_estimate(uint64_t n, uint32_t k);
+
+/* UC_OK when a job of n k-mers, x of them crossing, fits the 32-bit slot ids
+ * (n + x <= 2^32 - 2 placements); UC_E_LIMIT when it does not (the job then
+ * fails with UC_E_LIMIT after bucket counting, before any slot is written);
+ * UC_E_STATE when x > n.  Host only; replaces nothing in the reference,
+ * whose `HUGE` build widens its ids instead (defs.h:30-33). */
+int uc_check_placements(uint64_t n, uint64_t x);
 
 /* ---- serving --------------------------------------------------------------
  * No counterpart in the reference, which runs one job per process.  A server
@@ -344,7 +357,8 @@ int uc_result(uc_server *srv, uint64_t job, char *dst, uint64_t cap, uint64_t *s
               double *run_ms);
 /* Forgets a finished job and its text. */
 int uc_release(uc_server *srv, uint64_t job);
-/* Peak device bytes the workers held together, and jobs completed. */
+/* Peak device bytes the workers held together (every allocation and free
+ * of every worker context is counted as it happens), and jobs completed. */
 int uc_server_stats(uc_server *srv, uint64_t *peak_held, uint64_t *jobs_done);
 
 #ifdef __cplusplus

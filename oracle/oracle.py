@@ -54,7 +54,8 @@ class OrcStats(C.Structure):
     ]
 
 
-ORC_E = {-1: "ORC_E_ARG", -2: "ORC_E_NOMEM", -3: "ORC_E_FORMAT", -4: "ORC_E_SELFLOOP", -5: "ORC_E_DUP"}
+ORC_E = {-1: "ORC_E_ARG", -2: "ORC_E_NOMEM", -3: "ORC_E_FORMAT", -4: "ORC_E_SELFLOOP", -5: "ORC_E_DUP",
+         -6: "ORC_E_HANG"}
 
 
 class OracleError(RuntimeError):
@@ -96,6 +97,8 @@ def lib():
         L.orc_fnv_suf.argtypes = [C.c_void_p, C.c_uint32]
         L.orc_parse_text.restype = C.c_int
         L.orc_parse_text.argtypes = [C.c_char_p, C.c_size_t, C.c_uint32, C.POINTER(_U32P), _U64P]
+        L.orc_ref_load_kmers.restype = C.c_int
+        L.orc_ref_load_kmers.argtypes = [C.c_char_p, C.c_size_t, C.c_uint32, C.POINTER(_U32P), _U64P]
         L.orc_pack.restype = C.c_int
         L.orc_pack.argtypes = [C.c_char_p, C.c_uint64, C.c_uint32, C.c_void_p]
         L.orc_free.restype = None
@@ -212,10 +215,21 @@ def fnv_suf(kmer_words):
 
 
 def parse_text(text, k):
+    """This build's text rule (every line, whatever the count)."""
+    return _parse(lib().orc_parse_text, text, k)
+
+
+def ref_load_kmers(text, k):
+    """The reference's chunked read loop (driver.cpp:257-354); OracleError
+    with rc ORC_E_HANG where that loop would not end."""
+    return _parse(lib().orc_ref_load_kmers, text, k)
+
+
+def _parse(fn, text, k):
     L = lib()
     p = _U32P()
     n = C.c_uint64()
-    rc = L.orc_parse_text(text, len(text), k, C.byref(p), C.byref(n))
+    rc = fn(text, len(text), k, C.byref(p), C.byref(n))
     if rc:
         raise OracleError(rc)
     w = (k + 15) // 16

@@ -45,6 +45,7 @@ extern "C" {
 #define ORC_E_FORMAT (-3)   /* malformed text input */
 #define ORC_E_SELFLOOP (-4) /* input holds a self-looping k-mer: the reference's HEC never terminates */
 #define ORC_E_DUP (-5)      /* input holds the same k-mer twice */
+#define ORC_E_HANG (-6)     /* the reference's read loop would never end on this text */
 
 /* ---- shared helpers --------------------------------------------------- */
 
@@ -54,9 +55,16 @@ uint32_t orc_words_per_kmer(uint32_t k);
 /* Text input in the reference format (driver.cpp:251-354): line 1 holds the
  * k-mer count, each later line yields its first k characters.  Bytes other
  * than A/C/G/T map to 0 ('A') like the zero-initialised 256-entry table of
- * driver.cpp:228-232.  Reading stops after `count` k-mers.  Lines shorter
- * than k characters are rejected (ORC_E_FORMAT).  *packed is malloc'd. */
+ * driver.cpp:228-232.  Every line is read whatever the count says (the
+ * reference only compares the count after whole read chunks); newlines and
+ * carriage returns at the end of the text end it.  Other lines shorter than
+ * k characters are rejected (ORC_E_FORMAT).  *packed is malloc'd. */
 int orc_parse_text(const char *buf, size_t len, uint32_t k, uint32_t **packed, uint64_t *n);
+
+/* driver.cpp:257-354 load_kmers restated with its 64 read chunks, the count
+ * test after each chunk and no line-length check: the k-mers the reference
+ * reads from this text, or ORC_E_HANG where its loop would not end. */
+int orc_ref_load_kmers(const char *buf, size_t sz, uint32_t k, uint32_t **packed, uint64_t *n);
 
 /* driver.cpp:227-249 compact_kmers: n*k ASCII characters -> packed words. */
 int orc_pack(const char *chars, uint64_t n, uint32_t k, uint32_t *packed);

@@ -401,3 +401,135 @@ def test_small_groups_route_fine_buckets_to_every_path(k, l, mode):
     if l <= 3:
         assert info.largest_fine_bucket > 4096
     assert uc.sorted_lines(got) == textbook_sorted(kmers, k)
+
+
+# ---------------------------------------------------------------- the configured workloads, reduced
+
+
+def shuffled(kmers, seed):
+    """The same k-mer set in a seeded random order (the generator and the
+    golden files are sorted; slot order, LDS groups, runs and the stable
+    scatters all follow input order)."""
+    return np.ascontiguousarray(kmers[np.random.default_rng(seed).permutation(kmers.shape[0])])
+
+
+@pytest.fixture(scope="module")
+def c1_case():
+    # 1 Mbp uniform ACGT (seed 1), every distinct canonical 31-mer, l = 11
+    kmers = uc.synth_kmers(1, 1_000_000, 31)
+    tb = textbook_sorted(kmers, 31)
+    ref, _ = oracle.refpipe(kmers, 31, 11, 16, oracle.generate_permutation(11, 1))
+    assert uc.sorted_lines(ref) == tb
+    return kmers, tb
+
+
+@pytest.mark.parametrize("mode", [uc.UC_MODE_BATCHED, uc.UC_MODE_PLAIN])
+def test_config_uniform_1mbp_text_16_buckets(c1_case, mode, tmp_path):
+    # as k-mer text (count line, one k-mer per line) through the file reader
+    kmers, tb = c1_case
+    path = tmp_path / "c1.txt"
+    path.write_bytes(uc.format_text(kmers, 31))
+    log(f"uniform 1 Mbp: {kmers.shape[0]} k-mers, mode {mode}")
+    with uc.Compactor(n_buckets=16, mode=mode) as c:
+        c.load_file(str(path), 31)
+        c.compact_loaded(11)
+        got = c.text()
+    assert uc.sorted_lines(got) == tb
+
+
+def test_config_uniform_1mbp_shuffled(c1_case):
+    kmers, tb = c1_case
+    with uc.Compactor(n_buckets=16) as c:
+        got = c.compact(shuffled(kmers, 1), 31, 11)
+    assert uc.sorted_lines(got) == tb
+
+
+def uneven_sets(n_sets, seed0, median=2000, lo=100, hi=50_000):
+    # sizes ~ LogNormal(ln median, 1) clipped to [lo, hi] k-mers, each from the
+    # medium repeat/error model; every other set shuffled
+    rng = np.random.default_rng(seed0)
+    sizes = np.clip(np.exp(rng.normal(np.log(median), 1.0, n_sets)), lo, hi).astype(np.int64)
+    sets = []
+    for i, sz in enumerate(sizes):
+        s = uc.medium_model(seed0 + i, int(sz / 0.97) + 31, 31)
+        sets.append(shuffled(s, i) if i % 2 else s)
+    return sets
+
+
+def test_config_256_uneven_sets_batched():
+    # 256 sets in one batched run: every set's text equals the textbook's
+    # lines and, byte for byte, a plain per-bucket run of that set alone
+    sets = uneven_sets(256, 1000)
+    log(f"256 sets, {sum(s.shape[0] for s in sets)} k-mers, largest {max(s.shape[0] for s in sets)}")
+    with uc.Compactor(n_buckets=64) as c:
+        texts = c.compact_sets(sets, 31, 11)
+    assert len(texts) == 256
+    with uc.Compactor(n_buckets=64, mode=uc.UC_MODE_PLAIN) as p:
+        for i, (s, t) in enumerate(zip(sets, texts)):
+            assert t == p.compact(s, 31, 11), i
+            assert uc.sorted_lines(t) == textbook_sorted(s, 31), i
+
+
+@pytest.fixture(scope="module")
+def c4_case():
+    # the large instance's repeat-family model at 5 Mbp (45% repeats, copy
+    # number 2-1000, divergence 0.5-15%, 0.5% errors), canonical 63-mers
+    kmers = uc.large_model(4, 5_000_000, 63)
+    return kmers, textbook_sorted(kmers, 63)
+
+
+@pytest.mark.parametrize("order", ["sorted", "shuffled"])
+def test_config_repeat_families_k63_256_buckets(c4_case, order):
+    # l = 13, 256 buckets: repeat families make single minimizers frequent,
+    # so fine buckets above the LDS group size go alone into the larger LDS
+    # area and, above 4096 slots, into LDS hash slices
+    kmers, tb = c4_case
+    km = kmers if order == "sorted" else shuffled(kmers, 4)
+    with uc.Compactor(n_buckets=256) as c:
+        got = c.compact(km, 63, 13)
+        b = c.bucket_info()
+    log(f"repeat families {order}: n={b.n_kmers} largest fine {b.largest_fine_bucket} big {b.n_big_lds} "
+        f"slices {b.n_slice_tasks} hbm {b.n_hbm_tables}")
+    assert b.n_big_lds > 0 and b.n_slice_tasks > 0
+    assert uc.sorted_lines(got) == tb
+    if order == "sorted":
+        ref, _ = oracle.refpipe(kmers, 63, 13, 256, oracle.generate_permutation(13, 4))
+        assert uc.sorted_lines(ref) == tb
+
+
+def test_config_repeat_families_hbm_tables(c4_case):
+    # l = 5 leaves a few hundred distinct minimizers: fine buckets of more
+    # than 65535 slots take the HBM hash tables
+    kmers, tb = c4_case
+    with uc.Compactor(n_buckets=256) as c:
+        got = c.compact(shuffled(kmers, 9), 63, 5)
+        b = c.bucket_info()
+    log(f"repeat families l=5: largest fine {b.largest_fine_bucket} hbm tables {b.n_hbm_tables}")
+    assert b.n_hbm_tables > 0 and b.largest_fine_bucket > 65535
+    assert uc.sorted_lines(got) == tb
+
+
+@pytest.mark.parametrize("k", [15, 31, 33, 47, 63])
+def test_genome_shuffled_input(k):
+    rng = np.random.default_rng(k + 500)
+    kmers = uc.synth_kmers(int(rng.integers(1 << 30)), 60_000, k, repeat_frac=0.05, seg_min=200, seg_max=800,
+                           repeat_sub=0.01, error_frac=0.01)
+    km = shuffled(kmers, k)
+    log(f"shuffled genome k={k} n={km.shape[0]}")
+    got = run(km, k, min(11, k - 1))
+    assert uc.sorted_lines(got) == textbook_sorted(kmers, k)
+    # the same multiset of lines as the sorted input's run
+    assert uc.sorted_lines(got) == uc.sorted_lines(run(kmers, k, min(11, k - 1)))
+
+
+@pytest.mark.parametrize("k,l,B", [(31, 11, 64), (63, 13, 256)])
+def test_medium_model_shuffled_vs_textbook_and_refpipe(k, l, B):
+    kmers = shuffled(uc.medium_model(k + B + 1, 400_000, k), 7)
+    log(f"medium shuffled k={k} l={l} B={B} n={kmers.shape[0]}")
+    got = run(kmers, k, l, n_buckets=B)
+    tb = textbook_sorted(kmers, k)
+    assert uc.sorted_lines(got) == tb
+    ref, _ = oracle.refpipe(kmers, k, l, 64, oracle.generate_permutation(l, 5))
+    assert uc.sorted_lines(ref) == tb
+    # plain form byte-identical to batched on the same (unsorted) slots
+    assert run(kmers, k, l, n_buckets=B, mode=uc.UC_MODE_PLAIN) == got

@@ -27,18 +27,25 @@ def jobs_mix():
 
 
 def test_estimate_bounds_held_bytes():
-    for glen, k in [(5000, 15), (50000, 31), (200000, 31), (80000, 63), (120000, 47)]:
+    # the last job: k - l = 2 makes most k-mers cross (x/n near its bound of
+    # one), at a size where the fixed part of the estimate is under a tenth
+    for glen, k, l in [(5000, 15, 7), (50000, 31, 11), (200000, 31, 11), (80000, 63, 11), (120000, 47, 11),
+                       (3_000_000, 15, 13)]:
         km = uc.medium_model(7, glen, k)
-        log(f"estimate n={km.shape[0]} k={k}")
+        log(f"estimate n={km.shape[0]} k={k} l={l}")
         with uc.Compactor() as c:
-            c.compact(km, k, 11 if k > 15 else 7)
+            c.compact(km, k, l)
             held = c.held_bytes()
             est = uc.job_bytes_estimate(km.shape[0], k)
+            b = c.bucket_info()
+            log(f"  x/n = {b.n_cross / max(b.n_kmers, 1):.2f}, held {held}, estimate {est}")
             assert 0 < held <= est, (glen, k, held, est)
+            if glen >= 3_000_000:
+                assert (64 << 20) < est // 10 and b.n_cross > b.n_kmers // 2
             c.trim()
             assert c.held_bytes() == 0
             # the context works again after a trim
-            text = c.compact(km, k, 11 if k > 15 else 7)
+            text = c.compact(km, k, l)
             assert text.count(b"\n") > 0
 
 

@@ -33,16 +33,17 @@ def expect(text, k):
 
 
 CASES = [
-    (b"3\nACGTN 17\nacgtT\nTTTT\nGGGG\n", 4),      # trailing fields, lowercase and N read as A, count caps
+    (b"3\nACGTN 17\nacgtT\nTTTT\nGGGG\n", 4),      # trailing fields, lowercase and N read as A, lines past the count
     (b"2\r\nACGT\r\nCCCC\r\n", 4),                  # CRLF
     (b"9\nACGT\nTTTT", 4),                          # no final newline, count above the lines
     (b"0\n", 4),                                    # empty body
     (b"5", 4),                                      # count line only, no newline
     (b"  2\t\nACGT\nCCCC\n", 4),                    # blanks around the count
-    (b"1\nACGT\nAC\n", 4),                          # a short line past the count is never read
+    (b"1\nACGT\nCCCC\n\n\r\n\n", 4),               # newlines and CRs at the end end the text
     (b"2\nACGT\r\nCCCCG\r", 4),                     # '\r' at the end without a newline
     (b"2\nAC\rT\nCCCC\n", 4),                       # '\r' inside a line reads as A
     (b"2\nAC\n", 2),                                # k = 2
+    (b"0\n" + b"\n" * 5000, 4),                     # only newlines after the count, over a 4 KiB step
 ]
 
 
@@ -54,7 +55,8 @@ def test_edge_cases(text, k):
 @pytest.mark.parametrize("text,k", [
     (b"2\nACGT\nAC\n", 4),          # short line
     (b"2\nACGT\nACG\r\n", 4),       # k - 1 characters and a '\r'
-    (b"2\nACGT\n\n", 4),            # empty line inside the count
+    (b"2\nACGT\n\nCCCC\n", 4),      # empty line before another line
+    (b"1\nACGT\nAC\n", 4),          # a short line past the count is read too
     (b"3\nACGT\nTTTT\nGG", 4),      # short last line without newline
     (b"ACGT\n", 4),                 # no count line
 ])
@@ -65,6 +67,36 @@ def test_format_errors(text, k):
     with pytest.raises(uc.UcError) as e:
         device_parse(text, k)
     assert e.value.status == host == uc.UC_E_FORMAT
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_device_parse_matches_reference_loop(seed):
+    # the reference's own chunked read loop (driver.cpp:257-354, restated in
+    # the oracle): wherever it ends and does not stop at a chunk end on a
+    # count met by chance, its k-mers are the device parser's; where it
+    # stops early, ours start with its k-mers
+    import random
+
+    rng = random.Random(100 + seed)
+    k = rng.choice([4, 15, 31, 63])
+    n = rng.randint(200, 3000)
+    for count in (n, rng.randint(0, n - 1), n + 7):
+        for trailing in (True, False):
+            lines = ["".join(rng.choice("ACGT") for _ in range(k)) + ("\t%d" % rng.randint(1, 99) if rng.random() < 0.3
+                                                                        else "") for _ in range(n)]
+            text = (f"{count}\n" + "\n".join(lines) + ("\n" if trailing else "")).encode()
+            got = device_parse(text, k)
+            assert np.array_equal(got, expect(text, k))
+            assert got.shape[0] == n
+            try:
+                ref = oracle.ref_load_kmers(text, k)
+            except oracle.OracleError as e:
+                assert e.rc == -6 and count != n  # the reference would not end
+                continue
+            log(f"k={k} lines={n} count={count} trailing={trailing}: reference read {ref.shape[0]}")
+            assert np.array_equal(ref, got[:ref.shape[0]])
+            if count == n:
+                assert ref.shape[0] == n
 
 
 @pytest.mark.parametrize("k", [2, 4, 15, 16, 17, 31, 32, 33, 47, 63, 64])

@@ -57,10 +57,11 @@ def test_parser_matches_oracle(k):
 
 
 def test_parser_edge_cases():
-    # trailing fields are skipped, lowercase and N read as A, count caps input
+    # trailing fields are skipped, lowercase and N read as A; every line is
+    # read whatever the count says (the reference reads on past it)
     t = b"3\nACGTN 17\nacgtT\nTTTT\nGGGG\n"
     a = uc.parse_text(t, 4)
-    assert uc.decode(a, 4) == ["ACGT", "AAAA", "TTTT"]
+    assert uc.decode(a, 4) == ["ACGT", "AAAA", "TTTT", "GGGG"]
     assert np.array_equal(a, oracle.parse_text(t, 4))
     # CRLF line ends
     t = b"2\r\nACGT\r\nCCCC\r\n"
@@ -76,6 +77,33 @@ def test_parser_edge_cases():
     assert uc.decode(uc.parse_text(b"9\nACGT\nTTTT", 4), 4) == ["ACGT", "TTTT"]
     # empty body
     assert uc.parse_text(b"0\n", 4).shape == (0, 1)
+    # newlines and carriage returns at the end end the text; an empty line
+    # before another line is a short line
+    assert uc.decode(uc.parse_text(b"1\nACGT\n\n\r\n", 4), 4) == ["ACGT"]
+    with pytest.raises(uc.UcError):
+        uc.parse_text(b"2\nACGT\n\nCCCC\n", 4)
+
+
+def test_placement_limit_check():
+    # every k-mer has one home slot and crossing k-mers one more: n + x
+    # placements must stay below 2^32 - 1 (32-bit slot ids)
+    L = uc.lib()
+    assert L.uc_check_placements(10, 3) == uc.UC_OK
+    assert L.uc_check_placements(2**32 - 2, 0) == uc.UC_OK
+    assert L.uc_check_placements(2**31, 2**31 - 2) == uc.UC_OK
+    assert L.uc_check_placements(2**31, 2**31 - 1) == uc.UC_E_LIMIT
+    # a job of 3.9e9 k-mers with 10% crossing: the sum wraps a 32-bit count
+    assert L.uc_check_placements(3_900_000_000, 400_000_000) == uc.UC_E_LIMIT
+    assert L.uc_check_placements(5, 6) == uc.UC_E_STATE
+
+
+def test_job_estimate_bounds_every_crossing_rate():
+    # the estimate assumes up to two placements per k-mer; at any size it
+    # must be at least that many slots' worth of the per-slot buffers
+    for n, k in ((0, 31), (1000, 15), (50_000_000, 31), (50_000_000, 63)):
+        est = uc.job_bytes_estimate(n, k)
+        W = 1 if k <= 32 else 2
+        assert est >= 2 * n * (16 * W + 100) + n * (8 * W + k)
 
 
 def test_parser_large_multithreaded():

@@ -74,7 +74,8 @@ def test_encode_matches_oracle_pack():
 def test_parse_text_reference_format():
     text = b"3\nACGTACGTACGTACG 12\nTTTTTTTTTTTTTTTAAA\nCCCCCCCCCCCCCCC\nGGGGGGGGGGGGGGG\n"
     got = oracle.parse_text(text, 15)
-    assert uc.decode(got, 15) == ["ACGTACGTACGTACG", "TTTTTTTTTTTTTTT", "CCCCCCCCCCCCCCC"]
+    # the count (3) does not stop the read: the reference reads on past it
+    assert uc.decode(got, 15) == ["ACGTACGTACGTACG", "TTTTTTTTTTTTTTT", "CCCCCCCCCCCCCCC", "GGGGGGGGGGGGGGG"]
 
 
 def test_parse_text_short_line_rejected():
@@ -85,6 +86,75 @@ def test_parse_text_short_line_rejected():
 def test_parse_text_count_larger_than_file():
     got = oracle.parse_text(b"10\nACGT\nTTTT", 4)
     assert uc.decode(got, 4) == ["ACGT", "TTTT"]
+
+
+def kmer_text(rng, n_lines, k, count, trailing_nl, extra=False):
+    lines = []
+    for _ in range(n_lines):
+        s = "".join(rng.choice("ACGT") for _ in range(k))
+        if extra and rng.random() < 0.5:
+            s += "\t%d" % rng.randint(1, 999)
+        lines.append(s)
+    return (f"{count}\n" + "\n".join(lines) + ("\n" if trailing_nl else "")).encode(), lines
+
+
+def test_reference_read_loop_known_answers():
+    # driver.cpp:257-354 on texts small enough to follow by hand:
+    # chunk_size = sz // 64, and a text of fewer than about 64 lines has
+    # chunks too small to hold one k-mer line: the loop never advances
+    # unless the count is met at once
+    with pytest.raises(oracle.OracleError) as e:
+        oracle.ref_load_kmers(b"2\nACGT\nCCCC\n", 4)
+    assert e.value.rc == -6
+    assert oracle.ref_load_kmers(b"0\nACGT\nCCCC\n", 4).shape[0] == 0  # count met before any k-mer
+    # 200 lines of k = 4 under a 3-digit count: sz = 1004, chunk_size = 15.
+    # Chunk 0 holds the count line and 2 k-mer lines, every later chunk 3
+    # (it starts at the newline after the last k-mer taken), so the totals
+    # after chunks 0, 1, 2 ... are 2, 5, 8 ...
+    rng = random.Random(1)
+    text, lines = kmer_text(rng, 200, 4, 200, True)
+    assert text[:4] == b"200\n" and len(text) == 1004
+    assert uc.decode(oracle.ref_load_kmers(text, 4), 4) == lines
+    # count 5 is met after chunk 1: the loop stops there
+    assert uc.decode(oracle.ref_load_kmers(b"005" + text[3:], 4), 4) == lines[:5]
+    # count 6 is never met: after the last k-mer only the final newline is
+    # left, a pass over it takes nothing, and it repeats
+    with pytest.raises(oracle.OracleError):
+        oracle.ref_load_kmers(b"006" + text[3:], 4)
+    # without the final newline (sz = 1003, chunk_size still 15) the last
+    # chunk ends exactly at the end of the file and every line is read
+    assert uc.decode(oracle.ref_load_kmers(b"006" + text[3:-1], 4), 4) == lines
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_parse_rule_matches_reference_loop(seed):
+    # The build reads every line whatever the count says.  The reference's
+    # own loop does the same wherever it ends, except when the count happens
+    # to equal the k-mers read at the end of one of its 64 chunks: it stops
+    # there (a documented departure), and our k-mers start with its k-mers.
+    rng = random.Random(seed)
+    k = rng.choice([4, 15, 31, 33, 63])
+    n_lines = rng.randint(100, 2000)
+    for count_kind in ("equal", "below", "above"):
+        for trailing in (True, False):
+            count = {"equal": n_lines, "below": rng.randint(0, n_lines - 1),
+                     "above": n_lines + rng.randint(1, 50)}[count_kind]
+            text, lines = kmer_text(rng, n_lines, k, count, trailing, extra=seed % 2 == 1)
+            ours = oracle.parse_text(text, k)
+            assert np.array_equal(ours, uc.parse_text(text, k))
+            assert uc.decode(ours, k) == [s[:k] for s in lines]
+            try:
+                ref = oracle.ref_load_kmers(text, k)
+            except oracle.OracleError as e:
+                # bytes left after the last k-mer (a final newline, or the
+                # fields after it) make the last pass take nothing, forever
+                assert e.rc == -6 and count_kind != "equal"
+                continue
+            if ref.shape[0] == n_lines:
+                assert np.array_equal(ref, ours)
+            else:
+                assert count_kind == "below" and ref.shape[0] == count
+                assert np.array_equal(ref, ours[:count])
 
 
 # ---------------------------------------------------------------- hashing

@@ -72,6 +72,9 @@ class UcBucketInfo(C.Structure):
         ("lds_limit", C.c_uint32),
         ("n_oversized", C.c_uint32),
         ("oversized_slots", C.c_uint64),
+        ("n_big_lds", C.c_uint64),
+        ("n_slice_tasks", C.c_uint64),
+        ("n_hbm_tables", C.c_uint64),
     ]
 
 
@@ -196,6 +199,7 @@ API = {
     "uc_ctx_trim": (C.c_int, [_P]),
     "uc_ctx_held_bytes": (C.c_uint64, [_P]),
     "uc_job_bytes_estimate": (C.c_uint64, [C.c_uint64, C.c_uint32]),
+    "uc_check_placements": (C.c_int, [C.c_uint64, C.c_uint64]),
     "uc_server_create": (C.c_int, [C.POINTER(UcServerOpts), C.POINTER(_P)]),
     "uc_server_destroy": (C.c_int, [_P]),
     "uc_submit": (C.c_int, [_P, _P, C.c_uint64, C.c_uint32, C.c_uint32, _U64P]),

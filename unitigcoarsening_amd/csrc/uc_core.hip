@@ -37,6 +37,7 @@
 #include <vector>
 
 #include "../../include/uc_api.h"
+#include "uc_internal.h"
 #include "uc_kernels.h"
 
 namespace uc {
@@ -2456,6 +2457,7 @@ struct uc_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     size_t held = 0;
+    uc_meter *meter = nullptr;  // a server's account of its workers' device bytes
 
     // job
     uint32_t k = 0, l = 0, W = 1, B = 64, n_sets = 0;
@@ -2598,6 +2600,7 @@ int ensure(uc_ctx *c, DBuf &b, size_t bytes)
         HIPCHK(c, hipStreamSynchronize(c->stream));
         HIPCHK(c, hipFree(b.p));
         c->held -= b.cap;
+        if (c->meter) c->meter->sub(b.cap);
         b.p = nullptr;
         b.cap = 0;
     }
@@ -2607,6 +2610,7 @@ int ensure(uc_ctx *c, DBuf &b, size_t bytes)
     if (e != hipSuccess) return fail(c, UC_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
     b.cap = bytes;
     c->held += bytes;
+    if (c->meter) c->meter->add(bytes);
     return UC_OK;
 }
 
@@ -2935,30 +2939,40 @@ int parse_text(uc_ctx *c, uint64_t len, uint32_t k)
         if (q) nl = (uint64_t)((const char *)q - h.data());
         got += step;
     }
-    uint64_t p = 0, count = 0;
+    // the count is read, but every line is taken whatever it says: the
+    // reference compares it with the k-mers read only after whole read
+    // chunks (driver.cpp:343) and reads on past it otherwise
+    uint64_t p = 0;
     bool digit = false;
     while (p < got && (h[p] == ' ' || h[p] == '\t')) p++;
     while (p < got && h[p] >= '0' && h[p] <= '9') {
-        count = count * 10 + (uint64_t)(h[p] - '0');
         digit = true;
         p++;
     }
     if (!digit) return fail(c, UC_E_FORMAT, "no k-mer count on the first line");
     const uint64_t body = nl < len ? nl + 1 : len;
-    uint64_t lines = 0, n = 0, nt = 0, tile0 = 0;
-    if (body < len) {
-        char last = 0;
-        CHK(d2h(c, &last, txt + len - 1, 1));
+    // newlines and carriage returns at the end of the text end it
+    uint64_t end = len;
+    while (end > body) {
+        const uint64_t step = std::min<uint64_t>(end - body, 4096);
+        std::vector<char> tail(step);
+        CHK(d2h(c, tail.data(), txt + end - step, step));
+        uint64_t i = step;
+        while (i > 0 && (tail[i - 1] == '\n' || tail[i - 1] == '\r')) i--;
+        end -= step - i;
+        if (i > 0) break;
+    }
+    uint64_t n = 0, nt = 0, tile0 = 0;
+    if (body < end) {
         tile0 = body / TXT_TILE;
-        nt = (len + TXT_TILE - 1) / TXT_TILE - tile0;
+        nt = (end + TXT_TILE - 1) / TXT_TILE - tile0;
         CHK(ensure(c, c->txtcnt, (nt + 2) * 4));
         uint32_t *cnt = P<uint32_t>(c->txtcnt);
-        KLAUNCH(c, "k_txt_count", k_txt_count, ((uint32_t)nt, BLOCK, 0), txt, body, len, tile0, cnt);
+        KLAUNCH(c, "k_txt_count", k_txt_count, ((uint32_t)nt, BLOCK, 0), txt, body, end, tile0, cnt);
         CHK(scan_excl<uint32_t>(c, cnt, cnt, nt, cnt + nt, c->scan_tmp));
         uint32_t total = 0;
         CHK(d2h(c, &total, cnt + nt, 4));
-        lines = (uint64_t)total + (last != '\n' ? 1 : 0);
-        n = std::min<uint64_t>(count, lines);
+        n = (uint64_t)total + 1;  // the last line ends at `end`, not at a newline
     }
     const uint64_t offs[2] = {0, n};
     CHK(begin_job(c, offs, 1, k));
@@ -2966,7 +2980,7 @@ int parse_text(uc_ctx *c, uint64_t len, uint32_t k)
     if (n) {
         uint32_t *cnt = P<uint32_t>(c->txtcnt);
         CHK(fill<uint32_t>(c, cnt + nt + 1, 1, 0u));
-        KLAUNCH(c, "k_txt_pack", k_txt_pack<W>, ((uint32_t)nt, BLOCK, 0), txt, body, len, tile0, cnt, n, k,
+        KLAUNCH(c, "k_txt_pack", k_txt_pack<W>, ((uint32_t)nt, BLOCK, 0), txt, body, end, tile0, cnt, n, k,
                 P<uint64_t>(c->in), cnt + nt + 1);
         uint32_t err = 0;
         CHK(d2h(c, &err, cnt + nt + 1, 4));
@@ -3112,6 +3126,20 @@ int append_text(uc_ctx *c, const char *path)
 
 // ---- bucketing ---------------------------------------------------------------
 
+// Every k-mer gets one home slot and every crossing k-mer one more, so the
+// placements are m = n + x exactly.  The 32-bit histogram scan wraps when
+// that reaches 2^32; the count is therefore taken from n and x (x <= n, a
+// 32-bit tile scan of crossing flags) and checked before any buffer is sized
+// or any slot is written.
+int check_placements(uc_ctx *c, uint64_t n, uint64_t x)
+{
+    const int rc = uc_check_placements(n, x);
+    if (rc == UC_E_LIMIT)
+        return fail(c, rc, "more than 2^32-2 placements (" + std::to_string(n + x) + "): slot ids are 32-bit");
+    if (rc != UC_OK) return fail(c, rc, "crossing count above the k-mer count");
+    return UC_OK;
+}
+
 template <int W>
 int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
 {
@@ -3206,9 +3234,9 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     HIPCHK(c, hipMemcpyAsync(cnt + C_TOT + 1, P<uint32_t>(c->xhist) + T, 4, hipMemcpyDeviceToDevice, c->stream));
     uint32_t tot[2];
     CHK(d2h(c, tot, cnt + C_TOT, 8));
-    c->m = tot[0];
+    CHK(check_placements(c, c->n, tot[1]));
     c->x = tot[1];
-    if (c->m >= 0xFFFFFFFFull) return fail(c, UC_E_LIMIT, "more than 2^32-2 placements");
+    c->m = c->n + c->x;
     const uint64_t G = (uint64_t)S * B;
     DBuf &k1 = any_fine ? c->kmer2 : c->kmer, &f1 = any_fine ? c->flags2 : c->flags, &s1 = any_fine ? c->src2 : c->src;
     CHK(ensure(c, k1, std::max<uint64_t>(c->m, 1) * W * 8));
@@ -4076,6 +4104,7 @@ int uc_ctx_destroy(uc_ctx *c)
     drop_pass_graph(c);
     for (DBuf *b : all_bufs(c))
         if (b->p) (void)hipFree(b->p);
+    if (c->meter) c->meter->sub(c->held);
     free_pinned(c);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -4192,6 +4221,10 @@ int uc_bucket_info_get(uc_ctx *c, uc_bucket_info *info)
     info->largest_fine_bucket = 0;
     info->n_oversized = 0;
     info->oversized_slots = 0;
+    const uint64_t G = (uint64_t)c->n_sets * c->B;
+    info->n_big_lds = c->edges_ready ? c->bg_first[G] : 0;
+    info->n_slice_tasks = c->edges_ready ? c->sl_first[G] : 0;
+    info->n_hbm_tables = c->edges_ready ? c->fb_first[G] : 0;
     const uint64_t group_max = c->opts.group_slots ? c->opts.group_slots : LDS_GROUP;
     for (size_t f = 0; f + 1 < c->fboff_h.size(); f++) {
         const uint64_t sz = c->fboff_h[f + 1] - c->fboff_h[f];
@@ -4449,6 +4482,7 @@ int uc_ctx_trim(uc_ctx *c)
         b->cap = 0;
     }
     free_pinned(c);
+    if (c->meter) c->meter->sub(c->held);
     c->held = 0;
     c->kin = nullptr;
     reset_job(c);
@@ -4457,18 +4491,42 @@ int uc_ctx_trim(uc_ctx *c)
 
 uint64_t uc_ctx_held_bytes(const uc_ctx *c) { return c ? c->held : 0; }
 
+}  // extern "C"
+
+void uc_ctx_set_meter(uc_ctx *c, uc_meter *m)
+{
+    if (!c) return;
+    if (c->meter) c->meter->sub(c->held);
+    c->meter = m;
+    if (m) m->add(c->held);
+}
+
+extern "C" {
+
+int uc_check_placements(uint64_t n, uint64_t x)
+{
+    if (x > n) return UC_E_STATE;
+    if (n + x >= 0xFFFFFFFFull) return UC_E_LIMIT;
+    return UC_OK;
+}
+
 uint64_t uc_job_bytes_estimate(uint64_t n, uint32_t k)
 {
-    // per k-mer: input copy + bucket codes + the text of n isolated k-mers
-    // (the most unitigs a job can have); per slot (at most 1.25 n slots in
-    // the minimizer schemes used, crossing k-mers ~9% at k = 31): k-mers and
-    // flags twice, fine index, permutation, link, hit, bitmap, four segment
-    // arrays, two chain buffers, list, head offsets, run record, run
-    // characters, run head of lent k-mers, the per-tile head lists
+    // An upper bound whatever the crossing rate: every k-mer has one home
+    // slot and at most one borrowed copy, so m = n + x <= 2n slots.
+    // Per k-mer: the input copy, the bucket codes and the text of n isolated
+    // k-mers (the most text a job can have).  Per slot: k-mers and flags
+    // twice, fine index, permutation, slot sources (UC_FLAG_KEEP_SRC), the
+    // fine-sort histogram (at most 4 bytes per fine bucket and 2048-slot
+    // tile), hit, link, bitmap, run record and character, HBM hash tables
+    // (at most two 16-byte entries per key), the contraction arrays
+    // (segment, list entry, previous segment, unitig length, two chain
+    // buffers, list, and the cycle buffers), head text offsets.
     const uint64_t W = k <= 32 ? 1 : 2;
     const uint64_t per_kmer = 8 * W + 8 + (k + 1);
-    const uint64_t per_slot = 16 * W + 2 + 2 + 4 + 8 + 4 + 1 + 16 + 16 + 4 + 8 + 8 + 1 + 4 + 8;
-    return n * per_kmer + (n + n / 4) * per_slot + (64ull << 20);
+    const uint64_t per_slot = (16 * W + 2) + 2 + 4 + 8 + 2 + (4 + 4 + 1 + 8 + 1) + 32 +
+                              (8 + 8 + 4 + 4 + 8 + 8 + 4 + 8 + 4 + 4) + 8;
+    return n * per_kmer + 2 * n * per_slot + (64ull << 20);
 }
 
 }  // extern "C"

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/uc_api.h"
+#include "uc_internal.h"
 
 namespace {
 
@@ -33,12 +34,20 @@ struct Job {
 
 struct Worker {
     uc_ctx *ctx = nullptr;
-    bool busy = false;
-    uint64_t reserved = 0;  // bytes granted to the running job
+    bool busy = false;      // running a job or giving its buffers back
+    bool trim_req = false;  // asked, while idle, to give its buffers back
+    uint64_t held = 0;      // device bytes its context held when it last went idle
+    uint64_t reserved = 0;  // while busy: the most its context can hold meanwhile
 };
 
 }  // namespace
 
+// Every field below is guarded by `mu` except the contexts themselves: a
+// worker's context is used only by that worker's thread (creation and
+// destruction aside), and what the others need to know of it -- the bytes it
+// held when it went idle, the bound while it runs -- is published under `mu`.
+// The true peak of device bytes over all workers comes from the meter the
+// contexts update at every allocation and free.
 struct uc_server {
     uc_server_opts opts{};
     std::mutex mu;
@@ -47,35 +56,41 @@ struct uc_server {
     std::unordered_map<uint64_t, Job *> jobs;
     std::vector<Worker> workers;
     std::vector<std::thread> threads;
-    uint64_t next_id = 1, peak_held = 0, done = 0;
+    uc_meter meter;
+    uint64_t next_id = 1, done = 0;
     bool stopping = false;
 
-    // device bytes the other workers hold or have been granted
-    uint64_t held_by_others(size_t self) const
+    // device bytes the other workers hold, or may grow to while busy
+    uint64_t others(size_t self) const
     {
         uint64_t s = 0;
         for (size_t i = 0; i < workers.size(); i++)
-            if (i != self) s += std::max(workers[i].reserved, uc_ctx_held_bytes(workers[i].ctx));
+            if (i != self) s += workers[i].busy ? workers[i].reserved : workers[i].held;
         return s;
     }
 
-    // called with mu held by idle worker `self`: can the queue head run here?
-    // A worker keeps its buffers from job to job (no reallocation while the
-    // budget allows); they can grow by at most the job's estimate.  When the
-    // budget is short, this worker's buffers go back first, then those of
-    // idle workers.
-    bool admit(size_t self, Job *j)
+    // With mu held, for idle worker `self` and the queue head: 1 = start it
+    // keeping this worker's buffers (they grow by at most the job's
+    // estimate), 2 = start it after this worker gives its buffers back, 0 =
+    // not yet.  In the last case idle workers holding memory are asked to
+    // give it back; admission is tried again when they have.
+    int admit(size_t self, const Job *j)
     {
         const uint64_t budget = opts.hbm_budget;
-        if (!budget) return true;
-        const uint64_t need = j->estimate;
-        uc_ctx *mine = workers[self].ctx;
-        if (held_by_others(self) + uc_ctx_held_bytes(mine) + need <= budget) return true;
-        if (uc_ctx_held_bytes(mine)) uc_ctx_trim(mine);
-        if (held_by_others(self) + need <= budget) return true;
-        for (size_t i = 0; i < workers.size(); i++)
-            if (i != self && !workers[i].busy && uc_ctx_held_bytes(workers[i].ctx)) uc_ctx_trim(workers[i].ctx);
-        return held_by_others(self) + need <= budget;
+        if (!budget) return 1;
+        const uint64_t o = others(self);
+        if (o + workers[self].held + j->estimate <= budget) return 1;
+        if (o + j->estimate <= budget) return 2;
+        bool asked = false;
+        for (size_t i = 0; i < workers.size(); i++) {
+            Worker &x = workers[i];
+            if (i != self && !x.busy && x.held && !x.trim_req) {
+                x.trim_req = true;
+                asked = true;
+            }
+        }
+        if (asked) cv_work.notify_all();
+        return 0;
     }
 
     void run(size_t self)
@@ -83,9 +98,29 @@ struct uc_server {
         Worker &w = workers[self];
         std::unique_lock<std::mutex> lk(mu);
         for (;;) {
+            int how = 0;
             cv_work.wait(lk, [&] {
-                return stopping || (!queue.empty() && !w.busy && admit(self, queue.front()));
+                if (w.trim_req || (stopping && queue.empty())) return true;
+                if (queue.empty()) return false;
+                how = admit(self, queue.front());
+                return how != 0;
             });
+            if (w.trim_req) {
+                // give the buffers back outside the lock (hipFree waits for the device)
+                w.trim_req = false;
+                if (w.held) {
+                    w.busy = true;
+                    w.reserved = w.held;
+                    lk.unlock();
+                    uc_ctx_trim(w.ctx);
+                    lk.lock();
+                    w.held = 0;
+                    w.busy = false;
+                    w.reserved = 0;
+                    cv_work.notify_all();
+                }
+                continue;
+            }
             if (queue.empty()) {
                 if (stopping) return;
                 continue;
@@ -93,17 +128,19 @@ struct uc_server {
             Job *j = queue.front();
             queue.pop_front();
             w.busy = true;
-            w.reserved = uc_ctx_held_bytes(w.ctx) + j->estimate;  // what admit() counted
+            w.reserved = (how == 1 ? w.held : 0) + j->estimate;  // what admit() counted
             j->state = UC_JOB_RUNNING;
             j->started = Clock::now();
             lk.unlock();
 
+            if (how == 2) uc_ctx_trim(w.ctx);
             int rc = uc_compact(w.ctx, j->packed.data(), j->n, j->k, j->l, 0);
             std::vector<char> text;
             if (rc == UC_OK) {
                 text.resize(uc_text_size(w.ctx));
                 rc = uc_text_copy(w.ctx, text.data(), text.size());
             }
+            const uint64_t held_now = uc_ctx_held_bytes(w.ctx);
 
             lk.lock();
             j->text.swap(text);
@@ -111,12 +148,10 @@ struct uc_server {
             j->state = rc == UC_OK ? UC_JOB_DONE : UC_JOB_FAILED;
             j->finished = Clock::now();
             std::vector<uint32_t>().swap(j->packed);
-            uint64_t held = 0;
-            for (auto &x : workers) held += uc_ctx_held_bytes(x.ctx);
-            peak_held = std::max(peak_held, held);
             done++;
             w.busy = false;
             w.reserved = 0;
+            w.held = held_now;
             cv_done.notify_all();
             cv_work.notify_all();
         }
@@ -140,6 +175,7 @@ int uc_server_create(const uc_server_opts *opts, uc_server **out)
             delete s;
             return rc;
         }
+        uc_ctx_set_meter(w.ctx, &s->meter);
     }
     for (size_t i = 0; i < s->workers.size(); i++) s->threads.emplace_back([s, i] { s->run(i); });
     *out = s;
@@ -246,7 +282,7 @@ int uc_server_stats(uc_server *s, uint64_t *peak_held, uint64_t *jobs_done)
 {
     if (!s) return UC_E_ARG;
     std::lock_guard<std::mutex> lk(s->mu);
-    if (peak_held) *peak_held = s->peak_held;
+    if (peak_held) *peak_held = s->meter.peak.load();
     if (jobs_done) *jobs_done = s->done;
     return UC_OK;
 }

@@ -46,12 +46,14 @@ extern "C" int uc_parse_kmer_text(const char *buf, uint64_t len, uint32_t k, uin
     if (!packed || !n_out || (!buf && len) || k < 2 || k > 64) return UC_E_ARG;
     *packed = nullptr;
     *n_out = 0;
-    // line 1: the k-mer count (sscanf "%u" / "%lu", driver.cpp:296-304)
-    uint64_t p = 0, count = 0;
+    // line 1: the k-mer count (sscanf "%u" / "%lu", driver.cpp:296-304).
+    // The reference compares it with the k-mers read only after whole read
+    // chunks (driver.cpp:343) and reads on past it otherwise, so every line
+    // is taken whatever the count says.
+    uint64_t p = 0;
     bool digit = false;
     while (p < len && (buf[p] == ' ' || buf[p] == '\t')) p++;
     while (p < len && buf[p] >= '0' && buf[p] <= '9') {
-        count = count * 10 + (uint64_t)(buf[p] - '0');
         digit = true;
         p++;
     }
@@ -59,6 +61,8 @@ extern "C" int uc_parse_kmer_text(const char *buf, uint64_t len, uint32_t k, uin
     while (p < len && buf[p] != '\n') p++;
     if (p < len) p++;
     const uint64_t body = p;
+    // newlines and carriage returns at the end of the text end it
+    while (len > body && (buf[len - 1] == '\n' || buf[len - 1] == '\r')) len--;
     // chunk boundaries at line starts
     const unsigned T = n_threads(len - body);
     std::vector<uint64_t> start(T + 1, len);
@@ -87,7 +91,7 @@ extern "C" int uc_parse_kmer_text(const char *buf, uint64_t len, uint32_t k, uin
         for (auto &x : th) x.join();
     }
     for (unsigned t = 0; t < T; t++) lines[t + 1] += lines[t];
-    const uint64_t n = std::min<uint64_t>(count, lines[T]);
+    const uint64_t n = lines[T];
     const uint32_t rw = (k + 15) / 16;
     uint32_t *out = (uint32_t *)std::calloc(n ? n * rw : 1, sizeof(uint32_t));
     if (!out) return UC_E_NOMEM;
