@@ -88,6 +88,29 @@ const char *uc_strerror(int status);
  * copy).  The unitig text stays on the device (uc_text_*). */
 int uc_compact(uc_ctx *ctx, const uint32_t *packed, uint64_t n, uint32_t k, uint32_t l, int on_device);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc): uc_compact
+ * on n k-mers already in device memory (`packed`, reference packing), with
+ * every intermediate buffer carved from a static scratch array of
+ * UC_SCRATCH_BYTES that the library holds on each device until the process
+ * ends, and the unitig text written to the caller's device buffer `text`
+ * (text_cap bytes).  Nothing is allocated: no hipMalloc, hipFree, graph
+ * instantiation or event creation (profiling and probe counting are off for
+ * the call; the contraction pass runs eagerly).
+ *  - *text_size gets the text's size once contraction knows it, also when
+ *    the call then fails with UC_E_NOMEM because text_cap is smaller.
+ *  - A job too large for the scratch fails with UC_E_LIMIT: at once when n
+ *    alone rules it out, else when the first buffer that does not fit is
+ *    carved (before any kernel writes to it).  About 1.5 M 31-mers fit.
+ *  - Every call on one device shares that device's scratch, which keeps the
+ *    last call's contents: calls that overlap on one device (two contexts,
+ *    two threads) race and must be serialised by the caller.
+ *  - The context's own buffers and results are left as they were, except
+ *    that nothing of the last job stays readable (uc_text_size is 0). */
+#define UC_SCRATCH_BYTES (256ull << 20)
+int uc_compact_noalloc(uc_ctx *ctx, const uint32_t *packed, uint64_t n, uint32_t k, uint32_t l, char *text,
+                       uint64_t text_cap, uint64_t *text_size);
+
 /* Whole job on the k-mers of the last uc_load (already resident in device
  * memory): bucketing, edges, contraction, emission, timed per phase. */
 int uc_compact_loaded(uc_ctx *ctx, uint32_t l);

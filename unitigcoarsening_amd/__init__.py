@@ -30,6 +30,7 @@ UC_FLAG_KEEP_SRC = 4
 UC_FLAG_DIRECT_SCATTER = 8
 UC_FLAG_EAGER = 16
 UC_FLAG_COUNT_PROBES = 32
+UC_SCRATCH_BYTES = 256 << 20
 UC_TUNE_KEEP_BUCKETS = 1
 
 F_LENT = 1
@@ -163,6 +164,7 @@ API = {
     "uc_strerror": (C.c_char_p, [C.c_int]),
     "uc_compact": (C.c_int, [_P, _P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int]),
     "uc_compact_loaded": (C.c_int, [_P, C.c_uint32]),
+    "uc_compact_noalloc": (C.c_int, [_P, _P, C.c_uint64, C.c_uint32, C.c_uint32, _P, C.c_uint64, _U64P]),
     "uc_kernel_stat": (C.c_int, [_P, C.c_uint32, C.c_char_p, C.c_uint32, C.POINTER(C.c_double), _U64P]),
     "uc_kernel_stats_reset": (C.c_int, [_P]),
     "uc_kernel_stats_select": (C.c_int, [_P, C.c_char_p]),

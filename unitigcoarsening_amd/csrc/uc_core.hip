@@ -32,6 +32,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -2448,6 +2449,16 @@ struct DBuf {
     size_t cap = 0;
 };
 
+// Scratch of uc_compact_noalloc: one array per device, reserved with the
+// code object and held until the process ends.  Host code never passes the
+// array's name to a kernel or a copy: the name is a host-side stand-in, and
+// the device address comes from hipGetSymbolAddress (scratch_ptr below).
+static __device__ __align__(256) unsigned char scratch[UC_SCRATCH_BYTES];
+
+// bytes at the end of the scratch that no buffer is carved from: the few
+// kernels that read a whole aligned line past a buffer's end stay inside
+constexpr size_t SCRATCH_TAIL = 64 << 10;
+
 }  // namespace uc
 
 using namespace uc;
@@ -2522,6 +2533,14 @@ struct uc_ctx {
     std::string prof_only;  // time only this kernel ("" for all)
     std::vector<std::pair<size_t, std::pair<hipEvent_t, hipEvent_t>>> pending;
     std::vector<hipEvent_t> evpool;
+
+    // uc_compact_noalloc: the device address of this device's scratch
+    // array; during such a call every buffer is carved from it (arena) and
+    // the text goes to the caller's buffer (text_fixed)
+    unsigned char *scratch_dev = nullptr;
+    unsigned char *arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
+    bool text_fixed = false;
 };
 
 // every device buffer of a context
@@ -2596,6 +2615,19 @@ int ensure(uc_ctx *c, DBuf &b, size_t bytes)
 {
     if (bytes == 0) bytes = 8;
     if (b.cap >= bytes) return UC_OK;
+    if (c->arena) {
+        // uc_compact_noalloc: carve from the scratch, never allocate.  A
+        // buffer asked for again with more bytes gets a new piece; contents
+        // are never carried over (nor are they across hipMalloc below).
+        const size_t off = (c->arena_used + 255) & ~(size_t)255;
+        if (off > c->arena_cap || bytes > c->arena_cap - off)
+            return fail(c, UC_E_LIMIT, "job needs more than the " + std::to_string(c->arena_cap) +
+                                           " bytes of device scratch uc_compact_noalloc may use");
+        b.p = c->arena + off;
+        b.cap = bytes;
+        c->arena_used = off + bytes;
+        return UC_OK;
+    }
     if (b.p) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
         HIPCHK(c, hipFree(b.p));
@@ -2633,7 +2665,7 @@ struct Prof {
     hipEvent_t a = nullptr, b = nullptr;
     Prof(uc_ctx *ctx, const char *name) : c(ctx)
     {
-        if (!(c->opts.flags & UC_FLAG_PROFILE) || c->capturing ||
+        if (!(c->opts.flags & UC_FLAG_PROFILE) || c->capturing || c->arena ||
             (!c->prof_only.empty() && c->prof_only != name)) {
             c = nullptr;
             return;
@@ -3711,7 +3743,7 @@ static const char *const PASS_KERNELS[] = {"k_fill",       "k_seg_count",   "k_s
 // pass's kernels is to be timed (captured launches carry no events)
 static bool pass_graph(const uc_ctx *c)
 {
-    if (c->opts.flags & UC_FLAG_EAGER) return false;
+    if ((c->opts.flags & UC_FLAG_EAGER) || c->arena) return false;  // instantiating a graph allocates
     if (!(c->opts.flags & UC_FLAG_PROFILE)) return true;
     if (c->prof_only.empty()) return false;
     for (const char *n : PASS_KERNELS)
@@ -3952,7 +3984,14 @@ int run_emit(uc_ctx *c)
 {
     const uint32_t m = (uint32_t)c->m;
     const uint64_t T = (c->m + CT_TILE - 1) / CT_TILE;
-    CHK(ensure(c, c->text, c->text_bytes + 1));
+    if (c->text_fixed) {
+        // the caller's buffer (uc_compact_noalloc); the kernels write bytes [0, text_bytes) only
+        if (c->text.cap < c->text_bytes)
+            return fail(c, UC_E_NOMEM, "the text buffer holds " + std::to_string(c->text.cap) + " bytes; the job's " +
+                                           "text is " + std::to_string(c->text_bytes));
+    } else {
+        CHK(ensure(c, c->text, c->text_bytes + 1));
+    }
     CHK(ensure(c, c->htoff, c->m * 8));
     if (m) {
         // one-word k-mers from 256-thread groups, wider ones from 512-thread
@@ -4028,6 +4067,27 @@ int timed_job(uc_ctx *c, const uint32_t *packed, const uint64_t *offs, uint32_t 
     return timed_phases(c, l);
 }
 
+// Device address of the scratch array on `device`, which must be current:
+// looked up once per device (every device holds its own copy of the array at
+// its own address) and kept.  nullptr when the lookup failed.
+constexpr int SCRATCH_DEVS = 64;
+
+unsigned char *scratch_ptr(int device)
+{
+    static std::mutex mu;
+    static unsigned char *ptr[SCRATCH_DEVS] = {};
+    static bool done[SCRATCH_DEVS] = {};
+    if (device < 0 || device >= SCRATCH_DEVS) return nullptr;
+    std::lock_guard<std::mutex> g(mu);
+    if (!done[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        ptr[device] = e == hipSuccess ? static_cast<unsigned char *>(p) : nullptr;
+        done[device] = true;
+    }
+    return ptr[device];
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -4092,6 +4152,9 @@ int uc_ctx_create(const uc_opts *opts, uc_ctx **out)
     int lds = 0;
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, o.device) == hipSuccess && lds > 0)
         c->lds_limit = (uint32_t)lds;
+    // set-up of uc_compact_noalloc, with the context's device current (a
+    // failed lookup only disables that entry point)
+    c->scratch_dev = scratch_ptr(o.device);
     *out = c;
     return UC_OK;
 }
@@ -4126,6 +4189,60 @@ int uc_compact(uc_ctx *c, const uint32_t *packed, uint64_t n, uint32_t k, uint32
     CHK(select_dev(c));
     const uint64_t offs[2] = {0, n};
     return timed_job(c, packed, offs, 1, k, l, on_device);
+}
+
+int uc_compact_noalloc(uc_ctx *c, const uint32_t *packed, uint64_t n, uint32_t k, uint32_t l, char *text,
+                       uint64_t text_cap, uint64_t *text_size)
+{
+    if (!c || (!packed && n) || !text_size || (!text && text_cap)) return UC_E_ARG;
+    *text_size = 0;
+    if (k < 2 || k > 64) return fail(c, UC_E_ARG, "k must be in [2, 64]");
+    CHK(select_dev(c));
+    if (!c->scratch_dev) return fail(c, UC_E_HIP, "hipGetSymbolAddress found no scratch on this device");
+    // every job keeps at least its bucketed k-mers twice plus a flag byte,
+    // a link and a run record per slot: a job beyond that is refused before
+    // anything runs; the carving refuses the rest as it meets them
+    const uint64_t W = k <= 32 ? 1 : 2, floor_per_kmer = 16 * W + 13;
+    const uint64_t usable = UC_SCRATCH_BYTES - SCRATCH_TAIL;
+    if (n > usable / floor_per_kmer)
+        return fail(c, UC_E_LIMIT, std::to_string(n) + " k-mers cannot fit the " + std::to_string(UC_SCRATCH_BYTES) +
+                                       "-byte scratch of uc_compact_noalloc");
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // the context's own earlier work is done with its buffers
+    // the context's own buffers step aside for the call and come back after it
+    std::vector<DBuf *> bufs = all_bufs(c);
+    std::vector<DBuf> saved;
+    saved.reserve(bufs.size());
+    for (DBuf *b : bufs) {
+        saved.push_back(*b);
+        *b = DBuf{};
+    }
+    const uint32_t flags = c->opts.flags;
+    c->opts.flags = (flags & ~(UC_FLAG_PROFILE | UC_FLAG_COUNT_PROBES | UC_FLAG_KEEP_SRC)) | UC_FLAG_EAGER;
+    c->arena = c->scratch_dev;
+    c->arena_cap = usable;
+    c->arena_used = 0;
+    c->text = DBuf{text, text_cap};
+    c->text_fixed = true;
+    const uint64_t offs[2] = {0, n};
+    int rc = do_load(c, packed, offs, 1, k, 1);
+    if (rc == UC_OK) rc = do_bucketize(c, l, nullptr);
+    if (rc == UC_OK) rc = do_edges(c, UC_ALL_BUCKETS);
+    if (rc == UC_OK) rc = do_coarsen(c);
+    if (rc == UC_OK) {
+        *text_size = c->text_bytes;
+        rc = do_emit(c);
+    }
+    const hipError_t se = hipStreamSynchronize(c->stream);
+    if (rc == UC_OK && se != hipSuccess) rc = fail(c, UC_E_HIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(se));
+    for (size_t i = 0; i < bufs.size(); i++) *bufs[i] = saved[i];
+    c->arena = nullptr;
+    c->arena_cap = c->arena_used = 0;
+    c->text_fixed = false;
+    c->opts.flags = flags;
+    // nothing of this job stays readable through the context: its buffers were the scratch
+    reset_job(c);
+    c->kin = nullptr;
+    return rc;
 }
 
 int uc_compact_loaded(uc_ctx *c, uint32_t l)
