@@ -161,21 +161,19 @@ int orc_ref_load_kmers(const char *buf, size_t sz, uint32_t k, uint32_t **packed
     return ORC_OK;
 }
 
-/* Seeded bijection on 2l-bit codes: xor with the seed, then two rounds of
- * (multiply by an odd constant modulo 2^(2l), xorshift right), then xor with
- * the seed's high word.  Every step is invertible on [0, 4^l), so the
- * composition is a permutation.  The constants are odd and below 2^24 so a
- * GPU may use 24-bit multiplies when 2l <= 24 and get the same residues. */
+/* Seeded bijection on 2l-bit codes: add the seed's low word, multiply by an
+ * odd constant, xorshift right, multiply by a second odd constant, xorshift
+ * right, every step modulo 2^(2l).  Every step is invertible on [0, 4^l), so
+ * the composition is a permutation.  The constants are odd and below 2^24 so
+ * a GPU may use 24-bit multiplies when 2l <= 24 and get the same residues. */
 uint32_t orc_lmer_order(uint32_t code, uint32_t l, uint64_t seed)
 {
     uint32_t bits = 2 * l;
     uint32_t mask = bits >= 32 ? 0xFFFFFFFFu : ((1u << bits) - 1u);
     uint32_t h1 = (bits + 1) / 2, h2 = bits / 3 + 1;
-    uint32_t x = (code ^ (uint32_t)seed) & mask;
-    x = (x * ORC_MIX_C1) & mask;
+    uint32_t x = ((code + ((uint32_t)seed & mask)) * ORC_MIX_C1) & mask;
     x ^= x >> h1;
     x = (x * ORC_MIX_C2) & mask;
     x ^= x >> h2;
-    x ^= (uint32_t)(seed >> 32) & mask;
     return x;
 }

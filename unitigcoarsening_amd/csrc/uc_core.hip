@@ -170,27 +170,31 @@ __host__ __device__ constexpr int lmin_mode(uint32_t l, bool table)
 }
 
 // seeded bijection on 2l-bit l-mer codes (same function as the oracle's
-// orc_lmer_order): xor seed, two rounds of odd multiply + xorshift, xor the
-// second seed word.  A third round cost 25% of the minimizer scan (for
-// 2l > 24 its multiplies are quarter-rate v_mul_lo_u32) and left the bucket
-// sizes where they were.
+// orc_lmer_order): add the seed, multiply by an odd constant, xorshift,
+// multiply by another, xorshift, all modulo 2^(2l).  Only the low 2l bits of
+// `code` count: a product's low bits depend only on its operands' low bits,
+// so the rolling code may carry older characters above them.  Per round: one
+// (24-bit) multiply(-add), one bit-field extract for the shifted term and one
+// three-input op for mask and xor.
+constexpr uint32_t MIX_C1 = 0x3779B1u, MIX_C2 = 0xEBCA6Bu;
+
 template <int MODE>
 __device__ __forceinline__ uint32_t lmer_order_m(uint32_t code, const LminParams &p)
 {
     if constexpr (MODE == LM_TABLE) {
-        return p.table[code];
+        return p.table[code & p.lmask];
     } else {
-        uint32_t x = code ^ p.s0;  // both below 4^l: no mask needed
-        if constexpr (MODE == LM_MUL24) {
-            x = __umul24(x, 0x3779B1u) & p.lmask;
-            x ^= x >> p.h1;
-            x = __umul24(x, 0xEBCA6Bu) & p.lmask;
-        } else {
-            x = (x * 0x3779B1u) & p.lmask;
-            x ^= x >> p.h1;
-            x = (x * 0xEBCA6Bu) & p.lmask;
-        }
-        return x ^ (x >> p.h2) ^ p.s1;
+        uint32_t t;
+        if constexpr (MODE == LM_MUL24)
+            t = __umul24(code, MIX_C1) + p.k1;  // (code + s0) * C1
+        else
+            t = code * MIX_C1 + p.k1;
+        uint32_t x = (t & p.lmask) ^ __builtin_amdgcn_ubfe(t, p.h1, p.w1);
+        if constexpr (MODE == LM_MUL24)
+            t = __umul24(x, MIX_C2);
+        else
+            t = x * MIX_C2;
+        return (t & p.lmask) ^ __builtin_amdgcn_ubfe(t, p.h2, p.w2);
     }
 }
 
@@ -220,7 +224,8 @@ __device__ __forceinline__ void double_lmin(const Kmer<W> &x, const LminParams &
             const uint32_t w = (uint32_t)(x.w[q / 2] >> (32 * (q % 2)));
 #pragma unroll 1
             for (uint32_t i = lo; i < hi; i++) {  // uniform trip count
-                lmer = ((lmer << 2) | ((w >> (2 * (i - b))) & 3u)) & p.lmask;
+                // characters older than the l-mer stay above its 2l bits
+                lmer = (lmer << 2) | __builtin_amdgcn_ubfe(w, 2 * (i - b), 2);
                 if constexpr (decltype(hash)::value) mid = min(mid, lmer_order_m<MODE>(lmer, p));
             }
         }
@@ -378,9 +383,32 @@ __global__ __launch_bounds__(BLOCK) void k_scan_tile(const T *in, T *out, uint64
 // ---------------------------------------------------------------------------
 // 1. bucketing
 
-template <int W, int IPT, int MODE>
+// Fine codes of the prefix and suffix minimizers, one pair per k-mer: two
+// 16-bit halves of one word when every fine index fits 16 bits (C32), else
+// a uint2.
+template <bool C32>
+__device__ __forceinline__ void put_codes(void *__restrict__ bkt, uint64_t i, uint32_t f1, uint32_t f2)
+{
+    if constexpr (C32)
+        static_cast<uint32_t *>(bkt)[i] = f1 | (f2 << 16);
+    else
+        static_cast<uint2 *>(bkt)[i] = make_uint2(f1, f2);
+}
+
+template <bool C32>
+__device__ __forceinline__ uint2 get_codes(const void *__restrict__ bkt, uint64_t i)
+{
+    if constexpr (C32) {
+        const uint32_t v = static_cast<const uint32_t *>(bkt)[i];
+        return make_uint2(v & 0xFFFFu, v >> 16);
+    } else {
+        return static_cast<const uint2 *>(bkt)[i];
+    }
+}
+
+template <int W, int IPT, int MODE, bool C32>
 __global__ __launch_bounds__(BLOCK) void k_lmin_count(const uint64_t *__restrict__ kin, const BTile *__restrict__ tiles,
-                                                      LminParams p, uint64_t top_mask, uint2 *__restrict__ bkt,
+                                                      LminParams p, uint64_t top_mask, void *__restrict__ bkt,
                                                       uint32_t *__restrict__ hist, uint32_t *__restrict__ xhist)
 {
     extern __shared__ uint32_t cnt[];  // B buckets + 1 crossing counter
@@ -405,7 +433,7 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_count(const uint64_t *__restrict
         uint32_t o1, o2;
         double_lmin<W, MODE>(xs[j], p, o1, o2);
         const uint32_t f1 = o1 & t.fmask, f2 = o2 & t.fmask;
-        bkt[t.kbegin + idx] = make_uint2(f1, f2);
+        put_codes<C32>(bkt, t.kbegin + idx, f1, f2);
         atomicAdd(&cnt[f1 & bmask], 1u);
         if (f1 != f2) {
             atomicAdd(&cnt[f2 & bmask], 1u);
@@ -494,13 +522,23 @@ __host__ __device__ constexpr size_t lmin_stage_off(uint32_t B, uint32_t nw = WA
 template <int W>
 __host__ __device__ constexpr size_t lmin_stage_bytes(uint32_t B, uint32_t tile, bool keep, uint32_t nw = WAVES)
 {
-    return lmin_stage_off(B, nw) + (size_t)lmin_stage_cap(tile) * (8 * W + 2 + 2 + 1 + (keep ? 4 : 0));
+    return lmin_stage_off(B, nw) + (size_t)lmin_stage_cap(tile) * (8 * W + 2 + 2 + (keep ? 4 : 0));
 }
 
-template <int W, int IPT, bool STAGE, int NT>
+// A pass-1 slot's fine index (below MAX_SUB) and its crossing flag share one
+// 16-bit word, which is all the fine sort reads besides the k-mer
+constexpr uint32_t SUB_MASK = 0x3FFFu;
+constexpr uint32_t SUB_FLAG_SHIFT = 14;
+static_assert(MAX_SUB - 1 <= SUB_MASK, "fine indices fit below the flag bits");
+static_assert((F_LENT | F_BORROW) < 4, "crossing flags fit two bits");
+
+// Pass 1 writes, per placement, the k-mer and either the final flag byte
+// (flags != nullptr: no fine sort follows) or the fine index with the flag
+// bits (sub != nullptr).
+template <int W, int IPT, bool STAGE, int NT, bool C32>
 __global__ __launch_bounds__(NT) void k_lmin_scatter(const uint64_t *__restrict__ kin, const BTile *__restrict__ tiles,
                                                         uint32_t B, uint32_t nbits, uint64_t top_mask,
-                                                        const uint2 *__restrict__ bkt,
+                                                        const void *__restrict__ bkt,
                                                         const uint32_t *__restrict__ hscan,
                                                         const uint32_t *__restrict__ xscan, uint64_t *__restrict__ kout,
                                                         uint8_t *__restrict__ flags, uint32_t *__restrict__ src,
@@ -527,7 +565,7 @@ __global__ __launch_bounds__(NT) void k_lmin_scatter(const uint64_t *__restrict_
         const uint32_t idx = (uint32_t)j * NT + threadIdx.x;
         codes[j] = make_uint2(0u, 0u);
         if (idx < t.count) {
-            codes[j] = bkt[t.kbegin + idx];
+            codes[j] = get_codes<C32>(bkt, t.kbegin + idx);
             xs[j] = load_kmer<W>(kin, t.kbegin + idx, top_mask);
         }
     }
@@ -546,13 +584,12 @@ __global__ __launch_bounds__(NT) void k_lmin_scatter(const uint64_t *__restrict_
     const int lane = __lane_id(), w = threadIdx.x / 64;
     const uint64_t lt = (1ull << lane) - 1ull, me = 1ull << lane;
     const uint32_t xbase = xscan[ti];
-    // staged placements: k-mers, fine index, bucket, flags (and sources)
+    // staged placements: k-mers, fine index with flags, bucket (and sources)
     constexpr uint32_t CAP = lmin_stage_cap(IPT * NT);
     Kmer<W> *skm = reinterpret_cast<Kmer<W> *>(lds + lmin_stage_off(B, NW));
     uint16_t *ssub = reinterpret_cast<uint16_t *>(skm + CAP);
     uint16_t *sbk = ssub + CAP;
-    uint8_t *sfl = reinterpret_cast<uint8_t *>(sbk + CAP);
-    uint32_t *ssrc = reinterpret_cast<uint32_t *>(sfl + CAP);  // 4-byte aligned: CAP is a multiple of 4
+    uint32_t *ssrc = reinterpret_cast<uint32_t *>(sbk + CAP);  // 4-byte aligned: CAP is even
     const uint32_t placed = t.count + (xscan[ti + 1] - xbase);
     const bool staged = STAGE && placed <= CAP;  // uniform over the block
 #pragma unroll
@@ -596,34 +633,34 @@ __global__ __launch_bounds__(NT) void k_lmin_scatter(const uint64_t *__restrict_
             const Kmer<W> x = xs[j];
             uint32_t lh = tstart[bp] + run[bp] + (uint32_t)(__popcll(hpH & lt) + __popcll(hsH & lt));
             for (int q = 0; q < w; q++) lh += wcnt[q * (B + 1) + bp];
+            const uint32_t wh = fp | ((cross ? (uint32_t)F_LENT : 0u) << SUB_FLAG_SHIFT);
             if (staged) {
                 skm[lh] = x;
-                ssub[lh] = (uint16_t)fp;
+                ssub[lh] = (uint16_t)wh;
                 sbk[lh] = (uint16_t)bp;
-                sfl[lh] = cross ? F_LENT : 0;
                 if (keep) ssrc[lh] = (uint32_t)g;
             } else {
                 const uint32_t ns = delta[bp] + lh;
                 store_kmer<W>(kout, ns, x);
-                flags[ns] = cross ? F_LENT : 0;
-                sub[ns] = (uint16_t)fp;
+                if (flags) flags[ns] = (uint8_t)(wh >> SUB_FLAG_SHIFT);
+                if (sub) sub[ns] = (uint16_t)wh;
                 if (keep) src[ns] = (uint32_t)g;
             }
             if (cross) {
                 uint32_t lb = tstart[bs] + run[bs] + (uint32_t)(__popcll(hpB & lt) + __popcll(hsB & lt)) +
                               (bp == bs ? 1u : 0u);
                 for (int q = 0; q < w; q++) lb += wcnt[q * (B + 1) + bs];
+                const uint32_t wb = fs | ((uint32_t)F_BORROW << SUB_FLAG_SHIFT);
                 if (staged) {
                     skm[lb] = x;
-                    ssub[lb] = (uint16_t)fs;
+                    ssub[lb] = (uint16_t)wb;
                     sbk[lb] = (uint16_t)bs;
-                    sfl[lb] = F_BORROW;
                     if (keep) ssrc[lb] = (uint32_t)g;
                 } else {
                     const uint32_t ns = delta[bs] + lb;
                     store_kmer<W>(kout, ns, x);
-                    flags[ns] = F_BORROW;
-                    sub[ns] = (uint16_t)fs;
+                    if (flags) flags[ns] = F_BORROW;
+                    if (sub) sub[ns] = (uint16_t)wb;
                     if (keep) src[ns] = (uint32_t)g;
                 }
                 uint32_t px = run[B] + (uint32_t)__popcll(C & lt);
@@ -650,8 +687,9 @@ __global__ __launch_bounds__(NT) void k_lmin_scatter(const uint64_t *__restrict_
         for (uint32_t li = threadIdx.x; li < placed; li += NT) {
             const uint32_t ns = delta[sbk[li]] + li;
             store_kmer<W>(kout, ns, skm[li]);
-            flags[ns] = sfl[li];
-            sub[ns] = ssub[li];
+            const uint32_t v = ssub[li];
+            if (flags) flags[ns] = (uint8_t)(v >> SUB_FLAG_SHIFT);
+            if (sub) sub[ns] = (uint16_t)v;
             if (keep) src[ns] = ssrc[li];
         }
     }
@@ -796,18 +834,34 @@ __global__ void k_plan2_tiles(const uint32_t *__restrict__ boff, const uint32_t 
     tiles[i] = bt;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_sub_count(const uint16_t *__restrict__ sub, const BTile *__restrict__ tiles,
-                                                     uint32_t *__restrict__ hist)
+// counts per fine bucket of one planned fine-sort tile (up to SUB_TILE
+// slots); every thread has all of its loads in flight before its first LDS add
+constexpr int SUB_COUNT_NT = 512;
+
+__global__ __launch_bounds__(SUB_COUNT_NT) void k_sub_count(const uint16_t *__restrict__ sub,
+                                                            const BTile *__restrict__ tiles,
+                                                            uint32_t *__restrict__ hist)
 {
+    constexpr int PER = 8;
     extern __shared__ uint32_t cnt[];
     const BTile t = tiles[blockIdx.x];
     const uint32_t S = t.fmask;
     if (S == 0) return;  // past the last tile
-    for (uint32_t i = threadIdx.x; i < S; i += BLOCK) cnt[i] = 0;
+    for (uint32_t i = threadIdx.x; i < S; i += SUB_COUNT_NT) cnt[i] = 0;
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < t.count; i += BLOCK) atomicAdd(&cnt[sub[t.kbegin + i]], 1u);
+    for (uint32_t i0 = 0; i0 < t.count; i0 += PER * SUB_COUNT_NT) {
+        uint32_t v[PER];
+#pragma unroll
+        for (int j = 0; j < PER; j++) {
+            const uint32_t i = i0 + (uint32_t)j * SUB_COUNT_NT + threadIdx.x;
+            v[j] = i < t.count ? (uint32_t)sub[t.kbegin + i] & SUB_MASK : NONE;
+        }
+#pragma unroll
+        for (int j = 0; j < PER; j++)
+            if (v[j] != NONE) atomicAdd(&cnt[v[j]], 1u);
+    }
     __syncthreads();
-    for (uint32_t v = threadIdx.x; v < S; v += BLOCK) hist[t.hbase + (uint64_t)t.tlocal * S + v] = cnt[v];
+    for (uint32_t v = threadIdx.x; v < S; v += SUB_COUNT_NT) hist[t.hbase + (uint64_t)t.tlocal * S + v] = cnt[v];
 }
 
 // per coarse bucket: each fine bucket's counts over the tiles become
@@ -867,38 +921,41 @@ __global__ __launch_bounds__(BLOCK) void k_sub_rows(const uint32_t *__restrict__
     if (g == G - 1 && threadIdx.x == 0) fboff[cb_first[G]] = m;
 }
 
-// Pass-2 scatter with LDS staging.  A tile (`tile` slots of one coarse
-// bucket) is ranked per wave: each wave owns tile/WAVES consecutive slots,
-// counts its slots per fine bucket (ballot groups, one LDS add per group),
-// the counts are scanned once per tile, and a second sweep gives every slot
-// its stable position inside the tile, sorted by fine bucket.  The slots are
-// staged there in LDS and written out in that order, so consecutive lanes
-// store consecutive addresses of one fine bucket's run.
+// Pass-2 scatter with LDS staging.  A planned tile (up to ROUNDS rounds of
+// IPT * NT slots of one coarse bucket) is sorted round by round; each round is
+// ranked per wave: each wave owns a chunk of consecutive slots, counts its
+// slots per fine bucket (ballot groups, one LDS add per group), the counts are
+// scanned once per round, and a second sweep gives every slot its stable
+// position inside the round, sorted by fine bucket.  The slots are staged
+// there in LDS and written out in that order, so consecutive lanes store
+// consecutive addresses of one fine bucket's run.  The running base of every
+// fine bucket carries from round to round, so the histogram holds one row per
+// planned tile (ROUNDS times fewer than one per round), and the next round's
+// slots are loaded while the current one is ranked.
 __host__ __device__ constexpr size_t sub_book_bytes(uint32_t S, uint32_t nw = WAVES)
 {
-    return (((size_t)nw * S + 2 * (size_t)S) * 4 + 15) / 16 * 16;
+    return (((size_t)nw * S + 3 * (size_t)S) * 4 + 15) / 16 * 16;
 }
 
 template <int W>
 __host__ __device__ constexpr size_t sub_lds_bytes(uint32_t tile, uint32_t S, bool src, bool stage, uint32_t nw = WAVES)
 {
-    return sub_book_bytes(S, nw) +
-           (stage ? (size_t)tile * 8 * W + (src ? (size_t)tile * 4 : 0) + (size_t)tile * 2 + tile : 0);
+    return sub_book_bytes(S, nw) + (stage ? (size_t)tile * 8 * W + (src ? (size_t)tile * 4 : 0) + (size_t)tile * 2 : 0);
 }
 
+constexpr uint32_t SUB_ROUNDS = 4;
 
 template <int W, bool STAGE, int IPT, int NT>
 __global__ __launch_bounds__(NT) void k_sub_scatter(const BTile *__restrict__ tiles, const uint16_t *__restrict__ sub,
                                                        const uint32_t *__restrict__ hscan,
                                                        const uint32_t *__restrict__ fboff,
                                                        const uint32_t *__restrict__ cb_first, const uint64_t *__restrict__ kin,
-                                                       const uint8_t *__restrict__ fin, const uint32_t *__restrict__ sin,
-                                                       uint64_t *__restrict__ kout, uint8_t *__restrict__ fout,
-                                                       uint32_t *__restrict__ sout, uint32_t *__restrict__ perm,
-                                                       uint32_t n_tiles)
+                                                       const uint32_t *__restrict__ sin, uint64_t *__restrict__ kout,
+                                                       uint8_t *__restrict__ fout, uint32_t *__restrict__ sout,
+                                                       uint32_t *__restrict__ perm, uint32_t n_tiles)
 {
     constexpr int NW = NT / 64;
-    constexpr uint32_t TILE = (uint32_t)IPT * NT, CHUNK = TILE / NW;  // slots per tile / per wave
+    constexpr uint32_t TILE = (uint32_t)IPT * NT, CHUNK = TILE / NW;  // slots per round / per wave
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ uint32_t wsum[NW];
     const uint32_t ti = xcd_tile(n_tiles);
@@ -911,102 +968,121 @@ __global__ __launch_bounds__(NT) void k_sub_scatter(const BTile *__restrict__ ti
     uint32_t *wc = reinterpret_cast<uint32_t *>(lds);  // NW x S
     uint32_t *tstart = wc + NW * S;
     uint32_t *delta = tstart + S;
+    uint32_t *gbase = delta + S;
     // staging (STAGE): slots sorted by fine bucket
     unsigned char *st = lds + sub_book_bytes(S, NW);
     Kmer<W> *skm = reinterpret_cast<Kmer<W> *>(st);
     uint32_t *ssrc = reinterpret_cast<uint32_t *>(st + (size_t)TILE * 8 * W);
     uint16_t *ssub = reinterpret_cast<uint16_t *>(st + (size_t)TILE * 8 * W + (src ? (size_t)TILE * 4 : 0));
-    uint8_t *sfl = reinterpret_cast<uint8_t *>(ssub + TILE);
     const int lane = __lane_id(), w = threadIdx.x / 64;
     const uint64_t lt = (1ull << lane) - 1ull, me = 1ull << lane;
     const uint64_t zero[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     const uint32_t c0 = (uint32_t)w * CHUNK;
     uint32_t *mywc = wc + (uint32_t)w * S;
-    // all of this lane's slots in flight at once: slot c0 + 64 j + lane
-    uint32_t vf[IPT];  // fine index | flag << 16
-    Kmer<W> xs[IPT];
-    uint32_t sr[IPT];
+    // the tile's base in every fine bucket
+    for (uint32_t v = threadIdx.x; v < S; v += NT)
+        gbase[v] = fboff[cb_first[t.set] + v] + hscan[t.hbase + (uint64_t)t.tlocal * S + v];
+    // this lane's slots of a round: c0 + 64 j + lane; fine index | flags << 14
+    uint32_t vf[IPT], nvf[IPT];
+    Kmer<W> xs[IPT], nxs[IPT];
+    uint32_t sr[IPT], nsr[IPT];
+    auto load_round = [&](uint32_t r0, uint32_t *V, Kmer<W> *X, uint32_t *R) {
 #pragma unroll
-    for (int j = 0; j < IPT; j++) {
-        const uint32_t idx = c0 + 64u * j + (uint32_t)lane;
-        vf[j] = 0;
-        sr[j] = 0;
-        if (idx < t.count) {
-            const uint64_t s = t.kbegin + idx;
-            vf[j] = (uint32_t)sub[s] | ((uint32_t)fin[s] << 16);
-            xs[j] = load_kmer<W>(kin, s, ~0ull);
-            if (src) sr[j] = sin[s];
-        }
-    }
-    for (uint32_t i = threadIdx.x; i < NW * S; i += NT) wc[i] = 0;
-    __syncthreads();
-    // 1. per-wave counts
-#pragma unroll
-    for (int j = 0; j < IPT; j++)
-        if (c0 + 64u * j + (uint32_t)lane < t.count) atomicAdd(&mywc[vf[j] & 0xFFFFu], 1u);
-    __syncthreads();
-    // 2. tile starts per fine bucket; per-wave bases; global minus local offsets
-    for (uint32_t v = threadIdx.x; v < S; v += NT) {
-        uint32_t tot = 0;
-#pragma unroll
-        for (int q = 0; q < NW; q++) tot += wc[q * S + v];
-        tstart[v] = tot;
-    }
-    __syncthreads();
-    block_excl_scan_n<NT>(tstart, S, wsum);
-    for (uint32_t v = threadIdx.x; v < S; v += NT) {
-        uint32_t b = tstart[v];
-#pragma unroll
-        for (int q = 0; q < NW; q++) {
-            const uint32_t cq = wc[q * S + v];
-            wc[q * S + v] = b;
-            b += cq;
-        }
-        delta[v] = fboff[cb_first[t.set] + v] + hscan[t.hbase + (uint64_t)t.tlocal * S + v] - tstart[v];
-    }
-    __syncthreads();
-    // 3. stable positions: per round, ballot groups of equal fine index
-#pragma unroll
-    for (int j = 0; j < IPT; j++) {
-        const uint32_t idx = c0 + 64u * j + (uint32_t)lane;
-        const bool valid = idx < t.count;
-        const uint64_t V = __ballot(valid);
-        if (!V) break;  // uniform over the wave
-        const uint32_t v = vf[j] & 0xFFFFu;
-        uint64_t BP[10];
-#pragma unroll
-        for (int q = 0; q < 10; q++) BP[q] = (uint32_t)q < nbits ? (uint64_t)__ballot(valid && ((v >> q) & 1u)) : 0ull;
-        uint64_t M = 0, none;
-        match_bucket(v, nbits, V, 0ull, BP, zero, M, none);
-        uint32_t old = 0;
-        if (valid && (M & (0ull - M)) == me) old = atomicAdd(&mywc[v], (uint32_t)__popcll(M));
-        const int leader = valid ? __builtin_ctzll(M) : lane;
-        const uint32_t base = __shfl(old, leader, 64);
-        if (valid) {
-            const uint64_t s = t.kbegin + idx;
-            const uint32_t li = base + (uint32_t)__popcll(M & lt);
-            const uint32_t ns = delta[v] + li;
-            if constexpr (STAGE) {
-                skm[li] = xs[j];
-                sfl[li] = (uint8_t)(vf[j] >> 16);
-                ssub[li] = (uint16_t)v;
-                if (src) ssrc[li] = sr[j];
-            } else {
-                store_kmer<W>(kout, ns, xs[j]);
-                fout[ns] = (uint8_t)(vf[j] >> 16);
-                if (src) sout[ns] = sr[j];
+        for (int j = 0; j < IPT; j++) {
+            const uint32_t idx = r0 + c0 + 64u * j + (uint32_t)lane;
+            V[j] = NONE;
+            R[j] = 0;
+            if (idx < t.count) {
+                const uint64_t s = t.kbegin + idx;
+                V[j] = (uint32_t)sub[s];
+                X[j] = load_kmer<W>(kin, s, ~0ull);
+                if (src) R[j] = sin[s];
             }
-            perm[s] = ns;
         }
-    }
-    // 4. write out in sorted order
-    if constexpr (STAGE) {
+    };
+    load_round(0, vf, xs, sr);
+    for (uint32_t r0 = 0; r0 < t.count; r0 += TILE) {  // uniform over the block
+        if (r0 + TILE < t.count) load_round(r0 + TILE, nvf, nxs, nsr);
+        for (uint32_t i = threadIdx.x; i < NW * S; i += NT) wc[i] = 0;
         __syncthreads();
-        for (uint32_t li = threadIdx.x; li < t.count; li += NT) {
-            const uint32_t ns = delta[ssub[li]] + li;
-            store_kmer<W>(kout, ns, skm[li]);
-            fout[ns] = sfl[li];
-            if (src) sout[ns] = ssrc[li];
+        // 1. per-wave counts
+#pragma unroll
+        for (int j = 0; j < IPT; j++)
+            if (vf[j] != NONE) atomicAdd(&mywc[vf[j] & SUB_MASK], 1u);
+        __syncthreads();
+        // 2. round starts per fine bucket; per-wave bases; global minus local
+        // offsets; the bases move past this round's slots
+        for (uint32_t v = threadIdx.x; v < S; v += NT) {
+            uint32_t tot = 0;
+#pragma unroll
+            for (int q = 0; q < NW; q++) tot += wc[q * S + v];
+            tstart[v] = tot;
+        }
+        __syncthreads();
+        block_excl_scan_n<NT>(tstart, S, wsum);
+        for (uint32_t v = threadIdx.x; v < S; v += NT) {
+            const uint32_t t0 = tstart[v];
+            uint32_t b = t0;
+#pragma unroll
+            for (int q = 0; q < NW; q++) {
+                const uint32_t cq = wc[q * S + v];
+                wc[q * S + v] = b;
+                b += cq;
+            }
+            delta[v] = gbase[v] - t0;
+            gbase[v] += b - t0;
+        }
+        __syncthreads();
+        // 3. stable positions: per pass, ballot groups of equal fine index
+#pragma unroll
+        for (int j = 0; j < IPT; j++) {
+            const bool valid = vf[j] != NONE;
+            const uint64_t V = __ballot(valid);
+            if (!V) break;  // uniform over the wave
+            const uint32_t v = vf[j] & SUB_MASK;
+            uint64_t BP[10];
+#pragma unroll
+            for (int q = 0; q < 10; q++) BP[q] = (uint32_t)q < nbits ? (uint64_t)__ballot(valid && ((v >> q) & 1u)) : 0ull;
+            uint64_t M = 0, none;
+            match_bucket(v, nbits, V, 0ull, BP, zero, M, none);
+            uint32_t old = 0;
+            if (valid && (M & (0ull - M)) == me) old = atomicAdd(&mywc[v], (uint32_t)__popcll(M));
+            const int leader = valid ? __builtin_ctzll(M) : lane;
+            const uint32_t base = __shfl(old, leader, 64);
+            if (valid) {
+                const uint64_t s = t.kbegin + r0 + c0 + 64u * j + (uint32_t)lane;
+                const uint32_t li = base + (uint32_t)__popcll(M & lt);
+                const uint32_t ns = delta[v] + li;
+                if constexpr (STAGE) {
+                    skm[li] = xs[j];
+                    ssub[li] = (uint16_t)vf[j];
+                    if (src) ssrc[li] = sr[j];
+                } else {
+                    store_kmer<W>(kout, ns, xs[j]);
+                    fout[ns] = (uint8_t)(vf[j] >> SUB_FLAG_SHIFT);
+                    if (src) sout[ns] = sr[j];
+                }
+                perm[s] = ns;
+            }
+        }
+        // 4. write out in sorted order
+        if constexpr (STAGE) {
+            __syncthreads();
+            const uint32_t cnt = min(TILE, t.count - r0);
+            for (uint32_t li = threadIdx.x; li < cnt; li += NT) {
+                const uint32_t v = ssub[li];
+                const uint32_t ns = delta[v & SUB_MASK] + li;
+                store_kmer<W>(kout, ns, skm[li]);
+                fout[ns] = (uint8_t)(v >> SUB_FLAG_SHIFT);
+                if (src) sout[ns] = ssrc[li];
+            }
+        }
+        __syncthreads();  // the next round reuses the counts and the stage
+#pragma unroll
+        for (int j = 0; j < IPT; j++) {
+            vf[j] = nvf[j];
+            xs[j] = nxs[j];
+            sr[j] = nsr[j];
         }
     }
 }
@@ -3221,7 +3297,6 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     const uint64_t *sp = P<uint64_t>(c->setplan);
     KLAUNCH(c, "k_plan1_tiles", k_plan1_tiles, (grid_for(T), BLOCK, 0), sp, sp + S + 1,
             reinterpret_cast<const uint32_t *>(sp + 2 * (uint64_t)S + 2), S, B, tile1, T, P<BTile>(c->tiles));
-    CHK(ensure(c, c->bkt, c->n * 8));
     CHK(ensure(c, c->hist, (hb + 1) * 4));
     CHK(ensure(c, c->xhist, (T + 1) * 4));
     CHK(ensure(c, c->sethb, S * 8));
@@ -3238,24 +3313,32 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     p.h1 = (bits + 1) / 2;
     p.h2 = bits / 3 + 1;
     p.s0 = (uint32_t)c->opts.seed & p.lmask;
-    p.s1 = (uint32_t)(c->opts.seed >> 32) & p.lmask;
+    p.w1 = bits - p.h1;
+    p.w2 = bits - p.h2;
+    p.k1 = p.s0 * MIX_C1;
     p.table = table;
 
     const uint64_t *kin = c->kin;
+    // both fine codes of a k-mer in one word when every fine index fits 16 bits
+    uint32_t smax = 1;
+    for (uint32_t sb : c->set_sub) smax = std::max(smax, sb);
+    const bool c32 = (uint64_t)B * smax <= 65536;
+    CHK(ensure(c, c->bkt, std::max<uint64_t>(c->n, 1) * (c32 ? 4 : 8)));
+#define UC_CNT(MODE, C32)                                                                                          \
+    KLAUNCH(c, "k_lmin_count", (k_lmin_count<W, IPT1, MODE, C32>), ((uint32_t)T, BLOCK, (B + 1) * 4), kin,          \
+            P<BTile>(c->tiles), p, c->top_mask, c->bkt.p, P<uint32_t>(c->hist), P<uint32_t>(c->xhist))
     switch (lmin_mode(l, table != nullptr)) {
     case LM_TABLE:
-        KLAUNCH(c, "k_lmin_count", (k_lmin_count<W, IPT1, LM_TABLE>), ((uint32_t)T, BLOCK, (B + 1) * 4), kin,
-                P<BTile>(c->tiles), p, c->top_mask, P<uint2>(c->bkt), P<uint32_t>(c->hist), P<uint32_t>(c->xhist));
+        if (c32) { UC_CNT(LM_TABLE, true); } else { UC_CNT(LM_TABLE, false); }
         break;
     case LM_MUL24:
-        KLAUNCH(c, "k_lmin_count", (k_lmin_count<W, IPT1, LM_MUL24>), ((uint32_t)T, BLOCK, (B + 1) * 4), kin,
-                P<BTile>(c->tiles), p, c->top_mask, P<uint2>(c->bkt), P<uint32_t>(c->hist), P<uint32_t>(c->xhist));
+        if (c32) { UC_CNT(LM_MUL24, true); } else { UC_CNT(LM_MUL24, false); }
         break;
     default:
-        KLAUNCH(c, "k_lmin_count", (k_lmin_count<W, IPT1, LM_MUL32>), ((uint32_t)T, BLOCK, (B + 1) * 4), kin,
-                P<BTile>(c->tiles), p, c->top_mask, P<uint2>(c->bkt), P<uint32_t>(c->hist), P<uint32_t>(c->xhist));
+        if (c32) { UC_CNT(LM_MUL32, true); } else { UC_CNT(LM_MUL32, false); }
         break;
     }
+#undef UC_CNT
     // totals at hist[hb] and xhist[T], copies of both side by side for one read
     CHK(ensure(c, c->small, C_WORDS * 4));
     uint32_t *cnt = P<uint32_t>(c->small);
@@ -3270,11 +3353,15 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     c->x = tot[1];
     c->m = c->n + c->x;
     const uint64_t G = (uint64_t)S * B;
-    DBuf &k1 = any_fine ? c->kmer2 : c->kmer, &f1 = any_fine ? c->flags2 : c->flags, &s1 = any_fine ? c->src2 : c->src;
+    // pass 1 writes the final k-mers and flags when no set is split into
+    // fine buckets, else the k-mers and the fine index with the flag bits
+    DBuf &k1 = any_fine ? c->kmer2 : c->kmer, &s1 = any_fine ? c->src2 : c->src;
     CHK(ensure(c, k1, std::max<uint64_t>(c->m, 1) * W * 8));
-    CHK(ensure(c, f1, c->m + 64));
+    if (any_fine)
+        CHK(ensure(c, c->sub, c->m * 2 + 8));
+    else
+        CHK(ensure(c, c->flags, c->m + 64));
     if (keep_src) CHK(ensure(c, s1, c->m * 4));
-    CHK(ensure(c, c->sub, c->m * 2 + 8));
     CHK(ensure(c, c->pair_h, c->x * 4));
     CHK(ensure(c, c->pair_b, c->x * 4));
     CHK(ensure(c, c->boff, (G + 1) * 4));
@@ -3284,17 +3371,23 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     constexpr int NT1 = 512;
     const bool stage1 = !(c->opts.flags & UC_FLAG_DIRECT_SCATTER) &&
                         lmin_stage_bytes<W>(B, tile1, keep_src, NT1 / 64) <= c->lds_limit;
-#define UC_S1(ST)                                                                                                   \
+#define UC_S1(ST, C32)                                                                                              \
     do {                                                                                                           \
         const size_t lds = ST ? lmin_stage_bytes<W>(B, tile1, keep_src, NT1 / 64) : lmin_lds_bytes(B, NT1 / 64);      \
-        CHK(allow_lds(c, k_lmin_scatter<W, IPT1 * BLOCK / NT1, ST, NT1>, lds));                                     \
-        KLAUNCH(c, "k_lmin_scatter", (k_lmin_scatter<W, IPT1 * BLOCK / NT1, ST, NT1>), (xcd_grid(T), NT1, lds), kin, \
-                P<BTile>(c->tiles), B, nbits, c->top_mask, P<uint2>(c->bkt), P<uint32_t>(c->hist),                  \
-                P<uint32_t>(c->xhist), P<uint64_t>(k1), P<uint8_t>(f1), keep_src ? P<uint32_t>(s1) : nullptr,       \
-                P<uint32_t>(c->pair_h), P<uint32_t>(c->pair_b), P<uint16_t>(c->sub), tile_arg_xcd(T));             \
+        CHK(allow_lds(c, k_lmin_scatter<W, IPT1 * BLOCK / NT1, ST, NT1, C32>, lds));                                \
+        KLAUNCH(c, "k_lmin_scatter", (k_lmin_scatter<W, IPT1 * BLOCK / NT1, ST, NT1, C32>), (xcd_grid(T), NT1, lds), \
+                kin, P<BTile>(c->tiles), B, nbits, c->top_mask, c->bkt.p, P<uint32_t>(c->hist),                     \
+                P<uint32_t>(c->xhist), P<uint64_t>(k1), any_fine ? nullptr : P<uint8_t>(c->flags),                  \
+                keep_src ? P<uint32_t>(s1) : nullptr, P<uint32_t>(c->pair_h), P<uint32_t>(c->pair_b),               \
+                any_fine ? P<uint16_t>(c->sub) : nullptr, tile_arg_xcd(T));                                         \
     } while (0)
-    if (stage1) UC_S1(true);
-    else UC_S1(false);
+    if (stage1) {
+        if (c32) UC_S1(true, true);
+        else UC_S1(true, false);
+    } else {
+        if (c32) UC_S1(false, true);
+        else UC_S1(false, false);
+    }
 #undef UC_S1
     KLAUNCH(c, "k_gather_boff", k_gather_boff, (grid_for(G + 1), BLOCK, 0), P<uint32_t>(c->hist),
             P<uint64_t>(c->sethb), P<uint32_t>(c->setts), S, B, (uint32_t)c->m, P<uint32_t>(c->boff));
@@ -3311,20 +3404,19 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     if (!any_fine) {
         HIPCHK(c, hipMemcpyAsync(c->fboff.p, c->boff.p, (G + 1) * 4, hipMemcpyDeviceToDevice, c->stream));
     } else {
-        uint32_t smax = 1;
-        for (uint32_t sb : c->set_sub) smax = std::max(smax, sb);
-        // slots per fine-sort tile; LDS staging when the device has the room
-        // (2048-slot tiles: half the LDS of 4096, so more tiles in flight per
-        // CU; measured 0.82 -> 0.71 ms for the staged one-word scatter)
+        // slots per fine-sort round; LDS staging when the device has the room
+        // (2048-slot rounds: half the LDS of 4096, so more tiles in flight per
+        // CU; measured 0.82 -> 0.71 ms for the staged one-word scatter).  A
+        // planned tile is SUB_ROUNDS rounds.
         constexpr int IPT2 = 8;
-        const uint32_t tile2 = IPT2 * BLOCK;
+        const uint32_t round2 = IPT2 * BLOCK, tile2 = SUB_ROUNDS * round2;
         // (two-word k-mers measured faster unstaged here: 4.9 vs 5.5 ms on
         // 303 M slots; one-word ones staged: 0.82 vs 0.85 ms on 53 M)
         // threads per 2048-slot tile: 512 for the staged one-word form, 1024
         // for the direct two-word form (each measured best of 256/512/1024)
         constexpr int NT2 = W == 1 ? 512 : 1024;
         const bool stage2 = W == 1 && !(c->opts.flags & UC_FLAG_DIRECT_SCATTER) &&
-                            sub_lds_bytes<W>(tile2, smax, keep_src, true, NT2 / 64) <= c->lds_limit;
+                            sub_lds_bytes<W>(round2, smax, keep_src, true, NT2 / 64) <= c->lds_limit;
         c->tile2 = tile2;
         // tiles: at most one partial tile per coarse bucket beyond m / tile2
         const uint64_t T2 = G + (c->m + tile2 - 1) / tile2;
@@ -3339,7 +3431,7 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
         KLAUNCH(c, "k_plan2_tiles", k_plan2_tiles, (grid_for(T2), BLOCK, 0), P<uint32_t>(c->boff),
                 P<uint32_t>(c->cbfirst), P<uint32_t>(c->cbts), P<uint32_t>(c->cbtf), P<uint64_t>(c->cbhb), (uint32_t)G,
                 tile2, (uint32_t)T2, P<BTile>(c->tiles2));
-        KLAUNCH(c, "k_sub_count", k_sub_count, ((uint32_t)T2, BLOCK, smax * 4), P<uint16_t>(c->sub),
+        KLAUNCH(c, "k_sub_count", k_sub_count, ((uint32_t)T2, SUB_COUNT_NT, smax * 4), P<uint16_t>(c->sub),
                 P<BTile>(c->tiles2), P<uint32_t>(c->hist2));
         const uint32_t cpb = (smax + 63) / 64;  // column blocks per coarse bucket
         CHK(ensure(c, c->colsum, F * 4 + 4));
@@ -3353,13 +3445,13 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
         CHK(ensure(c, c->perm, c->m * 4));
 #define UC_S2(ST)                                                                                                   \
     do {                                                                                                           \
-        const size_t lds2 = sub_lds_bytes<W>(tile2, smax, keep_src, ST, NT2 / 64);                                  \
+        const size_t lds2 = sub_lds_bytes<W>(round2, smax, keep_src, ST, NT2 / 64);                                 \
         CHK(allow_lds(c, k_sub_scatter<W, ST, IPT2 * BLOCK / NT2, NT2>, lds2));                                    \
         KLAUNCH(c, "k_sub_scatter", (k_sub_scatter<W, ST, IPT2 * BLOCK / NT2, NT2>), (xcd_grid(T2), NT2, lds2),      \
                 P<BTile>(c->tiles2), P<uint16_t>(c->sub), P<uint32_t>(c->hist2), P<uint32_t>(c->fboff),             \
-                P<uint32_t>(c->cbfirst), P<uint64_t>(c->kmer2), P<uint8_t>(c->flags2),                              \
-                keep_src ? P<uint32_t>(c->src2) : nullptr, P<uint64_t>(c->kmer), P<uint8_t>(c->flags),               \
-                keep_src ? P<uint32_t>(c->src) : nullptr, P<uint32_t>(c->perm), tile_arg_xcd(T2));                  \
+                P<uint32_t>(c->cbfirst), P<uint64_t>(c->kmer2), keep_src ? P<uint32_t>(c->src2) : nullptr,          \
+                P<uint64_t>(c->kmer), P<uint8_t>(c->flags), keep_src ? P<uint32_t>(c->src) : nullptr,               \
+                P<uint32_t>(c->perm), tile_arg_xcd(T2));                                                           \
     } while (0)
         if (stage2) UC_S2(true);
         else UC_S2(false);
@@ -4666,7 +4758,9 @@ extern "C" int uc_lmer_order_table(uc_ctx *c, uint32_t l, uint32_t *table, int o
     p.h1 = (bits + 1) / 2;
     p.h2 = bits / 3 + 1;
     p.s0 = (uint32_t)c->opts.seed & p.lmask;
-    p.s1 = (uint32_t)(c->opts.seed >> 32) & p.lmask;
+    p.w1 = bits - p.h1;
+    p.w2 = bits - p.h2;
+    p.k1 = p.s0 * MIX_C1;
     p.table = nullptr;
     const uint64_t n = 1ull << bits;
     uint32_t *dst = table;

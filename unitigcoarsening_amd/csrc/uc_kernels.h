@@ -67,7 +67,8 @@ struct ETile {              // edge-finding tile: slots of one bucket
 
 struct LminParams {
     uint32_t k, l, B;
-    uint32_t lmask, h1, h2, s0, s1;  // order bijection constants
+    uint32_t lmask, h1, h2, s0;  // order bijection: mask, xorshift amounts, seed word
+    uint32_t w1, w2, k1;         // 2l - h1, 2l - h2, s0 * MIX_C1 (mod 2^32)
     const uint32_t *table;           // optional order table (4^l entries)
 };
 
