@@ -502,6 +502,9 @@ def main():
     ap.add_argument("--serve-budget-gb", type=float, default=288.0)
     ap.add_argument("--large", action="store_true",
                     help="the large instance's model (k=63, l=13, 256 buckets, repeat families) at --genome-bp")
+    ap.add_argument("--low-memory", action="store_true",
+                    help="UC_FLAG_LOW_MEMORY: free every buffer after its last use (the input is consumed, so it is "
+                         "loaded again before each step, outside the timed region); on by default with --large")
     args = ap.parse_args()
     if args.large:
         # defaults of the large instance unless given explicitly
@@ -516,6 +519,7 @@ def main():
         if "--genome-bp" not in sys.argv:
             args.genome_bp = 300_000_000
         args.no_sets = args.no_serve = True
+        args.low_memory = True
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the path shards by minimizer bucket "
@@ -535,8 +539,19 @@ def main():
     mode = uc.UC_MODE_PLAIN if args.mode == "plain" else uc.UC_MODE_BATCHED
     tun = {"load_pct": args.load_pct, "sample_log2": args.sample_log2, "fine_slots": args.fine_slots,
            "group_slots": args.group_slots, "direct_scatter": args.direct_scatter}
-    c = uc.Compactor(n_buckets=B, mode=mode, profile=True, xcd_map=args.xcd_map, eager=args.eager, **tun)
+    lowmem = args.low_memory
+    c = uc.Compactor(n_buckets=B, mode=mode, profile=True, xcd_map=args.xcd_map, eager=args.eager,
+                     low_memory=lowmem, **tun)
     c.load(kmers, k)  # one H2D copy; every step starts from HBM
+
+    fresh = [True]  # the loaded input has not been consumed by a job yet
+
+    def reload():
+        # a low-memory job consumes its loaded input: copy it in again, untimed
+        if lowmem and not fresh[0]:
+            c.load(kmers, k)
+            c.sync()
+        fresh[0] = not lowmem
     tune = None
     if args.autotune:
         t = time.time()
@@ -552,6 +567,7 @@ def main():
     for i in range(n_warm):
         if i == 1:
             c.kernel_stats_reset()  # the first step carries one-time costs (code loading, allocation)
+        reload()
         c.compact_loaded(l)
         log(f"warmup {i + 1}/{n_warm}: {c.timing().total_ms:.2f} ms")
     c.sync()
@@ -559,19 +575,35 @@ def main():
     warm_steps = n_warm - 1 if n_warm > 1 else 1
     dom_name = max(ks.items(), key=lambda kv: kv[1][0])[0]
     c.kernel_stats_select(dom_name)
+    reload()
     c.compact_loaded(l)  # one more untimed step: the contraction pass is captured as a graph here
     c.sync()
     c.kernel_stats_reset()
     phase = {"bucketize_ms": 0.0, "edges_ms": 0.0, "coarsen_ms": 0.0, "emit_ms": 0.0}
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        c.compact_loaded(l)
-        tm = c.timing()
-        for key in phase:
-            phase[key] += getattr(tm, key)
-    c.sync()
-    t1 = time.perf_counter()
-    wall = t1 - t0
+    peak = 0
+    if not lowmem:
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            c.compact_loaded(l)
+            tm = c.timing()
+            peak = max(peak, tm.device_peak_bytes)
+            for key in phase:
+                phase[key] += getattr(tm, key)
+        c.sync()
+        wall = time.perf_counter() - t0
+    else:
+        # each step times its own job; the input copy before it is not timed
+        wall = 0.0
+        for i in range(args.steps):
+            reload()
+            t0 = time.perf_counter()
+            c.compact_loaded(l)
+            c.sync()
+            wall += time.perf_counter() - t0
+            tm = c.timing()
+            peak = max(peak, tm.device_peak_bytes)
+            for key in phase:
+                phase[key] += getattr(tm, key)
     ms_per_step = 1e3 * wall / args.steps
     value = n * args.steps / wall
     log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step, {value / 1e6:.1f} M k-mers/s")
@@ -597,12 +629,13 @@ def main():
                for name, (ms, nl) in sorted(ks.items(), key=lambda kv: -kv[1][0])}
     text_bytes = c.text_size()
     device_gib = round(c.held_bytes() / 2 ** 30, 2)
+    device_peak_gib = round(peak / 2 ** 30, 2)
     c.close()
 
     # hash-table probes of the same job, counted in a step of its own (the
     # timed steps count nothing)
     with uc.Compactor(n_buckets=B, mode=mode, xcd_map=args.xcd_map, eager=args.eager, count_probes=True,
-                      **tun) as cp:
+                      low_memory=lowmem, **tun) as cp:
         cp.load(kmers, args.k)
         cp.compact_loaded(l)
         probes = cp.edge_probes()
@@ -664,6 +697,7 @@ def main():
             "largest_fine_bucket": b.largest_fine_bucket, "bucket_tile": b.bucket_tile, "fine_tile": b.fine_tile,
             "lds_limit": b.lds_limit, "n_oversized_fine": b.n_oversized, "oversized_slots": b.oversized_slots,
             "n_splitters": ci.n_splitters, "chain_rounds": ci.chain_rounds, "device_gib_held": device_gib,
+            "device_gib_peak": device_peak_gib, "low_memory": lowmem,
             "n_runs": ci.n_runs,
         },
         "phases_ms_per_step": {kk: round(v / args.steps, 3) for kk, v in phase.items()},

@@ -72,6 +72,11 @@ typedef struct {
 #define UC_FLAG_EAGER 16u /* launch the kernels of a contraction pass one by one instead of replaying
                              them as a captured HIP graph */
 #define UC_FLAG_COUNT_PROBES 32u /* count the hash-table probes of the edge phase (uc_edge_probes) */
+#define UC_FLAG_LOW_MEMORY 64u /* free every buffer at the phase boundary after its last use, the loaded input
+                                  (uc_load*) after the first scatter included: the device holds about half the
+                                  bytes, each job pays a few hipFree/hipMalloc, the input is consumed (load again
+                                  before the next uc_compact_loaded), and of the intermediate results only the
+                                  text and the counts (uc_*_info_get) stay readable after a job */
 
 void uc_default_opts(uc_opts *o);
 int uc_ctx_create(const uc_opts *opts, uc_ctx **out);
@@ -286,6 +291,7 @@ int uc_loaded_copy(uc_ctx *ctx, uint32_t *packed, uint64_t cap, uint64_t *n);
 typedef struct {
     float bucketize_ms, edges_ms, coarsen_ms, emit_ms, total_ms;
     uint64_t device_bytes; /* device memory held by the context */
+    uint64_t device_peak_bytes; /* most device memory the context held at once during the job */
 } uc_timing;
 
 /* Per-phase device times (HIP events on the context's stream) of the last

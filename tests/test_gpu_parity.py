@@ -533,3 +533,56 @@ def test_medium_model_shuffled_vs_textbook_and_refpipe(k, l, B):
     assert uc.sorted_lines(ref) == tb
     # plain form byte-identical to batched on the same (unsorted) slots
     assert run(kmers, k, l, n_buckets=B, mode=uc.UC_MODE_PLAIN) == got
+
+
+@pytest.mark.parametrize("k,l,B,model", [(31, 11, 64, "medium"), (63, 13, 256, "large")])
+def test_low_memory_mode_same_bytes_smaller_peak(k, l, B, model):
+    # UC_FLAG_LOW_MEMORY frees every buffer after its last use: the text is
+    # byte for byte the normal mode's, the peak is lower, and the loaded
+    # input is consumed by the job
+    kmers = uc.medium_model(31, 600_000, k) if model == "medium" else uc.large_model(8, 1_500_000, k)
+    with uc.Compactor(n_buckets=B) as c:
+        c.load(kmers, k)
+        c.compact_loaded(l)
+        want, peak_full = c.text(), c.timing().device_peak_bytes
+    with uc.Compactor(n_buckets=B, low_memory=True) as c:
+        for _ in range(2):  # twice: the freed buffers come back for the next job
+            c.load(kmers, k)
+            c.compact_loaded(l)
+            got, peak_low = c.text(), c.timing().device_peak_bytes
+            assert got == want
+            log(f"low memory {model}: peak {peak_low / 2**20:.1f} MiB against {peak_full / 2**20:.1f} MiB")
+            assert 0 < peak_low < 0.8 * peak_full
+            info = c.coarsen_info()
+            assert info.text_bytes == len(want)
+            # the input is consumed
+            with pytest.raises(uc.UcError) as e:
+                c.compact_loaded(l)
+            assert e.value.status == uc.UC_E_STATE
+        # a whole job from host memory works as usual, and so does one from
+        # the k-mer text parsed on the device
+        assert c.compact(kmers, k, l) == want
+        c.load_text(uc.format_text(kmers, k), k)
+        c.compact_loaded(l)
+        assert c.text() == want
+        held = c.held_bytes()
+        c.load(kmers, k)  # the pool stays: loading and running again allocates nothing new
+        c.compact_loaded(l)
+        assert c.text() == want and c.held_bytes() == held
+    assert uc.sorted_lines(want) == textbook_sorted(kmers, k)
+
+
+def test_low_memory_mode_with_isolated_cycles():
+    # cycles make contraction run twice with the cycle cut between; a
+    # low-memory job gives the pass's arrays back around the cut
+    rng = random.Random(77)
+    cyc = circular_kmers(rng, 3000, 31) + circular_kmers(rng, 1207, 31)
+    base = uc.decode(uc.medium_model(77, 200_000, 31), 31)
+    kmers = uc.encode(sorted(set(cyc) | set(base)), 31)
+    want = run(kmers, 31, 11, n_buckets=16)
+    with uc.Compactor(n_buckets=16, low_memory=True) as c:
+        got = c.compact(kmers, 31, 11)
+        info = c.coarsen_info()
+    assert info.n_cycles >= 2
+    assert got == want
+    assert uc.sorted_lines(got) == textbook_sorted(kmers, 31)

@@ -30,6 +30,7 @@ UC_FLAG_KEEP_SRC = 4
 UC_FLAG_DIRECT_SCATTER = 8
 UC_FLAG_EAGER = 16
 UC_FLAG_COUNT_PROBES = 32
+UC_FLAG_LOW_MEMORY = 64
 UC_SCRATCH_BYTES = 256 << 20
 UC_TUNE_KEEP_BUCKETS = 1
 
@@ -98,6 +99,7 @@ class UcTiming(C.Structure):
         ("emit_ms", C.c_float),
         ("total_ms", C.c_float),
         ("device_bytes", C.c_uint64),
+        ("device_peak_bytes", C.c_uint64),
     ]
 
 
@@ -355,7 +357,7 @@ class Compactor:
 
     def __init__(self, device=0, seed=0x5EED, n_buckets=64, mode=UC_MODE_BATCHED, load_pct=50, sample_log2=0,
                  hbm_budget=0, profile=False, xcd_map=False, keep_src=False, fine_slots=0, group_slots=0,
-                 direct_scatter=False, eager=False, count_probes=False):
+                 direct_scatter=False, eager=False, count_probes=False, low_memory=False):
         L = lib()
         o = UcOpts()
         L.uc_default_opts(C.byref(o))
@@ -363,7 +365,8 @@ class Compactor:
         o.load_pct, o.sample_log2, o.hbm_budget = load_pct, sample_log2, hbm_budget
         o.flags = ((UC_FLAG_PROFILE if profile else 0) | (UC_FLAG_XCD_MAP if xcd_map else 0)
                    | (UC_FLAG_KEEP_SRC if keep_src else 0) | (UC_FLAG_DIRECT_SCATTER if direct_scatter else 0)
-                   | (UC_FLAG_EAGER if eager else 0) | (UC_FLAG_COUNT_PROBES if count_probes else 0))
+                   | (UC_FLAG_EAGER if eager else 0) | (UC_FLAG_COUNT_PROBES if count_probes else 0)
+                   | (UC_FLAG_LOW_MEMORY if low_memory else 0))
         o.fine_slots, o.group_slots = fine_slots, group_slots
         self.opts = o
         self._ctx = _P()

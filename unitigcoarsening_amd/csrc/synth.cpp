@@ -27,6 +27,19 @@ namespace {
 
 typedef unsigned __int128 u128;
 
+// UC_SYNTH_PROGRESS=1: one line on stderr per stage of a large generation
+// (the sorts of a few billion k-mers take minutes)
+void progress(const char *stage, uint64_t a = 0)
+{
+    static const bool on = [] {
+        const char *e = std::getenv("UC_SYNTH_PROGRESS");
+        return e && e[0] == '1';
+    }();
+    if (!on) return;
+    std::fprintf(stderr, "synth: %s %llu (%.1f s)\n", stage, (unsigned long long)a, omp_get_wtime());
+    std::fflush(stderr);
+}
+
 struct Rng {
     uint64_t s[4];
     static uint64_t splitmix(uint64_t &x)
@@ -95,10 +108,19 @@ void parallel_sort_unique(std::vector<K> &v)
     std::vector<K> tmp(n);
     std::vector<size_t> pos(cnt.begin(), cnt.end() - 1);
     for (size_t i = 0; i < n; i++) tmp[pos[(size_t)(v[i] >> shift)]++] = v[i];
+    progress("sort: parts scattered", n);
+    int done = 0;
 #pragma omp parallel for schedule(dynamic, 1)
-    for (int p = 0; p < parts; p++) std::sort(tmp.begin() + cnt[p], tmp.begin() + cnt[p + 1]);
+    for (int p = 0; p < parts; p++) {
+        std::sort(tmp.begin() + cnt[p], tmp.begin() + cnt[p + 1]);
+        int d;
+#pragma omp atomic capture
+        d = ++done;
+        if (d % 32 == 0) progress("sort: parts sorted", (uint64_t)d);
+    }
     tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
     v.swap(tmp);
+    progress("sort: distinct", v.size());
 }
 
 template <class K>
@@ -140,6 +162,7 @@ int run(const uc_synth_params *p, uint32_t **out, uint64_t *n_out)
             }
         }
     }
+    progress("genome with repeats", G);
     const K mask = lex_mask<K>(k);
     const uint64_t nk = G - k + 1;
     std::vector<K> kmers(nk);
@@ -178,6 +201,7 @@ int run(const uc_synth_params *p, uint32_t **out, uint64_t *n_out)
         }
         parallel_sort_unique(kmers);
     }
+    progress("k-mers", kmers.size());
     const uint32_t w = (k + 15) / 16;
     const uint64_t n = kmers.size();
     uint32_t *buf = (uint32_t *)std::malloc((n ? n : 1) * w * sizeof(uint32_t));

@@ -32,6 +32,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <map>
 #include <mutex>
 #include <string>
 #include <type_traits>
@@ -1940,6 +1941,17 @@ __global__ __launch_bounds__(NT) void k_seg_count(const uint64_t *__restrict__ r
     }
 }
 
+// the ranked chain entries of the sampled splitters, from one chain buffer
+// to the other
+__global__ void k_chain_move(const uint32_t *__restrict__ list, uint32_t n, const uint64_t *__restrict__ from,
+                             uint64_t *__restrict__ to)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t r = list[i];
+        to[r] = from[r];
+    }
+}
+
 // The chain kernels run over the sampled splitters, whose count the device
 // holds (*np): a fixed grid strides over them, so no host read of the count
 // stands between the segment walks and the chain rounds.
@@ -2535,6 +2547,17 @@ static __device__ __align__(256) unsigned char scratch[UC_SCRATCH_BYTES];
 // kernels that read a whole aligned line past a buffer's end stay inside
 constexpr size_t SCRATCH_TAIL = 64 << 10;
 
+// UC_FLAG_LOW_MEMORY: one device block per context, kept from job to job.
+// Buffers are carved from it first-fit in address order (256-byte aligned)
+// and given back at their last use, so phases reuse each other's memory and
+// a job after the first calls no hipMalloc (a fresh allocation of tens of GB
+// costs seconds).  Free ranges coalesce.
+struct Pool {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+    std::map<size_t, size_t> free_;  // offset -> bytes, disjoint, not adjacent
+};
+
 }  // namespace uc
 
 using namespace uc;
@@ -2543,7 +2566,7 @@ struct uc_ctx {
     uc_opts opts;
     hipStream_t stream = nullptr;
     std::string err;
-    size_t held = 0;
+    size_t held = 0, held_peak = 0;  // device bytes held now / most held at once since the job began
     uc_meter *meter = nullptr;  // a server's account of its workers' device bytes
 
     // job
@@ -2617,6 +2640,10 @@ struct uc_ctx {
     unsigned char *arena = nullptr;
     size_t arena_cap = 0, arena_used = 0;
     bool text_fixed = false;
+
+    // UC_FLAG_LOW_MEMORY
+    Pool pool;
+    size_t pool_target = 0;  // bytes the pool should have for the current job (an upper bound)
 };
 
 // every device buffer of a context
@@ -2687,6 +2714,141 @@ int fail(uc_ctx *c, int code, const std::string &msg)
         if (rc_ != UC_OK) return rc_; \
     } while (0)
 
+// buffers a low-memory context carves from its pool: all but those of the
+// text and table loads (their sizes come before the job's), which are
+// allocated and freed on their own
+static bool pooled(const uc_ctx *c, const DBuf &b)
+{
+    return (c->opts.flags & UC_FLAG_LOW_MEMORY) && !c->arena && &b != &c->stage && &b != &c->txt &&
+           &b != &c->txtcnt && &b != &c->otable;
+}
+
+// Pool placement: buffers that live until the job's text is written are
+// carved upward from the bottom, every other one downward from the top, so
+// that when emission asks for the text the space above the survivors is one
+// free range.
+static bool keeps(const uc_ctx *c, const DBuf &b)
+{
+    for (const DBuf *k : {&c->kmer, &c->flags, &c->hasp, &c->rec, &c->rch, &c->mlist, &c->mcnt, &c->ulen,
+                          &c->chainA, &c->list, &c->tsum, &c->text, &c->small, &c->boff, &c->fboff, &c->cbfirst,
+                          &c->setoff, &c->probes})
+        if (k == &b) return true;
+    return false;
+}
+
+static bool in_pool(const uc_ctx *c, const void *p)
+{
+    return c->pool.base && p >= c->pool.base && p < c->pool.base + c->pool.cap;
+}
+
+static size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+void note_peak(uc_ctx *c)
+{
+    c->held_peak = std::max(c->held_peak, c->held - c->pool.cap + c->pool.used);
+}
+
+// the pool, sized for the current job once per context (again when a larger
+// job would get a larger block); everything free on the device but a margin
+// at most, within hbm_budget
+int pool_reserve(uc_ctx *c)
+{
+    Pool &P_ = c->pool;
+    size_t fr = 0, tot = 0;
+    HIPCHK(c, hipMemGetInfo(&fr, &tot));
+    const size_t margin = std::max<size_t>(2ull << 30, tot / 64);
+    auto room = [&](size_t extra) {
+        size_t r = fr + extra > margin ? fr + extra - margin : 0;
+        r = std::min(r, std::max<size_t>(c->pool_target, 64ull << 20));
+        if (c->opts.hbm_budget) {
+            const size_t other = c->held - P_.cap;
+            r = std::min(r, c->opts.hbm_budget > other ? c->opts.hbm_budget - other : 0);
+        }
+        return r & ~(size_t)255;
+    };
+    if (P_.base) {
+        // an idle pool grows when a larger job would get at least an eighth more
+        if (P_.used || room(P_.cap) <= P_.cap + P_.cap / 8) return UC_OK;
+        HIPCHK(c, hipFree(P_.base));
+        c->held -= P_.cap;
+        if (c->meter) c->meter->sub(P_.cap);
+        P_ = Pool{};
+        HIPCHK(c, hipMemGetInfo(&fr, &tot));
+    }
+    const size_t cap = room(0);
+    if (cap < SCRATCH_TAIL + 4096)
+        return fail(c, UC_E_NOMEM, "no device memory for the low-memory pool (" + std::to_string(fr) + " free)");
+    void *p = nullptr;
+    const hipError_t e = hipMalloc(&p, cap);
+    if (e != hipSuccess)
+        return fail(c, UC_E_NOMEM, "hipMalloc of the " + std::to_string(cap) + "-byte pool: " + hipGetErrorString(e));
+    P_.base = static_cast<char *>(p);
+    P_.cap = cap;
+    P_.used = 0;
+    P_.free_.clear();
+    P_.free_[0] = cap - SCRATCH_TAIL;  // a tail no buffer gets: reads of a line past an end stay inside
+    c->held += cap;
+    if (c->meter) c->meter->add(cap);
+    return UC_OK;
+}
+
+void pool_release(uc_ctx *c, DBuf &b)
+{
+    Pool &P_ = c->pool;
+    size_t off = (size_t)(static_cast<char *>(b.p) - P_.base), len = round256(b.cap);
+    P_.used -= len;
+    auto nx = P_.free_.lower_bound(off);
+    if (nx != P_.free_.end() && off + len == nx->first) {  // merge with the next range
+        len += nx->second;
+        nx = P_.free_.erase(nx);
+    }
+    if (nx != P_.free_.begin()) {
+        auto pv = std::prev(nx);
+        if (pv->first + pv->second == off) {  // and with the previous one
+            pv->second += len;
+            b.p = nullptr;
+            b.cap = 0;
+            return;
+        }
+    }
+    P_.free_[off] = len;
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+int pool_carve(uc_ctx *c, DBuf &b, size_t bytes)
+{
+    CHK(pool_reserve(c));
+    Pool &P_ = c->pool;
+    const size_t len = round256(bytes);
+    auto take = [&](size_t off) {
+        b.p = P_.base + off;
+        b.cap = bytes;
+        P_.used += len;
+        note_peak(c);
+        return UC_OK;
+    };
+    if (keeps(c, b)) {  // lowest range that fits, its low end
+        for (auto it = P_.free_.begin(); it != P_.free_.end(); ++it) {
+            if (it->second < len) continue;
+            const size_t off = it->first, rest = it->second - len;
+            P_.free_.erase(it);
+            if (rest) P_.free_[off + len] = rest;
+            return take(off);
+        }
+    } else {  // highest range that fits, its high end
+        for (auto it = P_.free_.rbegin(); it != P_.free_.rend(); ++it) {
+            if (it->second < len) continue;
+            it->second -= len;
+            const size_t off = it->first + it->second;
+            if (it->second == 0) P_.free_.erase(std::next(it).base());
+            return take(off);
+        }
+    }
+    return fail(c, UC_E_NOMEM, "the low-memory pool of " + std::to_string(P_.cap) + " bytes has no free range of " +
+                                   std::to_string(len) + " (" + std::to_string(P_.used) + " in use)");
+}
+
 int ensure(uc_ctx *c, DBuf &b, size_t bytes)
 {
     if (bytes == 0) bytes = 8;
@@ -2704,6 +2866,10 @@ int ensure(uc_ctx *c, DBuf &b, size_t bytes)
         c->arena_used = off + bytes;
         return UC_OK;
     }
+    if (pooled(c, b)) {
+        if (b.p) pool_release(c, b);  // (contents are not carried over)
+        return pool_carve(c, b, bytes);
+    }
     if (b.p) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
         HIPCHK(c, hipFree(b.p));
@@ -2715,10 +2881,34 @@ int ensure(uc_ctx *c, DBuf &b, size_t bytes)
     if (c->opts.hbm_budget && c->held + bytes > c->opts.hbm_budget)
         return fail(c, UC_E_NOMEM, "device budget exceeded: need " + std::to_string(c->held + bytes) + " bytes");
     hipError_t e = hipMalloc(&b.p, bytes);
-    if (e != hipSuccess) return fail(c, UC_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return fail(c, UC_E_NOMEM, std::string("hipMalloc of ") + std::to_string(bytes) + " bytes with " +
+                                       std::to_string(c->held) + " held: " + hipGetErrorString(e));
+    }
     b.cap = bytes;
     c->held += bytes;
+    note_peak(c);
     if (c->meter) c->meter->add(bytes);
+    return UC_OK;
+}
+
+// UC_FLAG_LOW_MEMORY: give a buffer back once the job is past its last use.
+// A pool range is free for the next carve at once: work queued on the stream
+// after this point is ordered after every kernel that used it.  Other
+// buffers go back with hipFree, which waits for the device.
+int drop(uc_ctx *c, DBuf &b)
+{
+    if (!(c->opts.flags & UC_FLAG_LOW_MEMORY) || c->arena || !b.p) return UC_OK;
+    if (in_pool(c, b.p)) {
+        pool_release(c, b);
+        return UC_OK;
+    }
+    HIPCHK(c, hipFree(b.p));
+    c->held -= b.cap;
+    if (c->meter) c->meter->sub(b.cap);
+    b.p = nullptr;
+    b.cap = 0;
     return UC_OK;
 }
 
@@ -2895,6 +3085,18 @@ int h2d(uc_ctx *c, void *dst, const void *src, size_t bytes)
     return UC_OK;
 }
 
+// UC_FLAG_LOW_MEMORY: a new load gives back every buffer of the last job
+// first (its results are no longer readable once a new job is loaded)
+int drop_all(uc_ctx *c)
+{
+    if (!(c->opts.flags & UC_FLAG_LOW_MEMORY) || c->arena) return UC_OK;
+    // (a captured contraction pass stays: its key holds the buffer addresses,
+    // which the pool hands out again in the same order)
+    for (DBuf *b : all_bufs(c)) CHK(drop(c, *b));
+    c->kin = nullptr;
+    return UC_OK;
+}
+
 void reset_job(uc_ctx *c)
 {
     c->loaded = c->bucketed = c->edged = c->coarsened = c->emitted = false;
@@ -2921,6 +3123,7 @@ int begin_job(uc_ctx *c, const uint64_t *set_offsets, uint32_t n_sets, uint32_t 
     if (c->set_off[0] != 0) return fail(c, UC_E_ARG, "set offsets must start at 0");
     c->n = c->set_off[n_sets];
     if (c->n >= 0xFFFFFFFFull) return fail(c, UC_E_LIMIT, "more than 2^32-2 k-mers");
+    c->pool_target = uc_job_bytes_estimate(c->n, k);
     const uint32_t W = c->W;
     const uint32_t top_bits = 2 * k - 64 * (W - 1);
     c->top_mask = top_bits >= 64 ? ~0ull : ((1ull << top_bits) - 1);
@@ -2997,6 +3200,7 @@ int staged_d2h(uc_ctx *c, void *dst, const void *src, uint64_t bytes)
 
 int do_load(uc_ctx *c, const uint32_t *packed, const uint64_t *set_offsets, uint32_t n_sets, uint32_t k, int on_device)
 {
+    CHK(drop_all(c));
     reset_job(c);
     CHK(begin_job(c, set_offsets, n_sets, k));
     const uint32_t rw = (k + 15) / 16;
@@ -3103,11 +3307,15 @@ int parse_loaded_text(uc_ctx *c, uint64_t len, uint32_t k)
 {
     if (k < 2 || k > 64) return fail(c, UC_E_ARG, "k must be in [2, 64]");
     CHK(fill<uint8_t>(c, P<uint8_t>(c->txt) + len, TXT_PAD, (uint8_t)0));
-    return k <= 32 ? parse_text<1>(c, len, k) : parse_text<2>(c, len, k);
+    CHK(k <= 32 ? parse_text<1>(c, len, k) : parse_text<2>(c, len, k));
+    // a low-memory context gives the text back once it is packed
+    CHK(drop(c, c->txt));
+    return drop(c, c->txtcnt);
 }
 
 int do_load_text(uc_ctx *c, const char *text, uint64_t len, uint32_t k, int on_device)
 {
+    CHK(drop_all(c));
     reset_job(c);
     CHK(ensure(c, c->txt, len + TXT_PAD));
     if (len) {
@@ -3151,6 +3359,7 @@ int stream_file(uc_ctx *c, FILE *f, uint64_t len, char *dst, const char *path)
 
 int do_load_file(uc_ctx *c, const char *path, uint32_t k)
 {
+    CHK(drop_all(c));
     reset_job(c);
     FILE *f = std::fopen(path, "rb");
     if (!f) return fail(c, UC_E_IO, std::string("could not open input file ") + path);
@@ -3169,6 +3378,7 @@ int do_load_file(uc_ctx *c, const char *path, uint32_t k)
 
 int do_load_packed_file(uc_ctx *c, const char *path, uint32_t *k_out)
 {
+    CHK(drop_all(c));
     reset_job(c);
     uc_packed_header h;
     const int hr = uc_read_packed_header(path, &h);
@@ -3392,6 +3602,18 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     KLAUNCH(c, "k_gather_boff", k_gather_boff, (grid_for(G + 1), BLOCK, 0), P<uint32_t>(c->hist),
             P<uint64_t>(c->sethb), P<uint32_t>(c->setts), S, B, (uint32_t)c->m, P<uint32_t>(c->boff));
     c->rows_valid = false;
+    if (c->opts.flags & UC_FLAG_LOW_MEMORY) {
+        // the codes, the input and the tile histogram are read by pass 1 and
+        // the row gather only; a loaded input is consumed
+        CHK(drop(c, c->bkt));
+        CHK(drop(c, c->hist));
+        CHK(drop(c, c->xhist));
+        if (c->kin == P<uint64_t>(c->in)) {
+            CHK(drop(c, c->in));
+            c->kin = nullptr;
+            c->loaded = false;
+        }
+    }
 
     // fine buckets: stable sort of each coarse bucket's slots by fine index,
     // planned and run on the device (no host round trip)
@@ -3459,6 +3681,12 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
         if (c->x)
             KLAUNCH(c, "k_pair_remap", k_pair_remap, (grid_for(c->x), BLOCK, 0), P<uint32_t>(c->pair_h),
                     P<uint32_t>(c->pair_b), (uint32_t)c->x, P<uint32_t>(c->perm));
+        // pass 2's input and the permutation are dead now
+        CHK(drop(c, c->kmer2));
+        CHK(drop(c, c->sub));
+        CHK(drop(c, c->src2));
+        CHK(drop(c, c->perm));
+        CHK(drop(c, c->hist2));
     }
     c->l = l;
     c->bucketed = true;
@@ -3810,7 +4038,12 @@ int do_edges(uc_ctx *c, uint32_t bucket)
 {
     if (!c->bucketed) return fail(c, UC_E_STATE, "uc_find_edges before uc_bucketize");
     c->coarsened = c->emitted = false;
-    return c->W == 1 ? run_edges<1>(c, bucket) : run_edges<2>(c, bucket);
+    CHK(c->W == 1 ? run_edges<1>(c, bucket) : run_edges<2>(c, bucket));
+    if (c->edged) {  // every bucket done: the hash tables and their hit positions are dead
+        CHK(drop(c, c->table));
+        CHK(drop(c, c->hit));
+    }
+    return UC_OK;
 }
 
 // ---- contraction ----------------------------------------------------------
@@ -4023,11 +4256,9 @@ int cut_cycles(uc_ctx *c, uint64_t *n_cut)
     return UC_OK;
 }
 
-int do_coarsen(uc_ctx *c)
+// every array a contraction pass writes or reads besides the edge result
+int ensure_contraction(uc_ctx *c)
 {
-    if (!c->edged) return fail(c, UC_E_STATE, "uc_coarsen before uc_find_edges");
-    c->emitted = false;
-    const uint32_t m = (uint32_t)c->m;
     const uint64_t T = (c->m + CT_TILE - 1) / CT_TILE;
     CHK(ensure(c, c->small, C_WORDS * 4));
     CHK(ensure(c, c->tcnt, (T + 1) * 8));
@@ -4040,6 +4271,15 @@ int do_coarsen(uc_ctx *c)
     CHK(ensure(c, c->chainB, c->m * 8));
     CHK(ensure(c, c->list, c->m * 4));
     CHK(ensure(c, c->tsum, (T + 1) * 8));
+    return UC_OK;
+}
+
+int do_coarsen(uc_ctx *c)
+{
+    if (!c->edged) return fail(c, UC_E_STATE, "uc_coarsen before uc_find_edges");
+    c->emitted = false;
+    const uint32_t m = (uint32_t)c->m;
+    CHK(ensure_contraction(c));
     c->chain_rounds = 0;
     c->n_cycles = 0;
     c->n_list = 0;
@@ -4055,12 +4295,19 @@ int do_coarsen(uc_ctx *c)
     uint32_t nonconv = 0;
     if (m) CHK(contract_pass(c, &live, &nonconv));
     if (live != c->n || nonconv) {
+        // cutting cycles reads the ranked chain and labels in the other chain
+        // buffer; the pass's other arrays are rewritten by the next pass, so
+        // a low-memory job gives them back meanwhile
+        for (DBuf *b : {&c->seg, &c->mlist, &c->prevseg, &c->ulen, &c->list, &c->tsum, &c->tcnt, &c->mcnt})
+            CHK(drop(c, *b));
         CHK(ensure(c, c->cycB, c->m * 8));
         CHK(ensure(c, c->clist, c->m * 4));
         CHK(ensure(c, c->pred, c->m * 4));
         uint64_t cut = 0;
         CHK(c->W == 1 ? cut_cycles<1>(c, &cut) : cut_cycles<2>(c, &cut));
         c->n_cycles = cut;
+        for (DBuf *b : {&c->cycB, &c->clist, &c->pred}) CHK(drop(c, *b));
+        CHK(ensure_contraction(c));
         CHK(contract_pass(c, &live, &nonconv));
         if (live != c->n || nonconv)
             return fail(c, UC_E_STATE, "contraction lost k-mers: " + std::to_string(live) + " of " +
@@ -4068,6 +4315,23 @@ int do_coarsen(uc_ctx *c)
     }
     c->text_bytes = live + c->n_unitigs * c->k;
     c->coarsened = true;
+    if (c->opts.flags & UC_FLAG_LOW_MEMORY) {
+        // emission reads the k-mers, flags, predecessor bits, run records and
+        // characters, the head lists, unitig lengths, tile sums, the sampled
+        // splitters and their ranked chain; everything else is dead.  The
+        // ranked entries move to chainA, which the pool keeps low.
+        if (c->chain != P<uint64_t>(c->chainA)) {
+            if (c->n_list)
+                KLAUNCH(c, "k_chain_move", k_chain_move, (std::min<uint32_t>(grid_for(c->n_list), 65536), BLOCK, 0),
+                        P<uint32_t>(c->list), c->n_list, c->chain, P<uint64_t>(c->chainA));
+            c->chain = P<uint64_t>(c->chainA);
+        }
+        // (small planning buffers included: left in the pool's top range
+        // they would split the space the text needs)
+        for (DBuf *b : all_bufs(c))
+            if (!keeps(c, *b)) CHK(drop(c, *b));
+        c->edged = false;  // the links are gone: contraction cannot run again on this edge result
+    }
     return UC_OK;
 }
 
@@ -4084,7 +4348,18 @@ int run_emit(uc_ctx *c)
     } else {
         CHK(ensure(c, c->text, c->text_bytes + 1));
     }
-    CHK(ensure(c, c->htoff, c->m * 8));
+    // Head text offsets of multi-segment unitigs (written by k_head_emit,
+    // read by k_rand_emit), one word per slot.  A low-memory job of one set
+    // keeps them in the head lists' memory instead: a tile reads its own list
+    // entries (inside its own slot range) before a barrier and writes offsets
+    // only inside its own slot range after it, and nothing reads the lists
+    // after k_head_emit (k_set_offsets runs for several sets only).
+    const bool alias_htoff = (c->opts.flags & UC_FLAG_LOW_MEMORY) && c->n_sets == 1;
+    if (alias_htoff)
+        CHK(drop(c, c->htoff));
+    else
+        CHK(ensure(c, c->htoff, c->m * 8));
+    uint64_t *htoff = alias_htoff ? P<uint64_t>(c->mlist) : P<uint64_t>(c->htoff);
     if (m) {
         // one-word k-mers from 256-thread groups, wider ones from 512-thread
         // groups, both staged in LDS (each measured faster than the other
@@ -4093,12 +4368,11 @@ int run_emit(uc_ctx *c)
         KLAUNCH(c, "k_head_emit", (k_head_emit<W, true, NT>), (xcd_grid(T), NT, 0), P<uint64_t>(c->kmer),
                 P<uint64_t>(c->rec), P<char>(c->rch), P<uint32_t>(c->hasp), P<uint8_t>(c->flags),
                 P<uint64_t>(c->mlist), P<uint32_t>(c->mcnt), P<uint32_t>(c->ulen), P<uint64_t>(c->tsum), m, c->k,
-                c->salt, c->shift, P<uint64_t>(c->htoff), P<char>(c->text), tile_arg(c, T));
+                c->salt, c->shift, htoff, P<char>(c->text), tile_arg(c, T));
     }
     if (c->n_list) {
         KLAUNCH(c, "k_rand_emit", k_rand_emit, (grid_for(c->n_list), BLOCK, 0), P<uint64_t>(c->rec), P<char>(c->rch),
-                P<uint32_t>(c->list), c->n_list, c->chain, P<uint64_t>(c->htoff), c->k, c->salt, c->shift, m,
-                P<char>(c->text));
+                P<uint32_t>(c->list), c->n_list, c->chain, htoff, c->k, c->salt, c->shift, m, P<char>(c->text));
     }
     c->set_text.resize(c->n_sets + 1);
     if (c->n_sets == 1) {  // one set: the whole text
@@ -4129,6 +4403,7 @@ int do_emit(uc_ctx *c)
 // bucketing .. emission on the loaded k-mers, one event per phase boundary
 int timed_phases(uc_ctx *c, uint32_t l)
 {
+    c->held_peak = c->held - c->pool.cap + c->pool.used;  // bytes in use, not the pool's reserve
     for (auto &e : c->ev)
         if (!e) HIPCHK(c, hipEventCreate(&e));
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
@@ -4149,6 +4424,7 @@ int timed_phases(uc_ctx *c, uint32_t l)
     c->timing.emit_ms = t[3];
     HIPCHK(c, hipEventElapsedTime(&c->timing.total_ms, c->ev[0], c->ev[4]));
     c->timing.device_bytes = c->held;
+    c->timing.device_peak_bytes = c->held_peak;
     return prof_flush(c);
 }
 
@@ -4258,7 +4534,8 @@ int uc_ctx_destroy(uc_ctx *c)
     (void)hipStreamSynchronize(c->stream);
     drop_pass_graph(c);
     for (DBuf *b : all_bufs(c))
-        if (b->p) (void)hipFree(b->p);
+        if (b->p && !in_pool(c, b->p)) (void)hipFree(b->p);
+    if (c->pool.base) (void)hipFree(c->pool.base);
     if (c->meter) c->meter->sub(c->held);
     free_pinned(c);
     for (auto &e : c->ev)
@@ -4463,6 +4740,8 @@ int uc_bucket_copy(uc_ctx *c, uint64_t *bucket_rows, uint32_t *slot_src, uint8_t
         CHK(d2h(c, slot_flags, c->flags.p, c->m));
         for (uint64_t i = 0; i < c->m; i++) slot_flags[i] &= (F_LENT | F_BORROW);
     }
+    if ((pair_home || pair_borrow) && c->x && !(c->pair_h.p && c->pair_b.p))
+        return fail(c, UC_E_STATE, "the pairs were freed (UC_FLAG_LOW_MEMORY)");
     if (pair_home) CHK(d2h(c, pair_home, c->pair_h.p, c->x * 4));
     if (pair_borrow) CHK(d2h(c, pair_borrow, c->pair_b.p, c->x * 4));
     return UC_OK;
@@ -4478,8 +4757,10 @@ int uc_find_edges(uc_ctx *c, uint32_t bucket)
 int uc_edge_probes(uc_ctx *c, uint64_t *probes)
 {
     if (!c || !probes) return UC_E_ARG;
-    if (!c->edged) return fail(c, UC_E_STATE, "no edge result");
+    // (after a job, also one whose edge result UC_FLAG_LOW_MEMORY has freed: the counters stay)
+    if (!c->edged && !c->coarsened) return fail(c, UC_E_STATE, "no edge result");
     if (!(c->opts.flags & UC_FLAG_COUNT_PROBES)) return fail(c, UC_E_STATE, "probes are counted with UC_FLAG_COUNT_PROBES");
+    if (!c->probes.p) return fail(c, UC_E_STATE, "no probe counters");
     CHK(select_dev(c));
     uint64_t v[PROBE_SLOTS];
     CHK(d2h(c, v, c->probes.p, sizeof(v)));
@@ -4493,6 +4774,7 @@ int uc_succ_copy(uc_ctx *c, uint32_t *succ)
     if (!c || !succ) return UC_E_ARG;
     if (!c->edged) return fail(c, UC_E_STATE, "no edge result");
     CHK(select_dev(c));
+    CHK(ensure(c, c->hit, c->m * 4));  // (UC_FLAG_LOW_MEMORY frees it after the edge phase)
     if (c->m) KLAUNCH(c, "k_link_succ", k_link_succ, (std::min<uint32_t>(grid_for(c->m), 65536), BLOCK, 0),
                       P<uint32_t>(c->link), P<uint8_t>(c->flags), (uint32_t)c->m, !c->spliced, P<uint32_t>(c->hit));
     return d2h(c, succ, c->hit.p, c->m * 4);
@@ -4611,6 +4893,8 @@ int uc_autotune(uc_ctx *c, uint32_t l, uint32_t reps, uint32_t tune_flags, uc_tu
 {
     if (!c || !out) return UC_E_ARG;
     if (!c->loaded) return fail(c, UC_E_STATE, "uc_autotune before a load");
+    if (c->opts.flags & UC_FLAG_LOW_MEMORY)
+        return fail(c, UC_E_ARG, "uc_autotune reruns the loaded job, which UC_FLAG_LOW_MEMORY consumes");
     CHK(select_dev(c));
     if (reps == 0) reps = 2;
     struct Cand {
@@ -4686,10 +4970,12 @@ int uc_ctx_trim(uc_ctx *c)
     HIPCHK(c, hipStreamSynchronize(c->stream));
     drop_pass_graph(c);
     for (DBuf *b : all_bufs(c)) {
-        if (b->p) HIPCHK(c, hipFree(b->p));
+        if (b->p && !in_pool(c, b->p)) HIPCHK(c, hipFree(b->p));
         b->p = nullptr;
         b->cap = 0;
     }
+    if (c->pool.base) HIPCHK(c, hipFree(c->pool.base));
+    c->pool = Pool{};
     free_pinned(c);
     if (c->meter) c->meter->sub(c->held);
     c->held = 0;
