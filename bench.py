@@ -133,35 +133,85 @@ def workload(args, genome_bp=None):
             f"substitutions, 1% error k-mers; distinct canonical {args.k}-mers")
 
 
-def cpu_baseline(args, l, B, gpu_ctx_kwargs):
-    """Serial restatement of the reference pipeline on a bounded sample."""
+def cpu_threads():
+    """Host threads of the CPU baseline: every core of the GPU's share."""
+    n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(n, int(env))) if env.isdigit() and int(env) > 0 else n
+
+
+def refpipe_leg(oracle, uc, kmers, k, l, B, seed, threads, gpu_text=None, gpu_kwargs=None):
+    """One run of the restated reference pipeline (OpenMP, `threads`) with its
+    parity against the GPU's text of the same k-mers (sorted lines)."""
+    order = oracle.generate_permutation(l, seed)
+    t0 = time.perf_counter()
+    text, st = oracle.refpipe(kmers, k, l, B, order, threads=threads)
+    wall = time.perf_counter() - t0
+    if gpu_text is None:
+        with uc.Compactor(**(gpu_kwargs or {"n_buckets": B})) as c:
+            gpu_text = c.compact(kmers, k, l)
+    parity = uc.sorted_lines(gpu_text) == uc.sorted_lines(text)
+    return wall, st, parity
+
+
+def cpu_baseline(args, l, B, gpu_ctx_kwargs, sets_data=None):
+    """The restated reference pipeline (oracle/refpipe.c) on the host's cores:
+    orc_refpipe_par, OpenMP threads over the minimizers and over whole
+    buckets, timed at the reference's phase boundaries (driver.cpp:443,506-525),
+    on the bench workload's sample, on the 1 Mbp uniform workload and on the
+    256-set batch run set after set; beside it the serial form on the sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     import unitigcoarsening_amd as uc
 
+    threads = cpu_threads()
     sample_bp, seed, k = args.cpu_sample_bp, args.seed, args.k
     kmers, desc = workload(args, sample_bp)
-    order = oracle.generate_permutation(l, seed)
+    wall, st, parity = refpipe_leg(oracle, uc, kmers, k, l, B, seed, threads, gpu_kwargs=gpu_ctx_kwargs)
+    log(f"cpu baseline, {threads} threads: {kmers.shape[0] / wall / 1e6:.2f} M k-mers/s, parity {parity}")
     t0 = time.perf_counter()
-    text, stats = oracle.refpipe(kmers, k, l, B, order)
-    wall = time.perf_counter() - t0
-    with uc.Compactor(**gpu_ctx_kwargs) as c:
-        got = c.compact(kmers, k, l)
-    parity = uc.sorted_lines(got) == uc.sorted_lines(text)
-    par = parallel_cpu_baseline(args, l, B)
-    return {
+    stext, sst = oracle.refpipe(kmers, k, l, B, oracle.generate_permutation(l, seed))
+    swall = time.perf_counter() - t0
+    serial = {"value": kmers.shape[0] / swall, "unit": "k-mers/s", "cores": 1, "wall_s": round(swall, 3),
+              "phases_s": {kk: round(sst[kk], 3) for kk in ("bucketing", "assemble", "coarsen", "write_intra",
+                                                              "minitigs", "cross_finish")}}
+    del stext
+    out = {
         "value": kmers.shape[0] / wall,
         "unit": "k-mers/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": f"{desc}: {kmers.shape[0]} k-mers, l={l}, {B} buckets; oracle/refpipe.c (serial restatement "
-                  f"of the reference pipeline), {wall:.1f} s",
-        "phases_s": {kk: round(stats[kk], 3) for kk in ("bucketing", "assemble", "coarsen", "write_intra",
-                                                          "minitigs", "cross_finish")},
+        "sample": f"{desc}: {kmers.shape[0]} k-mers, l={l}, {B} buckets; oracle/refpipe.c, the restated "
+                  f"reference pipeline on {threads} OpenMP threads (orc_refpipe_par), {wall:.2f} s",
+        "phases_s": {"bucketing": round(st["bucketing"], 3), "buckets": round(st["buckets_wall"], 3),
+                     "cross_finish": round(st["cross_finish"], 3), "total": round(st["total"], 3)},
+        "bucket_thread_s": {kk: round(st[kk], 3) for kk in ("assemble", "coarsen", "write_intra", "minitigs")},
         "parity_with_gpu": parity,
-        "parallel": par,
+        "serial": serial,
         "host": host_info(),
     }
+    if not args.large:
+        # the 1 Mbp uniform workload: 31-mers, l = 11, 16 buckets
+        c1 = uc.synth_kmers(1, 1_000_000, 31)
+        w1, _, p1 = refpipe_leg(oracle, uc, c1, 31, 11, 16, 1, threads)
+        out["uniform_1mbp"] = {"value": c1.shape[0] / w1, "n_kmers": int(c1.shape[0]), "wall_s": round(w1, 3),
+                               "cores": threads, "parity_with_gpu": p1}
+        log(f"cpu baseline 1 Mbp uniform: {c1.shape[0] / w1 / 1e6:.2f} M k-mers/s, parity {p1}")
+    if sets_data is not None:
+        # the 256-set batch, one set after another, against the GPU's batched texts
+        sets, texts = sets_data
+        order = oracle.generate_permutation(l, seed)
+        tot, wall3, same = 0, 0.0, 0
+        for km, gt in zip(sets, texts):
+            t0 = time.perf_counter()
+            text, _ = oracle.refpipe(km, k, l, B, order, threads=threads)
+            wall3 += time.perf_counter() - t0
+            tot += km.shape[0]
+            same += uc.sorted_lines(text) == uc.sorted_lines(gt)
+        out["sets_256"] = {"value": tot / wall3, "n_kmers": tot, "wall_s": round(wall3, 3), "cores": threads,
+                           "parity_with_gpu": f"{same} of {len(sets)} sets"}
+        log(f"cpu baseline 256 sets: {tot / wall3 / 1e6:.2f} M k-mers/s, {same} of {len(sets)} sets match")
+    return out
 
 
 def host_info():
@@ -178,32 +228,6 @@ def host_info():
     return info
 
 
-def parallel_cpu_baseline(args, l, B, workers=16):
-    """The same serial restatement on the host's cores at once: one
-    independent sample of a fifth of the serial sample's genome per worker
-    process (oracle/parallel_baseline.py), k-mers over the wall time from
-    the first start to the last finish."""
-    import subprocess
-
-    workers = min(workers, os.cpu_count() or 1)
-    bp = max(args.cpu_sample_bp // 5, 100_000)
-    model = "large" if args.large else "medium"
-    cmd = [sys.executable, os.path.join(ROOT, "oracle", "parallel_baseline.py"), str(workers), str(bp), str(args.k),
-           str(l), str(B), str(args.seed + 100), model]
-    try:
-        r = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
-        res = json.loads(r.stdout.strip().splitlines()[-1]) if r.returncode == 0 else None
-    except (OSError, subprocess.SubprocessError, ValueError, IndexError):
-        res = None
-    if res is None:
-        return None
-    log(f"cpu baseline, {workers} processes: {res['kmers_per_s'] / 1e6:.2f} M k-mers/s")
-    return {"value": res["kmers_per_s"], "unit": "k-mers/s", "cores": workers, "kind": "port",
-            "sample": f"{workers} independent {model}-model samples of {bp / 1e6:.0f} Mbp (seeds {args.seed + 100}.."
-                      f"{args.seed + 99 + workers}), {res['kmers']} k-mers in all, one serial refpipe.c run per "
-                      f"process, {res['wall_s']:.1f} s from the first start to the last finish"}
-
-
 def sets_workload(n_sets, seed0, k, median=200_000, sigma=1.0, lo=10_000, hi=5_000_000):
     """256 independent sets, sizes ~ LogNormal(ln median, sigma) clipped to
     [lo, hi] k-mers, each from the medium repeat/error model (seeds seed0..)."""
@@ -216,7 +240,7 @@ def sets_workload(n_sets, seed0, k, median=200_000, sigma=1.0, lo=10_000, hi=5_0
     return [uc.medium_model(seed0 + i, int(sz / 0.97) + k, k) for i, sz in enumerate(sizes)]
 
 
-def bench_sets(args):
+def bench_sets(args, keep=None):
     """Batched form (one run over all sets) vs the plain form run set by set,
     both from k-mers already resident in HBM: the sets are loaded once, the
     plain runs take device pointers into that buffer."""
@@ -233,6 +257,7 @@ def bench_sets(args):
     import numpy as np
 
     allk = np.ascontiguousarray(np.concatenate(sets, axis=0))
+    keep_sets = sets if keep is not None else None
     del sets
     tun = {"load_pct": args.load_pct, "sample_log2": args.sample_log2}
     L = uc.lib()
@@ -273,6 +298,8 @@ def bench_sets(args):
                     raise uc.UcError(rc, L.uc_last_error(cp._ctx).decode())
                 equal += cp.text() == batched_text[bounds[i]:bounds[i + 1]]
     log(f"plain per-bucket, set by set: {plain * 1e3:.2f} ms; {equal} of {len(sizes)} set texts byte-equal")
+    if keep is not None:  # the sets and their batched texts, for the CPU baseline's parity
+        keep.append((keep_sets, [batched_text[bounds[i]:bounds[i + 1]] for i in range(len(sizes))]))
     if equal != len(sizes):
         raise RuntimeError(f"batched and plain texts differ on {len(sizes) - equal} sets")
     return {"workload": f"{len(sizes)} sets, log-normal sizes (median 200k, sigma 1, clipped 10k-5M k-mers), "
@@ -378,7 +405,7 @@ def serve_run(args, pool, sizes, n_slots, load, n_jobs, mean_svc, seed):
         order = sorted(pool, key=lambda p: -uc.job_bytes_estimate(p[0].shape[0], p[1]))
         warm = [srv.submit(km, k, l) for km, k, l in order for _ in range(n_slots)]
         for i in warm:
-            srv.result(i)
+            srv.result(i, text=False)
         ids = []
         waiter_err = []
 
@@ -388,7 +415,9 @@ def serve_run(args, pool, sizes, n_slots, load, n_jobs, mean_svc, seed):
                 while len(ids) <= i:
                     time.sleep(0.0002)
                 try:
-                    text, qms, rms = srv.result(ids[i])
+                    # the text stays in the server: this measures the server, not
+                    # the client's copy of each result
+                    _, qms, rms = srv.result(ids[i], text=False)
                     lat.append(qms + rms)
                 except Exception as e:  # reported after the run
                     waiter_err.append(repr(e))
@@ -434,23 +463,40 @@ def bench_serve(args):
     sizes = np.array([p[0].shape[0] for p in pool])
     log(f"serve pool: {len(pool)} inputs, {sizes.sum()} k-mers, sizes {sizes.min()}..{sizes.max()} "
         f"in {time.time() - t:.1f}s")
-    # one context, jobs back to back, through the same host boundary as the
-    # server (input copied in, text copied out): service time per input
+    # one context, jobs back to back, through the same host boundary as a
+    # server worker (the input from host memory, the text copied out to host
+    # memory): service time per input
+    import ctypes as C
+
     svc = []
+    L = uc.lib()
+    out = np.empty(1, np.uint8)
     with uc.Compactor(n_buckets=args.buckets, sample_log2=args.sample_log2) as c:
+        def one(km, k, l):
+            nonlocal out
+            km = np.ascontiguousarray(km)
+            if L.uc_compact(c._ctx, km.ctypes.data, km.shape[0], k, l, 0) != 0:
+                raise uc.UcError(-1, L.uc_last_error(c._ctx).decode())
+            n = L.uc_text_size(c._ctx)
+            if out.size < n:
+                out = np.empty(n, np.uint8)
+            L.uc_text_copy(c._ctx, out.ctypes.data_as(C.c_char_p), n)
         for km, k, l in pool:
-            c.compact(km, k, l)
+            one(km, k, l)
         for km, k, l in pool:
             t0 = time.perf_counter()
-            c.compact(km, k, l)
+            one(km, k, l)
             svc.append(time.perf_counter() - t0)
     mean_svc = float(np.mean(svc))
     main_run = serve_run(args, pool, sizes, args.serve_slots, args.serve_load, args.serve_jobs, mean_svc, 6)
     sat = None
     if args.serve_saturation_jobs:
-        sat = {f"slots_{ns}": serve_run(args, pool, sizes, ns, 1.5, args.serve_saturation_jobs, mean_svc, 7)
-               for ns in (1, args.serve_slots)}
-        sat["slots_ratio"] = round(sat[f"slots_{args.serve_slots}"]["kmers_per_s"] / sat["slots_1"]["kmers_per_s"], 3)
+        sat = {}
+        for load in (1.5, 3.0):
+            pt = {f"slots_{ns}": serve_run(args, pool, sizes, ns, load, args.serve_saturation_jobs, mean_svc, 7)
+                  for ns in (1, args.serve_slots)}
+            pt["slots_ratio"] = round(pt[f"slots_{args.serve_slots}"]["kmers_per_s"] / pt["slots_1"]["kmers_per_s"], 3)
+            sat[f"load_{load:g}x"] = pt
     return {"workload": f"{args.serve_jobs} jobs, Poisson arrivals at {args.serve_load:.2f} x one-context capacity, "
                         f"drawn from {len(pool)} inputs of log-uniform 100k-50M k-mers (medium model, k drawn from "
                         f"{{31, 63}}, seeds 5000..{5000 + len(pool) - 1}), {args.serve_slots} worker slots, budget "
@@ -459,9 +505,10 @@ def bench_serve(args):
             "mean_service_ms_one_context": round(1e3 * mean_svc, 3),
             **{kk: v for kk, v in main_run.items() if kk not in ("n_slots",)},
             "saturation": sat,
-            "note": "latency is submit to result and includes the host copies of the input and of the result "
-                    "text; the saturation point offers 1.5x one context's capacity to one worker slot and to "
-                    f"{args.serve_slots}: served k-mers/s above the one-slot figure is work overlapped on the device"}
+            "note": "latency is submit to finish and includes the server's host copies of the input and of the "
+                    "result text (the client does not fetch the texts here); the saturation points offer 1.5x and 3x "
+                    f"one context's capacity to one worker slot and to {args.serve_slots}: served k-mers/s above the "
+                    "one-slot figure is work overlapped across the workers (device, copies and host threads)"}
 
 
 def main():
@@ -660,7 +707,8 @@ def main():
         plain = {"value": n / pw, "ms_per_step": round(1e3 * pw, 3), "batched_over_plain": round(pw / (wall / args.steps), 3)}
         log(f"plain form: {1e3 * pw:.2f} ms/step")
 
-    sets = None if args.no_sets else bench_sets(args)
+    sets_keep = [] if not args.no_cpu_baseline else None
+    sets = None if args.no_sets else bench_sets(args, sets_keep)
     serve = None if args.no_serve else bench_serve(args)
     f2f = None if args.no_file else bench_file(args, kmers, l, B)
     if f2f:
@@ -673,7 +721,7 @@ def main():
     base = None
     if not args.no_cpu_baseline:
         log("cpu baseline (serial restatement of the reference pipeline) ...")
-        base = cpu_baseline(args, l, B, {"n_buckets": B})
+        base = cpu_baseline(args, l, B, {"n_buckets": B}, sets_keep[0] if sets_keep else None)
         log(f"cpu baseline: {base['value'] / 1e6:.3f} M k-mers/s, parity with GPU: {base['parity_with_gpu']}")
 
     out = {

@@ -51,6 +51,8 @@ class OrcStats(C.Structure):
         ("n_cross", C.c_uint64),
         ("cross_offset", C.c_uint64),
         ("n_lines", C.c_uint64),
+        ("buckets_wall", C.c_double),
+        ("threads", C.c_uint64),
     ]
 
 
@@ -80,6 +82,9 @@ def lib():
         L.orc_refpipe.restype = C.c_int
         L.orc_refpipe.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
                                   C.POINTER(C.c_char_p), _U64P, C.POINTER(OrcStats)]
+        L.orc_refpipe_par.restype = C.c_int
+        L.orc_refpipe_par.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
+                                      C.c_int, C.POINTER(C.c_char_p), _U64P, C.POINTER(OrcStats)]
         L.orc_bucketize.restype = C.c_int
         L.orc_bucketize.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
                                     C.POINTER(OrcBuckets)]
@@ -188,16 +193,20 @@ def bucketize(packed, k, l, n_buckets, order):
     return res
 
 
-def refpipe(packed, k, l, n_buckets, order):
-    """-> (text bytes, stats dict)"""
+def refpipe(packed, k, l, n_buckets, order, threads=1):
+    """-> (text bytes, stats dict); threads > 1 runs orc_refpipe_par (same bytes)"""
     L = lib()
     a = _arr(packed)
     o = _arr(order)
     t = C.c_char_p()
     tl = C.c_uint64()
     st = OrcStats()
-    rc = L.orc_refpipe(a.ctypes.data, a.shape[0], k, l, n_buckets, o.ctypes.data, C.byref(t), C.byref(tl),
-                       C.byref(st))
+    if threads > 1:
+        rc = L.orc_refpipe_par(a.ctypes.data, a.shape[0], k, l, n_buckets, o.ctypes.data, threads, C.byref(t),
+                               C.byref(tl), C.byref(st))
+    else:
+        rc = L.orc_refpipe(a.ctypes.data, a.shape[0], k, l, n_buckets, o.ctypes.data, C.byref(t), C.byref(tl),
+                           C.byref(st))
     if rc:
         raise OracleError(rc)
     stats = {f: getattr(st, f) for f, _ in OrcStats._fields_}

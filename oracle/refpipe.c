@@ -194,8 +194,20 @@ void orc_buckets_free(orc_buckets *b)
     memset(b, 0, sizeof(*b));
 }
 
+static int bucketize_nt(const uint32_t *packed, uint64_t n, uint32_t k, uint32_t l, uint32_t n_buckets,
+                        const uint32_t *order, orc_buckets *out, int nt);
+
 int orc_bucketize(const uint32_t *packed, uint64_t n, uint32_t k, uint32_t l, uint32_t n_buckets,
                   const uint32_t *order, orc_buckets *out)
+{
+    return bucketize_nt(packed, n, k, l, n_buckets, order, out, 1);
+}
+
+/* nt threads compute the minimizers (the Kokkos parallel_for of "calc
+ * lmins"); counting and partitioning stay in k-mer order, so the layout is
+ * the serial one whatever nt is */
+static int bucketize_nt(const uint32_t *packed, uint64_t n, uint32_t k, uint32_t l, uint32_t n_buckets,
+                        const uint32_t *order, orc_buckets *out, int nt)
 {
     memset(out, 0, sizeof(*out));
     if (k < 2 || k > 64 || l < 1 || l > 15 || l >= k || n_buckets < 1 || (n_buckets & (n_buckets - 1)))
@@ -213,6 +225,7 @@ int orc_bucketize(const uint32_t *packed, uint64_t n, uint32_t k, uint32_t l, ui
     if (!out_l || !in_l || !small_cnt || !small_off || !cross_cnt || !out->bucket_rows || !out->cross_rows) goto fail;
     out->n_buckets = n_buckets;
     /* "calc lmins" (:299) */
+#pragma omp parallel for num_threads(nt) schedule(static)
     for (uint64_t i = 0; i < n; i++) orc_double_lmin(packed + i * w, k, l, order, &out_l[i], &in_l[i]);
     /* "count lmins" (:306-318) */
     for (uint64_t i = 0; i < n; i++) {
@@ -903,14 +916,29 @@ static int write_stitched_row(sbuf_t *sb, const minitigs_t *all, const ord_t *en
 int orc_refpipe(const uint32_t *packed, uint64_t n, uint32_t k, uint32_t l, uint32_t n_buckets,
                 const uint32_t *order, char **text, uint64_t *text_len, orc_refpipe_stats *st)
 {
+    return orc_refpipe_par(packed, n, k, l, n_buckets, order, 1, text, text_len, st);
+}
+
+/* The reference's Kokkos OpenMP build runs every bucket's hashmap, graph,
+ * coarsening and output on all host threads, bucket after bucket
+ * (driver.cpp:479-521).  Here the threads take whole buckets instead
+ * (buckets are independent until the cross graph): each bucket numbers its
+ * finished crossing paths from 0 and writes its own text, and a prefix over
+ * the buckets then gives the ids the serial loop's running offset gives, so
+ * the text is the same bytes for every thread count. */
+int orc_refpipe_par(const uint32_t *packed, uint64_t n, uint32_t k, uint32_t l, uint32_t n_buckets,
+                    const uint32_t *order, int nthreads, char **text, uint64_t *text_len, orc_refpipe_stats *st)
+{
     if (k % 16 != 15 || k > 63 || l < 1 || l >= k) return ORC_E_ARG;
+    const int nt = nthreads > 0 ? nthreads : 1;
     orc_refpipe_stats stats;
     memset(&stats, 0, sizeof(stats));
+    stats.threads = (uint64_t)nt;
     const uint32_t comp = orc_words_per_kmer(k);
     const uint64_t B = n_buckets;
     double t0 = now_s(), t = t0;
     orc_buckets bk;
-    int rc = orc_bucketize(packed, n, k, l, n_buckets, order, &bk);
+    int rc = bucketize_nt(packed, n, k, l, n_buckets, order, &bk, nt);
     if (rc) return rc;
     stats.bucketing = now_s() - t;
     stats.n_kmers = n;
@@ -921,43 +949,90 @@ int orc_refpipe(const uint32_t *packed, uint64_t n, uint32_t k, uint32_t l, uint
     uint64_t largest = 0;
     for (uint64_t i = 0; i < B; i++)
         if (bk.bucket_rows[i + 1] - bk.bucket_rows[i] > largest) largest = bk.bucket_rows[i + 1] - bk.bucket_rows[i];
-    uint64_t map_size = hashmap_size(largest);
-    ord_t *map = (ord_t *)malloc(map_size * sizeof(ord_t));
-    uint32_t *kb = (uint32_t *)malloc((largest ? largest : 1) * comp * sizeof(uint32_t));
+    const uint64_t map_size = hashmap_size(largest);
     crosses_t *cross_list = (crosses_t *)calloc(B, sizeof(crosses_t));
     minitigs_t *mini = (minitigs_t *)calloc(B, sizeof(minitigs_t));
+    sbuf_t *btext = (sbuf_t *)calloc(B, sizeof(sbuf_t));
+    ord_t *bcross = (ord_t *)calloc(B + 1, sizeof(ord_t)); /* finished crossing paths per bucket */
+    uint64_t *blines = (uint64_t *)calloc(B, sizeof(uint64_t));
+    double tph[4] = {0, 0, 0, 0}; /* thread-seconds: assemble, coarsen, write, minitigs */
     ord_t cross_offset = 0;
-    if (!map || !kb || !cross_list || !mini) {
+    if (!cross_list || !mini || !btext || !bcross || !blines) {
         rc = ORC_E_NOMEM;
         goto out;
     }
+    t = now_s();
+#pragma omp parallel num_threads(nt) reduction(+ : tph[:4])
+    {
+        /* the hashmap and the bucket's k-mers: per thread, sized for the
+         * largest bucket as driver.cpp:454-466 sizes them once */
+        ord_t *map = (ord_t *)malloc(map_size * sizeof(ord_t));
+        uint32_t *kb = (uint32_t *)malloc((largest ? largest : 1) * comp * sizeof(uint32_t));
+        int trc = (!map || !kb) ? ORC_E_NOMEM : ORC_OK;
+#pragma omp for schedule(dynamic, 1)
+        for (uint64_t b = 0; b < B; b++) {
+            if (trc) continue;
+            /* driver.cpp:479-521: the per-bucket loop */
+            uint64_t s = bk.bucket_rows[b], cnt = bk.bucket_rows[b + 1] - s;
+            for (uint64_t i = 0; i < cnt; i++)
+                memcpy(kb + i * comp, packed + bk.kmer_src[s + i] * comp, comp * sizeof(uint32_t));
+            uint64_t cs = bk.cross_rows[B * b], ce = bk.cross_rows[B * (b + 1)];
+            double tt = now_s();
+            generate_hashmap(map, map_size, kb, comp, (ord_t)cnt);
+            ord_t *g = (ord_t *)malloc((cnt ? cnt : 1) * sizeof(ord_t));
+            if (!g) { trc = ORC_E_NOMEM; continue; }
+            for (uint64_t i = 0; i < cnt; i++) g[i] = ORD_MAX;
+            trc = assemble_pruned_graph(kb, comp, (ord_t)cnt, map, map_size, bk.cross_lend + cs, bk.cross_borrow + cs,
+                                        ce - cs, g, &cross_list[b]);
+            tph[0] += now_s() - tt;
+            if (trc) { free(g); continue; }
+            tt = now_s();
+            crs_t glue, cross;
+            ord_t local = 0;
+            trc = coarsen_full_cycle(g, (ord_t)cnt, &cross_list[b], &local, &glue, &cross);
+            tph[1] += now_s() - tt;
+            if (trc) continue;
+            bcross[b] = local;
+            tt = now_s();
+            trc = write_intra(&btext[b], kb, k, comp, &glue, &blines[b]);
+            tph[2] += now_s() - tt;
+            tt = now_s();
+            if (!trc) trc = generate_minitigs(&cross, kb, k, comp, &mini[b]);
+            tph[3] += now_s() - tt;
+            crs_free(&glue);
+            crs_free(&cross);
+        }
+        free(map);
+        free(kb);
+        if (trc) {
+#pragma omp critical
+            rc = trc;
+        }
+    }
+    stats.buckets_wall = now_s() - t;
+    stats.assemble = tph[0];
+    stats.coarsen = tph[1];
+    stats.write_intra = tph[2];
+    stats.minitigs = tph[3];
+    if (rc) goto out;
+    /* crossing ids: bucket b's follow every earlier bucket's; the texts go
+     * out bucket by bucket */
     for (uint64_t b = 0; b < B && !rc; b++) {
-        /* driver.cpp:479-521: the per-bucket loop */
-        uint64_t s = bk.bucket_rows[b], cnt = bk.bucket_rows[b + 1] - s;
-        for (uint64_t i = 0; i < cnt; i++) memcpy(kb + i * comp, packed + bk.kmer_src[s + i] * comp, comp * sizeof(uint32_t));
-        uint64_t cs = bk.cross_rows[B * b], ce = bk.cross_rows[B * (b + 1)];
-        t = now_s();
-        generate_hashmap(map, map_size, kb, comp, (ord_t)cnt);
-        ord_t *g = (ord_t *)malloc((cnt ? cnt : 1) * sizeof(ord_t));
-        if (!g) { rc = ORC_E_NOMEM; break; }
-        for (uint64_t i = 0; i < cnt; i++) g[i] = ORD_MAX;
-        rc = assemble_pruned_graph(kb, comp, (ord_t)cnt, map, map_size, bk.cross_lend + cs, bk.cross_borrow + cs, ce - cs,
-                                   g, &cross_list[b]);
-        stats.assemble += now_s() - t;
-        if (rc) { free(g); break; }
-        t = now_s();
-        crs_t glue, cross;
-        rc = coarsen_full_cycle(g, (ord_t)cnt, &cross_list[b], &cross_offset, &glue, &cross);
-        stats.coarsen += now_s() - t;
-        if (rc) break;
-        t = now_s();
-        rc = write_intra(&sb, kb, k, comp, &glue, &stats.n_lines);
-        stats.write_intra += now_s() - t;
-        t = now_s();
-        if (!rc) rc = generate_minitigs(&cross, kb, k, comp, &mini[b]);
-        stats.minitigs += now_s() - t;
-        crs_free(&glue);
-        crs_free(&cross);
+        const ord_t off = cross_offset;
+        if (off)
+            for (uint64_t i = 0; i < cross_list[b].n; i++) {
+                if (cross_list[b].in[i] != ORD_MAX) cross_list[b].in[i] += off;
+                if (cross_list[b].out[i] != ORD_MAX) cross_list[b].out[i] += off;
+            }
+        cross_offset += bcross[b];
+        stats.n_lines += blines[b];
+        if (btext[b].len) {
+            if (sb_reserve(&sb, btext[b].len)) rc = ORC_E_NOMEM;
+            else {
+                memcpy(sb.buf + sb.len, btext[b].buf, btext[b].len);
+                sb.len += btext[b].len;
+            }
+        }
     }
     if (rc) goto out;
     t = now_s();
@@ -1032,10 +1107,13 @@ out:
             free(mini[b].rows);
             free(mini[b].chars);
         }
+    if (btext)
+        for (uint64_t b = 0; b < B; b++) free(btext[b].buf);
+    free(btext);
+    free(bcross);
+    free(blines);
     free(cross_list);
     free(mini);
-    free(map);
-    free(kb);
     orc_buckets_free(&bk);
     stats.total = now_s() - t0;
     if (st) *st = stats;

@@ -120,6 +120,10 @@ void orc_buckets_free(orc_buckets *b);
 typedef struct {
     double bucketing, assemble, coarsen, write_intra, minitigs, cross_finish, total;
     uint64_t n_kmers, m_placements, n_cross, cross_offset, n_lines;
+    /* orc_refpipe_par: the per-bucket loop's wall time (assemble .. minitigs
+     * are then thread-seconds summed over the threads) and the thread count */
+    double buckets_wall;
+    uint64_t threads;
 } orc_refpipe_stats;
 
 /* Whole reference pipeline on packed k-mers: returns the unitig text in the
@@ -128,6 +132,11 @@ typedef struct {
  * order: 4^l l-mer orders (e.g. from orc_generate_permutation). */
 int orc_refpipe(const uint32_t *packed, uint64_t n, uint32_t k, uint32_t l, uint32_t n_buckets,
                 const uint32_t *order, char **text, uint64_t *text_len, orc_refpipe_stats *stats);
+
+/* The same pipeline on nthreads OpenMP threads: minimizers in parallel,
+ * whole buckets per thread; the same text bytes as orc_refpipe. */
+int orc_refpipe_par(const uint32_t *packed, uint64_t n, uint32_t k, uint32_t l, uint32_t n_buckets,
+                    const uint32_t *order, int nthreads, char **text, uint64_t *text_len, orc_refpipe_stats *stats);
 
 /* Restated pieces exposed for unit tests (assembler.h:7-45). */
 uint32_t orc_fnv_pref(const uint32_t *kmer, uint32_t words);

@@ -148,17 +148,15 @@ def test_packed_file_roundtrip(tmp_path):
     assert e.value.status == uc.UC_E_IO
 
 
-def test_parallel_cpu_baseline_runs_independent_samples():
-    # the multi-process CPU baseline: every worker's k-mers are counted and
-    # the rate is their sum over the common wall time
-    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-    r = subprocess.run([sys.executable, os.path.join(root, "oracle", "parallel_baseline.py"), "2", "100000", "31", "11",
-                        "16", "5", "medium"], capture_output=True, text=True, timeout=300)
-    assert r.returncode == 0, r.stderr
-    res = json.loads(r.stdout.strip().splitlines()[-1])
-    want = sum(uc.medium_model(5 + i, 100_000, 31).shape[0] for i in range(2))
-    assert res["workers"] == 2 and res["kmers"] == want
-    assert res["kmers_per_s"] == pytest.approx(res["kmers"] / res["wall_s"])
+def test_cpu_threads_follow_the_host_share(monkeypatch):
+    # the CPU baseline runs on every core of the GPU's share (OMP_NUM_THREADS
+    # when the host sets it, else every core)
+    import bench
+
+    monkeypatch.setenv("OMP_NUM_THREADS", "3")
+    assert bench.cpu_threads() == min(3, os.cpu_count())
+    monkeypatch.delenv("OMP_NUM_THREADS")
+    assert bench.cpu_threads() == os.cpu_count()
 
 
 def test_bench_refuses_more_than_one_gpu():

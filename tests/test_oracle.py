@@ -413,3 +413,18 @@ def test_bucketize_invariants():
                     assert hk == bk and home[hk] == i and suf[hk] == j
                     pairs += 1
     assert pairs == int((home != suf).sum())
+
+
+@pytest.mark.parametrize("k,l,B", [(31, 11, 64), (63, 13, 16), (15, 7, 4)])
+def test_refpipe_threads_same_bytes(k, l, B):
+    # the OpenMP form (minimizers in parallel, whole buckets per thread, crossing
+    # ids shifted by a prefix over the buckets) writes the serial form's bytes
+    kmers = uc.medium_model(k * 3 + B, 120_000, k)
+    order = oracle.generate_permutation(l, 3)
+    want, st1 = oracle.refpipe(kmers, k, l, B, order)
+    for threads in (2, 5, 8):
+        got, st = oracle.refpipe(kmers, k, l, B, order, threads=threads)
+        assert got == want, threads
+        assert st["threads"] == threads and st["n_lines"] == st1["n_lines"]
+        assert st["cross_offset"] == st1["cross_offset"]
+    assert uc.sorted_lines(want) == uc.sorted_lines(oracle.textbook(kmers, k)[0])

@@ -530,9 +530,9 @@ class Compactor:
     def text(self) -> bytes:
         L = lib()
         n = L.uc_text_size(self._ctx)
-        buf = C.create_string_buffer(max(n, 1))
-        _check(L.uc_text_copy(self._ctx, buf, n), self._ctx)
-        return buf.raw[:n]
+        buf = np.empty(max(n, 1), np.uint8)
+        _check(L.uc_text_copy(self._ctx, buf.ctypes.data_as(C.c_char_p), n), self._ctx)
+        return buf[:n].tobytes()
 
     def text_size(self):
         return lib().uc_text_size(self._ctx)
@@ -631,18 +631,22 @@ class Server:
         _check(lib().uc_poll(self._srv, job, C.byref(st)))
         return st.value
 
-    def result(self, job, release=True):
+    def result(self, job, release=True, text=True):
+        """(text, queued ms, running ms) of a finished job; with text=False the
+        text stays in the server (only its size is returned in its place)."""
         L = lib()
         L.uc_wait(self._srv, job)
         size, qms, rms = C.c_uint64(), C.c_double(), C.c_double()
         rc = L.uc_result(self._srv, job, None, 0, C.byref(size), C.byref(qms), C.byref(rms))
-        if rc == UC_OK:
-            buf = C.create_string_buffer(max(size.value, 1))
-            rc = L.uc_result(self._srv, job, buf, size.value, None, None, None)
+        out = size.value
+        if rc == UC_OK and text:
+            buf = np.empty(max(size.value, 1), np.uint8)
+            rc = L.uc_result(self._srv, job, buf.ctypes.data_as(C.c_char_p), size.value, None, None, None)
+            out = buf[:size.value].tobytes()
         if release:
             L.uc_release(self._srv, job)
         _check(rc)
-        return buf.raw[:size.value], qms.value, rms.value
+        return out, qms.value, rms.value
 
     def stats(self):
         peak, done = C.c_uint64(), C.c_uint64()

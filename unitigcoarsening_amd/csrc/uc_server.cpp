@@ -8,6 +8,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -28,7 +29,8 @@ struct Job {
     uint64_t estimate = 0;
     int state = UC_JOB_QUEUED;
     int status = UC_OK;
-    std::vector<char> text;
+    std::unique_ptr<char[]> text;  // (not zero-filled first: the copy writes every byte)
+    uint64_t text_size = 0;
     Clock::time_point submitted, started, finished;
 };
 
@@ -135,15 +137,18 @@ struct uc_server {
 
             if (how == 2) uc_ctx_trim(w.ctx);
             int rc = uc_compact(w.ctx, j->packed.data(), j->n, j->k, j->l, 0);
-            std::vector<char> text;
+            std::unique_ptr<char[]> text;
+            uint64_t text_size = 0;
             if (rc == UC_OK) {
-                text.resize(uc_text_size(w.ctx));
-                rc = uc_text_copy(w.ctx, text.data(), text.size());
+                text_size = uc_text_size(w.ctx);
+                text.reset(new (std::nothrow) char[text_size ? text_size : 1]);
+                rc = text ? uc_text_copy(w.ctx, text.get(), text_size) : UC_E_NOMEM;
             }
             const uint64_t held_now = uc_ctx_held_bytes(w.ctx);
 
             lk.lock();
             j->text.swap(text);
+            j->text_size = rc == UC_OK ? text_size : 0;
             j->status = rc;
             j->state = rc == UC_OK ? UC_JOB_DONE : UC_JOB_FAILED;
             j->finished = Clock::now();
@@ -258,11 +263,11 @@ int uc_result(uc_server *s, uint64_t job, char *dst, uint64_t cap, uint64_t *siz
     if (it == s->jobs.end()) return UC_E_ARG;
     Job *j = it->second;
     if (j->state != UC_JOB_DONE && j->state != UC_JOB_FAILED) return UC_E_STATE;
-    if (size) *size = j->text.size();
+    if (size) *size = j->text_size;
     if (queue_ms) *queue_ms = std::chrono::duration<double, std::milli>(j->started - j->submitted).count();
     if (run_ms) *run_ms = std::chrono::duration<double, std::milli>(j->finished - j->started).count();
     if (j->status != UC_OK) return j->status;
-    if (dst && cap >= j->text.size()) std::memcpy(dst, j->text.data(), j->text.size());
+    if (dst && cap >= j->text_size && j->text_size) std::memcpy(dst, j->text.get(), j->text_size);
     return UC_OK;
 }
 
