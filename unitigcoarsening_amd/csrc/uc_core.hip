@@ -1141,11 +1141,18 @@ struct RunOut {
 // run record (8 bytes): next run's first k-mer | (length - 1) << 32 (12 bits,
 // a run never leaves its LDS group of at most 4096 slots) | (position of its
 // characters in rch minus its own slot) << 44 (13 bits, signed: both lie in
-// the group)
-__host__ __device__ __forceinline__ uint64_t make_rec(uint32_t next, uint32_t len, uint32_t slot, uint32_t pos)
+// the group) | its first REC_INLINE characters as 2-bit codes << 57.  Runs of
+// at most REC_INLINE k-mers (most: runs end where the minimizer changes) are
+// spelled from the record alone, so following a unitig into another bucket
+// reads one random line per run instead of two.
+constexpr uint32_t REC_INLINE = 3;
+__host__ __device__ __forceinline__ uint64_t make_rec(uint32_t next, uint32_t len, uint32_t slot, uint32_t pos,
+                                                      uint32_t chars)
 {
-    return (uint64_t)next | ((uint64_t)((len - 1) & 0xFFFu) << 32) | ((uint64_t)((pos - slot) & 0x1FFFu) << 44);
+    return (uint64_t)next | ((uint64_t)((len - 1) & 0xFFFu) << 32) | ((uint64_t)((pos - slot) & 0x1FFFu) << 44) |
+           ((uint64_t)(chars & 0x3Fu) << 57);
 }
+__device__ __forceinline__ uint32_t rec_chars(uint64_t r) { return (uint32_t)(r >> 57) & 0x3Fu; }
 __device__ __forceinline__ uint32_t rec_next(uint64_t r) { return (uint32_t)r; }
 __device__ __forceinline__ uint32_t rec_len(uint64_t r) { return (uint32_t)((r >> 32) & 0xFFFu) + 1u; }
 __device__ __forceinline__ uint32_t rec_pos(uint64_t r, uint32_t slot)
@@ -1330,9 +1337,10 @@ __global__ __launch_bounds__(BLOCK) void k_edge_resolve(const ETile *__restrict_
         link[s] = (f & F_LENT) ? s : l;
         if (!(f & F_BORROW)) {
             ro.flags[s] = (uint8_t)(f | F_RUNHEAD);
-            ro.rec[s] = make_rec((uint32_t)l, 1u, s, s);
             const uint64_t word = kmer[(uint64_t)s * W + (k - 1) / 32];
-            ro.rch[s] = base_char((uint32_t)(word >> (2 * ((k - 1) & 31))) & 3u);
+            const uint32_t c = (uint32_t)(word >> (2 * ((k - 1) & 31))) & 3u;
+            ro.rec[s] = make_rec((uint32_t)l, 1u, s, s, c);
+            ro.rch[s] = base_char(c);
         }
     }
 }
@@ -1592,6 +1600,9 @@ __global__ __launch_bounds__(EB) void k_edge_lds(const uint64_t *__restrict__ km
     uint32_t *pd = tab;
     uint16_t *succ = reinterpret_cast<uint16_t *>(tab + nb);
     uint32_t *sc = reinterpret_cast<uint32_t *>(km);
+    // every slot's last character (2-bit code), for the run records: in the
+    // k-mer area past sc (8 W nb bytes hold 4 nb + nb)
+    uint8_t *cc = reinterpret_cast<uint8_t *>(km) + 4 * (size_t)nb;
 #pragma unroll
     for (int e = 0; e < EPT; e++) {
         const uint32_t i = threadIdx.x + (uint32_t)e * EB;
@@ -1599,6 +1610,7 @@ __global__ __launch_bounds__(EB) void k_edge_lds(const uint64_t *__restrict__ km
             succ[i] = tgt[e];
             pd[i] = LDS_EMPTY;
             sc[i] = 0;
+            cc[i] = (uint8_t)((lc >> (2 * e)) & 3u);
         }
     }
     __syncthreads();
@@ -1644,8 +1656,16 @@ __global__ __launch_bounds__(EB) void k_edge_lds(const uint64_t *__restrict__ km
         const uint32_t a = cyc ? i : (x & 0xFFFFu), d = cyc ? 0u : (x >> 16);
         ro.rch[gr.begin + sc[a] + d] = base_char((lc >> (2 * e)) & 3u);
         ro.flags[g] = (uint8_t)(f | (a == i ? F_RUNHEAD : 0));
-        if (a == i)
-            ro.rec[g] = make_rec(cyc ? gr.begin + tgt[e] : NONE, cyc ? 1u : (uint32_t)rl[e], g, gr.begin + sc[i]);
+        if (a == i) {
+            const uint32_t len = cyc ? 1u : (uint32_t)rl[e];
+            uint32_t ch = cc[i];
+            if (len >= 2) {
+                const uint32_t v1 = succ[i];
+                ch |= (uint32_t)cc[v1] << 2;
+                if (len >= 3) ch |= (uint32_t)cc[succ[v1]] << 4;
+            }
+            ro.rec[g] = make_rec(cyc ? gr.begin + tgt[e] : NONE, len, g, gr.begin + sc[i], ch);
+        }
         if (f & F_LENT) link[g] = gr.begin + a;  // its run head, until the splice
     }
     if (CNT) add_probes<EB>(probes, np, pred_);
@@ -1742,7 +1762,7 @@ __global__ __launch_bounds__(1024) void k_edge_slice(const uint64_t *__restrict_
             if (fastrange(h, t.P) != t.p) continue;
             link[s] = s;  // its own run head, until the splice
             ro.flags[s] = (uint8_t)(f | F_RUNHEAD);
-            ro.rec[s] = make_rec(NONE, 1u, s, s);
+            ro.rec[s] = make_rec(NONE, 1u, s, s, char_at<W>(x, k - 1));
             ro.rch[s] = base_char(char_at<W>(x, k - 1));
             continue;
         }
@@ -1762,7 +1782,7 @@ __global__ __launch_bounds__(1024) void k_edge_slice(const uint64_t *__restrict_
         link[s] = l;
         if (!(f & F_BORROW)) {
             ro.flags[s] = (uint8_t)(f | F_RUNHEAD);
-            ro.rec[s] = make_rec((uint32_t)l, 1u, s, s);
+            ro.rec[s] = make_rec((uint32_t)l, 1u, s, s, char_at<W>(x, k - 1));
             ro.rch[s] = base_char(char_at<W>(x, k - 1));
         }
     }
@@ -2052,7 +2072,10 @@ __global__ void k_cycle_init(const uint32_t *__restrict__ link, uint8_t *__restr
     for (uint32_t j = 0; j < len && u != NONE; j++) {
         const uint32_t l = link[u];
         flags[u] |= F_CYCLE | F_RUNHEAD;
-        ro.rec[u] = make_rec((uint32_t)l, 1u, u, pos + j);
+        // the member's character: from the record, or its byte in rch ('A' 'C' 'G' 'T' -> 0..3)
+        const uint32_t ch = len <= REC_INLINE ? (rec_chars(r) >> (2 * j)) & 3u
+                                              : (((uint32_t)ro.rch[pos + j] >> 1) ^ ((uint32_t)ro.rch[pos + j] >> 2)) & 3u;
+        ro.rec[u] = make_rec((uint32_t)l, 1u, u, pos + j, ch);
         lab[u] = (uint64_t)u | ((uint64_t)(uint32_t)l << 32);
         list[atomicAdd(n_list, 1u)] = u;
         u = (uint32_t)l;
@@ -2136,10 +2159,11 @@ __device__ __forceinline__ void wave_sync()
 // the low bits), first character in the low byte
 __device__ __forceinline__ uint32_t chars4(uint32_t b)
 {
-    uint32_t r = 0;
-#pragma unroll
-    for (int q = 0; q < 4; q++) r |= (uint32_t)(uint8_t)base_char((b >> (2 * q)) & 3u) << (8 * q);
-    return r;
+    // code q to byte q (b << 6q puts bits 2q..2q+1 at 8q..8q+1), then one
+    // byte permute picks byte `code` of "ACGT" for each
+    b &= 0xFFu;  // (the caller's higher bits are later characters)
+    const uint32_t sel = (b | (b << 6) | (b << 12) | (b << 18)) & 0x03030303u;
+    return __builtin_amdgcn_perm(0u, 0x54474341u, sel);
 }
 
 // characters i .. i + 3 of x (i + 3 < 32 W)
@@ -2195,6 +2219,19 @@ __device__ __forceinline__ void copy_chars(char *o, const char *__restrict__ rch
 // one head's segment at o (address `at` modulo 4): first k-mer x, the later
 // characters of its run (record r), the characters of every later run up to
 // the next sampled splitter, '\n' when the unitig ends there
+// characters from .. len - 1 of the run with record r at slot s: from the
+// record when the run is short, else from rch
+__device__ __forceinline__ void put_run(char *o, uint64_t r, const char *__restrict__ rch, uint32_t s, uint32_t from,
+                                        uint32_t len)
+{
+    if (len <= REC_INLINE) {
+        const uint32_t c = rec_chars(r);
+        for (uint32_t j = from; j < len; j++) o[j - from] = base_char((c >> (2 * j)) & 3u);
+    } else {
+        copy_chars(o, rch, rec_pos(r, s) + from, len - from);
+    }
+}
+
 template <int W>
 __device__ __forceinline__ void put_head(char *o, uint64_t at, const Kmer<W> &x, uint64_t r,
                                          const uint64_t *__restrict__ rec, const char *__restrict__ rch, uint32_t s,
@@ -2202,13 +2239,13 @@ __device__ __forceinline__ void put_head(char *o, uint64_t at, const Kmer<W> &x,
 {
     put_kmer<W>(o, at, x, k);
     uint32_t len = rec_len(r), p = k;
-    copy_chars(o + p, rch, rec_pos(r, s) + 1, len - 1);  // the head's own character is the k-th above
+    put_run(o + p, r, rch, s, 1, len);  // the head's own character is the k-th above
     p += len - 1;
     uint32_t v = rec_next(r), steps = 0;
     while (v != NONE && !sampled(v, salt, shift) && ++steps < m) {
         r = rec[v];
         len = rec_len(r);
-        copy_chars(o + p, rch, rec_pos(r, v), len);
+        put_run(o + p, r, rch, v, 0, len);
         p += len;
         v = rec_next(r);
     }
@@ -2366,7 +2403,7 @@ __global__ void k_rand_emit(const uint64_t *__restrict__ rec, const char *__rest
     uint32_t v = r0, steps = 0;
     do {
         const uint64_t r = rec[v];
-        copy_chars(text + p, rch, rec_pos(r, v), rec_len(r));
+        put_run(text + p, r, rch, v, 0, rec_len(r));
         p += rec_len(r);
         v = rec_next(r);
     } while (v != NONE && !sampled(v, salt, shift) && ++steps < n_slots);
@@ -3631,6 +3668,9 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
         // CU; measured 0.82 -> 0.71 ms for the staged one-word scatter).  A
         // planned tile is SUB_ROUNDS rounds.
         constexpr int IPT2 = 8;
+        // (1, 2 and 4 rounds measured 0.73, 0.71 and 0.68 ms for the count,
+        // column scan and scatter together on the 50 Mbp job: the scatter takes
+        // the same time whatever the rounds, the histogram work shrinks)
         const uint32_t round2 = IPT2 * BLOCK, tile2 = SUB_ROUNDS * round2;
         // (two-word k-mers measured faster unstaged here: 4.9 vs 5.5 ms on
         // 303 M slots; one-word ones staged: 0.82 vs 0.85 ms on 53 M)
