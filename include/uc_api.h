@@ -234,16 +234,20 @@ int uc_text_append_file(uc_ctx *ctx, const char *path);
 /* ---- text input (driver.cpp:251-354) ---------------------------------- */
 
 /* Parses the reference's k-mer text (count line, then first k characters
- * of each line; bytes other than ACGT map to A) into the reference packing
- * on the host.  *packed is malloc'd (free with uc_free). */
+ * of every later line, whatever the count says; bytes other than ACGT map
+ * to A) into the reference packing on the host.  *packed is malloc'd (free
+ * with uc_free). */
 int uc_parse_kmer_text(const char *buf, uint64_t len, uint32_t k, uint32_t **packed, uint64_t *n);
 int uc_read_kmer_file(const char *path, uint32_t k, uint32_t **packed, uint64_t *n);
 void uc_free(void *p);
 
 /* Replaces driver.cpp:257-354 load_kmers + driver.cpp:227-249 compact_kmers
  * on the device: the same text format and rules as uc_parse_kmer_text
- * (n = min(count line, lines); a line shorter than k after dropping one '\r'
- * before its end is UC_E_FORMAT), parsed and packed by kernels and loaded as
+ * (the count line must hold a number, and every later line is a k-mer
+ * whatever it says: the reference compares the count with the k-mers read
+ * only after whole read chunks, driver.cpp:310-343, and reads on past it; a
+ * line shorter than k after dropping one '\r' before its end is
+ * UC_E_FORMAT), parsed and packed by kernels and loaded as
  * one k-mer set, ready for uc_bucketize or uc_compact_loaded.  text is a
  * host buffer, or a device buffer when on_device is nonzero. */
 int uc_load_text(uc_ctx *ctx, const char *text, uint64_t len, uint32_t k, int on_device);
