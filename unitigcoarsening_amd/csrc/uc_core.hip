@@ -2232,24 +2232,17 @@ __device__ __forceinline__ void put_run(char *o, uint64_t r, const char *__restr
     }
 }
 
+// a head's first run at o (address `at` modulo 4): its first k-mer x, the
+// later characters of its run (record r), '\n' when the run is the whole
+// unitig; the runs after it are written after every head's first run
 template <int W>
-__device__ __forceinline__ void put_head(char *o, uint64_t at, const Kmer<W> &x, uint64_t r,
-                                         const uint64_t *__restrict__ rec, const char *__restrict__ rch, uint32_t s,
-                                         uint32_t k, uint32_t m, uint32_t salt, uint32_t shift)
+__device__ __forceinline__ void put_first(char *o, uint64_t at, const Kmer<W> &x, uint64_t r,
+                                          const char *__restrict__ rch, uint32_t s, uint32_t k)
 {
     put_kmer<W>(o, at, x, k);
-    uint32_t len = rec_len(r), p = k;
-    put_run(o + p, r, rch, s, 1, len);  // the head's own character is the k-th above
-    p += len - 1;
-    uint32_t v = rec_next(r), steps = 0;
-    while (v != NONE && !sampled(v, salt, shift) && ++steps < m) {
-        r = rec[v];
-        len = rec_len(r);
-        put_run(o + p, r, rch, v, 0, len);
-        p += len;
-        v = rec_next(r);
-    }
-    if (v == NONE) o[p] = '\n';
+    const uint32_t len = rec_len(r);
+    put_run(o + k, r, rch, s, 1, len);  // the head's own character is the k-th above
+    if (rec_next(r) == NONE) o[k + len - 1] = '\n';
 }
 
 // LDS staging bytes per wave in k_head_emit (a range that does not fit is
@@ -2261,12 +2254,13 @@ template <int W>
 constexpr uint32_t emit_stage() { return W == 1 ? 2048 : 4096; }
 
 // per tile: exclusive prefix of (1 << 32 | length) over heads -> text
-// offsets.  Every head writes its segment (put_head).  The 64 slots a wave
-// takes at a time own one contiguous text range; when every head there is a
-// whole unitig (a single segment) and the range fits the wave's LDS stage,
-// the heads write into LDS and the range goes out as aligned 16-byte stores.
-// Otherwise each head writes its bytes to HBM itself.  A thread's next item
-// (record and k-mer) is loaded while it writes the current one.
+// offsets.  Every head writes its first run (put_first).  The 64 slots a
+// wave takes at a time own one contiguous text range; when the range fits
+// the wave's LDS stage, the heads write into LDS and the range goes out as
+// aligned 16-byte stores, else each head writes its bytes to HBM itself.  A
+// thread's next item (record and k-mer) is loaded while it writes the
+// current one.  Then the heads of several runs (the tile's list) walk their
+// later runs up to the first sampled splitter, many walks in flight at once.
 template <int W, bool STAGE, int NT>
 __global__ __launch_bounds__(NT) void k_head_emit(const uint64_t *__restrict__ kmer, const uint64_t *__restrict__ rec,
                                                      const char *__restrict__ rch, const uint32_t *__restrict__ hasp,
@@ -2285,7 +2279,7 @@ __global__ __launch_bounds__(NT) void k_head_emit(const uint64_t *__restrict__ k
     const uint32_t ti = xcd_tile(n_tiles);
     if (ti == NONE) return;
     const uint64_t t0 = (uint64_t)ti * CT_TILE;
-    uint32_t hd = 0, multi = 0;  // per item: live head; head of a chain of several segments
+    uint32_t hd = 0;  // per item: live head
     // the tile's list of heads of several runs, its entries, and every
     // item's flags, predecessor bit and run record, all in flight at once
     const uint32_t nml = mcnt[ti];
@@ -2315,22 +2309,18 @@ __global__ __launch_bounds__(NT) void k_head_emit(const uint64_t *__restrict__ k
         tile[j * NT + threadIdx.x] = v;
     }
     __syncthreads();
-    constexpr uint64_t MULTI = 1ull << 63;  // marks a list head of several segments until the scan
 #pragma unroll
     for (int j = 0; j < CTI; j++) {
         const uint32_t e = (uint32_t)j * NT + threadIdx.x;
-        if (e < nml)
-            tile[ml[j] & 0x7FFu] = (1ull << 32) | mlist_len(ml[j], ulen, t0) | ((ml[j] >> 11) & 1u ? MULTI : 0ull);
+        if (e < nml) tile[ml[j] & 0x7FFu] = (1ull << 32) | mlist_len(ml[j], ulen, t0);
     }
     __syncthreads();
-#pragma unroll
-    for (int j = 0; j < CTI; j++) multi |= (tile[j * NT + threadIdx.x] & MULTI ? 1u : 0u) << j;
     uint64_t loc[CTI];
     uint64_t run = 0;
 #pragma unroll
     for (int j = 0; j < CTI; j++) {
         loc[j] = run;
-        run += tile[threadIdx.x * CTI + j] & ~MULTI;
+        run += tile[threadIdx.x * CTI + j];
     }
     const uint64_t incl = wave_incl_scan(run);
     const int w = threadIdx.x / 64;
@@ -2361,7 +2351,7 @@ __global__ __launch_bounds__(NT) void k_head_emit(const uint64_t *__restrict__ k
             nx = load_kmer<W>(kmer, s + NT, ~0ull);
             nr = rec[s + NT];
         }
-        const bool h = (hd >> j) & 1u, mu = (multi >> j) & 1u;
+        const bool h = (hd >> j) & 1u;
         const uint64_t pre = tile[j * NT + threadIdx.x];
         const uint64_t off = (pre & 0xFFFFFFFFull) + (pre >> 32) * k;
         // the wave's text range [r0, r1): from its first slot to the slot after its last
@@ -2369,8 +2359,10 @@ __global__ __launch_bounds__(NT) void k_head_emit(const uint64_t *__restrict__ k
         const uint64_t p0 = tile[li], p1 = tile[li + 64];
         const uint64_t r0 = (p0 & 0xFFFFFFFFull) + (p0 >> 32) * k, r1 = (p1 & 0xFFFFFFFFull) + (p1 >> 32) * k;
         const uint64_t a = r0 & ~15ull;  // staged from r0's 16-byte line
-        if (STAGE && __ballot(mu) == 0 && r1 - a <= STG) {  // uniform over the wave
-            if (h) put_head<W>(stg + (off - a), off, x, r, rec, rch, s, k, m, salt, shift);
+        if (STAGE && r1 - a <= STG) {  // uniform over the wave
+            // (the bytes of runs after a head's first run are written by later
+            // kernels over what the stage holds there)
+            if (h) put_first<W>(stg + (off - a), off, x, r, rch, s, k);
             wave_sync();
             // whole lines [f, e) as 16-byte stores; the partial first and last
             // lines byte by byte, one lane per byte
@@ -2382,9 +2374,35 @@ __global__ __launch_bounds__(NT) void k_head_emit(const uint64_t *__restrict__ k
             if (lane < 32 && hb < hlim && (lane < 16 || e >= f)) text[hb] = stg[hb - a];
             wave_sync();
         } else if (h) {
-            if (mu) htoff[s] = off;
-            put_head<W>(text + off, off, x, r, rec, rch, s, k, m, salt, shift);
+            put_first<W>(text + off, off, x, r, rch, s, k);
         }
+    }
+    // the later runs of the heads of several runs (the tile's list), over
+    // what the waves' stores above left there: the whole tile's text range
+    // belongs to this group, so one barrier orders the two
+    __syncthreads();
+#pragma unroll 1
+    for (int j = 0; j < CTI; j++) {
+        const uint32_t e = (uint32_t)j * NT + threadIdx.x;
+        if (e >= nml) break;
+        const uint32_t ls = (uint32_t)(ml[j] & 0x7FFu);
+        const uint32_t s = (uint32_t)t0 + ls;
+        const uint64_t pre = tile[ls];
+        const uint64_t off = (pre & 0xFFFFFFFFull) + (pre >> 32) * k;
+        uint64_t r = rec[s];
+        uint64_t p = off + k + rec_len(r) - 1;
+        uint32_t v = rec_next(r), steps = 0;
+        while (v != NONE && !sampled(v, salt, shift) && ++steps < m) {
+            r = rec[v];
+            const uint32_t len = rec_len(r);
+            put_run(text + p, r, rch, v, 0, len);
+            p += len;
+            v = rec_next(r);
+        }
+        if (v == NONE)
+            text[p] = '\n';
+        else
+            htoff[s] = off;  // a head of several segments: k_rand_emit places the later ones from it
     }
 }
 
@@ -2768,7 +2786,7 @@ static bool keeps(const uc_ctx *c, const DBuf &b)
 {
     for (const DBuf *k : {&c->kmer, &c->flags, &c->hasp, &c->rec, &c->rch, &c->mlist, &c->mcnt, &c->ulen,
                           &c->chainA, &c->list, &c->tsum, &c->text, &c->small, &c->boff, &c->fboff, &c->cbfirst,
-                          &c->setoff, &c->probes})
+                          &c->setoff, &c->probes, &c->htoff})
         if (k == &b) return true;
     return false;
 }
@@ -4388,18 +4406,10 @@ int run_emit(uc_ctx *c)
     } else {
         CHK(ensure(c, c->text, c->text_bytes + 1));
     }
-    // Head text offsets of multi-segment unitigs (written by k_head_emit,
-    // read by k_rand_emit), one word per slot.  A low-memory job of one set
-    // keeps them in the head lists' memory instead: a tile reads its own list
-    // entries (inside its own slot range) before a barrier and writes offsets
-    // only inside its own slot range after it, and nothing reads the lists
-    // after k_head_emit (k_set_offsets runs for several sets only).
-    const bool alias_htoff = (c->opts.flags & UC_FLAG_LOW_MEMORY) && c->n_sets == 1;
-    if (alias_htoff)
-        CHK(drop(c, c->htoff));
-    else
-        CHK(ensure(c, c->htoff, c->m * 8));
-    uint64_t *htoff = alias_htoff ? P<uint64_t>(c->mlist) : P<uint64_t>(c->htoff);
+    // head text offsets of unitigs of several segments (written by
+    // k_head_emit, read by k_rand_emit), one word per slot
+    CHK(ensure(c, c->htoff, c->m * 8));
+    uint64_t *htoff = P<uint64_t>(c->htoff);
     if (m) {
         // one-word k-mers from 256-thread groups, wider ones from 512-thread
         // groups, both staged in LDS (each measured faster than the other
