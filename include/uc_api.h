@@ -315,6 +315,15 @@ int uc_kernel_stats_select(uc_ctx *ctx, const char *name);
 /* Blocks until the context's stream is idle. */
 int uc_sync(uc_ctx *ctx);
 
+/* No counterpart in the reference.  Most probes of one insert into an
+ * edge-finding slice table (fine buckets of 4097..65535 slots are split
+ * into hash slices of about 2048 keys, each in an 8192-entry LDS table):
+ * 1..8192, default 8192.  An insert that runs out of probes marks the slice
+ * as overflowed, and the job's edge phase then runs again with every sliced
+ * fine bucket on an HBM table (same result, one more edge phase).  A small
+ * bound forces that route, which tests use. */
+int uc_ctx_set_slice_probes(uc_ctx *ctx, uint32_t probes);
+
 /* ---- autotuner ---------------------------------------------------------
  * Times whole compactions of the loaded job (uc_load / uc_load_text / ...)
  * under candidate settings, by coordinate descent over n_buckets,

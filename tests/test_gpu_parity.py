@@ -403,6 +403,27 @@ def test_small_groups_route_fine_buckets_to_every_path(k, l, mode):
     assert uc.sorted_lines(got) == textbook_sorted(kmers, k)
 
 
+@pytest.mark.parametrize("low_memory", [False, True])
+def test_slice_overflow_takes_hbm_tables(low_memory):
+    # one probe per slice-table insert: the first collision marks its slice
+    # as overflowed, and the edge phase runs again with every sliced fine
+    # bucket on an HBM table.  The whole-job form reads the flag after
+    # emission (deferred counters), the low-memory form after contraction.
+    kmers = uc.medium_model(93, 200_000, 31)
+    with uc.Compactor(n_buckets=16, group_slots=256, low_memory=low_memory) as c:
+        c.set_slice_probes(1)
+        got = c.compact(kmers, 31, 3)
+        b = c.bucket_info()
+    log(f"slice overflow low_memory={low_memory}: largest fine {b.largest_fine_bucket} slices {b.n_slice_tasks} "
+        f"hbm {b.n_hbm_tables}")
+    assert b.largest_fine_bucket > 4096
+    assert b.n_slice_tasks == 0 and b.n_hbm_tables > 0
+    assert uc.sorted_lines(got) == textbook_sorted(kmers, 31)
+    with uc.Compactor(n_buckets=16, group_slots=256) as c:
+        c.compact(kmers, 31, 3)
+        assert c.bucket_info().n_slice_tasks > 0  # the same job with the default bound keeps its slices
+
+
 # ---------------------------------------------------------------- the configured workloads, reduced
 
 

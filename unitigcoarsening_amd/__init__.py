@@ -192,6 +192,7 @@ API = {
     "uc_free": (None, [_P]),
     "uc_timing_get": (C.c_int, [_P, C.POINTER(UcTiming)]),
     "uc_sync": (C.c_int, [_P]),
+    "uc_ctx_set_slice_probes": (C.c_int, [_P, C.c_uint32]),
     "uc_load_text": (C.c_int, [_P, C.c_char_p, C.c_uint64, C.c_uint32, C.c_int]),
     "uc_load_file": (C.c_int, [_P, C.c_char_p, C.c_uint32]),
     "uc_loaded_copy": (C.c_int, [_P, _U32P, C.c_uint64, _U64P]),
@@ -554,6 +555,10 @@ class Compactor:
 
     def sync(self):
         _check(lib().uc_sync(self._ctx), self._ctx)
+
+    def set_slice_probes(self, probes):
+        """Most probes of one slice-table insert (uc_ctx_set_slice_probes)."""
+        _check(lib().uc_ctx_set_slice_probes(self._ctx, probes), self._ctx)
 
     def autotune(self, l, reps=2, keep_buckets=False):
         """Tune the settings on the loaded job (uc_autotune); returns the choice as a dict."""

@@ -1704,7 +1704,7 @@ __global__ __launch_bounds__(1024) void k_edge_slice(const uint64_t *__restrict_
                                                      const SliceTask *__restrict__ tasks, uint32_t k, uint64_t pm0,
                                                      uint64_t pm1, uint32_t *__restrict__ link,
                                                      uint32_t *__restrict__ hasp, RunOut ro, uint32_t *__restrict__ err,
-                                                     unsigned long long *__restrict__ probes)
+                                                     unsigned long long *__restrict__ probes, uint32_t plimit)
 {
     constexpr uint32_t NT = 1024;
     __shared__ uint32_t tab[SLICE_TAB];
@@ -1724,7 +1724,7 @@ __global__ __launch_bounds__(1024) void k_edge_slice(const uint64_t *__restrict_
         if (fastrange(h, t.P) != t.p) continue;
         const uint32_t tg = (tag & 0x7Fu) << 17;
         uint32_t pos = h & (SLICE_TAB - 1), probe = 0;
-        for (; probe < SLICE_TAB; probe++) {
+        for (; probe < plimit; probe++) {
             if (CNT) np++;
             const uint32_t cur = atomicCAS(&tab[pos], LDS_EMPTY, i | tg);
             if (cur == LDS_EMPTY) break;
@@ -1735,7 +1735,9 @@ __global__ __launch_bounds__(1024) void k_edge_slice(const uint64_t *__restrict_
             }
             pos = (pos + 1) & (SLICE_TAB - 1);
         }
-        if (probe == SLICE_TAB) atomicOr(err, 1u);  // table full: not expected at SLICE_KEYS per slice
+        // table full (not expected at SLICE_KEYS per slice): the host runs
+        // the edge phase again with these fine buckets on HBM tables
+        if (probe == plimit) atomicOr(err, 1u);
     }
     __syncthreads();
     // 2. suffix hits of the copies that carry their out-edge here, this slice
@@ -1839,6 +1841,24 @@ enum {
     C_LIST = 0, C_ERR = 32, C_ACTIVE = 64, C_NONCONV = 96, C_CYC = 128, C_CUT = 160, C_TOT = 192, C_OVER = 224,
     C_SLICE = 256, C_WORDS = 288
 };
+
+// chain rounds launched before the host looks at the active counters; a
+// chain of up to 2^(SPEC_ROUNDS - 1) sampled segments converges within them
+constexpr uint32_t SPEC_ROUNDS = 2;
+
+// A speculative contraction pass whose result emission may use, from its
+// counters: no error flag, every chain at its head after the speculative
+// rounds, no chain without a head (an isolated cycle), every one of the n
+// k-mers reached.  Emission after a pass the host has not checked yet runs
+// only when this holds: otherwise the unitig lengths and chain ancestors
+// are not all written, and the kernels write nothing (the host then runs
+// the pass again in its checked form).
+__device__ __forceinline__ bool pass_settled(const uint32_t *__restrict__ cnt, uint32_t n)
+{
+    const uint64_t tot = *reinterpret_cast<const uint64_t *>(cnt + C_TOT + 2);
+    return !cnt[C_ERR] && !cnt[C_SLICE] && (!cnt[C_LIST] || !cnt[C_ACTIVE + SPEC_ROUNDS - 1]) && !cnt[C_NONCONV] &&
+           (uint32_t)tot == n;
+}
 
 __device__ __forceinline__ bool live_head(const uint8_t *__restrict__ flags, const uint32_t *__restrict__ hasp,
                                           uint32_t s)
@@ -2269,9 +2289,11 @@ __global__ __launch_bounds__(NT) void k_head_emit(const uint64_t *__restrict__ k
                                                      const uint32_t *__restrict__ ulen,
                                                      const uint64_t *__restrict__ tile_off, uint32_t m, uint32_t k,
                                                      uint32_t salt, uint32_t shift, uint64_t *__restrict__ htoff,
-                                                     char *__restrict__ text, uint32_t n_tiles)
+                                                     char *__restrict__ text, uint32_t n_tiles,
+                                                     const uint32_t *__restrict__ gate, uint32_t n_kmers)
 {
     constexpr uint32_t STG = emit_stage<W>();
+    if (gate && !pass_settled(gate, n_kmers)) return;  // (uniform over the grid)
     constexpr int NW = NT / 64, CTI = CT_TILE / NT;  // waves; items per thread
     __shared__ uint64_t tile[CT_TILE + 1];  // prefixes in slot order, then the tile's end
     __shared__ uint64_t wsum[NW];
@@ -2406,15 +2428,12 @@ __global__ __launch_bounds__(NT) void k_head_emit(const uint64_t *__restrict__ k
     }
 }
 
-// segments of sampled splitters: text positions from the ranked chain
-__global__ void k_rand_emit(const uint64_t *__restrict__ rec, const char *__restrict__ rch,
-                            const uint32_t *__restrict__ list, uint32_t n, const uint64_t *__restrict__ chain,
-                            const uint64_t *__restrict__ htoff, uint32_t k, uint32_t salt, uint32_t shift,
-                            uint32_t n_slots, char *__restrict__ text)
+// the segment of sampled splitter r0, placed by its ranked chain entry
+__device__ __forceinline__ void rand_segment(const uint64_t *__restrict__ rec, const char *__restrict__ rch, uint32_t r0,
+                                             const uint64_t *__restrict__ chain, const uint64_t *__restrict__ htoff,
+                                             uint32_t k, uint32_t salt, uint32_t shift, uint32_t n_slots,
+                                             char *__restrict__ text)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t r0 = list[i];
     const uint64_t c = chain[r0];
     const uint32_t a = (uint32_t)c, d = (uint32_t)(c >> 32);
     uint64_t p = htoff[a] + (k - 1) + d;
@@ -2428,6 +2447,20 @@ __global__ void k_rand_emit(const uint64_t *__restrict__ rec, const char *__rest
     if (v == NONE) text[p] = '\n';
 }
 
+// segments of sampled splitters: text positions from the ranked chain (the
+// list's length *np from the device; the grid strides over it)
+__global__ void k_rand_emit(const uint64_t *__restrict__ rec, const char *__restrict__ rch,
+                            const uint32_t *__restrict__ list, const uint32_t *__restrict__ np,
+                            const uint64_t *__restrict__ chain, const uint64_t *__restrict__ htoff, uint32_t k,
+                            uint32_t salt, uint32_t shift, uint32_t n_slots, char *__restrict__ text,
+                            const uint32_t *__restrict__ gate, uint32_t n_kmers)
+{
+    if (gate && !pass_settled(gate, n_kmers)) return;
+    const uint32_t n = *np;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        rand_segment(rec, rch, list[i], chain, htoff, k, salt, shift, n_slots, text);
+}
+
 // text offset of each set's first unitig: the prefix at the set's first slot
 // (one workgroup per set: the heads between the start of the set's first
 // slot's contraction tile and that slot)
@@ -2439,13 +2472,15 @@ __global__ __launch_bounds__(BLOCK) void k_set_offsets(const uint32_t *__restric
                                                        const uint32_t *__restrict__ mcnt,
                                                        const uint32_t *__restrict__ ulen,
                                                        const uint64_t *__restrict__ tile_off, uint32_t m,
-                                                       uint64_t total, uint64_t *__restrict__ out)
+                                                       const uint64_t *__restrict__ total, uint64_t *__restrict__ out,
+                                                       const uint32_t *__restrict__ gate, uint32_t n_kmers)
 {
     __shared__ uint64_t red[WAVES];
+    if (gate && !pass_settled(gate, n_kmers)) return;
     const uint32_t s = blockIdx.x;
     const uint32_t sb = s < n_sets ? boff[(uint64_t)s * B] : m;
     if (sb >= m) {
-        if (threadIdx.x == 0) out[s] = total;
+        if (threadIdx.x == 0) out[s] = *total;  // (unitigs << 32 | k-mers) of the whole job
         return;
     }
     const uint32_t t = sb / CT_TILE;
@@ -2617,6 +2652,8 @@ struct Pool {
 
 using namespace uc;
 
+constexpr size_t RBUF_BYTES = 64u << 10;  // bytes of a context's pinned read buffer
+
 struct uc_ctx {
     uc_opts opts;
     hipStream_t stream = nullptr;
@@ -2660,6 +2697,18 @@ struct uc_ctx {
         htoff, setoff, text, small, txt, txtcnt, rec, rch, tcnt, cbtf, over, setplan, colsum, bgroups, gfirst, stasks,
         mlist, mcnt, probes;
     // pinned host chunks of the file reader and writer
+    // pinned bounce buffer of the small device reads (counters, offsets):
+    // a copy into pinned memory completes sooner than one into pageable
+    // memory, which HIP stages (measured on MI355X: 14.8 against 22.0 us per
+    // read, launch and wait included)
+    char *rbuf = nullptr;
+    // a slice table of this bucketing overflowed: its sliced fine buckets
+    // take HBM tables (set when the edge phase runs again for that)
+    bool no_slices = false;
+    uint32_t slice_probes = SLICE_TAB;  // most probes of one slice-table insert
+    // a contraction pass whose counters the host has not read yet (the text
+    // buffer is sized for the most text the job can have meanwhile)
+    bool deferred = false;
     char *pin[2] = {nullptr, nullptr};
     hipEvent_t pin_ev[2] = {nullptr, nullptr};
     bool pin_used[2] = {false, false};
@@ -3113,6 +3162,12 @@ int select_dev(uc_ctx *c)
 int d2h(uc_ctx *c, void *dst, const void *src, size_t bytes)
 {
     if (bytes == 0) return UC_OK;
+    if (c->rbuf && bytes <= RBUF_BYTES) {
+        HIPCHK(c, hipMemcpyAsync(c->rbuf, src, bytes, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        std::memcpy(dst, c->rbuf, bytes);
+        return UC_OK;
+    }
     HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return UC_OK;
@@ -3155,7 +3210,7 @@ int drop_all(uc_ctx *c)
 void reset_job(uc_ctx *c)
 {
     c->loaded = c->bucketed = c->edged = c->coarsened = c->emitted = false;
-    c->edges_ready = false;
+    c->edges_ready = c->no_slices = c->deferred = false;
     c->n_unitigs = c->n_cycles = c->text_bytes = c->n_splitters = 0;
     c->chain_rounds = 0;
     c->m = c->x = c->largest = 0;
@@ -3756,7 +3811,7 @@ int do_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     if (!c->loaded) return fail(c, UC_E_STATE, "uc_bucketize before uc_load");
     if (l < 1 || l > 15 || l >= c->k) return fail(c, UC_E_ARG, "l must be in [1, min(15, k-1)]");
     c->edged = c->coarsened = c->emitted = false;
-    c->edges_ready = false;
+    c->edges_ready = c->no_slices = false;
     return c->W == 1 ? run_bucketize<1>(c, l, table) : run_bucketize<2>(c, l, table);
 }
 
@@ -3828,7 +3883,7 @@ int launch_slice_edges_(uc_ctx *c, size_t first, size_t count)
     if (count == 0) return UC_OK;
     KLAUNCH(c, "k_edge_slice", (k_edge_slice<W, CNT>), ((uint32_t)count, 1024, 0), P<uint64_t>(c->kmer),
             P<SliceTask>(c->stasks) + first, c->k, c->pm0, c->pm1, P<uint32_t>(c->link), P<uint32_t>(c->hasp),
-            run_out(c), P<uint32_t>(c->small) + C_SLICE, probe_counter(c));
+            run_out(c), P<uint32_t>(c->small) + C_SLICE, probe_counter(c), c->slice_probes);
     return UC_OK;
 }
 
@@ -3946,7 +4001,7 @@ int prepare_edges(uc_ctx *c)
             if (big_ok && sz <= BIG_GROUP) {
                 ovb.push_back(f);
                 big.push_back(EGroup{c->fboff_h[f], c->fboff_h[f + 1]});
-            } else if (sz <= SLICE_MAX) {
+            } else if (sz <= SLICE_MAX && !c->no_slices) {
                 ovs.push_back(f);
                 s_first.push_back(slices.size());
                 const uint32_t P = (uint32_t)((sz + SLICE_KEYS - 1) / SLICE_KEYS);
@@ -4109,9 +4164,10 @@ int do_edges(uc_ctx *c, uint32_t bucket)
 // one contraction pass: segment walks, chain ranking, unitig count/length
 // per tile, scan of the tiles.  *live = k-mers reached from a head,
 // *nonconv = sampled splitters whose chain has no head (on a cycle).
-// chain rounds launched before the host looks at the active counters; a
-// chain of up to 2^(SPEC_ROUNDS - 1) sampled segments converges within them
-constexpr uint32_t SPEC_ROUNDS = 2;
+// internal status of a contraction pass whose edge phase had a slice table
+// overflow: the edge phase runs again without slices (never returned by the
+// C ABI)
+constexpr int RETRY_SLICES = 1;
 constexpr uint32_t CHAIN_BLOCKS = 1024;  // grid of the chain kernels (they stride)
 
 // Segment walks, chain ranking and unitig totals.  The first attempt (spec)
@@ -4141,7 +4197,9 @@ void drop_pass_graph(uc_ctx *c)
     c->cgraph_key.clear();
 }
 
-int contract_pass(uc_ctx *c, uint64_t *live, uint32_t *nonconv, bool spec = true)
+// With `defer`, the speculative pass is only launched: its counters stay on
+// the device for finish_deferred().
+int contract_pass(uc_ctx *c, uint64_t *live, uint32_t *nonconv, bool spec = true, bool defer = false)
 {
     const uint32_t m = (uint32_t)c->m;
     const uint64_t T = (c->m + CT_TILE - 1) / CT_TILE;
@@ -4187,7 +4245,8 @@ int contract_pass(uc_ctx *c, uint64_t *live, uint32_t *nonconv, bool spec = true
     auto read_counts = [&]() -> int {
         CHK(d2h(c, h, cnt, sizeof(h)));
         if (h[C_ERR]) return fail(c, UC_E_STATE, "successor array corrupted: a segment walk exceeded m steps");
-        if (h[C_SLICE]) return fail(c, UC_E_STATE, "an edge-finding slice table overflowed");
+        if (h[C_SLICE]) return c->no_slices ? fail(c, UC_E_STATE, "an edge-finding slice table overflowed")
+                                            : RETRY_SLICES;
         return UC_OK;
     };
     if (spec && pass_graph(c)) {
@@ -4251,6 +4310,10 @@ int contract_pass(uc_ctx *c, uint64_t *live, uint32_t *nonconv, bool spec = true
         CHK(totals());
     }
     if (spec && SPEC_ROUNDS % 2) std::swap(ca, cb);  // where the last speculative round wrote
+    if (defer) {
+        c->chain = ca;
+        return UC_OK;
+    }
     // the unitig totals land beside the other counters: one read for all
     CHK(read_counts());
     const uint32_t nl = h[C_LIST];
@@ -4332,10 +4395,25 @@ int ensure_contraction(uc_ctx *c)
     return UC_OK;
 }
 
-int do_coarsen(uc_ctx *c)
+// the edge phase again with every fine bucket that had a slice on an HBM
+// table (a slice table overflowed), then contraction
+int do_coarsen(uc_ctx *c, bool defer = false);
+int redo_without_slices(uc_ctx *c, bool defer)
+{
+    c->no_slices = true;
+    c->edges_ready = false;
+    CHK(do_edges(c, UC_ALL_BUCKETS));
+    return do_coarsen(c, defer);
+}
+
+// Segment walks, chain ranking and cycle cutting.  With `defer` (whole jobs
+// that can hold the largest text, see can_defer()), the pass's counters are
+// not read here: emission follows at once and finish_deferred() reads them.
+int do_coarsen(uc_ctx *c, bool defer)
 {
     if (!c->edged) return fail(c, UC_E_STATE, "uc_coarsen before uc_find_edges");
     c->emitted = false;
+    c->deferred = false;
     const uint32_t m = (uint32_t)c->m;
     CHK(ensure_contraction(c));
     c->chain_rounds = 0;
@@ -4351,7 +4429,18 @@ int do_coarsen(uc_ctx *c)
     c->spliced = true;
     uint64_t live = 0;
     uint32_t nonconv = 0;
-    if (m) CHK(contract_pass(c, &live, &nonconv));
+    if (defer) {
+        CHK(contract_pass(c, &live, &nonconv, true, true));
+        c->deferred = true;
+        c->text_bytes = c->n * (c->k + 1);  // n isolated k-mers: the most text a job can have
+        c->coarsened = true;
+        return UC_OK;
+    }
+    if (m) {
+        const int rc = contract_pass(c, &live, &nonconv);
+        if (rc == RETRY_SLICES) return redo_without_slices(c, false);
+        CHK(rc);
+    }
     if (live != c->n || nonconv) {
         // cutting cycles reads the ranked chain and labels in the other chain
         // buffer; the pass's other arrays are rewritten by the next pass, so
@@ -4393,6 +4482,9 @@ int do_coarsen(uc_ctx *c)
     return UC_OK;
 }
 
+int set_ranges(uc_ctx *c);
+int finish_deferred(uc_ctx *c);
+
 template <int W>
 int run_emit(uc_ctx *c)
 {
@@ -4410,6 +4502,9 @@ int run_emit(uc_ctx *c)
     // k_head_emit, read by k_rand_emit), one word per slot
     CHK(ensure(c, c->htoff, c->m * 8));
     uint64_t *htoff = P<uint64_t>(c->htoff);
+    uint32_t *cnt = P<uint32_t>(c->small);
+    // a pass the host has not checked: the kernels check it themselves
+    const uint32_t *gate = c->deferred ? cnt : nullptr;
     if (m) {
         // one-word k-mers from 256-thread groups, wider ones from 512-thread
         // groups, both staged in LDS (each measured faster than the other
@@ -4418,36 +4513,100 @@ int run_emit(uc_ctx *c)
         KLAUNCH(c, "k_head_emit", (k_head_emit<W, true, NT>), (xcd_grid(T), NT, 0), P<uint64_t>(c->kmer),
                 P<uint64_t>(c->rec), P<char>(c->rch), P<uint32_t>(c->hasp), P<uint8_t>(c->flags),
                 P<uint64_t>(c->mlist), P<uint32_t>(c->mcnt), P<uint32_t>(c->ulen), P<uint64_t>(c->tsum), m, c->k,
-                c->salt, c->shift, htoff, P<char>(c->text), tile_arg(c, T));
+                c->salt, c->shift, htoff, P<char>(c->text), tile_arg(c, T), gate, (uint32_t)c->n);
     }
-    if (c->n_list) {
-        KLAUNCH(c, "k_rand_emit", k_rand_emit, (grid_for(c->n_list), BLOCK, 0), P<uint64_t>(c->rec), P<char>(c->rch),
-                P<uint32_t>(c->list), c->n_list, c->chain, htoff, c->k, c->salt, c->shift, m, P<char>(c->text));
+    if (c->deferred || c->n_list) {
+        // (the list's length is read by the kernel: a deferred pass has not
+        // told the host yet)
+        const uint32_t g = c->deferred ? std::max<uint32_t>(1, std::min<uint32_t>(grid_for(m), CHAIN_BLOCKS))
+                                       : grid_for(c->n_list);
+        KLAUNCH(c, "k_rand_emit", k_rand_emit, (g, BLOCK, 0), P<uint64_t>(c->rec), P<char>(c->rch),
+                P<uint32_t>(c->list), cnt + C_LIST, c->chain, htoff, c->k, c->salt, c->shift, m, P<char>(c->text),
+                gate, (uint32_t)c->n);
     }
-    c->set_text.resize(c->n_sets + 1);
-    if (c->n_sets == 1) {  // one set: the whole text
-        c->set_text[0] = 0;
-        c->set_text[1] = c->text_bytes;
-        c->emitted = true;
-        return UC_OK;
+    if (c->n_sets > 1) {
+        // per-set text ranges: a set's unitigs start at its first bucket row
+        CHK(ensure(c, c->setoff, (c->n_sets + 1) * 8));
+        KLAUNCH(c, "k_set_offsets", k_set_offsets, (c->n_sets + 1, BLOCK, 0), P<uint32_t>(c->boff), c->n_sets,
+                c->B, P<uint32_t>(c->hasp), P<uint8_t>(c->flags), P<uint64_t>(c->rec), P<uint64_t>(c->mlist),
+                P<uint32_t>(c->mcnt), P<uint32_t>(c->ulen), P<uint64_t>(c->tsum), m,
+                reinterpret_cast<const uint64_t *>(cnt + C_TOT + 2), P<uint64_t>(c->setoff), gate, (uint32_t)c->n);
     }
-    // per-set text ranges: a set's unitigs start at its first bucket row
-    const uint64_t total = (c->n_unitigs << 32) | (c->text_bytes - c->n_unitigs * c->k);
-    CHK(ensure(c, c->setoff, (c->n_sets + 1) * 8));
-    KLAUNCH(c, "k_set_offsets", k_set_offsets, (c->n_sets + 1, BLOCK, 0), P<uint32_t>(c->boff), c->n_sets,
-            c->B, P<uint32_t>(c->hasp), P<uint8_t>(c->flags), P<uint64_t>(c->rec), P<uint64_t>(c->mlist),
-            P<uint32_t>(c->mcnt), P<uint32_t>(c->ulen), P<uint64_t>(c->tsum), m, total, P<uint64_t>(c->setoff));
-    std::vector<uint64_t> so(c->n_sets + 1);
-    CHK(d2h(c, so.data(), c->setoff.p, (c->n_sets + 1) * 8));
-    for (uint32_t s = 0; s <= c->n_sets; s++) c->set_text[s] = (so[s] & 0xFFFFFFFFull) + (so[s] >> 32) * c->k;
-    c->emitted = true;
-    return UC_OK;
+    return c->deferred ? finish_deferred(c) : set_ranges(c);
 }
 
 int do_emit(uc_ctx *c)
 {
     if (!c->coarsened) return fail(c, UC_E_STATE, "uc_emit before uc_coarsen");
     return c->W == 1 ? run_emit<1>(c) : run_emit<2>(c);
+}
+
+// each set's text range (one set: the whole text), after emission
+int set_ranges(uc_ctx *c)
+{
+    c->set_text.resize(c->n_sets + 1);
+    if (c->n_sets == 1) {
+        c->set_text[0] = 0;
+        c->set_text[1] = c->text_bytes;
+    } else {
+        std::vector<uint64_t> so(c->n_sets + 1);
+        CHK(d2h(c, so.data(), c->setoff.p, (c->n_sets + 1) * 8));
+        for (uint32_t s = 0; s <= c->n_sets; s++) c->set_text[s] = (so[s] & 0xFFFFFFFFull) + (so[s] >> 32) * c->k;
+    }
+    c->emitted = true;
+    return UC_OK;
+}
+
+// The end of a job whose contraction counters were left on the device: one
+// read of them after emission.  A pass that did not settle (a chain still
+// open after the speculative rounds, a cycle no head reaches) or whose edge
+// phase had a slice table overflow runs again in the checked form, emission
+// included; everything those write is rewritten.
+int finish_deferred(uc_ctx *c)
+{
+    c->deferred = false;
+    uint32_t h[C_WORDS];
+    CHK(d2h(c, h, P<uint32_t>(c->small), sizeof(h)));
+    if (h[C_ERR]) return fail(c, UC_E_STATE, "successor array corrupted: a segment walk exceeded m steps");
+    const uint32_t nl = h[C_LIST];
+    uint32_t rounds = 0;
+    while (rounds < SPEC_ROUNDS && h[C_ACTIVE + rounds]) rounds++;
+    uint64_t st, tot;
+    std::memcpy(&st, h + C_TOT, 8);
+    std::memcpy(&tot, h + C_TOT + 2, 8);
+    const uint64_t live = tot & 0xFFFFFFFFull;
+    if (h[C_SLICE] || (nl && rounds == SPEC_ROUNDS) || live != c->n || h[C_NONCONV]) {
+        c->coarsened = false;
+        if (h[C_SLICE]) {
+            if (c->no_slices) return fail(c, UC_E_STATE, "an edge-finding slice table overflowed");
+            CHK(redo_without_slices(c, false));
+        } else {
+            CHK(do_coarsen(c));
+        }
+        return do_emit(c);
+    }
+    c->n_list = nl;
+    c->n_splitters = st >> 32;
+    c->n_runs = st & 0xFFFFFFFFull;
+    c->chain_rounds += nl ? rounds + 1 : 0;
+    c->n_unitigs = tot >> 32;
+    c->text_bytes = live + c->n_unitigs * c->k;
+    return set_ranges(c);
+}
+
+// Whole jobs whose largest possible text (n isolated k-mers) fits in
+// DEFER_TEXT_MAX bytes leave the contraction counters on the device until
+// emission is done: one host read fewer per job.  Not for low-memory jobs
+// (whose text buffer is sized exactly), the caller's text buffer or the
+// static scratch.
+constexpr uint64_t DEFER_TEXT_MAX = 256ull << 20;
+
+bool can_defer(const uc_ctx *c)
+{
+    const uint64_t bound = c->n * (c->k + 1) + 1;
+    if (!c->m || c->arena || c->text_fixed || (c->opts.flags & UC_FLAG_LOW_MEMORY) || bound > DEFER_TEXT_MAX)
+        return false;
+    return !c->opts.hbm_budget || c->held + bound <= c->opts.hbm_budget;
 }
 
 // bucketing .. emission on the loaded k-mers, one event per phase boundary
@@ -4461,7 +4620,7 @@ int timed_phases(uc_ctx *c, uint32_t l)
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     CHK(do_edges(c, UC_ALL_BUCKETS));
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-    CHK(do_coarsen(c));
+    CHK(do_coarsen(c, can_defer(c)));
     HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
     CHK(do_emit(c));
     HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
@@ -4573,7 +4732,16 @@ int uc_ctx_create(const uc_opts *opts, uc_ctx **out)
     // set-up of uc_compact_noalloc, with the context's device current (a
     // failed lookup only disables that entry point)
     c->scratch_dev = scratch_ptr(o.device);
+    // (without it the reads go to pageable memory)
+    if (hipHostMalloc((void **)&c->rbuf, RBUF_BYTES, hipHostMallocDefault) != hipSuccess) c->rbuf = nullptr;
     *out = c;
+    return UC_OK;
+}
+
+int uc_ctx_set_slice_probes(uc_ctx *c, uint32_t probes)
+{
+    if (!c || probes < 1 || probes > SLICE_TAB) return UC_E_ARG;
+    c->slice_probes = probes;
     return UC_OK;
 }
 
@@ -4588,6 +4756,7 @@ int uc_ctx_destroy(uc_ctx *c)
     if (c->pool.base) (void)hipFree(c->pool.base);
     if (c->meter) c->meter->sub(c->held);
     free_pinned(c);
+    if (c->rbuf) (void)hipHostFree(c->rbuf);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &p : c->pending) {
