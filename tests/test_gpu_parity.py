@@ -576,10 +576,15 @@ def test_low_memory_mode_same_bytes_smaller_peak(k, l, B, model):
             assert 0 < peak_low < 0.8 * peak_full
             info = c.coarsen_info()
             assert info.text_bytes == len(want)
-            # the input is consumed
+            # the input is consumed, and the head lists (which hold the
+            # text offsets of multi-segment unitigs now) are spent
             with pytest.raises(uc.UcError) as e:
                 c.compact_loaded(l)
             assert e.value.status == uc.UC_E_STATE
+            with pytest.raises(uc.UcError) as e:
+                c.emit()
+            assert e.value.status == uc.UC_E_STATE
+            assert c.text() == want
         # a whole job from host memory works as usual, and so does one from
         # the k-mer text parsed on the device
         assert c.compact(kmers, k, l) == want

@@ -2709,6 +2709,8 @@ struct uc_ctx {
     // a contraction pass whose counters the host has not read yet (the text
     // buffer is sized for the most text the job can have meanwhile)
     bool deferred = false;
+    // a low-memory emission wrote head text offsets over the head lists
+    bool lists_spent = false;
     char *pin[2] = {nullptr, nullptr};
     hipEvent_t pin_ev[2] = {nullptr, nullptr};
     bool pin_used[2] = {false, false};
@@ -3210,7 +3212,7 @@ int drop_all(uc_ctx *c)
 void reset_job(uc_ctx *c)
 {
     c->loaded = c->bucketed = c->edged = c->coarsened = c->emitted = false;
-    c->edges_ready = c->no_slices = c->deferred = false;
+    c->edges_ready = c->no_slices = c->deferred = c->lists_spent = false;
     c->n_unitigs = c->n_cycles = c->text_bytes = c->n_splitters = 0;
     c->chain_rounds = 0;
     c->m = c->x = c->largest = 0;
@@ -4413,7 +4415,7 @@ int do_coarsen(uc_ctx *c, bool defer)
 {
     if (!c->edged) return fail(c, UC_E_STATE, "uc_coarsen before uc_find_edges");
     c->emitted = false;
-    c->deferred = false;
+    c->deferred = c->lists_spent = false;
     const uint32_t m = (uint32_t)c->m;
     CHK(ensure_contraction(c));
     c->chain_rounds = 0;
@@ -4498,10 +4500,21 @@ int run_emit(uc_ctx *c)
     } else {
         CHK(ensure(c, c->text, c->text_bytes + 1));
     }
-    // head text offsets of unitigs of several segments (written by
-    // k_head_emit, read by k_rand_emit), one word per slot
-    CHK(ensure(c, c->htoff, c->m * 8));
-    uint64_t *htoff = P<uint64_t>(c->htoff);
+    // Head text offsets of unitigs of several segments (written by
+    // k_head_emit, read by k_rand_emit), one word per slot.  A low-memory job
+    // of one set keeps them in the head lists' memory instead: a tile reads
+    // all its own list entries (inside its own slot range) before its first
+    // barrier and writes offsets only inside its own slot range after its
+    // last, and nothing else reads the lists (k_set_offsets runs for several
+    // sets only).  Emission cannot run again on the spent lists.
+    if (c->lists_spent) return fail(c, UC_E_STATE, "uc_emit again after a low-memory emission (the head lists are spent)");
+    const bool alias_htoff = (c->opts.flags & UC_FLAG_LOW_MEMORY) && c->n_sets == 1 && !c->deferred;
+    if (alias_htoff)
+        CHK(drop(c, c->htoff));
+    else
+        CHK(ensure(c, c->htoff, c->m * 8));
+    uint64_t *htoff = alias_htoff ? P<uint64_t>(c->mlist) : P<uint64_t>(c->htoff);
+    c->lists_spent = alias_htoff;
     uint32_t *cnt = P<uint32_t>(c->small);
     // a pass the host has not checked: the kernels check it themselves
     const uint32_t *gate = c->deferred ? cnt : nullptr;
