@@ -339,9 +339,10 @@ __global__ __launch_bounds__(BLOCK) void k_scan_reduce(const T *__restrict__ in,
 
 // exclusive scan of one tile per block; `offset` holds per-block exclusive
 // offsets (or is null); the block owning the last element writes the total
+// (and a copy of it at `total2`, when given)
 template <typename T>
 __global__ __launch_bounds__(BLOCK) void k_scan_tile(const T *in, T *out, uint64_t n, const T *__restrict__ offset,
-                                                     T *__restrict__ total)
+                                                     T *__restrict__ total, T *__restrict__ total2)
 {
     __shared__ T tile[SCAN_TILE];
     __shared__ T wsum[WAVES];
@@ -378,7 +379,10 @@ __global__ __launch_bounds__(BLOCK) void k_scan_tile(const T *in, T *out, uint64
         uint64_t i = base + (uint64_t)j * BLOCK + threadIdx.x;
         if (i < n) out[i] = tile[j * BLOCK + threadIdx.x];
     }
-    if (total && threadIdx.x == 0 && base + SCAN_TILE >= n) *total = boff + btot;
+    if (threadIdx.x == 0 && base + SCAN_TILE >= n) {
+        if (total) *total = boff + btot;
+        if (total2) *total2 = boff + btot;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -725,18 +729,25 @@ __global__ void k_plan1_tiles(const uint64_t *__restrict__ set_off, const uint64
 }
 
 // bucket row starts: boff[set*B + b] = scanned histogram at the set's first tile
+// (and the same rows into boff2 when given: the fine rows of a job without
+// a fine sort)
 __global__ void k_gather_boff(const uint32_t *__restrict__ hscan, const uint64_t *__restrict__ set_hbase,
                               const uint32_t *__restrict__ set_ts, uint32_t n_sets, uint32_t B, uint32_t m,
-                              uint32_t *__restrict__ boff)
+                              uint32_t *__restrict__ boff, uint32_t *__restrict__ boff2)
 {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t G = n_sets * B;
+    uint32_t v;
     if (g < G) {
         const uint32_t s = g / B, b = g % B;
-        boff[g] = hscan[set_hbase[s] + (uint64_t)b * set_ts[s]];
+        v = hscan[set_hbase[s] + (uint64_t)b * set_ts[s]];
     } else if (g == G) {
-        boff[G] = m;
+        v = m;
+    } else {
+        return;
     }
+    boff[g] = v;
+    if (boff2) boff2[g] = v;
 }
 
 // ---- fine buckets: a stable counting sort of every coarse bucket's slots by
@@ -1838,7 +1849,7 @@ __device__ __forceinline__ uint32_t wave_append(bool flag, uint32_t *counter)
 // (uint32 indices: one 128-byte line each, so that atomics on different
 // counters never share a line; C_TOT holds two uint64 totals)
 enum {
-    C_LIST = 0, C_ERR = 32, C_ACTIVE = 64, C_NONCONV = 96, C_CYC = 128, C_CUT = 160, C_TOT = 192, C_OVER = 224,
+    C_LIST = 0, C_ERR = 32, C_ACTIVE = 64, C_NONCONV = 96, C_CYC = 128, C_CUT = 160, C_TOT = 192,
     C_SLICE = 256, C_WORDS = 288
 };
 
@@ -2692,9 +2703,9 @@ struct uc_ctx {
     // device buffers
     const uint64_t *kin = nullptr;  // k-mers in 64-bit layout (the caller's buffer or `in`)
     DBuf otable, in, stage, bkt, tiles, hist, xhist, kmer2, flags2, src2, sub, perm, tiles2, hist2, cbhb, cbts,
-        cbfirst, fboff, egroups, fbbase, scan_tmp, kmer, flags, src, pair_h, pair_b, boff, sethb, setts, etiles,
+        cbfirst, fboff, egroups, fbbase, scan_tmp, kmer, flags, src, pair_h, pair_b, boff, etiles,
         toff, tcap, table, hit, link, hasp, pred, seg, prevseg, chainA, chainB, cycB, list, clist, ulen, tsum,
-        htoff, setoff, text, small, txt, txtcnt, rec, rch, tcnt, cbtf, over, setplan, colsum, bgroups, gfirst, stasks,
+        htoff, setoff, text, small, txt, txtcnt, rec, rch, tcnt, cbtf, over, colsum, bgroups, gfirst, stasks,
         mlist, mcnt, probes;
     // pinned host chunks of the file reader and writer
     // pinned bounce buffer of the small device reads (counters, offsets):
@@ -2758,12 +2769,12 @@ static std::vector<DBuf *> all_bufs(uc_ctx *c)
     return {&c->kmer2,  &c->flags2,  &c->src2,   &c->sub,     &c->perm,    &c->tiles2, &c->hist2,
             &c->cbhb,   &c->cbts,    &c->cbfirst, &c->fboff,  &c->egroups, &c->fbbase, &c->otable,
             &c->in,     &c->stage,   &c->bkt,    &c->tiles,   &c->hist,    &c->xhist,  &c->scan_tmp,
-            &c->kmer,   &c->flags,   &c->src,    &c->pair_h,  &c->pair_b,  &c->boff,   &c->sethb,
-            &c->setts,  &c->etiles,  &c->toff,   &c->tcap,    &c->table,   &c->hit,    &c->link,
+            &c->kmer,   &c->flags,   &c->src,    &c->pair_h,  &c->pair_b,  &c->boff,
+            &c->etiles,  &c->toff,   &c->tcap,    &c->table,   &c->hit,    &c->link,
             &c->hasp,   &c->pred,    &c->seg,    &c->prevseg, &c->chainA, &c->chainB,
             &c->cycB,   &c->list,    &c->clist,  &c->ulen,    &c->tsum,    &c->htoff,  &c->setoff,
             &c->text,   &c->small,   &c->txt,    &c->txtcnt,  &c->rec,     &c->rch,
-            &c->tcnt,   &c->cbtf,    &c->over,   &c->setplan, &c->colsum, &c->bgroups, &c->gfirst, &c->stasks,
+            &c->tcnt,   &c->cbtf,    &c->over,   &c->colsum, &c->bgroups, &c->gfirst, &c->stasks,
             &c->mlist,  &c->mcnt,    &c->probes};
 }
 
@@ -3129,30 +3140,33 @@ size_t scan_scratch(uint64_t n)
 }
 
 template <typename T>
-int scan_level(uc_ctx *c, const T *in, T *out, uint64_t n, T *d_total, char *tmp)
+int scan_level(uc_ctx *c, const T *in, T *out, uint64_t n, T *d_total, T *d_total2, char *tmp)
 {
     const uint64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     if (nb <= 1) {
-        KLAUNCH(c, "k_scan_tile", k_scan_tile<T>, (1, BLOCK, 0), in, out, n, nullptr, d_total);
+        KLAUNCH(c, "k_scan_tile", k_scan_tile<T>, (1, BLOCK, 0), in, out, n, nullptr, d_total, d_total2);
         return UC_OK;
     }
     T *partial = reinterpret_cast<T *>(tmp);
     char *next = tmp + ((nb + 1) * sizeof(T) + 255) / 256 * 256;
     KLAUNCH(c, "k_scan_reduce", k_scan_reduce<T>, ((uint32_t)nb, BLOCK, 0), in, n, partial);
-    CHK(scan_level<T>(c, partial, partial, nb, nullptr, next));
-    KLAUNCH(c, "k_scan_tile", k_scan_tile<T>, ((uint32_t)nb, BLOCK, 0), in, out, n, partial, d_total);
+    CHK(scan_level<T>(c, partial, partial, nb, nullptr, nullptr, next));
+    KLAUNCH(c, "k_scan_tile", k_scan_tile<T>, ((uint32_t)nb, BLOCK, 0), in, out, n, partial, d_total, d_total2);
     return UC_OK;
 }
 
+// exclusive scan of n values; the total goes to d_total and, when given, to
+// d_total2 as well (a counter word the host reads)
 template <typename T>
-int scan_excl(uc_ctx *c, const T *in, T *out, uint64_t n, T *d_total, DBuf &tmp)
+int scan_excl(uc_ctx *c, const T *in, T *out, uint64_t n, T *d_total, DBuf &tmp, T *d_total2 = nullptr)
 {
     if (n == 0) {
         if (d_total) CHK(fill<T>(c, d_total, 1, T(0)));
+        if (d_total2) CHK(fill<T>(c, d_total2, 1, T(0)));
         return UC_OK;
     }
     CHK(ensure(c, tmp, scan_scratch<T>(n) + 256));
-    return scan_level<T>(c, in, out, n, d_total, reinterpret_cast<char *>(tmp.p));
+    return scan_level<T>(c, in, out, n, d_total, d_total2, reinterpret_cast<char *>(tmp.p));
 }
 
 int select_dev(uc_ctx *c)
@@ -3613,18 +3627,30 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     uint32_t *sub_h = reinterpret_cast<uint32_t *>(set_plan.data() + 2 * (uint64_t)S + 2);
     for (uint32_t s = 0; s < S; s++) sub_h[s] = c->set_sub[s];
     const uint64_t hb = (uint64_t)B * T;
-    CHK(ensure(c, c->setplan, set_plan.size() * 8));
-    CHK(h2d(c, c->setplan.p, set_plan.data(), set_plan.size() * 8));
+    const uint64_t G = (uint64_t)S * B;
+    // every coarse bucket's first fine bucket (from the sets' fine counts)
+    c->cb_first.assign(G + 1, 0);
+    for (uint64_t g = 0; g < G; g++) c->cb_first[g + 1] = c->cb_first[g] + c->set_sub[g / B];
+    const uint64_t F = c->cb_first[G];
+    // the host's plan in one upload, in 8-byte words: the coarse buckets'
+    // first fine buckets (G + 1 u32), the set plan, the sets' histogram bases
+    // (u64) and tile counts (u32); cbfirst lives on to emission
+    const uint64_t w_cbf = (G + 2) / 2, w_plan = set_plan.size(), w_hb = S, w_ts = (S + 1) / 2;
+    std::vector<uint64_t> up(w_cbf + w_plan + w_hb + w_ts, 0);
+    std::memcpy(up.data(), c->cb_first.data(), (G + 1) * 4);
+    std::memcpy(up.data() + w_cbf, set_plan.data(), w_plan * 8);
+    std::memcpy(up.data() + w_cbf + w_plan, set_hbase.data(), S * 8);
+    std::memcpy(up.data() + w_cbf + w_plan + w_hb, set_ts.data(), S * 4);
+    CHK(ensure(c, c->cbfirst, up.size() * 8));
+    CHK(h2d(c, c->cbfirst.p, up.data(), up.size() * 8));
+    const uint64_t *sp = P<uint64_t>(c->cbfirst) + w_cbf;
+    const uint64_t *sethb = sp + w_plan;
+    const uint32_t *setts = reinterpret_cast<const uint32_t *>(sethb + w_hb);
     CHK(ensure(c, c->tiles, T * sizeof(BTile)));
-    const uint64_t *sp = P<uint64_t>(c->setplan);
     KLAUNCH(c, "k_plan1_tiles", k_plan1_tiles, (grid_for(T), BLOCK, 0), sp, sp + S + 1,
             reinterpret_cast<const uint32_t *>(sp + 2 * (uint64_t)S + 2), S, B, tile1, T, P<BTile>(c->tiles));
     CHK(ensure(c, c->hist, (hb + 1) * 4));
     CHK(ensure(c, c->xhist, (T + 1) * 4));
-    CHK(ensure(c, c->sethb, S * 8));
-    CHK(ensure(c, c->setts, S * 4));
-    CHK(h2d(c, c->sethb.p, set_hbase.data(), S * 8));
-    CHK(h2d(c, c->setts.p, set_ts.data(), S * 4));
 
     LminParams p;
     p.k = c->k;
@@ -3661,20 +3687,18 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
         break;
     }
 #undef UC_CNT
-    // totals at hist[hb] and xhist[T], copies of both side by side for one read
+    // totals at hist[hb] and xhist[T], and side by side in the counters for one read
     CHK(ensure(c, c->small, C_WORDS * 4));
     uint32_t *cnt = P<uint32_t>(c->small);
-    CHK(scan_excl<uint32_t>(c, P<uint32_t>(c->hist), P<uint32_t>(c->hist), hb, P<uint32_t>(c->hist) + hb, c->scan_tmp));
+    CHK(scan_excl<uint32_t>(c, P<uint32_t>(c->hist), P<uint32_t>(c->hist), hb, P<uint32_t>(c->hist) + hb, c->scan_tmp,
+                            cnt + C_TOT));
     CHK(scan_excl<uint32_t>(c, P<uint32_t>(c->xhist), P<uint32_t>(c->xhist), T, P<uint32_t>(c->xhist) + T,
-                            c->scan_tmp));
-    HIPCHK(c, hipMemcpyAsync(cnt + C_TOT, P<uint32_t>(c->hist) + hb, 4, hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(cnt + C_TOT + 1, P<uint32_t>(c->xhist) + T, 4, hipMemcpyDeviceToDevice, c->stream));
+                            c->scan_tmp, cnt + C_TOT + 1));
     uint32_t tot[2];
     CHK(d2h(c, tot, cnt + C_TOT, 8));
     CHK(check_placements(c, c->n, tot[1]));
     c->x = tot[1];
     c->m = c->n + c->x;
-    const uint64_t G = (uint64_t)S * B;
     // pass 1 writes the final k-mers and flags when no set is split into
     // fine buckets, else the k-mers and the fine index with the flag bits
     DBuf &k1 = any_fine ? c->kmer2 : c->kmer, &s1 = any_fine ? c->src2 : c->src;
@@ -3711,8 +3735,10 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
         else UC_S1(false, false);
     }
 #undef UC_S1
+    // the fine rows are the coarse rows when no set has a fine sort
+    CHK(ensure(c, c->fboff, (F + 1) * 4));
     KLAUNCH(c, "k_gather_boff", k_gather_boff, (grid_for(G + 1), BLOCK, 0), P<uint32_t>(c->hist),
-            P<uint64_t>(c->sethb), P<uint32_t>(c->setts), S, B, (uint32_t)c->m, P<uint32_t>(c->boff));
+            sethb, setts, S, B, (uint32_t)c->m, P<uint32_t>(c->boff), any_fine ? nullptr : P<uint32_t>(c->fboff));
     c->rows_valid = false;
     if (c->opts.flags & UC_FLAG_LOW_MEMORY) {
         // the codes, the input and the tile histogram are read by pass 1 and
@@ -3729,15 +3755,7 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
 
     // fine buckets: stable sort of each coarse bucket's slots by fine index,
     // planned and run on the device (no host round trip)
-    c->cb_first.assign(G + 1, 0);
-    for (uint64_t g = 0; g < G; g++) c->cb_first[g + 1] = c->cb_first[g] + c->set_sub[g / B];
-    const uint64_t F = c->cb_first[G];
-    CHK(ensure(c, c->cbfirst, (G + 1) * 4));
-    CHK(h2d(c, c->cbfirst.p, c->cb_first.data(), (G + 1) * 4));
-    CHK(ensure(c, c->fboff, (F + 1) * 4));
-    if (!any_fine) {
-        HIPCHK(c, hipMemcpyAsync(c->fboff.p, c->boff.p, (G + 1) * 4, hipMemcpyDeviceToDevice, c->stream));
-    } else {
+    if (any_fine) {
         // slots per fine-sort round; LDS staging when the device has the room
         // (2048-slot rounds: half the LDS of 4096, so more tiles in flight per
         // CU; measured 0.82 -> 0.71 ms for the staged one-word scatter).  A
@@ -3955,18 +3973,17 @@ int prepare_edges(uc_ctx *c)
     CHK(ensure(c, c->gfirst, (G + 2) * 4));
     CHK(ensure(c, c->small, C_WORDS * 4));
     uint32_t *cnt = P<uint32_t>(c->small);
-    CHK(fill<uint32_t>(c, cnt + C_OVER, 1, 0u));
     CHK(clear_probes(c));
     std::vector<uint32_t> gf(G + 1, 0);
     if (G) {
+        // the oversized count is kept after the group offsets: one read for both
         uint32_t *gfd = P<uint32_t>(c->gfirst);
+        CHK(fill<uint32_t>(c, gfd + G + 1, 1, 0u));
         KLAUNCH(c, "k_plan_count", k_plan_count, ((uint32_t)G, BLOCK, 0), P<uint32_t>(c->fboff),
-                P<uint32_t>(c->cbfirst), group_max, gfd, P<uint32_t>(c->over), cnt + C_OVER);
+                P<uint32_t>(c->cbfirst), group_max, gfd, P<uint32_t>(c->over), gfd + G + 1);
         CHK(scan_excl<uint32_t>(c, gfd, gfd, G, gfd + G, c->scan_tmp));
         KLAUNCH(c, "k_plan_write", k_plan_write, ((uint32_t)G, BLOCK, 0), P<uint32_t>(c->fboff),
                 P<uint32_t>(c->cbfirst), group_max, gfd, P<EGroup>(c->egroups));
-        // the oversized count lands after the group offsets: one read for both
-        HIPCHK(c, hipMemcpyAsync(gfd + G + 1, cnt + C_OVER, 4, hipMemcpyDeviceToDevice, c->stream));
         gf.resize(G + 2);
         CHK(d2h(c, gf.data(), gfd, (G + 2) * 4));
     }
@@ -3974,8 +3991,6 @@ int prepare_edges(uc_ctx *c)
     if (G) {
         n_over = gf[G + 1];
         gf.resize(G + 1);
-    } else {
-        CHK(d2h(c, &n_over, cnt + C_OVER, 4));
     }
     c->lds_nb = group_max;
     c->grp_first.assign(gf.begin(), gf.end());
@@ -4066,8 +4081,10 @@ int prepare_edges(uc_ctx *c)
     CHK(h2d(c, c->fbbase.p, fb_base.data(), NF * 4));
     CHK(ensure(c, c->toff, (NF + 1) * 8));
     CHK(ensure(c, c->tcap, NF * 4 + 8));
-    CHK(h2d(c, c->toff.p, c->et_toff.data(), (NF + 1) * 8));
-    CHK(h2d(c, c->tcap.p, c->et_cap.data(), NF * 4));
+    if (NF) {  // (read by the HBM-table kernels only)
+        CHK(h2d(c, c->toff.p, c->et_toff.data(), (NF + 1) * 8));
+        CHK(h2d(c, c->tcap.p, c->et_cap.data(), NF * 4));
+    }
     CHK(ensure(c, c->table, total * sizeof(TEntry)));
     CHK(ensure(c, c->hit, c->m * 4));
     CHK(ensure(c, c->link, c->m * 4));
@@ -4103,13 +4120,16 @@ int run_fallback(uc_ctx *c, size_t f0, size_t f1)
 template <int W>
 int run_edges(uc_ctx *c, uint32_t bucket)
 {
+    const bool fresh = !c->edges_ready;  // planning clears the bitmap and the probe counters itself
     CHK(prepare_edges(c));
     const uint64_t G = (uint64_t)c->n_sets * c->B;
     const bool plain = c->opts.mode == UC_MODE_PLAIN;
     if (bucket == UC_ALL_BUCKETS) {
         c->spliced = false;
-        CHK(fill<uint32_t>(c, P<uint32_t>(c->hasp), (c->m + 31) / 32, 0u));
-        CHK(clear_probes(c));
+        if (!fresh) {
+            CHK(fill<uint32_t>(c, P<uint32_t>(c->hasp), (c->m + 31) / 32, 0u));
+            CHK(clear_probes(c));
+        }
         if (!plain) {
             CHK(launch_lds_edges<W>(c, 0, c->grp_first[G]));
             CHK(launch_big_edges<W>(c, 0, c->bg_first[G]));
