@@ -414,10 +414,15 @@ __device__ __forceinline__ uint2 get_codes(const void *__restrict__ bkt, uint64_
 template <int W, int IPT, int MODE, bool C32>
 __global__ __launch_bounds__(BLOCK) void k_lmin_count(const uint64_t *__restrict__ kin, const BTile *__restrict__ tiles,
                                                       LminParams p, uint64_t top_mask, void *__restrict__ bkt,
-                                                      uint32_t *__restrict__ hist, uint32_t *__restrict__ xhist)
+                                                      uint32_t *__restrict__ hist, uint32_t *__restrict__ xhist,
+                                                      uint32_t n_tiles)
 {
     extern __shared__ uint32_t cnt[];  // B buckets + 1 crossing counter
-    const BTile t = tiles[blockIdx.x];
+    // (XCD-contiguous tiles: neighbouring tiles write neighbouring words of
+    // every bucket's histogram row, which one L2 then merges into whole lines)
+    const uint32_t ti = xcd_tile(n_tiles);
+    if (ti == NONE) return;
+    const BTile t = tiles[ti];
     // fine bucket = order & (B*S - 1); its low log2(B) bits are the coarse
     // bucket.  A k-mer crosses when its two fine buckets differ (both copies
     // may then share a coarse bucket).
@@ -447,7 +452,7 @@ __global__ __launch_bounds__(BLOCK) void k_lmin_count(const uint64_t *__restrict
     }
     __syncthreads();
     for (uint32_t b = threadIdx.x; b < p.B; b += BLOCK) hist[t.hbase + (uint64_t)b * t.ts + t.tlocal] = cnt[b];
-    if (threadIdx.x == 0) xhist[blockIdx.x] = cnt[p.B];
+    if (threadIdx.x == 0) xhist[ti] = cnt[p.B];
 }
 
 // lanes whose home (mp) or borrow (ms) placement goes to bucket v
@@ -3673,8 +3678,8 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     const bool c32 = (uint64_t)B * smax <= 65536;
     CHK(ensure(c, c->bkt, std::max<uint64_t>(c->n, 1) * (c32 ? 4 : 8)));
 #define UC_CNT(MODE, C32)                                                                                          \
-    KLAUNCH(c, "k_lmin_count", (k_lmin_count<W, IPT1, MODE, C32>), ((uint32_t)T, BLOCK, (B + 1) * 4), kin,          \
-            P<BTile>(c->tiles), p, c->top_mask, c->bkt.p, P<uint32_t>(c->hist), P<uint32_t>(c->xhist))
+    KLAUNCH(c, "k_lmin_count", (k_lmin_count<W, IPT1, MODE, C32>), (xcd_grid(T), BLOCK, (B + 1) * 4), kin,        \
+            P<BTile>(c->tiles), p, c->top_mask, c->bkt.p, P<uint32_t>(c->hist), P<uint32_t>(c->xhist), tile_arg_xcd(T))
     switch (lmin_mode(l, table != nullptr)) {
     case LM_TABLE:
         if (c32) { UC_CNT(LM_TABLE, true); } else { UC_CNT(LM_TABLE, false); }
