@@ -70,7 +70,8 @@ typedef struct {
 #define UC_FLAG_DIRECT_SCATTER 8u /* both bucketing scatters write straight from registers instead of
                                      staging each tile in LDS sorted by bucket */
 #define UC_FLAG_EAGER 16u /* launch the kernels of a contraction pass one by one instead of replaying
-                             them as a captured HIP graph */
+                             them as a captured HIP graph, and read its counters before emission (small
+                             jobs otherwise read them once, after emission) */
 #define UC_FLAG_COUNT_PROBES 32u /* count the hash-table probes of the edge phase (uc_edge_probes) */
 #define UC_FLAG_LOW_MEMORY 64u /* free every buffer at the phase boundary after its last use, the loaded input
                                   (uc_load*) after the first scatter included: the device holds about half the

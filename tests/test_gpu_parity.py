@@ -367,6 +367,36 @@ def test_pass_graph_replays_match_eager():
             assert uc.sorted_lines(want) == uc.sorted_lines(tb)
 
 
+@pytest.mark.parametrize("case", ["cycles", "long_chains", "both"])
+def test_deferred_counters_match_checked_order(case):
+    # small whole jobs emit before the host reads the contraction counters;
+    # when the pass did not settle (isolated cycles, chains longer than the
+    # speculative rounds) the emission kernels write nothing and the job runs
+    # again in the checked order, which the eager form always takes: the two
+    # must agree byte for byte, and with the textbook
+    rng = random.Random(2024)
+    k = 31
+    kmers = []
+    if case in ("cycles", "both"):
+        kmers += circular_kmers(rng, 1500, k) + circular_kmers(rng, 433, k)
+    if case in ("long_chains", "both"):
+        g = "".join(rng.choice("ACGT") for _ in range(60_000))
+        kmers += [g[i:i + k] for i in range(len(g) - k + 1)]
+    packed = uc.encode(sorted(set(kmers)), k)
+    with uc.Compactor(n_buckets=16, sample_log2=3) as d, uc.Compactor(n_buckets=16, sample_log2=3, eager=True) as e:
+        for rep in range(2):  # twice: the second job reuses the first one's buffers
+            got = d.compact(packed, k, 11)
+            info = d.coarsen_info()
+            want = e.compact(packed, k, 11)
+            log(f"deferred {case} rep {rep}: cycles {info.n_cycles} chain rounds {info.chain_rounds}")
+            assert got == want
+    if case != "long_chains":
+        assert info.n_cycles > 0
+    if case != "cycles":
+        assert info.chain_rounds > 2
+    assert uc.sorted_lines(got) == textbook_sorted(packed, k)
+
+
 @pytest.mark.parametrize("l,B,mode", [(1, 2, uc.UC_MODE_BATCHED), (1, 2, uc.UC_MODE_PLAIN), (3, 2, uc.UC_MODE_BATCHED),
                                       (4, 4, uc.UC_MODE_PLAIN), (5, 8, uc.UC_MODE_BATCHED)])
 def test_oversized_fine_buckets(l, B, mode):

@@ -4635,14 +4635,15 @@ int finish_deferred(uc_ctx *c)
 // Whole jobs whose largest possible text (n isolated k-mers) fits in
 // DEFER_TEXT_MAX bytes leave the contraction counters on the device until
 // emission is done: one host read fewer per job.  Not for low-memory jobs
-// (whose text buffer is sized exactly), the caller's text buffer or the
-// static scratch.
+// (whose text buffer is sized exactly), the caller's text buffer, the static
+// scratch or the eager form (which takes every step in the checked order).
 constexpr uint64_t DEFER_TEXT_MAX = 256ull << 20;
 
 bool can_defer(const uc_ctx *c)
 {
     const uint64_t bound = c->n * (c->k + 1) + 1;
-    if (!c->m || c->arena || c->text_fixed || (c->opts.flags & UC_FLAG_LOW_MEMORY) || bound > DEFER_TEXT_MAX)
+    if (!c->m || c->arena || c->text_fixed || (c->opts.flags & (UC_FLAG_LOW_MEMORY | UC_FLAG_EAGER)) ||
+        bound > DEFER_TEXT_MAX)
         return false;
     return !c->opts.hbm_budget || c->held + bound <= c->opts.hbm_budget;
 }
