@@ -100,6 +100,18 @@ def test_genome_all_k(k):
     assert uc.sorted_lines(got) == textbook_sorted(kmers, k)
 
 
+@pytest.mark.parametrize("k,l", [(16, 15), (64, 15), (33, 15), (3, 2), (2, 1)])
+def test_extreme_minimizer_lengths(k, l):
+    # the largest l (15: 30-bit l-mer codes) and l = k - 1 (two l-mers per
+    # k-mer, so both minimizers come from the end l-mers)
+    kmers = uc.synth_kmers(k * 7 + l, 30_000, k, repeat_frac=0.05, seg_min=200, seg_max=800, repeat_sub=0.01,
+                           error_frac=0.01)
+    log(f"extreme l: k={k} l={l} n={kmers.shape[0]}")
+    for B in (1, 64):
+        got = run(kmers, k, l, n_buckets=B)
+        assert uc.sorted_lines(got) == textbook_sorted(kmers, k)
+
+
 @pytest.mark.parametrize("k,l,B", [(31, 11, 64), (31, 11, 16), (63, 11, 256), (15, 7, 64), (47, 9, 32)])
 def test_medium_model_vs_textbook_and_refpipe(k, l, B):
     kmers = uc.medium_model(k + B, 400_000, k)
