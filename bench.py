@@ -385,9 +385,14 @@ def bench_file(args, kmers, l, B):
         os.rmdir(d)
 
 
+SERVE_CLIENTS = 4  # client threads that submit the arrivals (each submission copies its input)
+
+
 def serve_run(args, pool, sizes, n_slots, load, n_jobs, mean_svc, seed):
-    """Poisson arrivals at `load` x one context's capacity onto a server of
-    n_slots workers; latency = submit to result of every job."""
+    """Poisson arrivals at `load` x one worker's capacity onto a server of
+    n_slots workers, submitted by SERVE_CLIENTS client threads; latency =
+    submit to result of every job."""
+    import queue
     import threading
 
     import numpy as np
@@ -406,13 +411,13 @@ def serve_run(args, pool, sizes, n_slots, load, n_jobs, mean_svc, seed):
         warm = [srv.submit(km, k, l) for km, k, l in order for _ in range(n_slots)]
         for i in warm:
             srv.result(i, text=False)
-        ids = []
+        ids = [None] * n_jobs
         waiter_err = []
 
         def waiter():
             i = 0
             while i < n_jobs:
-                while len(ids) <= i:
+                while ids[i] is None:
                     time.sleep(0.0002)
                 try:
                     # the text stays in the server: this measures the server, not
@@ -423,7 +428,20 @@ def serve_run(args, pool, sizes, n_slots, load, n_jobs, mean_svc, seed):
                     waiter_err.append(repr(e))
                 i += 1
 
+        arrivals = queue.Queue()
+
+        def client():
+            while True:
+                j = arrivals.get()
+                if j is None:
+                    return
+                km, k, l = pool[picks[j]]
+                ids[j] = srv.submit(km, k, l)  # (the copy runs with the GIL released)
+
+        clients = [threading.Thread(target=client) for _ in range(SERVE_CLIENTS)]
         th = threading.Thread(target=waiter)
+        for c in clients:
+            c.start()
         th.start()
         t0 = time.perf_counter()
         nxt = t0
@@ -432,8 +450,11 @@ def serve_run(args, pool, sizes, n_slots, load, n_jobs, mean_svc, seed):
             dt = nxt - time.perf_counter()
             if dt > 0:
                 time.sleep(dt)
-            km, k, l = pool[picks[j]]
-            ids.append(srv.submit(km, k, l))
+            arrivals.put(j)
+        for _ in clients:
+            arrivals.put(None)
+        for c in clients:
+            c.join()
         th.join()
         wall = time.perf_counter() - t0
         st = srv.stats()
@@ -452,9 +473,9 @@ def serve_run(args, pool, sizes, n_slots, load, n_jobs, mean_svc, seed):
 def bench_serve(args):
     """Request loop (uc_server) at the serving configuration: inputs of
     log-uniform 100k-50M k-mers with k drawn from {31, 63}, an HBM budget
-    of 288 GB, Poisson arrivals at a fixed fraction of one context's
-    capacity; then a saturation point (1.5x that capacity) with one worker
-    slot and with four."""
+    of 288 GB, Poisson arrivals at a fixed fraction of one worker's
+    capacity; then saturation points (1.5x and 3x that capacity) with one
+    worker slot and with four."""
     import numpy as np
     import unitigcoarsening_amd as uc
 
@@ -463,30 +484,17 @@ def bench_serve(args):
     sizes = np.array([p[0].shape[0] for p in pool])
     log(f"serve pool: {len(pool)} inputs, {sizes.sum()} k-mers, sizes {sizes.min()}..{sizes.max()} "
         f"in {time.time() - t:.1f}s")
-    # one context, jobs back to back, through the same host boundary as a
-    # server worker (the input from host memory, the text copied out to host
-    # memory): service time per input
-    import ctypes as C
-
+    # one worker's capacity: a one-slot server, jobs back to back after one
+    # warm-up pass over the inputs, each job's run time (the worker's copy of
+    # the input to the device, the job, the text copied out to host memory)
     svc = []
-    L = uc.lib()
-    out = np.empty(1, np.uint8)
-    with uc.Compactor(n_buckets=args.buckets, sample_log2=args.sample_log2) as c:
-        def one(km, k, l):
-            nonlocal out
-            km = np.ascontiguousarray(km)
-            if L.uc_compact(c._ctx, km.ctypes.data, km.shape[0], k, l, 0) != 0:
-                raise uc.UcError(-1, L.uc_last_error(c._ctx).decode())
-            n = L.uc_text_size(c._ctx)
-            if out.size < n:
-                out = np.empty(n, np.uint8)
-            L.uc_text_copy(c._ctx, out.ctypes.data_as(C.c_char_p), n)
+    with uc.Server(n_slots=1, hbm_budget=int(args.serve_budget_gb * 1e9), n_buckets=args.buckets,
+                   sample_log2=args.sample_log2) as srv:
         for km, k, l in pool:
-            one(km, k, l)
+            srv.result(srv.submit(km, k, l), text=False)
         for km, k, l in pool:
-            t0 = time.perf_counter()
-            one(km, k, l)
-            svc.append(time.perf_counter() - t0)
+            _, _, rms = srv.result(srv.submit(km, k, l), text=False)
+            svc.append(rms / 1e3)
     mean_svc = float(np.mean(svc))
     main_run = serve_run(args, pool, sizes, args.serve_slots, args.serve_load, args.serve_jobs, mean_svc, 6)
     sat = None
@@ -497,18 +505,20 @@ def bench_serve(args):
                   for ns in (1, args.serve_slots)}
             pt["slots_ratio"] = round(pt[f"slots_{args.serve_slots}"]["kmers_per_s"] / pt["slots_1"]["kmers_per_s"], 3)
             sat[f"load_{load:g}x"] = pt
-    return {"workload": f"{args.serve_jobs} jobs, Poisson arrivals at {args.serve_load:.2f} x one-context capacity, "
+    return {"workload": f"{args.serve_jobs} jobs, Poisson arrivals at {args.serve_load:.2f} x one worker's capacity "
+                        f"(submitted by {SERVE_CLIENTS} client threads), "
                         f"drawn from {len(pool)} inputs of log-uniform 100k-50M k-mers (medium model, k drawn from "
                         f"{{31, 63}}, seeds 5000..{5000 + len(pool) - 1}), {args.serve_slots} worker slots, budget "
                         f"{args.serve_budget_gb:g} GB",
             "pool_kmers": [int(x) for x in sizes], "pool_k": [p[1] for p in pool],
-            "mean_service_ms_one_context": round(1e3 * mean_svc, 3),
+            "mean_service_ms_one_worker": round(1e3 * mean_svc, 3),
             **{kk: v for kk, v in main_run.items() if kk not in ("n_slots",)},
             "saturation": sat,
-            "note": "latency is submit to finish and includes the server's host copies of the input and of the "
-                    "result text (the client does not fetch the texts here); the saturation points offer 1.5x and 3x "
-                    f"one context's capacity to one worker slot and to {args.serve_slots}: served k-mers/s above the "
-                    "one-slot figure is work overlapped across the workers (device, copies and host threads)"}
+            "note": "latency is submit to finish and includes the server's copies of the input to the device and of "
+                    "the result text to host memory (the client does not fetch the texts here); the saturation points "
+                    f"offer 1.5x and 3x one worker's capacity to one worker slot and to {args.serve_slots}: served "
+                    "k-mers/s above the one-slot figure is work overlapped across the workers (device, copies and host "
+                    "threads)"}
 
 
 def main():
@@ -543,7 +553,7 @@ def main():
     ap.add_argument("--serve-jobs", type=int, default=300)
     ap.add_argument("--serve-inputs", type=int, default=16, help="distinct job inputs drawn from")
     ap.add_argument("--serve-slots", type=int, default=4)
-    ap.add_argument("--serve-load", type=float, default=0.7, help="offered load, x one-context capacity")
+    ap.add_argument("--serve-load", type=float, default=0.7, help="offered load, x one worker's capacity")
     ap.add_argument("--serve-saturation-jobs", type=int, default=150,
                     help="jobs of each saturation run (1.5x load, 1 slot and --serve-slots); 0 skips them")
     ap.add_argument("--serve-budget-gb", type=float, default=288.0)

@@ -6,6 +6,7 @@
 // and returns a text that is a function of its k-mers alone.
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -92,3 +93,10 @@ void uc_ctx_set_meter(uc_ctx *c, uc_meter *m)
     c->meter = m;
     if (m) m->add(c->mem.size());
 }
+
+// host memory stands in for page-locked memory
+void *uc_host_pinned_alloc(uint64_t bytes) { return std::malloc(bytes ? bytes : 1); }
+
+void uc_host_pinned_free(void *p) { std::free(p); }
+
+int uc_text_copy_pinned(uc_ctx *c, char *dst, uint64_t cap) { return uc_text_copy(c, dst, cap); }

@@ -1819,7 +1819,6 @@ __global__ __launch_bounds__(1024) void k_edge_slice(const uint64_t *__restrict_
 // segments are ranked by pointer jumping over the sampled splitters only.
 // Emission then walks each segment again, copying its runs' characters.
 
-constexpr int CT_ITEMS = SCAN_ITEMS;
 constexpr int CT_TILE = SCAN_TILE;  // slots per contraction tile
 static_assert(CT_TILE <= 2048, "a tile's list entries hold slot offsets in 11 bits");
 
@@ -1845,7 +1844,7 @@ __device__ __forceinline__ uint32_t wave_append(bool flag, uint32_t *counter)
     if (!bal) return NONE;
     const int leader = __builtin_ctzll(bal);
     uint32_t base = 0;
-    if (__lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(bal));
+    if (__lane_id() == (uint32_t)leader) base = atomicAdd(counter, (uint32_t)__popcll(bal));
     base = __shfl(base, leader, 64);
     return base + (uint32_t)__popcll(bal & ((1ull << __lane_id()) - 1ull));
 }
@@ -5252,6 +5251,30 @@ void uc_ctx_set_meter(uc_ctx *c, uc_meter *m)
     if (c->meter) c->meter->sub(c->held);
     c->meter = m;
     if (m) m->add(c->held);
+}
+
+void *uc_host_pinned_alloc(uint64_t bytes)
+{
+    void *p = nullptr;
+    if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) return nullptr;
+    return p;
+}
+
+void uc_host_pinned_free(void *p)
+{
+    if (p) (void)hipHostFree(p);
+}
+
+int uc_text_copy_pinned(uc_ctx *c, char *dst, uint64_t cap)
+{
+    if (!c || (!dst && c->text_bytes)) return UC_E_ARG;
+    if (!c->emitted) return fail(c, UC_E_STATE, "no text");
+    if (cap < c->text_bytes) return fail(c, UC_E_ARG, "destination too small");
+    CHK(select_dev(c));
+    if (!c->text_bytes) return UC_OK;
+    HIPCHK(c, hipMemcpyAsync(dst, c->text.p, c->text_bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return UC_OK;
 }
 
 extern "C" {

@@ -27,3 +27,11 @@ struct uc_meter {
 // Attach (or detach, with nullptr) a meter to a context; the bytes the
 // context already holds move onto it.  Call from the context's own thread.
 void uc_ctx_set_meter(uc_ctx *c, uc_meter *m);
+
+// Page-locked host memory (nullptr when it cannot be had), and its release.
+void *uc_host_pinned_alloc(uint64_t bytes);
+void uc_host_pinned_free(void *p);
+
+// uc_text_copy into page-locked memory from uc_host_pinned_alloc: one
+// direct device-to-host copy, no staging.
+int uc_text_copy_pinned(uc_ctx *c, char *dst, uint64_t cap);

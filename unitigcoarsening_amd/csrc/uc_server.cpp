@@ -8,6 +8,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -21,15 +22,91 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
+// A host buffer of a job's input or text: page-locked when it could be had.
+struct HostBuf {
+    char *p = nullptr;
+    uint64_t cap = 0;
+    bool pinned = false;
+};
+
+// Input and text buffers recycled from job to job.  Page-locked, so that the
+// copies to and from the device are one DMA each, from and into memory that is
+// already mapped (a fresh host buffer costs a page fault per 4 KiB on first
+// touch: a 108 MB text took 8.3 ms that way, against 2-4 ms into page-locked
+// memory, on MI355X; a submission's copy of its input slowed likewise).  Sizes
+// are rounded up to an eighth of their power of two, so that the buffers of
+// jobs of similar size are interchangeable; at most KEEP bytes stay free.
+class HostPool {
+public:
+    static constexpr uint64_t KEEP = 8ull << 30;
+
+    HostBuf get(uint64_t bytes)
+    {
+        const uint64_t cap = round_up(bytes);
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            auto it = free_.lower_bound(cap);
+            if (it != free_.end() && it->first <= 2 * cap) {
+                HostBuf b{it->second, it->first, true};
+                free_bytes_ -= it->first;
+                free_.erase(it);
+                return b;
+            }
+        }
+        HostBuf b;
+        b.p = static_cast<char *>(uc_host_pinned_alloc(cap));
+        b.cap = cap;
+        b.pinned = b.p != nullptr;
+        if (!b.p) b.p = new (std::nothrow) char[cap];
+        if (!b.p) b.cap = 0;
+        return b;
+    }
+
+    void put(HostBuf &b)
+    {
+        if (b.p && !b.pinned) delete[] b.p;
+        if (b.p && b.pinned) {
+            std::unique_lock<std::mutex> lk(mu_);
+            if (free_bytes_ + b.cap <= KEEP) {
+                free_.emplace(b.cap, b.p);
+                free_bytes_ += b.cap;
+            } else {
+                lk.unlock();
+                uc_host_pinned_free(b.p);
+            }
+        }
+        b = HostBuf{};
+    }
+
+    ~HostPool()
+    {
+        for (auto &kv : free_) uc_host_pinned_free(kv.second);
+    }
+
+private:
+    static uint64_t round_up(uint64_t bytes)
+    {
+        if (bytes <= (64u << 10)) return 64u << 10;
+        uint64_t p2 = 1;
+        while (p2 <= bytes / 2) p2 <<= 1;  // largest power of two <= bytes
+        const uint64_t step = p2 / 8;
+        return (bytes + step - 1) / step * step;
+    }
+
+    std::mutex mu_;
+    std::multimap<uint64_t, char *> free_;
+    uint64_t free_bytes_ = 0;
+};
+
 struct Job {
     uint64_t id = 0;
-    std::vector<uint32_t> packed;
+    HostBuf in;  // the k-mers, from the server's pool; given back when the job has run
     uint64_t n = 0;
     uint32_t k = 0, l = 0;
     uint64_t estimate = 0;
     int state = UC_JOB_QUEUED;
     int status = UC_OK;
-    std::unique_ptr<char[]> text;  // (not zero-filled first: the copy writes every byte)
+    HostBuf text;  // from the server's pool; given back when the job is released
     uint64_t text_size = 0;
     Clock::time_point submitted, started, finished;
 };
@@ -49,7 +126,10 @@ struct Worker {
 // destruction aside), and what the others need to know of it -- the bytes it
 // held when it went idle, the bound while it runs -- is published under `mu`.
 // The true peak of device bytes over all workers comes from the meter the
-// contexts update at every allocation and free.
+// contexts update at every allocation and free.  The host buffer pool has a
+// lock of its own (taken without `mu` held, or after it, never before it), and
+// a job's input buffer belongs to the submitting thread until the job is
+// queued, then to the worker that runs it.
 struct uc_server {
     uc_server_opts opts{};
     std::mutex mu;
@@ -59,6 +139,7 @@ struct uc_server {
     std::vector<Worker> workers;
     std::vector<std::thread> threads;
     uc_meter meter;
+    HostPool bufs;
     uint64_t next_id = 1, done = 0;
     bool stopping = false;
 
@@ -136,23 +217,28 @@ struct uc_server {
             lk.unlock();
 
             if (how == 2) uc_ctx_trim(w.ctx);
-            int rc = uc_compact(w.ctx, j->packed.data(), j->n, j->k, j->l, 0);
-            std::unique_ptr<char[]> text;
+            int rc = uc_compact(w.ctx, reinterpret_cast<const uint32_t *>(j->in.p), j->n, j->k, j->l, 0);
+            bufs.put(j->in);
+            HostBuf text;
             uint64_t text_size = 0;
             if (rc == UC_OK) {
                 text_size = uc_text_size(w.ctx);
-                text.reset(new (std::nothrow) char[text_size ? text_size : 1]);
-                rc = text ? uc_text_copy(w.ctx, text.get(), text_size) : UC_E_NOMEM;
+                text = bufs.get(text_size);
+                if (!text.p)
+                    rc = UC_E_NOMEM;
+                else
+                    rc = text.pinned ? uc_text_copy_pinned(w.ctx, text.p, text_size)
+                                     : uc_text_copy(w.ctx, text.p, text_size);
+                if (rc != UC_OK) bufs.put(text);
             }
             const uint64_t held_now = uc_ctx_held_bytes(w.ctx);
 
             lk.lock();
-            j->text.swap(text);
+            j->text = text;
             j->text_size = rc == UC_OK ? text_size : 0;
             j->status = rc;
             j->state = rc == UC_OK ? UC_JOB_DONE : UC_JOB_FAILED;
             j->finished = Clock::now();
-            std::vector<uint32_t>().swap(j->packed);
             done++;
             w.busy = false;
             w.reserved = 0;
@@ -203,7 +289,11 @@ int uc_server_destroy(uc_server *s)
     s->cv_work.notify_all();
     for (auto &t : s->threads) t.join();
     for (auto &w : s->workers) uc_ctx_destroy(w.ctx);
-    for (auto &kv : s->jobs) delete kv.second;
+    for (auto &kv : s->jobs) {
+        s->bufs.put(kv.second->in);
+        s->bufs.put(kv.second->text);
+        delete kv.second;
+    }
     delete s;
     return UC_OK;
 }
@@ -211,16 +301,26 @@ int uc_server_destroy(uc_server *s)
 int uc_submit(uc_server *s, const uint32_t *packed, uint64_t n, uint32_t k, uint32_t l, uint64_t *job)
 {
     if (!s || !job || (!packed && n) || k < 2 || k > 64) return UC_E_ARG;
-    Job *j = new Job();
     const uint64_t words = n * ((k + 15) / 16);
-    j->packed.assign(packed, packed + words);
+    const uint64_t estimate = uc_job_bytes_estimate(n, k);
+    const bool over = s->opts.hbm_budget && estimate > s->opts.hbm_budget;
+    Job *j = new Job();
+    if (!over) {
+        // the caller's k-mers are copied before submission returns
+        j->in = s->bufs.get(words * 4);
+        if (!j->in.p) {
+            delete j;
+            return UC_E_NOMEM;
+        }
+        if (words) std::memcpy(j->in.p, packed, words * 4);
+    }
     j->n = n;
     j->k = k;
     j->l = l;
-    j->estimate = uc_job_bytes_estimate(n, k);
+    j->estimate = estimate;
     j->submitted = Clock::now();
     std::lock_guard<std::mutex> lk(s->mu);
-    if (s->opts.hbm_budget && j->estimate > s->opts.hbm_budget) {
+    if (over) {
         j->state = UC_JOB_FAILED;
         j->status = UC_E_NOMEM;
         j->started = j->finished = j->submitted;
@@ -267,19 +367,24 @@ int uc_result(uc_server *s, uint64_t job, char *dst, uint64_t cap, uint64_t *siz
     if (queue_ms) *queue_ms = std::chrono::duration<double, std::milli>(j->started - j->submitted).count();
     if (run_ms) *run_ms = std::chrono::duration<double, std::milli>(j->finished - j->started).count();
     if (j->status != UC_OK) return j->status;
-    if (dst && cap >= j->text_size && j->text_size) std::memcpy(dst, j->text.get(), j->text_size);
+    if (dst && cap >= j->text_size && j->text_size) std::memcpy(dst, j->text.p, j->text_size);
     return UC_OK;
 }
 
 int uc_release(uc_server *s, uint64_t job)
 {
     if (!s) return UC_E_ARG;
-    std::lock_guard<std::mutex> lk(s->mu);
-    auto it = s->jobs.find(job);
-    if (it == s->jobs.end()) return UC_E_ARG;
-    if (it->second->state != UC_JOB_DONE && it->second->state != UC_JOB_FAILED) return UC_E_STATE;
-    delete it->second;
-    s->jobs.erase(it);
+    Job *j = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->jobs.find(job);
+        if (it == s->jobs.end()) return UC_E_ARG;
+        if (it->second->state != UC_JOB_DONE && it->second->state != UC_JOB_FAILED) return UC_E_STATE;
+        j = it->second;
+        s->jobs.erase(it);
+    }
+    s->bufs.put(j->text);  // (outside the server's lock: a release of page-locked memory may wait)
+    delete j;
     return UC_OK;
 }
 
