@@ -516,23 +516,26 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t *a, uint32_t n, uin
 // STAGE, the tile's placements are gathered in LDS in bucket order and leave
 // as runs of consecutive slots; a tile with more placements than the stage
 // holds (more than a quarter of its k-mers crossing) stores them directly.
-__host__ __device__ constexpr size_t lmin_lds_bytes(uint32_t B, uint32_t nw = WAVES)
+// LDS of one pass-1 tile: the tile's start and global offset of every bucket,
+// then a 16-bit placement count per bucket (and crossing count) for each of
+// the tile's ng = tile / 64 (item, wave) groups
+__host__ __device__ constexpr size_t lmin_lds_bytes(uint32_t B, uint32_t ng)
 {
-    return ((size_t)(B + 1) * (nw + 1) + 2 * (size_t)B) * 4;
+    return 2 * (size_t)B * 4 + ((size_t)(B + 1) * ng * 2 + 3) / 4 * 4;
 }
 
 // placements one pass-1 tile may stage: its k-mers plus a quarter
 __host__ __device__ constexpr uint32_t lmin_stage_cap(uint32_t tile) { return tile + tile / 4; }
 
-__host__ __device__ constexpr size_t lmin_stage_off(uint32_t B, uint32_t nw = WAVES)
+__host__ __device__ constexpr size_t lmin_stage_off(uint32_t B, uint32_t ng)
 {
-    return (lmin_lds_bytes(B, nw) + 15) & ~(size_t)15;
+    return (lmin_lds_bytes(B, ng) + 15) & ~(size_t)15;
 }
 
 template <int W>
-__host__ __device__ constexpr size_t lmin_stage_bytes(uint32_t B, uint32_t tile, bool keep, uint32_t nw = WAVES)
+__host__ __device__ constexpr size_t lmin_stage_bytes(uint32_t B, uint32_t tile, bool keep)
 {
-    return lmin_stage_off(B, nw) + (size_t)lmin_stage_cap(tile) * (8 * W + 2 + 2 + (keep ? 4 : 0));
+    return lmin_stage_off(B, tile / 64) + (size_t)lmin_stage_cap(tile) * (8 * W + 2 + 2 + (keep ? 4 : 0));
 }
 
 // A pass-1 slot's fine index (below MAX_SUB) and its crossing flag share one
@@ -557,13 +560,15 @@ __global__ __launch_bounds__(NT) void k_lmin_scatter(const uint64_t *__restrict_
                                                         uint16_t *__restrict__ sub, uint32_t n_tiles)
 {
     constexpr int NW = NT / 64;
+    constexpr int NG = IPT * NW;  // (item, wave) groups, in the tile's order of k-mers
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ uint32_t wsum[NW];
     const bool keep = src != nullptr;
-    uint32_t *run = reinterpret_cast<uint32_t *>(lds);  // B+1 running counts
-    uint32_t *wcnt = run + (B + 1);                      // NW x (B+1)
-    uint32_t *tstart = wcnt + NW * (B + 1);
-    uint32_t *delta = tstart + B;
+    uint32_t *tstart = reinterpret_cast<uint32_t *>(lds);  // bucket starts inside the tile
+    uint32_t *delta = tstart + B;                          // slot of the tile's first placement of each bucket - start
+    // placements per group and bucket (column B: crossing k-mers), then their
+    // exclusive prefixes down each column
+    uint16_t *gcnt = reinterpret_cast<uint16_t *>(delta + B);
     const uint32_t ti = xcd_tile(n_tiles);
     if (ti == NONE) return;
     const BTile t = tiles[ti];
@@ -579,7 +584,10 @@ __global__ __launch_bounds__(NT) void k_lmin_scatter(const uint64_t *__restrict_
             xs[j] = load_kmer<W>(kin, t.kbegin + idx, top_mask);
         }
     }
-    for (uint32_t i = threadIdx.x; i < (B + 1) * (NW + 1); i += NT) run[i] = 0;
+    {
+        uint32_t *g32 = reinterpret_cast<uint32_t *>(gcnt);
+        for (uint32_t i = threadIdx.x; i < ((B + 1) * NG + 1) / 2; i += NT) g32[i] = 0;
+    }
     // this tile's count per bucket is the next histogram entry minus its own
     for (uint32_t v = threadIdx.x; v < B; v += NT) {
         const uint64_t hi = t.hbase + (uint64_t)v * t.ts + t.tlocal;
@@ -590,33 +598,23 @@ __global__ __launch_bounds__(NT) void k_lmin_scatter(const uint64_t *__restrict_
     __syncthreads();
     block_excl_scan_n<NT>(tstart, B, wsum);
     for (uint32_t v = threadIdx.x; v < B; v += NT) delta[v] -= tstart[v];
-    __syncthreads();
     const int lane = __lane_id(), w = threadIdx.x / 64;
     const uint64_t lt = (1ull << lane) - 1ull, me = 1ull << lane;
-    const uint32_t xbase = xscan[ti];
-    // staged placements: k-mers, fine index with flags, bucket (and sources)
-    constexpr uint32_t CAP = lmin_stage_cap(IPT * NT);
-    Kmer<W> *skm = reinterpret_cast<Kmer<W> *>(lds + lmin_stage_off(B, NW));
-    uint16_t *ssub = reinterpret_cast<uint16_t *>(skm + CAP);
-    uint16_t *sbk = ssub + CAP;
-    uint32_t *ssrc = reinterpret_cast<uint32_t *>(sbk + CAP);  // 4-byte aligned: CAP is even
-    const uint32_t placed = t.count + (xscan[ti + 1] - xbase);
-    const bool staged = STAGE && placed <= CAP;  // uniform over the block
+    // 1. every item's rank among its wave's placements of the same bucket
+    // (homes before borrows within a lane), and each wave's count per bucket
+    uint32_t rh[IPT], rb[IPT], rc[IPT];
 #pragma unroll
     for (int j = 0; j < IPT; j++) {
+        rh[j] = rb[j] = rc[j] = 0;
         if ((uint32_t)j * NT >= t.count) break;  // uniform over the block
         const uint32_t idx = (uint32_t)j * NT + threadIdx.x;
         const bool valid = idx < t.count;
-        const uint64_t g = t.kbegin + idx;
-        uint32_t bp = 0, bs = 0, fp = 0, fs = 0;
+        uint32_t bp = 0, bs = 0;
         bool cross = false;
         if (valid) {
-            const uint2 c = codes[j];
-            fp = c.x >> nbits;  // fine index inside the coarse bucket
-            fs = c.y >> nbits;
-            bp = c.x & (B - 1);
-            bs = c.y & (B - 1);
-            cross = c.x != c.y;
+            bp = codes[j].x & (B - 1);
+            bs = codes[j].y & (B - 1);
+            cross = codes[j].x != codes[j].y;
         }
         const uint64_t V = __ballot(valid), C = __ballot(cross);
         uint64_t BP[10], BS[10];
@@ -626,74 +624,95 @@ __global__ __launch_bounds__(NT) void k_lmin_scatter(const uint64_t *__restrict_
             BS[q] = (uint32_t)q < nbits ? (uint64_t)__ballot(cross && ((bs >> q) & 1u)) : 0ull;
         }
         // placements of lower lanes in the same bucket: homes (hp*) and borrows (hs*)
-        uint64_t hpH = 0, hsH = 0, hpB = 0, hsB = 0;
+        uint16_t *gc = gcnt + (size_t)(j * NW + w) * (B + 1);
         if (valid) {
-            match_bucket(bp, nbits, V, C, BP, BS, hpH, hsH);
-            const uint64_t M = hpH | hsH;
-            if ((M & (0ull - M)) == me) wcnt[w * (B + 1) + bp] = (uint32_t)(__popcll(hpH) + __popcll(hsH));
+            uint64_t hp = 0, hs = 0;
+            match_bucket(bp, nbits, V, C, BP, BS, hp, hs);
+            const uint64_t M = hp | hs;
+            if ((M & (0ull - M)) == me) gc[bp] = (uint16_t)(__popcll(hp) + __popcll(hs));
+            rh[j] = (uint32_t)(__popcll(hp & lt) + __popcll(hs & lt));
         }
         if (cross) {
-            match_bucket(bs, nbits, V, C, BP, BS, hpB, hsB);
-            const uint64_t M = hpB | hsB;
-            if ((M & (0ull - M)) == me) wcnt[w * (B + 1) + bs] = (uint32_t)(__popcll(hpB) + __popcll(hsB));
+            uint64_t hp = 0, hs = 0;
+            match_bucket(bs, nbits, V, C, BP, BS, hp, hs);
+            const uint64_t M = hp | hs;
+            if ((M & (0ull - M)) == me) gc[bs] = (uint16_t)(__popcll(hp) + __popcll(hs));
+            rb[j] = (uint32_t)(__popcll(hp & lt) + __popcll(hs & lt)) + (bp == bs ? 1u : 0u);
+            rc[j] = (uint32_t)__popcll(C & lt);
         }
-        if (C && (C & (0ull - C)) == me) wcnt[w * (B + 1) + B] = (uint32_t)__popcll(C);
-        __syncthreads();
-        if (valid) {
-            const Kmer<W> x = xs[j];
-            uint32_t lh = tstart[bp] + run[bp] + (uint32_t)(__popcll(hpH & lt) + __popcll(hsH & lt));
-            for (int q = 0; q < w; q++) lh += wcnt[q * (B + 1) + bp];
-            const uint32_t wh = fp | ((cross ? (uint32_t)F_LENT : 0u) << SUB_FLAG_SHIFT);
+        if (C && (C & (0ull - C)) == me) gc[B] = (uint16_t)__popcll(C);
+    }
+    __syncthreads();
+    // 2. each column's exclusive prefix over the groups in tile order
+    // (at most 2 * tile placements per bucket: 16 bits hold them)
+    for (uint32_t v = threadIdx.x; v <= B; v += NT) {
+        uint32_t acc = 0;
+#pragma unroll 8
+        for (int g = 0; g < NG; g++) {
+            const uint32_t c = gcnt[(size_t)g * (B + 1) + v];
+            gcnt[(size_t)g * (B + 1) + v] = (uint16_t)acc;
+            acc += c;
+        }
+    }
+    __syncthreads();
+    // 3. placements: k-mers, fine index with flags, bucket (and sources)
+    const uint32_t xbase = xscan[ti];
+    constexpr uint32_t CAP = lmin_stage_cap(IPT * NT);
+    Kmer<W> *skm = reinterpret_cast<Kmer<W> *>(lds + lmin_stage_off(B, NG));
+    uint16_t *ssub = reinterpret_cast<uint16_t *>(skm + CAP);
+    uint16_t *sbk = ssub + CAP;
+    uint32_t *ssrc = reinterpret_cast<uint32_t *>(sbk + CAP);  // 4-byte aligned: CAP is even
+    const uint32_t placed = t.count + (xscan[ti + 1] - xbase);
+    const bool staged = STAGE && placed <= CAP;  // uniform over the block
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+        const uint32_t idx = (uint32_t)j * NT + threadIdx.x;
+        if (idx >= t.count) continue;
+        const uint64_t g = t.kbegin + idx;
+        const uint2 c = codes[j];
+        const uint32_t fp = c.x >> nbits, fs = c.y >> nbits;  // fine indices inside the coarse buckets
+        const uint32_t bp = c.x & (B - 1), bs = c.y & (B - 1);
+        const bool cross = c.x != c.y;
+        const uint16_t *gc = gcnt + (size_t)(j * NW + w) * (B + 1);
+        const Kmer<W> x = xs[j];
+        const uint32_t lh = tstart[bp] + gc[bp] + rh[j];
+        const uint32_t wh = fp | ((cross ? (uint32_t)F_LENT : 0u) << SUB_FLAG_SHIFT);
+        if (staged) {
+            skm[lh] = x;
+            ssub[lh] = (uint16_t)wh;
+            sbk[lh] = (uint16_t)bp;
+            if (keep) ssrc[lh] = (uint32_t)g;
+        } else {
+            const uint32_t ns = delta[bp] + lh;
+            store_kmer<W>(kout, ns, x);
+            if (flags) flags[ns] = (uint8_t)(wh >> SUB_FLAG_SHIFT);
+            if (sub) sub[ns] = (uint16_t)wh;
+            if (keep) src[ns] = (uint32_t)g;
+        }
+        if (cross) {
+            const uint32_t lb = tstart[bs] + gc[bs] + rb[j];
+            const uint32_t wb = fs | ((uint32_t)F_BORROW << SUB_FLAG_SHIFT);
             if (staged) {
-                skm[lh] = x;
-                ssub[lh] = (uint16_t)wh;
-                sbk[lh] = (uint16_t)bp;
-                if (keep) ssrc[lh] = (uint32_t)g;
+                skm[lb] = x;
+                ssub[lb] = (uint16_t)wb;
+                sbk[lb] = (uint16_t)bs;
+                if (keep) ssrc[lb] = (uint32_t)g;
             } else {
-                const uint32_t ns = delta[bp] + lh;
+                const uint32_t ns = delta[bs] + lb;
                 store_kmer<W>(kout, ns, x);
-                if (flags) flags[ns] = (uint8_t)(wh >> SUB_FLAG_SHIFT);
-                if (sub) sub[ns] = (uint16_t)wh;
+                if (flags) flags[ns] = F_BORROW;
+                if (sub) sub[ns] = (uint16_t)wb;
                 if (keep) src[ns] = (uint32_t)g;
             }
-            if (cross) {
-                uint32_t lb = tstart[bs] + run[bs] + (uint32_t)(__popcll(hpB & lt) + __popcll(hsB & lt)) +
-                              (bp == bs ? 1u : 0u);
-                for (int q = 0; q < w; q++) lb += wcnt[q * (B + 1) + bs];
-                const uint32_t wb = fs | ((uint32_t)F_BORROW << SUB_FLAG_SHIFT);
-                if (staged) {
-                    skm[lb] = x;
-                    ssub[lb] = (uint16_t)wb;
-                    sbk[lb] = (uint16_t)bs;
-                    if (keep) ssrc[lb] = (uint32_t)g;
-                } else {
-                    const uint32_t ns = delta[bs] + lb;
-                    store_kmer<W>(kout, ns, x);
-                    if (flags) flags[ns] = F_BORROW;
-                    if (sub) sub[ns] = (uint16_t)wb;
-                    if (keep) src[ns] = (uint32_t)g;
-                }
-                uint32_t px = run[B] + (uint32_t)__popcll(C & lt);
-                for (int q = 0; q < w; q++) px += wcnt[q * (B + 1) + B];
-                pair_home[xbase + px] = delta[bp] + lh;
-                pair_borrow[xbase + px] = delta[bs] + lb;
-            }
+            const uint32_t px = gc[B] + rc[j];
+            pair_home[xbase + px] = delta[bp] + lh;
+            pair_borrow[xbase + px] = delta[bs] + lb;
         }
-        __syncthreads();
-        for (uint32_t v = threadIdx.x; v <= B; v += NT) {
-            uint32_t a = 0;
-#pragma unroll
-            for (int q = 0; q < NW; q++) {
-                a += wcnt[q * (B + 1) + v];
-                wcnt[q * (B + 1) + v] = 0;
-            }
-            run[v] += a;
-        }
-        __syncthreads();
     }
     // staged: slot li of the tile's bucket-ordered placements goes to
     // delta[bucket] + li, so consecutive li of one bucket are consecutive slots
     if (staged) {
+        __syncthreads();
         for (uint32_t li = threadIdx.x; li < placed; li += NT) {
             const uint32_t ns = delta[sbk[li]] + li;
             store_kmer<W>(kout, ns, skm[li]);
@@ -3720,10 +3739,10 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     // staged form holds four tiles per CU, and 8 waves each hide more latency)
     constexpr int NT1 = 512;
     const bool stage1 = !(c->opts.flags & UC_FLAG_DIRECT_SCATTER) &&
-                        lmin_stage_bytes<W>(B, tile1, keep_src, NT1 / 64) <= c->lds_limit;
+                        lmin_stage_bytes<W>(B, tile1, keep_src) <= c->lds_limit;
 #define UC_S1(ST, C32)                                                                                              \
     do {                                                                                                           \
-        const size_t lds = ST ? lmin_stage_bytes<W>(B, tile1, keep_src, NT1 / 64) : lmin_lds_bytes(B, NT1 / 64);      \
+        const size_t lds = ST ? lmin_stage_bytes<W>(B, tile1, keep_src) : lmin_lds_bytes(B, tile1 / 64);              \
         CHK(allow_lds(c, k_lmin_scatter<W, IPT1 * BLOCK / NT1, ST, NT1, C32>, lds));                                \
         KLAUNCH(c, "k_lmin_scatter", (k_lmin_scatter<W, IPT1 * BLOCK / NT1, ST, NT1, C32>), (xcd_grid(T), NT1, lds), \
                 kin, P<BTile>(c->tiles), B, nbits, c->top_mask, c->bkt.p, P<uint32_t>(c->hist),                     \
