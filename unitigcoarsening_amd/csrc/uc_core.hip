@@ -2307,6 +2307,9 @@ __device__ __forceinline__ void put_first(char *o, uint64_t at, const Kmer<W> &x
 template <int W>
 constexpr uint32_t emit_stage() { return W == 1 ? 2048 : 4096; }
 
+// LDS index of slot i of an emission tile (one pad word after every 8)
+__device__ __forceinline__ uint32_t tix(uint32_t i) { return i + (i >> 3); }
+
 // per tile: exclusive prefix of (1 << 32 | length) over heads -> text
 // offsets.  Every head writes its first run (put_first).  The 64 slots a
 // wave takes at a time own one contiguous text range; when the range fits
@@ -2329,7 +2332,12 @@ __global__ __launch_bounds__(NT) void k_head_emit(const uint64_t *__restrict__ k
     constexpr uint32_t STG = emit_stage<W>();
     if (gate && !pass_settled(gate, n_kmers)) return;  // (uniform over the grid)
     constexpr int NW = NT / 64, CTI = CT_TILE / NT;  // waves; items per thread
-    __shared__ uint64_t tile[CT_TILE + 1];  // prefixes in slot order, then the tile's end
+    // prefixes in slot order, then the tile's end; slot i at tix(i), one pad
+    // word after every 8, so that a thread's 8 consecutive slots (the scan
+    // below) sit 72 bytes from the next thread's: 4-way LDS bank access, as
+    // for any 64 lanes of 8-byte words, where 64 bytes put every lane on one
+    // of two bank pairs
+    __shared__ uint64_t tile[CT_TILE + CT_TILE / 8 + 1];
     __shared__ uint64_t wsum[NW];
     __shared__ __align__(16) char stage[STAGE ? NW : 1][STAGE ? STG : 16];
     const uint32_t ti = xcd_tile(n_tiles);
@@ -2362,13 +2370,13 @@ __global__ __launch_bounds__(NT) void k_head_emit(const uint64_t *__restrict__ k
             hd |= 1u << j;
             if (rec_next(hr[j]) == NONE) v = (1ull << 32) | rec_len(hr[j]);
         }
-        tile[j * NT + threadIdx.x] = v;
+        tile[tix(j * NT + threadIdx.x)] = v;
     }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < CTI; j++) {
         const uint32_t e = (uint32_t)j * NT + threadIdx.x;
-        if (e < nml) tile[ml[j] & 0x7FFu] = (1ull << 32) | mlist_len(ml[j], ulen, t0);
+        if (e < nml) tile[tix(ml[j] & 0x7FFu)] = (1ull << 32) | mlist_len(ml[j], ulen, t0);
     }
     __syncthreads();
     uint64_t loc[CTI];
@@ -2376,7 +2384,7 @@ __global__ __launch_bounds__(NT) void k_head_emit(const uint64_t *__restrict__ k
 #pragma unroll
     for (int j = 0; j < CTI; j++) {
         loc[j] = run;
-        run += tile[threadIdx.x * CTI + j];
+        run += tile[tix(threadIdx.x * CTI + j)];
     }
     const uint64_t incl = wave_incl_scan(run);
     const int w = threadIdx.x / 64;
@@ -2386,8 +2394,8 @@ __global__ __launch_bounds__(NT) void k_head_emit(const uint64_t *__restrict__ k
     for (int i = 0; i < w; i++) wpre += wsum[i];
     const uint64_t excl = tile_off[ti] + wpre + incl - run;
 #pragma unroll
-    for (int j = 0; j < CTI; j++) tile[threadIdx.x * CTI + j] = excl + loc[j];
-    if (threadIdx.x == NT - 1) tile[CT_TILE] = excl + run;
+    for (int j = 0; j < CTI; j++) tile[tix(threadIdx.x * CTI + j)] = excl + loc[j];
+    if (threadIdx.x == NT - 1) tile[tix(CT_TILE)] = excl + run;
     __syncthreads();
     char *stg = stage[STAGE ? w : 0];
     const uint32_t lane = __lane_id();
@@ -2408,11 +2416,11 @@ __global__ __launch_bounds__(NT) void k_head_emit(const uint64_t *__restrict__ k
             nr = rec[s + NT];
         }
         const bool h = (hd >> j) & 1u;
-        const uint64_t pre = tile[j * NT + threadIdx.x];
+        const uint64_t pre = tile[tix(j * NT + threadIdx.x)];
         const uint64_t off = (pre & 0xFFFFFFFFull) + (pre >> 32) * k;
         // the wave's text range [r0, r1): from its first slot to the slot after its last
         const uint32_t li = j * NT + (uint32_t)w * 64;
-        const uint64_t p0 = tile[li], p1 = tile[li + 64];
+        const uint64_t p0 = tile[tix(li)], p1 = tile[tix(li + 64)];
         const uint64_t r0 = (p0 & 0xFFFFFFFFull) + (p0 >> 32) * k, r1 = (p1 & 0xFFFFFFFFull) + (p1 >> 32) * k;
         const uint64_t a = r0 & ~15ull;  // staged from r0's 16-byte line
         if (STAGE && r1 - a <= STG) {  // uniform over the wave
@@ -2443,7 +2451,7 @@ __global__ __launch_bounds__(NT) void k_head_emit(const uint64_t *__restrict__ k
         if (e >= nml) break;
         const uint32_t ls = (uint32_t)(ml[j] & 0x7FFu);
         const uint32_t s = (uint32_t)t0 + ls;
-        const uint64_t pre = tile[ls];
+        const uint64_t pre = tile[tix(ls)];
         const uint64_t off = (pre & 0xFFFFFFFFull) + (pre >> 32) * k;
         uint64_t r = rec[s];
         uint64_t p = off + k + rec_len(r) - 1;
