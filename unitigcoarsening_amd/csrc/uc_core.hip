@@ -754,10 +754,12 @@ __global__ void k_plan1_tiles(const uint64_t *__restrict__ set_off, const uint64
 
 // bucket row starts: boff[set*B + b] = scanned histogram at the set's first tile
 // (and the same rows into boff2 when given: the fine rows of a job without
-// a fine sort)
+// a fine sort).  The slot count is *mdev when given (the host has not read
+// it), else m.
 __global__ void k_gather_boff(const uint32_t *__restrict__ hscan, const uint64_t *__restrict__ set_hbase,
                               const uint32_t *__restrict__ set_ts, uint32_t n_sets, uint32_t B, uint32_t m,
-                              uint32_t *__restrict__ boff, uint32_t *__restrict__ boff2)
+                              const uint32_t *__restrict__ mdev, uint32_t *__restrict__ boff,
+                              uint32_t *__restrict__ boff2)
 {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t G = n_sets * B;
@@ -766,7 +768,7 @@ __global__ void k_gather_boff(const uint32_t *__restrict__ hscan, const uint64_t
         const uint32_t s = g / B, b = g % B;
         v = hscan[set_hbase[s] + (uint64_t)b * set_ts[s]];
     } else if (g == G) {
-        v = m;
+        v = mdev ? *mdev : m;
     } else {
         return;
     }
@@ -944,7 +946,7 @@ __global__ __launch_bounds__(BLOCK) void k_sub_colscan(uint32_t *__restrict__ hi
 __global__ __launch_bounds__(BLOCK) void k_sub_rows(const uint32_t *__restrict__ colsum,
                                                     const uint32_t *__restrict__ cb_first,
                                                     const uint32_t *__restrict__ boff, uint32_t G, uint32_t m,
-                                                    uint32_t *__restrict__ fboff)
+                                                    const uint32_t *__restrict__ mdev, uint32_t *__restrict__ fboff)
 {
     __shared__ uint32_t tot[MAX_SUB];
     __shared__ uint32_t wsum[WAVES];
@@ -954,7 +956,7 @@ __global__ __launch_bounds__(BLOCK) void k_sub_rows(const uint32_t *__restrict__
     __syncthreads();
     block_excl_scan(tot, S, wsum);
     for (uint32_t v = threadIdx.x; v < S; v += BLOCK) fboff[f0 + v] = boff[g] + tot[v];
-    if (g == G - 1 && threadIdx.x == 0) fboff[cb_first[G]] = m;
+    if (g == G - 1 && threadIdx.x == 0) fboff[cb_first[G]] = mdev ? *mdev : m;
 }
 
 // Pass-2 scatter with LDS staging.  A planned tile (up to ROUNDS rounds of
@@ -1123,11 +1125,13 @@ __global__ __launch_bounds__(NT) void k_sub_scatter(const BTile *__restrict__ ti
     }
 }
 
+// (the pair count is *xdev when given: the host has not read it, and x is
+// then its bound)
 __global__ void k_pair_remap(uint32_t *__restrict__ pair_home, uint32_t *__restrict__ pair_borrow, uint32_t x,
-                             const uint32_t *__restrict__ perm)
+                             const uint32_t *__restrict__ xdev, const uint32_t *__restrict__ perm)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= x) return;
+    if (i >= x || (xdev && i >= *xdev)) return;
     pair_home[i] = perm[pair_home[i]];
     pair_borrow[i] = perm[pair_borrow[i]];
 }
@@ -1438,12 +1442,16 @@ __global__ __launch_bounds__(BLOCK) void k_plan_count(const uint32_t *__restrict
 }
 
 // per coarse bucket: its groups from its scanned first entry
+// (and, when given, a copy of two counters beside the offsets, for the same
+// host read)
 __global__ __launch_bounds__(BLOCK) void k_plan_write(const uint32_t *__restrict__ fboff,
                                                       const uint32_t *__restrict__ cb_first, uint32_t gmax,
                                                       const uint32_t *__restrict__ gfirst,
-                                                      EGroup *__restrict__ groups)
+                                                      EGroup *__restrict__ groups, const uint32_t *__restrict__ tot,
+                                                      uint32_t *__restrict__ tot_out)
 {
     __shared__ uint32_t wsum[WAVES];
+    if (tot && blockIdx.x == 0 && threadIdx.x < 2) tot_out[threadIdx.x] = tot[threadIdx.x];
     const uint32_t g = blockIdx.x, f0 = cb_first[g], S = cb_first[g + 1] - f0;
     const uint32_t v0 = PLAN_RUN * threadIdx.x, v1 = v0 + PLAN_RUN < S ? v0 + PLAN_RUN : S;
     const uint32_t ng = v0 < S ? pack_groups(fboff, f0, v0, v1, gmax, nullptr, nullptr, nullptr) : 0u;
@@ -2753,6 +2761,10 @@ struct uc_ctx {
     bool deferred = false;
     // a low-memory emission wrote head text offsets over the head lists
     bool lists_spent = false;
+    // whole small jobs: bucketing leaves the placement totals on the device
+    // (lazy_sizes), and m and x hold their bounds until the edge plan's read
+    // (sizes_pending)
+    bool lazy_sizes = false, sizes_pending = false;
     char *pin[2] = {nullptr, nullptr};
     hipEvent_t pin_ev[2] = {nullptr, nullptr};
     bool pin_used[2] = {false, false};
@@ -3258,6 +3270,7 @@ void reset_job(uc_ctx *c)
 {
     c->loaded = c->bucketed = c->edged = c->coarsened = c->emitted = false;
     c->edges_ready = c->no_slices = c->deferred = c->lists_spent = false;
+    c->lazy_sizes = c->sizes_pending = false;
     c->n_unitigs = c->n_cycles = c->text_bytes = c->n_splitters = 0;
     c->chain_rounds = 0;
     c->m = c->x = c->largest = 0;
@@ -3725,11 +3738,24 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
                             cnt + C_TOT));
     CHK(scan_excl<uint32_t>(c, P<uint32_t>(c->xhist), P<uint32_t>(c->xhist), T, P<uint32_t>(c->xhist) + T,
                             c->scan_tmp, cnt + C_TOT + 1));
-    uint32_t tot[2];
-    CHK(d2h(c, tot, cnt + C_TOT, 8));
-    CHK(check_placements(c, c->n, tot[1]));
-    c->x = tot[1];
-    c->m = c->n + c->x;
+    // A whole small job (lazy_sizes) does not read the totals here: the
+    // buffers and grids below take the bounds x <= n and m <= 2n, the kernels
+    // that need m or x read them on the device, and the host reads them with
+    // the edge plan.
+    const bool lazy = c->lazy_sizes;
+    const uint32_t *mdev = lazy ? cnt + C_TOT : nullptr;
+    if (lazy) {
+        c->x = c->n;
+        c->m = 2 * c->n;
+        c->sizes_pending = true;
+    } else {
+        uint32_t tot[2];
+        CHK(d2h(c, tot, cnt + C_TOT, 8));
+        CHK(check_placements(c, c->n, tot[1]));
+        c->x = tot[1];
+        c->m = c->n + c->x;
+        c->sizes_pending = false;
+    }
     // pass 1 writes the final k-mers and flags when no set is split into
     // fine buckets, else the k-mers and the fine index with the flag bits
     DBuf &k1 = any_fine ? c->kmer2 : c->kmer, &s1 = any_fine ? c->src2 : c->src;
@@ -3769,7 +3795,7 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
     // the fine rows are the coarse rows when no set has a fine sort
     CHK(ensure(c, c->fboff, (F + 1) * 4));
     KLAUNCH(c, "k_gather_boff", k_gather_boff, (grid_for(G + 1), BLOCK, 0), P<uint32_t>(c->hist),
-            sethb, setts, S, B, (uint32_t)c->m, P<uint32_t>(c->boff), any_fine ? nullptr : P<uint32_t>(c->fboff));
+            sethb, setts, S, B, (uint32_t)c->m, mdev, P<uint32_t>(c->boff), any_fine ? nullptr : P<uint32_t>(c->fboff));
     c->rows_valid = false;
     if (c->opts.flags & UC_FLAG_LOW_MEMORY) {
         // the codes, the input and the tile histogram are read by pass 1 and
@@ -3824,7 +3850,7 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
         KLAUNCH(c, "k_sub_colscan", k_sub_colscan, ((uint32_t)(G * cpb), BLOCK, 0), P<uint32_t>(c->hist2),
                 P<uint64_t>(c->cbhb), P<uint32_t>(c->cbts), P<uint32_t>(c->cbfirst), cpb, P<uint32_t>(c->colsum));
         KLAUNCH(c, "k_sub_rows", k_sub_rows, ((uint32_t)G, BLOCK, 0), P<uint32_t>(c->colsum), P<uint32_t>(c->cbfirst),
-                P<uint32_t>(c->boff), (uint32_t)G, (uint32_t)c->m, P<uint32_t>(c->fboff));
+                P<uint32_t>(c->boff), (uint32_t)G, (uint32_t)c->m, mdev, P<uint32_t>(c->fboff));
         CHK(ensure(c, c->kmer, std::max<uint64_t>(c->m, 1) * W * 8));
         CHK(ensure(c, c->flags, c->m + 64));
         if (keep_src) CHK(ensure(c, c->src, c->m * 4));
@@ -3844,7 +3870,7 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
 #undef UC_S2
         if (c->x)
             KLAUNCH(c, "k_pair_remap", k_pair_remap, (grid_for(c->x), BLOCK, 0), P<uint32_t>(c->pair_h),
-                    P<uint32_t>(c->pair_b), (uint32_t)c->x, P<uint32_t>(c->perm));
+                    P<uint32_t>(c->pair_b), (uint32_t)c->x, lazy ? cnt + C_TOT + 1 : nullptr, P<uint32_t>(c->perm));
         // pass 2's input and the permutation are dead now
         CHK(drop(c, c->kmer2));
         CHK(drop(c, c->sub));
@@ -4001,7 +4027,7 @@ int prepare_edges(uc_ctx *c)
     // groups are entries [gfirst[g], gfirst[g + 1])
     CHK(ensure(c, c->egroups, std::max<uint64_t>(F, 1) * sizeof(EGroup)));
     CHK(ensure(c, c->over, F * 4 + 4));
-    CHK(ensure(c, c->gfirst, (G + 2) * 4));
+    CHK(ensure(c, c->gfirst, (G + 4) * 4));
     CHK(ensure(c, c->small, C_WORDS * 4));
     uint32_t *cnt = P<uint32_t>(c->small);
     CHK(clear_probes(c));
@@ -4013,10 +4039,19 @@ int prepare_edges(uc_ctx *c)
         KLAUNCH(c, "k_plan_count", k_plan_count, ((uint32_t)G, BLOCK, 0), P<uint32_t>(c->fboff),
                 P<uint32_t>(c->cbfirst), group_max, gfd, P<uint32_t>(c->over), gfd + G + 1);
         CHK(scan_excl<uint32_t>(c, gfd, gfd, G, gfd + G, c->scan_tmp));
+        // (sizes the bucketing left on the device come along: placements, crossings)
+        const bool pend = c->sizes_pending;
         KLAUNCH(c, "k_plan_write", k_plan_write, ((uint32_t)G, BLOCK, 0), P<uint32_t>(c->fboff),
-                P<uint32_t>(c->cbfirst), group_max, gfd, P<EGroup>(c->egroups));
-        gf.resize(G + 2);
-        CHK(d2h(c, gf.data(), gfd, (G + 2) * 4));
+                P<uint32_t>(c->cbfirst), group_max, gfd, P<EGroup>(c->egroups), pend ? cnt + C_TOT : nullptr,
+                gfd + G + 2);
+        gf.resize(G + 4);
+        CHK(d2h(c, gf.data(), gfd, (G + (pend ? 4 : 2)) * 4));
+        if (pend) {
+            CHK(check_placements(c, c->n, gf[G + 3]));
+            c->x = gf[G + 3];
+            c->m = c->n + c->x;
+            c->sizes_pending = false;
+        }
     }
     uint32_t n_over = 0;
     if (G) {
@@ -4665,6 +4700,16 @@ int finish_deferred(uc_ctx *c)
 // scratch or the eager form (which takes every step in the checked order).
 constexpr uint64_t DEFER_TEXT_MAX = 256ull << 20;
 
+// a whole job that may bucket with its placement totals left on the device:
+// the jobs whose contraction counters are deferred too (their buffers are
+// small enough to size for m <= 2n)
+bool lazy_sizes_ok(const uc_ctx *c)
+{
+    const uint64_t bound = c->n * (c->k + 1) + 1;
+    return c->n && !c->arena && !c->text_fixed && !(c->opts.flags & (UC_FLAG_LOW_MEMORY | UC_FLAG_EAGER)) &&
+           bound <= DEFER_TEXT_MAX && (!c->opts.hbm_budget || c->held + 2 * bound <= c->opts.hbm_budget);
+}
+
 bool can_defer(const uc_ctx *c)
 {
     const uint64_t bound = c->n * (c->k + 1) + 1;
@@ -4681,7 +4726,10 @@ int timed_phases(uc_ctx *c, uint32_t l)
     for (auto &e : c->ev)
         if (!e) HIPCHK(c, hipEventCreate(&e));
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-    CHK(do_bucketize(c, l, nullptr));
+    c->lazy_sizes = lazy_sizes_ok(c);
+    const int rb = do_bucketize(c, l, nullptr);
+    c->lazy_sizes = false;
+    CHK(rb);
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
     CHK(do_edges(c, UC_ALL_BUCKETS));
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
