@@ -554,7 +554,7 @@ def main():
     ap.add_argument("--serve-inputs", type=int, default=16, help="distinct job inputs drawn from")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-load", type=float, default=0.7, help="offered load, x one worker's capacity")
-    ap.add_argument("--serve-saturation-jobs", type=int, default=150,
+    ap.add_argument("--serve-saturation-jobs", type=int, default=300,
                     help="jobs of each saturation run (1.5x load, 1 slot and --serve-slots); 0 skips them")
     ap.add_argument("--serve-budget-gb", type=float, default=288.0)
     ap.add_argument("--large", action="store_true",
