@@ -475,6 +475,67 @@ __device__ __forceinline__ void match_bucket(uint32_t v, uint32_t nbits, uint64_
     }
 }
 
+// block_excl_scan_n for PER = 2 or 4 values per thread at an aligned array:
+// each thread's values move as one 8- or 16-byte word and stay in registers
+// between the two passes (PER consecutive 4-byte words per lane put the lanes
+// of a wave on only 32 / PER banks)
+template <int NT, int PER>
+__device__ uint32_t block_excl_scan_vec(uint32_t *a, uint32_t n, uint32_t *wsum)
+{
+    constexpr int NW = NT / 64;
+    static_assert(PER == 2 || PER == 4, "vector scan of 2 or 4 values per thread");
+    using V = typename std::conditional<PER == 4, uint4, uint2>::type;
+    const uint32_t b0 = threadIdx.x * PER;
+    const bool whole = b0 + PER <= n;
+    uint32_t v[PER];
+    if (whole) {
+        const V x = *reinterpret_cast<const V *>(a + b0);
+        v[0] = x.x;
+        v[1] = x.y;
+        if constexpr (PER == 4) {
+            v[2] = reinterpret_cast<const uint4 &>(x).z;
+            v[3] = reinterpret_cast<const uint4 &>(x).w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < PER; i++) v[i] = b0 + i < n ? a[b0 + i] : 0u;
+    }
+    uint32_t loc = 0;
+#pragma unroll
+    for (int i = 0; i < PER; i++) loc += v[i];
+    const uint32_t incl = wave_incl_scan(loc);
+    const int w = threadIdx.x / 64;
+    if (__lane_id() == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t pre = incl - loc, tot = 0;
+    for (int q = 0; q < NW; q++) {
+        if (q < w) pre += wsum[q];
+        tot += wsum[q];
+    }
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const uint32_t x = v[i];
+        v[i] = pre;
+        pre += x;
+    }
+    if (whole) {
+        V y;
+        y.x = v[0];
+        y.y = v[1];
+        if constexpr (PER == 4) {
+            reinterpret_cast<uint4 &>(y).z = v[2];
+            reinterpret_cast<uint4 &>(y).w = v[3];
+        }
+        *reinterpret_cast<V *>(a + b0) = y;
+    } else {
+#pragma unroll
+        for (int i = 0; i < PER; i++)
+            if (b0 + i < n) a[b0 + i] = v[i];
+    }
+    __syncthreads();
+    return tot;
+}
+
 // exclusive scan of a[0..n) in place by a block of NT threads (wsum holds
 // NT / 64 words); returns the total
 template <int NT>
@@ -482,6 +543,9 @@ __device__ uint32_t block_excl_scan_n(uint32_t *a, uint32_t n, uint32_t *wsum)
 {
     constexpr int NW = NT / 64;
     const uint32_t per = (n + NT - 1) / NT;
+    // (per is uniform over the block, so every thread takes the same branch)
+    if (per == 4 && !(reinterpret_cast<uintptr_t>(a) & 15)) return block_excl_scan_vec<NT, 4>(a, n, wsum);
+    if (per == 2 && !(reinterpret_cast<uintptr_t>(a) & 7)) return block_excl_scan_vec<NT, 2>(a, n, wsum);
     const uint32_t b0 = threadIdx.x * per, b1 = min(n, b0 + per);
     uint32_t loc = 0;
     for (uint32_t i = b0; i < b1; i++) loc += a[i];
