@@ -37,7 +37,8 @@ int orc_pack(const char *chars, uint64_t n, uint32_t k, uint32_t *packed)
     return ORC_OK;
 }
 
-/* The k-mer text rule of this build (the reference's load_kmers,
+/* The k-mer text rule of the library's parsers (uc_parse_kmer_text,
+ * uc_load_text; the reference's load_kmers,
  * driver.cpp:251-354, on every file where that loop ends; see
  * orc_ref_load_kmers for the loop itself): the first line holds the count
  * (sscanf "%u"/"%lu"), which the reference checks only after whole read

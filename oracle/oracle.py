@@ -224,7 +224,7 @@ def fnv_suf(kmer_words):
 
 
 def parse_text(text, k):
-    """This build's text rule (every line, whatever the count)."""
+    """The library's text rule (every line, whatever the count): uc_parse_kmer_text and uc_load_text."""
     return _parse(lib().orc_parse_text, text, k)
 
 
