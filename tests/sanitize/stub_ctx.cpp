@@ -100,3 +100,5 @@ void *uc_host_pinned_alloc(uint64_t bytes) { return std::malloc(bytes ? bytes : 
 void uc_host_pinned_free(void *p) { std::free(p); }
 
 int uc_text_copy_pinned(uc_ctx *c, char *dst, uint64_t cap) { return uc_text_copy(c, dst, cap); }
+
+void uc_ctx_set_kernel_copies(uc_ctx *, bool) {}

@@ -272,6 +272,33 @@ __global__ void k_fill(T *__restrict__ a, uint64_t n, T v)
         a[i] = v;
 }
 
+// A copy from device memory into page-locked host memory, the host side
+// through its device mapping.  The copies of a server's workers went through
+// the DMA engine one after another, inputs in and texts out, and a worker's
+// small counter read waited behind another worker's text copy; a kernel's
+// copies out overlap the engine's copies in and each other.  16-byte words
+// when both ends allow them, else 4-byte words, else bytes (the branch is
+// uniform).
+__global__ __launch_bounds__(BLOCK) void k_host_copy(const char *__restrict__ src, char *__restrict__ dst,
+                                                     uint64_t bytes)
+{
+    const uint64_t t = (uint64_t)blockIdx.x * BLOCK + threadIdx.x, nt = (uint64_t)gridDim.x * BLOCK;
+    const uintptr_t al = reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst);
+    uint64_t done = 0;
+    if (al % 16 == 0) {
+        const uint64_t nw = bytes / 16;
+        for (uint64_t i = t; i < nw; i += nt)
+            reinterpret_cast<uint4 *>(dst)[i] = reinterpret_cast<const uint4 *>(src)[i];
+        done = nw * 16;
+    } else if (al % 4 == 0) {
+        const uint64_t nw = bytes / 4;
+        for (uint64_t i = t; i < nw; i += nt)
+            reinterpret_cast<uint32_t *>(dst)[i] = reinterpret_cast<const uint32_t *>(src)[i];
+        done = nw * 4;
+    }
+    for (uint64_t i = done + t; i < bytes; i += nt) dst[i] = src[i];
+}
+
 // reference packing with an odd number of 32-bit words -> 64-bit words
 __global__ void k_widen(const uint32_t *__restrict__ in, uint64_t n, uint32_t rw, uint32_t W, uint64_t *__restrict__ out)
 {
@@ -2816,6 +2843,9 @@ struct uc_ctx {
     // memory, which HIP stages (measured on MI355X: 14.8 against 22.0 us per
     // read, launch and wait included)
     char *rbuf = nullptr;
+    char *rbuf_dev = nullptr;  // its device mapping
+    // copies into page-locked host memory by k_host_copy (a server's workers)
+    bool kernel_copies = false;
     // a slice table of this bucketing overflowed: its sliced fine buckets
     // take HBM tables (set when the edge phase runs again for that)
     bool no_slices = false;
@@ -3282,11 +3312,37 @@ int select_dev(uc_ctx *c)
     return UC_OK;
 }
 
+// k_host_copy on the context's stream; `dst` or `src` is the device mapping
+// of page-locked host memory
+constexpr uint32_t HOST_COPY_GRID = 32;
+
+int host_copy(uc_ctx *c, void *dst, const void *src, uint64_t bytes)
+{
+    if (bytes == 0) return UC_OK;
+    const uint32_t g = (uint32_t)std::min<uint64_t>(HOST_COPY_GRID, (bytes + 16ull * BLOCK - 1) / (16ull * BLOCK));
+    KLAUNCH(c, "k_host_copy", k_host_copy, (g, BLOCK, 0), static_cast<const char *>(src), static_cast<char *>(dst),
+            bytes);
+    return UC_OK;
+}
+
+// the device mapping of page-locked host memory from hipHostMalloc, or
+// nullptr
+void *host_mapping(const void *p)
+{
+    void *d = nullptr;
+    if (hipHostGetDevicePointer(&d, const_cast<void *>(p), 0) != hipSuccess) d = nullptr;
+    (void)hipGetLastError();
+    return d;
+}
+
 int d2h(uc_ctx *c, void *dst, const void *src, size_t bytes)
 {
     if (bytes == 0) return UC_OK;
     if (c->rbuf && bytes <= RBUF_BYTES) {
-        HIPCHK(c, hipMemcpyAsync(c->rbuf, src, bytes, hipMemcpyDeviceToHost, c->stream));
+        if (c->kernel_copies && c->rbuf_dev)
+            CHK(host_copy(c, c->rbuf_dev, src, bytes));
+        else
+            HIPCHK(c, hipMemcpyAsync(c->rbuf, src, bytes, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         std::memcpy(dst, c->rbuf, bytes);
         return UC_OK;
@@ -4910,7 +4966,10 @@ int uc_ctx_create(const uc_opts *opts, uc_ctx **out)
     // failed lookup only disables that entry point)
     c->scratch_dev = scratch_ptr(o.device);
     // (without it the reads go to pageable memory)
-    if (hipHostMalloc((void **)&c->rbuf, RBUF_BYTES, hipHostMallocDefault) != hipSuccess) c->rbuf = nullptr;
+    // (coherent: a kernel's writes reach host memory without a cache write-back)
+    if (hipHostMalloc((void **)&c->rbuf, RBUF_BYTES, hipHostMallocCoherent) != hipSuccess) c->rbuf = nullptr;
+    if (c->rbuf && hipHostGetDevicePointer((void **)&c->rbuf_dev, c->rbuf, 0) != hipSuccess) c->rbuf_dev = nullptr;
+    (void)hipGetLastError();
     *out = c;
     return UC_OK;
 }
@@ -5395,7 +5454,8 @@ void uc_ctx_set_meter(uc_ctx *c, uc_meter *m)
 void *uc_host_pinned_alloc(uint64_t bytes)
 {
     void *p = nullptr;
-    if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) return nullptr;
+    // coherent: uc_text_copy_pinned may write it from a kernel
+    if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocCoherent) != hipSuccess) return nullptr;
     return p;
 }
 
@@ -5411,10 +5471,20 @@ int uc_text_copy_pinned(uc_ctx *c, char *dst, uint64_t cap)
     if (cap < c->text_bytes) return fail(c, UC_E_ARG, "destination too small");
     CHK(select_dev(c));
     if (!c->text_bytes) return UC_OK;
-    HIPCHK(c, hipMemcpyAsync(dst, c->text.p, c->text_bytes, hipMemcpyDeviceToHost, c->stream));
+    void *d = c->kernel_copies ? host_mapping(dst) : nullptr;
+    if (d)
+        CHK(host_copy(c, d, c->text.p, c->text_bytes));
+    else
+        HIPCHK(c, hipMemcpyAsync(dst, c->text.p, c->text_bytes, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return UC_OK;
 }
+
+void uc_ctx_set_kernel_copies(uc_ctx *c, bool on)
+{
+    if (c) c->kernel_copies = on;
+}
+
 
 extern "C" {
 

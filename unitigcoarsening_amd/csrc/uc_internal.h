@@ -35,3 +35,9 @@ void uc_host_pinned_free(void *p);
 // uc_text_copy into page-locked memory from uc_host_pinned_alloc: one
 // direct device-to-host copy, no staging.
 int uc_text_copy_pinned(uc_ctx *c, char *dst, uint64_t cap);
+
+// Copies from the context's device memory into page-locked host memory (its
+// small counter reads, uc_text_copy_pinned) by a copy kernel instead of the
+// DMA engine, so that they overlap the copies of other contexts sharing the
+// device.  Call from the context's own thread.
+void uc_ctx_set_kernel_copies(uc_ctx *c, bool on);

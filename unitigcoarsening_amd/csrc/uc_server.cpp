@@ -267,6 +267,9 @@ int uc_server_create(const uc_server_opts *opts, uc_server **out)
             return rc;
         }
         uc_ctx_set_meter(w.ctx, &s->meter);
+        // the workers' copies out overlap their copies in when a kernel makes
+        // them (the DMA engine took them all one after another)
+        uc_ctx_set_kernel_copies(w.ctx, true);
     }
     for (size_t i = 0; i < s->workers.size(); i++) s->threads.emplace_back([s, i] { s->run(i); });
     *out = s;
