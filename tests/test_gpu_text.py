@@ -183,6 +183,35 @@ def test_file_reader_errors(tmp_path):
         assert uc.lib().uc_load_file(c._ctx, os.fsencode(str(p)), 31) == uc.UC_E_FORMAT
 
 
+def test_file_reader_many_pieces(tmp_path):
+    # files of several reader pieces (16 MiB each, read by four threads, two
+    # pieces each for some): every byte lands in its place, from the start of
+    # a text file and from after a packed file's header, and a packed file cut
+    # short in its last piece is a format error
+    kmers = uc.medium_model(33, 2_600_000, 31)
+    text = uc.format_text(kmers, 31)
+    log(f"text {len(text)} bytes")
+    assert len(text) > 4 * (16 << 20) and len(text) % (16 << 20)
+    p = tmp_path / "big.txt"
+    p.write_bytes(text)
+    with uc.Compactor() as c:
+        c.load_file(str(p), 31)
+        assert np.array_equal(c.loaded(31), kmers)
+    k63 = uc.medium_model(34, 1_500_000, 63)
+    q = tmp_path / "big.ucb"
+    uc.write_packed_file(str(q), k63, 63)
+    assert q.stat().st_size > (16 << 20)
+    with uc.Compactor() as c:
+        assert c.load_packed_file(str(q)) == 63
+        assert np.array_equal(c.loaded(63), k63)
+    cut = tmp_path / "cut.ucb"
+    cut.write_bytes(q.read_bytes()[:-4])
+    with uc.Compactor() as c:
+        with pytest.raises(uc.UcError) as e:
+            c.load_packed_file(str(cut))
+        assert e.value.status == uc.UC_E_FORMAT
+
+
 def test_writer_multi_chunk(tmp_path):
     # a text above one 64 MiB pinned chunk: copy/write overlap across chunks
     kmers = uc.synth_kmers(8, 16_000_000, 63, error_frac=0.1)
