@@ -5018,10 +5018,7 @@ int uc_ctx_create(const uc_opts *opts, uc_ctx **out)
     // failed lookup only disables that entry point)
     c->scratch_dev = scratch_ptr(o.device);
     // (without it the reads go to pageable memory)
-    // (coherent: a kernel's writes reach host memory without a cache write-back)
-    if (hipHostMalloc((void **)&c->rbuf, RBUF_BYTES, hipHostMallocCoherent) != hipSuccess) c->rbuf = nullptr;
-    if (c->rbuf && hipHostGetDevicePointer((void **)&c->rbuf_dev, c->rbuf, 0) != hipSuccess) c->rbuf_dev = nullptr;
-    (void)hipGetLastError();
+    if (hipHostMalloc((void **)&c->rbuf, RBUF_BYTES, hipHostMallocDefault) != hipSuccess) c->rbuf = nullptr;
     *out = c;
     return UC_OK;
 }
@@ -5534,7 +5531,18 @@ int uc_text_copy_pinned(uc_ctx *c, char *dst, uint64_t cap)
 
 void uc_ctx_set_kernel_copies(uc_ctx *c, bool on)
 {
-    if (c) c->kernel_copies = on;
+    if (!c || on == c->kernel_copies) return;
+    (void)hipSetDevice(c->opts.device);
+    (void)hipStreamSynchronize(c->stream);
+    // the read buffer is written by k_host_copy now: coherent, so that the
+    // kernel's stores reach host memory without a cache write-back
+    if (c->rbuf) (void)hipHostFree(c->rbuf);
+    c->rbuf = c->rbuf_dev = nullptr;
+    if (hipHostMalloc((void **)&c->rbuf, RBUF_BYTES, on ? hipHostMallocCoherent : hipHostMallocDefault) != hipSuccess)
+        c->rbuf = nullptr;
+    if (on && c->rbuf) c->rbuf_dev = static_cast<char *>(host_mapping(c->rbuf));
+    (void)hipGetLastError();
+    c->kernel_copies = on;
 }
 
 
