@@ -3685,9 +3685,17 @@ int stream_file(uc_ctx *c, FILE *f, uint64_t len, char *dst, const char *path)
         }
     };
     if (rc == UC_OK) {
+        // a reader whose thread cannot be had runs on this one, after the first
         std::vector<std::thread> th;
-        for (int t = 1; t < T; t++) th.emplace_back(reader, t);
-        reader(0);
+        std::vector<int> here{0};
+        for (int t = 1; t < T; t++) {
+            try {
+                th.emplace_back(reader, t);
+            } catch (...) {
+                here.push_back(t);
+            }
+        }
+        for (int t : here) reader(t);
         for (auto &x : th) x.join();
         rc = err.load();
     }
