@@ -3825,6 +3825,8 @@ int check_placements(uc_ctx *c, uint64_t n, uint64_t x)
     return UC_OK;
 }
 
+constexpr uint64_t FINE_CAPPED_MAX = 1700;
+
 template <int W>
 int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
 {
@@ -3839,6 +3841,13 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
         const uint64_t want = (ns * 112 / 100 + (uint64_t)B * fine_target - 1) / ((uint64_t)B * fine_target);
         uint32_t sub = 1;
         while (sub < want && sub < MAX_SUB) sub <<= 1;
+        // A set that wants more fine buckets than MAX_SUB gets the fewest that
+        // still average at most FINE_CAPPED_MAX slots: the fine sort then writes
+        // longer pieces of each fine bucket per round, and the buckets still
+        // fit LDS groups (50 Mbp job: 512 instead of 1024 fine buckets per
+        // coarse bucket, 3.90 against 3.97 ms per step on MI355X)
+        if (!c->opts.fine_slots && want > MAX_SUB)
+            while (sub > 1 && ns * 112 / 100 <= (uint64_t)B * (sub / 2) * FINE_CAPPED_MAX) sub >>= 1;
         c->set_sub[s] = sub;
         any_fine |= sub > 1;
     }
