@@ -157,6 +157,22 @@ def test_result_independent_of_seed_buckets_sampling():
         assert uc.sorted_lines(got) == ref
 
 
+@pytest.mark.parametrize("k,l", [(31, 11), (63, 13)])
+def test_capped_fine_bucket_count(k, l):
+    # one coarse bucket and about 600 k k-mers want more than 1024 fine
+    # buckets: the batched form takes the fewest that average at most 1700
+    # slots (512 here), the plain form keeps 1024; both against the textbook
+    kmers = uc.medium_model(17, 600_000, k)
+    ref = textbook_sorted(kmers, k)
+    for mode, want in ((uc.UC_MODE_BATCHED, 512), (uc.UC_MODE_PLAIN, 1024)):
+        with uc.Compactor(n_buckets=1, mode=mode) as c:
+            got = c.compact(kmers, k, l)
+            fine = c.bucket_info().n_fine_buckets
+        log(f"k={k} mode={mode}: {kmers.shape[0]} k-mers, {fine} fine buckets")
+        assert fine == want
+        assert uc.sorted_lines(got) == ref
+
+
 def test_sets_equal_individual():
     sets = [uc.medium_model(100 + i, int(n), 31) for i, n in enumerate([5000, 120_000, 40, 0, 70_000, 31])]
     with uc.Compactor(n_buckets=16) as c:

@@ -3845,8 +3845,10 @@ int run_bucketize(uc_ctx *c, uint32_t l, const uint32_t *table)
         // still average at most FINE_CAPPED_MAX slots: the fine sort then writes
         // longer pieces of each fine bucket per round, and the buckets still
         // fit LDS groups (50 Mbp job: 512 instead of 1024 fine buckets per
-        // coarse bucket, 3.90 against 3.97 ms per step on MI355X)
-        if (!c->opts.fine_slots && want > MAX_SUB)
+        // coarse bucket, 3.91 against 3.96 ms per step on MI355X).  The plain
+        // form keeps the count (its per-bucket launches measured 5.81 against
+        // 5.31 ms with the fewer, fuller fine buckets).
+        if (!c->opts.fine_slots && want > MAX_SUB && c->opts.mode != UC_MODE_PLAIN)
             while (sub > 1 && ns * 112 / 100 <= (uint64_t)B * (sub / 2) * FINE_CAPPED_MAX) sub >>= 1;
         c->set_sub[s] = sub;
         any_fine |= sub > 1;
