@@ -59,7 +59,9 @@ typedef struct {
     uint32_t sample_log2;  /* splitter sampling rate 2^-sample_log2 of the run heads; 0 = default 9 */
     uint64_t hbm_budget;   /* bytes this context may allocate on the device; 0 = no limit */
     uint32_t flags;        /* UC_FLAG_* */
-    uint32_t fine_slots;   /* target slots per fine bucket (64..4096); 0 = default (512 for k <= 32, else 384) */
+    uint32_t fine_slots;   /* target slots per fine bucket (64..4096); 0 = default (512 for k <= 32, else 384; a
+                              set of the batched form that wants more than 1024 fine buckets per coarse bucket
+                              gets the fewest that average at most 1700 slots) */
     uint32_t group_slots;  /* most slots one LDS edge-finding workgroup takes (256..2048, power of two);
                               0 = default 2048 */
 } uc_opts;
@@ -333,7 +335,8 @@ int uc_ctx_set_slice_probes(uc_ctx *ctx, uint32_t probes);
  * of the HBM tables), and keeps the fastest in the context for every later
  * job.  The unitig set never depends on these settings; the line order
  * follows n_buckets and fine_slots.  Each candidate runs `reps` times
- * (0 = 2) after one warm-up; the minimum counts. */
+ * (0 = 2) after one warm-up; the minimum counts.  A fine_slots of 0 in the
+ * result is the library's default choice (uc_opts.fine_slots). */
 typedef struct {
     uint32_t n_buckets, fine_slots, group_slots, sample_log2, load_pct;
     uint32_t trials;       /* candidate settings timed */

@@ -18,7 +18,7 @@ def test_autotune_keeps_results(k, l):
         print(r, flush=True)
         assert r["trials"] >= 10
         assert r["n_buckets"] in (32, 64, 128, 256)
-        assert r["fine_slots"] in (256, 384, 512, 768, 1024)
+        assert r["fine_slots"] in (0, 256, 384, 512, 768, 1024)  # 0: the library's default choice
         assert r["group_slots"] in (1024, 2048)
         assert 4 <= r["sample_log2"] <= 12
         assert r["best_ms"] <= r["default_ms"]

@@ -5440,8 +5440,9 @@ int uc_autotune(uc_ctx *c, uint32_t l, uint32_t reps, uint32_t tune_flags, uc_tu
     struct Cand {
         uint32_t B, fine, group, s, load;
     };
-    const uint32_t W = c->W;
-    Cand cur{c->B, c->opts.fine_slots ? c->opts.fine_slots : (W == 1 ? 512u : 384u),
+    // (fine_slots 0 is the library's own choice, the default target with the
+    // capped rule for sets that want more than MAX_SUB fine buckets)
+    Cand cur{c->B, c->opts.fine_slots,
              c->opts.group_slots ? c->opts.group_slots : LDS_GROUP, c->opts.sample_log2 ? c->opts.sample_log2 : DEFAULT_SAMPLE_LOG2,
              c->opts.load_pct ? c->opts.load_pct : 50};
     uint32_t trials = 0;
@@ -5468,7 +5469,7 @@ int uc_autotune(uc_ctx *c, uint32_t l, uint32_t reps, uint32_t tune_flags, uc_tu
     float best_ms;
     CHK(time_of(cur, &best_ms));
     const float default_ms = best_ms;
-    const std::vector<uint32_t> Bs = {32, 64, 128, 256}, fines = {256, 384, 512, 768, 1024},
+    const std::vector<uint32_t> Bs = {32, 64, 128, 256}, fines = {0, 256, 384, 512, 768, 1024},
                                 groups = {1024, 2048}, ss = {4, 6, 8, 9, 10, 12}, loads = {35, 50, 70};
     for (int pass = 0; pass < 2; pass++) {
         bool moved = false;
